@@ -1,0 +1,265 @@
+/*
+ * scone_gpu.h -- C ABI of the MI355X multigroup transport engine (libscone_amd.so).
+ *
+ * The engine replaces the per-history OpenMP loop of SCONE's physics packages
+ * (PhysicsPackages/eigenPhysicsPackage_class.f90:211-252, fixedSourcePhysicsPackage_class.f90:
+ * 198-261) by a bank-level call.  Everything the loop reads is passed once as flat tables
+ * (geometry, multigroup data, collision-estimator clerks); banks cross the boundary as arrays
+ * of scn_particle_state, the bind(C) image of particleStateData (ParticleObjects/particle_class.f90:50-65).
+ *
+ * Conventions
+ *  - All indices the reference keeps 1-based stay 1-based (material, universe, local cell,
+ *    group, broodID); array offsets into the flat tables are 0-based.
+ *  - Every entry point returns 0 on success or an SCN_ERR_* code; scn_last_error() then holds
+ *    a message naming the reference check it mirrors (fatalError sites).
+ *  - One engine handle is used by one host thread at a time.  Several handles may coexist.
+ */
+#ifndef SCONE_GPU_H
+#define SCONE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------------------------- */
+enum {
+    SCN_OK = 0,
+    SCN_ERR_ARG = 1,         /* invalid argument or table                                       */
+    SCN_ERR_HIP = 2,         /* HIP runtime failure                                             */
+    SCN_ERR_OVERFLOW = 3,    /* bank / site buffer overflow (particleDungeon_class.f90:170-174) */
+    SCN_ERR_MATERIAL = 4,    /* UNDEF / OVERLAP material or invalid material index              */
+    SCN_ERR_NAN = 5,         /* NaN distance (transportOperatorST_class.f90:86-91)              */
+    SCN_ERR_MAJORANT = 6,    /* zero majorant (transportOperatorDT_class.f90:60)                */
+    SCN_ERR_NORM = 7,        /* population normalisation failed (particleDungeon_class.f90:597) */
+    SCN_ERR_SAMPLING = 8,    /* CDF scan fell through (multiScatterMG/fissionMG sampleOut)      */
+    SCN_ERR_UNSUPPORTED = 9, /* feature outside the engine's scope                              */
+    SCN_ERR_NO_DEVICE = 10   /* no GPU visible                                                  */
+};
+
+/* ---- reference constants ------------------------------------------------------------------ */
+#define SCN_MAX_NEST 8            /* deepest universe nesting the engine accepts (reference: 12) */
+#define SCN_OUTSIDE_MAT 0
+#define SCN_VOID_MAT 2147483647
+#define SCN_UNDEF_MAT 2147483646
+#define SCN_OVERLAP_MAT 2147483645
+#define SCN_BC_VACUUM 0
+#define SCN_BC_REFLECTIVE 1
+#define SCN_BC_PERIODIC 2
+
+enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2 };
+enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3 };
+enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2 };
+enum { SCN_GRID_LIN = 1, SCN_GRID_UNSTRUCT = 3 };
+/* macroscopic MT codes (SharedModules/endfConstants.f90); 0 selects fluxResponse */
+enum {
+    SCN_MT_FLUX = 0, SCN_MT_TOTAL = -1, SCN_MT_DISAPPEARANCE = -2, SCN_MT_ESCATTER = -3,
+    SCN_MT_IESCATTER = -4, SCN_MT_FISSION = -6, SCN_MT_NUFISSION = -7, SCN_MT_KAPPA = -80,
+    SCN_MT_ALLSCATTER = -20, SCN_MT_ABSORPTION = -21, SCN_MT_NONELASTIC = -22
+};
+
+/* ---- bank record: particleStateData (particle_class.f90:50-65, MPI layout mpi_func.f90:95-113) */
+typedef struct scn_particle_state {
+    double wgt;
+    double r[3];
+    double dir[3];
+    double E;
+    int32_t G;
+    int32_t isMG;
+    int32_t type;
+    int32_t pad0;
+    double time;
+    double lambda;
+    int32_t matIdx;
+    int32_t cellIdx;
+    int32_t uniqueID;
+    int32_t collisionN;
+    int32_t broodID;
+    int32_t pad1;
+} scn_particle_state;
+
+/* ---- geometry ------------------------------------------------------------------------------
+ * Border surfaces: box (Geometry/Surfaces/CompositeSurfaces/box_class.f90) with n_axes = 3, or
+ * x/y/zSquareCylinder (squareCylinder_class.f90) with n_axes = 2.  origin/halfwidth are stored
+ * per slot; axes[slot] is the coordinate (0 = x, 1 = y, 2 = z) of that slot.                  */
+typedef struct scn_box {
+    int32_t n_axes;
+    int32_t axes[3];
+    int32_t bc[6];           /* -x +x -y +y -z +z */
+    double origin[3];
+    double halfwidth[3];
+} scn_box;
+
+/* Universes (Geometry/Universes/{root,lat,pin}Universe_class.f90).  uniIdx is the 1-based position in the table.
+ *  ROOT: rootUniverse_class.f90 -- local cell 1 = inside border, 2 = outside.
+ *  LAT : latUniverse_class.f90  -- size/pitch after the 2D fix-up (size[2] = 1, pitch[2] = 2*INF),
+ *        corner, a_bar and out_local as init computes them; offset_mode 1 = offset every cell,
+ *        0 = none, 2 = per-cell flags at int_data[offset_map + localID - 1].
+ *  PIN : pinUniverse_class.f90  -- annuli sorted ascending, the last one of radius INF*1.1;
+ *        real_data[annuli + i] = radius_i, real_data[annuli + n_annuli + i] = radius_i**2.    */
+typedef struct scn_universe {
+    int32_t type;
+    int32_t n_local;
+    double origin[3];
+    int32_t border;          /* ROOT: index into boxes                       */
+    int32_t size[3];         /* LAT */
+    int32_t out_local;
+    int32_t offset_mode;
+    int32_t offset_map;
+    double pitch[3];
+    double corner[3];
+    double a_bar[3];
+    double outline_hw[3];
+    int32_t n_annuli;        /* PIN */
+    int32_t annuli;
+} scn_universe;
+
+/* ---- multigroup data (NuclearData/mgNeutronData/baseMgNeutron, Reactions/reactionMG) -----------
+ * xs[(m-1)*nG*6 + (g-1)*6 + k], k = TOTAL, IESCATTER, CAPTURE, FISSION, NU_FISSION, KAPPA_FISSION
+ * as baseMgNeutronMaterial%data holds them (fission columns zero for non-fissile materials).
+ * P0/prod[(m-1)*nG*nG + (gin-1)*nG + (gout-1)] is column gin of P0(G_out, G_in) (contiguous scan).
+ * chi[(m-1)*nG + g-1] is the normalised fission spectrum.                                     */
+typedef struct scn_mg_data {
+    int32_t n_mat;
+    int32_t n_groups;
+    const double* xs;
+    const double* P0;
+    const double* prod;
+    const double* chi;
+    const int32_t* fissile;
+    const double* majorant;  /* [nG] baseMgNeutronDatabase_class.f90:458-503 */
+    double collision_xs;     /* 1/avgDist, 0 when absent */
+} scn_mg_data;
+
+/* ---- tallies: collisionClerk (Tallies/TallyClerks/collisionClerk_class.f90) ----------------- */
+typedef struct scn_map1d {
+    int32_t type;            /* SCN_MAP_SPACE / SCN_MAP_MATERIAL */
+    int32_t axis;            /* SPACE: 0,1,2 */
+    int32_t nbins;
+    int32_t grid_type;       /* SPACE: SCN_GRID_LIN (real_data[data] = min, [data+1] = step)
+                                       SCN_GRID_UNSTRUCT (real_data[data .. data+nbins]) */
+    int32_t data;            /* MATERIAL: int_data[data + matIdx] = bin for matIdx in 0..n_mat,
+                                int_data[data + n_mat + 1] = bin for any other index */
+} scn_map1d;
+
+typedef struct scn_clerk {
+    int32_t gtop, glow;      /* energyFilter window; gtop = 0 means no filter */
+    int32_t n_maps;          /* 0 (single bin) .. 3 (multiMap) */
+    int32_t handle_virtual;
+    scn_map1d maps[3];
+    int32_t multi[3];
+    int32_t width;           /* number of responses */
+    int32_t resp[8];         /* SCN_MT_* */
+    int64_t mem_addr;        /* 0-based offset of this clerk's first bin in the score memory */
+    int64_t size;            /* width * product(nbins) */
+} scn_clerk;
+
+typedef struct scn_problem {
+    int32_t transport;       /* SCN_TRANSPORT_ST / SCN_TRANSPORT_DT */
+    int32_t st_cache;        /* transportOperatorST 'cache' flag (default 1) */
+    int32_t n_universes;
+    int32_t root_uni;        /* uniIdx of the root universe */
+    const scn_universe* universes;
+    int32_t n_boxes;
+    int32_t border_box;
+    const scn_box* boxes;
+    int32_t graph_size;      /* geomGraph: fill >= 0 material (0 outside), < 0 -uniIdx */
+    const int32_t* graph_fill;
+    const int32_t* graph_id; /* uniqueID (material cells) or rootID of the filling universe */
+    int32_t n_real;
+    const double* real_data;
+    int32_t n_int;
+    const int32_t* int_data;
+    scn_mg_data mg;
+    int32_t n_clerks;
+    const scn_clerk* clerks;
+    int64_t n_bins;          /* size of the collision-clerk score memory */
+    int64_t norm_bin;        /* tallyAdmin 'norm': 0-based BIN address of the norm clerk, -1 none */
+    double norm_value;       /* tallyAdmin 'normVal' */
+} scn_problem;
+
+/* ---- eigenvalue run ------------------------------------------------------------------------ */
+typedef struct scn_eigen_settings {
+    int32_t pop;             /* totalPop */
+    int32_t n_inactive;
+    int32_t n_active;
+    int32_t inactive_tally;  /* 1: the clerks also score during inactive cycles (own memory) */
+    int64_t seed;            /* pRNG initial state (eigenPhysicsPackage_class.f90:492-495) */
+    double keff0;            /* keff_0, default 1.0 */
+    int32_t events_per_launch; /* 0: autotuned schedule */
+    int32_t block_size;        /* 0: autotuned */
+} scn_eigen_settings;
+
+typedef struct scn_eigen_result {
+    double k_analog[2];      /* inactive keffAnalogClerk K_EST mean, sigma */
+    double k_implicit[2];    /* active keffImplicitClerk K_EFF mean, sigma */
+    double imp[4][2];        /* IMP_PROD, SCATTER_PROD, IMP_ABS, ANA_LEAK mean, sigma (active) */
+    double* k_history;       /* caller buffer [n_inactive + n_active] or NULL: k used next cycle */
+    double* bins_active;     /* caller buffers [n_bins * 2] (mean, sigma) or NULL */
+    double* bins_inactive;
+    double transport_time;   /* seconds, same span as SCONE's Total_Transport_Time */
+    int64_t n_collisions;    /* real collisions */
+    int64_t n_events;        /* flights + collisions processed */
+    int64_t n_histories;
+} scn_eigen_result;
+
+/* ---- counters of the engine (cumulative since creation or the last reset) ----------------- */
+typedef struct scn_stats {
+    double transport_kernel_ms; /* summed HIP-event time of the transport kernel launches */
+    int64_t transport_launches;
+    int64_t n_histories;
+    int64_t n_segments;          /* ST surface-tracking segments / DT tentative flights */
+    int64_t n_collisions;        /* real collisions */
+    int64_t n_sites;             /* fission sites banked */
+    int64_t n_scores;            /* collision-clerk score additions */
+    int64_t n_cycles;
+    double cycle_ms;             /* wall time of whole cycles (transport + cycle close) */
+} scn_stats;
+
+/* ---- entry points --------------------------------------------------------------------------- */
+
+/* Build an engine from flat tables (replaces geometry/nuclear-data/tally construction in
+ * eigenPhysicsPackage_class.f90:507-590).  capacity = largest bank the handle will transport. */
+int scn_create(const scn_problem* problem, int32_t capacity, void** engine);
+void scn_destroy(void* engine);
+const char* scn_last_error(const void* engine);
+int scn_device_info(char* name, int32_t name_len, int32_t* n_cu);
+
+/* One eigenvalue cycle of transport: the body of eigenPhysicsPackage%cycles between
+ * reportCycleStart and reportCycleEnd (eigenPhysicsPackage_class.f90:208-252).
+ *   bank_in   : thisCycle%prisoners(1:n_in); history n uses rng_state strided by n.
+ *   k_eff     : k_new loaded into every history (eigen...:227).
+ *   bank_out  : nextCycle in broodID order (what normSize_Repr's sortByBroodID produces).
+ *   bin_scores: this cycle's BIN values of the collision clerks (length n_bins), or NULL.
+ *   keff_scores[4]: keffImplicitClerk IMP_PROD, SCATTER_PROD, IMP_ABS, ANA_LEAK for the cycle.
+ * The RNG state is not advanced: the caller keeps doing the strides, as the package does.   */
+int scn_eigen_cycle(void* engine, const scn_particle_state* bank_in, int32_t n_in, int64_t rng_state,
+                    double k_eff, scn_particle_state* bank_out, int32_t bank_out_cap, int32_t* n_out,
+                    double* bin_scores, double* keff_scores);
+
+/* Population control of one cycle: particleDungeon%normSize_Repr (particleDungeon_class.f90:
+ * 431-602) on a broodID-sorted bank, single rank.  Works in place on bank[0..*n) (capacity cap). */
+int scn_norm_size_repr(void* engine, scn_particle_state* bank, int32_t* n, int32_t cap,
+                       int32_t tot_pop, int64_t rng_state);
+
+/* Whole power iteration with banks and tallies resident in HBM: eigenPhysicsPackage%run
+ * (eigenPhysicsPackage_class.f90:135-159) from an already generated initial bank. */
+int scn_eigen_run(void* engine, const scn_eigen_settings* settings, const scn_particle_state* initial_bank,
+                  scn_eigen_result* result);
+
+/* Initial fission source: fissionSource%sampleParticle for particles 1..n
+ * (ParticleObjects/Source/fissionSource_class.f90:149-271, source_inter.f90:98-119);
+ * bottom/top = sampling box, g_in = the group passed to fissionMG%sampleOut. */
+int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double bottom[3],
+                       const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank);
+
+/* Engine counters and timings (no counterpart in the reference; SCONE only times transport). */
+int scn_get_stats(void* engine, scn_stats* out);
+int scn_reset_stats(void* engine);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCONE_GPU_H */

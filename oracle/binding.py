@@ -1,0 +1,266 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE: imported by tests/, the smoke check and
+bench.py's CPU-baseline leg only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from scone_amd import tables as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libscn_oracle.so")
+
+MAX_NEST = T.MAX_NEST
+
+
+class PlaceOut(C.Structure):
+    _fields_ = [("matIdx", C.c_int32), ("uniqueID", C.c_int32), ("nesting", C.c_int32),
+                ("localID", C.c_int32 * MAX_NEST), ("uniIdx", C.c_int32 * MAX_NEST),
+                ("r", (C.c_double * 3) * MAX_NEST)]
+
+
+class Hist(C.Structure):
+    _fields_ = [("n_events", C.c_int64), ("n_coll", C.c_int32), ("n_sites", C.c_int32),
+                ("rng_end", C.c_uint64), ("w_end", C.c_double), ("fate", C.c_int32), ("G_end", C.c_int32),
+                ("r_end", C.c_double * 3)]
+
+
+HIST_DTYPE = np.dtype([("n_events", "i8"), ("n_coll", "i4"), ("n_sites", "i4"), ("rng_end", "u8"),
+                       ("w_end", "f8"), ("fate", "i4"), ("G_end", "i4"), ("r_end", "f8", 3)])
+assert HIST_DTYPE.itemsize == C.sizeof(Hist)
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        d3 = C.c_double * 3
+        P = C.POINTER(T.Problem)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_rng_get.restype = C.c_double
+        L.orc_rng_get.argtypes = [C.POINTER(C.c_uint64)]
+        L.orc_rng_skip.restype = C.c_uint64
+        L.orc_rng_skip.argtypes = [C.c_uint64, C.c_int64]
+        L.orc_log.restype = C.c_double
+        L.orc_log.argtypes = [C.c_double]
+        L.orc_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_norm2.restype = C.c_double
+        L.orc_norm2.argtypes = [d3]
+        L.orc_rotate.argtypes = [d3, C.c_double, C.c_double, d3]
+        L.orc_dsum.restype = C.c_double
+        L.orc_dsum.argtypes = [C.POINTER(C.c_double), C.c_int64]
+        L.orc_box_distance.restype = C.c_double
+        L.orc_box_distance.argtypes = [C.POINTER(T.Box), d3, d3]
+        L.orc_box_evaluate.restype = C.c_double
+        L.orc_box_evaluate.argtypes = [C.POINTER(T.Box), d3]
+        L.orc_box_halfspace.restype = C.c_int
+        L.orc_box_halfspace.argtypes = [C.POINTER(T.Box), d3, d3]
+        L.orc_box_explicit_bc.argtypes = [C.POINTER(T.Box), d3, d3]
+        L.orc_box_transform_bc.argtypes = [C.POINTER(T.Box), d3, d3]
+        L.orc_cyl_distance.restype = C.c_double
+        L.orc_cyl_distance.argtypes = [C.c_double, d3, d3]
+        L.orc_uni_findcell.restype = C.c_int
+        L.orc_uni_findcell.argtypes = [P, C.c_int32, d3, d3]
+        L.orc_uni_distance.restype = C.c_double
+        L.orc_uni_distance.argtypes = [P, C.c_int32, C.c_int32, d3, d3, C.POINTER(C.c_int32)]
+        L.orc_place.argtypes = [P, d3, d3, C.POINTER(PlaceOut)]
+        L.orc_move.argtypes = [P, d3, d3, C.c_double, C.POINTER(PlaceOut), d3, C.POINTER(C.c_double),
+                               C.POINTER(C.c_int32)]
+        L.orc_teleport.argtypes = [P, d3, d3, C.c_double, C.POINTER(PlaceOut), d3]
+        PS = C.POINTER(T.ParticleState)
+        L.orc_eigen_cycle.argtypes = [P, PS, C.c_int32, C.c_int64, C.c_double, PS, C.c_int32,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.c_void_p, C.c_int32]
+        L.orc_norm_size_repr.argtypes = [PS, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int64]
+        L.orc_fission_source.argtypes = [P, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
+        L.orc_eigen_run.argtypes = [P, C.POINTER(T.EigenSettings), PS, C.POINTER(T.EigenResult), C.c_int32]
+        _lib = L
+    return _lib
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def check(rc):
+    if rc:
+        raise RuntimeError(f"oracle error {rc}: {lib().orc_last_error().decode()}")
+
+
+def rng_get(state: int):
+    s = C.c_uint64(state)
+    u = lib().orc_rng_get(C.byref(s))
+    return u, s.value
+
+
+def rng_skip(state: int, k: int) -> int:
+    return lib().orc_rng_skip(C.c_uint64(state), k)
+
+
+def log(x: float) -> float:
+    return lib().orc_log(x)
+
+
+def sincos(x: float):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_sincos(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def norm2(v):
+    return lib().orc_norm2(_d3(v))
+
+
+def rotate(d, mu, phi):
+    o = _d3([0, 0, 0])
+    lib().orc_rotate(_d3(d), mu, phi, o)
+    return list(o)
+
+
+def dsum(x):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_dsum(a.ctypes.data_as(C.POINTER(C.c_double)), len(a))
+
+
+def box(n_axes, axes, origin, halfwidth, bc=(0, 0, 0, 0, 0, 0)):
+    b = T.Box()
+    b.n_axes = n_axes
+    b.axes[:] = list(axes) + [0] * (3 - len(axes))
+    b.origin[:] = list(origin) + [0.0] * (3 - len(origin))
+    b.halfwidth[:] = list(halfwidth) + [0.0] * (3 - len(halfwidth))
+    b.bc[:] = list(bc)
+    return b
+
+
+def box_distance(b, r, u):
+    return lib().orc_box_distance(C.byref(b), _d3(r), _d3(u))
+
+
+def box_evaluate(b, r):
+    return lib().orc_box_evaluate(C.byref(b), _d3(r))
+
+
+def box_halfspace(b, r, u):
+    return bool(lib().orc_box_halfspace(C.byref(b), _d3(r), _d3(u)))
+
+
+def box_explicit_bc(b, r, u):
+    rr, uu = _d3(r), _d3(u)
+    lib().orc_box_explicit_bc(C.byref(b), rr, uu)
+    return list(rr), list(uu)
+
+
+def box_transform_bc(b, r, u):
+    rr, uu = _d3(r), _d3(u)
+    lib().orc_box_transform_bc(C.byref(b), rr, uu)
+    return list(rr), list(uu)
+
+
+def cyl_distance(radius, r, u):
+    return lib().orc_cyl_distance(radius, _d3(r), _d3(u))
+
+
+def uni_findcell(comp, uniIdx, r, u):
+    return lib().orc_uni_findcell(C.byref(comp.problem), uniIdx, _d3(r), _d3(u))
+
+
+def uni_distance(comp, uniIdx, localID, r, u):
+    s = C.c_int32()
+    d = lib().orc_uni_distance(C.byref(comp.problem), uniIdx, localID, _d3(r), _d3(u), C.byref(s))
+    return d, s.value
+
+
+def place(comp, r, u=(1.0, 0.0, 0.0)):
+    o = PlaceOut()
+    check(lib().orc_place(C.byref(comp.problem), _d3(r), _d3(u), C.byref(o)))
+    return o
+
+
+def move(comp, r, u, max_dist):
+    o = PlaceOut()
+    uo = _d3([0, 0, 0])
+    d = C.c_double()
+    ev = C.c_int32()
+    check(lib().orc_move(C.byref(comp.problem), _d3(r), _d3(u), max_dist, C.byref(o), uo, C.byref(d), C.byref(ev)))
+    return o, list(uo), d.value, ev.value
+
+
+def teleport(comp, r, u, dist):
+    o = PlaceOut()
+    uo = _d3([0, 0, 0])
+    check(lib().orc_teleport(C.byref(comp.problem), _d3(r), _d3(u), dist, C.byref(o), uo))
+    return o, list(uo)
+
+
+def eigen_cycle(comp, bank_in, rng_state, k_eff, cap=None, bins=True, hist=False, threads=1):
+    n_in = len(bank_in)
+    cap = cap or max(16, 4 * n_in)
+    out = T.bank_array(cap)
+    n_out = C.c_int32()
+    nb = int(comp.problem.n_bins)
+    b = np.zeros(max(1, nb)) if bins else None
+    ks = np.zeros(4)
+    h = np.zeros(n_in, dtype=HIST_DTYPE) if hist else None
+    rc = lib().orc_eigen_cycle(C.byref(comp.problem), T.bank_ptr(bank_in), n_in, C.c_int64(_s64(rng_state)),
+                               k_eff, T.bank_ptr(out), cap, C.byref(n_out),
+                               b.ctypes.data_as(C.POINTER(C.c_double)) if b is not None else None,
+                               ks.ctypes.data_as(C.POINTER(C.c_double)),
+                               h.ctypes.data_as(C.c_void_p) if h is not None else None, threads)
+    check(rc)
+    return out[: n_out.value].copy(), (b[:nb] if b is not None else None), ks, h
+
+
+def norm_size_repr(bank, tot_pop, rng_state):
+    n = len(bank)
+    cap = max(4 * tot_pop + 16, n)
+    buf = T.bank_array(cap)
+    buf[:n] = bank
+    nn = C.c_int32(n)
+    check(lib().orc_norm_size_repr(T.bank_ptr(buf), C.byref(nn), cap, tot_pop, C.c_int64(_s64(rng_state))))
+    return buf[: nn.value].copy()
+
+
+def fission_source(comp, n, rng_state, bottom=None, top=None, g_in=1, attempts=10000):
+    bank = T.bank_array(n)
+    bottom = comp.bounds[:3] if bottom is None else bottom
+    top = comp.bounds[3:] if top is None else top
+    check(lib().orc_fission_source(C.byref(comp.problem), n, C.c_int64(_s64(rng_state)), _d3(bottom), _d3(top),
+                                   g_in, attempts, T.bank_ptr(bank)))
+    return bank
+
+
+def eigen_run(comp, settings: dict, initial_bank, threads=1):
+    S = T.EigenSettings()
+    S.pop = settings["pop"]
+    S.n_inactive = settings["inactive"]
+    S.n_active = settings["active"]
+    S.inactive_tally = settings.get("inactive_tally", 0)
+    S.seed = _s64(settings["seed"])
+    S.keff0 = settings.get("keff0", 1.0)
+    R = T.EigenResult()
+    nb = int(comp.problem.n_bins)
+    kh = np.zeros(S.n_inactive + S.n_active)
+    ba = np.zeros(max(1, 2 * nb))
+    bi = np.zeros(max(1, 2 * nb))
+    R.k_history = kh.ctypes.data_as(C.POINTER(C.c_double))
+    R.bins_active = ba.ctypes.data_as(C.POINTER(C.c_double))
+    R.bins_inactive = bi.ctypes.data_as(C.POINTER(C.c_double))
+    check(lib().orc_eigen_run(C.byref(comp.problem), C.byref(S), T.bank_ptr(initial_bank), C.byref(R), threads))
+    return R, kh, ba[: 2 * nb].reshape(-1, 2), bi[: 2 * nb].reshape(-1, 2)
+
+
+def _s64(x: int) -> int:
+    x &= (1 << 64) - 1
+    return x - (1 << 64) if x >= (1 << 63) else x

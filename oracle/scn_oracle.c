@@ -1,0 +1,1304 @@
+/*
+ * scn_oracle.c -- CPU restatement of SCONE's multigroup tracking loop (TEST INFRASTRUCTURE).
+ *
+ * History-based and scalar like the reference: one particle at a time walks
+ * transport -> collide -> transport ... with the tallyAdmin report order of the Fortran.
+ * Only the tests, the smoke check and bench.py's CPU-baseline leg use this file, as the checker.
+ * Every routine names the reference routine it restates (paths relative to the SCONE tree).
+ */
+#include "scn_oracle.h"
+#include "scn_fpmath.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- error handling (errors_mod.f90 fatalError -> error code + message) -------------------- */
+static int g_err = 0;
+static char g_msg[512];
+
+static void o_fail(int code, const char* fmt, ...)
+{
+#pragma omp critical(orc_err)
+    {
+        if (!g_err) {
+            va_list ap;
+            va_start(ap, fmt);
+            vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+            va_end(ap);
+            g_err = code;
+        }
+    }
+}
+const char* orc_last_error(void) { return g_msg; }
+int orc_error_code(void) { return g_err; }
+static void o_clear(void) { g_err = 0; g_msg[0] = 0; }
+
+/* ---- shared math exported for tests --------------------------------------------------------- */
+double orc_rng_get(uint64_t* s) { return scn_rng_get(s); }
+uint64_t orc_rng_skip(uint64_t s, int64_t k) { return scn_rng_skip(s, k); }
+double orc_log(double x) { return scn_log(x); }
+void orc_sincos(double x, double* s, double* c) { scn_sincos(x, s, c); }
+double orc_norm2(const double v[3]) { return scn_norm2(v); }
+void orc_rotate(const double dir[3], double mu, double phi, double out[3]) { scn_rotate(dir, mu, phi, out); }
+
+/* canonical cross-history sum: chunks of SCN_DSUM_CHUNK summed left to right, then pairwise */
+double orc_dsum(const double* x, int64_t n)
+{
+    if (n <= 0) return 0.0;
+    int64_t m = (n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    double* y = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int64_t c = 0; c < m; ++c) {
+        double s = 0.0;
+        int64_t e = (c + 1) * SCN_DSUM_CHUNK < n ? (c + 1) * SCN_DSUM_CHUNK : n;
+        for (int64_t i = c * SCN_DSUM_CHUNK; i < e; ++i) s = s + x[i];
+        y[c] = s;
+    }
+    while (m > 1) {
+        int64_t h = (m + 1) / 2;
+        for (int64_t i = 0; i < h; ++i) y[i] = (2 * i + 1 < m) ? y[2 * i] + y[2 * i + 1] : y[2 * i];
+        m = h;
+    }
+    double r = y[0];
+    free(y);
+    return r;
+}
+
+/* ======================================================================================
+ * Surfaces
+ * ====================================================================================== */
+
+/* box_class.f90:134-144 evaluate (squareCylinder_class.f90 same over its two plane axes) */
+double orc_box_evaluate(const scn_box* b, const double r[3])
+{
+    double c = -SCN_HUGE;
+    for (int i = 0; i < b->n_axes; ++i) {
+        double t = fabs(r[b->axes[i]] - b->origin[i]) - b->halfwidth[i];
+        if (t > c) c = t;
+    }
+    return c;
+}
+
+/* box_class.f90:168-237 distance */
+double orc_box_distance(const scn_box* b, const double r[3], const double u[3])
+{
+    double far = SCN_HUGE, near = -SCN_HUGE;
+    for (int i = 0; i < b->n_axes; ++i) {
+        int ax = b->axes[i];
+        double rb = r[ax] - b->origin[i];
+        double a_far = scn_sign(b->halfwidth[i], u[ax]);
+        double a_near = -a_far;
+        double tn, tf;
+        if (u[ax] != 0.0) {
+            tn = (a_near - rb) / u[ax];
+            tf = (a_far - rb) / u[ax];
+        } else {
+            tn = scn_sign(SCN_INF, a_near - rb);
+            tf = scn_sign(SCN_INF, a_far - rb);
+            if (tn > tf) { double t = tn; tn = tf; tf = t; }
+        }
+        far = scn_fmin(far, tf);
+        near = scn_fmax(near, tn);
+    }
+    double d;
+    if (far <= near * SCN_MISS_TOL) {
+        d = SCN_INF;
+    } else if (fabs(orc_box_evaluate(b, r)) < SCN_SURF_TOL) {
+        d = (fabs(far) >= fabs(near)) ? far : near;
+    } else {
+        d = (near <= 0.0) ? far : near;
+    }
+    if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+    return d;
+}
+
+/* box_class.f90:252-278 going */
+static int o_box_going(const scn_box* b, const double r[3], const double u[3])
+{
+    int maxCom = 0;
+    double best = 0.0;
+    for (int i = 0; i < b->n_axes; ++i) {
+        double t = fabs(r[b->axes[i]] - b->origin[i]) - b->halfwidth[i];
+        if (i == 0 || t > best) { best = t; maxCom = i; }
+    }
+    double rl = r[b->axes[maxCom]] - b->origin[maxCom];
+    double proj = u[b->axes[maxCom]] * scn_sign(1.0, rl);
+    int hs = proj > 0.0;
+    if (proj == 0.0) hs = orc_box_evaluate(b, r) >= 0.0;
+    return hs;
+}
+
+/* surface_inter.f90:363-378 halfspace */
+int orc_box_halfspace(const scn_box* b, const double r[3], const double u[3])
+{
+    double c = orc_box_evaluate(b, r);
+    int hs = c > 0.0;
+    if (fabs(c) < SCN_SURF_TOL) hs = o_box_going(b, r, u);
+    return hs;
+}
+
+/* box_class.f90:380-420 explicitBC */
+void orc_box_explicit_bc(const scn_box* b, double r[3], double u[3])
+{
+    for (int i = 0; i < b->n_axes; ++i) {
+        int ax = b->axes[i];
+        double r0 = r[ax] - b->origin[i];
+        if (fabs(r0) <= b->halfwidth[i] * (1.0 - SCN_SURF_TOL)) continue;
+        int bc = (r0 < 0.0) ? b->bc[2 * ax] : b->bc[2 * ax + 1];
+        if (bc == SCN_BC_REFLECTIVE) u[ax] = -u[ax];
+        else if (bc == SCN_BC_PERIODIC) r[ax] = r[ax] - 2.0 * scn_sign(b->halfwidth[i], r0);
+    }
+}
+
+/* box_class.f90:431-487 transformBC */
+void orc_box_transform_bc(const scn_box* b, double r[3], double u[3])
+{
+    int Ri[3];
+    for (int i = 0; i < b->n_axes; ++i) {
+        double a_bar = b->halfwidth[i] * (1.0 - SCN_SURF_TOL);
+        Ri[i] = (int)ceil(fabs(r[b->axes[i]] - b->origin[i]) / a_bar) / 2;
+    }
+    for (int i = 0; i < b->n_axes; ++i) {
+        int ax = b->axes[i];
+        for (int t = 1; t <= Ri[i]; ++t) {
+            double r0 = r[ax] - b->origin[i];
+            int bc = (r0 < 0.0) ? b->bc[2 * ax] : b->bc[2 * ax + 1];
+            if (bc == SCN_BC_REFLECTIVE) {
+                double a0 = scn_sign(b->halfwidth[i], r0) + b->origin[i];
+                double d = r[ax] - a0;
+                r[ax] = r[ax] - 2.0 * d;
+                u[ax] = -u[ax];
+            } else if (bc == SCN_BC_PERIODIC) {
+                double d = scn_sign(b->halfwidth[i], r0);
+                r[ax] = r[ax] - 2.0 * d;
+            }
+        }
+    }
+}
+
+/* cylinder_class.f90:209-270 distance, z-axis cylinder centred at the origin */
+static double o_cyl_distance(double radius, double r_sq, const double r[3], const double u[3])
+{
+    double tol = 2.0 * radius * SCN_SURF_TOL;
+    double c = (0.0 + r[0] * r[0] + r[1] * r[1]) - r_sq;
+    double k = 0.0 + r[0] * u[0] + r[1] * u[1];
+    double a = 1.0 - u[2] * u[2];
+    double delta = k * k - a * c;
+    double d;
+    if (delta < 0.0 || a == 0.0) {
+        d = SCN_INF;
+    } else if (fabs(c) < tol) {
+        if (k >= 0.0) d = SCN_INF;
+        else { d = -k + sqrt(delta); d = d / a; }
+    } else if (c < 0.0) {
+        d = -k + sqrt(delta);
+        d = d / a;
+    } else {
+        d = -k - sqrt(delta);
+        d = d / a;
+        if (d <= 0.0) d = SCN_INF;
+    }
+    return scn_fmin(d, SCN_INF);
+}
+double orc_cyl_distance(double radius, const double r[3], const double u[3])
+{
+    return o_cyl_distance(radius, radius * radius, r, u);
+}
+
+/* ======================================================================================
+ * Universes
+ * ====================================================================================== */
+enum { SURF_BORDER = 1 };                                   /* rootUniverse: border surface    */
+enum { PIN_MOVING_IN = -1, PIN_MOVING_OUT = -2 };           /* pinUniverse_class.f90           */
+enum { LAT_OUTLINE = -7 };                                  /* latUniverse_class.f90 OUTLINE_SURF */
+
+typedef struct o_coord {
+    double r[3];
+    double dir[3];
+    int uniIdx, uniRootID, localID, cellIdx;
+} o_coord;
+
+typedef struct o_coords {
+    int nesting;
+    o_coord lvl[SCN_MAX_NEST];
+    int matIdx;
+    int uniqueID;
+} o_coords;
+
+static const scn_universe* o_uni(const scn_problem* P, int idx) { return &P->universes[idx - 1]; }
+
+/* latUniverse_class.f90:489-506 get_ijk */
+static void o_get_ijk(int localID, const int32_t sizeN[3], int ijk[3])
+{
+    int temp = localID - 1;
+    int base = temp / sizeN[0];
+    ijk[0] = temp - sizeN[0] * base + 1;
+    temp = base;
+    base = temp / sizeN[1];
+    ijk[1] = temp - sizeN[1] * base + 1;
+    ijk[2] = base + 1;
+}
+
+/* findCell of every universe type; cellIdx is 0 for root/lattice/pin */
+int orc_uni_findcell(const scn_problem* P, int32_t uniIdx, const double r[3], const double u[3])
+{
+    const scn_universe* U = o_uni(P, uniIdx);
+    switch (U->type) {
+    case SCN_UNI_ROOT: /* rootUniverse_class.f90:127-145 */
+        return orc_box_halfspace(&P->boxes[U->border], r, u) ? 2 : 1;
+    case SCN_UNI_LAT: { /* latUniverse_class.f90:270-312 */
+        int ijk[3];
+        for (int i = 0; i < 3; ++i) ijk[i] = (int)floor((r[i] - U->corner[i]) / U->pitch[i]) + 1;
+        for (int i = 0; i < 3; ++i) {
+            double r_bar = r[i] - U->corner[i] - (double)ijk[i] * U->pitch[i] + 0.5 * U->pitch[i];
+            if (fabs(r_bar) > U->a_bar[i] && r_bar * u[i] > 0.0) ijk[i] += (u[i] < 0.0) ? -1 : 1;
+        }
+        for (int i = 0; i < 3; ++i)
+            if (ijk[i] <= 0 || ijk[i] > U->size[i]) return U->out_local;
+        return ijk[0] + U->size[0] * (ijk[1] - 1 + U->size[1] * (ijk[2] - 1));
+    }
+    case SCN_UNI_PIN: { /* pinUniverse_class.f90:150-175 */
+        double r_sq = r[0] * r[0] + r[1] * r[1];
+        double mul = (r[0] * u[0] + r[1] * u[1] >= 0.0) ? -1.0 : 1.0;
+        const double* rad = P->real_data + U->annuli;
+        const double* rsq = rad + U->n_annuli;
+        int l;
+        for (l = 1; l <= U->n_annuli; ++l)
+            if (r_sq < rsq[l - 1] + mul * (2.0 * rad[l - 1] * SCN_SURF_TOL)) return l;
+        return l;
+    }
+    }
+    o_fail(SCN_ERR_ARG, "findCell: unknown universe type %d", U->type);
+    return 1;
+}
+
+double orc_uni_distance(const scn_problem* P, int32_t uniIdx, int32_t localID, const double r[3],
+                        const double u[3], int32_t* surfIdx)
+{
+    const scn_universe* U = o_uni(P, uniIdx);
+    switch (U->type) {
+    case SCN_UNI_ROOT: /* rootUniverse_class.f90:152-162 */
+        *surfIdx = SURF_BORDER;
+        return orc_box_distance(&P->boxes[U->border], r, u);
+    case SCN_UNI_LAT: { /* latUniverse_class.f90:319-373 */
+        if (localID == U->out_local) {
+            scn_box outline = {3, {0, 1, 2}, {0, 0, 0, 0, 0, 0}, {0.0, 0.0, 0.0},
+                               {U->outline_hw[0], U->outline_hw[1], U->outline_hw[2]}};
+            *surfIdx = LAT_OUTLINE;
+            return orc_box_distance(&outline, r, u);
+        }
+        int ijk[3];
+        o_get_ijk(localID, U->size, ijk);
+        double d = SCN_INF;
+        int ax = 0;
+        for (int i = 0; i < 3; ++i) {
+            double r_bar = r[i] - U->corner[i];
+            r_bar = r_bar - ((double)ijk[i] - 0.5) * U->pitch[i];
+            double bound = scn_sign(U->pitch[i] * 0.5, u[i]);
+            double test_d = (bound - r_bar) / u[i];
+            if (test_d < d) { d = test_d; ax = i; }
+        }
+        d = scn_fmax(0.0, d);
+        d = scn_fmin(SCN_INF, d);
+        int s = (ax + 1) * 2;
+        if (u[ax] < 0.0) s = s - 1;
+        *surfIdx = -s;
+        return d;
+    }
+    case SCN_UNI_PIN: { /* pinUniverse_class.f90:185-226 */
+        const double* rad = P->real_data + U->annuli;
+        const double* rsq = rad + U->n_annuli;
+        int id = localID;
+        double d_out, d_in;
+        if (id < 1 || id > U->n_annuli + 1) { o_fail(SCN_ERR_ARG, "pin distance: invalid local ID %d", id); return SCN_INF; }
+        d_out = (id > U->n_annuli) ? SCN_INF : o_cyl_distance(rad[id - 1], rsq[id - 1], r, u);
+        d_in = (id == 1) ? SCN_INF : o_cyl_distance(rad[id - 2], rsq[id - 2], r, u);
+        if (d_in < d_out) { *surfIdx = PIN_MOVING_IN; return d_in; }
+        *surfIdx = PIN_MOVING_OUT;
+        return d_out;
+    }
+    }
+    o_fail(SCN_ERR_ARG, "distance: unknown universe type %d", U->type);
+    return SCN_INF;
+}
+
+/* universe%cross: root and lattice re-run findCell, pin steps the annulus index */
+static void o_uni_cross(const scn_problem* P, o_coord* c, int surfIdx)
+{
+    const scn_universe* U = o_uni(P, c->uniIdx);
+    if (U->type == SCN_UNI_PIN) { /* pinUniverse_class.f90:236-253 */
+        if (surfIdx == PIN_MOVING_IN) c->localID -= 1;
+        else if (surfIdx == PIN_MOVING_OUT) c->localID += 1;
+        else o_fail(SCN_ERR_ARG, "pin cross: unknown surface memento %d", surfIdx);
+    } else {
+        c->localID = orc_uni_findcell(P, c->uniIdx, c->r, c->dir);
+        c->cellIdx = 0;
+    }
+}
+
+/* universe%cellOffset: lattice cell centre (latUniverse_class.f90:389-414), zero otherwise */
+static void o_uni_offset(const scn_problem* P, const o_coord* c, double off[3])
+{
+    const scn_universe* U = o_uni(P, c->uniIdx);
+    off[0] = off[1] = off[2] = 0.0;
+    if (U->type != SCN_UNI_LAT) return;
+    int doOffset = (U->offset_mode == 2) ? (P->int_data[U->offset_map + c->localID - 1] == 1)
+                                         : (U->offset_mode == 1);
+    if (doOffset && c->localID != U->out_local) {
+        int ijk[3];
+        o_get_ijk(c->localID, U->size, ijk);
+        for (int i = 0; i < 3; ++i) off[i] = ((double)ijk[i] - 0.5) * U->pitch[i] + U->corner[i];
+    }
+}
+
+/* universe_inter.f90:400-424 enter (no rotation) */
+static void o_uni_enter(const scn_problem* P, int uniIdx, o_coord* nw, const double r[3], const double u[3])
+{
+    const scn_universe* U = o_uni(P, uniIdx);
+    for (int i = 0; i < 3; ++i) {
+        nw->r[i] = r[i];
+        nw->dir[i] = u[i];
+    }
+    nw->uniIdx = uniIdx;
+    for (int i = 0; i < 3; ++i) nw->r[i] = nw->r[i] - U->origin[i];
+    nw->localID = orc_uni_findcell(P, uniIdx, nw->r, nw->dir);
+    nw->cellIdx = 0;
+}
+
+/* ======================================================================================
+ * geometryStd (Geometry/geometryStd_class.f90)
+ * ====================================================================================== */
+
+/* coord_class.f90 takeAboveGeom / moveGlobal / moveLocal */
+static void o_take_above(o_coords* c) { c->nesting = 1; c->matIdx = SCN_UNDEF_MAT; c->uniqueID = -3; }
+static void o_move_global(o_coords* c, double d)
+{
+    o_take_above(c);
+    for (int i = 0; i < 3; ++i) c->lvl[0].r[i] = c->lvl[0].r[i] + d * c->lvl[0].dir[i];
+}
+static void o_move_local(o_coords* c, double d, int n)
+{
+    c->nesting = n;
+    for (int l = 0; l < n; ++l)
+        for (int i = 0; i < 3; ++i) c->lvl[l].r[i] = c->lvl[l].r[i] + d * c->lvl[l].dir[i];
+}
+
+/* geometryStd_class.f90:565-619 diveToMat (level is 1-based) */
+static void o_dive(const scn_problem* P, o_coords* c, int start)
+{
+    for (int i = start; i <= SCN_MAX_NEST; ++i) {
+        o_coord* L = &c->lvl[i - 1];
+        int pos = L->uniRootID + L->localID - 1;
+        if (pos < 1 || pos > P->graph_size) { o_fail(SCN_ERR_ARG, "diveToMat: graph index %d", pos); return; }
+        int fill = P->graph_fill[pos - 1];
+        int id = P->graph_id[pos - 1];
+        if (fill >= 0) {
+            c->matIdx = fill;
+            c->uniqueID = id;
+            return;
+        }
+        if (i == SCN_MAX_NEST) break;
+        double off[3], r[3];
+        o_uni_offset(P, L, off);
+        for (int k = 0; k < 3; ++k) r[k] = L->r[k] - off[k];
+        c->nesting += 1;
+        o_uni_enter(P, -fill, &c->lvl[i], r, L->dir);
+        c->lvl[i].uniRootID = id;
+    }
+    o_fail(SCN_ERR_ARG, "diveToMat: failed to find material cell");
+}
+
+/* geometryStd_class.f90:119-147 placeCoord */
+static void o_place_coord(const scn_problem* P, o_coords* c)
+{
+    double r[3], u[3];
+    o_take_above(c);
+    for (int i = 0; i < 3; ++i) { r[i] = c->lvl[0].r[i]; u[i] = c->lvl[0].dir[i]; }
+    o_uni_enter(P, P->root_uni, &c->lvl[0], r, u);
+    c->lvl[0].uniRootID = 1;
+    o_dive(P, c, 1);
+}
+
+typedef struct o_cache { int lvl; double dist[SCN_MAX_NEST]; int surf[SCN_MAX_NEST]; } o_cache;
+
+/* geometryStd_class.f90:633-717 closestDist / closestDist_cache (tie favours the outer level) */
+static void o_closest(const scn_problem* P, const o_coords* c, o_cache* cache, int use_cache,
+                      double* dist, int* surfIdx, int* lvl)
+{
+    *dist = SCN_INF;
+    *surfIdx = 0;
+    *lvl = 0;
+    for (int l = 1; l <= c->nesting; ++l) {
+        double td;
+        int ts;
+        if (use_cache) {
+            if (cache->lvl < l) {
+                cache->dist[l - 1] = orc_uni_distance(P, c->lvl[l - 1].uniIdx, c->lvl[l - 1].localID,
+                                                      c->lvl[l - 1].r, c->lvl[l - 1].dir, &cache->surf[l - 1]);
+                cache->lvl += 1;
+            }
+            td = cache->dist[l - 1];
+            ts = cache->surf[l - 1];
+        } else {
+            td = orc_uni_distance(P, c->lvl[l - 1].uniIdx, c->lvl[l - 1].localID, c->lvl[l - 1].r,
+                                  c->lvl[l - 1].dir, &ts);
+        }
+        if ((*dist - td) >= *dist * SCN_FP_REL_TOL) {
+            *dist = td;
+            *surfIdx = ts;
+            *lvl = l;
+        }
+    }
+}
+
+enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
+
+/* geometryStd_class.f90:214-352 move_withCache / move_noCache (no fields: fieldDist = INF) */
+static int o_move(const scn_problem* P, o_coords* c, double* maxDist, o_cache* cache, int use_cache)
+{
+    double dist;
+    int surfIdx, level;
+    o_closest(P, c, cache, use_cache, &dist, &surfIdx, &level);
+    if (*maxDist < dist && *maxDist < SCN_INF) { /* fieldDist = INF without fields */
+        o_move_local(c, *maxDist, c->nesting);
+        cache->lvl = 0;
+        return EV_COLL;
+    } else if (surfIdx == SURF_BORDER && level == 1) {
+        o_move_global(c, dist);
+        *maxDist = dist;
+        cache->lvl = 0;
+        orc_box_explicit_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
+        o_place_coord(P, c);
+        return EV_BOUNDARY;
+    } else {
+        o_move_local(c, dist, level);
+        *maxDist = dist;
+        for (int l = 0; l < level - 1; ++l) cache->dist[l] = cache->dist[l] - dist;
+        cache->lvl = level - 1;
+        o_uni_cross(P, &c->lvl[level - 1], surfIdx);
+        o_dive(P, c, level);
+        return EV_CROSS;
+    }
+}
+
+/* geometryStd_class.f90:492-514 teleport */
+static void o_teleport(const scn_problem* P, o_coords* c, double dist)
+{
+    o_move_global(c, dist);
+    o_place_coord(P, c);
+    if (c->matIdx == SCN_OUTSIDE_MAT) {
+        orc_box_transform_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
+        o_place_coord(P, c);
+    }
+}
+
+static void o_export_place(const o_coords* c, orc_place_out* out)
+{
+    out->matIdx = c->matIdx;
+    out->uniqueID = c->uniqueID;
+    out->nesting = c->nesting;
+    for (int l = 0; l < SCN_MAX_NEST; ++l) {
+        out->localID[l] = (l < c->nesting) ? c->lvl[l].localID : 0;
+        out->uniIdx[l] = (l < c->nesting) ? c->lvl[l].uniIdx : 0;
+        for (int i = 0; i < 3; ++i) out->r[l][i] = (l < c->nesting) ? c->lvl[l].r[i] : 0.0;
+    }
+}
+
+static void o_init_coords(o_coords* c, const double r[3], const double u[3])
+{
+    memset(c, 0, sizeof *c);
+    o_take_above(c);
+    for (int i = 0; i < 3; ++i) { c->lvl[0].r[i] = r[i]; c->lvl[0].dir[i] = u[i]; }
+}
+
+int orc_place(const scn_problem* P, const double r[3], const double u[3], orc_place_out* out)
+{
+    o_coords c;
+    o_clear();
+    o_init_coords(&c, r, u);
+    o_place_coord(P, &c);
+    o_export_place(&c, out);
+    return g_err;
+}
+
+int orc_move(const scn_problem* P, const double r[3], const double u[3], double max_dist,
+             orc_place_out* out, double u_out[3], double* dist_out, int32_t* event)
+{
+    o_coords c;
+    o_cache cache;
+    o_clear();
+    cache.lvl = 0;
+    o_init_coords(&c, r, u);
+    o_place_coord(P, &c);
+    double d = max_dist;
+    *event = o_move(P, &c, &d, &cache, 1);
+    *dist_out = d;
+    o_export_place(&c, out);
+    for (int i = 0; i < 3; ++i) u_out[i] = c.lvl[0].dir[i];
+    return g_err;
+}
+
+int orc_teleport(const scn_problem* P, const double r[3], const double u[3], double dist,
+                 orc_place_out* out, double u_out[3])
+{
+    o_coords c;
+    o_clear();
+    o_init_coords(&c, r, u);
+    o_place_coord(P, &c);
+    o_teleport(P, &c, dist);
+    o_export_place(&c, out);
+    for (int i = 0; i < 3; ++i) u_out[i] = c.lvl[0].dir[i];
+    return g_err;
+}
+
+/* ======================================================================================
+ * Nuclear data (baseMgNeutronDatabase_class.f90, baseMgNeutronMaterial_class.f90)
+ * ====================================================================================== */
+enum { XS_TOTAL = 0, XS_SCATTER = 1, XS_CAPTURE = 2, XS_FISSION = 3, XS_NUFISSION = 4, XS_KAPPA = 5 };
+
+static const double* o_xs(const scn_problem* P, int mat, int G)
+{
+    return P->mg.xs + ((size_t)(mat - 1) * P->mg.n_groups + (G - 1)) * 6;
+}
+
+/* getTrackMatXS (baseMgNeutronDatabase_class.f90:143-165) */
+static double o_track_mat_xs(const scn_problem* P, int mat, int G)
+{
+    if (mat == SCN_VOID_MAT) return 0.0;
+    if (mat < 1 || mat > P->mg.n_mat) {
+        o_fail(SCN_ERR_MATERIAL, "Particle is in an undefined material with index: %d", mat);
+        return 0.0;
+    }
+    return o_xs(P, mat, G)[XS_TOTAL] + 0.0; /* + getAlphaAbsorption() = 0 */
+}
+
+/* neutronMacroXSs%get (neutronXsPackages_class.f90) restricted to MG macro codes */
+static double o_macro_get(const double* x, int MT)
+{
+    switch (MT) {
+    case SCN_MT_TOTAL: return x[XS_TOTAL];
+    case SCN_MT_DISAPPEARANCE: return x[XS_CAPTURE];
+    case SCN_MT_ESCATTER: return 0.0;
+    case SCN_MT_NONELASTIC: return x[XS_SCATTER] + x[XS_FISSION] + x[XS_CAPTURE];
+    case SCN_MT_IESCATTER: return x[XS_SCATTER];
+    case SCN_MT_ALLSCATTER: return 0.0 + x[XS_SCATTER];
+    case SCN_MT_FISSION: return x[XS_FISSION];
+    case SCN_MT_NUFISSION: return x[XS_NUFISSION];
+    case SCN_MT_KAPPA: return x[XS_KAPPA];
+    case SCN_MT_ABSORPTION: return x[XS_FISSION] + x[XS_CAPTURE];
+    default: return 0.0;
+    }
+}
+
+/* ======================================================================================
+ * Particle, tallies and the per-history loop
+ * ====================================================================================== */
+typedef struct o_site_buf { scn_particle_state* v; int64_t n, cap; } o_site_buf;
+
+typedef struct o_part {
+    o_coords coords;
+    int G;
+    double w, w0, time, k_eff, track_xs, preColl_w;
+    double E, lambda;
+    int type, isMG;
+    uint64_t rng;
+    int isDead, fate, broodID, collisionN;
+} o_part;
+
+typedef struct o_hctx {
+    const scn_problem* P;
+    double* bins;          /* collision-clerk BIN memory, or NULL */
+    double keff[4];        /* per-history IMP_PROD, SCATTER_PROD, IMP_ABS, ANA_LEAK */
+    o_site_buf* sites;
+    orc_hist* h;
+    int64_t n_events;
+} o_hctx;
+
+enum { K_IMP_PROD = 0, K_SCATTER_PROD = 1, K_IMP_ABS = 2, K_ANA_LEAK = 3 };
+enum { FATE_NO = 5000, FATE_ABS = 5001, FATE_LEAK = 5002 };
+
+/* spaceMap / materialMap / multiMap (Tallies/TallyMaps) on the particle's global state */
+static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
+{
+    if (m->type == SCN_MAP_SPACE) {
+        double v = p->coords.lvl[0].r[m->axis];
+        int idx;
+        if (m->grid_type == SCN_GRID_LIN) { /* grid_class.f90:154-175 */
+            const double* g = P->real_data + m->data;
+            idx = (int)floor((v - g[0]) / g[1]) + 1;
+        } else { /* binarySearch: last bin edge <= value */
+            const double* g = P->real_data + m->data;
+            int lo = 0, hi = m->nbins; /* edges g[0..nbins] */
+            if (v < g[0] || v > g[m->nbins]) idx = -1;
+            else {
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) / 2;
+                    if (v >= g[mid]) lo = mid; else hi = mid;
+                }
+                idx = lo + 1;
+            }
+        }
+        if (idx < 1 || idx >= m->nbins + 1) idx = 0;
+        return idx;
+    } else {
+        int mat = p->coords.matIdx;
+        int nm = P->mg.n_mat;
+        return (mat >= 0 && mat <= nm) ? P->int_data[m->data + mat] : P->int_data[m->data + nm + 1];
+    }
+}
+
+/* tallyAdmin%reportInColl (tallyAdmin_class.f90:466-486): attachment (keffImplicitClerk,
+ * keffImplicitClerk_class.f90:180-223) first, then the collision clerks (collisionClerk_class.f90:192-242) */
+static void o_report_in_coll(o_hctx* X, const o_part* p, int isVirtual)
+{
+    const scn_problem* P = X->P;
+    int mat = p->coords.matIdx;
+    if (mat != SCN_VOID_MAT) {
+        double flux = p->w / p->track_xs;
+        const double* x = o_xs(P, mat, p->G);
+        double nuFiss = x[XS_NUFISSION];
+        double absXS = x[XS_CAPTURE] + x[XS_FISSION];
+        X->keff[K_IMP_PROD] = X->keff[K_IMP_PROD] + nuFiss * flux;
+        X->keff[K_IMP_ABS] = X->keff[K_IMP_ABS] + absXS * flux;
+    }
+    if (!X->bins) return;
+    for (int ci = 0; ci < P->n_clerks; ++ci) {
+        const scn_clerk* C = &P->clerks[ci];
+        if (!C->handle_virtual && isVirtual) continue;
+        if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
+        int bin = 1;
+        if (C->n_maps > 0) {
+            bin = 1;
+            for (int k = 0; k < C->n_maps; ++k) {
+                int b = o_map1d(P, &C->maps[k], p);
+                if (b == 0) { bin = 0; break; }
+                bin = bin + (b - 1) * C->multi[k];
+            }
+        }
+        if (bin == 0) continue;
+        double flux = C->handle_virtual ? p->w / p->track_xs : p->w / o_track_mat_xs(P, mat, p->G);
+        int64_t addr = C->mem_addr + (int64_t)C->width * (bin - 1);
+        for (int i = 0; i < C->width; ++i) {
+            double resp;
+            if (C->resp[i] == SCN_MT_FLUX) resp = 1.0;
+            else resp = (mat == SCN_VOID_MAT) ? 0.0 : o_macro_get(o_xs(P, mat, p->G), C->resp[i]);
+            X->bins[addr + i] = X->bins[addr + i] + resp * flux;
+        }
+    }
+}
+
+/* tallyAdmin%reportHist -> keffImplicitClerk%reportHist (keffImplicitClerk_class.f90:289-299) */
+static void o_report_hist(o_hctx* X, const o_part* p)
+{
+    if (p->fate == FATE_LEAK) X->keff[K_ANA_LEAK] = X->keff[K_ANA_LEAK] + p->w;
+}
+
+/* transportOperatorST_class.f90:48-166 surfaceTracking */
+static void o_surface_tracking(o_hctx* X, o_part* p)
+{
+    const scn_problem* P = X->P;
+    o_cache cache;
+    cache.lvl = 0;
+    for (;;) {
+        double sigmaTrack, sigmaT, dist, invSigmaTrack;
+        int mat = p->coords.matIdx;
+        /* getTrackingXS(MATERIAL_XS), baseMgNeutronDatabase_class.f90:95-137 */
+        if (mat == SCN_VOID_MAT) sigmaTrack = P->mg.collision_xs;
+        else sigmaTrack = scn_fmax(o_track_mat_xs(P, mat, p->G), P->mg.collision_xs);
+        p->track_xs = sigmaTrack;
+        if (sigmaTrack < SCN_ST_TOL) {
+            dist = SCN_INF;
+            invSigmaTrack = SCN_INF;
+            sigmaT = 0.0;
+        } else {
+            invSigmaTrack = 1.0 / sigmaTrack;
+            dist = -scn_log(scn_rng_get(&p->rng)) * invSigmaTrack;
+            sigmaT = o_track_mat_xs(P, mat, p->G);
+            if (dist != dist) { o_fail(SCN_ERR_NAN, "Distance is NaN"); p->isDead = 1; return; }
+        }
+        int event = o_move(P, &p->coords, &dist, &cache, P->st_cache);
+        X->n_events++;
+        p->time = p->time + dist / 1.0;
+        if (event == EV_COLL) p->fate = FATE_NO;
+        mat = p->coords.matIdx;
+        if (mat == SCN_OUTSIDE_MAT) {
+            p->isDead = 1;
+            p->fate = FATE_LEAK;
+        } else if (mat == SCN_UNDEF_MAT) {
+            o_fail(SCN_ERR_MATERIAL, "Particle is in undefined material");
+            p->isDead = 1;
+            return;
+        } else if (mat == SCN_OVERLAP_MAT) {
+            o_fail(SCN_ERR_MATERIAL, "Particle is in overlapping cells");
+            p->isDead = 1;
+            return;
+        }
+        if (p->isDead) break;
+        if (event == EV_COLL) {
+            if (scn_rng_get(&p->rng) < sigmaT * invSigmaTrack) break;
+            o_report_in_coll(X, p, 1);
+        }
+        if (g_err) { p->isDead = 1; return; }
+    }
+}
+
+/* transportOperatorDT_class.f90:47-130 deltaTracking */
+static void o_delta_tracking(o_hctx* X, o_part* p)
+{
+    const scn_problem* P = X->P;
+    /* getTrackingXS(MAJORANT_XS) */
+    double maj = scn_fmax(P->mg.majorant[p->G - 1] + 0.0, P->mg.collision_xs);
+    p->track_xs = maj;
+    double majorant_inv = 1.0 / maj;
+    if (fabs(majorant_inv) > SCN_HUGE) { o_fail(SCN_ERR_MAJORANT, "Majorant is 0"); p->isDead = 1; return; }
+    for (;;) {
+        double distance = -scn_log(scn_rng_get(&p->rng)) * majorant_inv;
+        o_teleport(P, &p->coords, distance);
+        X->n_events++;
+        p->time = p->time + distance / 1.0;
+        int mat = p->coords.matIdx;
+        if (mat == SCN_OUTSIDE_MAT) {
+            p->fate = FATE_LEAK;
+            p->isDead = 1;
+            break;
+        } else if (mat == SCN_VOID_MAT) {
+            o_report_in_coll(X, p, 1);
+            continue;
+        } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT) {
+            o_fail(SCN_ERR_MATERIAL, "Particle is in undefined or overlapping material");
+            p->isDead = 1;
+            return;
+        }
+        double sigmaT = o_track_mat_xs(P, mat, p->G);
+        if (scn_rng_get(&p->rng) < sigmaT * majorant_inv) break;
+        o_report_in_coll(X, p, 1);
+        if (g_err) { p->isDead = 1; return; }
+    }
+}
+
+/* transportOperator_inter.f90:105-130 transport */
+static void o_transport(o_hctx* X, o_part* p)
+{
+    if (X->P->transport == SCN_TRANSPORT_DT) o_delta_tracking(X, p);
+    else o_surface_tracking(X, p);
+    if (p->isDead) o_report_hist(X, p);
+}
+
+static void o_site_push(o_site_buf* b, const scn_particle_state* s)
+{
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 1024;
+        b->v = (scn_particle_state*)realloc(b->v, sizeof(scn_particle_state) * (size_t)b->cap);
+    }
+    b->v[b->n++] = *s;
+}
+
+/* collisionProcessor_inter.f90:114-194 collide with neutronMGstd_class.f90:85-282 */
+static void o_collide(o_hctx* X, o_part* p)
+{
+    const scn_problem* P = X->P;
+    const int nG = P->mg.n_groups;
+    int mat = p->coords.matIdx;
+    X->n_events++;
+    /* sampleCollision */
+    double denom = o_track_mat_xs(P, mat, p->G);
+    double probAlpha = 0.0 / denom;
+    if (scn_rng_get(&p->rng) < probAlpha) { o_fail(SCN_ERR_UNSUPPORTED, "alpha absorption"); p->isDead = 1; return; }
+    const double* x = o_xs(P, mat, p->G);
+    double r = scn_rng_get(&p->rng);
+    /* invert_macroXSs (neutronXsPackages_class.f90:211-…) */
+    int C = 1;
+    double xs = x[XS_TOTAL] * r - 0.0;
+    if (xs > 0.0) C++;
+    xs = xs - x[XS_SCATTER];
+    if (xs > 0.0) C++;
+    xs = xs - x[XS_CAPTURE];
+    if (xs > 0.0) C++;
+    o_report_in_coll(X, p, 0);
+    p->preColl_w = p->w;
+    /* implicit: fission site production */
+    if (P->mg.fissile[mat - 1]) {
+        double wgt = p->w, w0 = p->w0, k_eff = p->k_eff;
+        double rand1 = scn_rng_get(&p->rng);
+        double sig_tot = x[XS_TOTAL], sig_nuFiss = x[XS_NUFISSION];
+        int n = (int)(fabs((wgt * sig_nuFiss) / (w0 * sig_tot * k_eff)) + rand1);
+        if (n >= 1) {
+            wgt = scn_sign(w0, wgt);
+            const double* chi = P->mg.chi + (size_t)(mat - 1) * nG;
+            for (int i = 0; i < n; ++i) {
+                double mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
+                double phi = SCN_TWO_PI * scn_rng_get(&p->rng);
+                double rem = scn_rng_get(&p->rng);
+                int G_out;
+                for (G_out = 1; G_out <= nG; ++G_out) {
+                    rem = rem - chi[G_out - 1];
+                    if (rem < 0.0) break;
+                }
+                if (G_out > nG) { o_fail(SCN_ERR_SAMPLING, "fissionMG sampleOut: unnormalised chi"); p->isDead = 1; return; }
+                scn_particle_state s;
+                memset(&s, 0, sizeof s);
+                scn_rotate(p->coords.lvl[0].dir, mu, phi, s.dir);
+                for (int k = 0; k < 3; ++k) s.r[k] = p->coords.lvl[0].r[k];
+                s.G = G_out;
+                s.wgt = wgt;
+                s.collisionN = 0;
+                s.E = p->E;
+                s.isMG = 1;
+                s.type = p->type;
+                s.time = p->time;
+                s.lambda = p->lambda;
+                s.matIdx = mat;
+                s.uniqueID = p->coords.uniqueID;
+                s.cellIdx = 0;
+                s.broodID = p->broodID;
+                o_site_push(X->sites, &s);
+                if (X->h) X->h->n_sites++;
+            }
+        }
+    }
+    if (C == 2) { /* inelastic: multiScatterMG sampleOut */
+        double rem = scn_rng_get(&p->rng) * x[XS_SCATTER];
+        const double* P0 = P->mg.P0 + ((size_t)(mat - 1) * nG + (p->G - 1)) * nG;
+        const double* prod = P->mg.prod + ((size_t)(mat - 1) * nG + (p->G - 1)) * nG;
+        int G_out;
+        for (G_out = 1; G_out <= nG; ++G_out) {
+            rem = rem - P0[G_out - 1];
+            if (rem < 0.0) break;
+        }
+        if (G_out > nG) { o_fail(SCN_ERR_SAMPLING, "multiScatterMG sampleGout: sampling failed"); p->isDead = 1; return; }
+        double mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(&p->rng);
+        double w_mul = prod[G_out - 1];
+        p->G = G_out;
+        p->w = p->w * w_mul;
+        double nd[3];
+        scn_rotate(p->coords.lvl[0].dir, mu, phi, nd);
+        for (int l = 0; l < p->coords.nesting; ++l)
+            for (int k = 0; k < 3; ++k) p->coords.lvl[l].dir[k] = nd[k];
+    } else if (C == 3 || C == 4) {
+        p->isDead = 1;
+    } else {
+        o_fail(SCN_ERR_UNSUPPORTED, "Unsupported MT number: %d", -3);
+        p->isDead = 1;
+        return;
+    }
+    p->collisionN += 1;
+    /* reportOutColl -> keffImplicitClerk (scattering weight gain) */
+    if (C == 2) {
+        double score = scn_fmax(p->w - p->preColl_w, 0.0);
+        if (score > 0.0) X->keff[K_SCATTER_PROD] = X->keff[K_SCATTER_PROD] + score;
+    }
+    if (X->h) X->h->n_coll++;
+    if (p->isDead) {
+        p->fate = FATE_ABS;
+        o_report_hist(X, p);
+    }
+}
+
+/* eigenPhysicsPackage_class.f90:211-252, one history n (1-based) */
+static void o_history(o_hctx* X, const scn_particle_state* s, int n, uint64_t master, double k_eff)
+{
+    o_part p;
+    memset(&p, 0, sizeof p);
+    p.rng = scn_rng_stride(master, n);
+    o_init_coords(&p.coords, s->r, s->dir);
+    p.w = s->wgt;
+    p.w0 = s->wgt;
+    p.G = s->G;
+    p.E = s->E;
+    p.isMG = 1;
+    p.type = s->type;
+    p.time = s->time;
+    p.lambda = s->lambda;
+    p.collisionN = s->collisionN;
+    p.broodID = n;
+    p.fate = FATE_NO;
+    o_place_coord(X->P, &p.coords);
+    p.k_eff = k_eff;
+    p.preColl_w = p.w;
+    X->keff[0] = X->keff[1] = X->keff[2] = X->keff[3] = 0.0;
+    for (;;) {
+        o_transport(X, &p);
+        if (p.isDead || g_err) break;
+        o_collide(X, &p);
+        if (p.isDead || g_err) break;
+    }
+    if (X->h) {
+        X->h->n_events = X->n_events;
+        X->h->rng_end = p.rng;
+        X->h->w_end = p.w;
+        X->h->fate = p.fate;
+        X->h->G_end = p.G;
+        for (int k = 0; k < 3; ++k) X->h->r_end[k] = p.coords.lvl[0].r[k];
+    }
+}
+
+int orc_eigen_cycle(const scn_problem* P, const scn_particle_state* bank_in, int32_t n_in,
+                    int64_t rng_state, double k_eff, scn_particle_state* bank_out, int32_t cap,
+                    int32_t* n_out, double* bin_scores, double* keff_scores, orc_hist* hist,
+                    int32_t n_threads)
+{
+    o_clear();
+    if (n_in < 0 || !bank_in) { o_fail(SCN_ERR_ARG, "bad bank"); return g_err; }
+    int nt = 1;
+#ifdef _OPENMP
+    if (n_threads > 0) nt = n_threads;
+#else
+    (void)n_threads;
+#endif
+    double* hk = (double*)calloc((size_t)n_in * 4 + 1, sizeof(double));
+    int64_t* hs_start = (int64_t*)calloc((size_t)n_in + 1, sizeof(int64_t));
+    int32_t* hs_count = (int32_t*)calloc((size_t)n_in + 1, sizeof(int32_t));
+    int32_t* hs_thr = (int32_t*)calloc((size_t)n_in + 1, sizeof(int32_t));
+    o_site_buf* bufs = (o_site_buf*)calloc((size_t)nt, sizeof(o_site_buf));
+    double** tbins = (double**)calloc((size_t)nt, sizeof(double*));
+    if (bin_scores) {
+        memset(bin_scores, 0, sizeof(double) * (size_t)P->n_bins);
+        tbins[0] = bin_scores;
+        for (int t = 1; t < nt; ++t) tbins[t] = (double*)calloc((size_t)P->n_bins + 1, sizeof(double));
+    }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    for (int n = 1; n <= n_in; ++n) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        if (g_err) continue;
+        o_hctx X;
+        memset(&X, 0, sizeof X);
+        X.P = P;
+        X.bins = bin_scores ? tbins[t] : NULL;
+        X.sites = &bufs[t];
+        X.h = hist ? &hist[n - 1] : NULL;
+        if (X.h) memset(X.h, 0, sizeof *X.h);
+        int64_t start = bufs[t].n;
+        o_history(&X, &bank_in[n - 1], n, (uint64_t)rng_state, k_eff);
+        hs_start[n - 1] = start;
+        hs_count[n - 1] = (int32_t)(bufs[t].n - start);
+        hs_thr[n - 1] = t;
+        for (int k = 0; k < 4; ++k) hk[(size_t)k * n_in + (n - 1)] = X.keff[k];
+    }
+    int64_t total = 0;
+    for (int n = 0; n < n_in; ++n) total += hs_count[n];
+    if (!g_err && total > cap) o_fail(SCN_ERR_OVERFLOW, "Run out of space for particles. Max size: %d", cap);
+    if (!g_err) {
+        int64_t pos = 0;
+        for (int n = 0; n < n_in; ++n)
+            for (int j = 0; j < hs_count[n]; ++j) bank_out[pos++] = bufs[hs_thr[n]].v[hs_start[n] + j];
+        *n_out = (int32_t)total;
+        if (keff_scores)
+            for (int k = 0; k < 4; ++k) keff_scores[k] = orc_dsum(hk + (size_t)k * n_in, n_in);
+        if (bin_scores)
+            for (int t = 1; t < nt; ++t)
+                for (int64_t i = 0; i < P->n_bins; ++i) bin_scores[i] = bin_scores[i] + tbins[t][i];
+    }
+    for (int t = 0; t < nt; ++t) free(bufs[t].v);
+    for (int t = 1; t < nt; ++t) free(tbins[t]);
+    free(tbins);
+    free(bufs);
+    free(hk);
+    free(hs_start);
+    free(hs_count);
+    free(hs_thr);
+    return g_err;
+}
+
+/* ======================================================================================
+ * Population control: particleDungeon_class.f90:431-602 normSize_Repr (one rank) with
+ * heapQueue_class.f90:55-178 and sortByBroodID :923-981
+ * ====================================================================================== */
+typedef struct o_heap { double* v; int n, cap; } o_heap;
+
+static void o_heap_sift_up(o_heap* h, int i)
+{
+    while (i > 0) {
+        int par = (i - 1) / 2;
+        if (h->v[par] >= h->v[i]) break;
+        double t = h->v[par]; h->v[par] = h->v[i]; h->v[i] = t;
+        i = par;
+    }
+}
+static void o_heap_sift_down(o_heap* h, int i)
+{
+    for (;;) {
+        int l = 2 * i + 1, r = l + 1, m = i;
+        if (l < h->n && h->v[l] > h->v[m]) m = l;
+        if (r < h->n && h->v[r] > h->v[m]) m = r;
+        if (m == i) break;
+        double t = h->v[m]; h->v[m] = h->v[i]; h->v[i] = t;
+        i = m;
+    }
+}
+/* pushReplace: push while not full, otherwise replace the maximum */
+static void o_heap_push_replace(o_heap* h, double x)
+{
+    if (h->n < h->cap) {
+        h->v[h->n++] = x;
+        o_heap_sift_up(h, h->n - 1);
+    } else {
+        h->v[0] = x;
+        o_heap_sift_down(h, 0);
+    }
+}
+
+static void o_sort_by_brood(scn_particle_state* bank, int n)
+{
+    int maxB = 0;
+    for (int i = 0; i < n; ++i) if (bank[i].broodID > maxB) maxB = bank[i].broodID;
+    if (maxB == 0) return;
+    int* cnt = (int*)calloc((size_t)maxB + 2, sizeof(int));
+    for (int i = 0; i < n; ++i) cnt[bank[i].broodID]++;
+    int loc = 0;
+    for (int b = 0; b <= maxB; ++b) { int c = cnt[b]; cnt[b] = loc; loc += c; }
+    scn_particle_state* tmp = (scn_particle_state*)malloc(sizeof(scn_particle_state) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) tmp[cnt[bank[i].broodID]++] = bank[i];
+    memcpy(bank, tmp, sizeof(scn_particle_state) * (size_t)n);
+    free(tmp);
+    free(cnt);
+}
+
+int orc_norm_size_repr(scn_particle_state* bank, int32_t* np, int32_t cap, int32_t tot_pop, int64_t rng_state)
+{
+    o_clear();
+    int pop = *np;
+    o_sort_by_brood(bank, pop);
+    int totSites = pop;
+    if (totSites == 0) { o_fail(SCN_ERR_NORM, "Normalisation failed: no fission sites"); return g_err; }
+    int excess = totSites - tot_pop;
+    int heapSize = (excess < 0) ? ((-excess) % totSites) : excess;
+    double threshold = 1.0;
+    uint64_t seed = 0;
+    if (heapSize != 0) {
+        uint64_t mr = (uint64_t)rng_state;
+        o_heap h;
+        h.cap = heapSize;
+        h.n = 0;
+        h.v = (double*)malloc(sizeof(double) * (size_t)heapSize);
+        o_heap_push_replace(&h, 2.0);
+        seed = mr;
+        for (int j = 0; j < pop; ++j) {
+            double rn = scn_rng_get(&mr);
+            if (rn < h.v[0]) o_heap_push_replace(&h, rn);
+        }
+        threshold = h.v[0];
+        free(h.v);
+    }
+    uint64_t rr = seed;
+    if (excess > 0) {
+        int count = 0;
+        for (int i = 0; i < pop; ++i)
+            if (scn_rng_get(&rr) > threshold) bank[count++] = bank[i];
+        pop = count;
+    } else if (excess < 0) {
+        int n_copies = (-excess) / totSites;
+        int n_dup = (-excess) % totSites;
+        int64_t need = (int64_t)pop * (n_copies + 1) + n_dup;
+        if (need > cap) { o_fail(SCN_ERR_OVERFLOW, "normSize_Repr: bank capacity %d too small", cap); return g_err; }
+        for (int c = 1; c <= n_copies; ++c) memcpy(bank + (size_t)pop * c, bank, sizeof(scn_particle_state) * (size_t)pop);
+        int count = pop * (n_copies + 1);
+        if (n_dup != 0)
+            for (int i = 0; i < pop; ++i)
+                if (scn_rng_get(&rr) <= threshold) bank[count++] = bank[i];
+        pop = count;
+        o_sort_by_brood(bank, pop);
+    }
+    if (pop != tot_pop) { o_fail(SCN_ERR_NORM, "Normalisation failed!"); return g_err; }
+    *np = pop;
+    return 0;
+}
+
+/* ======================================================================================
+ * Sources
+ * ====================================================================================== */
+
+/* fissionSource_class.f90:149-271 via source_inter.f90:98-119 (particle i uses stride(i)) */
+int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const double bottom[3],
+                       const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank)
+{
+    o_clear();
+    (void)g_in;
+    const int nG = P->mg.n_groups;
+    for (int i = 1; i <= n && !g_err; ++i) {
+        uint64_t rng = scn_rng_stride((uint64_t)rng_state, i);
+        scn_particle_state* s = &bank[i - 1];
+        memset(s, 0, sizeof *s);
+        int it = 0;
+        for (;;) {
+            if (++it > attempts) { o_fail(SCN_ERR_SAMPLING, "Failed to find a fissile material"); break; }
+            double r3[3], r[3];
+            for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&rng);
+            for (int k = 0; k < 3; ++k) r[k] = (top[k] - bottom[k]) * r3[k] + bottom[k];
+            o_coords c;
+            double u0[3] = {1.0, 0.0, 0.0};
+            o_init_coords(&c, r, u0);
+            o_place_coord(P, &c);
+            int mat = c.matIdx;
+            if (mat == SCN_VOID_MAT || mat == SCN_OUTSIDE_MAT) continue;
+            if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT) { o_fail(SCN_ERR_MATERIAL, "source in undefined material"); break; }
+            if (!P->mg.fissile[mat - 1]) continue;
+            s->matIdx = mat;
+            s->uniqueID = c.uniqueID;
+            s->wgt = 1.0;
+            s->time = 0.0;
+            s->type = 1;
+            for (int k = 0; k < 3; ++k) s->r[k] = r[k];
+            double mu = 2.0 * scn_rng_get(&rng) - 1.0;
+            double phi = SCN_TWO_PI * scn_rng_get(&rng);
+            double rem = scn_rng_get(&rng);
+            const double* chi = P->mg.chi + (size_t)(mat - 1) * nG;
+            int G;
+            for (G = 1; G <= nG; ++G) {
+                rem = rem - chi[G - 1];
+                if (rem < 0.0) break;
+            }
+            if (G > nG) { o_fail(SCN_ERR_SAMPLING, "fissionMG sampleOut: unnormalised chi"); break; }
+            s->G = G;
+            s->isMG = 1;
+            double x1[3] = {1.0, 0.0, 0.0};
+            scn_rotate(x1, mu, phi, s->dir);
+            s->lambda = SCN_INF;
+            s->cellIdx = -1;
+            break;
+        }
+    }
+    return g_err;
+}
+
+/* ======================================================================================
+ * Power iteration: eigenPhysicsPackage_class.f90:135-343 with scoreMemory_class.f90 closes
+ * ====================================================================================== */
+static void o_result(double csum, double csum2, int N, double out[2])
+{
+    /* scoreMemory_class.f90:537-573 getResult_withSTD */
+    double mean = csum / N;
+    double inv_N = 1.0 / N;
+    double inv_Nm1 = (N != 1) ? 1.0 / (N - 1) : 1.0;
+    double std = csum2 * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
+    out[0] = mean;
+    out[1] = sqrt(std);
+}
+
+int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
+                  const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads)
+{
+    o_clear();
+    const int pop = S->pop;
+    const int cap = 4 * pop + 16;
+    scn_particle_state* thisC = (scn_particle_state*)malloc(sizeof(scn_particle_state) * (size_t)cap);
+    scn_particle_state* nextC = (scn_particle_state*)malloc(sizeof(scn_particle_state) * (size_t)cap);
+    memcpy(thisC, initial_bank, sizeof(scn_particle_state) * (size_t)pop);
+    int nThis = pop;
+    uint64_t master = (uint64_t)S->seed;
+    master = scn_rng_stride(master, 0);      /* run: stride(getOffset(totalPop)) = 0 */
+    master = scn_rng_stride(master, pop);    /* generateInitialState: stride(totalPop) */
+    double* bins = P->n_bins ? (double*)calloc((size_t)P->n_bins, sizeof(double)) : NULL;
+    double* csumA = P->n_bins ? (double*)calloc((size_t)P->n_bins * 2, sizeof(double)) : NULL;
+    double* csumI = P->n_bins ? (double*)calloc((size_t)P->n_bins * 2, sizeof(double)) : NULL;
+    double kA[2] = {0, 0}, kI[2] = {0, 0}, imp[4][2];
+    memset(imp, 0, sizeof imp);
+    int nA = 0, nI = 0;
+    double dk[2] = {1.0, 1.0};   /* particleDungeon%k_eff of the two dungeons */
+    int cur = 0;                 /* index of thisCycle dungeon */
+    double keff_0 = S->keff0;
+    int cyc = 0;
+    res->n_histories = 0;
+    res->n_collisions = 0;
+    for (int phase = 0; phase < 2 && !g_err; ++phase) {
+        int ncyc = phase == 0 ? S->n_inactive : S->n_active;
+        int score_bins = phase == 1 || S->inactive_tally;
+        double k_new = keff_0;
+        for (int c = 0; c < ncyc && !g_err; ++c) {
+            double startW;
+            {
+                double* w = (double*)malloc(sizeof(double) * (size_t)(nThis > 0 ? nThis : 1));
+                for (int i = 0; i < nThis; ++i) w[i] = thisC[i].wgt;
+                startW = orc_dsum(w, nThis);
+                free(w);
+            }
+            int nNext = 0;
+            double ks[4];
+            orc_hist* hist = NULL;
+            if (orc_eigen_cycle(P, thisC, nThis, (int64_t)master, k_new, nextC, cap, &nNext,
+                                score_bins ? bins : NULL, ks, hist, n_threads))
+                break;
+            res->n_histories += nThis;
+            master = scn_rng_stride(master, (int64_t)pop + 1);
+            double endW;
+            {
+                double* w = (double*)malloc(sizeof(double) * (size_t)(nNext > 0 ? nNext : 1));
+                for (int i = 0; i < nNext; ++i) w[i] = nextC[i].wgt;
+                endW = orc_dsum(w, nNext);
+                free(w);
+            }
+            double k_norm = dk[1 - cur];
+            if (phase == 0) { /* keffAnalogClerk_class.f90:162-181 */
+                double k_eff = endW / startW * k_norm;
+                kA[0] = kA[0] + k_eff;
+                kA[1] = kA[1] + k_eff * k_eff;
+                nA++;
+            } else { /* keffImplicitClerk_class.f90:306-326 */
+                double k_est = ks[0] / (ks[2] + ks[3] - ks[1]);
+                kI[0] = kI[0] + k_est;
+                kI[1] = kI[1] + k_est * k_est;
+                nI++;
+                for (int k = 0; k < 4; ++k) {
+                    imp[k][0] = imp[k][0] + ks[k];
+                    imp[k][1] = imp[k][1] + ks[k] * ks[k];
+                }
+            }
+            if (score_bins) { /* scoreMemory%closeCycle(normFactor = 1) */
+                double* cs = phase == 1 ? csumA : csumI;
+                for (int64_t i = 0; i < P->n_bins; ++i) {
+                    double r = bins[i] * 1.0;
+                    bins[i] = 0.0;
+                    cs[2 * i] = cs[2 * i] + r;
+                    cs[2 * i + 1] = cs[2 * i + 1] + r * r;
+                }
+            }
+            if (orc_norm_size_repr(nextC, &nNext, cap, pop, (int64_t)master)) break;
+            master = scn_rng_stride(master, 1);
+            scn_particle_state* t = thisC; thisC = nextC; nextC = t;
+            nThis = nNext;
+            cur = 1 - cur;
+            double kr[2];
+            if (phase == 0) o_result(kA[0], kA[1], nA, kr);
+            else o_result(kI[0], kI[1], nI, kr);
+            k_new = kr[0];
+            dk[1 - cur] = k_new;
+            keff_0 = k_new;
+            if (res->k_history) res->k_history[cyc] = k_new;
+            cyc++;
+        }
+    }
+    if (!g_err) {
+        if (nA) o_result(kA[0], kA[1], nA, res->k_analog);
+        if (nI) {
+            o_result(kI[0], kI[1], nI, res->k_implicit);
+            for (int k = 0; k < 4; ++k) o_result(imp[k][0], imp[k][1], nI, res->imp[k]);
+        }
+        for (int64_t i = 0; i < P->n_bins; ++i) {
+            if (res->bins_active && nI) {
+                double o[2];
+                o_result(csumA[2 * i], csumA[2 * i + 1], nI, o);
+                res->bins_active[2 * i] = o[0];
+                res->bins_active[2 * i + 1] = o[1];
+            }
+            if (res->bins_inactive && nA && S->inactive_tally) {
+                double o[2];
+                o_result(csumI[2 * i], csumI[2 * i + 1], nA, o);
+                res->bins_inactive[2 * i] = o[0];
+                res->bins_inactive[2 * i + 1] = o[1];
+            }
+        }
+    }
+    free(thisC);
+    free(nextC);
+    free(bins);
+    free(csumA);
+    free(csumI);
+    return g_err;
+}

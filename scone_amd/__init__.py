@@ -1,0 +1,9 @@
+"""MI355X-native engine for SCONE's multigroup tracking loop.
+
+``scone_amd.tables`` compiles SCONE input dictionaries into the flat tables of the C ABI
+(include/scone_gpu.h); ``scone_amd.engine`` loads the HIP library (libscone_amd.so) and drives it;
+``scone_amd.packages`` mirrors SCONE's eigenPhysicsPackage on top of the engine.
+"""
+from . import dictio, tables, configs  # noqa: F401
+
+__all__ = ["dictio", "tables", "configs"]

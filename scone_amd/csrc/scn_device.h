@@ -1,0 +1,896 @@
+// scn_device.h -- device-side physics of the multigroup tracking loop (gfx950).
+//
+// One particle lives in registers as a Pt<NL> (NL = compile-time nesting depth of the geometry,
+// so every level array is statically indexed and stays in VGPRs).  The arithmetic of every
+// routine reproduces the reference's operation order -- the same order the CPU oracle uses --
+// so a history computed here is bit-identical to the oracle's:
+//   surfaces      Geometry/Surfaces/CompositeSurfaces/{box,squareCylinder}_class.f90,
+//                 QuadSurfaces/cylinder_class.f90
+//   universes     Geometry/Universes/{root,lat,pin}Universe_class.f90, universe_inter.f90 enter
+//   geometry      Geometry/geometryStd_class.f90 placeCoord/diveToMat/closestDist_cache/
+//                 move_withCache/teleport, coord_class.f90 moveLocal/moveGlobal
+//   transport     TransportOperator/transportOperator{ST,DT}_class.f90
+//   collision     CollisionOperator/CollisionProcessors/{collisionProcessor_inter,neutronMGstd_class}.f90
+//   tallies       Tallies/TallyClerks/{collisionClerk,keffImplicitClerk}_class.f90 with the maps,
+//                 energyFilter and flux/macro responses
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "scone_gpu.h"
+#include "scn_fpmath.h"
+
+namespace scn {
+
+// ---- problem tables as one by-value kernel argument ---------------------------------------
+struct DevProblem {
+    int transport, st_cache, root_uni, border_box, n_mat, nG, n_clerks, graph_size;
+    const scn_universe* unis;
+    const scn_box* boxes;
+    const int* gfill;
+    const int* gid;
+    const double* real;
+    const int* ints;
+    const double* xs;
+    const double* P0;
+    const double* prod;
+    const double* chi;
+    const int* fissile;
+    const double* maj;
+    double collision_xs;
+    const scn_clerk* clerks;
+    const int* gclerk_off;   // [nG+1] clerks whose energy filter admits group g
+    const int* gclerk_idx;
+    int64_t n_bins;
+};
+
+enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
+enum { SURF_BORDER = 1, PIN_IN = -1, PIN_OUT = -2, LAT_OUTLINE = -7 };
+enum { XS_TOTAL = 0, XS_SCATTER = 1, XS_CAPTURE = 2, XS_FISSION = 3, XS_NUFISSION = 4, XS_KAPPA = 5 };
+enum { FATE_NO = 5000, FATE_ABS = 5001, FATE_LEAK = 5002 };
+enum { K_IMP_PROD = 0, K_SCATTER_PROD = 1, K_IMP_ABS = 2, K_ANA_LEAK = 3 };
+// a flight that needs more segments than this is treated as a lost particle (error, not a hang)
+constexpr int MAX_SEGMENTS_PER_FLIGHT = 1 << 22;
+constexpr int SCN_ERR_LOST_DETAIL = -77;
+
+// error flags raised by kernels (first one wins, host maps to SCN_ERR_*)
+struct DevStatus {
+    int code;
+    int detail;
+};
+
+__device__ __forceinline__ void raise_error(DevStatus* st, int code, int detail)
+{
+    atomicCAS(&st->code, 0, code);
+    st->detail = detail;
+}
+
+template <int NL>
+struct Pt {
+    double r[NL][3];   // position at each nesting level (coordList%lvl(l)%r)
+    double u[3];       // direction (identical at every level: no rotated universes)
+    int nest;
+    int uni[NL], root[NL], loc[NL];
+    int mat, uid;
+    int G;
+    double w, w0, t;
+    uint64_t rng;
+    int brood, nsite, coll;
+    double k[4];       // keffImplicitClerk scores of this history
+    double track;      // trackingCache(1)%xs
+    int fate;
+    int dead;
+};
+
+// register arrays are only ever indexed with compile-time constants (a runtime index would move
+// the whole array to scratch); these select the component of a runtime axis instead
+// (blended through the bit patterns: a plain select of the three loads gets folded into a load from
+// a selected address, which forces the array out of registers)
+__device__ __forceinline__ double get3(const double v[3], int i)
+{
+    uint64_t a = scn_dbits(v[0]), b = scn_dbits(v[1]), c = scn_dbits(v[2]);
+    uint64_t m0 = 0ull - (uint64_t)(i == 0), m1 = 0ull - (uint64_t)(i == 1), m2 = 0ull - (uint64_t)(i == 2);
+    return scn_bitsd((a & m0) | (b & m1) | (c & m2));
+}
+__device__ __forceinline__ void set3(double v[3], int i, double x)
+{
+    if (i == 0) v[0] = x;
+    else if (i == 1) v[1] = x;
+    else v[2] = x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// surfaces
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double box_evaluate(const scn_box& b, const double r[3])
+{
+    double c = -SCN_HUGE;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i >= b.n_axes) break;
+        double t = fabs(get3(r, b.axes[i]) - b.origin[i]) - b.halfwidth[i];
+        if (t > c) c = t;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double box_distance(const scn_box& b, const double r[3], const double u[3])
+{
+    double far = SCN_HUGE, near = -SCN_HUGE;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i >= b.n_axes) break;
+        int ax = b.axes[i];
+        double rax = get3(r, ax), uax = get3(u, ax);
+        double rb = rax - b.origin[i];
+        double a_far = scn_sign(b.halfwidth[i], uax);
+        double a_near = -a_far;
+        double tn, tf;
+        if (uax != 0.0) {
+            tn = (a_near - rb) / uax;
+            tf = (a_far - rb) / uax;
+        } else {
+            tn = scn_sign(SCN_INF, a_near - rb);
+            tf = scn_sign(SCN_INF, a_far - rb);
+            if (tn > tf) {
+                double t = tn;
+                tn = tf;
+                tf = t;
+            }
+        }
+        far = scn_fmin(far, tf);
+        near = scn_fmax(near, tn);
+    }
+    double d;
+    if (far <= near * SCN_MISS_TOL) {
+        d = SCN_INF;
+    } else if (fabs(box_evaluate(b, r)) < SCN_SURF_TOL) {
+        d = (fabs(far) >= fabs(near)) ? far : near;
+    } else {
+        d = (near <= 0.0) ? far : near;
+    }
+    if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+    return d;
+}
+
+__device__ __forceinline__ int box_halfspace(const scn_box& b, const double r[3], const double u[3])
+{
+    double c = box_evaluate(b, r);
+    int hs = c > 0.0;
+    if (fabs(c) < SCN_SURF_TOL) {
+        int maxCom = 0;
+        double best = 0.0, rl = 0.0, ul = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (i >= b.n_axes) break;
+            double ri = get3(r, b.axes[i]) - b.origin[i];
+            double t = fabs(ri) - b.halfwidth[i];
+            if (i == 0 || t > best) {
+                best = t;
+                maxCom = i;
+                rl = ri;
+                ul = get3(u, b.axes[i]);
+            }
+        }
+        (void)maxCom;
+        double proj = ul * scn_sign(1.0, rl);
+        hs = proj > 0.0;
+        if (proj == 0.0) hs = c >= 0.0;
+    }
+    return hs;
+}
+
+__device__ __forceinline__ void box_explicit_bc(const scn_box& b, double r[3], double u[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i >= b.n_axes) break;
+        int ax = b.axes[i];
+        double rax = get3(r, ax);
+        double r0 = rax - b.origin[i];
+        if (fabs(r0) <= b.halfwidth[i] * (1.0 - SCN_SURF_TOL)) continue;
+        int bc = (r0 < 0.0) ? b.bc[2 * ax] : b.bc[2 * ax + 1];
+        if (bc == SCN_BC_REFLECTIVE) set3(u, ax, -get3(u, ax));
+        else if (bc == SCN_BC_PERIODIC) set3(r, ax, rax - 2.0 * scn_sign(b.halfwidth[i], r0));
+    }
+}
+
+__device__ __forceinline__ void box_transform_bc(const scn_box& b, double r[3], double u[3])
+{
+    int Ri[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i >= b.n_axes) break;
+        double a_bar = b.halfwidth[i] * (1.0 - SCN_SURF_TOL);
+        double q = ceil(fabs(get3(r, b.axes[i]) - b.origin[i]) / a_bar);
+        Ri[i] = (q < 2.0e6) ? (int)q / 2 : 1000000;   // bounded image count (finite flights only)
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (i >= b.n_axes) break;
+        int ax = b.axes[i];
+        double rax = get3(r, ax), uax = get3(u, ax);
+        for (int t = 1; t <= Ri[i]; ++t) {
+            double r0 = rax - b.origin[i];
+            int bc = (r0 < 0.0) ? b.bc[2 * ax] : b.bc[2 * ax + 1];
+            if (bc == SCN_BC_REFLECTIVE) {
+                double a0 = scn_sign(b.halfwidth[i], r0) + b.origin[i];
+                double d = rax - a0;
+                rax = rax - 2.0 * d;
+                uax = -uax;
+            } else if (bc == SCN_BC_PERIODIC) {
+                double d = scn_sign(b.halfwidth[i], r0);
+                rax = rax - 2.0 * d;
+            }
+        }
+        set3(r, ax, rax);
+        set3(u, ax, uax);
+    }
+}
+
+__device__ __forceinline__ double cyl_distance(double radius, double r_sq, const double r[3], const double u[3])
+{
+    double tol = 2.0 * radius * SCN_SURF_TOL;
+    double c = (0.0 + r[0] * r[0] + r[1] * r[1]) - r_sq;
+    double k = 0.0 + r[0] * u[0] + r[1] * u[1];
+    double a = 1.0 - u[2] * u[2];
+    double delta = k * k - a * c;
+    double d;
+    if (delta < 0.0 || a == 0.0) {
+        d = SCN_INF;
+    } else if (fabs(c) < tol) {
+        if (k >= 0.0) {
+            d = SCN_INF;
+        } else {
+            d = -k + sqrt(delta);
+            d = d / a;
+        }
+    } else if (c < 0.0) {
+        d = -k + sqrt(delta);
+        d = d / a;
+    } else {
+        d = -k - sqrt(delta);
+        d = d / a;
+        if (d <= 0.0) d = SCN_INF;
+    }
+    return scn_fmin(d, SCN_INF);
+}
+
+// ---------------------------------------------------------------------------------------------
+// universes
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void get_ijk(int localID, const int32_t sizeN[3], int ijk[3])
+{
+    int temp = localID - 1;
+    int base = temp / sizeN[0];
+    ijk[0] = temp - sizeN[0] * base + 1;
+    temp = base;
+    base = temp / sizeN[1];
+    ijk[1] = temp - sizeN[1] * base + 1;
+    ijk[2] = base + 1;
+}
+
+__device__ __forceinline__ int uni_findcell(const DevProblem& P, int uniIdx, const double r[3], const double u[3])
+{
+    const scn_universe& U = P.unis[uniIdx - 1];
+    if (U.type == SCN_UNI_LAT) {
+        int ijk[3];
+        for (int i = 0; i < 3; ++i) ijk[i] = (int)floor((r[i] - U.corner[i]) / U.pitch[i]) + 1;
+        for (int i = 0; i < 3; ++i) {
+            double r_bar = r[i] - U.corner[i] - (double)ijk[i] * U.pitch[i] + 0.5 * U.pitch[i];
+            if (fabs(r_bar) > U.a_bar[i] && r_bar * u[i] > 0.0) ijk[i] += (u[i] < 0.0) ? -1 : 1;
+        }
+        for (int i = 0; i < 3; ++i)
+            if (ijk[i] <= 0 || ijk[i] > U.size[i]) return U.out_local;
+        return ijk[0] + U.size[0] * (ijk[1] - 1 + U.size[1] * (ijk[2] - 1));
+    } else if (U.type == SCN_UNI_PIN) {
+        double r_sq = r[0] * r[0] + r[1] * r[1];
+        double mul = (r[0] * u[0] + r[1] * u[1] >= 0.0) ? -1.0 : 1.0;
+        const double* rad = P.real + U.annuli;
+        const double* rsq = rad + U.n_annuli;
+        int l;
+        for (l = 1; l <= U.n_annuli; ++l)
+            if (r_sq < rsq[l - 1] + mul * (2.0 * rad[l - 1] * SCN_SURF_TOL)) return l;
+        return l;
+    }
+    return box_halfspace(P.boxes[U.border], r, u) ? 2 : 1;
+}
+
+__device__ __forceinline__ double uni_distance(const DevProblem& P, int uniIdx, int localID, const double r[3],
+                                      const double u[3], int* surfIdx)
+{
+    const scn_universe& U = P.unis[uniIdx - 1];
+    if (U.type == SCN_UNI_LAT) {
+        if (localID == U.out_local) {
+            scn_box outline;
+            outline.n_axes = 3;
+            outline.axes[0] = 0;
+            outline.axes[1] = 1;
+            outline.axes[2] = 2;
+            for (int i = 0; i < 3; ++i) {
+                outline.origin[i] = 0.0;
+                outline.halfwidth[i] = U.outline_hw[i];
+            }
+            *surfIdx = LAT_OUTLINE;
+            return box_distance(outline, r, u);
+        }
+        int ijk[3];
+        get_ijk(localID, U.size, ijk);
+        double d = SCN_INF;
+        int ax = 0;
+        double uax = u[0];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double r_bar = r[i] - U.corner[i];
+            r_bar = r_bar - ((double)ijk[i] - 0.5) * U.pitch[i];
+            double bound = scn_sign(U.pitch[i] * 0.5, u[i]);
+            double test_d = (bound - r_bar) / u[i];
+            if (test_d < d) {
+                d = test_d;
+                ax = i;
+                uax = u[i];
+            }
+        }
+        d = scn_fmax(0.0, d);
+        d = scn_fmin(SCN_INF, d);
+        int s = (ax + 1) * 2;
+        if (uax < 0.0) s = s - 1;
+        *surfIdx = -s;
+        return d;
+    } else if (U.type == SCN_UNI_PIN) {
+        const double* rad = P.real + U.annuli;
+        const double* rsq = rad + U.n_annuli;
+        int id = localID;
+        double d_out = (id > U.n_annuli) ? SCN_INF : cyl_distance(rad[id - 1], rsq[id - 1], r, u);
+        double d_in = (id == 1) ? SCN_INF : cyl_distance(rad[id - 2], rsq[id - 2], r, u);
+        if (d_in < d_out) {
+            *surfIdx = PIN_IN;
+            return d_in;
+        }
+        *surfIdx = PIN_OUT;
+        return d_out;
+    }
+    *surfIdx = SURF_BORDER;
+    return box_distance(P.boxes[U.border], r, u);
+}
+
+__device__ __forceinline__ void uni_offset(const DevProblem& P, int uniIdx, int localID, double off[3])
+{
+    const scn_universe& U = P.unis[uniIdx - 1];
+    off[0] = off[1] = off[2] = 0.0;
+    if (U.type != SCN_UNI_LAT) return;
+    int doOffset = (U.offset_mode == 2) ? (P.ints[U.offset_map + localID - 1] == 1) : (U.offset_mode == 1);
+    if (doOffset && localID != U.out_local) {
+        int ijk[3];
+        get_ijk(localID, U.size, ijk);
+        for (int i = 0; i < 3; ++i) off[i] = ((double)ijk[i] - 0.5) * U.pitch[i] + U.corner[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// geometryStd
+// ---------------------------------------------------------------------------------------------
+
+// diveToMat from level `start` (0-based); levels are statically unrolled.
+template <int NL>
+__device__ __forceinline__ void dive(const DevProblem& P, Pt<NL>& p, int start, DevStatus* st)
+{
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < start) continue;
+        int pos = p.root[l] + p.loc[l] - 1;
+        int fill = P.gfill[pos - 1];
+        int id = P.gid[pos - 1];
+        if (fill >= 0) {
+            p.mat = fill;
+            p.uid = id;
+            return;
+        }
+        if (l + 1 < NL) {
+            int child = -fill;
+            double off[3];
+            uni_offset(P, p.uni[l], p.loc[l], off);
+            const scn_universe& Uc = P.unis[child - 1];
+            p.nest = l + 2;
+            p.uni[l + 1] = child;
+            p.root[l + 1] = id;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l + 1][k] = (p.r[l][k] - off[k]) - Uc.origin[k];
+            p.loc[l + 1] = uni_findcell(P, child, p.r[l + 1], p.u);
+        }
+    }
+    raise_error(st, SCN_ERR_ARG, 1);
+    p.mat = SCN_UNDEF_MAT;
+}
+
+template <int NL>
+__device__ __forceinline__ void place_coord(const DevProblem& P, Pt<NL>& p, DevStatus* st)
+{
+    const scn_universe& R = P.unis[P.root_uni - 1];
+    p.nest = 1;
+    p.mat = SCN_UNDEF_MAT;
+    p.uid = -3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.r[0][k] = p.r[0][k] - R.origin[k];
+    p.uni[0] = P.root_uni;
+    p.root[0] = 1;
+    p.loc[0] = uni_findcell(P, P.root_uni, p.r[0], p.u);
+    dive(P, p, 0, st);
+}
+
+// rebuild uni/root of every level from the stored local IDs (geomGraph walk)
+template <int NL>
+__device__ __forceinline__ void relink(const DevProblem& P, Pt<NL>& p)
+{
+    p.uni[0] = P.root_uni;
+    p.root[0] = 1;
+#pragma unroll
+    for (int l = 0; l + 1 < NL; ++l) {
+        if (l + 1 < p.nest) {
+            int pos = p.root[l] + p.loc[l] - 1;
+            p.uni[l + 1] = -P.gfill[pos - 1];
+            p.root[l + 1] = P.gid[pos - 1];
+        }
+    }
+}
+
+template <int NL>
+__device__ __forceinline__ void move_local(Pt<NL>& p, double d, int n)
+{
+    p.nest = n;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+        if (l < n)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l][k] = p.r[l][k] + d * p.u[k];
+}
+
+template <int NL>
+__device__ __forceinline__ void move_global(Pt<NL>& p, double d)
+{
+    p.nest = 1;
+    p.mat = SCN_UNDEF_MAT;
+    p.uid = -3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.r[0][k] = p.r[0][k] + d * p.u[k];
+}
+
+template <int NL>
+struct DistCache {
+    int lvl;
+    double dist[NL];
+    int surf[NL];
+};
+
+// geometryStd%move_withCache (cache on) / move_noCache (cache off); returns the event
+template <int NL>
+__device__ __forceinline__ int move(const DevProblem& P, Pt<NL>& p, double& maxDist, DistCache<NL>& cache, DevStatus* st)
+{
+    double dist = SCN_INF;
+    int surfIdx = 0, level = 0;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < p.nest) {
+            double td;
+            int ts;
+            if (P.st_cache) {
+                if (cache.lvl < l + 1) {
+                    cache.dist[l] = uni_distance(P, p.uni[l], p.loc[l], p.r[l], p.u, &cache.surf[l]);
+                    cache.lvl += 1;
+                }
+                td = cache.dist[l];
+                ts = cache.surf[l];
+            } else {
+                td = uni_distance(P, p.uni[l], p.loc[l], p.r[l], p.u, &ts);
+            }
+            if ((dist - td) >= dist * SCN_FP_REL_TOL) {
+                dist = td;
+                surfIdx = ts;
+                level = l + 1;
+            }
+        }
+    }
+    if (maxDist < dist && maxDist < SCN_INF) {
+        move_local(p, maxDist, p.nest);
+        cache.lvl = 0;
+        return EV_COLL;
+    } else if (surfIdx == SURF_BORDER && level == 1) {
+        move_global(p, dist);
+        maxDist = dist;
+        cache.lvl = 0;
+        box_explicit_bc(P.boxes[P.border_box], p.r[0], p.u);
+        place_coord(P, p, st);
+        return EV_BOUNDARY;
+    }
+    move_local(p, dist, level);
+    maxDist = dist;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+        if (l < level - 1) cache.dist[l] = cache.dist[l] - dist;
+    cache.lvl = level - 1;
+    // universe%cross at `level`, then diveToMat(level).  The crossed level is a runtime value, so
+    // its data are gathered with masks (an `if (l == level-1)` lets the compiler substitute the
+    // runtime index into the register arrays, which would spill them to scratch).
+    {
+        const int lv = level - 1;
+        double rl[3] = {0.0, 0.0, 0.0};
+        int ul = 0, ll = 0;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            const uint64_t m = 0ull - (uint64_t)(l == lv);
+            const uint32_t m32 = (uint32_t)m;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rl[k] = scn_bitsd((scn_dbits(rl[k]) & ~m) | (scn_dbits(p.r[l][k]) & m));
+            ul = (int)(((uint32_t)ul & ~m32) | ((uint32_t)p.uni[l] & m32));
+            ll = (int)(((uint32_t)ll & ~m32) | ((uint32_t)p.loc[l] & m32));
+        }
+        const scn_universe& U = P.unis[ul - 1];
+        int nl = (U.type == SCN_UNI_PIN) ? ll + ((surfIdx == PIN_IN) ? -1 : 1) : uni_findcell(P, ul, rl, p.u);
+#pragma unroll
+        for (int l = 0; l < NL; ++l) p.loc[l] = (l == lv) ? nl : p.loc[l];
+    }
+    dive(P, p, level - 1, st);
+    return EV_CROSS;
+}
+
+template <int NL>
+__device__ __forceinline__ void teleport(const DevProblem& P, Pt<NL>& p, double dist, DevStatus* st)
+{
+    move_global(p, dist);
+    place_coord(P, p, st);
+    if (p.mat == SCN_OUTSIDE_MAT) {
+        box_transform_bc(P.boxes[P.border_box], p.r[0], p.u);
+        place_coord(P, p, st);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// nuclear data and tallies
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ const double* xs_of(const DevProblem& P, int mat, int G)
+{
+    return P.xs + ((size_t)(mat - 1) * P.nG + (G - 1)) * 6;
+}
+
+__device__ __forceinline__ double track_mat_xs(const DevProblem& P, int mat, int G, DevStatus* st)
+{
+    if (mat == SCN_VOID_MAT) return 0.0;
+    if (mat < 1 || mat > P.n_mat) {
+        raise_error(st, SCN_ERR_MATERIAL, mat);
+        return 1.0;
+    }
+    return xs_of(P, mat, G)[XS_TOTAL] + 0.0;
+}
+
+__device__ __forceinline__ double macro_get(const double* x, int MT)
+{
+    switch (MT) {
+    case SCN_MT_TOTAL: return x[XS_TOTAL];
+    case SCN_MT_DISAPPEARANCE: return x[XS_CAPTURE];
+    case SCN_MT_ESCATTER: return 0.0;
+    case SCN_MT_NONELASTIC: return x[XS_SCATTER] + x[XS_FISSION] + x[XS_CAPTURE];
+    case SCN_MT_IESCATTER: return x[XS_SCATTER];
+    case SCN_MT_ALLSCATTER: return 0.0 + x[XS_SCATTER];
+    case SCN_MT_FISSION: return x[XS_FISSION];
+    case SCN_MT_NUFISSION: return x[XS_NUFISSION];
+    case SCN_MT_KAPPA: return x[XS_KAPPA];
+    case SCN_MT_ABSORPTION: return x[XS_FISSION] + x[XS_CAPTURE];
+    default: return 0.0;
+    }
+}
+
+__device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, const double* rg, int mat)
+{
+    if (m.type == SCN_MAP_SPACE) {
+        double v = get3(rg, m.axis);
+        const double* g = P.real + m.data;
+        int idx;
+        if (m.grid_type == SCN_GRID_LIN) {
+            idx = (int)floor((v - g[0]) / g[1]) + 1;
+        } else {
+            if (v < g[0] || v > g[m.nbins]) {
+                idx = -1;
+            } else {
+                int lo = 0, hi = m.nbins;
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) / 2;
+                    if (v >= g[mid]) lo = mid;
+                    else hi = mid;
+                }
+                idx = lo + 1;
+            }
+        }
+        if (idx < 1 || idx >= m.nbins + 1) idx = 0;
+        return idx;
+    }
+    int nm = P.n_mat;
+    return (mat >= 0 && mat <= nm) ? P.ints[m.data + mat] : P.ints[m.data + nm + 1];
+}
+
+// score memory: either a block-private LDS copy (lds != nullptr) or the global BIN array
+struct Bins {
+    double* lds;
+    double* glob;
+};
+
+__device__ __forceinline__ void bin_add(const Bins& B, int64_t addr, double v)
+{
+    if (B.lds) atomicAdd(&B.lds[addr], v);
+    else unsafeAtomicAdd(&B.glob[addr], v);
+}
+
+// tallyAdmin%reportInColl: keffImplicitClerk first, then the collision clerks
+template <int NL>
+__device__ __forceinline__ void report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, const Bins& B, DevStatus* st)
+{
+    int mat = p.mat;
+    if (mat != SCN_VOID_MAT) {
+        double flux = p.w / p.track;
+        const double* x = xs_of(P, mat, p.G);
+        double nuFiss = x[XS_NUFISSION];
+        double absXS = x[XS_CAPTURE] + x[XS_FISSION];
+        p.k[K_IMP_PROD] = p.k[K_IMP_PROD] + nuFiss * flux;
+        p.k[K_IMP_ABS] = p.k[K_IMP_ABS] + absXS * flux;
+    }
+    if (P.n_clerks == 0) return;
+    int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];
+    for (int ci = c0; ci < c1; ++ci) {
+        const scn_clerk& C = P.clerks[P.gclerk_idx[ci]];
+        if (!C.handle_virtual && isVirtual) continue;
+        int bin = 1;
+        for (int k = 0; k < C.n_maps; ++k) {
+            int b = map1d(P, C.maps[k], p.r[0], mat);
+            if (b == 0) {
+                bin = 0;
+                break;
+            }
+            bin = bin + (b - 1) * C.multi[k];
+        }
+        if (bin == 0) continue;
+        double flux = C.handle_virtual ? p.w / p.track : p.w / track_mat_xs(P, mat, p.G, st);
+        int64_t addr = C.mem_addr + (int64_t)C.width * (bin - 1);
+        for (int i = 0; i < C.width; ++i) {
+            double resp;
+            if (C.resp[i] == SCN_MT_FLUX) resp = 1.0;
+            else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), C.resp[i]);
+            bin_add(B, addr + i, resp * flux);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// flights
+// ---------------------------------------------------------------------------------------------
+
+// surfaceTracking: segments until a real collision or a leak; returns 1 if alive
+template <int NL>
+__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, int64_t& nev)
+{
+    DistCache<NL> cache;
+    cache.lvl = 0;
+    for (int seg = 0;; ++seg) {
+        if (seg >= MAX_SEGMENTS_PER_FLIGHT) {
+            raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
+            p.dead = 1;
+            return 0;
+        }
+        double sigmaTrack, sigmaT, dist, invSigmaTrack;
+        int mat = p.mat;
+        if (mat == SCN_VOID_MAT) sigmaTrack = P.collision_xs;
+        else sigmaTrack = scn_fmax(track_mat_xs(P, mat, p.G, st), P.collision_xs);
+        p.track = sigmaTrack;
+        if (sigmaTrack < SCN_ST_TOL) {
+            dist = SCN_INF;
+            invSigmaTrack = SCN_INF;
+            sigmaT = 0.0;
+        } else {
+            invSigmaTrack = 1.0 / sigmaTrack;
+            dist = -scn_log(scn_rng_get(&p.rng)) * invSigmaTrack;
+            sigmaT = track_mat_xs(P, mat, p.G, st);
+            if (dist != dist) {
+                raise_error(st, SCN_ERR_NAN, p.brood);
+                p.dead = 1;
+                return 0;
+            }
+        }
+        int event = move(P, p, dist, cache, st);
+        nev++;
+        p.t = p.t + dist / 1.0;
+        if (event == EV_COLL) p.fate = FATE_NO;
+        mat = p.mat;
+        if (mat == SCN_OUTSIDE_MAT) {
+            p.dead = 1;
+            p.fate = FATE_LEAK;
+            p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
+            return 0;
+        } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
+            raise_error(st, SCN_ERR_MATERIAL, mat);
+            p.dead = 1;
+            return 0;
+        }
+        if (event == EV_COLL) {
+            if (scn_rng_get(&p.rng) < sigmaT * invSigmaTrack) return 1;
+            report_in_coll(P, p, 1, B, st);
+        }
+    }
+}
+
+// deltaTracking: tentative flights until a real collision or a leak; returns 1 if alive
+template <int NL>
+__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, int64_t& nev)
+{
+    double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
+    p.track = maj;
+    double majorant_inv = 1.0 / maj;
+    if (fabs(majorant_inv) > SCN_HUGE) {
+        raise_error(st, SCN_ERR_MAJORANT, p.G);
+        p.dead = 1;
+        return 0;
+    }
+    for (int seg = 0;; ++seg) {
+        if (seg >= MAX_SEGMENTS_PER_FLIGHT) {
+            raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
+            p.dead = 1;
+            return 0;
+        }
+        double distance = -scn_log(scn_rng_get(&p.rng)) * majorant_inv;
+        teleport(P, p, distance, st);
+        nev++;
+        p.t = p.t + distance / 1.0;
+        int mat = p.mat;
+        if (mat == SCN_OUTSIDE_MAT) {
+            p.fate = FATE_LEAK;
+            p.dead = 1;
+            p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
+            return 0;
+        } else if (mat == SCN_VOID_MAT) {
+            report_in_coll(P, p, 1, B, st);
+            continue;
+        } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
+            raise_error(st, SCN_ERR_MATERIAL, mat);
+            p.dead = 1;
+            return 0;
+        }
+        double sigmaT = track_mat_xs(P, mat, p.G, st);
+        if (scn_rng_get(&p.rng) < sigmaT * majorant_inv) return 1;
+        report_in_coll(P, p, 1, B, st);
+    }
+}
+
+// fission-site sink: structure-of-arrays append buffer, sorted by key afterwards
+struct Sites {
+    double* r;        // [3][cap]
+    double* u;        // [3][cap]
+    double* w;
+    double* t;
+    int* G;
+    int* mat;
+    int* uid;
+    uint64_t* key;    // broodID << 24 | emission index
+    uint32_t* order;  // identity, sorted along with the keys
+    int* count;
+    int cap;
+};
+
+// collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
+template <int NL>
+__device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, const Bins& B, const Sites& S,
+                              DevStatus* st, int64_t& nev)
+{
+    const int nG = P.nG;
+    int mat = p.mat;
+    nev++;
+    double denom = track_mat_xs(P, mat, p.G, st);
+    double probAlpha = 0.0 / denom;
+    if (scn_rng_get(&p.rng) < probAlpha) {
+        raise_error(st, SCN_ERR_UNSUPPORTED, 1);
+        p.dead = 1;
+        return 0;
+    }
+    const double* x = xs_of(P, mat, p.G);
+    double rr = scn_rng_get(&p.rng);
+    int C = 1;
+    double xs = x[XS_TOTAL] * rr - 0.0;
+    if (xs > 0.0) C++;
+    xs = xs - x[XS_SCATTER];
+    if (xs > 0.0) C++;
+    xs = xs - x[XS_CAPTURE];
+    if (xs > 0.0) C++;
+    report_in_coll(P, p, 0, B, st);
+    double preColl_w = p.w;
+    if (P.fissile[mat - 1]) {
+        double wgt = p.w, w0 = p.w0;
+        double rand1 = scn_rng_get(&p.rng);
+        double sig_tot = x[XS_TOTAL], sig_nuFiss = x[XS_NUFISSION];
+        double nd_ = fabs((wgt * sig_nuFiss) / (w0 * sig_tot * k_eff)) + rand1;
+        if (!(nd_ < (double)S.cap)) {
+            raise_error(st, SCN_ERR_OVERFLOW, -1);
+            p.dead = 1;
+            return 0;
+        }
+        int n = (int)nd_;
+        if (n >= 1) {
+            wgt = scn_sign(w0, wgt);
+            const double* chi = P.chi + (size_t)(mat - 1) * nG;
+            int base = atomicAdd(S.count, n);
+            for (int i = 0; i < n; ++i) {
+                double mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
+                double phi = SCN_TWO_PI * scn_rng_get(&p.rng);
+                double rem = scn_rng_get(&p.rng);
+                int G_out;
+                for (G_out = 1; G_out <= nG; ++G_out) {
+                    rem = rem - chi[G_out - 1];
+                    if (rem < 0.0) break;
+                }
+                if (G_out > nG) {
+                    raise_error(st, SCN_ERR_SAMPLING, 2);
+                    p.dead = 1;
+                    return 0;
+                }
+                double d[3];
+                scn_rotate(p.u, mu, phi, d);
+                int slot = base + i;
+                if (slot < S.cap) {
+                    for (int k = 0; k < 3; ++k) {
+                        S.r[(size_t)k * S.cap + slot] = p.r[0][k];
+                        S.u[(size_t)k * S.cap + slot] = d[k];
+                    }
+                    S.w[slot] = wgt;
+                    S.t[slot] = p.t;
+                    S.G[slot] = G_out;
+                    S.mat[slot] = mat;
+                    S.uid[slot] = p.uid;
+                    S.key[slot] = ((uint64_t)p.brood << 24) | (uint64_t)p.nsite;
+                    S.order[slot] = (uint32_t)slot;
+                } else {
+                    raise_error(st, SCN_ERR_OVERFLOW, slot);
+                }
+                p.nsite++;
+            }
+        }
+    }
+    if (C == 2) {
+        double rem = scn_rng_get(&p.rng) * x[XS_SCATTER];
+        const double* P0 = P.P0 + ((size_t)(mat - 1) * nG + (p.G - 1)) * nG;
+        const double* prod = P.prod + ((size_t)(mat - 1) * nG + (p.G - 1)) * nG;
+        int G_out;
+        for (G_out = 1; G_out <= nG; ++G_out) {
+            rem = rem - P0[G_out - 1];
+            if (rem < 0.0) break;
+        }
+        if (G_out > nG) {
+            raise_error(st, SCN_ERR_SAMPLING, 1);
+            p.dead = 1;
+            return 0;
+        }
+        double mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(&p.rng);
+        double w_mul = prod[G_out - 1];
+        p.G = G_out;
+        p.w = p.w * w_mul;
+        double nd[3];
+        scn_rotate(p.u, mu, phi, nd);
+        p.u[0] = nd[0];
+        p.u[1] = nd[1];
+        p.u[2] = nd[2];
+    } else if (C == 3 || C == 4) {
+        p.dead = 1;
+    } else {
+        raise_error(st, SCN_ERR_UNSUPPORTED, -3);
+        p.dead = 1;
+        return 0;
+    }
+    p.coll += 1;
+    if (C == 2) {
+        double score = scn_fmax(p.w - preColl_w, 0.0);
+        if (score > 0.0) p.k[K_SCATTER_PROD] = p.k[K_SCATTER_PROD] + score;
+    }
+    if (p.dead) {
+        p.fate = FATE_ABS;
+        return 0;
+    }
+    return 1;
+}
+
+}  // namespace scn

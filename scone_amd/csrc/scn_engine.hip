@@ -1,0 +1,1317 @@
+// scn_engine.hip -- event-based multigroup transport engine for gfx950 behind the C ABI of
+// include/scone_gpu.h.
+//
+// Cycle structure (eigenPhysicsPackage_class.f90:208-310 restated as device passes):
+//   1. k_transport(first)   bank (AoS, HBM) -> live SoA: per-history RNG stride, placeCoord, then up
+//                           to K (flight, collision) pairs per particle in registers; survivors are
+//                           compacted into the next live SoA with a wave ballot + prefix popcount.
+//   2. k_transport(...)     repeated over the compacted queue (K grows geometrically) until empty.
+//   3. sort sites           fission sites were appended with key broodID<<24 | emission index;
+//                           a radix sort restores the sequential detain order (sortByBroodID).
+//   4. canonical sums       population weights and keffImplicitClerk scores (fixed summation tree).
+//   5. k_close_bins         scoreMemory%closeCycle on the collision-clerk BIN array.
+//   6. normSize_Repr        per-site skip-ahead random numbers, radix-sorted threshold, stable
+//                           keep / copy / duplicate compaction.
+#include <cstring>
+#include <cstdio>
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include <chrono>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <functional>
+
+#include "scone_gpu.h"
+#include "scn_fpmath.h"
+#include "scn_device.h"
+
+using namespace scn;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int WAVE = 64;
+constexpr int LDS_BINS_MAX = 8192;   // collision-clerk memories up to 64 KiB are privatised in LDS
+
+// live particle queue, structure of arrays; level-major position arrays
+struct Live {
+    double* r;     // [NL*3][cap]
+    double* u;     // [3][cap]
+    double* w;
+    double* w0;
+    double* t;
+    double* k;     // [4][cap]
+    uint64_t* rng;
+    int* loc;      // [NL][cap]
+    int* nest;
+    int* mat;
+    int* uid;
+    int* G;
+    int* brood;
+    int* nsite;
+    int* coll;
+};
+
+struct Counters {
+    unsigned long long segments;
+    unsigned long long collisions;
+    unsigned long long scores;
+    unsigned long long pad;
+};
+
+template <int NL>
+__device__ inline void load_live(const Live& L, int cap, int i, Pt<NL>& p)
+{
+    p.nest = L.nest[i];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < p.nest) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l][k] = L.r[(size_t)(l * 3 + k) * cap + i];
+            p.loc[l] = L.loc[(size_t)l * cap + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+            p.loc[l] = 0;
+        }
+        p.uni[l] = 0;
+        p.root[l] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.u[k] = L.u[(size_t)k * cap + i];
+    p.w = L.w[i];
+    p.w0 = L.w0[i];
+    p.t = L.t[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p.k[k] = L.k[(size_t)k * cap + i];
+    p.rng = L.rng[i];
+    p.mat = L.mat[i];
+    p.uid = L.uid[i];
+    p.G = L.G[i];
+    p.brood = L.brood[i];
+    p.nsite = L.nsite[i];
+    p.coll = L.coll[i];
+    p.dead = 0;
+    p.fate = FATE_NO;
+    p.track = 0.0;
+}
+
+template <int NL>
+__device__ inline void store_live(const Live& L, int cap, int i, const Pt<NL>& p)
+{
+    L.nest[i] = p.nest;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < p.nest) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) L.r[(size_t)(l * 3 + k) * cap + i] = p.r[l][k];
+            L.loc[(size_t)l * cap + i] = p.loc[l];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L.u[(size_t)k * cap + i] = p.u[k];
+    L.w[i] = p.w;
+    L.w0[i] = p.w0;
+    L.t[i] = p.t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) L.k[(size_t)k * cap + i] = p.k[k];
+    L.rng[i] = p.rng;
+    L.mat[i] = p.mat;
+    L.uid[i] = p.uid;
+    L.G[i] = p.G;
+    L.brood[i] = p.brood;
+    L.nsite[i] = p.nsite;
+    L.coll[i] = p.coll;
+}
+
+// One launch of the event loop.  first != 0: the queue is the cycle's bank (AoS); history n = i+1.
+template <int NL, int MODE>
+__global__ void __launch_bounds__(BLOCK)
+k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first, int n_first, Live in,
+            const int* __restrict__ n_in_ptr, Live out, int* __restrict__ n_out, int cap, int max_steps,
+            uint64_t master, double k_eff, Sites S, double* __restrict__ bins, int use_lds,
+            double* __restrict__ hist_k, int hist_cap, Counters* __restrict__ cnt, DevStatus* st)
+{
+    extern __shared__ double lds_bins[];
+    __shared__ unsigned long long s_seg, s_col;
+    const int n_in = first ? n_first : *n_in_ptr;
+    if (threadIdx.x == 0) {
+        s_seg = 0;
+        s_col = 0;
+    }
+    if (use_lds) {
+        for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) lds_bins[i] = 0.0;
+    }
+    __syncthreads();
+    Bins B;
+    B.lds = use_lds ? lds_bins : nullptr;
+    B.glob = bins;
+    int64_t nseg = 0, ncol = 0;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int base = blockIdx.x * BLOCK; base < n_in; base += gridDim.x * BLOCK) {
+        const int i = base + threadIdx.x;
+        const int valid = i < n_in;
+        Pt<NL> p;
+        int alive = 0;
+        if (valid) {
+            if (first) {
+                const scn_particle_state& s = bank[i];
+                p.rng = scn_rng_stride(master, (int64_t)i + 1);
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    p.loc[l] = 0;
+                    p.uni[l] = 0;
+                    p.root[l] = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    p.r[0][k] = s.r[k];
+                    p.u[k] = s.dir[k];
+                }
+                p.w = s.wgt;
+                p.w0 = s.wgt;
+                p.G = s.G;
+                p.t = s.time;
+                p.coll = s.collisionN;
+                p.brood = i + 1;
+                p.nsite = 0;
+                p.k[0] = p.k[1] = p.k[2] = p.k[3] = 0.0;
+                p.dead = 0;
+                p.fate = FATE_NO;
+                p.track = 0.0;
+                place_coord(P, p, st);
+            } else {
+                load_live(in, cap, i, p);
+                relink(P, p);
+            }
+            alive = 1;
+            for (int step = 0; step < max_steps; ++step) {
+                int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, nseg) : flight_st(P, p, B, st, nseg);
+                if (!ok) {
+                    alive = 0;
+                    break;
+                }
+                ok = collide(P, p, k_eff, B, S, st, ncol);
+                if (!ok) {
+                    alive = 0;
+                    break;
+                }
+                if (st->code) break;
+            }
+            if (!alive) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) hist_k[(size_t)k * hist_cap + (p.brood - 1)] = p.k[k];
+            }
+        }
+        // wave-level queue compaction: ballot of survivors, one atomic per wave
+        unsigned long long mask = __ballot(alive);
+        int cntw = __popcll(mask);
+        int obase = 0;
+        if (lane == 0 && cntw) obase = atomicAdd(n_out, cntw);
+        obase = __shfl(obase, 0);
+        if (alive) {
+            int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            int o = obase + rank;
+            if (o < cap) store_live(out, cap, o, p);
+            else raise_error(st, SCN_ERR_OVERFLOW, o);
+        }
+    }
+    ncol = ncol - 0;
+    atomicAdd(&s_seg, (unsigned long long)(nseg));
+    atomicAdd(&s_col, (unsigned long long)(ncol));
+    __syncthreads();
+    if (use_lds) {
+        for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) {
+            double v = lds_bins[i];
+            if (v != 0.0) unsafeAtomicAdd(&bins[i], v);
+        }
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&cnt->segments, s_seg);
+        atomicAdd(&cnt->collisions, s_col);
+    }
+}
+
+// gather the key-sorted fission sites into the next bank (particleState fields of pTemp,
+// neutronMGstd_class.f90:177-198)
+__global__ void k_gather_sites(Sites S, const uint32_t* __restrict__ order, int n, scn_particle_state* __restrict__ out,
+                               const uint64_t* __restrict__ keys_sorted)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int s = (int)order[j];
+    scn_particle_state q;
+    q.wgt = S.w[s];
+    for (int k = 0; k < 3; ++k) {
+        q.r[k] = S.r[(size_t)k * S.cap + s];
+        q.dir[k] = S.u[(size_t)k * S.cap + s];
+    }
+    q.E = 0.0;
+    q.G = S.G[s];
+    q.isMG = 1;
+    q.type = 1;
+    q.pad0 = 0;
+    q.time = S.t[s];
+    q.lambda = SCN_INF;
+    q.matIdx = S.mat[s];
+    q.cellIdx = 0;
+    q.uniqueID = S.uid[s];
+    q.collisionN = 0;
+    q.broodID = (int)(keys_sorted[j] >> 24);
+    q.pad1 = 0;
+    out[j] = q;
+}
+
+// ---- canonical sums ------------------------------------------------------------------------
+struct SumJob {
+    const double* x;
+    int64_t n;
+    int64_t stride;   // in doubles
+};
+struct SumJobs {
+    SumJob j[8];
+    int njobs;
+};
+
+__global__ void k_dsum_chunks(SumJobs J, double* y, int64_t ystride)
+{
+    int job = blockIdx.y;
+    const SumJob& q = J.j[job];
+    int64_t m = (q.n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    double s = 0.0;
+    int64_t e = (c + 1) * SCN_DSUM_CHUNK < q.n ? (c + 1) * SCN_DSUM_CHUNK : q.n;
+    for (int64_t i = c * SCN_DSUM_CHUNK; i < e; ++i) s = s + q.x[i * q.stride];
+    y[job * ystride + c] = s;
+}
+
+__global__ void k_dsum_level(const double* y, double* y2, const int64_t* m_in, int64_t ystride)
+{
+    int job = blockIdx.y;
+    int64_t m = m_in[job];
+    int64_t h = (m + 1) / 2;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m <= 1 || i >= h) return;
+    const double* a = y + job * ystride;
+    y2[job * ystride + i] = (2 * i + 1 < m) ? a[2 * i] + a[2 * i + 1] : a[2 * i];
+}
+
+__global__ void __launch_bounds__(1024) k_dsum_final(const double* y, const int64_t* m_in, int64_t ystride, double* out)
+{
+    __shared__ double a[2048];
+    __shared__ double b[2048];
+    int job = blockIdx.x;
+    int64_t m = m_in[job];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) a[i] = y[job * ystride + i];
+    __syncthreads();
+    double* src = a;
+    double* dst = b;
+    while (m > 1) {
+        int64_t h = (m + 1) / 2;
+        for (int i = threadIdx.x; i < h; i += blockDim.x) dst[i] = (2 * i + 1 < m) ? src[2 * i] + src[2 * i + 1] : src[2 * i];
+        __syncthreads();
+        double* t = src;
+        src = dst;
+        dst = t;
+        m = h;
+    }
+    if (threadIdx.x == 0) out[job] = (m_in[job] > 0) ? src[0] : 0.0;
+}
+
+// ---- scoreMemory%closeCycle ------------------------------------------------------------------
+__global__ void k_close_bins(double* bins, double* csum, double* csum2, int64_t n, const double* norm_score,
+                             double norm_value)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double nf = norm_score ? norm_value / *norm_score : 1.0;
+    double res = bins[i] * nf;
+    bins[i] = 0.0;
+    csum[i] = csum[i] + res;
+    csum2[i] = csum2[i] + res * res;
+}
+
+__global__ void k_copy_scalar(const double* src, double* dst) { *dst = *src; }
+
+// ---- normSize_Repr ---------------------------------------------------------------------------
+__global__ void k_site_rn(uint64_t master, int n, double* rn, uint64_t* rn_bits)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint64_t s = scn_rng_skip(master, (int64_t)j);
+    double u = scn_rng_get(&s);
+    rn[j] = u;
+    rn_bits[j] = scn_dbits(u);
+}
+
+__global__ void k_norm_flags(const double* rn, int n, const uint64_t* thr_bits, int keep_mode, int* flag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double thr = scn_bitsd(*thr_bits);
+    flag[j] = keep_mode ? (rn[j] > thr) : (rn[j] <= thr);
+}
+
+__global__ void k_norm_keep(const scn_particle_state* in, const int* flag, const int* pos, int n, scn_particle_state* out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (flag[j]) out[pos[j]] = in[j];
+}
+
+__global__ void k_brood_bounds(const scn_particle_state* in, int n, int max_brood, int* start_of, int* end_of)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int b = in[j].broodID;
+    if (b < 0 || b > max_brood) return;   // excluded by the host check before launch
+    if (j == 0 || in[j - 1].broodID != b) start_of[b] = j;
+    if (j == n - 1 || in[j + 1].broodID != b) end_of[b] = j + 1;
+}
+
+__global__ void k_norm_dup(const scn_particle_state* in, const int* dflag, const int* D, const int* start_of,
+                           const int* end_of, int n, int n_copies, scn_particle_state* out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int b = in[j].broodID;
+    int s = start_of[b];
+    int kb = end_of[b] - s;
+    int64_t base = (int64_t)(n_copies + 1) * s + D[s];
+    int rho = j - s;
+    for (int c = 0; c <= n_copies; ++c) out[base + (int64_t)c * kb + rho] = in[j];
+    if (dflag[j]) out[base + (int64_t)(n_copies + 1) * kb + (D[j] - D[s])] = in[j];
+}
+
+// ---- initial fission source (fissionSource_class.f90:149-271) ---------------------------------
+template <int NL>
+__global__ void k_fission_source(DevProblem P, int n, uint64_t master, double b0, double b1, double b2, double t0,
+                                 double t1, double t2, int attempts, scn_particle_state* bank, DevStatus* st)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t rng = scn_rng_stride(master, (int64_t)i + 1);
+    const double bottom[3] = {b0, b1, b2}, top[3] = {t0, t1, t2};
+    scn_particle_state s;
+    memset(&s, 0, sizeof s);
+    int it = 0;
+    for (;;) {
+        if (++it > attempts) {
+            raise_error(st, SCN_ERR_SAMPLING, 3);
+            break;
+        }
+        Pt<NL> p;
+        double r3[3];
+        for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&rng);
+        for (int k = 0; k < 3; ++k) {
+            p.r[0][k] = (top[k] - bottom[k]) * r3[k] + bottom[k];
+            s.r[k] = p.r[0][k];
+        }
+        p.u[0] = 1.0;
+        p.u[1] = 0.0;
+        p.u[2] = 0.0;
+        place_coord(P, p, st);
+        int mat = p.mat;
+        if (mat == SCN_VOID_MAT || mat == SCN_OUTSIDE_MAT) continue;
+        if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT) {
+            raise_error(st, SCN_ERR_MATERIAL, mat);
+            break;
+        }
+        if (!P.fissile[mat - 1]) continue;
+        s.matIdx = mat;
+        s.uniqueID = p.uid;
+        s.wgt = 1.0;
+        s.time = 0.0;
+        s.type = 1;
+        double mu = 2.0 * scn_rng_get(&rng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(&rng);
+        double rem = scn_rng_get(&rng);
+        const double* chi = P.chi + (size_t)(mat - 1) * P.nG;
+        int G;
+        for (G = 1; G <= P.nG; ++G) {
+            rem = rem - chi[G - 1];
+            if (rem < 0.0) break;
+        }
+        if (G > P.nG) {
+            raise_error(st, SCN_ERR_SAMPLING, 2);
+            break;
+        }
+        s.G = G;
+        s.isMG = 1;
+        const double x1[3] = {1.0, 0.0, 0.0};
+        scn_rotate(x1, mu, phi, s.dir);
+        s.lambda = SCN_INF;
+        s.cellIdx = -1;
+        break;
+    }
+    bank[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Engine {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int nl = 1;             // compiled nesting depth
+    int transport = SCN_TRANSPORT_ST;
+    DevProblem P{};
+    int64_t n_bins = 0;
+    int64_t norm_bin = -1;
+    double norm_value = 1.0;
+    int nG = 0;
+    int cap = 0;            // live capacity (histories per cycle)
+    int site_cap = 0;
+    int bank_cap = 0;
+    std::vector<DevBuf> owned;
+    Live live[2]{};
+    Sites sites{};
+    uint64_t* keys_alt = nullptr;
+    uint32_t* order_alt = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    scn_particle_state* bankA = nullptr;   // thisCycle
+    scn_particle_state* bankB = nullptr;   // nextCycle
+    double* hist_k = nullptr;
+    double* bins = nullptr;
+    double* csum[2] = {nullptr, nullptr};
+    double* csum2[2] = {nullptr, nullptr};
+    double* norm_score = nullptr;
+    int* counters_i = nullptr;             // [0]: queue A count, [1]: queue B count
+    Counters* counters = nullptr;
+    DevStatus* status = nullptr;
+    double* sum_y = nullptr;
+    double* sum_y2 = nullptr;
+    int64_t* sum_m = nullptr;
+    double* sum_out = nullptr;
+    int64_t sum_cap = 0;
+    // normSize scratch
+    double* rn = nullptr;
+    uint64_t* rn_bits = nullptr;
+    uint64_t* rn_sorted = nullptr;
+    int* flag = nullptr;
+    int* pos = nullptr;
+    int* start_of = nullptr;
+    int* end_of = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    void* rsort_tmp = nullptr;
+    size_t rsort_tmp_bytes = 0;
+    int n_cu = 256;
+    int use_lds = 0;
+    int lds_bytes = 0;
+    scn_stats stats{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // tuning
+    int k0 = 8;
+    int tail_frac = 16;
+};
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            E->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+            return SCN_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+
+// every copy goes through the engine stream and waits for it (kernels run on that stream)
+#define COPY_SYNC(dst, src, bytes, kind)                                                  \
+    do {                                                                                 \
+        HIPCHK(hipMemcpyAsync((dst), (src), (bytes), (kind), E->stream));                \
+        HIPCHK(hipStreamSynchronize(E->stream));                                         \
+    } while (0)
+
+template <typename T>
+int dalloc(Engine* E, T** p, size_t count)
+{
+    size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        E->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+        *p = nullptr;
+        return SCN_ERR_HIP;
+    }
+    E->owned.push_back({q, bytes});
+    *p = (T*)q;
+    return 0;
+}
+
+template <typename T>
+int upload(Engine* E, T** d, const T* h, size_t count)
+{
+    int rc = dalloc(E, d, count);
+    if (rc) return rc;
+    if (count && h) COPY_SYNC(*d, h, count * sizeof(T), hipMemcpyHostToDevice);
+    return 0;
+}
+
+int alloc_live(Engine* E, Live& L, int cap, int nl)
+{
+    int rc = 0;
+    rc |= dalloc(E, &L.r, (size_t)nl * 3 * cap);
+    rc |= dalloc(E, &L.u, (size_t)3 * cap);
+    rc |= dalloc(E, &L.w, cap);
+    rc |= dalloc(E, &L.w0, cap);
+    rc |= dalloc(E, &L.t, cap);
+    rc |= dalloc(E, &L.k, (size_t)4 * cap);
+    rc |= dalloc(E, &L.rng, cap);
+    rc |= dalloc(E, &L.loc, (size_t)nl * cap);
+    rc |= dalloc(E, &L.nest, cap);
+    rc |= dalloc(E, &L.mat, cap);
+    rc |= dalloc(E, &L.uid, cap);
+    rc |= dalloc(E, &L.G, cap);
+    rc |= dalloc(E, &L.brood, cap);
+    rc |= dalloc(E, &L.nsite, cap);
+    rc |= dalloc(E, &L.coll, cap);
+    return rc;
+}
+
+int check_status(Engine* E)
+{
+    DevStatus h{};
+    HIPCHK(hipMemcpyAsync(&h, E->status, sizeof h, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (h.code) {
+        char buf[256];
+        const char* what = "engine error";
+        switch (h.code) {
+        case SCN_ERR_OVERFLOW: what = "Run out of space for particles (site or queue buffer)"; break;
+        case SCN_ERR_MATERIAL: what = "Particle is in an undefined or overlapping material"; break;
+        case SCN_ERR_NAN: what = "Distance is NaN"; break;
+        case SCN_ERR_MAJORANT: what = "Majorant is 0"; break;
+        case SCN_ERR_SAMPLING: what = "Sampling failed: unnormalised CDF (scatter or chi)"; break;
+        case SCN_ERR_UNSUPPORTED: what = "Unsupported reaction (MT) in collision"; break;
+        case SCN_ERR_ARG: what = "Failed to find material cell (diveToMat)"; break;
+        }
+        snprintf(buf, sizeof buf, "%s [detail %d]", what, h.detail);
+        E->err = buf;
+        DevStatus z{};
+        COPY_SYNC(E->status, &z, sizeof z, hipMemcpyHostToDevice);
+        return h.code;
+    }
+    return 0;
+}
+
+int ensure_sum(Engine* E, int64_t n)
+{
+    int64_t m = (n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK + 1;
+    if (m <= E->sum_cap) return 0;
+    int rc = 0;
+    rc |= dalloc(E, &E->sum_y, (size_t)8 * m);
+    rc |= dalloc(E, &E->sum_y2, (size_t)8 * m);
+    E->sum_cap = m;
+    return rc;
+}
+
+// canonical sums of up to 8 strided arrays; results land in E->sum_out[0..njobs)
+int dsum_batch(Engine* E, SumJobs J)
+{
+    int64_t maxn = 0;
+    for (int q = 0; q < J.njobs; ++q) maxn = std::max(maxn, J.j[q].n);
+    int rc = ensure_sum(E, maxn);
+    if (rc) return rc;
+    const int64_t ys = E->sum_cap;
+    int64_t m[8];
+    int64_t mmax = 0;
+    for (int q = 0; q < J.njobs; ++q) {
+        m[q] = (J.j[q].n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+        mmax = std::max(mmax, m[q]);
+    }
+    if (mmax > 0) {
+        dim3 g((unsigned)((mmax + 255) / 256), J.njobs);
+        hipLaunchKernelGGL(k_dsum_chunks, g, dim3(256), 0, E->stream, J, E->sum_y, ys);
+    }
+    double* a = E->sum_y;
+    double* b = E->sum_y2;
+    while (mmax > 2048) {
+        HIPCHK(hipMemcpyAsync(E->sum_m, m, sizeof(int64_t) * J.njobs, hipMemcpyHostToDevice, E->stream));
+        int64_t h = (mmax + 1) / 2;
+        dim3 g((unsigned)((h + 255) / 256), J.njobs);
+        hipLaunchKernelGGL(k_dsum_level, g, dim3(256), 0, E->stream, a, b, E->sum_m, ys);
+        // jobs already at m <= 1 keep their value in place: copy forward through the level kernel
+        for (int q = 0; q < J.njobs; ++q) {
+            if (m[q] == 1) HIPCHK(hipMemcpyAsync(b + q * ys, a + q * ys, sizeof(double), hipMemcpyDeviceToDevice, E->stream));
+            m[q] = (m[q] + 1) / 2;
+        }
+        mmax = (mmax + 1) / 2;
+        std::swap(a, b);
+        HIPCHK(hipStreamSynchronize(E->stream));
+    }
+    HIPCHK(hipMemcpyAsync(E->sum_m, m, sizeof(int64_t) * J.njobs, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_dsum_final, dim3(J.njobs), dim3(1024), 0, E->stream, a, E->sum_m, ys, E->sum_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+typedef void (*TransportFn)(dim3, dim3, size_t, hipStream_t, const DevProblem&, const scn_particle_state*, int, int,
+                            Live, const int*, Live, int*, int, int, uint64_t, double, Sites, double*, int, double*,
+                            int, Counters*, DevStatus*);
+
+template <int NL, int MODE>
+void launch_transport(dim3 g, dim3 b, size_t sh, hipStream_t s, const DevProblem& P, const scn_particle_state* bank,
+                      int first, int n_first, Live in, const int* n_in_ptr, Live out, int* n_out, int cap,
+                      int max_steps, uint64_t master, double k_eff, Sites S, double* bins, int use_lds,
+                      double* hist_k, int hist_cap, Counters* cnt, DevStatus* st)
+{
+    hipLaunchKernelGGL((k_transport<NL, MODE>), g, b, sh, s, P, bank, first, n_first, in, n_in_ptr, out, n_out, cap,
+                       max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st);
+}
+
+template <int MODE>
+TransportFn pick_nl(int nl)
+{
+    switch (nl) {
+    case 1: return launch_transport<1, MODE>;
+    case 3: return launch_transport<3, MODE>;
+    case 4: return launch_transport<4, MODE>;
+    default: return launch_transport<8, MODE>;
+    }
+}
+
+int compiled_nl(int depth)
+{
+    if (depth <= 1) return 1;
+    if (depth <= 3) return 3;
+    if (depth <= 4) return 4;
+    return 8;
+}
+
+// transport one cycle's bank (E->bankA, n_in histories) into E->bankB (sites sorted by brood);
+// returns the number of sites in *n_sites and the per-history keff scores in E->hist_k.
+int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_sites, int score_bins)
+{
+    if (n_in > E->cap) {
+        E->err = "bank larger than the engine capacity";
+        return SCN_ERR_OVERFLOW;
+    }
+    HIPCHK(hipMemsetAsync(E->counters_i, 0, sizeof(int) * 4, E->stream));
+    HIPCHK(hipMemsetAsync(E->sites.count, 0, sizeof(int), E->stream));
+    TransportFn fn = (E->transport == SCN_TRANSPORT_DT) ? pick_nl<SCN_TRANSPORT_DT>(E->nl) : pick_nl<SCN_TRANSPORT_ST>(E->nl);
+    double* bins = score_bins ? E->bins : nullptr;
+    // a null BIN pointer means "do not score": point at a scratch copy instead
+    double* bins_arg = bins ? bins : E->bins;
+    int use_lds = score_bins ? E->use_lds : 0;
+    size_t sh = use_lds ? (size_t)E->lds_bytes : 0;
+    DevProblem P = E->P;
+    if (!score_bins) P.n_clerks = 0;
+    int n_live = n_in;
+    int cur = 0;
+    int it = 0;
+    int K = E->k0;
+    int first = 1;
+    while (n_live > 0) {
+        int grid = std::max(1, std::min((n_live + BLOCK - 1) / BLOCK, E->n_cu * 8));
+        int tail = n_live <= std::max(65536, n_in / E->tail_frac);
+        int steps = tail ? (1 << 30) : K;
+        int* nin_ptr = E->counters_i + cur;
+        int* nout_ptr = E->counters_i + (1 - cur);
+        HIPCHK(hipMemsetAsync(nout_ptr, 0, sizeof(int), E->stream));
+        HIPCHK(hipEventRecord(E->ev0, E->stream));
+        fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, first, n_in, E->live[cur], nin_ptr, E->live[1 - cur],
+           nout_ptr, E->cap, steps, master, k_eff, E->sites, bins_arg, use_lds, E->hist_k, E->cap, E->counters,
+           E->status);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E->ev1, E->stream));
+        int host_n = 0;
+        HIPCHK(hipMemcpyAsync(&host_n, nout_ptr, sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+        E->stats.transport_kernel_ms += ms;
+        E->stats.transport_launches += 1;
+        int rc = check_status(E);
+        if (rc) return rc;
+        n_live = host_n;
+        cur = 1 - cur;
+        first = 0;
+        K *= 2;
+        ++it;
+    }
+    int ns = 0;
+    HIPCHK(hipMemcpyAsync(&ns, E->sites.count, sizeof(int), hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (ns > E->site_cap || ns > E->bank_cap) {
+        E->err = "Run out of space for fission sites";
+        return SCN_ERR_OVERFLOW;
+    }
+    if (ns > 0) {
+        int end_bit = 24;
+        while (((int64_t)1 << (end_bit - 24)) <= (int64_t)n_in + 1) ++end_bit;
+        size_t need = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, need, E->sites.key, E->keys_alt, E->sites.order, E->order_alt,
+                                         (size_t)ns, 0, end_bit, E->stream));
+        if (need > E->sort_tmp_bytes) {
+            int rc = dalloc(E, (char**)&E->sort_tmp, need);
+            if (rc) return rc;
+            E->sort_tmp_bytes = need;
+        }
+        HIPCHK(rocprim::radix_sort_pairs(E->sort_tmp, need, E->sites.key, E->keys_alt, E->sites.order, E->order_alt,
+                                         (size_t)ns, 0, end_bit, E->stream));
+        hipLaunchKernelGGL(k_gather_sites, dim3((ns + 255) / 256), dim3(256), 0, E->stream, E->sites, E->order_alt, ns,
+                           E->bankB, E->keys_alt);
+        HIPCHK(hipGetLastError());
+    }
+    *n_sites = ns;
+    E->stats.n_histories += n_in;
+    E->stats.n_sites += ns;
+    return 0;
+}
+
+// sums after a cycle: START (bankA weights), END (bankB weights), 4 keff scores
+int cycle_sums(Engine* E, int n_in, int n_out, double out[6])
+{
+    SumJobs J{};
+    J.njobs = 6;
+    J.j[0] = {&E->bankA[0].wgt, n_in, (int64_t)(sizeof(scn_particle_state) / sizeof(double))};
+    J.j[1] = {&E->bankB[0].wgt, n_out, (int64_t)(sizeof(scn_particle_state) / sizeof(double))};
+    for (int k = 0; k < 4; ++k) J.j[2 + k] = {E->hist_k + (size_t)k * E->cap, n_in, 1};
+    int rc = dsum_batch(E, J);
+    if (rc) return rc;
+    COPY_SYNC(out, E->sum_out, sizeof(double) * 6, hipMemcpyDeviceToHost);
+    return 0;
+}
+
+int close_bins(Engine* E, int which)
+{
+    if (E->n_bins == 0) return 0;
+    const double* ns = nullptr;
+    if (E->norm_bin >= 0) {
+        hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, E->stream, E->bins + E->norm_bin, E->norm_score);
+        ns = E->norm_score;
+    }
+    hipLaunchKernelGGL(k_close_bins, dim3((unsigned)((E->n_bins + 255) / 256)), dim3(256), 0, E->stream, E->bins,
+                       E->csum[which], E->csum2[which], E->n_bins, ns, E->norm_value);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// particleDungeon%normSize_Repr on E->bankB[0..n) (sorted by brood); result in E->bankA
+int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
+{
+    if (n == 0) {
+        E->err = "Normalisation failed: no fission sites";
+        return SCN_ERR_NORM;
+    }
+    int excess = n - tot_pop;
+    int heap = excess < 0 ? ((-excess) % n) : excess;
+    if ((int64_t)tot_pop > E->bank_cap) {
+        E->err = "normSize_Repr: bank capacity too small";
+        return SCN_ERR_OVERFLOW;
+    }
+    if (excess == 0) {
+        HIPCHK(hipMemcpyAsync(E->bankA, E->bankB, sizeof(scn_particle_state) * n, hipMemcpyDeviceToDevice, E->stream));
+        *n_new = n;
+        return 0;
+    }
+    const unsigned g = (unsigned)((n + 255) / 256);
+    uint64_t* thr = nullptr;
+    if (heap != 0) {
+        hipLaunchKernelGGL(k_site_rn, dim3(g), dim3(256), 0, E->stream, master, n, E->rn, E->rn_bits);
+        size_t need = 0;
+        HIPCHK(rocprim::radix_sort_keys(nullptr, need, E->rn_bits, E->rn_sorted, (size_t)n, 0, 64, E->stream));
+        if (need > E->rsort_tmp_bytes) {
+            int rc = dalloc(E, (char**)&E->rsort_tmp, need);
+            if (rc) return rc;
+            E->rsort_tmp_bytes = need;
+        }
+        HIPCHK(rocprim::radix_sort_keys(E->rsort_tmp, need, E->rn_bits, E->rn_sorted, (size_t)n, 0, 64, E->stream));
+        thr = E->rn_sorted + (heap - 1);
+    }
+    size_t need = 0;
+    HIPCHK(rocprim::exclusive_scan(nullptr, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
+    if (need > E->scan_tmp_bytes) {
+        int rc = dalloc(E, (char**)&E->scan_tmp, need);
+        if (rc) return rc;
+        E->scan_tmp_bytes = need;
+    }
+    if (excess > 0) {
+        hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, E->stream, E->rn, n, thr, 1, E->flag);
+        HIPCHK(rocprim::exclusive_scan(E->scan_tmp, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
+        hipLaunchKernelGGL(k_norm_keep, dim3(g), dim3(256), 0, E->stream, E->bankB, E->flag, E->pos, n, E->bankA);
+        int last[2];
+        HIPCHK(hipMemcpyAsync(&last[0], E->flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipMemcpyAsync(&last[1], E->pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+        *n_new = last[0] + last[1];
+    } else {
+        int n_copies = (-excess) / n;
+        int n_dup = (-excess) % n;
+        if (n_dup != 0) {
+            hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, E->stream, E->rn, n, thr, 0, E->flag);
+        } else {
+            HIPCHK(hipMemsetAsync(E->flag, 0, sizeof(int) * n, E->stream));
+        }
+        HIPCHK(rocprim::exclusive_scan(E->scan_tmp, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
+        hipLaunchKernelGGL(k_brood_bounds, dim3(g), dim3(256), 0, E->stream, E->bankB, n, E->cap + 1, E->start_of,
+                           E->end_of);
+        hipLaunchKernelGGL(k_norm_dup, dim3(g), dim3(256), 0, E->stream, E->bankB, E->flag, E->pos, E->start_of,
+                           E->end_of, n, n_copies, E->bankA);
+        int last[2];
+        HIPCHK(hipMemcpyAsync(&last[0], E->flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipMemcpyAsync(&last[1], E->pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+        *n_new = n * (n_copies + 1) + last[0] + last[1];
+    }
+    HIPCHK(hipGetLastError());
+    if (*n_new != tot_pop) {
+        E->err = "Normalisation failed!";
+        return SCN_ERR_NORM;
+    }
+    return 0;
+}
+
+void result2(double csum, double csum2, int N, double out[2])
+{
+    double mean = csum / N;
+    double inv_N = 1.0 / N;
+    double inv_Nm1 = (N != 1) ? 1.0 / (N - 1) : 1.0;
+    double std = csum2 * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
+    out[0] = mean;
+    out[1] = sqrt(std);
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+static thread_local std::string g_create_err;
+
+const char* scn_last_error(const void* engine)
+{
+    if (!engine) return g_create_err.c_str();
+    return ((const Engine*)engine)->err.c_str();
+}
+
+int scn_device_info(char* name, int32_t name_len, int32_t* n_cu)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SCN_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return SCN_ERR_HIP;
+    if (name && name_len > 0) {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return 0;
+}
+
+int scn_create(const scn_problem* pr, int32_t capacity, void** out)
+{
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_err = "no GPU visible to HIP";
+        return SCN_ERR_NO_DEVICE;
+    }
+    if (!pr || capacity <= 0 || pr->mg.n_mat <= 0 || pr->mg.n_groups <= 0 || pr->n_universes <= 0) {
+        g_create_err = "invalid problem tables";
+        return SCN_ERR_ARG;
+    }
+    Engine* E = new Engine();
+    const int nG = pr->mg.n_groups, nM = pr->mg.n_mat;
+    E->nG = nG;
+    E->transport = pr->transport;
+    E->n_bins = pr->n_bins;
+    E->norm_bin = pr->norm_bin;
+    E->norm_value = pr->norm_value;
+    // nesting depth of the geometry graph
+    {
+        std::vector<int> depth(pr->n_universes + 1, 0);
+        // uniIdx -> first graph position from any parent reference (rootID)
+        std::vector<int> rootpos(pr->n_universes + 1, 0);
+        rootpos[pr->root_uni] = 1;
+        for (int i = 0; i < pr->graph_size; ++i)
+            if (pr->graph_fill[i] < 0) rootpos[-pr->graph_fill[i]] = pr->graph_id[i];
+        std::function<int(int)> dep = [&](int u) -> int {
+            int best = 0;
+            const scn_universe& U = pr->universes[u - 1];
+            for (int l = 1; l <= U.n_local; ++l) {
+                int pos = rootpos[u] + l - 1;
+                if (pos < 1 || pos > pr->graph_size) continue;
+                int f = pr->graph_fill[pos - 1];
+                if (f < 0) best = std::max(best, dep(-f));
+            }
+            return best + 1;
+        };
+        int d = dep(pr->root_uni);
+        if (d > SCN_MAX_NEST) {
+            g_create_err = "geometry nesting deeper than SCN_MAX_NEST";
+            delete E;
+            return SCN_ERR_UNSUPPORTED;
+        }
+        E->nl = compiled_nl(d);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) E->n_cu = prop.multiProcessorCount;
+    int rc = 0;
+    if (hipStreamCreate(&E->stream) != hipSuccess) rc = SCN_ERR_HIP;
+    if (!rc && (hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)) rc = SCN_ERR_HIP;
+    scn_universe* d_unis = nullptr;
+    scn_box* d_boxes = nullptr;
+    int *d_gf = nullptr, *d_gi = nullptr, *d_ints = nullptr, *d_fiss = nullptr, *d_goff = nullptr, *d_gidx = nullptr;
+    double *d_real = nullptr, *d_xs = nullptr, *d_P0 = nullptr, *d_prod = nullptr, *d_chi = nullptr, *d_maj = nullptr;
+    scn_clerk* d_clerks = nullptr;
+    // clerks admitted per group (energyFilter evaluated once, host side)
+    std::vector<int> goff(nG + 1, 0), gidx;
+    for (int g = 1; g <= nG; ++g) {
+        for (int c = 0; c < pr->n_clerks; ++c) {
+            const scn_clerk& K = pr->clerks[c];
+            if (K.gtop > 0 && !(K.gtop <= g && g <= K.glow)) continue;
+            gidx.push_back(c);
+        }
+        goff[g] = (int)gidx.size();
+    }
+    if (!rc) rc |= upload(E, &d_unis, pr->universes, pr->n_universes);
+    if (!rc) rc |= upload(E, &d_boxes, pr->boxes, pr->n_boxes);
+    if (!rc) rc |= upload(E, &d_gf, pr->graph_fill, pr->graph_size);
+    if (!rc) rc |= upload(E, &d_gi, pr->graph_id, pr->graph_size);
+    if (!rc) rc |= upload(E, &d_real, pr->real_data, std::max(1, pr->n_real));
+    if (!rc) rc |= upload(E, &d_ints, pr->int_data, std::max(1, pr->n_int));
+    if (!rc) rc |= upload(E, &d_xs, pr->mg.xs, (size_t)nM * nG * 6);
+    if (!rc) rc |= upload(E, &d_P0, pr->mg.P0, (size_t)nM * nG * nG);
+    if (!rc) rc |= upload(E, &d_prod, pr->mg.prod, (size_t)nM * nG * nG);
+    if (!rc) rc |= upload(E, &d_chi, pr->mg.chi, (size_t)nM * nG);
+    if (!rc) rc |= upload(E, &d_fiss, pr->mg.fissile, (size_t)nM);
+    if (!rc) rc |= upload(E, &d_maj, pr->mg.majorant, (size_t)nG);
+    if (!rc) rc |= upload(E, &d_clerks, pr->clerks, std::max(1, pr->n_clerks));
+    if (!rc) rc |= upload(E, &d_goff, goff.data(), goff.size());
+    if (!rc) rc |= upload(E, &d_gidx, gidx.data(), std::max<size_t>(1, gidx.size()));
+    DevProblem& P = E->P;
+    P.transport = pr->transport;
+    P.st_cache = pr->st_cache;
+    P.root_uni = pr->root_uni;
+    P.border_box = pr->border_box;
+    P.n_mat = nM;
+    P.nG = nG;
+    P.n_clerks = pr->n_clerks;
+    P.graph_size = pr->graph_size;
+    P.unis = d_unis;
+    P.boxes = d_boxes;
+    P.gfill = d_gf;
+    P.gid = d_gi;
+    P.real = d_real;
+    P.ints = d_ints;
+    P.xs = d_xs;
+    P.P0 = d_P0;
+    P.prod = d_prod;
+    P.chi = d_chi;
+    P.fissile = d_fiss;
+    P.maj = d_maj;
+    P.collision_xs = pr->mg.collision_xs;
+    P.clerks = d_clerks;
+    P.gclerk_off = d_goff;
+    P.gclerk_idx = d_gidx;
+    P.n_bins = pr->n_bins;
+    // buffers
+    const int cap = capacity;
+    E->cap = cap;
+    E->site_cap = 4 * cap + 1024;
+    E->bank_cap = 4 * cap + 1024;
+    if (!rc) rc |= alloc_live(E, E->live[0], cap, E->nl);
+    if (!rc) rc |= alloc_live(E, E->live[1], cap, E->nl);
+    Sites& S = E->sites;
+    S.cap = E->site_cap;
+    if (!rc) rc |= dalloc(E, &S.r, (size_t)3 * S.cap);
+    if (!rc) rc |= dalloc(E, &S.u, (size_t)3 * S.cap);
+    if (!rc) rc |= dalloc(E, &S.w, S.cap);
+    if (!rc) rc |= dalloc(E, &S.t, S.cap);
+    if (!rc) rc |= dalloc(E, &S.G, S.cap);
+    if (!rc) rc |= dalloc(E, &S.mat, S.cap);
+    if (!rc) rc |= dalloc(E, &S.uid, S.cap);
+    if (!rc) rc |= dalloc(E, &S.key, S.cap);
+    if (!rc) rc |= dalloc(E, &S.order, S.cap);
+    if (!rc) rc |= dalloc(E, &S.count, 1);
+    if (!rc) rc |= dalloc(E, &E->keys_alt, S.cap);
+    if (!rc) rc |= dalloc(E, &E->order_alt, S.cap);
+    if (!rc) rc |= dalloc(E, &E->bankA, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->bankB, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->hist_k, (size_t)4 * cap);
+    if (!rc) rc |= dalloc(E, &E->bins, std::max<int64_t>(1, E->n_bins));
+    for (int w = 0; w < 2 && !rc; ++w) {
+        rc |= dalloc(E, &E->csum[w], std::max<int64_t>(1, E->n_bins));
+        rc |= dalloc(E, &E->csum2[w], std::max<int64_t>(1, E->n_bins));
+    }
+    if (!rc) rc |= dalloc(E, &E->norm_score, 1);
+    if (!rc) rc |= dalloc(E, &E->counters_i, 4);
+    if (!rc) rc |= dalloc(E, &E->counters, 1);
+    if (!rc) rc |= dalloc(E, &E->status, 1);
+    if (!rc) rc |= dalloc(E, &E->sum_m, 8);
+    if (!rc) rc |= dalloc(E, &E->sum_out, 8);
+    if (!rc) rc |= dalloc(E, &E->rn, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->rn_bits, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->rn_sorted, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->flag, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->pos, E->bank_cap);
+    if (!rc) rc |= dalloc(E, &E->start_of, (size_t)cap + 2);
+    if (!rc) rc |= dalloc(E, &E->end_of, (size_t)cap + 2);
+    if (!rc) rc |= ensure_sum(E, (int64_t)E->bank_cap);
+    if (!rc) {
+        if (hipMemset(E->bins, 0, sizeof(double) * std::max<int64_t>(1, E->n_bins)) != hipSuccess ||
+            hipMemset(E->status, 0, sizeof(DevStatus)) != hipSuccess ||
+            hipMemset(E->counters, 0, sizeof(Counters)) != hipSuccess)
+            rc = SCN_ERR_HIP;
+    }
+    E->use_lds = (E->n_bins > 0 && E->n_bins <= LDS_BINS_MAX) ? 1 : 0;
+    E->lds_bytes = E->use_lds ? (int)(E->n_bins * sizeof(double)) : 0;
+    if (rc) {
+        g_create_err = E->err.empty() ? "engine creation failed" : E->err;
+        scn_destroy(E);
+        return rc;
+    }
+    *out = E;
+    return 0;
+}
+
+void scn_destroy(void* engine)
+{
+    Engine* E = (Engine*)engine;
+    if (!E) return;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    for (auto& b : E->owned) (void)hipFree(b.p);
+    if (E->ev0) (void)hipEventDestroy(E->ev0);
+    if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+}
+
+int scn_get_stats(void* engine, scn_stats* out)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !out) return SCN_ERR_ARG;
+    Counters c{};
+    COPY_SYNC(&c, E->counters, sizeof c, hipMemcpyDeviceToHost);
+    *out = E->stats;
+    out->n_segments = (int64_t)c.segments;
+    out->n_collisions = (int64_t)c.collisions;
+    return 0;
+}
+
+int scn_reset_stats(void* engine)
+{
+    Engine* E = (Engine*)engine;
+    if (!E) return SCN_ERR_ARG;
+    E->stats = scn_stats{};
+    HIPCHK(hipMemsetAsync(E->counters, 0, sizeof(Counters), E->stream));
+    return 0;
+}
+
+int scn_eigen_cycle(void* engine, const scn_particle_state* bank_in, int32_t n_in, int64_t rng_state, double k_eff,
+                    scn_particle_state* bank_out, int32_t bank_out_cap, int32_t* n_out, double* bin_scores,
+                    double* keff_scores)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !bank_in || n_in < 0 || !n_out) return SCN_ERR_ARG;
+    if (n_in > E->cap) {
+        E->err = "bank larger than the engine capacity";
+        return SCN_ERR_OVERFLOW;
+    }
+    COPY_SYNC(E->bankA, bank_in, sizeof(scn_particle_state) * n_in, hipMemcpyHostToDevice);
+    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->n_bins, E->stream));
+    int ns = 0;
+    int rc = transport_cycle(E, n_in, (uint64_t)rng_state, k_eff, &ns, bin_scores != nullptr);
+    if (rc) return rc;
+    if (ns > bank_out_cap) {
+        E->err = "Run out of space for particles in the output bank";
+        return SCN_ERR_OVERFLOW;
+    }
+    COPY_SYNC(bank_out, E->bankB, sizeof(scn_particle_state) * ns, hipMemcpyDeviceToHost);
+    *n_out = ns;
+    if (keff_scores) {
+        double sums[6];
+        rc = cycle_sums(E, n_in, ns, sums);
+        if (rc) return rc;
+        for (int k = 0; k < 4; ++k) keff_scores[k] = sums[2 + k];
+    }
+    if (bin_scores && E->n_bins) COPY_SYNC(bin_scores, E->bins, sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+    return 0;
+}
+
+int scn_norm_size_repr(void* engine, scn_particle_state* bank, int32_t* n, int32_t cap, int32_t tot_pop,
+                       int64_t rng_state)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !bank || !n) return SCN_ERR_ARG;
+    if (*n > E->bank_cap || tot_pop > cap) {
+        E->err = "normSize_Repr: bank capacity too small";
+        return SCN_ERR_OVERFLOW;
+    }
+    // sortByBroodID (particleDungeon_class.f90:923-981): stable, host side for caller-supplied banks
+    for (int i = 0; i < *n; ++i) {
+        if (bank[i].broodID < 0 || bank[i].broodID > E->cap + 1) {
+            E->err = "Brood ID out of range";
+            return SCN_ERR_ARG;
+        }
+    }
+    std::stable_sort(bank, bank + *n, [](const scn_particle_state& a, const scn_particle_state& b) {
+        return a.broodID < b.broodID;
+    });
+    COPY_SYNC(E->bankB, bank, sizeof(scn_particle_state) * (*n), hipMemcpyHostToDevice);
+    int nn = 0;
+    int rc = norm_size_device(E, *n, tot_pop, (uint64_t)rng_state, &nn);
+    if (rc) return rc;
+    COPY_SYNC(bank, E->bankA, sizeof(scn_particle_state) * nn, hipMemcpyDeviceToHost);
+    *n = nn;
+    return 0;
+}
+
+int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double bottom[3], const double top[3],
+                       int32_t g_in, int32_t attempts, scn_particle_state* bank)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || n < 0 || n > E->bank_cap || !bank) return SCN_ERR_ARG;
+    (void)g_in;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    switch (E->nl) {
+    case 1:
+        hipLaunchKernelGGL(k_fission_source<1>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
+                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_fission_source<3>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
+                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
+        break;
+    case 4:
+        hipLaunchKernelGGL(k_fission_source<4>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
+                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
+        break;
+    default:
+        hipLaunchKernelGGL(k_fission_source<8>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
+                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
+    }
+    HIPCHK(hipGetLastError());
+    int rc = check_status(E);
+    if (rc) return rc;
+    COPY_SYNC(bank, E->bankA, sizeof(scn_particle_state) * n, hipMemcpyDeviceToHost);
+    return 0;
+}
+
+int scn_eigen_run(void* engine, const scn_eigen_settings* S, const scn_particle_state* initial_bank,
+                  scn_eigen_result* res)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !S || !initial_bank || !res) return SCN_ERR_ARG;
+    const int pop = S->pop;
+    if (pop > E->cap) {
+        E->err = "pop larger than the engine capacity";
+        return SCN_ERR_OVERFLOW;
+    }
+    if (S->events_per_launch > 0) E->k0 = S->events_per_launch;
+    COPY_SYNC(E->bankA, initial_bank, sizeof(scn_particle_state) * pop, hipMemcpyHostToDevice);
+    for (int w = 0; w < 2 && E->n_bins; ++w) {
+        HIPCHK(hipMemsetAsync(E->csum[w], 0, sizeof(double) * E->n_bins, E->stream));
+        HIPCHK(hipMemsetAsync(E->csum2[w], 0, sizeof(double) * E->n_bins, E->stream));
+    }
+    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->n_bins, E->stream));
+    uint64_t master = (uint64_t)S->seed;
+    master = scn_rng_stride(master, 0);
+    master = scn_rng_stride(master, pop);
+    double kA[2] = {0, 0}, kI[2] = {0, 0}, imp[4][2];
+    memset(imp, 0, sizeof imp);
+    int nA = 0, nI = 0;
+    double dk[2] = {1.0, 1.0};
+    int cur = 0;
+    double keff_0 = S->keff0;
+    int cyc = 0;
+    int n_this = pop;
+    res->n_histories = 0;
+    double transport_s = 0.0;
+    Counters c0{};
+    COPY_SYNC(&c0, E->counters, sizeof c0, hipMemcpyDeviceToHost);
+    for (int phase = 0; phase < 2; ++phase) {
+        int ncyc = phase == 0 ? S->n_inactive : S->n_active;
+        int score_bins = (phase == 1 || S->inactive_tally) && E->n_bins > 0;
+        double k_new = keff_0;
+        for (int c = 0; c < ncyc; ++c) {
+            auto t0 = std::chrono::steady_clock::now();
+            int ns = 0;
+            int rc = transport_cycle(E, n_this, master, k_new, &ns, score_bins);
+            if (rc) return rc;
+            res->n_histories += n_this;
+            master = scn_rng_stride(master, (int64_t)pop + 1);
+            double sums[6];
+            rc = cycle_sums(E, n_this, ns, sums);
+            if (rc) return rc;
+            double startW = sums[0], endW = sums[1];
+            double k_norm = dk[1 - cur];
+            if (phase == 0) {
+                double k_eff = endW / startW * k_norm;
+                kA[0] = kA[0] + k_eff;
+                kA[1] = kA[1] + k_eff * k_eff;
+                nA++;
+            } else {
+                double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
+                kI[0] = kI[0] + k_est;
+                kI[1] = kI[1] + k_est * k_est;
+                nI++;
+                for (int k = 0; k < 4; ++k) {
+                    imp[k][0] = imp[k][0] + sums[2 + k];
+                    imp[k][1] = imp[k][1] + sums[2 + k] * sums[2 + k];
+                }
+            }
+            if (score_bins) {
+                rc = close_bins(E, phase == 1 ? 0 : 1);
+                if (rc) return rc;
+            }
+            int nn = 0;
+            rc = norm_size_device(E, ns, pop, master, &nn);
+            if (rc) return rc;
+            master = scn_rng_stride(master, 1);
+            n_this = nn;
+            cur = 1 - cur;
+            double kr[2];
+            if (phase == 0) result2(kA[0], kA[1], nA, kr);
+            else result2(kI[0], kI[1], nI, kr);
+            k_new = kr[0];
+            dk[1 - cur] = k_new;
+            keff_0 = k_new;
+            if (res->k_history) res->k_history[cyc] = k_new;
+            cyc++;
+            HIPCHK(hipStreamSynchronize(E->stream));
+            auto t1 = std::chrono::steady_clock::now();
+            double dt = std::chrono::duration<double>(t1 - t0).count();
+            transport_s += dt;
+            E->stats.cycle_ms += dt * 1e3;
+            E->stats.n_cycles += 1;
+        }
+    }
+    res->transport_time = transport_s;
+    if (nA) result2(kA[0], kA[1], nA, res->k_analog);
+    if (nI) {
+        result2(kI[0], kI[1], nI, res->k_implicit);
+        for (int k = 0; k < 4; ++k) result2(imp[k][0], imp[k][1], nI, res->imp[k]);
+    }
+    Counters c1{};
+    COPY_SYNC(&c1, E->counters, sizeof c1, hipMemcpyDeviceToHost);
+    res->n_collisions = (int64_t)(c1.collisions - c0.collisions);
+    res->n_events = (int64_t)((c1.collisions - c0.collisions) + (c1.segments - c0.segments));
+    for (int w = 0; w < 2 && E->n_bins; ++w) {
+        double* dst = w == 0 ? res->bins_active : res->bins_inactive;
+        int N = w == 0 ? nI : nA;
+        if (!dst || N == 0 || (w == 1 && !S->inactive_tally)) continue;
+        std::vector<double> cs(E->n_bins), cs2(E->n_bins);
+        COPY_SYNC(cs.data(), E->csum[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+        COPY_SYNC(cs2.data(), E->csum2[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+        for (int64_t i = 0; i < E->n_bins; ++i) result2(cs[i], cs2[i], N, dst + 2 * i);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""ctypes driver of libscone_amd.so (the HIP engine behind include/scone_gpu.h).
+
+There is no CPU fallback: if the library or a GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import tables as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscone_amd.so")
+
+# exported symbols of include/scone_gpu.h
+EXPORTS = ["scn_create", "scn_destroy", "scn_last_error", "scn_device_info", "scn_eigen_cycle",
+           "scn_norm_size_repr", "scn_eigen_run", "scn_fission_source", "scn_get_stats", "scn_reset_stats"]
+
+
+class Stats(C.Structure):
+    _fields_ = [("transport_kernel_ms", C.c_double), ("transport_launches", C.c_int64),
+                ("n_histories", C.c_int64), ("n_segments", C.c_int64), ("n_collisions", C.c_int64),
+                ("n_sites", C.c_int64), ("n_scores", C.c_int64), ("n_cycles", C.c_int64),
+                ("cycle_ms", C.c_double)]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it was not built (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER(T.Problem)
+    PS = C.POINTER(T.ParticleState)
+    d3 = C.c_double * 3
+    L.scn_last_error.restype = C.c_char_p
+    L.scn_last_error.argtypes = [C.c_void_p]
+    L.scn_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
+    L.scn_create.argtypes = [P, C.c_int32, C.POINTER(C.c_void_p)]
+    L.scn_destroy.argtypes = [C.c_void_p]
+    L.scn_destroy.restype = None
+    L.scn_eigen_cycle.argtypes = [C.c_void_p, PS, C.c_int32, C.c_int64, C.c_double, PS, C.c_int32,
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.scn_norm_size_repr.argtypes = [C.c_void_p, PS, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int64]
+    L.scn_eigen_run.argtypes = [C.c_void_p, C.POINTER(T.EigenSettings), PS, C.POINTER(T.EigenResult)]
+    L.scn_fission_source.argtypes = [C.c_void_p, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
+    L.scn_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.scn_reset_stats.argtypes = [C.c_void_p]
+    _lib = L
+    return L
+
+
+def device_info():
+    L = load()
+    buf = C.create_string_buffer(256)
+    ncu = C.c_int32()
+    rc = L.scn_device_info(buf, 256, C.byref(ncu))
+    if rc:
+        raise EngineError(f"no GPU visible (code {rc})")
+    return buf.value.decode(), ncu.value
+
+
+def _s64(x: int) -> int:
+    x &= (1 << 64) - 1
+    return x - (1 << 64) if x >= (1 << 63) else x
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+class Engine:
+    """One engine handle: device-resident tables, banks and tallies for one problem."""
+
+    def __init__(self, comp: T.Compiled, capacity: int):
+        self.L = load()
+        self.comp = comp
+        self.capacity = int(capacity)
+        h = C.c_void_p()
+        rc = self.L.scn_create(C.byref(comp.problem), self.capacity, C.byref(h))
+        if rc:
+            raise EngineError(f"scn_create failed ({rc}): {self.L.scn_last_error(None).decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.scn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(f"engine error {rc}: {self.L.scn_last_error(self.h).decode()}")
+
+    def eigen_cycle(self, bank_in: np.ndarray, rng_state: int, k_eff: float, bins: bool = True):
+        n_in = len(bank_in)
+        cap = max(16, 4 * n_in)
+        out = T.bank_array(cap)
+        n_out = C.c_int32()
+        nb = int(self.comp.problem.n_bins)
+        b = np.zeros(max(1, nb)) if bins else None
+        ks = np.zeros(4)
+        bank_in = np.ascontiguousarray(bank_in)
+        self._check(self.L.scn_eigen_cycle(self.h, T.bank_ptr(bank_in), n_in, _s64(rng_state), k_eff,
+                                           T.bank_ptr(out), cap, C.byref(n_out),
+                                           b.ctypes.data_as(C.POINTER(C.c_double)) if b is not None else None,
+                                           ks.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[: n_out.value].copy(), (b[:nb] if b is not None else None), ks
+
+    def norm_size_repr(self, bank: np.ndarray, tot_pop: int, rng_state: int):
+        n = len(bank)
+        cap = max(4 * tot_pop + 16, n)
+        buf = T.bank_array(cap)
+        buf[:n] = bank
+        nn = C.c_int32(n)
+        self._check(self.L.scn_norm_size_repr(self.h, T.bank_ptr(buf), C.byref(nn), cap, tot_pop, _s64(rng_state)))
+        return buf[: nn.value].copy()
+
+    def fission_source(self, n: int, rng_state: int, bottom=None, top=None, g_in: int = 1, attempts: int = 10000):
+        bank = T.bank_array(n)
+        bottom = self.comp.bounds[:3] if bottom is None else bottom
+        top = self.comp.bounds[3:] if top is None else top
+        self._check(self.L.scn_fission_source(self.h, n, _s64(rng_state), _d3(bottom), _d3(top), g_in, attempts,
+                                              T.bank_ptr(bank)))
+        return bank
+
+    def eigen_run(self, settings: dict, initial_bank: np.ndarray):
+        S = T.EigenSettings()
+        S.pop = settings["pop"]
+        S.n_inactive = settings["inactive"]
+        S.n_active = settings["active"]
+        S.inactive_tally = settings.get("inactive_tally", 0)
+        S.seed = _s64(settings["seed"])
+        S.keff0 = settings.get("keff0", 1.0)
+        S.events_per_launch = settings.get("events_per_launch", 0)
+        S.block_size = settings.get("block_size", 0)
+        R = T.EigenResult()
+        nb = int(self.comp.problem.n_bins)
+        kh = np.zeros(S.n_inactive + S.n_active)
+        ba = np.zeros(max(1, 2 * nb))
+        bi = np.zeros(max(1, 2 * nb))
+        R.k_history = kh.ctypes.data_as(C.POINTER(C.c_double))
+        R.bins_active = ba.ctypes.data_as(C.POINTER(C.c_double))
+        R.bins_inactive = bi.ctypes.data_as(C.POINTER(C.c_double))
+        bank = np.ascontiguousarray(initial_bank)
+        self._check(self.L.scn_eigen_run(self.h, C.byref(S), T.bank_ptr(bank), C.byref(R)))
+        return R, kh, ba[: 2 * nb].reshape(-1, 2), bi[: 2 * nb].reshape(-1, 2)
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.L.scn_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def reset_stats(self):
+        self._check(self.L.scn_reset_stats(self.h))

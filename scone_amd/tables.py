@@ -1,0 +1,656 @@
+"""Flat tables of the C ABI (include/scone_gpu.h) and the compiler that builds them from SCONE
+input dictionaries.
+
+The compiler restates the reference's construction code for the multigroup path:
+
+* geometry: csg surfaces/universes (Geometry/csg_class.f90), universe init routines
+  (Geometry/Universes/{root,lat,pin}Universe_class.f90 ``init``) and the geometry graph
+  (Geometry/geomGraph_class.f90 buildShrunk/buildExtended/setUniqueIDs);
+* multigroup data: baseMgNeutronMaterial_class.f90 ``init``, multiScatterMG_class.f90
+  ``buildFromDict``, fissionMG_class.f90 ``buildFromDict`` and the majorant of
+  baseMgNeutronDatabase_class.f90 ``initMajorant``;
+* tallies: tallyAdmin_class.f90 ``init`` (memory addresses), collisionClerk/maps/filters/responses.
+
+All derived floating-point values are computed with the reference's operation order (sequential
+sums from 0.0, products as written), so the tables are bit-identical to what SCONE builds.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import dictio
+
+# ---- constants (include/scone_gpu.h) ------------------------------------------------------
+MAX_NEST = 8
+OUTSIDE_MAT = 0
+VOID_MAT = 2147483647
+UNDEF_MAT = 2147483646
+INF = 2.0 ** 63
+SURF_TOL = 1.0e-12
+FP_REL_TOL = 1.0e-7
+TRANSPORT_ST, TRANSPORT_DT = 1, 2
+UNI_ROOT, UNI_LAT, UNI_PIN = 1, 2, 3
+MAP_SPACE, MAP_MATERIAL = 1, 2
+GRID_LIN, GRID_UNSTRUCT = 1, 3
+KAPPA_DEFAULT = float(np.float32(202.27))   # fissionMG_class.f90:64 single-precision literal
+CHI_TOL = float(np.float32(0.01)) * FP_REL_TOL  # fissionMG_class.f90:268
+MACRO_MT = {"total": -1}
+
+
+# ---- ctypes mirrors of the ABI structs ----------------------------------------------------
+class ParticleState(C.Structure):
+    _fields_ = [("wgt", C.c_double), ("r", C.c_double * 3), ("dir", C.c_double * 3), ("E", C.c_double),
+                ("G", C.c_int32), ("isMG", C.c_int32), ("type", C.c_int32), ("pad0", C.c_int32),
+                ("time", C.c_double), ("lambda_", C.c_double), ("matIdx", C.c_int32),
+                ("cellIdx", C.c_int32), ("uniqueID", C.c_int32), ("collisionN", C.c_int32),
+                ("broodID", C.c_int32), ("pad1", C.c_int32)]
+
+
+PARTICLE_DTYPE = np.dtype([("wgt", "f8"), ("r", "f8", 3), ("dir", "f8", 3), ("E", "f8"), ("G", "i4"),
+                           ("isMG", "i4"), ("type", "i4"), ("pad0", "i4"), ("time", "f8"),
+                           ("lambda", "f8"), ("matIdx", "i4"), ("cellIdx", "i4"), ("uniqueID", "i4"),
+                           ("collisionN", "i4"), ("broodID", "i4"), ("pad1", "i4")])
+assert PARTICLE_DTYPE.itemsize == C.sizeof(ParticleState)
+
+
+class Box(C.Structure):
+    _fields_ = [("n_axes", C.c_int32), ("axes", C.c_int32 * 3), ("bc", C.c_int32 * 6),
+                ("origin", C.c_double * 3), ("halfwidth", C.c_double * 3)]
+
+
+class Universe(C.Structure):
+    _fields_ = [("type", C.c_int32), ("n_local", C.c_int32), ("origin", C.c_double * 3),
+                ("border", C.c_int32), ("size", C.c_int32 * 3), ("out_local", C.c_int32),
+                ("offset_mode", C.c_int32), ("offset_map", C.c_int32), ("pitch", C.c_double * 3),
+                ("corner", C.c_double * 3), ("a_bar", C.c_double * 3), ("outline_hw", C.c_double * 3),
+                ("n_annuli", C.c_int32), ("annuli", C.c_int32)]
+
+
+class MGData(C.Structure):
+    _fields_ = [("n_mat", C.c_int32), ("n_groups", C.c_int32), ("xs", C.POINTER(C.c_double)),
+                ("P0", C.POINTER(C.c_double)), ("prod", C.POINTER(C.c_double)),
+                ("chi", C.POINTER(C.c_double)), ("fissile", C.POINTER(C.c_int32)),
+                ("majorant", C.POINTER(C.c_double)), ("collision_xs", C.c_double)]
+
+
+class Map1D(C.Structure):
+    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("nbins", C.c_int32), ("grid_type", C.c_int32),
+                ("data", C.c_int32)]
+
+
+class Clerk(C.Structure):
+    _fields_ = [("gtop", C.c_int32), ("glow", C.c_int32), ("n_maps", C.c_int32), ("handle_virtual", C.c_int32),
+                ("maps", Map1D * 3), ("multi", C.c_int32 * 3), ("width", C.c_int32), ("resp", C.c_int32 * 8),
+                ("mem_addr", C.c_int64), ("size", C.c_int64)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("transport", C.c_int32), ("st_cache", C.c_int32), ("n_universes", C.c_int32),
+                ("root_uni", C.c_int32), ("universes", C.POINTER(Universe)), ("n_boxes", C.c_int32),
+                ("border_box", C.c_int32), ("boxes", C.POINTER(Box)), ("graph_size", C.c_int32),
+                ("graph_fill", C.POINTER(C.c_int32)), ("graph_id", C.POINTER(C.c_int32)),
+                ("n_real", C.c_int32), ("real_data", C.POINTER(C.c_double)), ("n_int", C.c_int32),
+                ("int_data", C.POINTER(C.c_int32)), ("mg", MGData), ("n_clerks", C.c_int32),
+                ("clerks", C.POINTER(Clerk)), ("n_bins", C.c_int64), ("norm_bin", C.c_int64),
+                ("norm_value", C.c_double)]
+
+
+class EigenSettings(C.Structure):
+    _fields_ = [("pop", C.c_int32), ("n_inactive", C.c_int32), ("n_active", C.c_int32),
+                ("inactive_tally", C.c_int32), ("seed", C.c_int64), ("keff0", C.c_double),
+                ("events_per_launch", C.c_int32), ("block_size", C.c_int32)]
+
+
+class EigenResult(C.Structure):
+    _fields_ = [("k_analog", C.c_double * 2), ("k_implicit", C.c_double * 2), ("imp", (C.c_double * 2) * 4),
+                ("k_history", C.POINTER(C.c_double)), ("bins_active", C.POINTER(C.c_double)),
+                ("bins_inactive", C.POINTER(C.c_double)), ("transport_time", C.c_double),
+                ("n_collisions", C.c_int64), ("n_events", C.c_int64), ("n_histories", C.c_int64)]
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+# ---- MG material (baseMgNeutronMaterial_class.f90:186-291) --------------------------------
+class MGMaterial:
+    """One material's multigroup data exactly as baseMgNeutronMaterial%init builds it."""
+
+    def __init__(self, d: dict, name: str = ""):
+        self.name = name
+        nG = int(d["numberOfGroups"])
+        if nG < 1:
+            raise ValueError("Number of groups is invalid")
+        self.nG = nG
+        self.fissile = "fission" in d
+        P0 = [float(x) for x in d["P0"]]
+        prod = [float(x) for x in d["scatteringMultiplicity"]]
+        if len(P0) != nG * nG or len(prod) != nG * nG:
+            raise ValueError("Invalid size of P0 / scatteringMultiplicity")
+        # reshape(temp, [nG, nG]): flat index (gin-1)*nG + (gout-1) is P0(gout, gin)
+        self.P0 = P0
+        self.prod = prod
+        scatter = []
+        for gin in range(nG):  # sum(P0, 1): sequential over G_out from 0.0
+            s = 0.0
+            for gout in range(nG):
+                s = s + P0[gin * nG + gout]
+            scatter.append(s)
+        capture = [float(x) for x in d["capture"]]
+        if len(capture) != nG:
+            raise ValueError("Capture XSs have wrong size")
+        data = np.zeros((nG, 6))
+        for g in range(nG):
+            data[g, 1] = scatter[g]
+            data[g, 2] = capture[g]
+        self.chi = [0.0] * nG
+        if self.fissile:
+            fission = [float(x) for x in d["fission"]]
+            nu = [float(x) for x in d["nu"]]
+            chi = [float(x) for x in d["chi"]]
+            if len(fission) != nG or len(nu) != nG or len(chi) != nG:
+                raise ValueError("fission data of wrong size")
+            if "kappa" in d:
+                kappa = [float(x) for x in d["kappa"]]
+            else:
+                kappa = [KAPPA_DEFAULT] * nG
+            s = 0.0
+            for c in chi:
+                s = s + c
+            if abs(s - 1.0) > CHI_TOL:
+                chi = [c / s for c in chi]
+            self.chi = chi
+            for g in range(nG):
+                data[g, 3] = fission[g]
+                data[g, 4] = nu[g] * fission[g]
+                data[g, 5] = kappa[g] * fission[g]
+        for g in range(nG):
+            t = data[g, 1] + data[g, 2]
+            if self.fissile:
+                t = t + data[g, 3]
+            data[g, 0] = t
+        self.data = data
+
+
+# ---- geometry compiler -------------------------------------------------------------------
+_BC_NAMES = {}
+
+
+def _is_universe_fill(name):
+    return isinstance(name, str) and name.startswith("u<")
+
+
+class Compiled:
+    """Holds numpy arrays behind a ctypes Problem (keeps them alive)."""
+
+    def __init__(self):
+        self.keep = []
+        self.problem = Problem()
+        self.mat_names = []
+        self.clerk_names = []
+        self.clerk_shapes = []
+        self.clerk_info = []
+        self.nG = 0
+        self.bounds = None
+
+    def hold(self, arr):
+        self.keep.append(arr)
+        return arr
+
+
+def _surface_box(sd: dict, bc):
+    t = sd["type"]
+    b = Box()
+    b.bc[:] = [int(x) for x in bc] if bc is not None else [0] * 6
+    origin = [float(x) for x in sd["origin"]]
+    hw = [float(x) for x in sd["halfwidth"]]
+    if t == "box":
+        b.n_axes = 3
+        b.axes[:] = [0, 1, 2]
+        b.origin[:] = origin
+        b.halfwidth[:] = hw
+        bounds = [origin[i] - hw[i] for i in range(3)] + [origin[i] + hw[i] for i in range(3)]
+    elif t in ("xSquareCylinder", "ySquareCylinder", "zSquareCylinder"):
+        plane = {"x": [1, 2], "y": [0, 2], "z": [0, 1]}[t[0]]
+        axis = {"x": 0, "y": 1, "z": 2}[t[0]]
+        b.n_axes = 2
+        b.axes[:] = plane + [0]
+        # squareCylinder keeps origin/halfwidth of its plane (squareCylinder_class.f90 init)
+        b.origin[:] = [origin[plane[0]], origin[plane[1]], 0.0]
+        b.halfwidth[:] = [hw[plane[0]], hw[plane[1]], 0.0]
+        bounds = [0.0] * 6
+        for i, ax in enumerate(plane):
+            bounds[ax] = b.origin[i] - b.halfwidth[i]
+            bounds[3 + ax] = b.origin[i] + b.halfwidth[i]
+        bounds[axis] = -INF
+        bounds[3 + axis] = INF
+    else:
+        raise NotImplementedError(f"border surface type {t} is outside the engine's scope")
+    for i in range(3):  # geometryStd%bounds: infinite axes become (0, 0)
+        if bounds[i] <= -INF and bounds[i + 3] >= INF:
+            bounds[i] = 0.0
+            bounds[i + 3] = 0.0
+    return b, bounds
+
+
+def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = ".",
+                    tally_key: str = "activeTally") -> Compiled:
+    """Compile a SCONE input dictionary (eigenPhysicsPackage / fixedSourcePhysicsPackage keys)
+    into the flat tables of include/scone_gpu.h.
+
+    mg_dicts: optional {material name: parsed xsFile dictionary} used instead of reading files.
+    """
+    out = Compiled()
+    P = out.problem
+
+    # --- transport operator (transportOperatorFactory_func.f90:36-67)
+    top = inp.get("transportOperator", {"type": "transportOperatorST"})
+    ttype = top["type"]
+    if ttype == "transportOperatorST":
+        P.transport = TRANSPORT_ST
+    elif ttype == "transportOperatorDT":
+        P.transport = TRANSPORT_DT
+    else:
+        raise NotImplementedError(f"{ttype} is outside the engine's scope")
+    P.st_cache = int(top.get("cache", 1))
+
+    # --- materials (materialMenu order = order of the materials dictionary)
+    nd = inp["nuclearData"]
+    mats_d = nd["materials"]
+    names = list(mats_d.keys())
+    out.mat_names = names
+    mat_idx = {n: i + 1 for i, n in enumerate(names)}
+    mats = []
+    for n in names:
+        md = mats_d[n]
+        if mg_dicts is not None and n in mg_dicts:
+            dd = mg_dicts[n]
+        else:
+            path = md["xsFile"]
+            if not os.path.isabs(path):
+                path = os.path.join(base_dir, path)
+            dd = dictio.read(path)
+        mats.append(MGMaterial(dd, n))
+    nG = mats[0].nG
+    for m in mats:
+        if m.nG != nG:
+            raise ValueError("Inconsistent # of groups in materials")
+    out.nG = nG
+    handles = nd.get("handles", {}).get("mg", {})
+    if handles.get("PN", "P0") != "P0":
+        raise NotImplementedError("P1 scattering is outside the engine's scope")
+    collision_xs = 0.0
+    if "avgDist" in handles:
+        collision_xs = 1.0 / float(handles["avgDist"])
+
+    # --- geometry
+    gd = inp["geometry"]
+    bc = gd.get("boundary", [0] * 6)
+    surfs = gd.get("surfaces", {})
+    surf_by_id = {int(sd["id"]): (name, sd) for name, sd in surfs.items()}
+    unis = gd["universes"]
+    uni_by_id = {}
+    order = []
+    for name, ud in unis.items():
+        uni_by_id[int(ud["id"])] = ud
+        order.append(int(ud["id"]))
+    uni_idx = {uid: i + 1 for i, uid in enumerate(order)}
+    real_data = []
+    int_data = []
+    universes = []
+    boxes = []
+    fills = {}
+    root_uid = None
+
+    def fill_of(name):
+        # universe_inter.f90 charToFill: 'u<id>' -> -id, material name -> matIdx
+        if _is_universe_fill(name):
+            return -int(name[2:name.rindex(">")])
+        if name == "void":
+            return VOID_MAT
+        if name not in mat_idx:
+            raise ValueError(f"Unknown material: {name}")
+        return mat_idx[name]
+
+    for uid in order:
+        ud = uni_by_id[uid]
+        U = Universe()
+        org = [float(x) for x in ud.get("origin", [0.0, 0.0, 0.0])]
+        if any(float(x) != 0.0 for x in ud.get("rotation", [0.0, 0.0, 0.0])):
+            raise NotImplementedError("rotated universes are outside the engine's scope")
+        U.origin[:] = org
+        t = ud["type"]
+        if t == "rootUniverse":
+            root_uid = uid
+            U.type = UNI_ROOT
+            sid = int(ud["border"])
+            b, bounds = _surface_box(surf_by_id[sid][1], bc)
+            out.bounds = bounds
+            U.border = len(boxes)
+            boxes.append(b)
+            U.n_local = 2
+            fills[uid] = [fill_of(ud["fill"]), OUTSIDE_MAT]
+        elif t == "latUniverse":
+            U.type = UNI_LAT
+            pitch = [float(x) for x in ud["pitch"]]
+            shape = [int(x) for x in ud["shape"]]
+            if shape[2] == 0:
+                shape[2] = 1
+                pitch[2] = 2.0 * INF
+            sizeN = shape
+            a_bar = [pitch[i] * 0.5 - pitch[i] * SURF_TOL for i in range(3)]
+            corner = [-(float(sizeN[i]) * 0.5 * pitch[i]) for i in range(3)]
+            U.size[:] = sizeN
+            U.pitch[:] = pitch
+            U.a_bar[:] = a_bar
+            U.corner[:] = corner
+            U.outline_hw[:] = [abs(c) for c in corner]
+            nloc = sizeN[0] * sizeN[1] * sizeN[2]
+            U.out_local = nloc + 1
+            U.n_local = nloc + 1
+            mp = [int(x) for x in ud["map"]]
+            if len(mp) != nloc:
+                raise ValueError("Lattice map size not equal to size implied by shape")
+            R = sizeN[1] * sizeN[2]
+            N1 = sizeN[0]
+            f = [0] * (nloc + 1)
+            for j in range(R):          # rows flipped: WYSIWYG map (latUniverse_class.f90:200-206)
+                src = R - 1 - j
+                for i in range(N1):
+                    f[i + j * N1] = -mp[src * N1 + i]
+            f[nloc] = fill_of(ud["padMat"])
+            fills[uid] = f
+            offset = ud.get("offset", 1)
+            offset = 1 if (offset in (1, "true", "yes", "y", "T")) else 0
+            U.offset_mode = offset
+            if "offsetMap" in ud:
+                om = [int(x) for x in ud["offsetMap"]]
+                U.offset_mode = 2
+                U.offset_map = len(int_data)
+                flat = [0] * (nloc + 1)
+                for j in range(R):
+                    src = R - 1 - j
+                    for i in range(N1):
+                        flat[i + j * N1] = om[src * N1 + i]
+                int_data.extend(flat)
+        elif t == "pinUniverse":
+            U.type = UNI_PIN
+            radii = [float(x) for x in ud["radii"]]
+            fnames = list(ud["fills"])
+            if len(radii) != len(fnames):
+                raise ValueError("Size of radii and fills does not match")
+            N = len(radii)
+            # pinUniverse_class.f90 init: zero radius -> last with INF*1.1, selection sort ascending
+            idx = min(range(N), key=lambda k: (radii[k], k))
+            if radii[idx] != 0.0:
+                raise ValueError("Did not find outermost element with radius 0.0")
+            radii[idx], radii[N - 1] = radii[N - 1], radii[idx]
+            fnames[idx], fnames[N - 1] = fnames[N - 1], fnames[idx]
+            radii[N - 1] = INF * 1.1
+            for i in range(N - 2, -1, -1):
+                sub = radii[: i + 1]
+                m = max(range(i + 1), key=lambda k: (sub[k], -k))
+                radii[m], radii[i] = radii[i], radii[m]
+                fnames[m], fnames[i] = fnames[i], fnames[m]
+            for i in range(N - 1):
+                if radii[i] == radii[i + 1]:
+                    raise ValueError("Duplicate value of radius")
+            U.n_annuli = N
+            U.annuli = len(real_data)
+            real_data.extend(radii)
+            real_data.extend([r * r for r in radii])
+            U.n_local = N
+            fills[uid] = [fill_of(n) for n in fnames]
+        else:
+            raise NotImplementedError(f"universe type {t} is outside the engine's scope")
+        universes.append(U)
+    if root_uid is None:
+        raise ValueError("no rootUniverse")
+
+    # universe fills: -ID -> -uniIdx
+    idx_fills = {}
+    for uid, f in fills.items():
+        idx_fills[uni_idx[uid]] = [(-uni_idx[-x] if x < 0 else x) for x in f]
+
+    # geomGraph (geomGraph_class.f90)
+    graph_type = gd.get("graph", {"type": "shrunk"}).get("type", "shrunk")
+    root_idx = uni_idx[root_uid]
+    used = set()
+
+    def collect(u):
+        if u in used:
+            return
+        used.add(u)
+        for x in idx_fills[u]:
+            if x < 0:
+                collect(-x)
+    collect(root_idx)
+    arr_fill, arr_id = [], []
+    if graph_type == "shrunk":
+        layed = {root_idx: 1}
+        arr_fill.extend(idx_fills[root_idx])
+        arr_id.extend([0] * len(idx_fills[root_idx]))
+        loc = 0
+        while loc < len(arr_fill):
+            f = arr_fill[loc]
+            if f < 0:
+                if -f not in layed:
+                    layed[-f] = len(arr_fill) + 1
+                    arr_fill.extend(idx_fills[-f])
+                    arr_id.extend([0] * len(idx_fills[-f]))
+                arr_id[loc] = layed[-f]
+            loc += 1
+    elif graph_type == "extended":
+        arr_fill.extend(idx_fills[root_idx])
+        arr_id.extend([0] * len(idx_fills[root_idx]))
+        loc = 0
+        while loc < len(arr_fill):
+            f = arr_fill[loc]
+            if f < 0:
+                arr_id[loc] = len(arr_fill) + 1
+                arr_fill.extend(idx_fills[-f])
+                arr_id.extend([0] * len(idx_fills[-f]))
+            loc += 1
+    else:
+        raise ValueError(f"Unknown geometry graph type: {graph_type}")
+    cells = 0
+    used_mats = set()
+    for i, f in enumerate(arr_fill):
+        if f > 0:
+            cells += 1
+            arr_id[i] = cells
+            used_mats.add(f)
+    active = sorted(m for m in used_mats if m not in (VOID_MAT, UNDEF_MAT, VOID_MAT - 2))
+
+    # nesting depth check
+    def depth(u):
+        return 1 + max([depth(-x) for x in idx_fills[u] if x < 0] or [0])
+    if depth(root_idx) > MAX_NEST:
+        raise NotImplementedError(f"nesting deeper than {MAX_NEST} levels")
+
+    # --- MG tables
+    xs = np.zeros((len(mats), nG, 6))
+    P0 = np.zeros((len(mats), nG * nG))
+    prod = np.zeros((len(mats), nG * nG))
+    chi = np.zeros((len(mats), nG))
+    fiss = np.zeros(len(mats), dtype=np.int32)
+    for i, m in enumerate(mats):
+        xs[i] = m.data
+        P0[i] = m.P0
+        prod[i] = m.prod
+        chi[i] = m.chi
+        fiss[i] = 1 if m.fissile else 0
+    majorant = np.zeros(nG)
+    for g in range(nG):   # initMajorant: max over active materials, densityFactor 1
+        x = 0.0
+        for mi in active:
+            v = xs[mi - 1, g, 0]
+            x = v if v > x else x
+        majorant[g] = x * 1.0
+    out.active_mats = active
+
+    # --- tallies
+    tal = inp.get(tally_key, {}) or {}
+    clerks = []
+    addr = 0
+    norm_name = tal.get("norm")
+    norm_bin = -1
+    for cname, cd in tal.items():
+        if not isinstance(cd, dict):
+            continue
+        ctype = cd.get("type")
+        if ctype != "collisionClerk":
+            raise NotImplementedError(f"tally clerk {ctype} is outside the engine's scope")
+        K = Clerk()
+        K.handle_virtual = 1
+        hv = cd.get("handleVirtual", 1)
+        K.handle_virtual = 0 if hv in (0, "false", "no", "n", "F") else 1
+        if "filter" in cd:
+            fd = cd["filter"]
+            if fd.get("type") != "energyFilter" or "Gtop" not in fd:
+                raise NotImplementedError("only MG energyFilter filters are supported")
+            K.gtop, K.glow = int(fd["Gtop"]), int(fd["Glow"])
+            if K.gtop > K.glow:
+                raise ValueError("Gtop larger than Glow")
+        shape = []
+        axes = []
+        if "map" in cd:
+            md = cd["map"]
+            if md["type"] == "multiMap":
+                subs = [md[n] for n in md["maps"]]
+            else:
+                subs = [md]
+            if len(subs) > 3:
+                raise NotImplementedError("multiMap with more than 3 maps")
+            K.n_maps = len(subs)
+            mul = 1
+            for k, sd in enumerate(subs):
+                M = Map1D()
+                if sd["type"] == "spaceMap":
+                    M.type = MAP_SPACE
+                    M.axis = {"x": 0, "y": 1, "z": 2}[sd["axis"]]
+                    if sd["grid"] == "lin":
+                        mn, mx, N = float(sd["min"]), float(sd["max"]), int(sd["N"])
+                        M.grid_type = GRID_LIN
+                        M.nbins = N
+                        M.data = len(real_data)
+                        real_data.extend([mn, (mx - mn) / N])
+                        edges = [mn] + [mn + (i - 1) * ((mx - mn) / N) for i in range(2, N + 2)]
+                    else:
+                        edges = [float(x) for x in sd["bins"]]
+                        M.grid_type = GRID_UNSTRUCT
+                        M.nbins = len(edges) - 1
+                        M.data = len(real_data)
+                        real_data.extend(edges)
+                    axes.append(("space", sd["axis"], edges))
+                elif sd["type"] == "materialMap":
+                    M.type = MAP_MATERIAL
+                    mlist = list(sd["materials"])
+                    undef = sd.get("undefBin", "false") in ("yes", "y", "true", "TRUE", "T")
+                    nb = len(mlist) + (1 if undef else 0)
+                    default = len(mlist) + 1 if undef else 0
+                    table = [default] * (len(mats) + 2)
+                    for bi, mn in enumerate(mlist):
+                        table[mat_idx[mn]] = bi + 1
+                    M.nbins = nb
+                    M.data = len(int_data)
+                    int_data.extend(table)
+                    axes.append(("material", mlist, undef))
+                else:
+                    raise NotImplementedError(f"tally map {sd['type']} is outside the engine's scope")
+                K.maps[k] = M
+                K.multi[k] = mul
+                mul *= M.nbins
+                shape.append(M.nbins)
+        resp = list(cd["response"])
+        if len(resp) > 8:
+            raise NotImplementedError("more than 8 responses")
+        K.width = len(resp)
+        for i, rn in enumerate(resp):
+            rd = cd[rn]
+            if rd["type"] == "fluxResponse":
+                K.resp[i] = 0
+            elif rd["type"] == "macroResponse":
+                mt = int(rd["MT"])
+                pos = {1: -1, 2: -3, 3: -22, 101: -2, 18: -6, 27: -21}
+                if mt > 0:
+                    mt = pos.get(mt, None)
+                    if mt is None:
+                        raise NotImplementedError("microscopic MT responses need CE data")
+                K.resp[i] = mt
+            else:
+                raise NotImplementedError(f"response {rd['type']} is outside the engine's scope")
+        size = K.width * int(np.prod(shape)) if shape else K.width
+        K.mem_addr = addr
+        K.size = size
+        if norm_name == cname:
+            norm_bin = addr
+        addr += size
+        clerks.append(K)
+        out.clerk_names.append(cname)
+        out.clerk_shapes.append([K.width] + shape)
+        out.clerk_info.append({"name": cname, "addr": int(K.mem_addr), "size": int(size), "axes": axes,
+                               "width": K.width, "shape": [K.width] + shape})
+
+    # --- assemble
+    uni_arr = (Universe * len(universes))(*universes)
+    box_arr = (Box * len(boxes))(*boxes)
+    clerk_arr = (Clerk * max(1, len(clerks)))(*clerks)
+    out.hold(uni_arr)
+    out.hold(box_arr)
+    out.hold(clerk_arr)
+    gf = out.hold(np.asarray(arr_fill, dtype=np.int32))
+    gi = out.hold(np.asarray(arr_id, dtype=np.int32))
+    rd = out.hold(np.asarray(real_data + [0.0], dtype=np.float64))
+    idt = out.hold(np.asarray(int_data + [0], dtype=np.int32))
+    xs_c = out.hold(np.ascontiguousarray(xs.reshape(-1)))
+    P0_c = out.hold(np.ascontiguousarray(P0.reshape(-1)))
+    prod_c = out.hold(np.ascontiguousarray(prod.reshape(-1)))
+    chi_c = out.hold(np.ascontiguousarray(chi.reshape(-1)))
+    fiss_c = out.hold(fiss)
+    maj_c = out.hold(majorant)
+    P.n_universes = len(universes)
+    P.root_uni = root_idx
+    P.universes = C.cast(uni_arr, C.POINTER(Universe))
+    P.n_boxes = len(boxes)
+    P.border_box = universes[root_idx - 1].border
+    P.boxes = C.cast(box_arr, C.POINTER(Box))
+    P.graph_size = len(arr_fill)
+    P.graph_fill = _ptr(gf, C.c_int32)
+    P.graph_id = _ptr(gi, C.c_int32)
+    P.n_real = len(real_data)
+    P.real_data = _ptr(rd, C.c_double)
+    P.n_int = len(int_data)
+    P.int_data = _ptr(idt, C.c_int32)
+    P.mg.n_mat = len(mats)
+    P.mg.n_groups = nG
+    P.mg.xs = _ptr(xs_c, C.c_double)
+    P.mg.P0 = _ptr(P0_c, C.c_double)
+    P.mg.prod = _ptr(prod_c, C.c_double)
+    P.mg.chi = _ptr(chi_c, C.c_double)
+    P.mg.fissile = _ptr(fiss_c, C.c_int32)
+    P.mg.majorant = _ptr(maj_c, C.c_double)
+    P.mg.collision_xs = collision_xs
+    P.n_clerks = len(clerks)
+    P.clerks = C.cast(clerk_arr, C.POINTER(Clerk))
+    P.n_bins = addr
+    P.norm_bin = norm_bin
+    P.norm_value = float(tal.get("normVal", 1.0))
+    out.xs = xs
+    out.majorant = majorant
+    out.mats = mats
+    out.graph = (arr_fill, arr_id)
+    out.uni_idx = uni_idx
+    return out
+
+
+def bank_array(n: int) -> np.ndarray:
+    return np.zeros(n, dtype=PARTICLE_DTYPE)
+
+
+def bank_ptr(bank: np.ndarray):
+    return bank.ctypes.data_as(C.POINTER(ParticleState))

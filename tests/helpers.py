@@ -1,0 +1,33 @@
+"""Shared fixtures for the parity tests: problems, banks and comparisons."""
+import numpy as np
+
+from scone_amd import configs, tables as T
+
+BANK_FIELDS = ["wgt", "r", "dir", "E", "G", "isMG", "type", "time", "lambda", "matIdx", "cellIdx",
+               "uniqueID", "collisionN", "broodID"]
+
+
+def problem(name, **kw):
+    inp, mg = configs.CONFIGS[name](**kw)
+    return inp, mg, T.compile_problem(inp, mg)
+
+
+def assert_banks_identical(a, b, fields=None):
+    assert len(a) == len(b), f"bank sizes differ: {len(a)} vs {len(b)}"
+    for f in fields or BANK_FIELDS:
+        x, y = a[f], b[f]
+        if x.dtype.kind == "f":
+            bad = np.nonzero(x.view(np.uint64) != y.view(np.uint64))[0] if x.ndim == 1 else \
+                np.nonzero((x.view(np.uint64) != y.view(np.uint64)).any(axis=1))[0]
+        else:
+            bad = np.nonzero(x != y)[0] if x.ndim == 1 else np.nonzero((x != y).any(axis=1))[0]
+        assert len(bad) == 0, f"field {f} differs at {len(bad)} sites, first {bad[:5]}: {x[bad[:3]]} vs {y[bad[:3]]}"
+
+
+def rel_close(a, b, rtol=1e-12):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    diff = np.abs(a - b)
+    ok = diff <= rtol * scale
+    return bool(ok.all()), float(np.max(np.where(scale > 0, diff / np.where(scale > 0, scale, 1), 0.0)))
