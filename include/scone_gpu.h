@@ -196,6 +196,7 @@ typedef struct scn_eigen_result {
     double k_implicit[2];    /* active keffImplicitClerk K_EFF mean, sigma */
     double imp[4][2];        /* IMP_PROD, SCATTER_PROD, IMP_ABS, ANA_LEAK mean, sigma (active) */
     double* k_history;       /* caller buffer [n_inactive + n_active] or NULL: k used next cycle */
+    double* cycle_time;      /* caller buffer [n_inactive + n_active] or NULL: wall seconds per cycle */
     double* bins_active;     /* caller buffers [n_bins * 2] (mean, sigma) or NULL */
     double* bins_inactive;
     double transport_time;   /* seconds, same span as SCONE's Total_Transport_Time */
@@ -247,6 +248,13 @@ int scn_norm_size_repr(void* engine, scn_particle_state* bank, int32_t* n, int32
  * (eigenPhysicsPackage_class.f90:135-159) from an already generated initial bank. */
 int scn_eigen_run(void* engine, const scn_eigen_settings* settings, const scn_particle_state* initial_bank,
                   scn_eigen_result* result);
+
+/* The same power iteration split into calls, so a caller can time or interleave cycles:
+ * start = run() up to the first cycle, step = the next n cycles (inactive, then active; *done
+ * receives how many ran), finish = collectResults.  scn_eigen_run = start + step(all) + finish. */
+int scn_eigen_start(void* engine, const scn_eigen_settings* settings, const scn_particle_state* initial_bank);
+int scn_eigen_step(void* engine, int32_t n_cycles, int32_t* done);
+int scn_eigen_finish(void* engine, scn_eigen_result* result);
 
 /* Initial fission source: fissionSource%sampleParticle for particles 1..n
  * (ParticleObjects/Source/fissionSource_class.f90:149-271, source_inter.f90:98-119);

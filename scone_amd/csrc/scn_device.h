@@ -66,6 +66,11 @@ __device__ __forceinline__ void raise_error(DevStatus* st, int code, int detail)
     st->detail = detail;
 }
 
+// per-thread event counters (segments, real collisions, clerk score additions)
+struct Cnt {
+    int64_t seg, col, sc;
+};
+
 template <int NL>
 struct Pt {
     double r[NL][3];   // position at each nesting level (coordList%lvl(l)%r)
@@ -622,8 +627,9 @@ __device__ __forceinline__ void bin_add(const Bins& B, int64_t addr, double v)
 
 // tallyAdmin%reportInColl: keffImplicitClerk first, then the collision clerks
 template <int NL>
-__device__ __forceinline__ void report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, const Bins& B, DevStatus* st)
+__device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, const Bins& B, DevStatus* st)
 {
+    int nsc = 0;
     int mat = p.mat;
     if (mat != SCN_VOID_MAT) {
         double flux = p.w / p.track;
@@ -633,7 +639,7 @@ __device__ __forceinline__ void report_in_coll(const DevProblem& P, Pt<NL>& p, i
         p.k[K_IMP_PROD] = p.k[K_IMP_PROD] + nuFiss * flux;
         p.k[K_IMP_ABS] = p.k[K_IMP_ABS] + absXS * flux;
     }
-    if (P.n_clerks == 0) return;
+    if (P.n_clerks == 0) return 0;
     int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];
     for (int ci = c0; ci < c1; ++ci) {
         const scn_clerk& C = P.clerks[P.gclerk_idx[ci]];
@@ -656,7 +662,9 @@ __device__ __forceinline__ void report_in_coll(const DevProblem& P, Pt<NL>& p, i
             else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), C.resp[i]);
             bin_add(B, addr + i, resp * flux);
         }
+        nsc += C.width;
     }
+    return nsc;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -665,7 +673,7 @@ __device__ __forceinline__ void report_in_coll(const DevProblem& P, Pt<NL>& p, i
 
 // surfaceTracking: segments until a real collision or a leak; returns 1 if alive
 template <int NL>
-__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, int64_t& nev)
+__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
 {
     DistCache<NL> cache;
     cache.lvl = 0;
@@ -695,7 +703,7 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const B
             }
         }
         int event = move(P, p, dist, cache, st);
-        nev++;
+        cnt.seg++;
         p.t = p.t + dist / 1.0;
         if (event == EV_COLL) p.fate = FATE_NO;
         mat = p.mat;
@@ -711,14 +719,14 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const B
         }
         if (event == EV_COLL) {
             if (scn_rng_get(&p.rng) < sigmaT * invSigmaTrack) return 1;
-            report_in_coll(P, p, 1, B, st);
+            cnt.sc += report_in_coll(P, p, 1, B, st);
         }
     }
 }
 
 // deltaTracking: tentative flights until a real collision or a leak; returns 1 if alive
 template <int NL>
-__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, int64_t& nev)
+__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
 {
     double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
     p.track = maj;
@@ -736,7 +744,7 @@ __device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const B
         }
         double distance = -scn_log(scn_rng_get(&p.rng)) * majorant_inv;
         teleport(P, p, distance, st);
-        nev++;
+        cnt.seg++;
         p.t = p.t + distance / 1.0;
         int mat = p.mat;
         if (mat == SCN_OUTSIDE_MAT) {
@@ -745,7 +753,7 @@ __device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const B
             p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
             return 0;
         } else if (mat == SCN_VOID_MAT) {
-            report_in_coll(P, p, 1, B, st);
+            cnt.sc += report_in_coll(P, p, 1, B, st);
             continue;
         } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
             raise_error(st, SCN_ERR_MATERIAL, mat);
@@ -754,7 +762,7 @@ __device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const B
         }
         double sigmaT = track_mat_xs(P, mat, p.G, st);
         if (scn_rng_get(&p.rng) < sigmaT * majorant_inv) return 1;
-        report_in_coll(P, p, 1, B, st);
+        cnt.sc += report_in_coll(P, p, 1, B, st);
     }
 }
 
@@ -776,11 +784,11 @@ struct Sites {
 // collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
 template <int NL>
 __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, const Bins& B, const Sites& S,
-                              DevStatus* st, int64_t& nev)
+                              DevStatus* st, Cnt& cnt)
 {
     const int nG = P.nG;
     int mat = p.mat;
-    nev++;
+    cnt.col++;
     double denom = track_mat_xs(P, mat, p.G, st);
     double probAlpha = 0.0 / denom;
     if (scn_rng_get(&p.rng) < probAlpha) {
@@ -797,7 +805,7 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
     if (xs > 0.0) C++;
     xs = xs - x[XS_CAPTURE];
     if (xs > 0.0) C++;
-    report_in_coll(P, p, 0, B, st);
+    cnt.sc += report_in_coll(P, p, 0, B, st);
     double preColl_w = p.w;
     if (P.fissile[mat - 1]) {
         double wgt = p.w, w0 = p.w0;

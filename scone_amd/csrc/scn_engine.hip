@@ -135,11 +135,12 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             double* __restrict__ hist_k, int hist_cap, Counters* __restrict__ cnt, DevStatus* st)
 {
     extern __shared__ double lds_bins[];
-    __shared__ unsigned long long s_seg, s_col;
+    __shared__ unsigned long long s_seg, s_col, s_sc;
     const int n_in = first ? n_first : *n_in_ptr;
     if (threadIdx.x == 0) {
         s_seg = 0;
         s_col = 0;
+        s_sc = 0;
     }
     if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) lds_bins[i] = 0.0;
@@ -148,7 +149,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     Bins B;
     B.lds = use_lds ? lds_bins : nullptr;
     B.glob = bins;
-    int64_t nseg = 0, ncol = 0;
+    Cnt cn{0, 0, 0};
     const int lane = threadIdx.x & (WAVE - 1);
     for (int base = blockIdx.x * BLOCK; base < n_in; base += gridDim.x * BLOCK) {
         const int i = base + threadIdx.x;
@@ -190,12 +191,12 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             }
             alive = 1;
             for (int step = 0; step < max_steps; ++step) {
-                int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, nseg) : flight_st(P, p, B, st, nseg);
+                int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, cn) : flight_st(P, p, B, st, cn);
                 if (!ok) {
                     alive = 0;
                     break;
                 }
-                ok = collide(P, p, k_eff, B, S, st, ncol);
+                ok = collide(P, p, k_eff, B, S, st, cn);
                 if (!ok) {
                     alive = 0;
                     break;
@@ -220,9 +221,9 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             else raise_error(st, SCN_ERR_OVERFLOW, o);
         }
     }
-    ncol = ncol - 0;
-    atomicAdd(&s_seg, (unsigned long long)(nseg));
-    atomicAdd(&s_col, (unsigned long long)(ncol));
+    atomicAdd(&s_seg, (unsigned long long)(cn.seg));
+    atomicAdd(&s_col, (unsigned long long)(cn.col));
+    atomicAdd(&s_sc, (unsigned long long)(cn.sc));
     __syncthreads();
     if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) {
@@ -233,6 +234,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     if (threadIdx.x == 0) {
         atomicAdd(&cnt->segments, s_seg);
         atomicAdd(&cnt->collisions, s_col);
+        atomicAdd(&cnt->scores, s_sc);
     }
 }
 
@@ -460,8 +462,26 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct EigenState {
+    int open = 0;
+    scn_eigen_settings S{};
+    uint64_t master = 0;
+    double kA[2] = {0, 0}, kI[2] = {0, 0}, imp[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int nA = 0, nI = 0;
+    double dk[2] = {1.0, 1.0};
+    int cur = 0;
+    double keff_0 = 1.0, k_new = 1.0;
+    int phase = 0, c_in_phase = 0;
+    int n_this = 0;
+    int64_t n_hist = 0;
+    double transport_s = 0.0;
+    std::vector<double> k_history, cycle_time;
+    Counters c0{};
+};
+
 struct Engine {
     std::string err;
+    EigenState eig;
     hipStream_t stream = nullptr;
     int nl = 1;             // compiled nesting depth
     int transport = SCN_TRANSPORT_ST;
@@ -1101,6 +1121,7 @@ int scn_get_stats(void* engine, scn_stats* out)
     *out = E->stats;
     out->n_segments = (int64_t)c.segments;
     out->n_collisions = (int64_t)c.collisions;
+    out->n_scores = (int64_t)c.scores;
     return 0;
 }
 
@@ -1203,16 +1224,18 @@ int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double 
     return 0;
 }
 
-int scn_eigen_run(void* engine, const scn_eigen_settings* S, const scn_particle_state* initial_bank,
-                  scn_eigen_result* res)
+int scn_eigen_start(void* engine, const scn_eigen_settings* S, const scn_particle_state* initial_bank)
 {
     Engine* E = (Engine*)engine;
-    if (!E || !S || !initial_bank || !res) return SCN_ERR_ARG;
+    if (!E || !S || !initial_bank) return SCN_ERR_ARG;
     const int pop = S->pop;
-    if (pop > E->cap) {
+    if (pop <= 0 || pop > E->cap) {
         E->err = "pop larger than the engine capacity";
         return SCN_ERR_OVERFLOW;
     }
+    EigenState& X = E->eig;
+    X = EigenState{};
+    X.S = *S;
     if (S->events_per_launch > 0) E->k0 = S->events_per_launch;
     COPY_SYNC(E->bankA, initial_bank, sizeof(scn_particle_state) * pop, hipMemcpyHostToDevice);
     for (int w = 0; w < 2 && E->n_bins; ++w) {
@@ -1220,98 +1243,134 @@ int scn_eigen_run(void* engine, const scn_eigen_settings* S, const scn_particle_
         HIPCHK(hipMemsetAsync(E->csum2[w], 0, sizeof(double) * E->n_bins, E->stream));
     }
     if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->n_bins, E->stream));
-    uint64_t master = (uint64_t)S->seed;
-    master = scn_rng_stride(master, 0);
-    master = scn_rng_stride(master, pop);
-    double kA[2] = {0, 0}, kI[2] = {0, 0}, imp[4][2];
-    memset(imp, 0, sizeof imp);
-    int nA = 0, nI = 0;
-    double dk[2] = {1.0, 1.0};
-    int cur = 0;
-    double keff_0 = S->keff0;
-    int cyc = 0;
-    int n_this = pop;
-    res->n_histories = 0;
-    double transport_s = 0.0;
-    Counters c0{};
-    COPY_SYNC(&c0, E->counters, sizeof c0, hipMemcpyDeviceToHost);
-    for (int phase = 0; phase < 2; ++phase) {
-        int ncyc = phase == 0 ? S->n_inactive : S->n_active;
-        int score_bins = (phase == 1 || S->inactive_tally) && E->n_bins > 0;
-        double k_new = keff_0;
-        for (int c = 0; c < ncyc; ++c) {
-            auto t0 = std::chrono::steady_clock::now();
-            int ns = 0;
-            int rc = transport_cycle(E, n_this, master, k_new, &ns, score_bins);
-            if (rc) return rc;
-            res->n_histories += n_this;
-            master = scn_rng_stride(master, (int64_t)pop + 1);
-            double sums[6];
-            rc = cycle_sums(E, n_this, ns, sums);
-            if (rc) return rc;
-            double startW = sums[0], endW = sums[1];
-            double k_norm = dk[1 - cur];
-            if (phase == 0) {
-                double k_eff = endW / startW * k_norm;
-                kA[0] = kA[0] + k_eff;
-                kA[1] = kA[1] + k_eff * k_eff;
-                nA++;
-            } else {
-                double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
-                kI[0] = kI[0] + k_est;
-                kI[1] = kI[1] + k_est * k_est;
-                nI++;
-                for (int k = 0; k < 4; ++k) {
-                    imp[k][0] = imp[k][0] + sums[2 + k];
-                    imp[k][1] = imp[k][1] + sums[2 + k] * sums[2 + k];
-                }
-            }
-            if (score_bins) {
-                rc = close_bins(E, phase == 1 ? 0 : 1);
-                if (rc) return rc;
-            }
-            int nn = 0;
-            rc = norm_size_device(E, ns, pop, master, &nn);
-            if (rc) return rc;
-            master = scn_rng_stride(master, 1);
-            n_this = nn;
-            cur = 1 - cur;
-            double kr[2];
-            if (phase == 0) result2(kA[0], kA[1], nA, kr);
-            else result2(kI[0], kI[1], nI, kr);
-            k_new = kr[0];
-            dk[1 - cur] = k_new;
-            keff_0 = k_new;
-            if (res->k_history) res->k_history[cyc] = k_new;
-            cyc++;
-            HIPCHK(hipStreamSynchronize(E->stream));
-            auto t1 = std::chrono::steady_clock::now();
-            double dt = std::chrono::duration<double>(t1 - t0).count();
-            transport_s += dt;
-            E->stats.cycle_ms += dt * 1e3;
-            E->stats.n_cycles += 1;
+    // eigenPhysicsPackage%run: stride(getOffset) = 0 on one rank; generateInitialState: stride(totalPop)
+    X.master = scn_rng_stride((uint64_t)S->seed, 0);
+    X.master = scn_rng_stride(X.master, pop);
+    X.dk[0] = X.dk[1] = 1.0;
+    X.keff_0 = S->keff0;
+    X.k_new = X.keff_0;
+    X.n_this = pop;
+    X.open = 1;
+    COPY_SYNC(&X.c0, E->counters, sizeof(Counters), hipMemcpyDeviceToHost);
+    return 0;
+}
+
+int scn_eigen_step(void* engine, int32_t n_cycles, int32_t* done)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !E->eig.open) return SCN_ERR_ARG;
+    EigenState& X = E->eig;
+    const scn_eigen_settings& S = X.S;
+    const int pop = S.pop;
+    int ran = 0;
+    while (ran < n_cycles && X.phase < 2) {
+        int ncyc = X.phase == 0 ? S.n_inactive : S.n_active;
+        if (X.c_in_phase >= ncyc) {
+            X.phase += 1;
+            X.c_in_phase = 0;
+            X.k_new = X.keff_0;   // cycles(): k_new = keff_0 at entry
+            continue;
         }
+        int score_bins = (X.phase == 1 || S.inactive_tally) && E->n_bins > 0;
+        auto t0 = std::chrono::steady_clock::now();
+        int ns = 0;
+        int rc = transport_cycle(E, X.n_this, X.master, X.k_new, &ns, score_bins);
+        if (rc) return rc;
+        X.n_hist += X.n_this;
+        X.master = scn_rng_stride(X.master, (int64_t)pop + 1);
+        double sums[6];
+        rc = cycle_sums(E, X.n_this, ns, sums);
+        if (rc) return rc;
+        double startW = sums[0], endW = sums[1];
+        double k_norm = X.dk[1 - X.cur];
+        if (X.phase == 0) {   // keffAnalogClerk_class.f90:162-181
+            double k_eff = endW / startW * k_norm;
+            X.kA[0] = X.kA[0] + k_eff;
+            X.kA[1] = X.kA[1] + k_eff * k_eff;
+            X.nA++;
+        } else {              // keffImplicitClerk_class.f90:306-326
+            double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
+            X.kI[0] = X.kI[0] + k_est;
+            X.kI[1] = X.kI[1] + k_est * k_est;
+            X.nI++;
+            for (int k = 0; k < 4; ++k) {
+                X.imp[k][0] = X.imp[k][0] + sums[2 + k];
+                X.imp[k][1] = X.imp[k][1] + sums[2 + k] * sums[2 + k];
+            }
+        }
+        if (score_bins) {
+            rc = close_bins(E, X.phase == 1 ? 0 : 1);
+            if (rc) return rc;
+        }
+        int nn = 0;
+        rc = norm_size_device(E, ns, pop, X.master, &nn);
+        if (rc) return rc;
+        X.master = scn_rng_stride(X.master, 1);
+        X.n_this = nn;
+        X.cur = 1 - X.cur;
+        double kr[2];
+        if (X.phase == 0) result2(X.kA[0], X.kA[1], X.nA, kr);
+        else result2(X.kI[0], X.kI[1], X.nI, kr);
+        X.k_new = kr[0];
+        X.dk[1 - X.cur] = X.k_new;
+        X.keff_0 = X.k_new;
+        X.k_history.push_back(X.k_new);
+        HIPCHK(hipStreamSynchronize(E->stream));
+        auto t1 = std::chrono::steady_clock::now();
+        double dt = std::chrono::duration<double>(t1 - t0).count();
+        X.cycle_time.push_back(dt);
+        X.transport_s += dt;
+        E->stats.cycle_ms += dt * 1e3;
+        E->stats.n_cycles += 1;
+        X.c_in_phase += 1;
+        ++ran;
     }
-    res->transport_time = transport_s;
-    if (nA) result2(kA[0], kA[1], nA, res->k_analog);
-    if (nI) {
-        result2(kI[0], kI[1], nI, res->k_implicit);
-        for (int k = 0; k < 4; ++k) result2(imp[k][0], imp[k][1], nI, res->imp[k]);
+    if (done) *done = ran;
+    return 0;
+}
+
+int scn_eigen_finish(void* engine, scn_eigen_result* res)
+{
+    Engine* E = (Engine*)engine;
+    if (!E || !res || !E->eig.open) return SCN_ERR_ARG;
+    EigenState& X = E->eig;
+    res->n_histories = X.n_hist;
+    res->transport_time = X.transport_s;
+    if (X.nA) result2(X.kA[0], X.kA[1], X.nA, res->k_analog);
+    if (X.nI) {
+        result2(X.kI[0], X.kI[1], X.nI, res->k_implicit);
+        for (int k = 0; k < 4; ++k) result2(X.imp[k][0], X.imp[k][1], X.nI, res->imp[k]);
+    }
+    for (size_t i = 0; i < X.k_history.size(); ++i) {
+        if (res->k_history) res->k_history[i] = X.k_history[i];
+        if (res->cycle_time) res->cycle_time[i] = X.cycle_time[i];
     }
     Counters c1{};
     COPY_SYNC(&c1, E->counters, sizeof c1, hipMemcpyDeviceToHost);
-    res->n_collisions = (int64_t)(c1.collisions - c0.collisions);
-    res->n_events = (int64_t)((c1.collisions - c0.collisions) + (c1.segments - c0.segments));
+    res->n_collisions = (int64_t)(c1.collisions - X.c0.collisions);
+    res->n_events = (int64_t)((c1.collisions - X.c0.collisions) + (c1.segments - X.c0.segments));
     for (int w = 0; w < 2 && E->n_bins; ++w) {
         double* dst = w == 0 ? res->bins_active : res->bins_inactive;
-        int N = w == 0 ? nI : nA;
-        if (!dst || N == 0 || (w == 1 && !S->inactive_tally)) continue;
+        int N = w == 0 ? X.nI : X.nA;
+        if (!dst || N == 0 || (w == 1 && !X.S.inactive_tally)) continue;
         std::vector<double> cs(E->n_bins), cs2(E->n_bins);
         COPY_SYNC(cs.data(), E->csum[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
         COPY_SYNC(cs2.data(), E->csum2[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
         for (int64_t i = 0; i < E->n_bins; ++i) result2(cs[i], cs2[i], N, dst + 2 * i);
     }
+    X.open = 0;
     return 0;
+}
+
+int scn_eigen_run(void* engine, const scn_eigen_settings* S, const scn_particle_state* initial_bank,
+                  scn_eigen_result* res)
+{
+    int rc = scn_eigen_start(engine, S, initial_bank);
+    if (rc) return rc;
+    int32_t done = 0;
+    rc = scn_eigen_step(engine, S->n_inactive + S->n_active, &done);
+    if (rc) return rc;
+    return scn_eigen_finish(engine, res);
 }
 
 }  // extern "C"

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libscone_amd.so")
 
 # exported symbols of include/scone_gpu.h
 EXPORTS = ["scn_create", "scn_destroy", "scn_last_error", "scn_device_info", "scn_eigen_cycle",
-           "scn_norm_size_repr", "scn_eigen_run", "scn_fission_source", "scn_get_stats", "scn_reset_stats"]
+           "scn_norm_size_repr", "scn_eigen_run", "scn_eigen_start", "scn_eigen_step", "scn_eigen_finish",
+           "scn_fission_source", "scn_get_stats", "scn_reset_stats"]
 
 
 class Stats(C.Structure):
@@ -54,6 +55,9 @@ def load():
                                   C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.scn_norm_size_repr.argtypes = [C.c_void_p, PS, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int64]
     L.scn_eigen_run.argtypes = [C.c_void_p, C.POINTER(T.EigenSettings), PS, C.POINTER(T.EigenResult)]
+    L.scn_eigen_start.argtypes = [C.c_void_p, C.POINTER(T.EigenSettings), PS]
+    L.scn_eigen_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    L.scn_eigen_finish.argtypes = [C.c_void_p, C.POINTER(T.EigenResult)]
     L.scn_fission_source.argtypes = [C.c_void_p, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
     L.scn_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.scn_reset_stats.argtypes = [C.c_void_p]
@@ -140,7 +144,8 @@ class Engine:
                                               T.bank_ptr(bank)))
         return bank
 
-    def eigen_run(self, settings: dict, initial_bank: np.ndarray):
+    @staticmethod
+    def _settings(settings: dict):
         S = T.EigenSettings()
         S.pop = settings["pop"]
         S.n_inactive = settings["inactive"]
@@ -150,17 +155,47 @@ class Engine:
         S.keff0 = settings.get("keff0", 1.0)
         S.events_per_launch = settings.get("events_per_launch", 0)
         S.block_size = settings.get("block_size", 0)
+        return S
+
+    def _result(self, S):
         R = T.EigenResult()
         nb = int(self.comp.problem.n_bins)
-        kh = np.zeros(S.n_inactive + S.n_active)
-        ba = np.zeros(max(1, 2 * nb))
-        bi = np.zeros(max(1, 2 * nb))
-        R.k_history = kh.ctypes.data_as(C.POINTER(C.c_double))
-        R.bins_active = ba.ctypes.data_as(C.POINTER(C.c_double))
-        R.bins_inactive = bi.ctypes.data_as(C.POINTER(C.c_double))
+        n = S.n_inactive + S.n_active
+        arrs = dict(kh=np.zeros(max(1, n)), ct=np.zeros(max(1, n)), ba=np.zeros(max(1, 2 * nb)),
+                    bi=np.zeros(max(1, 2 * nb)))
+        R.k_history = arrs["kh"].ctypes.data_as(C.POINTER(C.c_double))
+        R.cycle_time = arrs["ct"].ctypes.data_as(C.POINTER(C.c_double))
+        R.bins_active = arrs["ba"].ctypes.data_as(C.POINTER(C.c_double))
+        R.bins_inactive = arrs["bi"].ctypes.data_as(C.POINTER(C.c_double))
+        return R, arrs
+
+    def eigen_run(self, settings: dict, initial_bank: np.ndarray):
+        S = self._settings(settings)
+        R, a = self._result(S)
+        nb = int(self.comp.problem.n_bins)
         bank = np.ascontiguousarray(initial_bank)
         self._check(self.L.scn_eigen_run(self.h, C.byref(S), T.bank_ptr(bank), C.byref(R)))
-        return R, kh, ba[: 2 * nb].reshape(-1, 2), bi[: 2 * nb].reshape(-1, 2)
+        n = S.n_inactive + S.n_active
+        self.last_cycle_time = a["ct"][:n].copy()
+        return R, a["kh"][:n], a["ba"][: 2 * nb].reshape(-1, 2), a["bi"][: 2 * nb].reshape(-1, 2)
+
+    def eigen_start(self, settings: dict, initial_bank: np.ndarray):
+        self._S = self._settings(settings)
+        bank = np.ascontiguousarray(initial_bank)
+        self._check(self.L.scn_eigen_start(self.h, C.byref(self._S), T.bank_ptr(bank)))
+
+    def eigen_step(self, n_cycles: int) -> int:
+        done = C.c_int32()
+        self._check(self.L.scn_eigen_step(self.h, n_cycles, C.byref(done)))
+        return done.value
+
+    def eigen_finish(self):
+        S = self._S
+        R, a = self._result(S)
+        nb = int(self.comp.problem.n_bins)
+        self._check(self.L.scn_eigen_finish(self.h, C.byref(R)))
+        n = S.n_inactive + S.n_active
+        return R, a["kh"][:n], a["ct"][:n], a["ba"][: 2 * nb].reshape(-1, 2), a["bi"][: 2 * nb].reshape(-1, 2)
 
     def stats(self) -> dict:
         s = Stats()

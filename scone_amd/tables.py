@@ -107,7 +107,8 @@ class EigenSettings(C.Structure):
 
 class EigenResult(C.Structure):
     _fields_ = [("k_analog", C.c_double * 2), ("k_implicit", C.c_double * 2), ("imp", (C.c_double * 2) * 4),
-                ("k_history", C.POINTER(C.c_double)), ("bins_active", C.POINTER(C.c_double)),
+                ("k_history", C.POINTER(C.c_double)), ("cycle_time", C.POINTER(C.c_double)),
+                ("bins_active", C.POINTER(C.c_double)),
                 ("bins_inactive", C.POINTER(C.c_double)), ("transport_time", C.c_double),
                 ("n_collisions", C.c_int64), ("n_events", C.c_int64), ("n_histories", C.c_int64)]
 
