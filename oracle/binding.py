@@ -86,6 +86,11 @@ def lib():
         L.orc_norm_size_repr.argtypes = [PS, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int64]
         L.orc_fission_source.argtypes = [P, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
         L.orc_eigen_run.argtypes = [P, C.POINTER(T.EigenSettings), PS, C.POINTER(T.EigenResult), C.c_int32]
+        dp = C.POINTER(C.c_double)
+        L.orc_score_result.argtypes = [C.c_double, C.c_double, C.c_int, dp]
+        L.orc_close_cycle.argtypes = [dp, dp, dp, C.c_int64, C.c_double]
+        L.orc_keff_implicit.restype = C.c_double
+        L.orc_keff_implicit.argtypes = [dp]
         _lib = L
     return _lib
 
@@ -259,6 +264,22 @@ def eigen_run(comp, settings: dict, initial_bank, threads=1):
     R.bins_inactive = bi.ctypes.data_as(C.POINTER(C.c_double))
     check(lib().orc_eigen_run(C.byref(comp.problem), C.byref(S), T.bank_ptr(initial_bank), C.byref(R), threads))
     return R, kh, ba[: 2 * nb].reshape(-1, 2), bi[: 2 * nb].reshape(-1, 2)
+
+
+def score_result(csum, csum2, N):
+    o = (C.c_double * 2)()
+    lib().orc_score_result(csum, csum2, N, o)
+    return o[0], o[1]
+
+
+def close_cycle(bins, csum, csum2, norm):
+    dp = C.POINTER(C.c_double)
+    lib().orc_close_cycle(bins.ctypes.data_as(dp), csum.ctypes.data_as(dp), csum2.ctypes.data_as(dp), len(bins), norm)
+
+
+def keff_implicit(scores):
+    a = np.ascontiguousarray(scores, dtype=np.float64)
+    return lib().orc_keff_implicit(a.ctypes.data_as(C.POINTER(C.c_double)))
 
 
 def _s64(x: int) -> int:

@@ -1171,6 +1171,7 @@ int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const
 /* ======================================================================================
  * Power iteration: eigenPhysicsPackage_class.f90:135-343 with scoreMemory_class.f90 closes
  * ====================================================================================== */
+void orc_score_result(double csum, double csum2, int N, double out[2]);
 static void o_result(double csum, double csum2, int N, double out[2])
 {
     /* scoreMemory_class.f90:537-573 getResult_withSTD */
@@ -1181,6 +1182,22 @@ static void o_result(double csum, double csum2, int N, double out[2])
     out[0] = mean;
     out[1] = sqrt(std);
 }
+void orc_score_result(double csum, double csum2, int N, double out[2]) { o_result(csum, csum2, N, out); }
+
+/* scoreMemory_class.f90:309-342 closeCycle (batchSize 1): res = BIN*norm; CSUM += res; CSUM2 += res^2 */
+void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double r = bins[i] * norm;
+        bins[i] = 0.0;
+        csum[i] = csum[i] + r;
+        csum2[i] = csum2[i] + r * r;
+    }
+}
+
+/* keffImplicitClerk_class.f90:306-326 closeCycle estimate; scores in IMP_PROD, SCATTER_PROD,
+ * IMP_ABS, ANA_LEAK order */
+double orc_keff_implicit(const double s[4]) { return s[0] / (s[2] + s[3] - s[1]); }
 
 int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                   const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads)

@@ -84,6 +84,9 @@ int orc_norm_size_repr(scn_particle_state* bank, int32_t* n, int32_t cap, int32_
                        int64_t rng_state);
 int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const double bottom[3],
                        const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank);
+void orc_score_result(double csum, double csum2, int N, double out[2]);
+void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm);
+double orc_keff_implicit(const double s[4]);
 int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                   const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads);
 

@@ -31,3 +31,21 @@ def rel_close(a, b, rtol=1e-12):
     diff = np.abs(a - b)
     ok = diff <= rtol * scale
     return bool(ok.all()), float(np.max(np.where(scale > 0, diff / np.where(scale > 0, scale, 1), 0.0)))
+
+
+def dummy_mg(nG=1):
+    return {"numberOfGroups": nG, "capture": [0.5] * nG, "scatteringMultiplicity": [1.0] * (nG * nG),
+            "P0": [0.5 / nG] * (nG * nG)}
+
+
+def geometry_problem(universes, surfaces, boundary=(0, 0, 0, 0, 0, 0), materials=("m1",), graph="shrunk"):
+    """Compile a geometry-only SCONE input (1-group dummy materials) for geometry unit tests."""
+    inp = {
+        "transportOperator": {"type": "transportOperatorST"},
+        "geometry": {"type": "geometryStd", "boundary": list(boundary), "graph": {"type": graph},
+                     "surfaces": surfaces, "cells": {}, "universes": universes},
+        "nuclearData": {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
+                        "materials": {m: {"xsFile": m} for m in materials}},
+        "activeTally": {},
+    }
+    return T.compile_problem(inp, {m: dummy_mg() for m in materials})

@@ -1,0 +1,92 @@
+"""Host-side checks without a GPU: the C ABI library loads and exports every entry point the
+header declares; the ctypes mirrors match the header's struct layouts; the dictionary reader and
+the table compiler accept SCONE's input format."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from scone_amd import configs, dictio, engine, tables as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "scone_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(scn_\w+)\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_entry_point():
+    L = engine.load()
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(engine.EXPORTS) == names
+
+
+def test_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "scone_gpu.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(scn_particle_state), sizeof(scn_box),
+         sizeof(scn_universe), sizeof(scn_mg_data), sizeof(scn_map1d), sizeof(scn_clerk), sizeof(scn_problem),
+         sizeof(scn_eigen_settings), sizeof(scn_eigen_result), sizeof(scn_stats));
+  printf("%zu %zu %zu\n", offsetof(scn_particle_state, time), offsetof(scn_particle_state, broodID),
+         offsetof(scn_problem, mg));
+  return 0; }
+'''
+    tmp = os.path.join(ROOT, "_work", "layout_probe")
+    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    with open(tmp + ".c", "w") as f:
+        f.write(src)
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), tmp + ".c", "-o", tmp], check=True)
+    out = subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split("\n")
+    sizes = [int(x) for x in out[0].split()]
+    py = [C.sizeof(T.ParticleState), C.sizeof(T.Box), C.sizeof(T.Universe), C.sizeof(T.MGData),
+          C.sizeof(T.Map1D), C.sizeof(T.Clerk), C.sizeof(T.Problem), C.sizeof(T.EigenSettings),
+          C.sizeof(T.EigenResult), C.sizeof(engine.Stats)]
+    assert sizes == py
+    offs = [int(x) for x in out[1].split()]
+    assert offs == [T.ParticleState.time.offset, T.ParticleState.broodID.offset, T.Problem.mg.offset]
+
+
+def test_engine_fails_loudly_without_gpu():
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is visible here")
+    with pytest.raises(engine.EngineError):
+        engine.device_info()
+
+
+def test_dictionary_reader_on_reference_format():
+    text = """
+    type eigenPhysicsPackage;   // comment
+    pop 1000; active 5; inactive 2;
+    geometry { type geometryStd; boundary (1 1 1 1 1 1);
+      surfaces { b { id 1; type box; origin (0.0 0.0 0.0); halfwidth (5.0 5.0 5.0); } }
+      universes { root { id 1; type rootUniverse; border 1; fill fuel; } } }
+    ! another comment
+    """
+    d = dictio.parse(text)
+    assert d["pop"] == 1000 and d["geometry"]["boundary"] == [1] * 6
+    assert d["geometry"]["surfaces"]["b"]["halfwidth"] == [5.0, 5.0, 5.0]
+    mg = dictio.read(os.path.join(ROOT, "tests", "golden", "mgMat2"))
+    assert mg["numberOfGroups"] == 4 and len(mg["P0"]) == 16 and mg["kappa"] == [202, 193, 180, 200]
+    assert dictio.parse(dictio.dumps(d)) == d
+
+
+def test_configs_compile():
+    for name, fn in configs.CONFIGS.items():
+        inp, mg = fn(pop=100)
+        comp = T.compile_problem(inp, mg)
+        assert comp.problem.n_universes >= 1
+    inp, mg = configs.pin_lattice_7g(pop=100)
+    comp = T.compile_problem(inp, mg)
+    assert comp.problem.n_bins == 7 * 2 * 289
+    assert comp.bounds == pytest.approx([-10.71] * 3 + [10.71] * 3)

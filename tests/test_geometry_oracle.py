@@ -1,0 +1,282 @@
+"""Geometry of the oracle pinned to the reference's own unit tests (CPU).
+
+Known answers transcribed from Geometry/Surfaces/Tests/{box,cylinder}_test.f90,
+Geometry/Universes/Tests/{pin,lat}Universe_test.f90 and Geometry/Tests/geometryStd_iTest.f90
+(tolerances as in those tests).  The integration geometry IntegrationTestFiles/Geometry/test_lat
+uses a cellUniverse square pin, which the engine does not implement; it is replaced by a
+pinUniverse of radius 0.5, for which every asserted point and distance is unchanged (the crossing
+straight down from the cell centre is at 0.5 either way, and the other probes are in water).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import binding as O
+from helpers import geometry_problem
+from scone_amd import tables as T
+
+SQRT2 = math.sqrt(2.0)
+INF = 2.0 ** 63
+SURF_TOL = 1e-12
+
+
+def unit(v):
+    v = np.asarray(v, dtype=float)
+    return list(v / np.linalg.norm(v))
+
+
+# ---- box_test.f90 ---------------------------------------------------------------------------
+def make_box(bc=(0, 0, 0, 0, 0, 0)):
+    return O.box(3, [0, 1, 2], [1.0, 2.0, 1.0], [1.0, 2.0, 3.0], bc)
+
+
+def test_box_distance():
+    b = make_box()
+    TOL = 1e-7
+    r = [-2.0, 0.001, 1.0]
+    assert O.box_distance(b, r, [1, 0, 0]) == pytest.approx(2.0, rel=TOL)
+    assert O.box_distance(b, r, [-1, 0, 0]) == INF
+    assert O.box_distance(b, r, unit([1, -0.002, 1])) == INF
+    assert O.box_distance(b, r, unit([1, 0, 1])) == pytest.approx(2 * SQRT2, rel=TOL)
+    assert O.box_distance(b, [-2.7, -0.3, 1.0 / 3.0], unit([2.7, 0.3, 3.0 + 2.0 / 3.0])) == INF
+    assert O.box_distance(b, [-2.0, 0.0, 1.0], [1, 0, 0]) == INF
+    assert O.box_distance(b, [1, 2, -2.0], [0, 0, 1]) == pytest.approx(6.0, rel=TOL)
+    assert O.box_distance(b, [1, 2, -2.0], [0, 0, -1]) == INF
+    assert O.box_distance(b, [1, 2, -2.0 - 0.5 * SURF_TOL], [0, 0, 1]) == pytest.approx(6.0 + 0.5 * SURF_TOL, rel=TOL)
+    u = unit([0, 1, 1])
+    assert O.box_distance(b, [1, 2, -1.0], u) == pytest.approx(2 * SQRT2, rel=TOL)
+    assert O.box_distance(b, [1, 2, -1.0], [-x for x in u]) == pytest.approx(SQRT2, rel=TOL)
+
+
+def test_box_halfspace():
+    b = make_box()
+    assert not O.box_halfspace(b, [0.5, 1.0, 3.6], [0, 0, 1])
+    assert not O.box_halfspace(b, [1.5, 1.0, 0.5], [0, 0, 1])
+    assert O.box_halfspace(b, [-0.5, 0.0, 3.6], [0, 0, 1])
+    assert O.box_halfspace(b, [2.0, 5.0, 0.5], [0, 0, 1])
+    r = np.array([1.5, 4.0, 0.5])
+    u = np.array([0.0, 1.0, 0.0])
+    assert O.box_halfspace(b, r, u)
+    assert O.box_halfspace(b, r - 0.5 * SURF_TOL * u, u)
+    assert not O.box_halfspace(b, r - 1.00001 * SURF_TOL * u, u)
+    u2 = [1.0, 0.0, 0.0]
+    assert O.box_halfspace(b, r + 0.5 * SURF_TOL * u, u2)
+    assert not O.box_halfspace(b, r - 0.5 * SURF_TOL * u, u2)
+
+
+def test_box_bc():
+    b = make_box(bc=(0, 1, 2, 2, 1, 1))
+    TOL = 1e-7
+    r, u = O.box_explicit_bc(b, [0.0, 1.0, -1.0], [1, 0, 0])
+    assert r == pytest.approx([0.0, 1.0, -1.0], abs=TOL) and u == pytest.approx([1, 0, 0], abs=TOL)
+    r, u = O.box_explicit_bc(b, [1.0, 1.0, -2.0], [0, 0, -1])
+    assert r == pytest.approx([1.0, 1.0, -2.0], abs=TOL) and u == pytest.approx([0, 0, 1], abs=TOL)
+    r, u = O.box_explicit_bc(b, [1.0, 0.0, -1.0], [0, -1, 0])
+    assert r == pytest.approx([1.0, 4.0, -1.0], abs=TOL) and u == pytest.approx([0, -1, 0], abs=TOL)
+    r, u = O.box_explicit_bc(b, [2.0, 0.0, 4.0], unit([1, -1, 1]))
+    assert r == pytest.approx([2.0, 4.0, 4.0], abs=TOL) and u == pytest.approx(unit([-1, -1, -1]), abs=TOL)
+    r, u = O.box_transform_bc(b, [4.5, -10.0, 4.0], unit([1, 1, 1]))
+    assert r == pytest.approx([-0.5, 2.0, 4.0], abs=TOL) and u == pytest.approx(unit([-1, 1, -1]), abs=TOL)
+    r, u = O.box_transform_bc(b, [2.0, 0.0, 4.0], unit([1, -1, 1]))
+    assert r == pytest.approx([2.0, 4.0, 4.0], abs=TOL) and u == pytest.approx(unit([-1, -1, -1]), abs=TOL)
+
+
+def test_square_cylinder_is_a_two_axis_box():
+    # zSquareCylinder: only x and y bound it (squareCylinder_class.f90)
+    b = O.box(2, [0, 1], [0.0, 0.0], [1.26, 1.26])
+    assert O.box_distance(b, [0.0, 0.0, 5.0], [0, 0, 1]) == INF
+    assert O.box_distance(b, [0.0, 0.0, 5.0], [1, 0, 0]) == pytest.approx(1.26, rel=1e-12)
+    assert not O.box_halfspace(b, [1.0, -1.0, 1e9], [1, 0, 0])
+
+
+# ---- cylinder_test.f90 (z cylinder, origin (2,2,2), radius 2: coordinates shifted by the origin)
+def cyl(r, u):
+    return O.cyl_distance(2.0, [r[0] - 2.0, r[1] - 2.0, r[2] - 2.0], u)
+
+
+def test_cylinder_distance():
+    TOL = 1e-7
+    u = unit([1, 0, 1])     # u(p1)=1, u(a)=1
+    assert cyl([-1.0, 2.0, 2.0], u) == pytest.approx(SQRT2, rel=TOL)
+    u = unit([math.sqrt(3.0) / 2.0, 0.5, 1.0])
+    assert cyl([-1.0, 2.0, 2.0], u) == pytest.approx(1.275200556 * SQRT2, rel=TOL)
+    assert cyl([-1.0, 2.0, 2.0], unit([1.0, 0.0, 1e20])) == INF
+    u = unit([1, 0, 1])
+    assert cyl([0.0, 2.0, 2.0], u) == pytest.approx(4 * SQRT2, rel=TOL)
+    assert cyl([0.0, 2.0, 2.0], [-x for x in u]) == INF
+    assert cyl([0.0, 2.0, 2.0], [-x for x in unit([0, 1, 1])]) == INF
+    assert cyl([0.0 - 0.5 * SURF_TOL, 2.0, 2.0], u) == pytest.approx((4 + 0.5 * SURF_TOL) * SQRT2, rel=TOL)
+    assert cyl([1.0, 2.0, 2.0], u) == pytest.approx(3 * SQRT2, rel=TOL)
+    assert cyl([1.0, 2.0, 2.0], [-x for x in u]) == pytest.approx(SQRT2, rel=TOL)
+
+
+# ---- pinUniverse_test.f90 ------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def pin_geom():
+    unis = {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "u<7>"},
+            "pin": {"id": 7, "type": "pinUniverse", "radii": [2.5, 1.5, 0.0], "fills": ["m2", "m1", "void"]}}
+    surfs = {"b": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [100.0, 100.0, 1000.0]}}
+    return geometry_problem(unis, surfs, materials=("m1", "m2"))
+
+
+def test_pin_fill_order(pin_geom):
+    # radii sorted ascending with the zero radius last (pinUniverse_class.f90:110-141):
+    # reference fills (u<7> u<14> void) -> [-14, -7, void]; here m2 -> 1.5 annulus first
+    fill, _ = pin_geom.graph
+    pin_fills = fill[2:5]
+    assert pin_fills == [1, 2, T.VOID_MAT]
+
+
+def test_pin_findcell_distance(pin_geom):
+    u_pin = pin_geom.uni_idx[7]
+    assert O.uni_findcell(pin_geom, u_pin, [0.0, 1.0, 0.0], [0, 0, 1]) == 1
+    assert O.uni_findcell(pin_geom, u_pin, [2.3, 0.0, -980.0], [0, 0, 1]) == 2
+    assert O.uni_findcell(pin_geom, u_pin, [2.6, 0.0, -980.0], [0, 0, 1]) == 3
+    d, s = O.uni_distance(pin_geom, u_pin, 1, [1.0, 0, 0], [1, 0, 0])
+    assert d == pytest.approx(0.5, rel=1e-7) and s == -2          # MOVING_OUT
+    d, s = O.uni_distance(pin_geom, u_pin, 3, [2.0, 1.6, 0], [1, 0, 0])
+    assert d == INF
+    d, s = O.uni_distance(pin_geom, u_pin, 2, [0.0, 1.6, 0], [0, -1, 0])
+    assert d == pytest.approx(0.1, rel=1e-7) and s == -1          # MOVING_IN
+    # edge case: just inside the 1.5 surface, grazing outwards
+    eps = 0.5 * SURF_TOL
+    u = unit([1.0, -0.00001, 0.0])
+    assert O.uni_findcell(pin_geom, u_pin, [0.0, 1.5 - eps, 0.0], u) == 1
+    d, s = O.uni_distance(pin_geom, u_pin, 1, [0.0, 1.5 - eps, 0.0], u)
+    assert d == pytest.approx(0.0, abs=1e-3) and s == -2
+
+
+# ---- latUniverse_test.f90 --------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def lat_geom():
+    unis = {"root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<100>"},
+            "lat1": {"id": 100, "type": "latUniverse", "pitch": [1.0, 2.0, 3.0], "shape": [3, 2, 2],
+                     "padMat": "void", "map": [3, 4, 5, 7, 4, 8, 1, 2, 3, 4, 5, 6],
+                     "offsetMap": [1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1]},
+            "lat2": {"id": 200, "type": "latUniverse", "pitch": [1.0, 2.0, 0.0], "shape": [2, 1, 0],
+                     "padMat": "u<1>", "map": [1, 2]}}
+    for i in range(1, 9):
+        unis[f"p{i}"] = {"id": i, "type": "pinUniverse", "radii": [0.0], "fills": ["m1"]}
+    surfs = {"b": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [50.0, 50.0, 50.0]}}
+    comp = geometry_problem(unis, surfs)
+    inv = {v: k for k, v in comp.uni_idx.items()}
+    return comp, inv
+
+
+def lat_fill_ids(comp, inv, uid, n):
+    """fill vector of a lattice as -universe ID / material index, from the compiled graph."""
+    fill, gid = comp.graph
+    U = comp.problem.universes[comp.uni_idx[uid] - 1]
+    # the lattice's fills start at the rootID recorded by whichever graph entry references it
+    for f, i in zip(fill, gid):
+        if f == -comp.uni_idx[uid]:
+            start = i
+            break
+    else:
+        start = 1
+    out = []
+    for x in fill[start - 1:start - 1 + n]:
+        out.append(-inv[-x] if x < 0 else x)
+    return out
+
+
+def test_lattice_fill_vector(lat_geom):
+    comp, inv = lat_geom
+    # latUniverse_test.f90 setUp: ref = [-4,-5,-6,-1,-2,-3,-7,-4,-8,-3,-4,-5, pad]
+    assert lat_fill_ids(comp, inv, 100, 13) == [-4, -5, -6, -1, -2, -3, -7, -4, -8, -3, -4, -5, T.VOID_MAT]
+
+
+def test_lattice_enter(lat_geom):
+    comp, _ = lat_geom
+    u1 = comp.uni_idx[100]
+    u2 = comp.uni_idx[200]
+    assert O.uni_findcell(comp, u1, [1.0, 1.0, 0.5], [0, 0, 1]) == 12
+    assert O.uni_findcell(comp, u1, [1.6, 0.5, 0.5], [0, 0, 1]) == 13
+    assert O.uni_findcell(comp, u1, [-0.5, 0.0, 0.0], unit([-1, 1, -1])) == 4
+    assert O.uni_findcell(comp, u2, [0.5, 0.5, 13.5], [0, 0, 1]) == 2
+    assert O.uni_findcell(comp, u2, [1.6, 0.5, 0.5], [0, 0, 1]) == 3
+    assert O.uni_findcell(comp, u2, [0.0, 0.0, 0.0], unit([-1, 1, -1])) == 1
+
+
+def test_lattice_distance(lat_geom):
+    comp, _ = lat_geom
+    u1 = comp.uni_idx[100]
+    u2 = comp.uni_idx[200]
+    TOL = 1e-7
+    d, s = O.uni_distance(comp, u1, 11, [0.0, 0.1, 0.5], [0.0, -0.0, 1.0])
+    assert d == pytest.approx(2.5, rel=TOL) and s == -6
+    d, s = O.uni_distance(comp, u1, 13, [-4.0, 0.1, 0.5], unit([1, 1, 0]))
+    assert d == INF and s == -7
+    eps = 0.5 * SURF_TOL
+    d, s = O.uni_distance(comp, u1, 4, [-1.0, 0.0 - eps, -0.5], unit([1, 1, 0]))
+    assert d == pytest.approx(SQRT2 * 0.5, rel=TOL) and s == -2
+    d, s = O.uni_distance(comp, u1, 4, [-0.5 + eps, 0.0 + eps, -0.5], unit([1, 1, 0]))
+    assert d == pytest.approx(0.0, abs=TOL) and s == -2
+    d, s = O.uni_distance(comp, u2, 2, [0.5, 0.6, 0.5], [0, 0, 1])
+    assert d == INF
+    d, s = O.uni_distance(comp, u2, 2, [0.5, 0.6, 0.5], unit([0, 0.01, 1]))
+    assert d == pytest.approx(math.sqrt(40.0 ** 2 + 0.4 ** 2), rel=TOL) and s == -4
+    d, s = O.uni_distance(comp, u2, 3, [-1.5, 0.6, 0.5], unit([1, 0, 1]))
+    assert d == pytest.approx(0.5 * SQRT2, rel=TOL) and s == -7
+
+
+def test_lattice_cross(lat_geom):
+    # universe%cross of a lattice re-runs findCell at the crossing point (latUniverse_class.f90:380-387)
+    comp, _ = lat_geom
+    u1 = comp.uni_idx[100]
+    u2 = comp.uni_idx[200]
+    assert O.uni_findcell(comp, u1, [-1.0, 0.0, -0.5], unit([-1, 1, -1])) == 4
+    assert O.uni_findcell(comp, u1, [1.0, 2.0, -0.5], unit([1, -1, -1])) == 6
+    assert O.uni_findcell(comp, u1, [1.5, 1.0, -1.0], [1, 0, 0]) == 13
+    assert O.uni_findcell(comp, u2, [0.0, 0.0, 16.5], unit([1, 1, -1])) == 2
+    assert O.uni_findcell(comp, u2, [-1.0, -0.5, -78.5], unit([1, 1, 0])) == 1
+
+
+# ---- geometryStd_iTest.f90 on IntegrationTestFiles/Geometry/test_lat --------------------------
+@pytest.fixture(scope="module")
+def test_lat():
+    unis = {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "u<10>"},
+            "pin31": {"id": 31, "type": "pinUniverse", "radii": [0.54, 0.0], "fills": ["mox43", "water"]},
+            "sqPin": {"id": 32, "type": "pinUniverse", "radii": [0.5, 0.0], "fills": ["uox", "water"]},
+            "lat10": {"id": 10, "type": "latUniverse", "shape": [2, 2, 0], "pitch": [1.26, 1.26, 0.0],
+                      "padMat": "water", "map": [31, 32, 32, 31]}}
+    surfs = {"squareBound": {"id": 1, "type": "zSquareCylinder", "origin": [0.0, 0.0, 0.0],
+                             "halfwidth": [1.26, 1.26, 0.0]}}
+    comp = geometry_problem(unis, surfs, boundary=(1, 1, 2, 2, 0, 0), materials=("water", "mox43", "uox"))
+    return comp
+
+
+def test_geometry_std_integration(test_lat):
+    comp = test_lat
+    water, mox, uox = 1, 2, 3
+    TOL = 1e-7
+    assert O.place(comp, [0, 0, 0]).matIdx == water
+    assert O.place(comp, [0.63, -0.09, 0.0], [0, -1, 0]).matIdx == mox
+    o = O.place(comp, [0.1, 0.1, 0.0], [0, 0, 1])
+    assert list(o.r[0]) == pytest.approx([0.1, 0.1, 0.0], abs=TOL)
+    assert list(o.r[1]) == pytest.approx([0.1, 0.1, 0.0], abs=TOL)
+    assert list(o.r[2]) == pytest.approx([0.1 - 0.63, 0.1 - 0.63, 0.0], abs=TOL)
+    assert comp.bounds == pytest.approx([-1.26, -1.26, 0.0, 1.26, 1.26, 0.0], abs=TOL)
+    # teleport through a reflective x face and a periodic y face
+    o, u = O.teleport(comp, [0, 0, 0], unit([-1, -2, 0]), 3.0)
+    assert list(o.r[0]) == pytest.approx([-1.1783592, -0.1632816, 0.0], abs=TOL)
+    assert u == pytest.approx(unit([1, -2, 0]), abs=TOL)
+    assert o.matIdx == water
+    # move_noCache sequence
+    o, u, d, ev = O.move(comp, [-0.63, -0.63, 0.0], [0, -1, 0], 1.0)
+    assert list(o.r[0]) == pytest.approx([-0.63, -1.13, 0.0], abs=TOL) and ev == 3 and o.matIdx == water
+    assert d == pytest.approx(0.5, abs=TOL)
+    o, u, d, ev = O.move(comp, list(o.r[0]), [0, -1, 0], 1.0)
+    assert list(o.r[0]) == pytest.approx([-0.63, 1.26, 0.0], abs=TOL) and ev == 2 and o.matIdx == water
+    assert d == pytest.approx(0.13, abs=TOL)
+    o, u, d, ev = O.move(comp, list(o.r[0]), [0, -1, 0], 0.08)
+    assert list(o.r[0]) == pytest.approx([-0.63, 1.18, 0.0], abs=TOL) and ev == 1 and o.matIdx == water
+    assert d == pytest.approx(0.08, abs=TOL)
+
+
+def test_unique_ids_shrunk_graph(test_lat):
+    # geomGraph_class.f90 setUniqueIDs: material cells numbered in layout order (root, lat, pins)
+    fill, gid = test_lat.graph
+    mats = [(f, i) for f, i in zip(fill, gid) if f > 0]
+    assert [i for _, i in mats] == list(range(1, len(mats) + 1))
