@@ -90,3 +90,40 @@ def test_configs_compile():
     comp = T.compile_problem(inp, mg)
     assert comp.problem.n_bins == 7 * 2 * 289
     assert comp.bounds == pytest.approx([-10.71] * 3 + [10.71] * 3)
+
+
+FORTRAN_DRIVER = """
+program drive
+  use iso_c_binding
+  use scone_gpu_mod
+  implicit none
+  type(c_ptr) :: eng
+  integer(c_int) :: rc
+  type(scn_particle_state) :: p
+  rc = scn_create(c_null_ptr, 10_c_int32_t, eng)
+  print '(I0,1X,I0)', rc, c_sizeof(p)
+end program drive
+"""
+
+
+def test_fortran_binding_links_and_calls_the_library():
+    """The iso_c_binding module compiles with the image's Fortran compiler (amdflang) and calls
+    into libscone_amd.so: scn_create with no problem tables must return an error code."""
+    import shutil
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler in this image")
+    work = os.path.join(ROOT, "_work", "fortran_probe")
+    os.makedirs(work, exist_ok=True)
+    with open(os.path.join(work, "drive.f90"), "w") as f:
+        f.write(FORTRAN_DRIVER)
+    lib_dir = os.path.join(ROOT, "scone_amd")
+    subprocess.run([fc, "-c", os.path.join(ROOT, "include", "scone_gpu_mod.f90"), "-J", work, "-o",
+                    os.path.join(work, "mod.o")], check=True, cwd=work)
+    subprocess.run([fc, "-I", work, os.path.join(work, "drive.f90"), os.path.join(work, "mod.o"),
+                    "-L", lib_dir, "-lscone_amd", f"-Wl,-rpath,{lib_dir}", "-o", os.path.join(work, "drive")],
+                   check=True, cwd=work)
+    out = subprocess.run([os.path.join(work, "drive")], capture_output=True, text=True, check=True).stdout.split()
+    rc, size = int(out[0]), int(out[1])
+    assert rc != 0                       # SCN_ERR_NO_DEVICE here, SCN_ERR_ARG on a GPU box
+    assert size == C.sizeof(T.ParticleState)
