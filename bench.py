@@ -1,11 +1,20 @@
 #!/usr/bin/env python3
-"""Throughput of the plain (one problem at a time) GPU engine on the 17x17 pin-lattice eigenvalue
-problem: 7 groups, 1e6 histories per cycle, synthetic multigroup data from seed 2, FP64.
+"""Histories/s and solve latency of the MI355X engine, batched vs plain, plus the serving loop.
 
-A step is one power-iteration cycle: transport of the whole bank (event kernels), fission-site
-sort, canonical k-eff sums, tally close and reproducible population control, all on the device.
-The W warmup steps are inactive cycles; the K timed steps are active cycles, which also score the
-7 x 2 x 289 per-pin, per-group flux / fission-rate collision clerks.  Prints one JSON line.
+Default workload (the batched fixed-source benchmark): 512 independent fixed-source problems
+(1D-3D boxes with 1-8 material regions, 2-70 groups, 1e3-1e6 histories in 10 batches, point or
+volume sources, a 1e3-1e5-bin mesh flux tally each; synthetic seeded data), FP64, delta tracking.
+A step is one cycle: one batch of every resident problem.  W warmup cycles, then K timed cycles
+(W + K = batches per problem).
+  batched  all 512 problems resident in one batch engine, cycles replayed from a hipGraph, launch
+           shape from the autotuner
+  plain    the same engine holding one problem at a time (each problem's batches run alone)
+Both run the identical histories (checked on a sample of problems: score results equal to 1e-12).
+Also reported: the serving loop (continuous batching with HBM-budget admission and 1 % early stop)
+on the 17x17 pin-lattice request stream at 50/75/100 % of measured capacity with p50/p99
+latency, and the plain eigenvalue engine on the 17x17 lattice (1e6 histories per cycle).
+
+--workload eigen prints the eigenvalue line instead.  Prints one JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -15,14 +24,20 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "histories/s & p50/p99 solve latency, batched vs plain GPU, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
-S_P = 96            # bytes of compulsory particle state per event (load + store counted twice below)
+S_P = 96            # bytes of compulsory particle state per event (loaded and stored: 2 * S_P)
 SCORE_B = 16        # FP64 read-modify-write per tally score
 SITE_B = 64         # fission-site write
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(d):
@@ -32,11 +47,215 @@ def algorithmic_bytes(d):
     return 2 * S_P * events + SCORE_B * d["n_scores"] + SITE_B * d["n_sites"]
 
 
+def max_rel_diff(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    scale = np.maximum(np.abs(a), np.abs(b))
+    d = np.abs(a - b)
+    return float(np.max(np.where(scale > 0, d / np.where(scale > 0, scale, 1.0), 0.0))) if a.size else 0.0
+
+
+def stats_dict(s):
+    return {k: getattr(s, k) for k, _ in s._fields_}
+
+
 def diff(a, b):
     return {k: b[k] - a[k] for k in a}
 
 
-def run_gpu(transport, pop, warmup, steps, seed=2):
+def measured_traffic(workload_key):
+    path = os.path.join(ROOT, "profiles", "transport_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload_key)
+    except (OSError, ValueError):
+        return None
+
+
+# ---------------------------------------------------------------------------------------------
+# batched fixed-source workload
+# ---------------------------------------------------------------------------------------------
+def batch_problems(n, transport, scale=1.0):
+    from scone_amd import configs, tables as T
+    out = []
+    for p in range(n):
+        inp, mg, pop, cycles = configs.batch_problem(p, transport=transport, scale=scale)
+        comp = T.compile_problem(inp, mg)
+        src = T.compile_source(inp["source"], comp)
+        out.append((comp, src, pop, cycles, inp["seed"]))
+    return out
+
+
+def budget_for(problems, n_batches, slack=1.25):
+    """Device bytes the problems need resident at once (queues, score memory, tables)."""
+    tot = 0
+    for (comp, _, pop, _, _) in problems:
+        depth = 3
+        live = pop * (8 * (3 * depth + 7) + 4 * (depth + 6))
+        tot += 2 * live + 24 * int(comp.problem.n_bins) + (1 << 20)
+    return int(tot * slack) + (256 << 20)
+
+
+def run_batched(problems, transport, settings, warmup, steps, budget):
+    from scone_amd.engine import Batch
+    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=1, **settings)
+    t = time.perf_counter()
+    slots = [B.add(c, s, pop, warmup + steps, seed) for (c, s, pop, _, seed) in problems]
+    t_admit = time.perf_counter() - t
+    t_start = time.perf_counter()
+    if warmup:
+        B.cycle(warmup)
+    s0 = stats_dict(B.stats())
+    t0 = time.perf_counter()
+    cyc_t = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        B.cycle(1)
+        cyc_t.append(time.perf_counter() - a)
+    t1 = time.perf_counter()
+    s1 = stats_dict(B.stats())
+    st = B.states()
+    assert all(st[s] == 2 for s in slots), "batched run: some problems did not finish"
+    results = {}
+    for i in SAMPLE:
+        if i < len(slots):
+            results[i] = B.query(slots[i])[1]
+    d = diff(s0, s1)
+    B.close()
+    wall = t1 - t0
+    return {"hist_per_s": d["n_histories"] / wall, "coll_per_s": d["n_collisions"] / wall,
+            "ms_per_step": wall / steps * 1e3, "wall_s": wall, "histories": d["n_histories"],
+            "collisions": d["n_collisions"], "admit_s": t_admit,
+            "latency_s": t1 - t_start,                 # every problem completes with the last cycle
+            "cycle_ms": [x * 1e3 for x in cyc_t]}, results
+
+
+def run_batched_kernel_timing(problems, transport, settings, budget, cycles=2):
+    """Same workload without the graph, HIP events around every transport launch."""
+    from scone_amd.engine import Batch
+    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=0, **settings)
+    for (c, s, pop, _, seed) in problems:
+        B.add(c, s, pop, cycles + 1, seed)
+    B.cycle(1)
+    s0 = stats_dict(B.stats())
+    B.cycle(cycles)
+    s1 = stats_dict(B.stats())
+    B.close()
+    d = diff(s0, s1)
+    launches = max(1, d["transport_launches"])
+    kernel_s = d["transport_kernel_ms"] / 1e3
+    alg = algorithmic_bytes(d)
+    return {"kernel_ms_per_launch": d["transport_kernel_ms"] / launches, "launches": launches,
+            "alg_bytes_per_launch": alg / launches, "achieved_gbs": alg / kernel_s / 1e9 if kernel_s > 0 else 0.0,
+            "kernel_share": kernel_s / max(1e-12, d["cycle_ms"] / 1e3)}
+
+
+def run_plain(problems, transport, settings, warmup, steps):
+    from scone_amd.engine import Batch
+    B = Batch(transport=transport, max_slots=1, hbm_budget=budget_for(problems[:1], 1) + max(
+        budget_for([p], 1) for p in problems), use_graph=1, **settings)
+    busy = 0.0
+    hist = 0
+    lat = []
+    results = {}
+    for i, (c, s, pop, _, seed) in enumerate(problems):
+        slot = B.add(c, s, pop, warmup + steps, seed)
+        t = time.perf_counter()
+        B.run_all()
+        busy += time.perf_counter() - t
+        lat.append(busy)                 # all submitted at t = 0, served one after the other
+        hist += pop * (warmup + steps)
+        if i in SAMPLE:
+            results[i] = B.query(slot)[1]
+        B.release(slot)
+        if i % 128 == 127:
+            log(f"plain: {i + 1} problems, {busy:.1f} s")
+    B.close()
+    lat = np.array(lat)
+    return {"hist_per_s": hist / busy, "wall_s": busy, "histories": hist,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}, results
+
+
+SAMPLE = (0, 1, 2, 3, 100, 257, 511)
+
+
+def cpu_batch_baseline(problems, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import binding as O
+    threads = min(16, os.cpu_count() or 1)
+    n_hist = 0
+    done = 0
+    t0 = time.perf_counter()
+    for (c, s, pop, cycles, seed) in problems:
+        O.fixed_run(c, s, pop, cycles, seed, threads=threads)
+        n_hist += pop * cycles
+        done += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n_hist / dt, "unit": "histories/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of the 512 problems, all 10 batches each ({n_hist} histories), "
+                      f"OpenMP oracle, {dt:.1f} s"}
+
+
+# ---------------------------------------------------------------------------------------------
+# serving
+# ---------------------------------------------------------------------------------------------
+def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7):
+    from scone_amd import configs, tables as T
+    from scone_amd.engine import Batch
+    from scone_amd.serve import Request, Server, latency_summary
+    inp, mg, _, _ = configs.lattice_request(0, transport=transport)
+    comp = T.compile_problem(inp, mg)
+    src = T.compile_source(inp["source"], comp)
+
+    def request(i, arrival):
+        _, _, pop, cycles = configs.lattice_request(i, transport=transport)
+        return Request(comp, src, pop, cycles, 1000 + i, stop_bin=0, stop_rel=stop_rel, arrival=arrival)
+
+    slots, budget = 256, 16 << 30
+    # capacity: a saturated pool of requests, all admitted at once
+    B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
+    reqs = [request(100000 + i, 0.0) for i in range(slots)]
+    t = time.perf_counter()
+    Server(B).run_wall(reqs)
+    dt = time.perf_counter() - t
+    hist = sum(r.result.n_histories for r in reqs)
+    cap_hist = hist / dt
+    mean_req = hist / len(reqs)
+    cap_req = cap_hist / mean_req
+    B.close()
+    log(f"serving capacity {cap_hist:.3e} histories/s, {cap_req:.1f} requests/s (mean {mean_req:.0f} histories)")
+    out = {"capacity_hist_per_s": cap_hist, "capacity_req_per_s": cap_req, "mean_request_histories": mean_req,
+           "loads": {}}
+    rng = np.random.default_rng(seed)
+    k = 0
+    for load in loads:
+        lam = load * cap_req
+        n = max(1, int(lam * seconds_per_load))
+        arr = np.cumsum(rng.exponential(1.0 / lam, n))
+        reqs = [request(k + i, float(a)) for i, a in enumerate(arr)]
+        k += n
+        B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
+        srv = Server(B)
+        el = srv.run_wall(reqs, progress=lambda t, p, w, r: log(f"serving load {load}: {t:.0f} s, pending {p}, "
+                                                                 f"waiting {w}, running {r}"))
+        hist = sum(r.result.n_histories for r in reqs)
+        summ = latency_summary(reqs)
+        summ.update({"offered_req_per_s": lam, "hist_per_s": hist / el, "elapsed_s": el, "cycles": srv.cycles,
+                     "busy_rejections": srv.busy_rejections,
+                     "early_stopped": int(sum(1 for r in reqs if r.result.n_batches < r.n_batches))})
+        out["loads"][str(load)] = summ
+        B.close()
+        log(f"serving load {load}: p50 {summ['p50_s'] * 1e3:.1f} ms, p99 {summ['p99_s'] * 1e3:.1f} ms, "
+            f"{summ['hist_per_s']:.3e} histories/s")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# eigenvalue (plain engine, 17x17 lattice)
+# ---------------------------------------------------------------------------------------------
+def run_eigen(transport, pop, warmup, steps, seed=2):
     from scone_amd import configs, tables as T
     from scone_amd.engine import Engine
     inp, mg = configs.pin_lattice_7g(pop=pop, inactive=warmup, active=steps, transport=transport)
@@ -54,36 +273,18 @@ def run_gpu(transport, pop, warmup, steps, seed=2):
     assert done == steps
     d = diff(s0, s1)
     wall = t1 - t0
-    timed_cycles = ct[warmup:warmup + steps]
-    hist = d["n_histories"]
     kernel_s = d["transport_kernel_ms"] / 1e3
     launches = max(1, d["transport_launches"])
     bytes_alg = algorithmic_bytes(d)
     eng.close()
-    return {
-        "hist_per_s": hist / wall,
-        "coll_per_s": d["n_collisions"] / wall,
-        "ms_per_step": wall / steps * 1e3,
-        "cycle_ms_p50": float(sorted(timed_cycles)[len(timed_cycles) // 2] * 1e3),
-        "wall_s": wall,
-        "histories": hist,
-        "collisions": d["n_collisions"],
-        "segments": d["n_segments"],
-        "scores": d["n_scores"],
-        "sites": d["n_sites"],
-        "kernel_s": kernel_s,
-        "launches": launches,
-        "kernel_ms_per_launch": d["transport_kernel_ms"] / launches,
-        "alg_bytes": bytes_alg,
-        "achieved_gbs": bytes_alg / kernel_s / 1e9 if kernel_s > 0 else 0.0,
-        "k_implicit": list(R.k_implicit),
-        "k_last": float(kh[-1]),
-    }
+    return {"hist_per_s": d["n_histories"] / wall, "coll_per_s": d["n_collisions"] / wall,
+            "ms_per_step": wall / steps * 1e3, "kernel_ms_per_launch": d["transport_kernel_ms"] / launches,
+            "launches": launches, "alg_bytes_per_launch": bytes_alg / launches,
+            "achieved_gbs": bytes_alg / kernel_s / 1e9 if kernel_s > 0 else 0.0,
+            "k_implicit": list(R.k_implicit)}
 
 
-def cpu_baseline(transport, pop, seconds=10.0, seed=2):
-    """The CPU oracle (OpenMP, history-based restatement) on the same workload: active cycles of
-    the same bank size, repeated until ~`seconds` of wall time."""
+def cpu_eigen_baseline(transport, pop, seconds=10.0, seed=2):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as O
     from scone_amd import configs, tables as T
@@ -91,8 +292,7 @@ def cpu_baseline(transport, pop, seconds=10.0, seed=2):
     inp, mg = configs.pin_lattice_7g(pop=pop, transport=transport)
     comp = T.compile_problem(inp, mg)
     bank = O.fission_source(comp, pop, seed)
-    n_hist = 0
-    cycles = 0
+    n_hist = cycles = 0
     master = seed
     t0 = time.perf_counter()
     while True:
@@ -105,88 +305,121 @@ def cpu_baseline(transport, pop, seconds=10.0, seed=2):
             break
     dt = time.perf_counter() - t0
     return {"value": n_hist / dt, "unit": "histories/s", "cores": threads, "kind": "port",
-            "sample": f"{cycles} active cycle(s) of {pop} histories ({transport[-2:]}, 17x17 lattice, 7 groups, "
+            "sample": f"{cycles} cycle(s) of {pop} histories ({transport[-2:]}, 17x17 lattice, 7 groups, "
                       f"collision clerks on), OpenMP oracle, {dt:.1f} s"}
 
 
-def measured_traffic(workload_key):
-    path = os.path.join(ROOT, "profiles", "transport_traffic.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        return d.get(workload_key)
-    except (OSError, ValueError):
-        return None
+def eigen_line(args):
+    log(f"eigen: plain engine, {args.transport}, pop {args.pop}, warmup {args.warmup}, steps {args.steps}")
+    r = run_eigen(args.transport, args.pop, args.warmup, args.steps)
+    cpu = None if args.no_cpu else cpu_eigen_baseline(args.transport, args.pop, args.cpu_seconds)
+    tag = args.transport[-2:]
+    return {
+        "metric": METRIC, "value": r["hist_per_s"], "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded multigroup data, C5G7 UO2 pin map)",
+        "config": {"workload": "plain MG eigenvalue, 17x17 square pin lattice (pitch 1.26, r 0.54), reflective, "
+                               "7 groups, 1e6 histories/cycle", "pop": args.pop, "groups": 7, "transport": tag,
+                   "engine": "plain", "tallies": "7 group clerks x (flux, fission) x 289 pins"},
+        "collisions_per_s": r["coll_per_s"], "k_implicit": r["k_implicit"],
+        "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"pin_lattice_17x17_7g_pop{args.pop}_{tag}"),
+                     "kernel": "k_transport", "kernel_ms_per_launch": r["kernel_ms_per_launch"],
+                     "alg_bytes_per_launch": r["alg_bytes_per_launch"], "launches": r["launches"]},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (r["hist_per_s"] / cpu["value"]) if cpu else None,
+    }
+
+
+def batch_line(args):
+    from scone_amd import autotune, tables as T
+    transport = T.TRANSPORT_DT if args.transport == "transportOperatorDT" else T.TRANSPORT_ST
+    log(f"batch: compiling {args.problems} problems")
+    problems = batch_problems(args.problems, args.transport, args.scale)
+    budget = budget_for(problems, args.warmup + args.steps)
+    tot_hist = sum(p[2] for p in problems)
+    log(f"batch: {tot_hist} histories per cycle, device budget {budget / 2**30:.2f} GiB")
+    if args.tune:
+        t = time.perf_counter()
+        best, table = autotune.tune([(c, s, pop, seed) for (c, s, pop, _, seed) in problems], transport, budget,
+                                    tiles=(64, 128, 256), k0s=(4, 8, 16), launches=(3, 4, 6), log=log)
+        tune_s = time.perf_counter() - t
+    else:
+        best, table, tune_s = {"tile": 64, "k0": 8, "n_launch": 4}, [], 0.0
+    log(f"batch: tuned {best} in {tune_s:.1f} s")
+    bat, res_b = run_batched(problems, transport, best, args.warmup, args.steps, budget)
+    log(f"batch: batched {bat['hist_per_s']:.3e} histories/s, {bat['ms_per_step']:.2f} ms/cycle")
+    kt = run_batched_kernel_timing(problems, transport, best, budget)
+    log(f"batch: transport kernel {kt['kernel_ms_per_launch']:.3f} ms/launch, {kt['achieved_gbs']:.1f} GB/s")
+    pla, res_p = (None, {})
+    if not args.no_plain:
+        pla, res_p = run_plain(problems, transport, best, args.warmup, args.steps)
+        log(f"batch: plain {pla['hist_per_s']:.3e} histories/s over {pla['wall_s']:.1f} s")
+    identical = None
+    if res_p:
+        identical = max(max_rel_diff(res_b[i], res_p[i]) for i in res_b) <= 1e-12
+    cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_seconds)
+    serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds)
+    eig = None
+    if args.eigen:
+        log("eigen leg")
+        e = run_eigen(args.transport, args.pop, 3, 10)
+        eig = {k: e[k] for k in ("hist_per_s", "coll_per_s", "ms_per_step", "achieved_gbs", "kernel_ms_per_launch",
+                                 "k_implicit")}
+    line = {
+        "metric": METRIC, "value": bat["hist_per_s"], "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded problems and multigroup data)",
+        "config": {"workload": "batched fixed source: 512 problems, 1D-3D boxes, 1-8 regions, 2-70 groups, "
+                               "1e3-1e6 histories in 10 batches, point/volume sources, 1e3-1e5-bin mesh flux tallies",
+                   "problems": args.problems, "histories_per_cycle": tot_hist, "transport": args.transport[-2:],
+                   "engine": "batched (persistent launch, hipGraph cycles, autotuned)", "launch_shape": best},
+        "collisions_per_s": bat["coll_per_s"],
+        "latency": {"batched_p50_s": bat["latency_s"], "batched_p99_s": bat["latency_s"],
+                    "plain_p50_s": pla["latency_p50_s"] if pla else None,
+                    "plain_p99_s": pla["latency_p99_s"] if pla else None,
+                    "note": "all 512 solves submitted at t=0"},
+        "plain": {"hist_per_s": pla["hist_per_s"], "wall_s": pla["wall_s"]} if pla else None,
+        "batched_over_plain": (bat["hist_per_s"] / pla["hist_per_s"]) if pla else None,
+        "identical_results_sample": identical,
+        "roofline": {"bound": "hbm", "achieved": kt["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kt["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"batch512_{args.transport[-2:]}"),
+                     "kernel": "k_btransport", "kernel_ms_per_launch": kt["kernel_ms_per_launch"],
+                     "alg_bytes_per_launch": kt["alg_bytes_per_launch"], "launches": kt["launches"],
+                     "kernel_share_of_cycle": kt["kernel_share"]},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (bat["hist_per_s"] / cpu["value"]) if cpu else None,
+        "autotune": {"seconds": tune_s, "candidates": len(table)},
+        "serving": serve,
+        "eigen_17x17_1e6": eig,
+    }
+    return line
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--pop", type=int, default=1000000)
-    ap.add_argument("--transport", default="transportOperatorDT",
-                    choices=["transportOperatorDT", "transportOperatorST"])
-    ap.add_argument("--compare", action="store_true", default=True,
-                    help="also time the other tracking method")
-    ap.add_argument("--no-compare", dest="compare", action="store_false")
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batch", choices=["batch", "eigen"])
+    ap.add_argument("--transport", default="transportOperatorDT", choices=["transportOperatorDT", "transportOperatorST"])
+    ap.add_argument("--problems", type=int, default=512)
+    ap.add_argument("--scale", type=float, default=1.0, help="multiplies every problem's history count")
+    ap.add_argument("--pop", type=int, default=1000000, help="eigenvalue histories per cycle")
+    ap.add_argument("--no-tune", dest="tune", action="store_false", default=True)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=15.0, help="offered-load duration per load level")
+    ap.add_argument("--no-eigen", dest="eigen", action="store_false", default=True)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this engine runs on a "
-              f"single GPU; see the sharding design in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (the engine runs on a single "
+              f"GPU; DESIGN.md describes how it would shard)", file=sys.stderr)
         sys.exit(2)
-
-    print(f"[bench] plain engine, {args.transport}, pop {args.pop}, warmup {args.warmup}, steps {args.steps}",
-          file=sys.stderr, flush=True)
-    main_r = run_gpu(args.transport, args.pop, args.warmup, args.steps)
-    other = None
-    if args.compare:
-        other_t = "transportOperatorST" if args.transport == "transportOperatorDT" else "transportOperatorDT"
-        print(f"[bench] compare {other_t}", file=sys.stderr, flush=True)
-        other = run_gpu(other_t, args.pop, args.warmup, args.steps)
-        other["transport"] = other_t
-    cpu = None
-    if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.transport, args.pop, args.cpu_seconds)
-    tag = args.transport[-2:]
-    workload = f"pin_lattice_17x17_7g_pop{args.pop}_{tag}"
-    traffic = measured_traffic(workload)
-    achieved = main_r["achieved_gbs"]
-    line = {
-        "metric": METRIC,
-        "value": main_r["hist_per_s"],
-        "unit": "histories/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": main_r["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded multigroup data, C5G7 UO2 pin map)",
-        "config": {"workload": "plain MG eigenvalue, 17x17 square pin lattice (pitch 1.26, r 0.54), "
-                               "reflective, 7 groups, 1e6 histories/cycle",
-                   "pop": args.pop, "groups": 7, "transport": tag, "engine": "plain",
-                   "tallies": "7 group clerks x (flux, fission) x 289 pins"},
-        "collisions_per_s": main_r["coll_per_s"],
-        "k_implicit": main_r["k_implicit"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "kernel": "k_transport", "kernel_ms_per_launch": main_r["kernel_ms_per_launch"],
-                     "alg_bytes_per_launch": main_r["alg_bytes"] / main_r["launches"],
-                     "launches": main_r["launches"]},
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": (main_r["hist_per_s"] / cpu["value"]) if cpu else None,
-    }
-    if other:
-        line["compare"] = {other["transport"][-2:]: {k: other[k] for k in
-                           ("hist_per_s", "coll_per_s", "ms_per_step", "achieved_gbs", "kernel_ms_per_launch",
-                            "k_implicit")}}
+    line = eigen_line(args) if args.workload == "eigen" else batch_line(args)
     print(json.dumps(line))
 
 

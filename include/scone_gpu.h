@@ -35,7 +35,8 @@ enum {
     SCN_ERR_NORM = 7,        /* population normalisation failed (particleDungeon_class.f90:597) */
     SCN_ERR_SAMPLING = 8,    /* CDF scan fell through (multiScatterMG/fissionMG sampleOut)      */
     SCN_ERR_UNSUPPORTED = 9, /* feature outside the engine's scope                              */
-    SCN_ERR_NO_DEVICE = 10   /* no GPU visible                                                  */
+    SCN_ERR_NO_DEVICE = 10,  /* no GPU visible                                                  */
+    SCN_ERR_BUSY = 11        /* batch engine: no free slot or HBM budget for the request now    */
 };
 
 /* ---- reference constants ------------------------------------------------------------------ */
@@ -205,6 +206,69 @@ typedef struct scn_eigen_result {
     int64_t n_histories;
 } scn_eigen_result;
 
+/* ---- fixed-source problems ------------------------------------------------------------------
+ * Sources of ParticleObjects/Source, sampled on the device with the particle's own stream:
+ *   POINT    pointSource_class.f90:91-270 through configSource_inter.f90:129-141: position r,
+ *            direction isotropic (mu = 2u-1, phi = 2 pi u, rotateVector([1,0,0])) or `dir`, then
+ *            the group: G, or a scan of probG with one draw.  wgt 1, time 0.
+ *   MATERIAL materialSource_class.f90:150-231: per attempt three position draws in
+ *            [bottom, top) and one time draw; rejected while outside the geometry or in another
+ *            material (VOID/UNDEF/OVERLAP is an error); 200 attempts; then mu, phi; group G.
+ * `dir` and `probG` are given as written in the input: the library normalises them as
+ * pointSource%init does (dir / norm2(dir), probG / sum(probG)).                                  */
+enum { SCN_SRC_POINT = 1, SCN_SRC_MATERIAL = 2 };
+typedef struct scn_source {
+    int32_t type;            /* SCN_SRC_* */
+    int32_t isotropic;       /* POINT: 1 = isotropic, 0 = fixed `dir` */
+    int32_t G;               /* group when n_probG == 0 (MATERIAL: always G) */
+    int32_t mat;             /* MATERIAL: source material index (1-based) */
+    int32_t n_probG;         /* POINT: length of probG (= number of groups) or 0 */
+    int32_t pad;
+    double r[3];             /* POINT position */
+    double dir[3];           /* POINT direction */
+    double bottom[3];        /* MATERIAL bounding box */
+    double top[3];
+    double t_low, t_high;    /* MATERIAL boundingTime */
+    const double* probG;     /* POINT: group distribution or NULL */
+} scn_source;
+
+/* One fixed-source solve: fixedSourcePhysicsPackage%run and %cycles
+ * (PhysicsPackages/fixedSourcePhysicsPackage_class.f90:135-289).  Batch b (0-based) samples
+ * source particle j (1-based) from the seed strided by 2*pop*b + j and runs history j on the seed
+ * strided by 2*pop*b + pop + j, exactly the strides the package applies.                        */
+typedef struct scn_fixed_request {
+    const scn_problem* problem; /* its `transport` must match the batch engine's */
+    const scn_source* source;
+    int32_t pop;             /* histories per batch ('pop') */
+    int32_t n_batches;       /* maximum number of batches ('cycles') */
+    int64_t seed;
+    int64_t stop_bin;        /* bin whose relative standard deviation may end the solve early; -1: never */
+    double stop_rel;         /* target relative standard deviation for stop_bin (after >= 2 batches) */
+    int32_t record;          /* 1: keep each history's final RNG state and collision count */
+    int32_t pad;
+} scn_fixed_request;
+
+enum { SCN_SLOT_FREE = 0, SCN_SLOT_RUNNING = 1, SCN_SLOT_DONE = 2 };
+
+typedef struct scn_fixed_result {
+    int32_t state;           /* SCN_SLOT_* */
+    int32_t n_batches;       /* batches run (scoreMemory batchN) */
+    int64_t n_bins;
+    int64_t n_histories;
+    int64_t n_collisions;    /* real collisions */
+    int64_t n_segments;      /* tracking events (ST segments / DT tentative flights) */
+} scn_fixed_result;
+
+typedef struct scn_batch_settings {
+    int32_t transport;       /* SCN_TRANSPORT_ST or _DT, shared by every resident problem */
+    int32_t max_slots;       /* problems resident at once */
+    int64_t hbm_budget;      /* device bytes for problem tables, particle queues and score memory */
+    int32_t tile;            /* histories per work tile (multiple of 64); 0 = default */
+    int32_t k0;              /* (flight, collision) pairs per particle in the first launch; 0 = default */
+    int32_t n_launch;        /* launches per cycle, the last runs to completion; 0 = default */
+    int32_t use_graph;       /* 1: replay each cycle's launch sequence from a captured hipGraph */
+} scn_batch_settings;
+
 /* ---- counters of the engine (cumulative since creation or the last reset) ----------------- */
 typedef struct scn_stats {
     double transport_kernel_ms; /* summed HIP-event time of the transport kernel launches */
@@ -261,6 +325,33 @@ int scn_eigen_finish(void* engine, scn_eigen_result* result);
  * bottom/top = sampling box, g_in = the group passed to fissionMG%sampleOut. */
 int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double bottom[3],
                        const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank);
+
+/* Fixed-source engine with resident problem slots (batched and served solves).
+ * A cycle advances every running slot by one batch (fixedSourcePhysicsPackage_class.f90:187-275):
+ * all their histories share one launch sequence, each wave of the persistent transport kernel
+ * taking tiles of one problem's queue.  Every slot keeps its own tables, RNG stream and score
+ * memory, so a solve gives the same histories alone as in a mix, and tallies equal to summation
+ * order.  Problems with fissile materials are refused (secondaries on the history's own stream,
+ * fixedSource...:214-243, are not implemented).                                                */
+int scn_batch_create(const scn_batch_settings* settings, void** batch);
+void scn_batch_destroy(void* batch);
+const char* scn_batch_last_error(const void* batch);
+/* admit a solve: returns SCN_ERR_BUSY (no slot or not enough of the HBM budget free) or SCN_OK
+ * with its slot id.  The solve starts with the next cycle. */
+int scn_batch_add(void* batch, const scn_fixed_request* request, int32_t* slot);
+/* run n_cycles cycles (or fewer if no slot is running); *n_running = slots still running */
+int scn_batch_cycle(void* batch, int32_t n_cycles, int32_t* n_running);
+/* state of a slot; for a DONE slot also the score results: bins [n_bins*2] (mean, sd) as
+ * scoreMemory%getResult gives them, and the recorded histories (n_batches*pop, batch-major) when
+ * the request asked for them.  Pointers may be NULL. */
+int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* result, double* bins, uint64_t* hist_rng,
+                    int32_t* hist_coll);
+/* SCN_SLOT_* of slots 0..n-1 in one call (n <= max_slots) */
+int scn_batch_states(void* batch, int32_t* states, int32_t n);
+/* free a DONE slot (its memory returns to the budget) */
+int scn_batch_release(void* batch, int32_t slot);
+/* device time and counters of the batch engine */
+int scn_batch_stats(void* batch, scn_stats* out);
 
 /* Engine counters and timings (no counterpart in the reference; SCONE only times transport). */
 int scn_get_stats(void* engine, scn_stats* out);

@@ -11,6 +11,8 @@ module scone_gpu_mod
   private
 
   integer(c_int32_t), parameter, public :: SCN_OK = 0, SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2
+  integer(c_int32_t), parameter, public :: SCN_ERR_BUSY = 11, SCN_SRC_POINT = 1, SCN_SRC_MATERIAL = 2
+  integer(c_int32_t), parameter, public :: SCN_SLOT_FREE = 0, SCN_SLOT_RUNNING = 1, SCN_SLOT_DONE = 2
 
   !! particleStateData (ParticleObjects/particle_class.f90:50-65) as bind(C)
   type, public, bind(C) :: scn_particle_state
@@ -42,6 +44,54 @@ module scone_gpu_mod
     integer(c_int32_t) :: events_per_launch
     integer(c_int32_t) :: block_size
   end type scn_eigen_settings
+
+  !! pointSource / materialSource parameters (ParticleObjects/Source/*.f90 init)
+  type, public, bind(C) :: scn_source
+    integer(c_int32_t) :: type
+    integer(c_int32_t) :: isotropic
+    integer(c_int32_t) :: G
+    integer(c_int32_t) :: mat
+    integer(c_int32_t) :: n_probG
+    integer(c_int32_t) :: pad
+    real(c_double)     :: r(3)
+    real(c_double)     :: dir(3)
+    real(c_double)     :: bottom(3)
+    real(c_double)     :: top(3)
+    real(c_double)     :: t_low, t_high
+    type(c_ptr)        :: probG
+  end type scn_source
+
+  !! one fixedSourcePhysicsPackage solve
+  type, public, bind(C) :: scn_fixed_request
+    type(c_ptr)        :: problem
+    type(c_ptr)        :: source
+    integer(c_int32_t) :: pop
+    integer(c_int32_t) :: n_batches
+    integer(c_int64_t) :: seed
+    integer(c_int64_t) :: stop_bin
+    real(c_double)     :: stop_rel
+    integer(c_int32_t) :: record
+    integer(c_int32_t) :: pad
+  end type scn_fixed_request
+
+  type, public, bind(C) :: scn_fixed_result
+    integer(c_int32_t) :: state
+    integer(c_int32_t) :: n_batches
+    integer(c_int64_t) :: n_bins
+    integer(c_int64_t) :: n_histories
+    integer(c_int64_t) :: n_collisions
+    integer(c_int64_t) :: n_segments
+  end type scn_fixed_result
+
+  type, public, bind(C) :: scn_batch_settings
+    integer(c_int32_t) :: transport
+    integer(c_int32_t) :: max_slots
+    integer(c_int64_t) :: hbm_budget
+    integer(c_int32_t) :: tile
+    integer(c_int32_t) :: k0
+    integer(c_int32_t) :: n_launch
+    integer(c_int32_t) :: use_graph
+  end type scn_batch_settings
 
   interface
     !! engine lifetime (tables built host side from the same dictionaries SCONE reads)
@@ -100,8 +150,57 @@ module scone_gpu_mod
       type(scn_particle_state), intent(in) :: initial_bank(*)
       type(c_ptr), value                   :: result
     end function scn_eigen_run
+
+    !! fixed-source engine with resident slots: replaces the history loop of
+    !! fixedSourcePhysicsPackage%cycles (PhysicsPackages/fixedSourcePhysicsPackage_class.f90:187-275)
+    integer(c_int) function scn_batch_create(settings, batch) bind(C, name="scn_batch_create")
+      import :: c_int, c_ptr, scn_batch_settings
+      type(scn_batch_settings), intent(in) :: settings
+      type(c_ptr)                          :: batch
+    end function scn_batch_create
+
+    subroutine scn_batch_destroy(batch) bind(C, name="scn_batch_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end subroutine scn_batch_destroy
+
+    type(c_ptr) function scn_batch_last_error(batch) bind(C, name="scn_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end function scn_batch_last_error
+
+    integer(c_int) function scn_batch_add(batch, request, slot) bind(C, name="scn_batch_add")
+      import :: c_int, c_int32_t, c_ptr, scn_fixed_request
+      type(c_ptr), value                  :: batch
+      type(scn_fixed_request), intent(in) :: request
+      integer(c_int32_t), intent(out)     :: slot
+    end function scn_batch_add
+
+    integer(c_int) function scn_batch_cycle(batch, n_cycles, n_running) bind(C, name="scn_batch_cycle")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value              :: batch
+      integer(c_int32_t), value       :: n_cycles
+      integer(c_int32_t), intent(out) :: n_running
+    end function scn_batch_cycle
+
+    integer(c_int) function scn_batch_query(batch, slot, result, bins, hist_rng, hist_coll) &
+                                            bind(C, name="scn_batch_query")
+      import :: c_int, c_int32_t, c_ptr, scn_fixed_result
+      type(c_ptr), value                   :: batch
+      integer(c_int32_t), value            :: slot
+      type(scn_fixed_result), intent(out)  :: result
+      type(c_ptr), value                   :: bins, hist_rng, hist_coll
+    end function scn_batch_query
+
+    integer(c_int) function scn_batch_release(batch, slot) bind(C, name="scn_batch_release")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value        :: batch
+      integer(c_int32_t), value :: slot
+    end function scn_batch_release
   end interface
 
   public :: scn_create, scn_destroy, scn_last_error, scn_eigen_cycle, scn_norm_size_repr, scn_eigen_run
+  public :: scn_batch_create, scn_batch_destroy, scn_batch_last_error, scn_batch_add, scn_batch_cycle, &
+            scn_batch_query, scn_batch_release
 
 end module scone_gpu_mod

@@ -87,6 +87,10 @@ def lib():
         L.orc_fission_source.argtypes = [P, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
         L.orc_eigen_run.argtypes = [P, C.POINTER(T.EigenSettings), PS, C.POINTER(T.EigenResult), C.c_int32]
         dp = C.POINTER(C.c_double)
+        SRC = C.POINTER(T.Source)
+        L.orc_source_sample.argtypes = [P, SRC, C.c_int32, C.c_int64, PS]
+        L.orc_fixed_run.argtypes = [P, SRC, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_double, dp,
+                                    C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
         L.orc_score_result.argtypes = [C.c_double, C.c_double, C.c_int, dp]
         L.orc_close_cycle.argtypes = [dp, dp, dp, C.c_int64, C.c_double]
         L.orc_keff_implicit.restype = C.c_double
@@ -264,6 +268,28 @@ def eigen_run(comp, settings: dict, initial_bank, threads=1):
     R.bins_inactive = bi.ctypes.data_as(C.POINTER(C.c_double))
     check(lib().orc_eigen_run(C.byref(comp.problem), C.byref(S), T.bank_ptr(initial_bank), C.byref(R), threads))
     return R, kh, ba[: 2 * nb].reshape(-1, 2), bi[: 2 * nb].reshape(-1, 2)
+
+
+def source_sample(comp, src, n, rng_state):
+    """source%generate of n particles (particle i on the master strided by i)."""
+    bank = T.bank_array(n)
+    check(lib().orc_source_sample(C.byref(comp.problem), C.byref(src.source), n, C.c_int64(_s64(rng_state)),
+                                  T.bank_ptr(bank)))
+    return bank
+
+
+def fixed_run(comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, hist=False, threads=1):
+    """fixedSourcePhysicsPackage run; returns (bins (n_bins, 2) of mean/sd, batches run, hist)."""
+    nb = int(comp.problem.n_bins)
+    res = np.zeros(max(1, 2 * nb))
+    h = np.zeros(n_batches * pop, dtype=HIST_DTYPE) if hist else None
+    done = C.c_int32()
+    check(lib().orc_fixed_run(C.byref(comp.problem), C.byref(src.source), pop, n_batches, C.c_int64(_s64(seed)),
+                              stop_bin, stop_rel, res.ctypes.data_as(C.POINTER(C.c_double)),
+                              h.ctypes.data_as(C.c_void_p) if h is not None else None, C.byref(done), threads))
+    if h is not None:
+        h = h[: done.value * pop]
+    return res[: 2 * nb].reshape(-1, 2), done.value, h
 
 
 def score_result(csum, csum2, N):

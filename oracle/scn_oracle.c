@@ -1115,6 +1115,8 @@ int orc_norm_size_repr(scn_particle_state* bank, int32_t* np, int32_t cap, int32
  * Sources
  * ====================================================================================== */
 
+static void o_result(double csum, double csum2, int N, double out[2]);
+
 /* fissionSource_class.f90:149-271 via source_inter.f90:98-119 (particle i uses stride(i)) */
 int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const double bottom[3],
                        const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank)
@@ -1165,6 +1167,174 @@ int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const
             break;
         }
     }
+    return g_err;
+}
+
+/* pointSource%init (pointSource_class.f90:120-128,150-152): dir / norm2(dir) and
+ * probG / sum(probG), the sum taken left to right */
+static void o_source_prepare(const scn_source* in, scn_source* out, double* probG)
+{
+    *out = *in;
+    if (in->type == SCN_SRC_POINT && !in->isotropic) {
+        double n = scn_norm2(in->dir);
+        for (int k = 0; k < 3; ++k) out->dir[k] = in->dir[k] / n;
+    }
+    if (in->type == SCN_SRC_POINT && in->n_probG > 0) {
+        double t = 0.0;
+        for (int g = 0; g < in->n_probG; ++g) t = t + in->probG[g];
+        for (int g = 0; g < in->n_probG; ++g) probG[g] = in->probG[g] / t;
+        out->probG = probG;
+    }
+}
+
+/* configSource_inter.f90:129-141 sampleParticle with pointSource's sampleType, samplePosition,
+ * sampleEnergyAngle (:197-212) and sampleEnergy (:221-246), in that order */
+static int o_point_sample(const scn_source* S, uint64_t* rng, scn_particle_state* s)
+{
+    memset(s, 0, sizeof *s);
+    s->matIdx = -1;
+    s->cellIdx = -1;
+    s->uniqueID = -1;
+    s->lambda = SCN_INF;
+    s->type = 1;
+    for (int k = 0; k < 3; ++k) s->r[k] = S->r[k];
+    if (S->isotropic) {
+        double mu = 2.0 * scn_rng_get(rng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(rng);
+        double x1[3] = {1.0, 0.0, 0.0};
+        scn_rotate(x1, mu, phi, s->dir);
+    } else {
+        for (int k = 0; k < 3; ++k) s->dir[k] = S->dir[k];
+    }
+    if (S->n_probG > 0) {
+        double r = scn_rng_get(rng);
+        int g;
+        for (g = 1; g <= S->n_probG; ++g) {
+            r = r - S->probG[g - 1];
+            if (r < 0.0) break;
+        }
+        if (g > S->n_probG) { o_fail(SCN_ERR_SAMPLING, "pointSource sampleEnergy: probG scan fell through"); return g_err; }
+        s->G = g;
+    } else {
+        s->G = S->G;
+    }
+    s->isMG = 1;
+    s->time = 0.0;
+    s->wgt = 1.0;
+    return 0;
+}
+
+/* materialSource_class.f90:150-231 sampleParticle (MG data) */
+static int o_material_sample(const scn_problem* P, const scn_source* S, uint64_t* rng, scn_particle_state* s)
+{
+    memset(s, 0, sizeof *s);
+    s->matIdx = -1;
+    s->cellIdx = -1;
+    s->uniqueID = -1;
+    s->lambda = SCN_INF;
+    for (int i = 1;; ++i) {
+        if (i > 200) {
+            o_fail(SCN_ERR_SAMPLING, "Infinite loop in sampling source. Please check that defined volume contains source material.");
+            return g_err;
+        }
+        double rand3[3], r[3];
+        for (int k = 0; k < 3; ++k) rand3[k] = scn_rng_get(rng);
+        for (int k = 0; k < 3; ++k) r[k] = (S->top[k] - S->bottom[k]) * rand3[k] + S->bottom[k];
+        double time = S->t_low + scn_rng_get(rng) * (S->t_high - S->t_low);
+        o_coords c;
+        double u0[3] = {1.0, 0.0, 0.0};   /* geometryStd%whatIsAt without u */
+        o_init_coords(&c, r, u0);
+        o_place_coord(P, &c);
+        int mat = c.matIdx;
+        if (mat == SCN_OUTSIDE_MAT) continue;
+        if (mat < 1 || mat > P->mg.n_mat) {
+            o_fail(SCN_ERR_MATERIAL, "Nuclear data did not return neutron material.");
+            return g_err;
+        }
+        if (mat != S->mat) continue;
+        s->matIdx = mat;
+        s->uniqueID = c.uniqueID;
+        s->wgt = 1.0;
+        s->time = time;
+        s->type = 1;
+        for (int k = 0; k < 3; ++k) s->r[k] = r[k];
+        double mu = 2.0 * scn_rng_get(rng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(rng);
+        double x1[3] = {1.0, 0.0, 0.0};
+        scn_rotate(x1, mu, phi, s->dir);
+        s->G = S->G;
+        s->isMG = 1;
+        return 0;
+    }
+}
+
+/* source_inter.f90:98-119 generate: particle i (1-based) is sampled with the master strided by i */
+int orc_source_sample(const scn_problem* P, const scn_source* src, int32_t n, int64_t rng_state,
+                      scn_particle_state* bank)
+{
+    o_clear();
+    double probG[512];
+    scn_source S;
+    if (src->n_probG > 512) { o_fail(SCN_ERR_ARG, "probG longer than 512 groups"); return g_err; }
+    o_source_prepare(src, &S, probG);
+    for (int i = 1; i <= n && !g_err; ++i) {
+        uint64_t rng = scn_rng_stride((uint64_t)rng_state, i);
+        if (S.type == SCN_SRC_POINT) o_point_sample(&S, &rng, &bank[i - 1]);
+        else if (S.type == SCN_SRC_MATERIAL) o_material_sample(P, &S, &rng, &bank[i - 1]);
+        else o_fail(SCN_ERR_ARG, "unknown source type %d", S.type);
+    }
+    return g_err;
+}
+
+/* fixedSourcePhysicsPackage_class.f90:135-147 run and :149-289 cycles for a problem without
+ * fissile material (so the per-thread buffer of secondaries stays empty): per batch,
+ * generate(pop, pRNG); pRNG%stride(totalPop); history n on pRNG strided by n; pRNG%stride(totalPop);
+ * reportCycleEnd -> scoreMemory%closeCycle(normFactor) (tallyAdmin_class.f90:736-786).
+ * stop_bin >= 0 adds an early stop: after a batch (N >= 2 batches closed), the solve ends when the
+ * bin's sd / |mean| <= stop_rel.  bins_result gets (mean, sd) per bin from N batches. */
+int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int32_t n_batches, int64_t seed,
+                  int64_t stop_bin, double stop_rel, double* bins_result, orc_hist* hist, int32_t* n_done,
+                  int32_t n_threads)
+{
+    o_clear();
+    for (int m = 0; m < P->mg.n_mat; ++m)
+        if (P->mg.fissile[m]) { o_fail(SCN_ERR_UNSUPPORTED, "fixed source with fissile material"); return g_err; }
+    const int64_t nb = P->n_bins;
+    scn_particle_state* bank = (scn_particle_state*)calloc((size_t)pop + 1, sizeof(scn_particle_state));
+    double* bins = (double*)calloc((size_t)nb + 1, sizeof(double));
+    double* csum = (double*)calloc((size_t)nb + 1, sizeof(double));
+    double* csum2 = (double*)calloc((size_t)nb + 1, sizeof(double));
+    uint64_t master = scn_rng_stride((uint64_t)seed, 0);   /* stride(getOffset(totalPop)), one rank */
+    int done = 0;
+    for (int b = 0; b < n_batches && !g_err; ++b) {
+        if (orc_source_sample(P, src, pop, (int64_t)master, bank)) break;
+        master = scn_rng_stride(master, pop);
+        int32_t n_out = 0;
+        if (orc_eigen_cycle(P, bank, pop, (int64_t)master, 1.0, NULL, 0, &n_out, nb ? bins : NULL, NULL,
+                            hist ? hist + (size_t)b * pop : NULL, n_threads))
+            break;
+        master = scn_rng_stride(master, pop);
+        double norm = 1.0;
+        if (P->norm_bin >= 0) {
+            double ns = bins[P->norm_bin];
+            if (ns == 0.0) { o_fail(SCN_ERR_NORM, "Normalisation score from clerk is 0"); break; }
+            norm = P->norm_value / ns;
+        }
+        orc_close_cycle(bins, csum, csum2, nb, norm);
+        done = b + 1;
+        if (stop_bin >= 0 && stop_bin < nb && done >= 2) {
+            double o[2];
+            o_result(csum[stop_bin], csum2[stop_bin], done, o);
+            if (o[0] != 0.0 && o[1] / fabs(o[0]) <= stop_rel) break;
+        }
+    }
+    if (!g_err && bins_result)
+        for (int64_t i = 0; i < nb; ++i) o_result(csum[i], csum2[i], done, bins_result + 2 * i);
+    if (n_done) *n_done = done;
+    free(bank);
+    free(bins);
+    free(csum);
+    free(csum2);
     return g_err;
 }
 

@@ -84,6 +84,11 @@ int orc_norm_size_repr(scn_particle_state* bank, int32_t* n, int32_t cap, int32_
                        int64_t rng_state);
 int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const double bottom[3],
                        const double top[3], int32_t g_in, int32_t attempts, scn_particle_state* bank);
+int orc_source_sample(const scn_problem* P, const scn_source* src, int32_t n, int64_t rng_state,
+                      scn_particle_state* bank);
+int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int32_t n_batches, int64_t seed,
+                  int64_t stop_bin, double stop_rel, double* bins_result, orc_hist* hist, int32_t* n_done,
+                  int32_t n_threads);
 void orc_score_result(double csum, double csum2, int N, double out[2]);
 void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm);
 double orc_keff_implicit(const double s[4]);

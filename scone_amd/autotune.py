@@ -1,0 +1,44 @@
+"""Autotuner of the batch engine's launch shape.
+
+The cycle of the batch engine has three free parameters (scn_batch_settings): the work tile
+(histories a wave takes from the queue at once), K0 (flight/collision pairs per particle in the
+first launch; K doubles per launch) and the number of launches per cycle (the last runs every
+survivor to completion).  Small tiles balance uneven problems better, large ones cut atomics on
+the work counter; small K0 compacts the queues early (fewer idle lanes) at the cost of more
+launches.  The tuner times one cycle of a representative sample for each candidate and keeps the
+fastest.  Results do not depend on the choice: every candidate runs the same histories.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+from .engine import Batch
+
+TILES = (64, 128, 256)
+K0S = (2, 4, 8, 16)
+LAUNCHES = (2, 3, 4, 6)
+
+
+def tune(problems, transport, hbm_budget, max_slots=None, tiles=TILES, k0s=K0S, launches=LAUNCHES, cycles=1,
+         log=None):
+    """problems: list of (comp, src, pop, seed).  Returns (best settings dict, table of timings)."""
+    max_slots = max_slots or max(1, len(problems))
+    table = []
+    best = None
+    for tile, k0, nl in itertools.product(tiles, k0s, launches):
+        B = Batch(transport=transport, max_slots=max_slots, hbm_budget=hbm_budget, tile=tile, k0=k0, n_launch=nl,
+                  use_graph=1)
+        for (comp, src, pop, seed) in problems:
+            B.add(comp, src, pop, cycles + 1, seed)
+        B.cycle(1)            # first cycle records the graph
+        t = time.perf_counter()
+        B.cycle(cycles)
+        dt = (time.perf_counter() - t) / cycles
+        B.close()
+        table.append({"tile": tile, "k0": k0, "n_launch": nl, "ms_per_cycle": dt * 1e3})
+        if log:
+            log(f"[autotune] tile {tile:4d} k0 {k0:3d} launches {nl}: {dt * 1e3:8.3f} ms/cycle")
+        if best is None or dt < best[0]:
+            best = (dt, {"tile": tile, "k0": k0, "n_launch": nl})
+    return best[1], table

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libscone_amd.so")
-SRC = [os.path.join(HERE, "csrc", "scn_engine.hip")]
+SRC = [os.path.join(HERE, "csrc", "scn_engine.hip"), os.path.join(HERE, "csrc", "scn_batch.hip")]
 ARCH = os.environ.get("SCN_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -11,6 +11,20 @@ generator with a fixed seed, so a problem is reproducible from its name alone.
 * ``pin_lattice_7g``  -- 17x17 square lattice of cylindrical pins (pitch 1.26, r 0.54) with the
                          C5G7 UO2 assembly pin map, reflective box, 7 groups, fuel / structural /
                          moderator materials, per-pin x per-group flux and fission-rate clerks.
+
+Fixed-source problems (fixedSourcePhysicsPackage dictionaries, ``source`` and ``tally`` keys):
+
+* ``inf_medium_fixed`` -- non-multiplying infinite medium (reflective cube) with an isotropic
+                         point source of group distribution probG; the collision-estimator flux per
+                         source neutron is (T - S)^-1 probG.
+* ``slab_fixed``       -- 1D slab (vacuum in x, reflective in y and z) of several regions with a
+                         material (volume) source and a mesh flux tally.
+* ``batch_problem``    -- problem p of the 512-problem batch: 1D-3D boxes, 1-8 regions, 2-70
+                         groups, 1e3-1e6 histories in 10 batches, point or volume source, mesh flux
+                         tally of 1e3-1e5 bins; seeds from splitmix64(0xC3 ^ p).
+* ``lattice_request``  -- a fixed-source solve on the 17x17 pin-lattice geometry (fission turned
+                         into capture), histories log-uniform in 1e3-1e6, with a total-flux bin for
+                         early stopping; the serving workload.
 """
 from __future__ import annotations
 
@@ -190,6 +204,169 @@ def pin_lattice_7g(seed: int = 2, pop: int = 1000000, inactive: int = 50, active
                         "materials": {n: {"temp": 300, "composition": {}, "xsFile": n} for n in mg}},
     }
     return inp, mg
+
+
+def splitmix64(x: int) -> int:
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014) of x + golden gamma."""
+    m = (1 << 64) - 1
+    z = (x + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def _scatter_matrix(rng, nG, sig_t, c):
+    """Full P0[g_out, g_in] with downscatter-dominant shape and a little upscatter, column g_in
+    summing to c[g_in] * sig_t[g_in]."""
+    P0 = np.zeros((nG, nG))
+    for g in range(nG):
+        w = np.zeros(nG)
+        for go in range(nG):
+            d = go - g
+            w[go] = rng.uniform(0.2, 1.0) * (np.exp(-0.7 * d) if d >= 0 else 0.05 * np.exp(1.5 * d))
+        P0[:, g] = w / w.sum() * c[g] * sig_t[g]
+    return P0
+
+
+def fixed_material(rng, nG, c_lo=0.3, c_hi=0.95, st_lo=0.2, st_hi=2.0):
+    sig_t = rng.uniform(st_lo, st_hi, nG)
+    c = rng.uniform(c_lo, c_hi, nG)
+    P0 = _scatter_matrix(rng, nG, sig_t, c)
+    return _mg_dict(sig_t, P0, (1.0 - c) * sig_t)
+
+
+def _fixed_inp(geometry, materials, source, tally, pop, cycles, seed, transport):
+    return {
+        "type": "fixedSourcePhysicsPackage", "pop": pop, "cycles": cycles, "XSdata": "mg", "dataType": "mg",
+        "seed": seed,
+        "collisionOperator": {"neutronMG": {"type": "neutronMGstd"}},
+        "transportOperator": {"type": transport},
+        "source": source, "tally": tally, "geometry": geometry,
+        "nuclearData": {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
+                        "materials": {n: {"temp": 300, "composition": {}, "xsFile": n} for n in materials}},
+    }
+
+
+def _flux_clerk(gtop=None):
+    d = {"type": "collisionClerk", "response": ["flux"], "flux": {"type": "fluxResponse"}}
+    if gtop is not None:
+        d["filter"] = {"type": "energyFilter", "Gtop": gtop, "Glow": gtop}
+    return d
+
+
+def inf_medium_fixed(seed: int = 3, nG: int = 3, pop: int = 2000, cycles: int = 20,
+                     transport: str = "transportOperatorST"):
+    rng = np.random.default_rng(seed)
+    mg = fixed_material(rng, nG, 0.5, 0.9, 0.5, 2.0)
+    probG = rng.uniform(0.1, 1.0, nG)
+    tally = {"total": _flux_clerk()}
+    for g in range(1, nG + 1):
+        tally[f"flux_g{g}"] = _flux_clerk(g)
+    geom = {"type": "geometryStd", "boundary": [1, 1, 1, 1, 1, 1], "graph": {"type": "shrunk"},
+            "surfaces": {"cube": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [5.0, 5.0, 5.0]}},
+            "cells": {},
+            "universes": {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "medium"}}}
+    src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "particle": "neutron", "probG": [float(x) for x in probG]}
+    return _fixed_inp(geom, {"medium": mg}, src, tally, pop, cycles, seed, transport), {"medium": mg}
+
+
+def inf_medium_fixed_flux(mg: dict, probG):
+    """Expected collision-estimator flux per source neutron: (T - S)^-1 q, q = probG / sum."""
+    nG = mg["numberOfGroups"]
+    P0 = np.array(mg["P0"]).reshape(nG, nG).T
+    tot = P0.sum(axis=0) + np.array(mg["capture"])
+    q = np.asarray(probG, dtype=float)
+    q = q / q.sum()
+    return np.linalg.solve(np.diag(tot) - P0, q)
+
+
+def _region_geometry(dim, hw, mats, bc):
+    """Box root -> lattice (k,1,1 or k,1,0) -> single-fill pin universe per region."""
+    k = len(mats)
+    pitch_z = 2.0 * hw[2] if dim == 3 else 0.0
+    unis = {"root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<500>"},
+            "lat": {"id": 500, "type": "latUniverse", "origin": [0.0, 0.0, 0.0],
+                    "pitch": [2.0 * hw[0] / k, 2.0 * hw[1], pitch_z], "shape": [k, 1, 1 if dim == 3 else 0],
+                    "padMat": mats[0], "map": list(range(1, k + 1))}}
+    for i, m in enumerate(mats):
+        unis[f"reg{i + 1}"] = {"id": i + 1, "type": "pinUniverse", "radii": [0.0], "fills": [m]}
+    return {"type": "geometryStd", "boundary": bc, "graph": {"type": "shrunk"},
+            "surfaces": {"box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": list(hw)}},
+            "cells": {}, "universes": unis}
+
+
+def _mesh_clerk(dim, hw, n_bins):
+    axes = ["x", "y", "z"][:dim]
+    n_ax = max(1, int(round(n_bins ** (1.0 / dim))))
+    maps = {"type": "multiMap", "maps": [f"{a}ax" for a in axes]}
+    for i, a in enumerate(axes):
+        maps[f"{a}ax"] = {"type": "spaceMap", "axis": a, "grid": "lin", "min": -hw[i], "max": hw[i], "N": n_ax}
+    return {"type": "collisionClerk", "map": maps, "response": ["flux"], "flux": {"type": "fluxResponse"}}
+
+
+def slab_fixed(seed: int = 4, nG: int = 4, regions: int = 3, pop: int = 2000, cycles: int = 10,
+               transport: str = "transportOperatorST", n_bins: int = 40):
+    rng = np.random.default_rng(seed)
+    mats = {f"m{i + 1}": fixed_material(rng, nG) for i in range(regions)}
+    hw = [float(rng.uniform(5.0, 20.0)), 10.0, 10.0]
+    geom = _region_geometry(1, hw, list(mats), [0, 0, 1, 1, 1, 1])
+    tally = {"total": _flux_clerk(), "mesh": _mesh_clerk(1, hw, n_bins)}
+    src = {"type": "materialSource", "data": "mg", "mat": "m2" if regions > 1 else "m1", "G": 1}
+    return _fixed_inp(geom, mats, src, tally, pop, cycles, seed, transport), mats
+
+
+def batch_problem(p: int, transport: str = "transportOperatorDT", scale: float = 1.0):
+    """Problem p (0..511) of the 512-problem batch; ``scale`` multiplies the history count (for
+    small test versions).  Returns (input dict, mg dicts, histories per batch, batches)."""
+    seed = splitmix64(0xC3 ^ p)
+    rng = np.random.default_rng(seed)
+    dim = int(rng.integers(1, 4))
+    hw = [float(x) for x in rng.uniform(5.0, 50.0, 3)]
+    k = int(rng.integers(1, 9))
+    nG = int(round(np.exp(rng.uniform(np.log(2.0), np.log(70.0)))))
+    hist = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
+    cycles = 10
+    pop = max(1, int(round(hist * scale / cycles)))
+    mats = {f"m{i + 1}": fixed_material(rng, nG) for i in range(k)}
+    n_bins = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e5)))))
+    bc = {1: [0, 0, 1, 1, 1, 1], 2: [0, 0, 0, 0, 1, 1], 3: [0, 0, 0, 0, 0, 0]}[dim]
+    geom = _region_geometry(dim, hw, list(mats), bc)
+    tally = {"total": _flux_clerk(), "mesh": _mesh_clerk(dim, hw, n_bins)}
+    if rng.uniform() < 0.5:
+        probG = rng.uniform(0.0, 1.0, nG) * np.exp(-0.3 * np.arange(nG))
+        src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "particle": "neutron",
+               "probG": [float(x) for x in probG]}
+    else:
+        src = {"type": "materialSource", "data": "mg", "mat": f"m{int(rng.integers(1, k + 1))}", "G": 1}
+    inp = _fixed_inp(geom, mats, src, tally, pop, cycles, int(seed & 0x7FFFFFFF), transport)
+    return inp, mats, pop, cycles
+
+
+def lattice_request(i: int, transport: str = "transportOperatorDT", scale: float = 1.0):
+    """Serving request i: the 17x17 pin-lattice geometry with fission turned into capture, a
+    material source in the fuel, histories log-uniform in 1e3-1e6 split in 10 batches.
+    Returns (input dict, mg dicts, histories per batch, batches)."""
+    rng = np.random.default_rng(splitmix64(0xC4 ^ i))
+    inp, mg = pin_lattice_7g(pop=1000, transport=transport, tallies=False)
+    mg = {n: dict(d) for n, d in mg.items()}
+    f = mg["fuel"]
+    f["capture"] = [c + x for c, x in zip(f["capture"], f["fission"])]
+    for key in ("fission", "nu", "chi"):
+        f.pop(key)
+    hist = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
+    cycles = 10
+    pop = max(1, int(round(hist * scale / cycles)))
+    tally = {"total": _flux_clerk(),
+             "pins": {"type": "collisionClerk",
+                      "map": {"type": "multiMap", "maps": ["xax", "yax"],
+                              "xax": {"type": "spaceMap", "axis": "x", "grid": "lin", "min": -10.71, "max": 10.71,
+                                      "N": 17},
+                              "yax": {"type": "spaceMap", "axis": "y", "grid": "lin", "min": -10.71, "max": 10.71,
+                                      "N": 17}},
+                      "response": ["flux"], "flux": {"type": "fluxResponse"}}}
+    src = {"type": "materialSource", "data": "mg", "mat": "fuel", "G": 1}
+    out = _fixed_inp(inp["geometry"], mg, src, tally, pop, cycles, int(rng.integers(1, 1 << 31)), transport)
+    return out, mg, pop, cycles
 
 
 CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g}

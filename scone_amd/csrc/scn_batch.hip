@@ -1,0 +1,1226 @@
+// scn_batch.hip -- fixed-source engine with resident problem slots (batched and served solves)
+// behind the scn_batch_* entry points of include/scone_gpu.h.
+//
+// Every admitted solve owns a slot: its tables, two live-particle queues (SoA), its score memory
+// (BIN, CSUM, CSUM2) and optional per-history records, all in one block of a device arena that
+// is sized by the HBM budget.  A cycle advances every running slot by one batch of
+// fixedSourcePhysicsPackage%cycles (PhysicsPackages/fixedSourcePhysicsPackage_class.f90:187-275):
+//   k_tiles          per slot ceil(n / tile) work tiles (slot, first index) -> tile table
+//   k_btransport     persistent kernel: each wave takes tiles from an atomic counter, runs up to K
+//                    (flight, collision) pairs per particle in registers and compacts survivors
+//                    into its slot's other queue (wave ballot, one atomic per wave).  The first
+//                    launch samples the batch's source particles in place.
+//   (k_tiles, k_btransport) x (n_launch - 1), K doubling, the last launch runs to completion
+//   k_bnorm/k_bclose tallyAdmin%reportCycleEnd normalisation and scoreMemory%closeCycle
+//   k_bend           batchN + 1, early stop on a bin's relative sd, slot completion
+// Nothing in a cycle depends on host values, so the sequence can be replayed from a hipGraph.
+#include <cstring>
+#include <cstdio>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "scone_gpu.h"
+#include "scn_fpmath.h"
+#include "scn_device.h"
+#include "scn_host.h"
+
+using namespace scn;
+
+namespace {
+
+constexpr int BBLOCK = 256;
+constexpr int BWAVE = 64;
+constexpr int MAX_SLOTS = 4096;
+constexpr int MATERIAL_SOURCE_ATTEMPTS = 200;   // materialSource_class.f90:168
+constexpr int MAX_PROBG = 4096;
+
+// live queue of one slot, structure of arrays; position arrays level-major
+struct BLive {
+    double* r;     // [depth*3][cap]
+    double* u;     // [3][cap]
+    double* w;
+    double* w0;
+    double* t;
+    uint64_t* rng;
+    int* loc;      // [depth][cap]
+    int* nest;
+    int* mat;
+    int* uid;
+    int* G;
+    int* brood;
+    int* coll;
+};
+
+// read-only description of a resident solve
+struct SlotDev {
+    DevProblem P;
+    BLive q[2];
+    double* bin;
+    double* csum;
+    double* csum2;
+    uint64_t* hist_rng;
+    int* hist_coll;
+    const double* probG;
+    double src_r[3], src_dir[3], bottom[3], top[3];
+    double t_low, t_high;
+    double stop_rel, norm_value;
+    int64_t stop_bin, norm_bin;
+    uint64_t seed;
+    int src_type, src_iso, src_G, src_mat, n_probG;
+    int pop, n_batches;
+};
+
+// mutable state of a slot, written by the kernels
+struct SlotState {
+    int running;        // 1 while batches remain
+    int batch;          // batches closed (scoreMemory batchN)
+    int count[2];       // particles written into queue side k by the last launch
+    int n_in;           // particles the current launch reads
+    int stopped;        // 1 if the early-stop target ended the solve
+    double norm;        // normalisation factor of the batch being closed
+    unsigned long long segments, collisions, scores, histories;
+};
+
+// control words: [0] tiles in the current launch, [1] work counter, [2] running slots after the
+// cycle, [3] tile-table overflow, [4] dummy site counter
+constexpr int CTL_WORDS = 8;
+
+template <int NL>
+__device__ __forceinline__ void bload(const BLive& L, int cap, int i, Pt<NL>& p)
+{
+    p.nest = L.nest[i];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < p.nest) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l][k] = L.r[(size_t)(l * 3 + k) * cap + i];
+            p.loc[l] = L.loc[(size_t)l * cap + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+            p.loc[l] = 0;
+        }
+        p.uni[l] = 0;
+        p.root[l] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.u[k] = L.u[(size_t)k * cap + i];
+    p.w = L.w[i];
+    p.w0 = L.w0[i];
+    p.t = L.t[i];
+    p.rng = L.rng[i];
+    p.mat = L.mat[i];
+    p.uid = L.uid[i];
+    p.G = L.G[i];
+    p.brood = L.brood[i];
+    p.coll = L.coll[i];
+    p.nsite = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p.k[k] = 0.0;
+    p.dead = 0;
+    p.fate = FATE_NO;
+    p.track = 0.0;
+}
+
+template <int NL>
+__device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<NL>& p)
+{
+    L.nest[i] = p.nest;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        if (l < p.nest) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) L.r[(size_t)(l * 3 + k) * cap + i] = p.r[l][k];
+            L.loc[(size_t)l * cap + i] = p.loc[l];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L.u[(size_t)k * cap + i] = p.u[k];
+    L.w[i] = p.w;
+    L.w0[i] = p.w0;
+    L.t[i] = p.t;
+    L.rng[i] = p.rng;
+    L.mat[i] = p.mat;
+    L.uid[i] = p.uid;
+    L.G[i] = p.G;
+    L.brood[i] = p.brood;
+    L.coll[i] = p.coll;
+}
+
+// Source particle j = i + 1 of batch `batch`, then the start of its history
+// (fixedSourcePhysicsPackage_class.f90:187-213): the source is sampled on the seed strided by
+// 2*pop*batch + j (generate, source_inter.f90:110-116) and the history runs on the seed strided
+// by 2*pop*batch + pop + j.  Returns 0 if sampling failed (error raised).
+template <int NL>
+__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int i, Pt<NL>& p,
+                                    DevStatus* st)
+{
+    const int64_t j = (int64_t)i + 1;
+    const int64_t b2 = 2 * (int64_t)D.pop * batch;
+    uint64_t srng = scn_rng_stride(D.seed, b2 + j);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        p.loc[l] = 0;
+        p.uni[l] = 0;
+        p.root[l] = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+    }
+    double r[3], dir[3], time = 0.0;
+    int G;
+    if (D.src_type == SCN_SRC_POINT) {
+        // configSource%sampleParticle: position, energy-angle, energy (pointSource_class.f90:183-246)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = D.src_r[k];
+        if (D.src_iso) {
+            double mu = 2.0 * scn_rng_get(&srng) - 1.0;
+            double phi = SCN_TWO_PI * scn_rng_get(&srng);
+            const double x1[3] = {1.0, 0.0, 0.0};
+            scn_rotate(x1, mu, phi, dir);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dir[k] = D.src_dir[k];
+        }
+        if (D.n_probG > 0) {
+            double u = scn_rng_get(&srng);
+            int g;
+            for (g = 1; g <= D.n_probG; ++g) {
+                u = u - D.probG[g - 1];
+                if (u < 0.0) break;
+            }
+            if (g > D.n_probG) {
+                raise_error(st, SCN_ERR_SAMPLING, 5);
+                return 0;
+            }
+            G = g;
+        } else {
+            G = D.src_G;
+        }
+    } else {
+        // materialSource%sampleParticle: rejection on whatIsAt(r) (u = [1,0,0])
+        int it = 0;
+        for (;;) {
+            if (++it > MATERIAL_SOURCE_ATTEMPTS) {
+                raise_error(st, SCN_ERR_SAMPLING, 4);
+                return 0;
+            }
+            double r3[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&srng);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) r[k] = (D.top[k] - D.bottom[k]) * r3[k] + D.bottom[k];
+            time = D.t_low + scn_rng_get(&srng) * (D.t_high - D.t_low);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[0][k] = r[k];
+            p.u[0] = 1.0;
+            p.u[1] = 0.0;
+            p.u[2] = 0.0;
+            place_coord(P, p, st);
+            int mat = p.mat;
+            if (mat == SCN_OUTSIDE_MAT) continue;
+            if (mat < 1 || mat > P.n_mat) {
+                raise_error(st, SCN_ERR_MATERIAL, mat);
+                return 0;
+            }
+            if (mat != D.src_mat) continue;
+            double mu = 2.0 * scn_rng_get(&srng) - 1.0;
+            double phi = SCN_TWO_PI * scn_rng_get(&srng);
+            const double x1[3] = {1.0, 0.0, 0.0};
+            scn_rotate(x1, mu, phi, dir);
+            G = D.src_G;
+            break;
+        }
+    }
+    p.rng = scn_rng_stride(D.seed, b2 + D.pop + j);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        p.r[0][k] = r[k];
+        p.u[k] = dir[k];
+    }
+    p.w = 1.0;
+    p.w0 = 1.0;
+    p.G = G;
+    p.t = time;
+    p.coll = 0;
+    p.brood = (int)j;
+    p.nsite = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p.k[k] = 0.0;
+    p.dead = 0;
+    p.fate = FATE_NO;
+    p.track = 0.0;
+    place_coord(P, p, st);
+    if (p.mat == SCN_OUTSIDE_MAT) {   // pointSource%init: 'Source has been placed outside geometry'
+        raise_error(st, SCN_ERR_ARG, 6);
+        return 0;
+    }
+    return 1;
+}
+
+// tile table of one launch.  One block of 1024 threads; slots in LDS for the search.
+__global__ void __launch_bounds__(1024)
+k_tiles(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first, int in_side, int out_side,
+        int tile, int2* __restrict__ tiles, int tiles_cap, int* ctl)
+{
+    __shared__ int s_start[MAX_SLOTS + 1];
+    __shared__ int s_scan[1024];
+    int carry = 0;
+    for (int base = 0; base < n_slots; base += 1024) {
+        const int s = base + (int)threadIdx.x;
+        int n = 0;
+        if (s < n_slots) {
+            SlotState& S = ss[s];
+            const int pop = slots[s].pop;
+            if (first) n = S.running ? pop : 0;
+            else n = S.count[in_side] < pop ? S.count[in_side] : pop;   // a queue never holds more than pop
+            S.n_in = n;
+            S.count[out_side] = 0;
+        }
+        const int nt = (n + tile - 1) / tile;
+        s_scan[threadIdx.x] = nt;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int v = (int)threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (s < n_slots) s_start[s] = carry + s_scan[threadIdx.x] - nt;
+        carry += s_scan[1023];
+        __syncthreads();
+    }
+    const int total = carry < tiles_cap ? carry : tiles_cap;
+    for (int t = threadIdx.x; t < total; t += 1024) {
+        int lo = 0, hi = n_slots;   // s_start[lo] <= t < s_start[hi] (s_start[n_slots] = total)
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (s_start[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        tiles[t] = make_int2(lo, (t - s_start[lo]) * tile);
+    }
+    if (threadIdx.x == 0) {
+        ctl[0] = total;
+        ctl[1] = 0;
+        if (first) ctl[2] = 0;
+        if (carry > tiles_cap) ctl[3] = 1;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int NL, int MODE>
+__global__ void __launch_bounds__(BBLOCK)
+k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __restrict__ tiles, int* ctl, int first,
+             int in_side, int out_side, int tile, int max_steps, DevStatus* st)
+{
+    const int lane = threadIdx.x & (BWAVE - 1);
+    const int n_tiles = ctl[0];
+    Sites S0;   // fixed-source slots hold no fissile material: the site bank is never written
+    S0.cap = 0;
+    S0.count = ctl + 4;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&ctl[1], 1);
+        t = __shfl(t, 0);
+        if (t >= n_tiles) break;
+        const int2 tl = tiles[t];
+        const int s = __builtin_amdgcn_readfirstlane(tl.x);
+        const int base = __builtin_amdgcn_readfirstlane(tl.y);
+        const SlotDev& D = slots[s];
+        const DevProblem P = D.P;
+        SlotState& SS = ss[s];
+        const int n_in = SS.n_in;
+        const int batch = SS.batch;
+        const int cap = D.pop;
+        Bins B;
+        B.lds = nullptr;
+        B.glob = D.bin;
+        Cnt cn{0, 0, 0};
+        for (int sub = 0; sub < tile; sub += BWAVE) {
+            const int i = base + sub + lane;
+            Pt<NL> p;
+            int alive = 0;
+            if (i < n_in) {
+                if (first) {
+                    alive = born(D, P, batch, i, p, st);
+                } else {
+                    bload(D.q[in_side], cap, i, p);
+                    relink(P, p);
+                    alive = 1;
+                }
+                if (alive) {
+                    for (int step = 0; step < max_steps; ++step) {
+                        int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, cn) : flight_st(P, p, B, st, cn);
+                        if (!ok) {
+                            alive = 0;
+                            break;
+                        }
+                        ok = collide(P, p, 1.0, B, S0, st, cn);
+                        if (!ok) {
+                            alive = 0;
+                            break;
+                        }
+                        if (st->code) break;
+                    }
+                    if (!alive && D.hist_rng) {
+                        const size_t h = (size_t)batch * D.pop + (size_t)(p.brood - 1);
+                        D.hist_rng[h] = p.rng;
+                        D.hist_coll[h] = p.coll;
+                    }
+                }
+            }
+            const unsigned long long mask = __ballot(alive);
+            const int cntw = __popcll(mask);
+            int obase = 0;
+            if (lane == 0 && cntw) obase = atomicAdd(&SS.count[out_side], cntw);
+            obase = __shfl(obase, 0);
+            if (alive) {
+                const int o = obase + __popcll(mask & ((1ull << lane) - 1ull));
+                if (o < cap) bstore(D.q[out_side], cap, o, p);
+                else raise_error(st, SCN_ERR_OVERFLOW, s);
+            }
+        }
+        const unsigned long long seg = wave_sum((unsigned long long)cn.seg);
+        const unsigned long long col = wave_sum((unsigned long long)cn.col);
+        const unsigned long long sc = wave_sum((unsigned long long)cn.sc);
+        if (lane == 0) {
+            if (seg) atomicAdd(&SS.segments, seg);
+            if (col) atomicAdd(&SS.collisions, col);
+            if (sc) atomicAdd(&SS.scores, sc);
+        }
+    }
+}
+
+// tallyAdmin_class.f90:768-784: normFactor = normValue / BIN(normBin), or 1
+__global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, DevStatus* st)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    SlotState& S = ss[s];
+    if (!S.running) return;
+    const SlotDev& D = slots[s];
+    double f = 1.0;
+    if (D.norm_bin >= 0) {
+        double ns = D.bin[D.norm_bin];
+        if (ns == 0.0) {
+            raise_error(st, SCN_ERR_NORM, s);
+            f = 0.0;
+        } else {
+            f = D.norm_value / ns;
+        }
+    }
+    S.norm = f;
+}
+
+// scoreMemory_class.f90:309-342 closeCycle of every running slot (grid.y = slot)
+__global__ void k_bclose(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss)
+{
+    const int s = blockIdx.y;
+    if (!ss[s].running) return;
+    const SlotDev& D = slots[s];
+    const double f = ss[s].norm;
+    const int64_t n = D.P.n_bins;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double r = D.bin[i] * f;
+        D.bin[i] = 0.0;
+        D.csum[i] = D.csum[i] + r;
+        D.csum2[i] = D.csum2[i] + r * r;
+    }
+}
+
+// end of the batch: batchN + 1, completion, early stop on the relative sd of stop_bin
+__global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int* ctl)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    SlotState& S = ss[s];
+    if (!S.running) return;
+    const SlotDev& D = slots[s];
+    const int N = S.batch + 1;
+    S.batch = N;
+    S.histories += (unsigned long long)D.pop;
+    int done = N >= D.n_batches;
+    if (!done && D.stop_bin >= 0 && N >= 2) {
+        // scoreMemory%getResult (scoreMemory_class.f90:537-573)
+        double mean = D.csum[D.stop_bin] / N;
+        double inv_N = 1.0 / N;
+        double inv_Nm1 = 1.0 / (N - 1);
+        double var = D.csum2[D.stop_bin] * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
+        double sd = sqrt(var);
+        if (mean != 0.0 && sd / fabs(mean) <= D.stop_rel) {
+            done = 1;
+            S.stopped = 1;
+        }
+    }
+    if (done) S.running = 0;
+    else atomicAdd(&ctl[2], 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+typedef void (*BTransportFn)(dim3, dim3, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int, int,
+                             int, int, DevStatus*);
+
+template <int NL, int MODE>
+void launch_bt(dim3 g, dim3 b, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles, int* ctl,
+               int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st)
+{
+    hipLaunchKernelGGL((k_btransport<NL, MODE>), g, b, 0, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
+                       max_steps, st);
+}
+
+template <int NL, int MODE>
+const void* bt_kernel()
+{
+    return (const void*)k_btransport<NL, MODE>;
+}
+
+template <int MODE>
+void pick_bt(int nl, BTransportFn* fn, const void** kern)
+{
+    switch (nl) {
+    case 1: *fn = launch_bt<1, MODE>; *kern = bt_kernel<1, MODE>(); break;
+    case 3: *fn = launch_bt<3, MODE>; *kern = bt_kernel<3, MODE>(); break;
+    case 4: *fn = launch_bt<4, MODE>; *kern = bt_kernel<4, MODE>(); break;
+    default: *fn = launch_bt<8, MODE>; *kern = bt_kernel<8, MODE>(); break;
+    }
+}
+
+struct SlotHost {
+    int used = 0;
+    size_t off = 0, bytes = 0;
+    int depth = 0;
+    int64_t n_bins = 0;
+    int pop = 0, n_batches = 0, record = 0;
+};
+
+struct Batch {
+    std::string err;
+    scn_batch_settings S{};
+    hipStream_t stream = nullptr;
+    char* arena = nullptr;
+    size_t arena_size = 0;
+    std::map<size_t, size_t> freel;      // offset -> size of free blocks
+    int n_slots = 0;
+    SlotDev* d_slots = nullptr;
+    SlotState* d_state = nullptr;
+    int2* d_tiles = nullptr;
+    int tiles_cap = 0;
+    int* d_ctl = nullptr;
+    DevStatus* d_status = nullptr;
+    std::vector<SlotDev> h_slots;
+    std::vector<SlotHost> hs;
+    int n_cu = 256;
+    // kernel variant of the recorded cycle
+    int nl_cur = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t c0 = nullptr, c1 = nullptr;
+    scn_stats stats{};
+    int64_t rel_seg = 0, rel_col = 0, rel_sc = 0, rel_hist = 0;   // counters of released slots
+};
+
+#define BHIPCHK(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            B->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+            return SCN_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// first-fit allocation in the arena; returns SIZE_MAX if nothing fits
+size_t arena_alloc(Batch* B, size_t bytes)
+{
+    bytes = align256(std::max<size_t>(bytes, 256));
+    for (auto it = B->freel.begin(); it != B->freel.end(); ++it) {
+        if (it->second >= bytes) {
+            size_t off = it->first, sz = it->second;
+            B->freel.erase(it);
+            if (sz > bytes) B->freel[off + bytes] = sz - bytes;
+            return off;
+        }
+    }
+    return SIZE_MAX;
+}
+
+void arena_free(Batch* B, size_t off, size_t bytes)
+{
+    bytes = align256(std::max<size_t>(bytes, 256));
+    auto it = B->freel.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != B->freel.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        B->freel.erase(nx);
+    }
+    if (it != B->freel.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            B->freel.erase(it);
+        }
+    }
+}
+
+// byte layout of one slot block
+struct SlotLayout {
+    size_t tables = 0;          // bytes of the table region (copied from the host image)
+    size_t queue = 0;           // offset of queue 0; queue 1 follows
+    size_t queue_bytes = 0;
+    size_t bins = 0;            // BIN, CSUM, CSUM2
+    size_t hist = 0;
+    size_t total = 0;
+};
+
+size_t live_bytes(int depth, int cap)
+{
+    size_t c = (size_t)cap;
+    size_t b = 0;
+    b += align256(sizeof(double) * depth * 3 * c);
+    b += align256(sizeof(double) * 3 * c);
+    b += 3 * align256(sizeof(double) * c);
+    b += align256(sizeof(uint64_t) * c);
+    b += align256(sizeof(int) * depth * c);
+    b += 6 * align256(sizeof(int) * c);
+    return b;
+}
+
+void carve_live(char* base, int depth, int cap, BLive& L)
+{
+    size_t c = (size_t)cap;
+    char* p = base;
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += align256(bytes);
+        return q;
+    };
+    L.r = (double*)take(sizeof(double) * depth * 3 * c);
+    L.u = (double*)take(sizeof(double) * 3 * c);
+    L.w = (double*)take(sizeof(double) * c);
+    L.w0 = (double*)take(sizeof(double) * c);
+    L.t = (double*)take(sizeof(double) * c);
+    L.rng = (uint64_t*)take(sizeof(uint64_t) * c);
+    L.loc = (int*)take(sizeof(int) * depth * c);
+    L.nest = (int*)take(sizeof(int) * c);
+    L.mat = (int*)take(sizeof(int) * c);
+    L.uid = (int*)take(sizeof(int) * c);
+    L.G = (int*)take(sizeof(int) * c);
+    L.brood = (int*)take(sizeof(int) * c);
+    L.coll = (int*)take(sizeof(int) * c);
+}
+
+// host image of a slot's tables: appends arrays, returns their offsets
+struct Image {
+    std::vector<char> bytes;
+    template <typename T>
+    size_t put(const T* src, size_t count)
+    {
+        size_t off = align256(bytes.size());
+        size_t n = sizeof(T) * std::max<size_t>(count, 1);
+        bytes.resize(off + n, 0);
+        if (src && count) memcpy(bytes.data() + off, src, sizeof(T) * count);
+        return off;
+    }
+};
+
+int validate(Batch* B, const scn_fixed_request* R, int* depth)
+{
+    const scn_problem* pr = R->problem;
+    const scn_source* sr = R->source;
+    if (!pr || !sr) {
+        B->err = "request without problem or source";
+        return SCN_ERR_ARG;
+    }
+    if (pr->mg.n_mat <= 0 || pr->mg.n_groups <= 0 || pr->n_universes <= 0) {
+        B->err = "invalid problem tables";
+        return SCN_ERR_ARG;
+    }
+    if (pr->transport != B->S.transport) {
+        B->err = "problem transport operator differs from the batch engine's";
+        return SCN_ERR_ARG;
+    }
+    for (int m = 0; m < pr->mg.n_mat; ++m)
+        if (pr->mg.fissile[m]) {
+            B->err = "fixed-source problems with fissile material are not supported (secondaries)";
+            return SCN_ERR_UNSUPPORTED;
+        }
+    int d = problem_depth(pr);
+    if (d < 1 || d > SCN_MAX_NEST) {
+        B->err = "geometry nesting deeper than SCN_MAX_NEST or malformed graph";
+        return SCN_ERR_UNSUPPORTED;
+    }
+    *depth = d;
+    if (R->pop < 1 || R->n_batches < 1) {
+        B->err = "pop and n_batches must be >= 1";
+        return SCN_ERR_ARG;
+    }
+    if ((int64_t)R->pop * 2 * (int64_t)R->n_batches > ((int64_t)1 << 40)) {
+        B->err = "pop * n_batches too large";
+        return SCN_ERR_ARG;
+    }
+    if (R->stop_bin >= pr->n_bins) {
+        B->err = "stop_bin outside the score memory";
+        return SCN_ERR_ARG;
+    }
+    const int nG = pr->mg.n_groups;
+    if (sr->type == SCN_SRC_POINT) {
+        if (sr->n_probG > 0) {
+            if (sr->n_probG != nG || !sr->probG || sr->n_probG > MAX_PROBG) {
+                B->err = "Source energy group distribution must have nG groups";
+                return SCN_ERR_ARG;
+            }
+        } else if (sr->G < 1 || sr->G > nG) {
+            B->err = "source group outside 1..nG";
+            return SCN_ERR_ARG;
+        }
+    } else if (sr->type == SCN_SRC_MATERIAL) {
+        if (sr->mat < 1 || sr->mat > pr->mg.n_mat) {
+            B->err = "Source material was not found in the material definitions";
+            return SCN_ERR_ARG;
+        }
+        if (sr->G < 1 || sr->G > nG) {
+            B->err = "source group outside 1..nG";
+            return SCN_ERR_ARG;
+        }
+        if (sr->t_high < sr->t_low) {
+            B->err = "tHigh is less than tLow";
+            return SCN_ERR_ARG;
+        }
+    } else {
+        B->err = "unknown source type";
+        return SCN_ERR_ARG;
+    }
+    return 0;
+}
+
+int check_bstatus(Batch* B)
+{
+    DevStatus h{};
+    BHIPCHK(hipMemcpyAsync(&h, B->d_status, sizeof h, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    if (h.code) {
+        char buf[256];
+        const char* what = "engine error";
+        switch (h.code) {
+        case SCN_ERR_OVERFLOW: what = "Run out of space for particles (slot queue)"; break;
+        case SCN_ERR_MATERIAL: what = "Particle is in an undefined or overlapping material"; break;
+        case SCN_ERR_NAN: what = "Distance is NaN"; break;
+        case SCN_ERR_MAJORANT: what = "Majorant is 0"; break;
+        case SCN_ERR_SAMPLING: what = "Sampling failed (source rejection, probG, scatter CDF)"; break;
+        case SCN_ERR_UNSUPPORTED: what = "Unsupported reaction in collision"; break;
+        case SCN_ERR_ARG: what = "Source outside the geometry or particle lost"; break;
+        case SCN_ERR_NORM: what = "Normalisation score is 0"; break;
+        }
+        snprintf(buf, sizeof buf, "%s [detail %d]", what, h.detail);
+        B->err = buf;
+        DevStatus z{};
+        BHIPCHK(hipMemcpyAsync(B->d_status, &z, sizeof z, hipMemcpyHostToDevice, B->stream));
+        BHIPCHK(hipStreamSynchronize(B->stream));
+        return h.code;
+    }
+    return 0;
+}
+
+int max_depth_nl(Batch* B)
+{
+    int d = 1;
+    for (const SlotHost& h : B->hs)
+        if (h.used) d = std::max(d, h.depth);
+    return compiled_nl(d);
+}
+
+// persistent grid of the transport kernel variant: resident blocks per CU x CUs
+int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
+{
+    const void* kern;
+    if (B->S.transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(nl, fn, &kern);
+    else pick_bt<SCN_TRANSPORT_ST>(nl, fn, &kern);
+    int occ = 0;
+    BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, 0));
+    *grid = B->n_cu * std::max(1, occ);
+    return 0;
+}
+
+// enqueue one cycle on B->stream (also the body of the captured graph); timed = per-launch events
+int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
+{
+    const int ns = B->n_slots;
+    const int tile = B->S.tile;
+    int side = 0;
+    for (int L = 0; L < B->S.n_launch; ++L) {
+        const int first = L == 0;
+        const int in_side = first ? 0 : side;
+        const int out_side = first ? 0 : 1 - side;
+        hipLaunchKernelGGL(k_tiles, dim3(1), dim3(1024), 0, B->stream, B->d_slots, B->d_state, ns, first, in_side,
+                           out_side, tile, B->d_tiles, B->tiles_cap, B->d_ctl);
+        const int last = L == B->S.n_launch - 1;
+        const int steps = last ? (1 << 30) : (B->S.k0 << L);
+        if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
+        fn(dim3(grid), dim3(BBLOCK), B->stream, B->d_slots, B->d_state, B->d_tiles, B->d_ctl, first, in_side, out_side,
+           tile, steps, B->d_status);
+        if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L + 1], B->stream));
+        side = out_side;
+    }
+    const int sb = (ns + 255) / 256;
+    hipLaunchKernelGGL(k_bnorm, dim3(sb), dim3(256), 0, B->stream, B->d_slots, B->d_state, ns, B->d_status);
+    hipLaunchKernelGGL(k_bclose, dim3(16, ns), dim3(256), 0, B->stream, B->d_slots, B->d_state);
+    hipLaunchKernelGGL(k_bend, dim3(sb), dim3(256), 0, B->stream, B->d_slots, B->d_state, ns, B->d_ctl);
+    BHIPCHK(hipGetLastError());
+    return 0;
+}
+
+void drop_graph(Batch* B)
+{
+    if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->gexec = nullptr;
+    B->graph = nullptr;
+}
+
+int run_cycle(Batch* B, int* n_running_out)
+{
+    const int nl = max_depth_nl(B);
+    BTransportFn fn;
+    int grid = 0;
+    int rc0 = transport_variant(B, nl, &fn, &grid);
+    if (rc0) return rc0;
+    BHIPCHK(hipEventRecord(B->c0, B->stream));
+    if (B->S.use_graph) {
+        if (!B->gexec || B->nl_cur != nl) {
+            drop_graph(B);
+            BHIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_cycle(B, fn, grid, 0);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(B->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                B->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+                return SCN_ERR_HIP;
+            }
+            B->graph = g;
+            BHIPCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+            B->nl_cur = nl;
+        }
+        BHIPCHK(hipGraphLaunch(B->gexec, B->stream));
+    } else {
+        int rc = enqueue_cycle(B, fn, grid, 1);
+        if (rc) return rc;
+    }
+    BHIPCHK(hipEventRecord(B->c1, B->stream));
+    int n_running = 0, ovf = 0;
+    BHIPCHK(hipMemcpyAsync(&n_running, B->d_ctl + 2, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipMemcpyAsync(&ovf, B->d_ctl + 3, sizeof(int), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    float ms = 0.f;
+    BHIPCHK(hipEventElapsedTime(&ms, B->c0, B->c1));
+    B->stats.cycle_ms += ms;
+    B->stats.n_cycles += 1;
+    if (!B->S.use_graph) {
+        for (int L = 0; L < B->S.n_launch; ++L) {
+            float m = 0.f;
+            BHIPCHK(hipEventElapsedTime(&m, B->ev[2 * L], B->ev[2 * L + 1]));
+            B->stats.transport_kernel_ms += m;
+        }
+    }
+    B->stats.transport_launches += B->S.n_launch;
+    int rc = check_bstatus(B);
+    if (rc) return rc;
+    if (ovf) {
+        B->err = "tile table overflow";
+        return SCN_ERR_OVERFLOW;
+    }
+    *n_running_out = n_running;
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+static thread_local std::string g_batch_create_err;
+
+const char* scn_batch_last_error(const void* batch)
+{
+    if (!batch) return g_batch_create_err.c_str();
+    return ((const Batch*)batch)->err.c_str();
+}
+
+int scn_batch_create(const scn_batch_settings* settings, void** out)
+{
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_batch_create_err = "no GPU visible to HIP";
+        return SCN_ERR_NO_DEVICE;
+    }
+    if (!settings || settings->max_slots < 1 || settings->max_slots > MAX_SLOTS || settings->hbm_budget < (1 << 20) ||
+        (settings->transport != SCN_TRANSPORT_ST && settings->transport != SCN_TRANSPORT_DT) ||
+        (settings->tile != 0 && (settings->tile % BWAVE != 0 || settings->tile > 4096)) || settings->k0 < 0 ||
+        settings->n_launch < 0 || settings->n_launch > 16) {
+        g_batch_create_err = "invalid batch settings";
+        return SCN_ERR_ARG;
+    }
+    Batch* B = new Batch();
+    B->S = *settings;
+    if (B->S.tile == 0) B->S.tile = 64;
+    if (B->S.k0 == 0) B->S.k0 = 8;
+    if (B->S.n_launch == 0) B->S.n_launch = 4;
+    B->n_slots = settings->max_slots;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) B->n_cu = prop.multiProcessorCount;
+    auto fail = [&](const std::string& m, int code) {
+        g_batch_create_err = m;
+        scn_batch_destroy(B);
+        return code;
+    };
+    if (hipStreamCreate(&B->stream) != hipSuccess) return fail("hipStreamCreate failed", SCN_ERR_HIP);
+    if (hipEventCreate(&B->c0) != hipSuccess || hipEventCreate(&B->c1) != hipSuccess)
+        return fail("hipEventCreate failed", SCN_ERR_HIP);
+    B->ev.assign(2 * B->S.n_launch, nullptr);
+    for (auto& e : B->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail("hipEventCreate failed", SCN_ERR_HIP);
+    B->arena_size = (size_t)settings->hbm_budget;
+    if (hipMalloc((void**)&B->arena, B->arena_size) != hipSuccess)
+        return fail("hipMalloc of the HBM budget failed", SCN_ERR_HIP);
+    B->freel[0] = B->arena_size & ~(size_t)255;
+    // tiles: a history of pop takes two queue entries of more than 100 B each, plus one partial
+    // tile per slot
+    size_t max_hist = B->arena_size / 128;
+    B->tiles_cap = (int)std::min<size_t>(max_hist / BWAVE + (size_t)B->n_slots + 16, (size_t)1 << 30);
+    if (hipMalloc((void**)&B->d_tiles, sizeof(int2) * (size_t)B->tiles_cap) != hipSuccess ||
+        hipMalloc((void**)&B->d_slots, sizeof(SlotDev) * B->n_slots) != hipSuccess ||
+        hipMalloc((void**)&B->d_state, sizeof(SlotState) * B->n_slots) != hipSuccess ||
+        hipMalloc((void**)&B->d_ctl, sizeof(int) * CTL_WORDS) != hipSuccess ||
+        hipMalloc((void**)&B->d_status, sizeof(DevStatus)) != hipSuccess)
+        return fail("hipMalloc failed", SCN_ERR_HIP);
+    B->h_slots.assign(B->n_slots, SlotDev{});
+    B->hs.assign(B->n_slots, SlotHost{});
+    if (hipMemsetAsync(B->d_state, 0, sizeof(SlotState) * B->n_slots, B->stream) != hipSuccess ||
+        hipMemsetAsync(B->d_slots, 0, sizeof(SlotDev) * B->n_slots, B->stream) != hipSuccess ||
+        hipMemsetAsync(B->d_ctl, 0, sizeof(int) * CTL_WORDS, B->stream) != hipSuccess ||
+        hipMemsetAsync(B->d_status, 0, sizeof(DevStatus), B->stream) != hipSuccess ||
+        hipStreamSynchronize(B->stream) != hipSuccess)
+        return fail("device initialisation failed", SCN_ERR_HIP);
+    *out = B;
+    return 0;
+}
+
+void scn_batch_destroy(void* batch)
+{
+    Batch* B = (Batch*)batch;
+    if (!B) return;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    drop_graph(B);
+    if (B->arena) (void)hipFree(B->arena);
+    if (B->d_tiles) (void)hipFree(B->d_tiles);
+    if (B->d_slots) (void)hipFree(B->d_slots);
+    if (B->d_state) (void)hipFree(B->d_state);
+    if (B->d_ctl) (void)hipFree(B->d_ctl);
+    if (B->d_status) (void)hipFree(B->d_status);
+    for (auto e : B->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (B->c0) (void)hipEventDestroy(B->c0);
+    if (B->c1) (void)hipEventDestroy(B->c1);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || !R || !slot) return SCN_ERR_ARG;
+    *slot = -1;
+    int depth = 0;
+    int rc = validate(B, R, &depth);
+    if (rc) return rc;
+    int s = -1;
+    for (int i = 0; i < B->n_slots; ++i)
+        if (!B->hs[i].used) {
+            s = i;
+            break;
+        }
+    if (s < 0) {
+        B->err = "no free slot";
+        return SCN_ERR_BUSY;
+    }
+    const scn_problem* pr = R->problem;
+    const scn_source* sr = R->source;
+    const int nG = pr->mg.n_groups, nM = pr->mg.n_mat;
+    // table image
+    std::vector<int> goff, gidx;
+    group_clerks(pr, goff, gidx);
+    Image I;
+    size_t o_unis = I.put(pr->universes, pr->n_universes);
+    size_t o_boxes = I.put(pr->boxes, pr->n_boxes);
+    size_t o_gf = I.put(pr->graph_fill, pr->graph_size);
+    size_t o_gi = I.put(pr->graph_id, pr->graph_size);
+    size_t o_real = I.put(pr->real_data, pr->n_real);
+    size_t o_ints = I.put(pr->int_data, pr->n_int);
+    size_t o_xs = I.put(pr->mg.xs, (size_t)nM * nG * 6);
+    size_t o_P0 = I.put(pr->mg.P0, (size_t)nM * nG * nG);
+    size_t o_prod = I.put(pr->mg.prod, (size_t)nM * nG * nG);
+    size_t o_chi = I.put(pr->mg.chi, (size_t)nM * nG);
+    size_t o_fiss = I.put(pr->mg.fissile, (size_t)nM);
+    size_t o_maj = I.put(pr->mg.majorant, (size_t)nG);
+    size_t o_clerks = I.put(pr->clerks, (size_t)pr->n_clerks);
+    size_t o_goff = I.put(goff.data(), goff.size());
+    size_t o_gidx = I.put(gidx.data(), gidx.size());
+    // pointSource%init normalisation (pointSource_class.f90:120-128,150-152)
+    std::vector<double> probG;
+    double dir[3] = {sr->dir[0], sr->dir[1], sr->dir[2]};
+    if (sr->type == SCN_SRC_POINT && !sr->isotropic) {
+        double n = scn_norm2(sr->dir);
+        for (int k = 0; k < 3; ++k) dir[k] = sr->dir[k] / n;
+    }
+    if (sr->type == SCN_SRC_POINT && sr->n_probG > 0) {
+        double t = 0.0;
+        for (int g = 0; g < sr->n_probG; ++g) t = t + sr->probG[g];
+        probG.resize(sr->n_probG);
+        for (int g = 0; g < sr->n_probG; ++g) probG[g] = sr->probG[g] / t;
+    }
+    size_t o_probG = I.put(probG.data(), probG.size());
+    SlotLayout Ly;
+    Ly.tables = align256(I.bytes.size());
+    Ly.queue = Ly.tables;
+    Ly.queue_bytes = live_bytes(depth, R->pop);
+    Ly.bins = Ly.queue + 2 * Ly.queue_bytes;
+    const size_t nb = (size_t)std::max<int64_t>(1, pr->n_bins);
+    size_t bins_bytes = 3 * align256(sizeof(double) * nb);
+    Ly.hist = Ly.bins + bins_bytes;
+    size_t nh = R->record ? (size_t)R->n_batches * (size_t)R->pop : 0;
+    size_t hist_bytes = R->record ? align256(sizeof(uint64_t) * nh) + align256(sizeof(int) * nh) : 0;
+    Ly.total = Ly.hist + hist_bytes;
+    size_t off = arena_alloc(B, Ly.total);
+    if (off == SIZE_MAX) {
+        B->err = "not enough of the HBM budget free for this request";
+        return SCN_ERR_BUSY;
+    }
+    char* base = B->arena + off;
+    SlotDev D{};
+    DevProblem& P = D.P;
+    P.transport = pr->transport;
+    P.st_cache = pr->st_cache;
+    P.root_uni = pr->root_uni;
+    P.border_box = pr->border_box;
+    P.n_mat = nM;
+    P.nG = nG;
+    P.n_clerks = pr->n_clerks;
+    P.graph_size = pr->graph_size;
+    P.unis = (const scn_universe*)(base + o_unis);
+    P.boxes = (const scn_box*)(base + o_boxes);
+    P.gfill = (const int*)(base + o_gf);
+    P.gid = (const int*)(base + o_gi);
+    P.real = (const double*)(base + o_real);
+    P.ints = (const int*)(base + o_ints);
+    P.xs = (const double*)(base + o_xs);
+    P.P0 = (const double*)(base + o_P0);
+    P.prod = (const double*)(base + o_prod);
+    P.chi = (const double*)(base + o_chi);
+    P.fissile = (const int*)(base + o_fiss);
+    P.maj = (const double*)(base + o_maj);
+    P.collision_xs = pr->mg.collision_xs;
+    P.clerks = (const scn_clerk*)(base + o_clerks);
+    P.gclerk_off = (const int*)(base + o_goff);
+    P.gclerk_idx = (const int*)(base + o_gidx);
+    P.n_bins = pr->n_bins;
+    carve_live(base + Ly.queue, depth, R->pop, D.q[0]);
+    carve_live(base + Ly.queue + Ly.queue_bytes, depth, R->pop, D.q[1]);
+    D.bin = (double*)(base + Ly.bins);
+    D.csum = (double*)(base + Ly.bins + align256(sizeof(double) * nb));
+    D.csum2 = (double*)(base + Ly.bins + 2 * align256(sizeof(double) * nb));
+    D.hist_rng = R->record ? (uint64_t*)(base + Ly.hist) : nullptr;
+    D.hist_coll = R->record ? (int*)(base + Ly.hist + align256(sizeof(uint64_t) * nh)) : nullptr;
+    D.probG = probG.empty() ? nullptr : (const double*)(base + o_probG);
+    for (int k = 0; k < 3; ++k) {
+        D.src_r[k] = sr->r[k];
+        D.src_dir[k] = dir[k];
+        D.bottom[k] = sr->bottom[k];
+        D.top[k] = sr->top[k];
+    }
+    D.t_low = sr->t_low;
+    D.t_high = sr->t_high;
+    D.stop_rel = R->stop_rel;
+    D.stop_bin = R->stop_bin;
+    D.norm_bin = pr->norm_bin;
+    D.norm_value = pr->norm_value;
+    D.seed = (uint64_t)R->seed;
+    D.src_type = sr->type;
+    D.src_iso = sr->isotropic;
+    D.src_G = sr->G;
+    D.src_mat = sr->mat;
+    D.n_probG = (int)probG.size();
+    D.pop = R->pop;
+    D.n_batches = R->n_batches;
+    SlotState st{};
+    st.running = 1;
+    auto undo = [&](int code) {
+        arena_free(B, off, Ly.total);
+        return code;
+    };
+    if (hipMemcpyAsync(base, I.bytes.data(), I.bytes.size(), hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipMemsetAsync(base + Ly.bins, 0, bins_bytes, B->stream) != hipSuccess ||
+        hipMemcpyAsync(B->d_slots + s, &D, sizeof D, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipMemcpyAsync(B->d_state + s, &st, sizeof st, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipStreamSynchronize(B->stream) != hipSuccess) {
+        B->err = "upload of the request failed";
+        return undo(SCN_ERR_HIP);
+    }
+    B->h_slots[s] = D;
+    SlotHost& h = B->hs[s];
+    h.used = 1;
+    h.off = off;
+    h.bytes = Ly.total;
+    h.depth = depth;
+    h.n_bins = pr->n_bins;
+    h.pop = R->pop;
+    h.n_batches = R->n_batches;
+    h.record = R->record;
+    *slot = s;
+    return 0;
+}
+
+int scn_batch_cycle(void* batch, int32_t n_cycles, int32_t* n_running)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || n_cycles < 0) return SCN_ERR_ARG;
+    int running = 0;
+    for (int c = 0; c < n_cycles; ++c) {
+        int rc = run_cycle(B, &running);
+        if (rc) return rc;
+        if (running == 0) break;
+    }
+    if (n_cycles == 0) {
+        std::vector<SlotState> ss(B->n_slots);
+        BHIPCHK(hipMemcpyAsync(ss.data(), B->d_state, sizeof(SlotState) * B->n_slots, hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipStreamSynchronize(B->stream));
+        for (int i = 0; i < B->n_slots; ++i) running += (B->hs[i].used && ss[i].running) ? 1 : 0;
+    }
+    if (n_running) *n_running = running;
+    return 0;
+}
+
+int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bins, uint64_t* hist_rng,
+                    int32_t* hist_coll)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || slot < 0 || slot >= B->n_slots) return SCN_ERR_ARG;
+    const SlotHost& h = B->hs[slot];
+    scn_fixed_result r{};
+    if (!h.used) {
+        r.state = SCN_SLOT_FREE;
+        if (res) *res = r;
+        return 0;
+    }
+    SlotState st{};
+    BHIPCHK(hipMemcpyAsync(&st, B->d_state + slot, sizeof st, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    r.state = st.running ? SCN_SLOT_RUNNING : SCN_SLOT_DONE;
+    r.n_batches = st.batch;
+    r.n_bins = h.n_bins;
+    r.n_histories = (int64_t)st.histories;
+    r.n_collisions = (int64_t)st.collisions;
+    r.n_segments = (int64_t)st.segments;
+    if (res) *res = r;
+    if (r.state != SCN_SLOT_DONE) return 0;
+    const SlotDev& D = B->h_slots[slot];
+    if (bins && h.n_bins > 0) {
+        std::vector<double> cs(h.n_bins), cs2(h.n_bins);
+        BHIPCHK(hipMemcpyAsync(cs.data(), D.csum, sizeof(double) * h.n_bins, hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipMemcpyAsync(cs2.data(), D.csum2, sizeof(double) * h.n_bins, hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipStreamSynchronize(B->stream));
+        const int N = std::max(1, st.batch);
+        for (int64_t i = 0; i < h.n_bins; ++i) score_result(cs[i], cs2[i], N, bins + 2 * i);
+    }
+    if (h.record && (hist_rng || hist_coll)) {
+        size_t nh = (size_t)st.batch * (size_t)h.pop;
+        if (hist_rng) BHIPCHK(hipMemcpyAsync(hist_rng, D.hist_rng, sizeof(uint64_t) * nh, hipMemcpyDeviceToHost, B->stream));
+        if (hist_coll) BHIPCHK(hipMemcpyAsync(hist_coll, D.hist_coll, sizeof(int) * nh, hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipStreamSynchronize(B->stream));
+    }
+    return 0;
+}
+
+int scn_batch_states(void* batch, int32_t* states, int32_t n)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || !states || n < 0 || n > B->n_slots) return SCN_ERR_ARG;
+    if (n == 0) return 0;
+    std::vector<SlotState> ss(n);
+    BHIPCHK(hipMemcpyAsync(ss.data(), B->d_state, sizeof(SlotState) * n, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    for (int i = 0; i < n; ++i)
+        states[i] = !B->hs[i].used ? SCN_SLOT_FREE : (ss[i].running ? SCN_SLOT_RUNNING : SCN_SLOT_DONE);
+    return 0;
+}
+
+int scn_batch_release(void* batch, int32_t slot)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || slot < 0 || slot >= B->n_slots) return SCN_ERR_ARG;
+    SlotHost& h = B->hs[slot];
+    if (!h.used) return 0;
+    SlotState st{};
+    BHIPCHK(hipMemcpyAsync(&st, B->d_state + slot, sizeof st, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    if (st.running) {
+        B->err = "slot still running";
+        return SCN_ERR_ARG;
+    }
+    B->rel_seg += (int64_t)st.segments;
+    B->rel_col += (int64_t)st.collisions;
+    B->rel_sc += (int64_t)st.scores;
+    B->rel_hist += (int64_t)st.histories;
+    SlotState z{};
+    BHIPCHK(hipMemcpyAsync(B->d_state + slot, &z, sizeof z, hipMemcpyHostToDevice, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    arena_free(B, h.off, h.bytes);
+    h = SlotHost{};
+    B->h_slots[slot] = SlotDev{};
+    return 0;
+}
+
+int scn_batch_stats(void* batch, scn_stats* out)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || !out) return SCN_ERR_ARG;
+    std::vector<SlotState> ss(B->n_slots);
+    BHIPCHK(hipMemcpyAsync(ss.data(), B->d_state, sizeof(SlotState) * B->n_slots, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    *out = B->stats;
+    int64_t seg = B->rel_seg, col = B->rel_col, sc = B->rel_sc, hist = B->rel_hist;
+    for (int i = 0; i < B->n_slots; ++i) {
+        if (!B->hs[i].used) continue;
+        seg += (int64_t)ss[i].segments;
+        col += (int64_t)ss[i].collisions;
+        sc += (int64_t)ss[i].scores;
+        hist += (int64_t)ss[i].histories;
+    }
+    out->n_segments = seg;
+    out->n_collisions = col;
+    out->n_scores = sc;
+    out->n_histories = hist;
+    return 0;
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@
 #include "scone_gpu.h"
 #include "scn_fpmath.h"
 #include "scn_device.h"
+#include "scn_host.h"
 
 using namespace scn;
 
@@ -703,14 +704,6 @@ TransportFn pick_nl(int nl)
     }
 }
 
-int compiled_nl(int depth)
-{
-    if (depth <= 1) return 1;
-    if (depth <= 3) return 3;
-    if (depth <= 4) return 4;
-    return 8;
-}
-
 // transport one cycle's bank (E->bankA, n_in histories) into E->bankB (sites sorted by brood);
 // returns the number of sites in *n_sites and the per-history keff scores in E->hist_k.
 int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_sites, int score_bins)
@@ -895,16 +888,6 @@ int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
     return 0;
 }
 
-void result2(double csum, double csum2, int N, double out[2])
-{
-    double mean = csum / N;
-    double inv_N = 1.0 / N;
-    double inv_Nm1 = (N != 1) ? 1.0 / (N - 1) : 1.0;
-    double std = csum2 * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
-    out[0] = mean;
-    out[1] = sqrt(std);
-}
-
 }  // namespace
 
 // =============================================================================================
@@ -952,28 +935,10 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     E->n_bins = pr->n_bins;
     E->norm_bin = pr->norm_bin;
     E->norm_value = pr->norm_value;
-    // nesting depth of the geometry graph
     {
-        std::vector<int> depth(pr->n_universes + 1, 0);
-        // uniIdx -> first graph position from any parent reference (rootID)
-        std::vector<int> rootpos(pr->n_universes + 1, 0);
-        rootpos[pr->root_uni] = 1;
-        for (int i = 0; i < pr->graph_size; ++i)
-            if (pr->graph_fill[i] < 0) rootpos[-pr->graph_fill[i]] = pr->graph_id[i];
-        std::function<int(int)> dep = [&](int u) -> int {
-            int best = 0;
-            const scn_universe& U = pr->universes[u - 1];
-            for (int l = 1; l <= U.n_local; ++l) {
-                int pos = rootpos[u] + l - 1;
-                if (pos < 1 || pos > pr->graph_size) continue;
-                int f = pr->graph_fill[pos - 1];
-                if (f < 0) best = std::max(best, dep(-f));
-            }
-            return best + 1;
-        };
-        int d = dep(pr->root_uni);
-        if (d > SCN_MAX_NEST) {
-            g_create_err = "geometry nesting deeper than SCN_MAX_NEST";
+        int d = problem_depth(pr);
+        if (d < 1 || d > SCN_MAX_NEST) {
+            g_create_err = "geometry nesting deeper than SCN_MAX_NEST or malformed graph";
             delete E;
             return SCN_ERR_UNSUPPORTED;
         }
@@ -990,15 +955,8 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     double *d_real = nullptr, *d_xs = nullptr, *d_P0 = nullptr, *d_prod = nullptr, *d_chi = nullptr, *d_maj = nullptr;
     scn_clerk* d_clerks = nullptr;
     // clerks admitted per group (energyFilter evaluated once, host side)
-    std::vector<int> goff(nG + 1, 0), gidx;
-    for (int g = 1; g <= nG; ++g) {
-        for (int c = 0; c < pr->n_clerks; ++c) {
-            const scn_clerk& K = pr->clerks[c];
-            if (K.gtop > 0 && !(K.gtop <= g && g <= K.glow)) continue;
-            gidx.push_back(c);
-        }
-        goff[g] = (int)gidx.size();
-    }
+    std::vector<int> goff, gidx;
+    group_clerks(pr, goff, gidx);
     if (!rc) rc |= upload(E, &d_unis, pr->universes, pr->n_universes);
     if (!rc) rc |= upload(E, &d_boxes, pr->boxes, pr->n_boxes);
     if (!rc) rc |= upload(E, &d_gf, pr->graph_fill, pr->graph_size);
@@ -1309,8 +1267,8 @@ int scn_eigen_step(void* engine, int32_t n_cycles, int32_t* done)
         X.n_this = nn;
         X.cur = 1 - X.cur;
         double kr[2];
-        if (X.phase == 0) result2(X.kA[0], X.kA[1], X.nA, kr);
-        else result2(X.kI[0], X.kI[1], X.nI, kr);
+        if (X.phase == 0) score_result(X.kA[0], X.kA[1], X.nA, kr);
+        else score_result(X.kI[0], X.kI[1], X.nI, kr);
         X.k_new = kr[0];
         X.dk[1 - X.cur] = X.k_new;
         X.keff_0 = X.k_new;
@@ -1336,10 +1294,10 @@ int scn_eigen_finish(void* engine, scn_eigen_result* res)
     EigenState& X = E->eig;
     res->n_histories = X.n_hist;
     res->transport_time = X.transport_s;
-    if (X.nA) result2(X.kA[0], X.kA[1], X.nA, res->k_analog);
+    if (X.nA) score_result(X.kA[0], X.kA[1], X.nA, res->k_analog);
     if (X.nI) {
-        result2(X.kI[0], X.kI[1], X.nI, res->k_implicit);
-        for (int k = 0; k < 4; ++k) result2(X.imp[k][0], X.imp[k][1], X.nI, res->imp[k]);
+        score_result(X.kI[0], X.kI[1], X.nI, res->k_implicit);
+        for (int k = 0; k < 4; ++k) score_result(X.imp[k][0], X.imp[k][1], X.nI, res->imp[k]);
     }
     for (size_t i = 0; i < X.k_history.size(); ++i) {
         if (res->k_history) res->k_history[i] = X.k_history[i];
@@ -1356,7 +1314,7 @@ int scn_eigen_finish(void* engine, scn_eigen_result* res)
         std::vector<double> cs(E->n_bins), cs2(E->n_bins);
         COPY_SYNC(cs.data(), E->csum[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
         COPY_SYNC(cs2.data(), E->csum2[w], sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
-        for (int64_t i = 0; i < E->n_bins; ++i) result2(cs[i], cs2[i], N, dst + 2 * i);
+        for (int64_t i = 0; i < E->n_bins; ++i) score_result(cs[i], cs2[i], N, dst + 2 * i);
     }
     X.open = 0;
     return 0;

@@ -17,7 +17,11 @@ LIB_PATH = os.path.join(HERE, "libscone_amd.so")
 # exported symbols of include/scone_gpu.h
 EXPORTS = ["scn_create", "scn_destroy", "scn_last_error", "scn_device_info", "scn_eigen_cycle",
            "scn_norm_size_repr", "scn_eigen_run", "scn_eigen_start", "scn_eigen_step", "scn_eigen_finish",
-           "scn_fission_source", "scn_get_stats", "scn_reset_stats"]
+           "scn_fission_source", "scn_get_stats", "scn_reset_stats",
+           "scn_batch_create", "scn_batch_destroy", "scn_batch_last_error", "scn_batch_add", "scn_batch_cycle",
+           "scn_batch_query", "scn_batch_states", "scn_batch_release", "scn_batch_stats"]
+
+SCN_ERR_BUSY = 11
 
 
 class Stats(C.Structure):
@@ -61,6 +65,18 @@ def load():
     L.scn_fission_source.argtypes = [C.c_void_p, C.c_int32, C.c_int64, d3, d3, C.c_int32, C.c_int32, PS]
     L.scn_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.scn_reset_stats.argtypes = [C.c_void_p]
+    L.scn_batch_create.argtypes = [C.POINTER(T.BatchSettings), C.POINTER(C.c_void_p)]
+    L.scn_batch_destroy.argtypes = [C.c_void_p]
+    L.scn_batch_destroy.restype = None
+    L.scn_batch_last_error.restype = C.c_char_p
+    L.scn_batch_last_error.argtypes = [C.c_void_p]
+    L.scn_batch_add.argtypes = [C.c_void_p, C.POINTER(T.FixedRequest), C.POINTER(C.c_int32)]
+    L.scn_batch_cycle.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+    L.scn_batch_query.argtypes = [C.c_void_p, C.c_int32, C.POINTER(T.FixedResult), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_uint64), C.POINTER(C.c_int32)]
+    L.scn_batch_states.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
+    L.scn_batch_release.argtypes = [C.c_void_p, C.c_int32]
+    L.scn_batch_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     _lib = L
     return L
 
@@ -204,3 +220,115 @@ class Engine:
 
     def reset_stats(self):
         self._check(self.L.scn_reset_stats(self.h))
+
+
+class Busy(EngineError):
+    """The batch engine has no free slot or budget for a request right now."""
+
+
+class Batch:
+    """Fixed-source engine with resident problem slots (scn_batch_* of include/scone_gpu.h)."""
+
+    def __init__(self, transport=T.TRANSPORT_DT, max_slots=512, hbm_budget=8 << 30, tile=0, k0=0, n_launch=0,
+                 use_graph=1):
+        L = load()
+        S = T.BatchSettings()
+        S.transport = transport
+        S.max_slots = max_slots
+        S.hbm_budget = int(hbm_budget)
+        S.tile = tile
+        S.k0 = k0
+        S.n_launch = n_launch
+        S.use_graph = use_graph
+        self.settings = S
+        h = C.c_void_p()
+        rc = L.scn_batch_create(C.byref(S), C.byref(h))
+        if rc:
+            raise EngineError(f"scn_batch_create failed ({rc}): {L.scn_batch_last_error(None).decode()}")
+        self.h = h
+        self.L = L
+        self._held = {}
+
+    def close(self):
+        if self.h:
+            self.L.scn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            msg = self.L.scn_batch_last_error(self.h).decode()
+            if rc == SCN_ERR_BUSY:
+                raise Busy(msg)
+            raise EngineError(f"batch engine error {rc}: {msg}")
+
+    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False) -> int:
+        """Admit a solve; raises Busy if it does not fit now.  Returns its slot."""
+        R = T.FixedRequest()
+        R.problem = C.pointer(comp.problem)
+        R.source = C.pointer(src.source)
+        R.pop = pop
+        R.n_batches = n_batches
+        R.seed = _s64(seed)
+        R.stop_bin = stop_bin
+        R.stop_rel = stop_rel
+        R.record = 1 if record else 0
+        slot = C.c_int32(-1)
+        self._check(self.L.scn_batch_add(self.h, C.byref(R), C.byref(slot)))
+        self._held[slot.value] = (comp, src, pop, record)
+        return slot.value
+
+    def cycle(self, n=1) -> int:
+        """Run up to n cycles; returns the number of slots still running."""
+        nr = C.c_int32()
+        self._check(self.L.scn_batch_cycle(self.h, n, C.byref(nr)))
+        return nr.value
+
+    def query(self, slot):
+        """(FixedResult, bins (n_bins, 2) or None, hist_rng or None, hist_coll or None)."""
+        r = T.FixedResult()
+        self._check(self.L.scn_batch_query(self.h, slot, C.byref(r), None, None, None))
+        if r.state != T.SLOT_DONE:
+            return r, None, None, None
+        nb = int(r.n_bins)
+        bins = np.zeros(max(1, 2 * nb))
+        comp, src, pop, record = self._held[slot]
+        nh = r.n_batches * pop if record else 0
+        hr = np.zeros(max(1, nh), dtype=np.uint64) if record else None
+        hc = np.zeros(max(1, nh), dtype=np.int32) if record else None
+        self._check(self.L.scn_batch_query(
+            self.h, slot, C.byref(r), bins.ctypes.data_as(C.POINTER(C.c_double)),
+            hr.ctypes.data_as(C.POINTER(C.c_uint64)) if hr is not None else None,
+            hc.ctypes.data_as(C.POINTER(C.c_int32)) if hc is not None else None))
+        return r, bins[: 2 * nb].reshape(-1, 2), (hr[:nh] if hr is not None else None), \
+            (hc[:nh] if hc is not None else None)
+
+    def states(self) -> np.ndarray:
+        """SLOT_* of every slot."""
+        n = self.settings.max_slots
+        out = np.zeros(n, dtype=np.int32)
+        self._check(self.L.scn_batch_states(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return out
+
+    def release(self, slot):
+        self._check(self.L.scn_batch_release(self.h, slot))
+        self._held.pop(slot, None)
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(self.L.scn_batch_stats(self.h, C.byref(s)))
+        return s
+
+    def run_all(self, max_cycles=1 << 30):
+        """Cycle until no slot is running."""
+        n = 0
+        while n < max_cycles:
+            if self.cycle(1) == 0:
+                break
+            n += 1
+        return n

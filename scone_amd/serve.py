@@ -1,0 +1,130 @@
+"""Continuous-batching serving loop over the batch engine (scn_batch_* of include/scone_gpu.h).
+
+Requests are fixed-source solves.  A request is admitted as soon as a slot and enough of the HBM
+budget are free (scn_batch_add refuses it otherwise and it waits, first come first served).
+Every engine cycle advances all resident solves by one batch together, so the work of requests
+that arrive while others run overlaps on the device.  A solve leaves when its batches are done or
+when the relative standard deviation of its stop bin reaches the target; its slot and memory
+return to the pool at once and the next waiting request takes them.
+
+Each solve's histories depend only on its own seed and tables, so its result is the same alone
+and in any mix (tallies to summation order).
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import tables as T
+from .engine import Batch, Busy
+
+
+@dataclass
+class Request:
+    comp: object            # tables.Compiled
+    src: object             # tables.CompiledSource
+    pop: int
+    n_batches: int
+    seed: int
+    stop_bin: int = -1
+    stop_rel: float = 0.0
+    record: bool = False
+    arrival: float = 0.0    # seconds (wall mode) or cycle index (cycle mode)
+    # filled by the server
+    admitted: float = -1.0
+    done: float = -1.0
+    slot: int = -1
+    result: object = None
+    bins: object = None
+    hist_rng: object = None
+    hist_coll: object = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class Server:
+    def __init__(self, batch: Batch):
+        self.B = batch
+        self.cycles = 0
+        self.busy_rejections = 0
+
+    def _admit(self, waiting, running, now):
+        while waiting:
+            rq = waiting[0]
+            try:
+                rq.slot = self.B.add(rq.comp, rq.src, rq.pop, rq.n_batches, rq.seed, stop_bin=rq.stop_bin,
+                                     stop_rel=rq.stop_rel, record=rq.record)
+            except Busy:
+                self.busy_rejections += 1
+                return
+            rq.admitted = now
+            running[rq.slot] = rq
+            waiting.popleft()
+
+    def _harvest(self, running, now, keep):
+        st = self.B.states()
+        for slot in [s for s in running if st[s] == T.SLOT_DONE]:
+            rq = running.pop(slot)
+            r, bins, hr, hc = self.B.query(slot)
+            rq.result = r
+            rq.done = now()
+            if keep:
+                rq.bins, rq.hist_rng, rq.hist_coll = bins, hr, hc
+            self.B.release(slot)
+
+    def run_cycles(self, requests, keep=True):
+        """Deterministic mode: request.arrival is the cycle before which it is submitted."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        pending = collections.deque(order)
+        waiting = collections.deque()
+        running = {}
+        c = 0
+        while pending or waiting or running:
+            while pending and pending[0].arrival <= c:
+                waiting.append(pending.popleft())
+            self._admit(waiting, running, c)
+            if running:
+                self.B.cycle(1)
+                self.cycles += 1
+            c += 1
+            self._harvest(running, lambda: c, keep)
+        return requests
+
+    def run_wall(self, requests, keep=False, progress=None):
+        """Wall-clock mode: request.arrival is seconds after the start.  Returns the elapsed time."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        pending = collections.deque(order)
+        waiting = collections.deque()
+        running = {}
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        last = 0.0
+        while pending or waiting or running:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            self._admit(waiting, running, t)
+            if running:
+                self.B.cycle(1)
+                self.cycles += 1
+                self._harvest(running, now, keep)
+            elif pending:
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+            if progress and now() - last > 20.0:
+                last = now()
+                progress(last, len(pending), len(waiting), len(running))
+        return now()
+
+
+def latency_summary(requests):
+    lat = np.array([r.latency for r in requests if r.done >= 0])
+    if lat.size == 0:
+        return {"n": 0}
+    return {"n": int(lat.size), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

@@ -113,6 +113,34 @@ class EigenResult(C.Structure):
                 ("n_collisions", C.c_int64), ("n_events", C.c_int64), ("n_histories", C.c_int64)]
 
 
+SRC_POINT = 1
+SRC_MATERIAL = 2
+SLOT_FREE, SLOT_RUNNING, SLOT_DONE = 0, 1, 2
+
+
+class Source(C.Structure):
+    _fields_ = [("type", C.c_int32), ("isotropic", C.c_int32), ("G", C.c_int32), ("mat", C.c_int32),
+                ("n_probG", C.c_int32), ("pad", C.c_int32), ("r", C.c_double * 3), ("dir", C.c_double * 3),
+                ("bottom", C.c_double * 3), ("top", C.c_double * 3), ("t_low", C.c_double),
+                ("t_high", C.c_double), ("probG", C.POINTER(C.c_double))]
+
+
+class FixedRequest(C.Structure):
+    _fields_ = [("problem", C.POINTER(Problem)), ("source", C.POINTER(Source)), ("pop", C.c_int32),
+                ("n_batches", C.c_int32), ("seed", C.c_int64), ("stop_bin", C.c_int64),
+                ("stop_rel", C.c_double), ("record", C.c_int32), ("pad", C.c_int32)]
+
+
+class FixedResult(C.Structure):
+    _fields_ = [("state", C.c_int32), ("n_batches", C.c_int32), ("n_bins", C.c_int64),
+                ("n_histories", C.c_int64), ("n_collisions", C.c_int64), ("n_segments", C.c_int64)]
+
+
+class BatchSettings(C.Structure):
+    _fields_ = [("transport", C.c_int32), ("max_slots", C.c_int32), ("hbm_budget", C.c_int64),
+                ("tile", C.c_int32), ("k0", C.c_int32), ("n_launch", C.c_int32), ("use_graph", C.c_int32)]
+
+
 def _ptr(arr, ctype):
     return arr.ctypes.data_as(C.POINTER(ctype))
 
@@ -495,6 +523,8 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     out.active_mats = active
 
     # --- tallies
+    if inp.get("type") == "fixedSourcePhysicsPackage" and tally_key == "activeTally":
+        tally_key = "tally"
     tal = inp.get(tally_key, {}) or {}
     clerks = []
     addr = 0
@@ -646,6 +676,81 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     out.mats = mats
     out.graph = (arr_fill, arr_id)
     out.uni_idx = uni_idx
+    return out
+
+
+class CompiledSource:
+    """A ctypes Source and the arrays behind it."""
+
+    def __init__(self):
+        self.source = Source()
+        self.keep = []
+
+
+def compile_source(sd: dict, comp: Compiled) -> CompiledSource:
+    """Compile a fixed-source ``source`` dictionary (sourceFactory: pointSource_class.f90:73-158,
+    materialSource_class.f90:82-148) against the compiled problem ``comp``.  Values stay as
+    written; the engine normalises ``dir`` and ``probG`` as pointSource%init does."""
+    out = CompiledSource()
+    S = out.source
+    t = sd["type"]
+    if t == "pointSource":
+        if sd.get("particle", "neutron") != "neutron":
+            raise NotImplementedError("photon sources are outside the engine's scope")
+        S.type = SRC_POINT
+        r = [float(x) for x in sd["r"]]
+        if len(r) != 3:
+            raise ValueError("Source position must have three components")
+        S.r[:] = r
+        S.isotropic = 0 if "dir" in sd else 1
+        if "dir" in sd:
+            d = [float(x) for x in sd["dir"]]
+            if len(d) != 3:
+                raise ValueError("Source direction must have three components")
+            S.dir[:] = d
+        if "E" in sd:
+            raise NotImplementedError("continuous-energy sources are outside the engine's scope")
+        if "probG" in sd and "G" in sd:
+            raise ValueError("Source may be either monoenergetic or a distribution, not both")
+        if "probG" in sd:
+            pg = np.asarray([float(x) for x in sd["probG"]], dtype=np.float64)
+            if pg.size != comp.nG:
+                raise ValueError(f"Source energy group distribution must have {comp.nG} groups")
+            out.keep.append(pg)
+            S.n_probG = pg.size
+            S.probG = _ptr(pg, C.c_double)
+        elif "G" in sd:
+            S.G = int(sd["G"])
+        else:
+            raise ValueError("Must specify source energy, either Energy(E) or Group distribution (probG)")
+    elif t == "materialSource":
+        if sd.get("data", "ce") != "mg":
+            raise ValueError("materialSource of an MG problem needs 'data mg;'")
+        S.type = SRC_MATERIAL
+        S.G = int(sd.get("G", 1))
+        name = sd["mat"]
+        if name not in comp.mat_names:
+            raise ValueError(f"Source material {name} was not found in the material definitions")
+        S.mat = comp.mat_names.index(name) + 1
+        if "boundingBox" in sd:
+            bb = [float(x) for x in sd["boundingBox"]]
+            if len(bb) != 6:
+                raise ValueError("Bounding box must have 6 entries")
+        else:
+            bb = list(comp.bounds)
+        S.bottom[:] = bb[0:3]
+        S.top[:] = bb[3:6]
+        if "boundingTime" in sd:
+            bt = [float(x) for x in sd["boundingTime"]]
+            if len(bt) != 2:
+                raise ValueError("Bounding time must have 2 entries")
+            if bt[1] < bt[0]:
+                raise ValueError("tHigh is less than tLow")
+            S.t_low, S.t_high = bt
+    else:
+        raise NotImplementedError(f"source type {t} is outside the engine's scope")
+    if S.G < 0 or S.G > comp.nG:
+        raise ValueError("source group outside 1..nG")
     return out
 
 

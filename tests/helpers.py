@@ -49,3 +49,23 @@ def geometry_problem(universes, surfaces, boundary=(0, 0, 0, 0, 0, 0), materials
         "activeTally": {},
     }
     return T.compile_problem(inp, {m: dummy_mg() for m in materials})
+
+
+def assert_results_close(ref, got, n_batches, rtol=1e-12, label=""):
+    """(mean, sd) score results that differ only by summation order: means within rtol; each sd
+    within rtol times the condition number of scoreMemory%getResult's
+    sd = sqrt(CSUM2/(N(N-1)) - mean^2/(N-1)), which cancels:
+    cond = (1 + 3 mean^2 / ((N-1) sd^2)) / 2."""
+    ref = np.asarray(ref, dtype=float).reshape(-1, 2)
+    got = np.asarray(got, dtype=float).reshape(-1, 2)
+    ok, worst = rel_close(ref[:, 0], got[:, 0], rtol)
+    assert ok, f"{label}: means differ, worst rel {worst}"
+    N = n_batches
+    if N < 2:
+        return
+    m, s = ref[:, 0], ref[:, 1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        cond = np.where(s > 0, 0.5 * (1.0 + 3.0 * m * m / ((N - 1) * s * s)), 1.0)
+    tol = rtol * np.maximum(cond, 1.0) * np.maximum(np.abs(s), np.abs(got[:, 1]))
+    bad = np.nonzero(np.abs(s - got[:, 1]) > tol + 1e-300)[0]
+    assert len(bad) == 0, f"{label}: sd differs beyond its conditioning at bins {bad[:5]}: {s[bad[:3]]} vs {got[bad[:3], 1]}"

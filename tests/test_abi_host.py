@@ -29,7 +29,7 @@ def test_library_exports_every_declared_entry_point():
     assert sorted(engine.EXPORTS) == names
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
     src = r'''
 #include <stdio.h>
 #include <stddef.h>
@@ -40,10 +40,11 @@ int main(void) {
          sizeof(scn_eigen_settings), sizeof(scn_eigen_result), sizeof(scn_stats));
   printf("%zu %zu %zu\n", offsetof(scn_particle_state, time), offsetof(scn_particle_state, broodID),
          offsetof(scn_problem, mg));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(scn_source), sizeof(scn_fixed_request), sizeof(scn_fixed_result),
+         sizeof(scn_batch_settings), offsetof(scn_source, probG), offsetof(scn_fixed_request, stop_rel));
   return 0; }
 '''
-    tmp = os.path.join(ROOT, "_work", "layout_probe")
-    os.makedirs(os.path.dirname(tmp), exist_ok=True)
+    tmp = str(tmp_path / "layout_probe")
     with open(tmp + ".c", "w") as f:
         f.write(src)
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), tmp + ".c", "-o", tmp], check=True)
@@ -55,6 +56,9 @@ int main(void) {
     assert sizes == py
     offs = [int(x) for x in out[1].split()]
     assert offs == [T.ParticleState.time.offset, T.ParticleState.broodID.offset, T.Problem.mg.offset]
+    fixed = [int(x) for x in out[2].split()]
+    assert fixed == [C.sizeof(T.Source), C.sizeof(T.FixedRequest), C.sizeof(T.FixedResult),
+                     C.sizeof(T.BatchSettings), T.Source.probG.offset, T.FixedRequest.stop_rel.offset]
 
 
 def test_engine_fails_loudly_without_gpu():
@@ -97,24 +101,32 @@ program drive
   use iso_c_binding
   use scone_gpu_mod
   implicit none
-  type(c_ptr) :: eng
-  integer(c_int) :: rc
+  type(c_ptr) :: eng, bat
+  integer(c_int) :: rc, rc2
   type(scn_particle_state) :: p
+  type(scn_source) :: src
+  type(scn_fixed_request) :: req
+  type(scn_fixed_result) :: res
+  type(scn_batch_settings) :: bs
   rc = scn_create(c_null_ptr, 10_c_int32_t, eng)
-  print '(I0,1X,I0)', rc, c_sizeof(p)
+  bs % transport = SCN_TRANSPORT_DT
+  bs % max_slots = 0
+  bs % hbm_budget = 0
+  rc2 = scn_batch_create(bs, bat)
+  print '(I0,1X,I0,1X,I0,1X,I0,1X,I0,1X,I0,1X,I0)', rc, c_sizeof(p), rc2, c_sizeof(src), c_sizeof(req), &
+        c_sizeof(res), c_sizeof(bs)
 end program drive
 """
 
 
-def test_fortran_binding_links_and_calls_the_library():
+def test_fortran_binding_links_and_calls_the_library(tmp_path):
     """The iso_c_binding module compiles with the image's Fortran compiler (amdflang) and calls
     into libscone_amd.so: scn_create with no problem tables must return an error code."""
     import shutil
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         pytest.skip("no Fortran compiler in this image")
-    work = os.path.join(ROOT, "_work", "fortran_probe")
-    os.makedirs(work, exist_ok=True)
+    work = str(tmp_path)
     with open(os.path.join(work, "drive.f90"), "w") as f:
         f.write(FORTRAN_DRIVER)
     lib_dir = os.path.join(ROOT, "scone_amd")
@@ -127,3 +139,6 @@ def test_fortran_binding_links_and_calls_the_library():
     rc, size = int(out[0]), int(out[1])
     assert rc != 0                       # SCN_ERR_NO_DEVICE here, SCN_ERR_ARG on a GPU box
     assert size == C.sizeof(T.ParticleState)
+    assert int(out[2]) != 0              # invalid batch settings (or no device)
+    assert [int(x) for x in out[3:7]] == [C.sizeof(T.Source), C.sizeof(T.FixedRequest), C.sizeof(T.FixedResult),
+                                          C.sizeof(T.BatchSettings)]

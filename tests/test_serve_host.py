@@ -1,0 +1,61 @@
+"""Host logic of the serving loop (CPU): first-come-first-served admission under a slot and
+memory limit, harvest of finished solves, latency bookkeeping.  The batch engine is replaced by a
+stand-in with the same interface whose solves finish after a fixed number of cycles."""
+import numpy as np
+
+from scone_amd import tables as T
+from scone_amd.engine import Busy
+from scone_amd.serve import Request, Server, latency_summary
+
+
+class FakeBatch:
+    def __init__(self, slots, budget):
+        self.slots = [None] * slots
+        self.budget = budget
+        self.used = 0
+        self.log = []
+
+    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False):
+        if self.used + pop > self.budget or None not in self.slots:
+            raise Busy("full")
+        s = self.slots.index(None)
+        self.slots[s] = [pop, n_batches, 0, seed]
+        self.used += pop
+        self.log.append(("add", seed))
+        return s
+
+    def cycle(self, n=1):
+        for v in self.slots:
+            if v is not None and v[2] < v[1]:
+                v[2] += 1
+        return sum(1 for v in self.slots if v is not None and v[2] < v[1])
+
+    def states(self):
+        return np.array([T.SLOT_FREE if v is None else (T.SLOT_DONE if v[2] >= v[1] else T.SLOT_RUNNING)
+                         for v in self.slots], dtype=np.int32)
+
+    def query(self, s):
+        r = T.FixedResult()
+        v = self.slots[s]
+        r.state, r.n_batches, r.n_histories = T.SLOT_DONE, v[2], v[0] * v[2]
+        return r, np.zeros((1, 2)), None, None
+
+    def release(self, s):
+        self.used -= self.slots[s][0]
+        self.slots[s] = None
+
+
+def test_fifo_admission_and_latency():
+    B = FakeBatch(slots=2, budget=100)
+    reqs = [Request(None, None, pop, nb, seed=i, arrival=a) for i, (pop, nb, a) in
+            enumerate([(60, 3, 0), (30, 2, 0), (50, 1, 0), (10, 1, 1), (40, 2, 5)])]
+    Server(B).run_cycles(reqs)
+    # request 2 (50) cannot join 0 (60) + 1 (30): it waits, and request 3 waits behind it (FIFO)
+    order = [s for k, s in B.log if k == "add"]
+    assert order == [0, 1, 2, 3, 4]
+    assert all(r.done >= 0 for r in reqs)
+    assert reqs[0].done == 3 and reqs[1].done == 2
+    assert reqs[2].admitted >= reqs[1].done and reqs[3].admitted >= reqs[2].admitted
+    assert [r.result.n_batches for r in reqs] == [3, 2, 1, 1, 2]
+    s = latency_summary(reqs)
+    assert s["n"] == 5 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
