@@ -15,9 +15,20 @@ import time
 
 from .engine import Batch
 
-TILES = (64, 128, 256)
-K0S = (2, 4, 8, 16)
-LAUNCHES = (2, 3, 4, 6)
+TILES = (64, 128, 256, 512, 1024)
+K0S = (4, 16)
+LAUNCHES = (1, 2, 3)
+
+
+def candidates(tiles=TILES, k0s=K0S, launches=LAUNCHES):
+    """(tile, k0, n_launch); with one launch per cycle K0 plays no part."""
+    out = []
+    for tile, nl in itertools.product(tiles, launches):
+        if nl == 1:
+            out.append((tile, 8, 1))
+        else:
+            out.extend((tile, k0, nl) for k0 in k0s)
+    return out
 
 
 def tune(problems, transport, hbm_budget, max_slots=None, tiles=TILES, k0s=K0S, launches=LAUNCHES, cycles=1,
@@ -26,7 +37,7 @@ def tune(problems, transport, hbm_budget, max_slots=None, tiles=TILES, k0s=K0S, 
     max_slots = max_slots or max(1, len(problems))
     table = []
     best = None
-    for tile, k0, nl in itertools.product(tiles, k0s, launches):
+    for tile, k0, nl in candidates(tiles, k0s, launches):
         B = Batch(transport=transport, max_slots=max_slots, hbm_budget=hbm_budget, tile=tile, k0=k0, n_launch=nl,
                   use_graph=1)
         for (comp, src, pop, seed) in problems:

@@ -6,10 +6,11 @@
 // is sized by the HBM budget.  A cycle advances every running slot by one batch of
 // fixedSourcePhysicsPackage%cycles (PhysicsPackages/fixedSourcePhysicsPackage_class.f90:187-275):
 //   k_tiles          per slot ceil(n / tile) work tiles (slot, first index) -> tile table
-//   k_btransport     persistent kernel: each wave takes tiles from an atomic counter, runs up to K
-//                    (flight, collision) pairs per particle in registers and compacts survivors
-//                    into its slot's other queue (wave ballot, one atomic per wave).  The first
-//                    launch samples the batch's source particles in place.
+//   k_btransport     persistent kernel: each wave takes tiles from an atomic counter; each lane runs
+//                    one particle in registers and takes the next particle of the tile as soon as
+//                    its own dies, so lanes stay busy; with a step limit K, survivors are parked in
+//                    the slot's other queue (wave ballot, one atomic per wave).  The first launch
+//                    samples the batch's source particles in place.
 //   (k_tiles, k_btransport) x (n_launch - 1), K doubling, the last launch runs to completion
 //   k_bnorm/k_bclose tallyAdmin%reportCycleEnd normalisation and scoreMemory%closeCycle
 //   k_bend           batchN + 1, early stop on a bin's relative sd, slot completion
@@ -38,6 +39,9 @@ constexpr int BWAVE = 64;
 constexpr int MAX_SLOTS = 4096;
 constexpr int MATERIAL_SOURCE_ATTEMPTS = 200;   // materialSource_class.f90:168
 constexpr int MAX_PROBG = 4096;
+// a history still alive after this many (flight, collision) pairs in one launch is reported as lost
+// (an error, not a hang)
+constexpr int MAX_PAIRS_PER_LAUNCH = 1 << 22;
 
 // live queue of one slot, structure of arrays; position arrays level-major
 struct BLive {
@@ -319,86 +323,186 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// counters of the wave's current slot, summed over the wave by lane 0
+__device__ __forceinline__ void flush_counts(SlotState* SS, Cnt& cn, int lane)
+{
+    const unsigned long long seg = wave_sum((unsigned long long)cn.seg);
+    const unsigned long long col = wave_sum((unsigned long long)cn.col);
+    const unsigned long long sc = wave_sum((unsigned long long)cn.sc);
+    if (lane == 0) {
+        if (seg) atomicAdd(&SS->segments, seg);
+        if (col) atomicAdd(&SS->collisions, col);
+        if (sc) atomicAdd(&SS->scores, sc);
+    }
+    cn.seg = cn.col = cn.sc = 0;
+}
+
+// add the wave's LDS copy of the low bins to BIN and clear it (all lanes of the wave call this
+// at the same point; LDS operations of one wave complete in order)
+__device__ __forceinline__ void flush_lds_bins(const Bins& B, int lane)
+{
+    __builtin_amdgcn_wave_barrier();
+    for (int64_t i = lane; i < B.lds_n; i += BWAVE) {
+        const double v = B.lds[i];
+        if (v != 0.0) {
+            unsafeAtomicAdd(&B.glob[i], v);
+            B.lds[i] = 0.0;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One launch of the event loop over the tile table.  A wave works on one slot at a time (its
+// tables are then wave-uniform).  Every lane holds one particle and advances it by one tracking
+// event per iteration (plus the collision that event may lead to), so lanes whose flights are
+// short do not wait for long ones.  A lane whose particle dies (or, in a K-limited launch, has run
+// K (flight, collision) pairs and is parked in the slot's other queue) takes the next particle of
+// the tile at once, so lanes do not idle while others finish.
+// When the tile is used up the wave takes the next tile: one of the same slot continues the
+// refill directly, one of another slot starts once the live lanes have finished.
 template <int NL, int MODE>
 __global__ void __launch_bounds__(BBLOCK)
 k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __restrict__ tiles, int* ctl, int first,
-             int in_side, int out_side, int tile, int max_steps, DevStatus* st)
+             int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave)
 {
+    extern __shared__ double lds_all[];
     const int lane = threadIdx.x & (BWAVE - 1);
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // this wave's private copy of the first lds_per_wave bins of its current slot
+    double* wl = lds_all + (size_t)(threadIdx.x / BWAVE) * lds_per_wave;
+    for (int i = lane; i < lds_per_wave; i += BWAVE) wl[i] = 0.0;
     const int n_tiles = ctl[0];
     Sites S0;   // fixed-source slots hold no fissile material: the site bank is never written
     S0.cap = 0;
     S0.count = ctl + 4;
-    for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&ctl[1], 1);
-        t = __shfl(t, 0);
-        if (t >= n_tiles) break;
-        const int2 tl = tiles[t];
-        const int s = __builtin_amdgcn_readfirstlane(tl.x);
-        const int base = __builtin_amdgcn_readfirstlane(tl.y);
-        const SlotDev& D = slots[s];
-        const DevProblem P = D.P;
-        SlotState& SS = ss[s];
-        const int n_in = SS.n_in;
-        const int batch = SS.batch;
-        const int cap = D.pop;
-        Bins B;
-        B.lds = nullptr;
-        B.glob = D.bin;
-        Cnt cn{0, 0, 0};
-        for (int sub = 0; sub < tile; sub += BWAVE) {
-            const int i = base + sub + lane;
-            Pt<NL> p;
-            int alive = 0;
-            if (i < n_in) {
+    int s = -1;                      // current slot (wave-uniform)
+    int next = 0, end = 0;           // particles [next, end) of the current tile not yet started
+    int pend_s = -1, pend_b = 0;     // a tile of another slot, started once the lanes are idle
+    int exhausted = 0;               // the tile table is used up
+    const SlotDev* D = slots;
+    SlotState* SS = ss;
+    int n_in = 0, batch = 0, cap = 0;
+    Bins B;
+    B.lds = wl;
+    B.glob = nullptr;
+    B.lds_n = 0;
+    Cnt cn{0, 0, 0};
+    Pt<NL> p;
+    DistCache<NL> cache;   // surface tracking: the current flight's distance cache
+    cache.lvl = 0;
+    int alive = 0, steps = 0, nseg = 0;
+    for (int iter = 0;; ++iter) {
+        while (next >= end && !exhausted && pend_s < 0) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(&ctl[1], 1);
+            t = __shfl(t, 0);
+            if (t >= n_tiles) {
+                exhausted = 1;
+                break;
+            }
+            const int2 tl = tiles[t];
+            const int ts = __builtin_amdgcn_readfirstlane(tl.x);
+            const int tb = __builtin_amdgcn_readfirstlane(tl.y);
+            if (ts == s) {
+                next = tb;
+                end = min(tb + tile, n_in);
+            } else {
+                pend_s = ts;
+                pend_b = tb;
+            }
+        }
+        const unsigned long long live = __ballot(alive);
+        if (pend_s >= 0 && live == 0) {
+            if (s >= 0) {
+                flush_counts(SS, cn, lane);
+                flush_lds_bins(B, lane);
+            }
+            s = pend_s;
+            pend_s = -1;
+            D = slots + s;
+            SS = ss + s;
+            n_in = SS->n_in;
+            batch = SS->batch;
+            cap = D->pop;
+            B.glob = D->bin;
+            B.lds_n = min((int64_t)lds_per_wave, D->P.n_bins);
+            next = pend_b;
+            end = min(pend_b + tile, n_in);
+        }
+        if (next < end) {
+            const unsigned long long idle = ~live;
+            const int i = next + __popcll(idle & lt_mask);
+            if (!alive && i < end) {
                 if (first) {
-                    alive = born(D, P, batch, i, p, st);
+                    alive = born(*D, D->P, batch, i, p, st);
                 } else {
-                    bload(D.q[in_side], cap, i, p);
-                    relink(P, p);
+                    bload(D->q[in_side], cap, i, p);
+                    relink(D->P, p);
                     alive = 1;
                 }
-                if (alive) {
-                    for (int step = 0; step < max_steps; ++step) {
-                        int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, cn) : flight_st(P, p, B, st, cn);
-                        if (!ok) {
-                            alive = 0;
-                            break;
-                        }
-                        ok = collide(P, p, 1.0, B, S0, st, cn);
-                        if (!ok) {
-                            alive = 0;
-                            break;
-                        }
-                        if (st->code) break;
+                steps = 0;
+                nseg = 0;
+                cache.lvl = 0;
+            }
+            next = min(next + __popcll(idle), end);
+        }
+        if (__ballot(alive) == 0) {
+            if (exhausted && pend_s < 0 && next >= end) break;
+            continue;
+        }
+        int park = 0;
+        if (alive) {
+            // one tracking event; a real collision is processed in the same iteration
+            const DevProblem& P = D->P;
+            int r = (MODE == SCN_TRANSPORT_DT) ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
+            if (r == STEP_MORE && ++nseg >= MAX_SEGMENTS_PER_FLIGHT) {
+                raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
+                r = STEP_DEAD;
+            }
+            if (r == STEP_COLLIDE) {
+                if (collide(P, p, 1.0, B, S0, st, cn)) {
+                    cache.lvl = 0;
+                    nseg = 0;
+                    if (++steps >= max_steps) {
+                        park = 1;
+                        alive = 0;
+                    } else if (steps >= MAX_PAIRS_PER_LAUNCH) {
+                        raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
+                        alive = 0;
                     }
-                    if (!alive && D.hist_rng) {
-                        const size_t h = (size_t)batch * D.pop + (size_t)(p.brood - 1);
-                        D.hist_rng[h] = p.rng;
-                        D.hist_coll[h] = p.coll;
-                    }
+                } else {
+                    r = STEP_DEAD;
                 }
             }
-            const unsigned long long mask = __ballot(alive);
-            const int cntw = __popcll(mask);
+            if (r == STEP_DEAD) {
+                alive = 0;
+                if (D->hist_rng) {
+                    const size_t h = (size_t)batch * D->pop + (size_t)(p.brood - 1);
+                    D->hist_rng[h] = p.rng;
+                    D->hist_coll[h] = p.coll;
+                }
+            }
+        }
+        const unsigned long long pm = __ballot(park);
+        if (pm) {
             int obase = 0;
-            if (lane == 0 && cntw) obase = atomicAdd(&SS.count[out_side], cntw);
+            if (lane == 0) obase = atomicAdd(&SS->count[out_side], __popcll(pm));
             obase = __shfl(obase, 0);
-            if (alive) {
-                const int o = obase + __popcll(mask & ((1ull << lane) - 1ull));
-                if (o < cap) bstore(D.q[out_side], cap, o, p);
+            if (park) {
+                const int o = obase + __popcll(pm & lt_mask);
+                if (o < cap) bstore(D->q[out_side], cap, o, p);
                 else raise_error(st, SCN_ERR_OVERFLOW, s);
             }
         }
-        const unsigned long long seg = wave_sum((unsigned long long)cn.seg);
-        const unsigned long long col = wave_sum((unsigned long long)cn.col);
-        const unsigned long long sc = wave_sum((unsigned long long)cn.sc);
-        if (lane == 0) {
-            if (seg) atomicAdd(&SS.segments, seg);
-            if (col) atomicAdd(&SS.collisions, col);
-            if (sc) atomicAdd(&SS.scores, sc);
+        if ((iter & 63) == 63) {
+            int e = 0;
+            if (lane == 0) e = __atomic_load_n(&st->code, __ATOMIC_RELAXED);
+            if (__shfl(e, 0)) break;
         }
+    }
+    if (s >= 0) {
+        flush_counts(SS, cn, lane);
+        flush_lds_bins(B, lane);
     }
 }
 
@@ -470,15 +574,15 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
-typedef void (*BTransportFn)(dim3, dim3, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int, int,
-                             int, int, DevStatus*);
+typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int,
+                             int, int, int, DevStatus*, int);
 
 template <int NL, int MODE>
-void launch_bt(dim3 g, dim3 b, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles, int* ctl,
-               int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st)
+void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles,
+               int* ctl, int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave)
 {
-    hipLaunchKernelGGL((k_btransport<NL, MODE>), g, b, 0, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
-                       max_steps, st);
+    hipLaunchKernelGGL((k_btransport<NL, MODE>), g, b, sh, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
+                       max_steps, st, lds_per_wave);
 }
 
 template <int NL, int MODE>
@@ -531,6 +635,7 @@ struct Batch {
     hipEvent_t c0 = nullptr, c1 = nullptr;
     scn_stats stats{};
     int64_t rel_seg = 0, rel_col = 0, rel_sc = 0, rel_hist = 0;   // counters of released slots
+    int S_lds_per_wave = 2048;   // low bins of a slot privatised per wave in LDS (16 KiB per wave)
 };
 
 #define BHIPCHK(expr)                                                                    \
@@ -744,6 +849,8 @@ int max_depth_nl(Batch* B)
     return compiled_nl(d);
 }
 
+size_t lds_bytes(const Batch* B) { return sizeof(double) * (size_t)B->S_lds_per_wave * (BBLOCK / BWAVE); }
+
 // persistent grid of the transport kernel variant: resident blocks per CU x CUs
 int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
 {
@@ -751,7 +858,7 @@ int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
     if (B->S.transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(nl, fn, &kern);
     else pick_bt<SCN_TRANSPORT_ST>(nl, fn, &kern);
     int occ = 0;
-    BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, 0));
+    BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, lds_bytes(B)));
     *grid = B->n_cu * std::max(1, occ);
     return 0;
 }
@@ -771,8 +878,8 @@ int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
         const int last = L == B->S.n_launch - 1;
         const int steps = last ? (1 << 30) : (B->S.k0 << L);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
-        fn(dim3(grid), dim3(BBLOCK), B->stream, B->d_slots, B->d_state, B->d_tiles, B->d_ctl, first, in_side, out_side,
-           tile, steps, B->d_status);
+        fn(dim3(grid), dim3(BBLOCK), lds_bytes(B), B->stream, B->d_slots, B->d_state, B->d_tiles, B->d_ctl, first,
+           in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L + 1], B->stream));
         side = out_side;
     }
@@ -883,9 +990,9 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     }
     Batch* B = new Batch();
     B->S = *settings;
-    if (B->S.tile == 0) B->S.tile = 64;
+    if (B->S.tile == 0) B->S.tile = 1024;
     if (B->S.k0 == 0) B->S.k0 = 8;
-    if (B->S.n_launch == 0) B->S.n_launch = 4;
+    if (B->S.n_launch == 0) B->S.n_launch = 1;
     B->n_slots = settings->max_slots;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) B->n_cu = prop.multiProcessorCount;

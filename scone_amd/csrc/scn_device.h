@@ -613,15 +613,17 @@ __device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, co
     return (mat >= 0 && mat <= nm) ? P.ints[m.data + mat] : P.ints[m.data + nm + 1];
 }
 
-// score memory: either a block-private LDS copy (lds != nullptr) or the global BIN array
+// score memory: bins [0, lds_n) go to a private LDS copy (flushed to BIN by the kernel), the
+// rest straight to the global BIN array
 struct Bins {
     double* lds;
     double* glob;
+    int64_t lds_n;
 };
 
 __device__ __forceinline__ void bin_add(const Bins& B, int64_t addr, double v)
 {
-    if (B.lds) atomicAdd(&B.lds[addr], v);
+    if (addr < B.lds_n) atomicAdd(&B.lds[addr], v);
     else unsafeAtomicAdd(&B.glob[addr], v);
 }
 
@@ -671,6 +673,95 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL>& p, in
 // flights
 // ---------------------------------------------------------------------------------------------
 
+// outcome of one tracking event (an ST segment or a DT tentative flight)
+enum { STEP_DEAD = 0, STEP_COLLIDE = 1, STEP_MORE = 2 };
+
+// one segment of surfaceTracking (transportOperatorST_class.f90:59-160): to the nearest of the
+// sampled collision point and the closest surface; `cache` is the flight's distCache (fresh at the
+// start of every flight)
+template <int NL>
+__device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache<NL>& cache, const Bins& B,
+                                       DevStatus* st, Cnt& cnt)
+{
+    double sigmaTrack, sigmaT, dist, invSigmaTrack;
+    int mat = p.mat;
+    if (mat == SCN_VOID_MAT) sigmaTrack = P.collision_xs;
+    else sigmaTrack = scn_fmax(track_mat_xs(P, mat, p.G, st), P.collision_xs);
+    p.track = sigmaTrack;
+    if (sigmaTrack < SCN_ST_TOL) {
+        dist = SCN_INF;
+        invSigmaTrack = SCN_INF;
+        sigmaT = 0.0;
+    } else {
+        invSigmaTrack = 1.0 / sigmaTrack;
+        dist = -scn_log(scn_rng_get(&p.rng)) * invSigmaTrack;
+        sigmaT = track_mat_xs(P, mat, p.G, st);
+        if (dist != dist) {
+            raise_error(st, SCN_ERR_NAN, p.brood);
+            p.dead = 1;
+            return STEP_DEAD;
+        }
+    }
+    int event = move(P, p, dist, cache, st);
+    cnt.seg++;
+    p.t = p.t + dist / 1.0;
+    if (event == EV_COLL) p.fate = FATE_NO;
+    mat = p.mat;
+    if (mat == SCN_OUTSIDE_MAT) {
+        p.dead = 1;
+        p.fate = FATE_LEAK;
+        p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
+        return STEP_DEAD;
+    } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
+        raise_error(st, SCN_ERR_MATERIAL, mat);
+        p.dead = 1;
+        return STEP_DEAD;
+    }
+    if (event == EV_COLL) {
+        if (scn_rng_get(&p.rng) < sigmaT * invSigmaTrack) return STEP_COLLIDE;
+        cnt.sc += report_in_coll(P, p, 1, B, st);
+    }
+    return STEP_MORE;
+}
+
+// one tentative flight of deltaTracking (transportOperatorDT_class.f90:47-130).  The majorant
+// and its inverse depend on the group only, so evaluating them per tentative flight gives the
+// values the flight computes once.
+template <int NL>
+__device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
+{
+    double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
+    p.track = maj;
+    double majorant_inv = 1.0 / maj;
+    if (fabs(majorant_inv) > SCN_HUGE) {
+        raise_error(st, SCN_ERR_MAJORANT, p.G);
+        p.dead = 1;
+        return STEP_DEAD;
+    }
+    double distance = -scn_log(scn_rng_get(&p.rng)) * majorant_inv;
+    teleport(P, p, distance, st);
+    cnt.seg++;
+    p.t = p.t + distance / 1.0;
+    int mat = p.mat;
+    if (mat == SCN_OUTSIDE_MAT) {
+        p.fate = FATE_LEAK;
+        p.dead = 1;
+        p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
+        return STEP_DEAD;
+    } else if (mat == SCN_VOID_MAT) {
+        cnt.sc += report_in_coll(P, p, 1, B, st);
+        return STEP_MORE;
+    } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
+        raise_error(st, SCN_ERR_MATERIAL, mat);
+        p.dead = 1;
+        return STEP_DEAD;
+    }
+    double sigmaT = track_mat_xs(P, mat, p.G, st);
+    if (scn_rng_get(&p.rng) < sigmaT * majorant_inv) return STEP_COLLIDE;
+    cnt.sc += report_in_coll(P, p, 1, B, st);
+    return STEP_MORE;
+}
+
 // surfaceTracking: segments until a real collision or a leak; returns 1 if alive
 template <int NL>
 __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
@@ -683,44 +774,8 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const B
             p.dead = 1;
             return 0;
         }
-        double sigmaTrack, sigmaT, dist, invSigmaTrack;
-        int mat = p.mat;
-        if (mat == SCN_VOID_MAT) sigmaTrack = P.collision_xs;
-        else sigmaTrack = scn_fmax(track_mat_xs(P, mat, p.G, st), P.collision_xs);
-        p.track = sigmaTrack;
-        if (sigmaTrack < SCN_ST_TOL) {
-            dist = SCN_INF;
-            invSigmaTrack = SCN_INF;
-            sigmaT = 0.0;
-        } else {
-            invSigmaTrack = 1.0 / sigmaTrack;
-            dist = -scn_log(scn_rng_get(&p.rng)) * invSigmaTrack;
-            sigmaT = track_mat_xs(P, mat, p.G, st);
-            if (dist != dist) {
-                raise_error(st, SCN_ERR_NAN, p.brood);
-                p.dead = 1;
-                return 0;
-            }
-        }
-        int event = move(P, p, dist, cache, st);
-        cnt.seg++;
-        p.t = p.t + dist / 1.0;
-        if (event == EV_COLL) p.fate = FATE_NO;
-        mat = p.mat;
-        if (mat == SCN_OUTSIDE_MAT) {
-            p.dead = 1;
-            p.fate = FATE_LEAK;
-            p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
-            return 0;
-        } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
-            raise_error(st, SCN_ERR_MATERIAL, mat);
-            p.dead = 1;
-            return 0;
-        }
-        if (event == EV_COLL) {
-            if (scn_rng_get(&p.rng) < sigmaT * invSigmaTrack) return 1;
-            cnt.sc += report_in_coll(P, p, 1, B, st);
-        }
+        int r = st_step(P, p, cache, B, st, cnt);
+        if (r != STEP_MORE) return r == STEP_COLLIDE;
     }
 }
 
@@ -728,41 +783,14 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const B
 template <int NL>
 __device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
 {
-    double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
-    p.track = maj;
-    double majorant_inv = 1.0 / maj;
-    if (fabs(majorant_inv) > SCN_HUGE) {
-        raise_error(st, SCN_ERR_MAJORANT, p.G);
-        p.dead = 1;
-        return 0;
-    }
     for (int seg = 0;; ++seg) {
         if (seg >= MAX_SEGMENTS_PER_FLIGHT) {
             raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
             p.dead = 1;
             return 0;
         }
-        double distance = -scn_log(scn_rng_get(&p.rng)) * majorant_inv;
-        teleport(P, p, distance, st);
-        cnt.seg++;
-        p.t = p.t + distance / 1.0;
-        int mat = p.mat;
-        if (mat == SCN_OUTSIDE_MAT) {
-            p.fate = FATE_LEAK;
-            p.dead = 1;
-            p.k[K_ANA_LEAK] = p.k[K_ANA_LEAK] + p.w;
-            return 0;
-        } else if (mat == SCN_VOID_MAT) {
-            cnt.sc += report_in_coll(P, p, 1, B, st);
-            continue;
-        } else if (mat == SCN_UNDEF_MAT || mat == SCN_OVERLAP_MAT || st->code) {
-            raise_error(st, SCN_ERR_MATERIAL, mat);
-            p.dead = 1;
-            return 0;
-        }
-        double sigmaT = track_mat_xs(P, mat, p.G, st);
-        if (scn_rng_get(&p.rng) < sigmaT * majorant_inv) return 1;
-        cnt.sc += report_in_coll(P, p, 1, B, st);
+        int r = dt_step(P, p, B, st, cnt);
+        if (r != STEP_MORE) return r == STEP_COLLIDE;
     }
 }
 
@@ -780,6 +808,31 @@ struct Sites {
     int* count;
     int cap;
 };
+
+// CDF scan of multiScatterMG%sampleGout / fissionMG%sampleOut: subtract x[0], x[1], ... from
+// rem until it drops below zero; returns the 1-based index, or n + 1 if it never does.  The
+// subtractions are the reference's sequence; the loads are issued four at a time (all in bounds)
+// so the scan is not one memory latency per group.
+__device__ __forceinline__ int cdf_scan(const double* __restrict__ x, int n, double rem)
+{
+    int g = 0;
+    for (; g + 4 <= n; g += 4) {
+        const double a0 = x[g], a1 = x[g + 1], a2 = x[g + 2], a3 = x[g + 3];
+        rem = rem - a0;
+        if (rem < 0.0) return g + 1;
+        rem = rem - a1;
+        if (rem < 0.0) return g + 2;
+        rem = rem - a2;
+        if (rem < 0.0) return g + 3;
+        rem = rem - a3;
+        if (rem < 0.0) return g + 4;
+    }
+    for (; g < n; ++g) {
+        rem = rem - x[g];
+        if (rem < 0.0) return g + 1;
+    }
+    return n + 1;
+}
 
 // collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
 template <int NL>
@@ -826,11 +879,7 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
                 double mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
                 double phi = SCN_TWO_PI * scn_rng_get(&p.rng);
                 double rem = scn_rng_get(&p.rng);
-                int G_out;
-                for (G_out = 1; G_out <= nG; ++G_out) {
-                    rem = rem - chi[G_out - 1];
-                    if (rem < 0.0) break;
-                }
+                int G_out = cdf_scan(chi, nG, rem);
                 if (G_out > nG) {
                     raise_error(st, SCN_ERR_SAMPLING, 2);
                     p.dead = 1;
@@ -862,11 +911,7 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
         double rem = scn_rng_get(&p.rng) * x[XS_SCATTER];
         const double* P0 = P.P0 + ((size_t)(mat - 1) * nG + (p.G - 1)) * nG;
         const double* prod = P.prod + ((size_t)(mat - 1) * nG + (p.G - 1)) * nG;
-        int G_out;
-        for (G_out = 1; G_out <= nG; ++G_out) {
-            rem = rem - P0[G_out - 1];
-            if (rem < 0.0) break;
-        }
+        int G_out = cdf_scan(P0, nG, rem);
         if (G_out > nG) {
             raise_error(st, SCN_ERR_SAMPLING, 1);
             p.dead = 1;

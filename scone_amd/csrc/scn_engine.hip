@@ -127,22 +127,30 @@ __device__ inline void store_live(const Live& L, int cap, int i, const Pt<NL>& p
     L.coll[i] = p.coll;
 }
 
-// One launch of the event loop.  first != 0: the queue is the cycle's bank (AoS); history n = i+1.
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One launch of the event loop (persistent grid).  first != 0: the input is the cycle's bank
+// (AoS) and history n = i + 1 starts on the master strided by n.  Waves take chunks of the input
+// from an atomic counter; every lane holds one particle and advances it by one tracking event per
+// iteration (plus the collision it may lead to), and a lane whose particle dies -- or, with a
+// step limit, has run max_steps (flight, collision) pairs and is parked in the output queue --
+// takes the next particle of the chunk at once.  Lanes therefore stay busy until the input is
+// used up, whatever the spread of history lengths.
 template <int NL, int MODE>
 __global__ void __launch_bounds__(BLOCK)
 k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first, int n_first, Live in,
             const int* __restrict__ n_in_ptr, Live out, int* __restrict__ n_out, int cap, int max_steps,
             uint64_t master, double k_eff, Sites S, double* __restrict__ bins, int use_lds,
-            double* __restrict__ hist_k, int hist_cap, Counters* __restrict__ cnt, DevStatus* st)
+            double* __restrict__ hist_k, int hist_cap, Counters* __restrict__ cnt, DevStatus* st, int* work,
+            int chunk)
 {
     extern __shared__ double lds_bins[];
-    __shared__ unsigned long long s_seg, s_col, s_sc;
     const int n_in = first ? n_first : *n_in_ptr;
-    if (threadIdx.x == 0) {
-        s_seg = 0;
-        s_col = 0;
-        s_sc = 0;
-    }
     if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) lds_bins[i] = 0.0;
     }
@@ -150,92 +158,133 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     Bins B;
     B.lds = use_lds ? lds_bins : nullptr;
     B.glob = bins;
+    B.lds_n = use_lds ? P.n_bins : 0;
     Cnt cn{0, 0, 0};
     const int lane = threadIdx.x & (WAVE - 1);
-    for (int base = blockIdx.x * BLOCK; base < n_in; base += gridDim.x * BLOCK) {
-        const int i = base + threadIdx.x;
-        const int valid = i < n_in;
-        Pt<NL> p;
-        int alive = 0;
-        if (valid) {
-            if (first) {
-                const scn_particle_state& s = bank[i];
-                p.rng = scn_rng_stride(master, (int64_t)i + 1);
-#pragma unroll
-                for (int l = 0; l < NL; ++l) {
-                    p.loc[l] = 0;
-                    p.uni[l] = 0;
-                    p.root[l] = 0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    p.r[0][k] = s.r[k];
-                    p.u[k] = s.dir[k];
-                }
-                p.w = s.wgt;
-                p.w0 = s.wgt;
-                p.G = s.G;
-                p.t = s.time;
-                p.coll = s.collisionN;
-                p.brood = i + 1;
-                p.nsite = 0;
-                p.k[0] = p.k[1] = p.k[2] = p.k[3] = 0.0;
-                p.dead = 0;
-                p.fate = FATE_NO;
-                p.track = 0.0;
-                place_coord(P, p, st);
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    int next = 0, end = 0, exhausted = 0;
+    Pt<NL> p;
+    DistCache<NL> cache;
+    cache.lvl = 0;
+    int alive = 0, steps = 0, nseg = 0;
+    for (int iter = 0;; ++iter) {
+        if (next >= end && !exhausted) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(work, 1);
+            c = __shfl(c, 0);
+            const long long base = (long long)c * chunk;
+            if (base >= n_in) {
+                exhausted = 1;
             } else {
-                load_live(in, cap, i, p);
-                relink(P, p);
+                next = (int)base;
+                end = (int)min((long long)n_in, base + chunk);
             }
-            alive = 1;
-            for (int step = 0; step < max_steps; ++step) {
-                int ok = (MODE == SCN_TRANSPORT_DT) ? flight_dt(P, p, B, st, cn) : flight_st(P, p, B, st, cn);
-                if (!ok) {
-                    alive = 0;
-                    break;
+        }
+        const unsigned long long live = __ballot(alive);
+        if (next < end) {
+            const unsigned long long idle = ~live;
+            const int i = next + __popcll(idle & lt_mask);
+            if (!alive && i < end) {
+                if (first) {
+                    const scn_particle_state& s = bank[i];
+                    p.rng = scn_rng_stride(master, (int64_t)i + 1);
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) {
+                        p.loc[l] = 0;
+                        p.uni[l] = 0;
+                        p.root[l] = 0;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        p.r[0][k] = s.r[k];
+                        p.u[k] = s.dir[k];
+                    }
+                    p.w = s.wgt;
+                    p.w0 = s.wgt;
+                    p.G = s.G;
+                    p.t = s.time;
+                    p.coll = s.collisionN;
+                    p.brood = i + 1;
+                    p.nsite = 0;
+                    p.k[0] = p.k[1] = p.k[2] = p.k[3] = 0.0;
+                    p.dead = 0;
+                    p.fate = FATE_NO;
+                    p.track = 0.0;
+                    place_coord(P, p, st);
+                } else {
+                    load_live(in, cap, i, p);
+                    relink(P, p);
                 }
-                ok = collide(P, p, k_eff, B, S, st, cn);
-                if (!ok) {
-                    alive = 0;
-                    break;
-                }
-                if (st->code) break;
+                alive = 1;
+                steps = 0;
+                nseg = 0;
+                cache.lvl = 0;
             }
-            if (!alive) {
+            next = min(next + __popcll(idle), end);
+        }
+        if (__ballot(alive) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        int park = 0;
+        if (alive) {
+            int r = (MODE == SCN_TRANSPORT_DT) ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
+            if (r == STEP_MORE && ++nseg >= MAX_SEGMENTS_PER_FLIGHT) {
+                raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
+                r = STEP_DEAD;
+            }
+            if (r == STEP_COLLIDE) {
+                if (collide(P, p, k_eff, B, S, st, cn)) {
+                    cache.lvl = 0;
+                    nseg = 0;
+                    if (++steps >= max_steps) {
+                        park = 1;
+                        alive = 0;
+                    }
+                } else {
+                    r = STEP_DEAD;
+                }
+            }
+            if (r == STEP_DEAD) {
+                alive = 0;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) hist_k[(size_t)k * hist_cap + (p.brood - 1)] = p.k[k];
             }
         }
-        // wave-level queue compaction: ballot of survivors, one atomic per wave
-        unsigned long long mask = __ballot(alive);
-        int cntw = __popcll(mask);
-        int obase = 0;
-        if (lane == 0 && cntw) obase = atomicAdd(n_out, cntw);
-        obase = __shfl(obase, 0);
-        if (alive) {
-            int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            int o = obase + rank;
-            if (o < cap) store_live(out, cap, o, p);
-            else raise_error(st, SCN_ERR_OVERFLOW, o);
+        // park step-limited survivors: wave ballot, one atomic per wave
+        const unsigned long long pm = __ballot(park);
+        if (pm) {
+            int obase = 0;
+            if (lane == 0) obase = atomicAdd(n_out, __popcll(pm));
+            obase = __shfl(obase, 0);
+            if (park) {
+                const int o = obase + __popcll(pm & lt_mask);
+                if (o < cap) store_live(out, cap, o, p);
+                else raise_error(st, SCN_ERR_OVERFLOW, o);
+            }
+        }
+        if ((iter & 63) == 63) {
+            int e = 0;
+            if (lane == 0) e = __atomic_load_n(&st->code, __ATOMIC_RELAXED);
+            if (__shfl(e, 0)) break;
         }
     }
-    atomicAdd(&s_seg, (unsigned long long)(cn.seg));
-    atomicAdd(&s_col, (unsigned long long)(cn.col));
-    atomicAdd(&s_sc, (unsigned long long)(cn.sc));
+    const unsigned long long seg = wave_sum64((unsigned long long)cn.seg);
+    const unsigned long long col = wave_sum64((unsigned long long)cn.col);
+    const unsigned long long sc = wave_sum64((unsigned long long)cn.sc);
+    if (lane == 0) {
+        if (seg) atomicAdd(&cnt->segments, seg);
+        if (col) atomicAdd(&cnt->collisions, col);
+        if (sc) atomicAdd(&cnt->scores, sc);
+    }
     __syncthreads();
     if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) {
             double v = lds_bins[i];
             if (v != 0.0) unsafeAtomicAdd(&bins[i], v);
         }
-    }
-    if (threadIdx.x == 0) {
-        atomicAdd(&cnt->segments, s_seg);
-        atomicAdd(&cnt->collisions, s_col);
-        atomicAdd(&cnt->scores, s_sc);
     }
 }
 
@@ -534,8 +583,9 @@ struct Engine {
     scn_stats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // tuning
-    int k0 = 8;
+    int k0 = 0;             // 0: one launch per cycle runs every history to completion
     int tail_frac = 16;
+    int chunk = 256;        // histories a wave takes from the input at once
 };
 
 #define HIPCHK(expr)                                                                     \
@@ -681,26 +731,32 @@ int dsum_batch(Engine* E, SumJobs J)
 
 typedef void (*TransportFn)(dim3, dim3, size_t, hipStream_t, const DevProblem&, const scn_particle_state*, int, int,
                             Live, const int*, Live, int*, int, int, uint64_t, double, Sites, double*, int, double*,
-                            int, Counters*, DevStatus*);
+                            int, Counters*, DevStatus*, int*, int);
 
 template <int NL, int MODE>
 void launch_transport(dim3 g, dim3 b, size_t sh, hipStream_t s, const DevProblem& P, const scn_particle_state* bank,
                       int first, int n_first, Live in, const int* n_in_ptr, Live out, int* n_out, int cap,
                       int max_steps, uint64_t master, double k_eff, Sites S, double* bins, int use_lds,
-                      double* hist_k, int hist_cap, Counters* cnt, DevStatus* st)
+                      double* hist_k, int hist_cap, Counters* cnt, DevStatus* st, int* work, int chunk)
 {
     hipLaunchKernelGGL((k_transport<NL, MODE>), g, b, sh, s, P, bank, first, n_first, in, n_in_ptr, out, n_out, cap,
-                       max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st);
+                       max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st, work, chunk);
+}
+
+template <int NL, int MODE>
+const void* transport_kernel()
+{
+    return (const void*)k_transport<NL, MODE>;
 }
 
 template <int MODE>
-TransportFn pick_nl(int nl)
+TransportFn pick_nl(int nl, const void** kern)
 {
     switch (nl) {
-    case 1: return launch_transport<1, MODE>;
-    case 3: return launch_transport<3, MODE>;
-    case 4: return launch_transport<4, MODE>;
-    default: return launch_transport<8, MODE>;
+    case 1: *kern = transport_kernel<1, MODE>(); return launch_transport<1, MODE>;
+    case 3: *kern = transport_kernel<3, MODE>(); return launch_transport<3, MODE>;
+    case 4: *kern = transport_kernel<4, MODE>(); return launch_transport<4, MODE>;
+    default: *kern = transport_kernel<8, MODE>(); return launch_transport<8, MODE>;
     }
 }
 
@@ -714,12 +770,17 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     }
     HIPCHK(hipMemsetAsync(E->counters_i, 0, sizeof(int) * 4, E->stream));
     HIPCHK(hipMemsetAsync(E->sites.count, 0, sizeof(int), E->stream));
-    TransportFn fn = (E->transport == SCN_TRANSPORT_DT) ? pick_nl<SCN_TRANSPORT_DT>(E->nl) : pick_nl<SCN_TRANSPORT_ST>(E->nl);
+    const void* kern = nullptr;
+    TransportFn fn = (E->transport == SCN_TRANSPORT_DT) ? pick_nl<SCN_TRANSPORT_DT>(E->nl, &kern)
+                                                        : pick_nl<SCN_TRANSPORT_ST>(E->nl, &kern);
     double* bins = score_bins ? E->bins : nullptr;
     // a null BIN pointer means "do not score": point at a scratch copy instead
     double* bins_arg = bins ? bins : E->bins;
     int use_lds = score_bins ? E->use_lds : 0;
     size_t sh = use_lds ? (size_t)E->lds_bytes : 0;
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, sh));
+    const int persistent = E->n_cu * std::max(1, occ);
     DevProblem P = E->P;
     if (!score_bins) P.n_clerks = 0;
     int n_live = n_in;
@@ -728,16 +789,17 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     int K = E->k0;
     int first = 1;
     while (n_live > 0) {
-        int grid = std::max(1, std::min((n_live + BLOCK - 1) / BLOCK, E->n_cu * 8));
-        int tail = n_live <= std::max(65536, n_in / E->tail_frac);
+        int grid = std::max(1, std::min((n_live + BLOCK - 1) / BLOCK, persistent));
+        int tail = K <= 0 || n_live <= std::max(65536, n_in / E->tail_frac);
         int steps = tail ? (1 << 30) : K;
         int* nin_ptr = E->counters_i + cur;
         int* nout_ptr = E->counters_i + (1 - cur);
         HIPCHK(hipMemsetAsync(nout_ptr, 0, sizeof(int), E->stream));
+        HIPCHK(hipMemsetAsync(E->counters_i + 2, 0, sizeof(int), E->stream));
         HIPCHK(hipEventRecord(E->ev0, E->stream));
         fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, first, n_in, E->live[cur], nin_ptr, E->live[1 - cur],
            nout_ptr, E->cap, steps, master, k_eff, E->sites, bins_arg, use_lds, E->hist_k, E->cap, E->counters,
-           E->status);
+           E->status, E->counters_i + 2, E->chunk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(E->ev1, E->stream));
         int host_n = 0;
