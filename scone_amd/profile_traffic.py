@@ -6,8 +6,8 @@ Run under rocprofv3 twice, once per counter (the two cannot share a pass with th
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir_write> -- python3 -m scone_amd.profile_traffic run ...
     python3 -m scone_amd.profile_traffic summarize <dir_fetch> <dir_write> --key <workload key>
 
-`run` first copies a buffer of known size with a plain device copy (torch's copy kernel), so the
-counters can be scaled: factor = counted KiB of the copy / its true bytes.  Then it runs the
+`run` first copies a 1 GiB device buffer with a plain device-to-device copy (the HIP runtime's
+copy kernel), so the counters can be scaled: factor = counted KiB of that dispatch / its true KiB.  Then it runs the
 workload without hipGraph replay, so every transport launch is a separately counted dispatch.
 `summarize` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, divides them by
 the copy's factor and writes bytes per launch of the transport kernel into
@@ -47,7 +47,8 @@ def run(args):
         for (c, s, pop, _, seed) in problems:
             B.add(c, s, pop, args.cycles, seed)
         B.run_all()
-        st = B.stats()
+        s = B.stats()
+        st = {k: getattr(s, k) for k, _ in s._fields_}
         B.close()
     else:
         from scone_amd import configs, tables as T
@@ -61,8 +62,8 @@ def run(args):
         st = eng.stats()
         eng.eigen_finish()
         eng.close()
-    print(json.dumps({"segments": st.n_segments, "collisions": st.n_collisions, "scores": st.n_scores,
-                      "sites": st.n_sites, "launches": st.transport_launches}), flush=True)
+    print(json.dumps({"segments": st["n_segments"], "collisions": st["n_collisions"], "scores": st["n_scores"],
+                      "sites": st["n_sites"], "launches": st["transport_launches"]}), flush=True)
 
 
 def _read_counters(d):
@@ -92,11 +93,10 @@ def summarize(args):
                 return k
         return None
 
-    copy_k = pick(fetch, "elementwise") or pick(fetch, "copy")
+    # the calibration copy is the largest dispatch of the runtime's copy kernel
     true_kib = COPY_MIB * 1024.0
-    f_fetch = (sum(fetch[copy_k]) / len(fetch[copy_k])) / true_kib
-    cw = pick(write, "elementwise") or pick(write, "copy")
-    f_write = (sum(write[cw]) / len(write[cw])) / true_kib
+    f_fetch = max(fetch[pick(fetch, "copyBuffer")]) / true_kib
+    f_write = max(write[pick(write, "copyBuffer")]) / true_kib
     tk = pick(fetch, args.kernel)
     tw = pick(write, args.kernel)
     fk = sum(fetch[tk]) / len(fetch[tk])
