@@ -11,8 +11,9 @@ A step is one cycle: one batch of every resident problem.  W warmup cycles, then
   plain    the same engine holding one problem at a time (each problem's batches run alone)
 Both run the identical histories (checked on a sample of problems: score results equal to 1e-12).
 Also reported: the serving loop (continuous batching with HBM-budget admission and 1 % early stop)
-on the 17x17 pin-lattice request stream at 50/75/100 % of measured capacity with p50/p99
-latency, and the plain eigenvalue engine on the 17x17 lattice (1e6 histories per cycle).
+on the 17x17 pin-lattice request stream (fixed-source solves of the fissile lattice, fission chains
+followed as secondaries) at 50/75/100 % of measured capacity with p50/p99 latency, and the plain
+eigenvalue engine on the 17x17 lattice (1e6 histories per cycle).
 
 --workload eigen prints the eigenvalue line instead.  Prints one JSON line on stdout.
 """
@@ -219,7 +220,8 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
 
     def request(i, arrival):
         _, _, pop, cycles = configs.lattice_request(i, transport=transport)
-        return Request(comp, src, pop, cycles, 1000 + i, stop_bin=0, stop_rel=stop_rel, arrival=arrival)
+        return Request(comp, src, pop, cycles, 1000 + i, stop_bin=0, stop_rel=stop_rel, arrival=arrival,
+                       buffer=inp["buffer"])
 
     slots, budget = 256, 16 << 30
     # capacity: a saturated pool of requests, all admitted at once

@@ -245,7 +245,7 @@ typedef struct scn_fixed_request {
     int64_t stop_bin;        /* bin whose relative standard deviation may end the solve early; -1: never */
     double stop_rel;         /* target relative standard deviation for stop_bin (after >= 2 batches) */
     int32_t record;          /* 1: keep each history's final RNG state and collision count */
-    int32_t pad;
+    int32_t buffer;          /* 'buffer': depth of a history's LIFO buffer of fission secondaries (0: 50) */
 } scn_fixed_request;
 
 enum { SCN_SLOT_FREE = 0, SCN_SLOT_RUNNING = 1, SCN_SLOT_DONE = 2 };
@@ -331,8 +331,9 @@ int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double 
  * all their histories share one launch sequence, each wave of the persistent transport kernel
  * taking tiles of one problem's queue.  Every slot keeps its own tables, RNG stream and score
  * memory, so a solve gives the same histories alone as in a mix, and tallies equal to summation
- * order.  Problems with fissile materials are refused (secondaries on the history's own stream,
- * fixedSource...:214-243, are not implemented).                                                */
+ * order.  Fission sites of a history are detained in its LIFO buffer and transported after it on
+ * the same RNG stream (fixedSourcePhysicsPackage_class.f90:214-243); more than `buffer` pending
+ * secondaries is an error, as in particleDungeon%detain.                                       */
 int scn_batch_create(const scn_batch_settings* settings, void** batch);
 void scn_batch_destroy(void* batch);
 const char* scn_batch_last_error(const void* batch);

@@ -71,7 +71,7 @@ module scone_gpu_mod
     integer(c_int64_t) :: stop_bin
     real(c_double)     :: stop_rel
     integer(c_int32_t) :: record
-    integer(c_int32_t) :: pad
+    integer(c_int32_t) :: buffer
   end type scn_fixed_request
 
   type, public, bind(C) :: scn_fixed_result

@@ -89,7 +89,7 @@ def lib():
         dp = C.POINTER(C.c_double)
         SRC = C.POINTER(T.Source)
         L.orc_source_sample.argtypes = [P, SRC, C.c_int32, C.c_int64, PS]
-        L.orc_fixed_run.argtypes = [P, SRC, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_double, dp,
+        L.orc_fixed_run.argtypes = [P, SRC, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_double, C.c_int32, dp,
                                     C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
         L.orc_score_result.argtypes = [C.c_double, C.c_double, C.c_int, dp]
         L.orc_close_cycle.argtypes = [dp, dp, dp, C.c_int64, C.c_double]
@@ -278,14 +278,14 @@ def source_sample(comp, src, n, rng_state):
     return bank
 
 
-def fixed_run(comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, hist=False, threads=1):
+def fixed_run(comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, hist=False, threads=1, buffer=50):
     """fixedSourcePhysicsPackage run; returns (bins (n_bins, 2) of mean/sd, batches run, hist)."""
     nb = int(comp.problem.n_bins)
     res = np.zeros(max(1, 2 * nb))
     h = np.zeros(n_batches * pop, dtype=HIST_DTYPE) if hist else None
     done = C.c_int32()
     check(lib().orc_fixed_run(C.byref(comp.problem), C.byref(src.source), pop, n_batches, C.c_int64(_s64(seed)),
-                              stop_bin, stop_rel, res.ctypes.data_as(C.POINTER(C.c_double)),
+                              stop_bin, stop_rel, buffer, res.ctypes.data_as(C.POINTER(C.c_double)),
                               h.ctypes.data_as(C.c_void_p) if h is not None else None, C.byref(done), threads))
     if h is not None:
         h = h[: done.value * pop]

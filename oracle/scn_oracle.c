@@ -614,6 +614,7 @@ typedef struct o_hctx {
     double* bins;          /* collision-clerk BIN memory, or NULL */
     double keff[4];        /* per-history IMP_PROD, SCATTER_PROD, IMP_ABS, ANA_LEAK */
     o_site_buf* sites;
+    int64_t stack_cap;     /* > 0: sites form the history's LIFO buffer of this size (fixed source) */
     orc_hist* h;
     int64_t n_events;
 } o_hctx;
@@ -855,6 +856,12 @@ static void o_collide(o_hctx* X, o_part* p)
                 s.uniqueID = p->coords.uniqueID;
                 s.cellIdx = 0;
                 s.broodID = p->broodID;
+                if (X->stack_cap > 0 && X->sites->n >= X->stack_cap) {
+                    /* particleDungeon_class.f90:167-172 detain beyond the buffer size */
+                    o_fail(SCN_ERR_OVERFLOW, "Run out of space for particles. Max size:%lld", (long long)X->stack_cap);
+                    p->isDead = 1;
+                    return;
+                }
                 o_site_push(X->sites, &s);
                 if (X->h) X->h->n_sites++;
             }
@@ -935,6 +942,99 @@ static void o_history(o_hctx* X, const scn_particle_state* s, int n, uint64_t ma
         X->h->G_end = p.G;
         for (int k = 0; k < 3; ++k) X->h->r_end[k] = p.coords.lvl[0].r[k];
     }
+}
+
+/* fixedSourcePhysicsPackage_class.f90:201-258, one history n (1-based) with its secondaries: fission
+ * sites go to the per-thread buffer (collide(p, tally, buffer, buffer)); when a particle dies the
+ * last detained site is released (particleDungeon_class.f90:266-281, LIFO) and transported on the
+ * same RNG stream, until the buffer is empty.  k_eff = 1. */
+static void o_history_fixed(o_hctx* X, const scn_particle_state* s, int n, uint64_t master)
+{
+    o_part p;
+    memset(&p, 0, sizeof p);
+    p.rng = scn_rng_stride(master, n);
+    scn_particle_state cur = *s;
+    cur.broodID = n;                 /* thisCycle%copy(p, n) */
+    X->sites->n = 0;
+    X->keff[0] = X->keff[1] = X->keff[2] = X->keff[3] = 0.0;
+    int32_t n_coll = 0;
+    for (;;) {
+        /* particle_fromParticleState (particle_class.f90:290-310) */
+        o_init_coords(&p.coords, cur.r, cur.dir);
+        p.w = cur.wgt;
+        p.w0 = cur.wgt;
+        p.G = cur.G;
+        p.E = cur.E;
+        p.isMG = 1;
+        p.type = cur.type;
+        p.time = cur.time;
+        p.lambda = cur.lambda;
+        p.fate = FATE_NO;
+        p.collisionN = cur.collisionN;
+        p.broodID = cur.broodID;
+        p.isDead = 0;
+        p.k_eff = 1.0;
+        o_place_coord(X->P, &p.coords);
+        p.preColl_w = p.w;
+        for (;;) {
+            o_transport(X, &p);
+            if (p.isDead || g_err) break;
+            o_collide(X, &p);
+            n_coll++;
+            if (p.isDead || g_err) break;
+        }
+        if (g_err || X->sites->n == 0) break;
+        cur = X->sites->v[--X->sites->n];
+    }
+    if (X->h) {
+        X->h->n_events = X->n_events;
+        X->h->n_coll = n_coll;
+        X->h->rng_end = p.rng;
+        X->h->w_end = p.w;
+        X->h->fate = p.fate;
+        X->h->G_end = p.G;
+        for (int k = 0; k < 3; ++k) X->h->r_end[k] = p.coords.lvl[0].r[k];
+    }
+}
+
+/* the histories of one fixed-source batch (fixedSourcePhysicsPackage_class.f90:199-261) into BIN */
+static int o_fixed_batch(const scn_problem* P, const scn_particle_state* bank, int32_t n_in, uint64_t master,
+                         double* bins, int32_t buffer_size, orc_hist* hist, int32_t n_threads)
+{
+    int nt = 1;
+#ifdef _OPENMP
+    if (n_threads > 0) nt = n_threads;
+#else
+    (void)n_threads;
+#endif
+    o_site_buf* bufs = (o_site_buf*)calloc((size_t)nt, sizeof(o_site_buf));
+    double** tbins = (double**)calloc((size_t)nt, sizeof(double*));
+    tbins[0] = bins;
+    for (int t = 1; t < nt; ++t) tbins[t] = (double*)calloc((size_t)P->n_bins + 1, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    for (int n = 1; n <= n_in; ++n) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        if (g_err) continue;
+        o_hctx X;
+        memset(&X, 0, sizeof X);
+        X.P = P;
+        X.bins = P->n_bins ? tbins[t] : NULL;
+        X.sites = &bufs[t];
+        X.stack_cap = buffer_size;
+        X.h = hist ? &hist[n - 1] : NULL;
+        if (X.h) memset(X.h, 0, sizeof *X.h);
+        o_history_fixed(&X, &bank[n - 1], n, master);
+    }
+    for (int t = 1; t < nt; ++t)
+        for (int64_t i = 0; i < P->n_bins; ++i) bins[i] = bins[i] + tbins[t][i];
+    for (int t = 0; t < nt; ++t) free(bufs[t].v);
+    for (int t = 1; t < nt; ++t) free(tbins[t]);
+    free(tbins);
+    free(bufs);
+    return g_err;
 }
 
 int orc_eigen_cycle(const scn_problem* P, const scn_particle_state* bank_in, int32_t n_in,
@@ -1286,19 +1386,18 @@ int orc_source_sample(const scn_problem* P, const scn_source* src, int32_t n, in
     return g_err;
 }
 
-/* fixedSourcePhysicsPackage_class.f90:135-147 run and :149-289 cycles for a problem without
- * fissile material (so the per-thread buffer of secondaries stays empty): per batch,
+/* fixedSourcePhysicsPackage_class.f90:135-147 run and :149-289 cycles (one rank, no common
+ * buffer), buffer_size = 'buffer' (default 50): per batch,
  * generate(pop, pRNG); pRNG%stride(totalPop); history n on pRNG strided by n; pRNG%stride(totalPop);
  * reportCycleEnd -> scoreMemory%closeCycle(normFactor) (tallyAdmin_class.f90:736-786).
  * stop_bin >= 0 adds an early stop: after a batch (N >= 2 batches closed), the solve ends when the
  * bin's sd / |mean| <= stop_rel.  bins_result gets (mean, sd) per bin from N batches. */
 int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int32_t n_batches, int64_t seed,
-                  int64_t stop_bin, double stop_rel, double* bins_result, orc_hist* hist, int32_t* n_done,
-                  int32_t n_threads)
+                  int64_t stop_bin, double stop_rel, int32_t buffer_size, double* bins_result, orc_hist* hist,
+                  int32_t* n_done, int32_t n_threads)
 {
     o_clear();
-    for (int m = 0; m < P->mg.n_mat; ++m)
-        if (P->mg.fissile[m]) { o_fail(SCN_ERR_UNSUPPORTED, "fixed source with fissile material"); return g_err; }
+    if (buffer_size < 1) { o_fail(SCN_ERR_ARG, "buffer size must be >= 1"); return g_err; }
     const int64_t nb = P->n_bins;
     scn_particle_state* bank = (scn_particle_state*)calloc((size_t)pop + 1, sizeof(scn_particle_state));
     double* bins = (double*)calloc((size_t)nb + 1, sizeof(double));
@@ -1309,9 +1408,7 @@ int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int3
     for (int b = 0; b < n_batches && !g_err; ++b) {
         if (orc_source_sample(P, src, pop, (int64_t)master, bank)) break;
         master = scn_rng_stride(master, pop);
-        int32_t n_out = 0;
-        if (orc_eigen_cycle(P, bank, pop, (int64_t)master, 1.0, NULL, 0, &n_out, nb ? bins : NULL, NULL,
-                            hist ? hist + (size_t)b * pop : NULL, n_threads))
+        if (o_fixed_batch(P, bank, pop, master, bins, buffer_size, hist ? hist + (size_t)b * pop : NULL, n_threads))
             break;
         master = scn_rng_stride(master, pop);
         double norm = 1.0;

@@ -87,8 +87,8 @@ int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const
 int orc_source_sample(const scn_problem* P, const scn_source* src, int32_t n, int64_t rng_state,
                       scn_particle_state* bank);
 int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int32_t n_batches, int64_t seed,
-                  int64_t stop_bin, double stop_rel, double* bins_result, orc_hist* hist, int32_t* n_done,
-                  int32_t n_threads);
+                  int64_t stop_bin, double stop_rel, int32_t buffer_size, double* bins_result, orc_hist* hist,
+                  int32_t* n_done, int32_t n_threads);
 void orc_score_result(double csum, double csum2, int N, double out[2]);
 void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm);
 double orc_keff_implicit(const double s[4]);

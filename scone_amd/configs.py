@@ -22,9 +22,11 @@ Fixed-source problems (fixedSourcePhysicsPackage dictionaries, ``source`` and ``
 * ``batch_problem``    -- problem p of the 512-problem batch: 1D-3D boxes, 1-8 regions, 2-70
                          groups, 1e3-1e6 histories in 10 batches, point or volume source, mesh flux
                          tally of 1e3-1e5 bins; seeds from splitmix64(0xC3 ^ p).
-* ``lattice_request``  -- a fixed-source solve on the 17x17 pin-lattice geometry (fission turned
-                         into capture), histories log-uniform in 1e3-1e6, with a total-flux bin for
-                         early stopping; the serving workload.
+* ``lattice_request``  -- a fixed-source solve on the 17x17 pin-lattice problem (fission chains
+                         followed as secondaries), histories log-uniform in 1e3-1e6, with a
+                         total-flux bin for early stopping; the serving workload.
+* ``inf_medium_fixed_fissile`` -- the 2-group infinite medium at k_inf < 1 with a point source;
+                         flux per source neutron (T - S - chi nuSf^T)^-1 q.
 """
 from __future__ import annotations
 
@@ -280,6 +282,36 @@ def inf_medium_fixed_flux(mg: dict, probG):
     return np.linalg.solve(np.diag(tot) - P0, q)
 
 
+def inf_medium_fixed_fissile(k_inf: float = 0.6, pop: int = 2000, cycles: int = 20,
+                             transport: str = "transportOperatorST", seed: int = 1):
+    """The 2-group infinite medium with nu scaled to k_inf < 1, driven by a fixed point source
+    (probG): secondaries from fission are transported in the source neutron's history."""
+    inp, mg = inf_medium_2g(seed=seed, transport=transport)
+    m = dict(mg["medium"])
+    m["nu"] = [x * k_inf for x in m["nu"]]
+    tally = {"total": _flux_clerk()}
+    for g in (1, 2):
+        tally[f"flux_g{g}"] = _flux_clerk(g)
+    src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "particle": "neutron", "probG": [0.3, 0.7]}
+    out = _fixed_inp(inp["geometry"], {"medium": m}, src, tally, pop, cycles, seed, transport)
+    return out, {"medium": m}
+
+
+def fixed_flux_with_fission(mg: dict, probG):
+    """Collision-estimator flux per source neutron with fission secondaries (k_eff = 1 in the
+    site count): (T - S - chi nuSf^T)^-1 q."""
+    nG = mg["numberOfGroups"]
+    P0 = np.array(mg["P0"]).reshape(nG, nG).T
+    fis = np.array(mg.get("fission", [0.0] * nG))
+    nu = np.array(mg.get("nu", [0.0] * nG))
+    chi = np.array(mg.get("chi", [0.0] * nG))
+    chi = chi / chi.sum() if chi.sum() > 0 else chi
+    tot = P0.sum(axis=0) + np.array(mg["capture"]) + fis
+    q = np.asarray(probG, dtype=float)
+    q = q / q.sum()
+    return np.linalg.solve(np.diag(tot) - P0 - np.outer(chi, nu * fis), q)
+
+
 def _region_geometry(dim, hw, mats, bc):
     """Box root -> lattice (k,1,1 or k,1,0) -> single-fill pin universe per region."""
     k = len(mats)
@@ -342,17 +374,21 @@ def batch_problem(p: int, transport: str = "transportOperatorDT", scale: float =
     return inp, mats, pop, cycles
 
 
-def lattice_request(i: int, transport: str = "transportOperatorDT", scale: float = 1.0):
-    """Serving request i: the 17x17 pin-lattice geometry with fission turned into capture, a
-    material source in the fuel, histories log-uniform in 1e3-1e6 split in 10 batches.
+def lattice_request(i: int, transport: str = "transportOperatorDT", scale: float = 1.0, fissile: bool = True):
+    """Serving request i: a fixed-source solve on the 17x17 pin-lattice problem of
+    ``pin_lattice_7g`` (its fissile fuel, structural and moderator materials; k_inf 0.87, so
+    fission chains are finite and are followed as secondaries), a material source in the fuel,
+    histories log-uniform in 1e3-1e6 split in 10 batches, a total-flux bin (bin 0) for early
+    stopping and a per-pin flux map.  ``fissile=False`` turns fission into capture.
     Returns (input dict, mg dicts, histories per batch, batches)."""
     rng = np.random.default_rng(splitmix64(0xC4 ^ i))
     inp, mg = pin_lattice_7g(pop=1000, transport=transport, tallies=False)
     mg = {n: dict(d) for n, d in mg.items()}
-    f = mg["fuel"]
-    f["capture"] = [c + x for c, x in zip(f["capture"], f["fission"])]
-    for key in ("fission", "nu", "chi"):
-        f.pop(key)
+    if not fissile:
+        f = mg["fuel"]
+        f["capture"] = [c + x for c, x in zip(f["capture"], f["fission"])]
+        for key in ("fission", "nu", "chi"):
+            f.pop(key)
     hist = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
     cycles = 10
     pop = max(1, int(round(hist * scale / cycles)))
@@ -366,6 +402,7 @@ def lattice_request(i: int, transport: str = "transportOperatorDT", scale: float
                       "response": ["flux"], "flux": {"type": "fluxResponse"}}}
     src = {"type": "materialSource", "data": "mg", "mat": "fuel", "G": 1}
     out = _fixed_inp(inp["geometry"], mg, src, tally, pop, cycles, int(rng.integers(1, 1 << 31)), transport)
+    out["buffer"] = 1000
     return out, mg, pop, cycles
 
 

@@ -39,6 +39,8 @@ constexpr int BWAVE = 64;
 constexpr int MAX_SLOTS = 4096;
 constexpr int MATERIAL_SOURCE_ATTEMPTS = 200;   // materialSource_class.f90:168
 constexpr int MAX_PROBG = 4096;
+constexpr int DEFAULT_BUFFER = 50;   // fixedSourcePhysicsPackage_class.f90:372 'buffer' default
+constexpr int MAX_BUFFER = 1024;
 // a history still alive after this many (flight, collision) pairs in one launch is reported as lost
 // (an error, not a hang)
 constexpr int MAX_PAIRS_PER_LAUNCH = 1 << 22;
@@ -77,6 +79,7 @@ struct SlotDev {
     uint64_t seed;
     int src_type, src_iso, src_G, src_mat, n_probG;
     int pop, n_batches;
+    int buffer;         // 'buffer' of the package: depth of a history's LIFO buffer of secondaries
 };
 
 // mutable state of a slot, written by the kernels
@@ -363,7 +366,7 @@ __device__ __forceinline__ void flush_lds_bins(const Bins& B, int lane)
 template <int NL, int MODE>
 __global__ void __launch_bounds__(BBLOCK)
 k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __restrict__ tiles, int* ctl, int first,
-             int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave)
+             int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave, Sites SB)
 {
     extern __shared__ double lds_all[];
     const int lane = threadIdx.x & (BWAVE - 1);
@@ -372,9 +375,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     double* wl = lds_all + (size_t)(threadIdx.x / BWAVE) * lds_per_wave;
     for (int i = lane; i < lds_per_wave; i += BWAVE) wl[i] = 0.0;
     const int n_tiles = ctl[0];
-    Sites S0;   // fixed-source slots hold no fissile material: the site bank is never written
-    S0.cap = 0;
-    S0.count = ctl + 4;
+    const int lane_gid = blockIdx.x * BBLOCK + threadIdx.x;   // owner of one LIFO buffer in SB
     int s = -1;                      // current slot (wave-uniform)
     int next = 0, end = 0;           // particles [next, end) of the current tile not yet started
     int pend_s = -1, pend_b = 0;     // a tile of another slot, started once the lanes are idle
@@ -426,6 +427,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
             cap = D->pop;
             B.glob = D->bin;
             B.lds_n = min((int64_t)lds_per_wave, D->P.n_bins);
+            SB.depth = D->buffer;
             next = pend_b;
             end = min(pend_b + tile, n_in);
         }
@@ -440,6 +442,8 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                     relink(D->P, p);
                     alive = 1;
                 }
+                p.sp = 0;
+                p.lane = lane_gid;
                 steps = 0;
                 nseg = 0;
                 cache.lvl = 0;
@@ -460,10 +464,10 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 r = STEP_DEAD;
             }
             if (r == STEP_COLLIDE) {
-                if (collide(P, p, 1.0, B, S0, st, cn)) {
+                if (collide(P, p, 1.0, B, SB, st, cn)) {
                     cache.lvl = 0;
                     nseg = 0;
-                    if (++steps >= max_steps) {
+                    if (++steps >= max_steps && p.sp == 0) {
                         park = 1;
                         alive = 0;
                     } else if (steps >= MAX_PAIRS_PER_LAUNCH) {
@@ -473,6 +477,35 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 } else {
                     r = STEP_DEAD;
                 }
+            }
+            if (r == STEP_DEAD && p.sp > 0 && !st->code) {
+                // buffer%release: the last detained secondary continues the history on the same
+                // RNG stream (fixedSourcePhysicsPackage_class.f90:239-243, particle_class.f90:290-310)
+                const size_t e = (size_t)(--p.sp) * SB.stride + p.lane;
+#pragma unroll
+                for (int l = 0; l < NL; ++l) {
+                    p.loc[l] = 0;
+                    p.uni[l] = 0;
+                    p.root[l] = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) p.r[l][k] = 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    p.r[0][k] = SB.r[(size_t)k * SB.cap + e];
+                    p.u[k] = SB.u[(size_t)k * SB.cap + e];
+                }
+                p.w = SB.w[e];
+                p.w0 = p.w;
+                p.t = SB.t[e];
+                p.G = SB.G[e];
+                p.dead = 0;
+                p.fate = FATE_NO;
+                p.track = 0.0;
+                place_coord(P, p, st);
+                cache.lvl = 0;
+                nseg = 0;
+                r = STEP_MORE;
             }
             if (r == STEP_DEAD) {
                 alive = 0;
@@ -575,14 +608,15 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
 // host side
 // ---------------------------------------------------------------------------------------------
 typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int,
-                             int, int, int, DevStatus*, int);
+                             int, int, int, DevStatus*, int, Sites);
 
 template <int NL, int MODE>
 void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles,
-               int* ctl, int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave)
+               int* ctl, int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave,
+               Sites SB)
 {
     hipLaunchKernelGGL((k_btransport<NL, MODE>), g, b, sh, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
-                       max_steps, st, lds_per_wave);
+                       max_steps, st, lds_per_wave, SB);
 }
 
 template <int NL, int MODE>
@@ -608,6 +642,7 @@ struct SlotHost {
     int depth = 0;
     int64_t n_bins = 0;
     int pop = 0, n_batches = 0, record = 0;
+    int fissile = 0, buffer = 0;
 };
 
 struct Batch {
@@ -636,6 +671,9 @@ struct Batch {
     scn_stats stats{};
     int64_t rel_seg = 0, rel_col = 0, rel_sc = 0, rel_hist = 0;   // counters of released slots
     int S_lds_per_wave = 2048;   // low bins of a slot privatised per wave in LDS (16 KiB per wave)
+    // per-lane LIFO buffers of fixed-source secondaries (allocated when a fissile problem is resident)
+    Sites stack{};
+    void* stack_mem = nullptr;
 };
 
 #define BHIPCHK(expr)                                                                    \
@@ -759,11 +797,10 @@ int validate(Batch* B, const scn_fixed_request* R, int* depth)
         B->err = "problem transport operator differs from the batch engine's";
         return SCN_ERR_ARG;
     }
-    for (int m = 0; m < pr->mg.n_mat; ++m)
-        if (pr->mg.fissile[m]) {
-            B->err = "fixed-source problems with fissile material are not supported (secondaries)";
-            return SCN_ERR_UNSUPPORTED;
-        }
+    if (R->buffer < 0 || R->buffer > MAX_BUFFER) {
+        B->err = "buffer size outside 0..MAX_BUFFER";
+        return SCN_ERR_ARG;
+    }
     int d = problem_depth(pr);
     if (d < 1 || d > SCN_MAX_NEST) {
         B->err = "geometry nesting deeper than SCN_MAX_NEST or malformed graph";
@@ -822,7 +859,7 @@ int check_bstatus(Batch* B)
         char buf[256];
         const char* what = "engine error";
         switch (h.code) {
-        case SCN_ERR_OVERFLOW: what = "Run out of space for particles (slot queue)"; break;
+        case SCN_ERR_OVERFLOW: what = "Run out of space for particles (buffer of secondaries or slot queue)"; break;
         case SCN_ERR_MATERIAL: what = "Particle is in an undefined or overlapping material"; break;
         case SCN_ERR_NAN: what = "Distance is NaN"; break;
         case SCN_ERR_MAJORANT: what = "Majorant is 0"; break;
@@ -879,7 +916,7 @@ int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
         const int steps = last ? (1 << 30) : (B->S.k0 << L);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
         fn(dim3(grid), dim3(BBLOCK), lds_bytes(B), B->stream, B->d_slots, B->d_state, B->d_tiles, B->d_ctl, first,
-           in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave);
+           in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave, B->stack);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L + 1], B->stream));
         side = out_side;
     }
@@ -899,12 +936,46 @@ void drop_graph(Batch* B)
     B->graph = nullptr;
 }
 
+// per-lane LIFO buffers for the persistent grid, as deep as the deepest resident fissile slot needs
+int ensure_stack(Batch* B, int grid)
+{
+    int depth = 0;
+    for (const SlotHost& h : B->hs)
+        if (h.used && h.fissile) depth = std::max(depth, h.buffer);
+    if (depth == 0) return 0;
+    const int lanes = grid * BBLOCK;
+    if (B->stack_mem && B->stack.stride >= lanes && B->stack.depth >= depth) return 0;
+    drop_graph(B);
+    if (B->stack_mem) {
+        BHIPCHK(hipStreamSynchronize(B->stream));
+        BHIPCHK(hipFree(B->stack_mem));
+        B->stack_mem = nullptr;
+    }
+    const size_t n = (size_t)lanes * depth;
+    const size_t bytes = n * (8 * sizeof(double) + sizeof(int)) + 1024;
+    BHIPCHK(hipMalloc(&B->stack_mem, bytes));
+    char* q = (char*)B->stack_mem;
+    Sites S{};
+    S.r = (double*)q;
+    S.u = S.r + 3 * n;
+    S.w = S.u + 3 * n;
+    S.t = S.w + n;
+    S.G = (int*)(S.t + n);
+    S.cap = (int)n;
+    S.stride = lanes;
+    S.depth = depth;
+    B->stack = S;
+    return 0;
+}
+
 int run_cycle(Batch* B, int* n_running_out)
 {
     const int nl = max_depth_nl(B);
     BTransportFn fn;
     int grid = 0;
     int rc0 = transport_variant(B, nl, &fn, &grid);
+    if (rc0) return rc0;
+    rc0 = ensure_stack(B, grid);
     if (rc0) return rc0;
     BHIPCHK(hipEventRecord(B->c0, B->stream));
     if (B->S.use_graph) {
@@ -1040,6 +1111,7 @@ void scn_batch_destroy(void* batch)
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     drop_graph(B);
     if (B->arena) (void)hipFree(B->arena);
+    if (B->stack_mem) (void)hipFree(B->stack_mem);
     if (B->d_tiles) (void)hipFree(B->d_tiles);
     if (B->d_slots) (void)hipFree(B->d_slots);
     if (B->d_state) (void)hipFree(B->d_state);
@@ -1179,6 +1251,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     D.n_probG = (int)probG.size();
     D.pop = R->pop;
     D.n_batches = R->n_batches;
+    D.buffer = R->buffer > 0 ? R->buffer : DEFAULT_BUFFER;
     SlotState st{};
     st.running = 1;
     auto undo = [&](int code) {
@@ -1203,6 +1276,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     h.pop = R->pop;
     h.n_batches = R->n_batches;
     h.record = R->record;
+    h.fissile = 0;
+    for (int m = 0; m < nM; ++m) h.fissile |= pr->mg.fissile[m] ? 1 : 0;
+    h.buffer = D.buffer;
     *slot = s;
     return 0;
 }

@@ -82,6 +82,7 @@ struct Pt {
     double w, w0, t;
     uint64_t rng;
     int brood, nsite, coll;
+    int sp, lane;      // fixed source: depth of the lane's LIFO buffer of secondaries, lane index
     double k[4];       // keffImplicitClerk scores of this history
     double track;      // trackingCache(1)%xs
     int fate;
@@ -794,7 +795,10 @@ __device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const B
     }
 }
 
-// fission-site sink: structure-of-arrays append buffer, sorted by key afterwards
+// fission-site sink.  Eigenvalue (stride == 0): structure-of-arrays append buffer, sorted by key
+// afterwards.  Fixed source (stride > 0): every lane owns a LIFO buffer of `depth` entries (the
+// package's per-thread `buffer`); entry j of lane l is element j*stride + l, and only r, u, w, t, G
+// are kept (the rest of the released state is fixed by particle_fromParticleState).
 struct Sites {
     double* r;        // [3][cap]
     double* u;        // [3][cap]
@@ -807,6 +811,8 @@ struct Sites {
     uint32_t* order;  // identity, sorted along with the keys
     int* count;
     int cap;
+    int stride;       // lanes of the per-lane buffers (0: append buffer)
+    int depth;        // capacity of one lane's buffer
 };
 
 // CDF scan of multiScatterMG%sampleGout / fissionMG%sampleOut: subtract x[0], x[1], ... from
@@ -874,7 +880,19 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
         if (n >= 1) {
             wgt = scn_sign(w0, wgt);
             const double* chi = P.chi + (size_t)(mat - 1) * nG;
-            int base = atomicAdd(S.count, n);
+            int base;
+            if (S.stride > 0) {
+                // detain into the history's buffer (particleDungeon_class.f90:159-176)
+                if (p.sp + n > S.depth) {
+                    raise_error(st, SCN_ERR_OVERFLOW, -2);
+                    p.dead = 1;
+                    return 0;
+                }
+                base = p.sp;
+                p.sp += n;
+            } else {
+                base = atomicAdd(S.count, n);
+            }
             for (int i = 0; i < n; ++i) {
                 double mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
                 double phi = SCN_TWO_PI * scn_rng_get(&p.rng);
@@ -887,6 +905,18 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
                 }
                 double d[3];
                 scn_rotate(p.u, mu, phi, d);
+                if (S.stride > 0) {
+                    const size_t e = (size_t)(base + i) * S.stride + p.lane;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        S.r[(size_t)k * S.cap + e] = p.r[0][k];
+                        S.u[(size_t)k * S.cap + e] = d[k];
+                    }
+                    S.w[e] = wgt;
+                    S.t[e] = p.t;
+                    S.G[e] = G_out;
+                    continue;
+                }
                 int slot = base + i;
                 if (slot < S.cap) {
                     for (int k = 0; k < 3; ++k) {

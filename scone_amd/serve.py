@@ -32,6 +32,7 @@ class Request:
     stop_bin: int = -1
     stop_rel: float = 0.0
     record: bool = False
+    buffer: int = 0         # depth of a history's buffer of secondaries (0: the package default, 50)
     arrival: float = 0.0    # seconds (wall mode) or cycle index (cycle mode)
     # filled by the server
     admitted: float = -1.0
@@ -59,7 +60,7 @@ class Server:
             rq = waiting[0]
             try:
                 rq.slot = self.B.add(rq.comp, rq.src, rq.pop, rq.n_batches, rq.seed, stop_bin=rq.stop_bin,
-                                     stop_rel=rq.stop_rel, record=rq.record)
+                                     stop_rel=rq.stop_rel, record=rq.record, buffer=rq.buffer)
             except Busy:
                 self.busy_rejections += 1
                 return

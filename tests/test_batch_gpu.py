@@ -11,7 +11,7 @@ import pytest
 import binding as O
 from helpers import assert_results_close
 from scone_amd import configs, tables as T
-from scone_amd.engine import Batch, Busy
+from scone_amd.engine import Batch, Busy, EngineError
 
 pytestmark = pytest.mark.gpu
 
@@ -21,9 +21,9 @@ def compiled(inp, mg):
     return comp, T.compile_source(inp["source"], comp)
 
 
-def oracle_ref(comp, src, pop, cycles, seed, stop_bin=-1, stop_rel=0.0):
+def oracle_ref(comp, src, pop, cycles, seed, stop_bin=-1, stop_rel=0.0, buffer=50):
     res, done, h = O.fixed_run(comp, src, pop, cycles, seed, stop_bin=stop_bin, stop_rel=stop_rel, hist=True,
-                               threads=8)
+                               threads=8, buffer=buffer)
     return res, done, h
 
 
@@ -44,6 +44,10 @@ CASES = [
     ("inf-DT", lambda: configs.inf_medium_fixed(pop=2000, cycles=5, transport="transportOperatorDT"), 2000, 5),
     ("slab-ST", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorST"), 1500, 4),
     ("slab-DT", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorDT"), 1500, 4),
+    ("fissile-ST", lambda: configs.inf_medium_fixed_fissile(pop=1500, cycles=4, transport="transportOperatorST"),
+     1500, 4),
+    ("fissile-DT", lambda: configs.inf_medium_fixed_fissile(pop=1500, cycles=4, transport="transportOperatorDT"),
+     1500, 4),
 ]
 
 
@@ -120,6 +124,28 @@ def test_early_stop_and_slot_reuse():
     B.close()
 
 
+def test_fissile_lattice_with_secondaries_and_buffer_overflow():
+    """Fission chains of the 17x17 lattice (k_inf 0.87) followed as secondaries on the history's
+    stream; a buffer too small for them is the package's 'Run out of space' error."""
+    print("case lattice secondaries", flush=True)
+    inp, mg, pop, cycles = configs.lattice_request(3, scale=0.02)
+    comp, src = compiled(inp, mg)
+    ref = oracle_ref(comp, src, pop, 3, inp["seed"], buffer=1000)
+    assert ref[2]["n_coll"].max() > 50
+    B = Batch(transport=comp.problem.transport, max_slots=2, hbm_budget=256 << 20, n_launch=3, k0=4)
+    s = B.add(comp, src, pop, 3, inp["seed"], record=True, buffer=1000)
+    B.run_all()
+    check_slot(B, s, ref, "lattice")
+    B.close()
+    with pytest.raises(RuntimeError, match="Run out of space"):
+        O.fixed_run(comp, src, pop, 1, inp["seed"], buffer=1)
+    B = Batch(transport=comp.problem.transport, max_slots=1, hbm_budget=256 << 20)
+    B.add(comp, src, pop, 1, inp["seed"], buffer=1)
+    with pytest.raises(EngineError, match="Run out of space"):
+        B.run_all()
+    B.close()
+
+
 def test_budget_admission():
     print("case budget", flush=True)
     inp, mg = configs.inf_medium_fixed(pop=100000, cycles=1, transport="transportOperatorDT")
@@ -144,7 +170,7 @@ def test_serving_results_same_alone_and_in_mix():
     reqs = []
     for i in range(10):
         _, _, pop, cycles = configs.lattice_request(i, transport="transportOperatorDT", scale=0.005)
-        reqs.append(Request(comp, src, pop, cycles, 500 + i, stop_bin=0, stop_rel=0.05, record=True,
+        reqs.append(Request(comp, src, pop, cycles, 500 + i, stop_bin=0, stop_rel=0.05, record=True, buffer=1000,
                             arrival=(i * 3) // 2))
     B = Batch(transport=comp.problem.transport, max_slots=4, hbm_budget=256 << 20)
     srv = Server(B)
@@ -152,12 +178,12 @@ def test_serving_results_same_alone_and_in_mix():
     assert srv.busy_rejections > 0          # more requests than slots: some had to wait
     B.close()
     for i, rq in enumerate(reqs):
-        ref = oracle_ref(comp, src, rq.pop, rq.n_batches, rq.seed, stop_bin=0, stop_rel=0.05)
+        ref = oracle_ref(comp, src, rq.pop, rq.n_batches, rq.seed, stop_bin=0, stop_rel=0.05, buffer=1000)
         assert rq.result.n_batches == ref[1], i
         assert np.array_equal(rq.hist_rng, ref[2]["rng_end"]), i
         assert_results_close(ref[0], rq.bins, ref[1], 1e-12, f"request {i}")
         A = Batch(transport=comp.problem.transport, max_slots=1, hbm_budget=64 << 20)
-        a = A.add(comp, src, rq.pop, rq.n_batches, rq.seed, stop_bin=0, stop_rel=0.05, record=True)
+        a = A.add(comp, src, rq.pop, rq.n_batches, rq.seed, stop_bin=0, stop_rel=0.05, record=True, buffer=1000)
         A.run_all()
         r, bins, hr, hc = A.query(a)
         A.close()

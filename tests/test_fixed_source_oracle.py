@@ -8,7 +8,9 @@
 * The stride bookkeeping of fixedSourcePhysicsPackage%cycles: one skip by 2*pop*b + j equals the
   package's sequence of strides (what the engine relies on to start any batch directly).
 * Whole-run closed forms: non-multiplying infinite medium, flux per source neutron (T - S)^-1 q;
-  pure absorber slab with a centred isotropic point source, flux (1 - E2(Sigma_t a)) / Sigma_t.
+  subcritical infinite medium (k_inf = 0.6) whose fission secondaries are transported in the
+  source neutron's history, (T - S - chi nuSf^T)^-1 q; pure absorber slab with a centred
+  isotropic point source, flux (1 - E2(Sigma_t a)) / Sigma_t.
 """
 import math
 
@@ -194,12 +196,32 @@ def test_early_stop_on_relative_sd():
     assert stop[0, 1] / abs(stop[0, 0]) <= rel * 2.0
 
 
-def test_fissile_fixed_source_is_refused():
-    inp, mg = configs.inf_medium_2g(pop=100)
-    comp = T.compile_problem(inp, mg)
-    src = T.compile_source({"type": "pointSource", "r": [0.0, 0.0, 0.0], "G": 1}, comp)
-    with pytest.raises(RuntimeError, match="fissile"):
-        O.fixed_run(comp, src, 10, 1, 1)
+@pytest.mark.parametrize("top", ["transportOperatorST", "transportOperatorDT"])
+def test_subcritical_multiplication_closed_form(top):
+    """Fission secondaries in the source neutron's history (fixedSourcePhysicsPackage buffer):
+    infinite medium at k_inf = 0.6, flux per source neutron (T - S - chi nuSf^T)^-1 q."""
+    pop, cycles = 4000, 20
+    inp, mg = configs.inf_medium_fixed_fissile(k_inf=0.6, pop=pop, cycles=cycles, transport=top)
+    comp, src = compiled(inp, mg)
+    k_inf, _ = configs.k_inf_and_flux(mg["medium"])
+    assert abs(k_inf - 0.6) < 1e-12
+    res, done, h = O.fixed_run(comp, src, pop, cycles, 1, hist=True, threads=4)
+    phi = configs.fixed_flux_with_fission(mg["medium"], inp["source"]["probG"])
+    expect = np.concatenate([[phi.sum()], phi]) * pop
+    z = (res[:, 0] - expect) / res[:, 1]
+    assert np.all(np.abs(z) < 3), z
+    # without secondaries the flux would be (T - S)^-1 q: the multiplication is real
+    phi0 = configs.inf_medium_fixed_flux(mg["medium"], inp["source"]["probG"])
+    assert phi.sum() > 1.15 * phi0.sum()
+    assert (res[0, 0] - pop * phi0.sum()) / res[0, 1] > 10
+    assert h["n_coll"].max() > h["n_coll"].mean() * 5
+
+
+def test_secondary_buffer_overflow_is_an_error():
+    inp, mg = configs.inf_medium_fixed_fissile(k_inf=0.9, pop=200, cycles=1)
+    comp, src = compiled(inp, mg)
+    with pytest.raises(RuntimeError, match="Run out of space"):
+        O.fixed_run(comp, src, 200, 1, 1, buffer=1)
 
 
 def test_batch_problems_compile_and_run_small():
