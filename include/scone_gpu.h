@@ -49,7 +49,7 @@ enum {
 #define SCN_BC_REFLECTIVE 1
 #define SCN_BC_PERIODIC 2
 
-enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2 };
+enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2, SCN_TRANSPORT_HT = 3 };
 enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3 };
 enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2 };
 enum { SCN_GRID_LIN = 1, SCN_GRID_UNSTRUCT = 3 };
@@ -157,7 +157,7 @@ typedef struct scn_clerk {
 } scn_clerk;
 
 typedef struct scn_problem {
-    int32_t transport;       /* SCN_TRANSPORT_ST / SCN_TRANSPORT_DT */
+    int32_t transport;       /* SCN_TRANSPORT_ST / _DT / _HT (transportOperatorFactory_func.f90:36-67) */
     int32_t st_cache;        /* transportOperatorST 'cache' flag (default 1) */
     int32_t n_universes;
     int32_t root_uni;        /* uniIdx of the root universe */
@@ -178,6 +178,7 @@ typedef struct scn_problem {
     int64_t n_bins;          /* size of the collision-clerk score memory */
     int64_t norm_bin;        /* tallyAdmin 'norm': 0-based BIN address of the norm clerk, -1 none */
     double norm_value;       /* tallyAdmin 'normVal' */
+    double ht_cutoff;        /* transportOperatorHT 'cutoff' (default 0.9): DT when sigmaT/majorant > 1 - cutoff */
 } scn_problem;
 
 /* ---- eigenvalue run ------------------------------------------------------------------------ */

@@ -781,10 +781,23 @@ static void o_delta_tracking(o_hctx* X, o_part* p)
     }
 }
 
+/* transportOperatorHT_class.f90:41-64 tracking_selection: delta tracking when
+ * sigmaT * (1 / majorant) > 1 - cutoff, surface tracking otherwise (sigmaT = 0 in void) */
+static int o_ht_use_dt(const scn_problem* P, const o_part* p)
+{
+    double majorant_inv = 1.0 / scn_fmax(P->mg.majorant[p->G - 1] + 0.0, P->mg.collision_xs);
+    int mat = p->coords.matIdx;
+    double sigmaT = (mat == SCN_VOID_MAT) ? 0.0 : o_track_mat_xs(P, mat, p->G);
+    double ratio = sigmaT * majorant_inv;
+    return ratio > (1.0 - P->ht_cutoff);
+}
+
 /* transportOperator_inter.f90:105-130 transport */
 static void o_transport(o_hctx* X, o_part* p)
 {
-    if (X->P->transport == SCN_TRANSPORT_DT) o_delta_tracking(X, p);
+    int dt = X->P->transport == SCN_TRANSPORT_DT ||
+             (X->P->transport == SCN_TRANSPORT_HT && o_ht_use_dt(X->P, p));
+    if (dt) o_delta_tracking(X, p);
     else o_surface_tracking(X, p);
     if (p->isDead) o_report_hist(X, p);
 }

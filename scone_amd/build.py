@@ -1,7 +1,8 @@
 """Build libscone_amd.so (HIP, gfx950) in-tree with hipcc.
 
 FP discipline: no contraction and no fast-math on host or device, so the engine's arithmetic is
-the IEEE sequence the CPU oracle evaluates.  Packed FP32 instructions are disabled.
+the IEEE sequence the CPU oracle evaluates.  Packed FP32 instructions are disabled.  The
+translation units compile in parallel, then link into one shared library.
 """
 from __future__ import annotations
 
@@ -13,10 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libscone_amd.so")
 SRC = [os.path.join(HERE, "csrc", "scn_engine.hip"), os.path.join(HERE, "csrc", "scn_batch.hip")]
+OBJ_DIR = os.path.join(HERE, "_build")
 ARCH = os.environ.get("SCN_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -26,14 +28,31 @@ FLAGS = [
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs, procs = [], []
+    for src in SRC:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(os.path.basename(src))[0] + ".o")
+        cmd = [hipcc] + FLAGS + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = False
+    for src, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed compiling libscone_amd.so sources")
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + FLAGS + SRC + ["-o", tmp]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libscone_amd.so")
+        raise RuntimeError("hipcc failed linking libscone_amd.so")
     os.replace(tmp, LIB)
     return LIB
 

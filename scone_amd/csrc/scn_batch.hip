@@ -392,6 +392,8 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     DistCache<NL> cache;   // surface tracking: the current flight's distance cache
     cache.lvl = 0;
     int alive = 0, steps = 0, nseg = 0;
+    int use_dt = MODE == SCN_TRANSPORT_DT;   // this flight's tracking method (HT: chosen per flight)
+    int fresh = 1;                           // the next event starts a flight
     for (int iter = 0;; ++iter) {
         while (next >= end && !exhausted && pend_s < 0) {
             int t = 0;
@@ -447,6 +449,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 steps = 0;
                 nseg = 0;
                 cache.lvl = 0;
+                fresh = 1;
             }
             next = min(next + __popcll(idle), end);
         }
@@ -458,7 +461,9 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
         if (alive) {
             // one tracking event; a real collision is processed in the same iteration
             const DevProblem& P = D->P;
-            int r = (MODE == SCN_TRANSPORT_DT) ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
+            if (MODE == SCN_TRANSPORT_HT && fresh) use_dt = ht_use_dt(P, p, st);
+            fresh = 0;
+            int r = use_dt ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
             if (r == STEP_MORE && ++nseg >= MAX_SEGMENTS_PER_FLIGHT) {
                 raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
                 r = STEP_DEAD;
@@ -467,6 +472,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 if (collide(P, p, 1.0, B, SB, st, cn)) {
                     cache.lvl = 0;
                     nseg = 0;
+                    fresh = 1;
                     if (++steps >= max_steps && p.sp == 0) {
                         park = 1;
                         alive = 0;
@@ -505,6 +511,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 place_coord(P, p, st);
                 cache.lvl = 0;
                 nseg = 0;
+                fresh = 1;
                 r = STEP_MORE;
             }
             if (r == STEP_DEAD) {
@@ -893,6 +900,7 @@ int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
 {
     const void* kern;
     if (B->S.transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(nl, fn, &kern);
+    else if (B->S.transport == SCN_TRANSPORT_HT) pick_bt<SCN_TRANSPORT_HT>(nl, fn, &kern);
     else pick_bt<SCN_TRANSPORT_ST>(nl, fn, &kern);
     int occ = 0;
     BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, lds_bytes(B)));
@@ -1053,7 +1061,7 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
         return SCN_ERR_NO_DEVICE;
     }
     if (!settings || settings->max_slots < 1 || settings->max_slots > MAX_SLOTS || settings->hbm_budget < (1 << 20) ||
-        (settings->transport != SCN_TRANSPORT_ST && settings->transport != SCN_TRANSPORT_DT) ||
+        (settings->transport < SCN_TRANSPORT_ST || settings->transport > SCN_TRANSPORT_HT) ||
         (settings->tile != 0 && (settings->tile % BWAVE != 0 || settings->tile > 4096)) || settings->k0 < 0 ||
         settings->n_launch < 0 || settings->n_launch > 16) {
         g_batch_create_err = "invalid batch settings";
@@ -1223,6 +1231,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.gclerk_off = (const int*)(base + o_goff);
     P.gclerk_idx = (const int*)(base + o_gidx);
     P.n_bins = pr->n_bins;
+    P.ht_cutoff = pr->ht_cutoff;
     carve_live(base + Ly.queue, depth, R->pop, D.q[0]);
     carve_live(base + Ly.queue + Ly.queue_bytes, depth, R->pop, D.q[1]);
     D.bin = (double*)(base + Ly.bins);

@@ -43,6 +43,7 @@ struct DevProblem {
     const int* gclerk_off;   // [nG+1] clerks whose energy filter admits group g
     const int* gclerk_idx;
     int64_t n_bins;
+    double ht_cutoff;
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -761,6 +762,17 @@ __device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, const Bin
     if (scn_rng_get(&p.rng) < sigmaT * majorant_inv) return STEP_COLLIDE;
     cnt.sc += report_in_coll(P, p, 1, B, st);
     return STEP_MORE;
+}
+
+// transportOperatorHT tracking_selection (transportOperatorHT_class.f90:41-64): delta tracking for
+// this flight when sigmaT * (1 / majorant) > 1 - cutoff (sigmaT = 0 in void)
+template <int NL>
+__device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL>& p, DevStatus* st)
+{
+    double majorant_inv = 1.0 / scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
+    double sigmaT = (p.mat == SCN_VOID_MAT) ? 0.0 : track_mat_xs(P, p.mat, p.G, st);
+    double ratio = sigmaT * majorant_inv;
+    return ratio > (1.0 - P.ht_cutoff);
 }
 
 // surfaceTracking: segments until a real collision or a leak; returns 1 if alive

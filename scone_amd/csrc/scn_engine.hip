@@ -167,6 +167,8 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     DistCache<NL> cache;
     cache.lvl = 0;
     int alive = 0, steps = 0, nseg = 0;
+    int use_dt = MODE == SCN_TRANSPORT_DT;   // this flight's tracking method (HT: chosen per flight)
+    int fresh = 1;                           // the next event starts a flight
     for (int iter = 0;; ++iter) {
         if (next >= end && !exhausted) {
             int c = 0;
@@ -221,6 +223,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
                 steps = 0;
                 nseg = 0;
                 cache.lvl = 0;
+                fresh = 1;
             }
             next = min(next + __popcll(idle), end);
         }
@@ -230,7 +233,9 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
         }
         int park = 0;
         if (alive) {
-            int r = (MODE == SCN_TRANSPORT_DT) ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
+            if (MODE == SCN_TRANSPORT_HT && fresh) use_dt = ht_use_dt(P, p, st);
+            fresh = 0;
+            int r = use_dt ? dt_step(P, p, B, st, cn) : st_step(P, p, cache, B, st, cn);
             if (r == STEP_MORE && ++nseg >= MAX_SEGMENTS_PER_FLIGHT) {
                 raise_error(st, SCN_ERR_ARG, SCN_ERR_LOST_DETAIL);
                 r = STEP_DEAD;
@@ -239,6 +244,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
                 if (collide(P, p, k_eff, B, S, st, cn)) {
                     cache.lvl = 0;
                     nseg = 0;
+                    fresh = 1;
                     if (++steps >= max_steps) {
                         park = 1;
                         alive = 0;
@@ -771,8 +777,9 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     HIPCHK(hipMemsetAsync(E->counters_i, 0, sizeof(int) * 4, E->stream));
     HIPCHK(hipMemsetAsync(E->sites.count, 0, sizeof(int), E->stream));
     const void* kern = nullptr;
-    TransportFn fn = (E->transport == SCN_TRANSPORT_DT) ? pick_nl<SCN_TRANSPORT_DT>(E->nl, &kern)
-                                                        : pick_nl<SCN_TRANSPORT_ST>(E->nl, &kern);
+    TransportFn fn = (E->transport == SCN_TRANSPORT_DT)   ? pick_nl<SCN_TRANSPORT_DT>(E->nl, &kern)
+                     : (E->transport == SCN_TRANSPORT_HT) ? pick_nl<SCN_TRANSPORT_HT>(E->nl, &kern)
+                                                          : pick_nl<SCN_TRANSPORT_ST>(E->nl, &kern);
     double* bins = score_bins ? E->bins : nullptr;
     // a null BIN pointer means "do not score": point at a scratch copy instead
     double* bins_arg = bins ? bins : E->bins;
@@ -1060,6 +1067,7 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.gclerk_off = d_goff;
     P.gclerk_idx = d_gidx;
     P.n_bins = pr->n_bins;
+    P.ht_cutoff = pr->ht_cutoff;
     // buffers
     const int cap = capacity;
     E->cap = cap;

@@ -32,7 +32,7 @@ UNDEF_MAT = 2147483646
 INF = 2.0 ** 63
 SURF_TOL = 1.0e-12
 FP_REL_TOL = 1.0e-7
-TRANSPORT_ST, TRANSPORT_DT = 1, 2
+TRANSPORT_ST, TRANSPORT_DT, TRANSPORT_HT = 1, 2, 3
 UNI_ROOT, UNI_LAT, UNI_PIN = 1, 2, 3
 MAP_SPACE, MAP_MATERIAL = 1, 2
 GRID_LIN, GRID_UNSTRUCT = 1, 3
@@ -96,7 +96,7 @@ class Problem(C.Structure):
                 ("n_real", C.c_int32), ("real_data", C.POINTER(C.c_double)), ("n_int", C.c_int32),
                 ("int_data", C.POINTER(C.c_int32)), ("mg", MGData), ("n_clerks", C.c_int32),
                 ("clerks", C.POINTER(Clerk)), ("n_bins", C.c_int64), ("norm_bin", C.c_int64),
-                ("norm_value", C.c_double)]
+                ("norm_value", C.c_double), ("ht_cutoff", C.c_double)]
 
 
 class EigenSettings(C.Structure):
@@ -283,6 +283,10 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         P.transport = TRANSPORT_ST
     elif ttype == "transportOperatorDT":
         P.transport = TRANSPORT_DT
+    elif ttype == "transportOperatorHT":
+        # transportOperatorHT_class.f90 init: cutoff default 0.9
+        P.transport = TRANSPORT_HT
+        P.ht_cutoff = float(top.get("cutoff", 0.9))
     else:
         raise NotImplementedError(f"{ttype} is outside the engine's scope")
     P.st_cache = int(top.get("cache", 1))

@@ -1,0 +1,63 @@
+"""The reference's C5G7 benchmark input on the MI355X (needs a GPU).
+
+* One cycle of hybrid tracking on the unmodified input: next bank, keffImplicit scores bit-exact
+  and the 34x34 fission map to 1e-12 against the oracle.
+* The whole power iteration: k_eff against the benchmark's reference eigenvalue for the 2D core,
+  1.18655 +/- 0.00008 (OECD/NEA C5G7 MOX benchmark, NEA/NSC/DOC(2003)16, MCNP reference
+  solution).  With 2e5 histories per cycle, 40 inactive and 120 active cycles the engine's
+  standard deviation is ~1e-4; the check is |k - 1.18655| < 3 sqrt(sd^2 + 0.00008^2) + 0.0002, the
+  last term covering cycle-to-cycle correlation that the batch sd does not see.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import binding as O
+from helpers import assert_banks_identical, rel_close
+from scone_amd import dictio, tables as T
+from scone_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C5G7 = os.path.join(HERE, "golden", "c5g7")
+K_REF, K_REF_SD = 1.18655, 0.00008
+
+
+def c5g7():
+    inp = dictio.read(os.path.join(C5G7, "C5G7"))
+    return inp, T.compile_problem(inp, base_dir=C5G7)
+
+
+def test_c5g7_hybrid_cycle_bit_exact():
+    print("case c5g7 cycle", flush=True)
+    inp, comp = c5g7()
+    pop = 20000
+    bank = O.fission_source(comp, pop, 3)
+    eng = Engine(comp, pop)
+    g_bank = eng.fission_source(pop, 3)
+    assert_banks_identical(bank, g_bank)
+    o_out, o_bins, o_ks, _ = O.eigen_cycle(comp, bank, 4242, 1.18, threads=8)
+    g_out, g_bins, g_ks = eng.eigen_cycle(bank, 4242, 1.18)
+    assert_banks_identical(o_out, g_out)
+    assert np.array_equal(o_ks.view(np.uint64), g_ks.view(np.uint64))
+    ok, worst = rel_close(o_bins, g_bins, 1e-12)
+    assert ok, worst
+    eng.close()
+
+
+def test_c5g7_keff_matches_benchmark_reference():
+    print("case c5g7 keff", flush=True)
+    inp, comp = c5g7()
+    pop, ni, na = 200000, 40, 120
+    eng = Engine(comp, pop)
+    bank = eng.fission_source(pop, 11)
+    R, kh, bins_a, _ = eng.eigen_run(dict(pop=pop, inactive=ni, active=na, seed=11), bank)
+    eng.close()
+    k, sd = R.k_implicit
+    print(f"k_implicit {k:.6f} +/- {sd:.6f}, k_analog {R.k_analog[0]:.6f}", flush=True)
+    tol = 3.0 * math.sqrt(sd * sd + K_REF_SD * K_REF_SD) + 0.0002
+    assert abs(k - K_REF) < tol, (k, sd)
+    assert bins_a[:, 0].max() > 0
