@@ -131,6 +131,7 @@ typedef struct scn_mg_data {
     const int32_t* fissile;
     const double* majorant;  /* [nG] baseMgNeutronDatabase_class.f90:458-503 */
     double collision_xs;     /* 1/avgDist, 0 when absent */
+    const double* P1;        /* PN P1: [mat][G_in][G_out] 3*P1/P0 (multiScatterP1MG buildFromDict), NULL for P0 */
 } scn_mg_data;
 
 /* ---- tallies: collisionClerk (Tallies/TallyClerks/collisionClerk_class.f90) ----------------- */

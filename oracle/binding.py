@@ -93,6 +93,7 @@ def lib():
                                     C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
         L.orc_score_result.argtypes = [C.c_double, C.c_double, C.c_int, dp]
         L.orc_close_cycle.argtypes = [dp, dp, dp, C.c_int64, C.c_double]
+        L.orc_legendre_p1.argtypes = [C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         L.orc_keff_implicit.restype = C.c_double
         L.orc_keff_implicit.argtypes = [dp]
         _lib = L
@@ -301,6 +302,14 @@ def score_result(csum, csum2, N):
 def close_cycle(bins, csum, csum2, norm):
     dp = C.POINTER(C.c_double)
     lib().orc_close_cycle(bins.ctypes.data_as(dp), csum.ctypes.data_as(dp), csum2.ctypes.data_as(dp), len(bins), norm)
+
+
+def legendre_p1(c, rng_state):
+    """sampleLegendre_P1 with the coefficient c = 3 P1/P0; returns (mu, rng state after)."""
+    s = C.c_uint64(rng_state)
+    mu = C.c_double()
+    check(lib().orc_legendre_p1(c, C.byref(s), C.byref(mu)))
+    return mu.value, s.value
 
 
 def keff_implicit(scores):

@@ -811,6 +811,35 @@ static void o_site_push(o_site_buf* b, const scn_particle_state* s)
     b->v[b->n++] = *s;
 }
 
+/* legendrePoly_func.f90:35-99 sampleLegendre_P1: mu from 1/2 (1 + c mu) on [-1, 1] as a mixture
+ * of uniform / linear (|c| < 1) or linear / delta at 1 (1 <= |c| <= 3), mirrored for c < 0 */
+static int o_legendre_p1(double c, uint64_t* rng, double* mu)
+{
+    enum { UNIFORM = 1, LIN = 2, DELTA = 3 };
+    double a = fabs(c), threshold;
+    int top, low;
+    if (a < 1.0) {
+        threshold = a;
+        top = LIN;
+        low = UNIFORM;
+    } else if (a <= 3.0) {
+        threshold = 0.5 * (a - 1.0);
+        top = DELTA;
+        low = LIN;
+    } else {
+        o_fail(SCN_ERR_SAMPLING, "P1 must have absolute value < 3.0");
+        return g_err;
+    }
+    int ex = (scn_rng_get(rng) < threshold) ? top : low;
+    double x;
+    if (ex == UNIFORM) x = 2.0 * scn_rng_get(rng) - 1.0;
+    else if (ex == LIN) x = 2.0 * sqrt(scn_rng_get(rng)) - 1.0;
+    else x = 1.0;
+    if (c < 0.0) x = -x;
+    *mu = x;
+    return 0;
+}
+
 /* collisionProcessor_inter.f90:114-194 collide with neutronMGstd_class.f90:85-282 */
 static void o_collide(o_hctx* X, o_part* p)
 {
@@ -890,7 +919,14 @@ static void o_collide(o_hctx* X, o_part* p)
             if (rem < 0.0) break;
         }
         if (G_out > nG) { o_fail(SCN_ERR_SAMPLING, "multiScatterMG sampleGout: sampling failed"); p->isDead = 1; return; }
-        double mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
+        double mu;
+        if (P->mg.P1) {
+            /* multiScatterP1MG_class.f90:69-85 sampleOut -> legendrePoly_func.f90:35-99 */
+            double c = P->mg.P1[((size_t)(mat - 1) * nG + (p->G - 1)) * nG + (G_out - 1)];
+            if (o_legendre_p1(c, &p->rng, &mu)) { p->isDead = 1; return; }
+        } else {
+            mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
+        }
         double phi = SCN_TWO_PI * scn_rng_get(&p->rng);
         double w_mul = prod[G_out - 1];
         p->G = G_out;
@@ -1478,6 +1514,12 @@ void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, doubl
 /* keffImplicitClerk_class.f90:306-326 closeCycle estimate; scores in IMP_PROD, SCATTER_PROD,
  * IMP_ABS, ANA_LEAK order */
 double orc_keff_implicit(const double s[4]) { return s[0] / (s[2] + s[3] - s[1]); }
+
+int orc_legendre_p1(double c, uint64_t* rng, double* mu)
+{
+    o_clear();
+    return o_legendre_p1(c, rng, mu);
+}
 
 int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                   const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads)

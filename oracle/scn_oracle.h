@@ -92,6 +92,7 @@ int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int3
 void orc_score_result(double csum, double csum2, int N, double out[2]);
 void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm);
 double orc_keff_implicit(const double s[4]);
+int orc_legendre_p1(double c, uint64_t* rng, double* mu);
 int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                   const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads);
 

@@ -1187,6 +1187,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
         for (int g = 0; g < sr->n_probG; ++g) probG[g] = sr->probG[g] / t;
     }
     size_t o_probG = I.put(probG.data(), probG.size());
+    size_t o_P1 = pr->mg.P1 ? I.put(pr->mg.P1, (size_t)nM * nG * nG) : 0;
     SlotLayout Ly;
     Ly.tables = align256(I.bytes.size());
     Ly.queue = Ly.tables;
@@ -1232,6 +1233,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.gclerk_idx = (const int*)(base + o_gidx);
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
+    P.P1 = pr->mg.P1 ? (const double*)(base + o_P1) : nullptr;
     carve_live(base + Ly.queue, depth, R->pop, D.q[0]);
     carve_live(base + Ly.queue + Ly.queue_bytes, depth, R->pop, D.q[1]);
     D.bin = (double*)(base + Ly.bins);

@@ -44,6 +44,7 @@ struct DevProblem {
     const int* gclerk_idx;
     int64_t n_bins;
     double ht_cutoff;
+    const double* P1;    // [mat][G_in][G_out] 3 P1/P0, or null (P0 scattering)
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -852,6 +853,35 @@ __device__ __forceinline__ int cdf_scan(const double* __restrict__ x, int n, dou
     return n + 1;
 }
 
+// legendrePoly_func.f90:35-99 sampleLegendre_P1: mu with density 1/2 (1 + c mu), c = 3 P1/P0, as a
+// mixture of uniform / linear (|c| < 1) or linear / delta at 1 (1 <= |c| <= 3), mirrored for c < 0.
+// Returns 0 if |c| > 3.
+__device__ __forceinline__ int legendre_p1(double c, uint64_t* rng, double* mu)
+{
+    const double a = fabs(c);
+    double threshold;
+    int top, low;   // 1 uniform, 2 linear, 3 delta
+    if (a < 1.0) {
+        threshold = a;
+        top = 2;
+        low = 1;
+    } else if (a <= 3.0) {
+        threshold = 0.5 * (a - 1.0);
+        top = 3;
+        low = 2;
+    } else {
+        return 0;
+    }
+    const int ex = (scn_rng_get(rng) < threshold) ? top : low;
+    double x;
+    if (ex == 1) x = 2.0 * scn_rng_get(rng) - 1.0;
+    else if (ex == 2) x = 2.0 * sqrt(scn_rng_get(rng)) - 1.0;
+    else x = 1.0;
+    if (c < 0.0) x = -x;
+    *mu = x;
+    return 1;
+}
+
 // collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
 template <int NL>
 __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, const Bins& B, const Sites& S,
@@ -959,7 +989,18 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_
             p.dead = 1;
             return 0;
         }
-        double mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
+        double mu;
+        if (P.P1) {
+            // multiScatterP1MG_class.f90:69-85 sampleOut -> legendrePoly_func.f90:35-99
+            const double c = P.P1[((size_t)(mat - 1) * nG + (p.G - 1)) * nG + (G_out - 1)];
+            if (!legendre_p1(c, &p.rng, &mu)) {
+                raise_error(st, SCN_ERR_SAMPLING, 3);
+                p.dead = 1;
+                return 0;
+            }
+        } else {
+            mu = 2.0 * scn_rng_get(&p.rng) - 1.0;
+        }
         double phi = SCN_TWO_PI * scn_rng_get(&p.rng);
         double w_mul = prod[G_out - 1];
         p.G = G_out;

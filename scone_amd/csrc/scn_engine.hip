@@ -1068,6 +1068,11 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.gclerk_idx = d_gidx;
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
+    if (pr->mg.P1) {
+        double* d_P1 = nullptr;
+        if (!rc) rc |= upload(E, &d_P1, pr->mg.P1, (size_t)nM * nG * nG);
+        P.P1 = d_P1;
+    }
     // buffers
     const int cap = capacity;
     E->cap = cap;

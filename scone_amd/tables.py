@@ -74,7 +74,8 @@ class MGData(C.Structure):
     _fields_ = [("n_mat", C.c_int32), ("n_groups", C.c_int32), ("xs", C.POINTER(C.c_double)),
                 ("P0", C.POINTER(C.c_double)), ("prod", C.POINTER(C.c_double)),
                 ("chi", C.POINTER(C.c_double)), ("fissile", C.POINTER(C.c_int32)),
-                ("majorant", C.POINTER(C.c_double)), ("collision_xs", C.c_double)]
+                ("majorant", C.POINTER(C.c_double)), ("collision_xs", C.c_double),
+                ("P1", C.POINTER(C.c_double))]
 
 
 class Map1D(C.Structure):
@@ -149,7 +150,7 @@ def _ptr(arr, ctype):
 class MGMaterial:
     """One material's multigroup data exactly as baseMgNeutronMaterial%init builds it."""
 
-    def __init__(self, d: dict, name: str = ""):
+    def __init__(self, d: dict, name: str = "", pn: str = "P0"):
         self.name = name
         nG = int(d["numberOfGroups"])
         if nG < 1:
@@ -163,6 +164,15 @@ class MGMaterial:
         # reshape(temp, [nG, nG]): flat index (gin-1)*nG + (gout-1) is P0(gout, gin)
         self.P0 = P0
         self.prod = prod
+        self.P1 = None
+        if pn == "P1":
+            # multiScatterP1MG_class.f90 buildFromDict: P1 = P1 / P0 * 3 where P0 /= 0, else 0
+            P1 = [float(x) for x in d["P1"]]
+            if len(P1) != nG * nG:
+                raise ValueError(f"Invalid size of P1. Expected: {nG * nG} got: {len(P1)}")
+            self.P1 = [(P1[i] / P0[i] * 3.0) if P0[i] != 0.0 else 0.0 for i in range(nG * nG)]
+        elif pn != "P0":
+            raise ValueError(f"scatterKey: {pn} is wrong. Must be P0 or P1")
         scatter = []
         for gin in range(nG):  # sum(P0, 1): sequential over G_out from 0.0
             s = 0.0
@@ -297,6 +307,8 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     names = list(mats_d.keys())
     out.mat_names = names
     mat_idx = {n: i + 1 for i, n in enumerate(names)}
+    handles = nd.get("handles", {}).get("mg", {})
+    pn = handles.get("PN", "P0")
     mats = []
     for n in names:
         md = mats_d[n]
@@ -307,15 +319,12 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
             if not os.path.isabs(path):
                 path = os.path.join(base_dir, path)
             dd = dictio.read(path)
-        mats.append(MGMaterial(dd, n))
+        mats.append(MGMaterial(dd, n, pn))
     nG = mats[0].nG
     for m in mats:
         if m.nG != nG:
             raise ValueError("Inconsistent # of groups in materials")
     out.nG = nG
-    handles = nd.get("handles", {}).get("mg", {})
-    if handles.get("PN", "P0") != "P0":
-        raise NotImplementedError("P1 scattering is outside the engine's scope")
     collision_xs = 0.0
     if "avgDist" in handles:
         collision_xs = 1.0 / float(handles["avgDist"])
@@ -511,9 +520,12 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     prod = np.zeros((len(mats), nG * nG))
     chi = np.zeros((len(mats), nG))
     fiss = np.zeros(len(mats), dtype=np.int32)
+    P1 = np.zeros((len(mats), nG * nG)) if pn == "P1" else None
     for i, m in enumerate(mats):
         xs[i] = m.data
         P0[i] = m.P0
+        if P1 is not None:
+            P1[i] = m.P1
         prod[i] = m.prod
         chi[i] = m.chi
         fiss[i] = 1 if m.fissile else 0
@@ -666,6 +678,8 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     P.mg.xs = _ptr(xs_c, C.c_double)
     P.mg.P0 = _ptr(P0_c, C.c_double)
     P.mg.prod = _ptr(prod_c, C.c_double)
+    if P1 is not None:
+        P.mg.P1 = _ptr(out.hold(np.ascontiguousarray(P1.reshape(-1))), C.c_double)
     P.mg.chi = _ptr(chi_c, C.c_double)
     P.mg.fissile = _ptr(fiss_c, C.c_int32)
     P.mg.majorant = _ptr(maj_c, C.c_double)

@@ -1,4 +1,4 @@
-"""The reference's C5G7 benchmark input on the MI355X (needs a GPU).
+"""The reference's own benchmark inputs on the MI355X (needs a GPU): C5G7 and SCONE_Slab.
 
 * One cycle of hybrid tracking on the unmodified input: next bank, keffImplicit scores bit-exact
   and the 34x34 fission map to 1e-12 against the oracle.
@@ -7,6 +7,8 @@
   solution).  With 2e5 histories per cycle, 40 inactive and 120 active cycles the engine's
   standard deviation is ~1e-4; the check is |k - 1.18655| < 3 sqrt(sd^2 + 0.00008^2) + 0.0002, the
   last term covering cycle-to-cycle correlation that the batch sd does not see.
+* SCONE_Slab (URR-2-1-SL, P1 scattering, analytically critical): one cycle bit-exact against the
+  oracle, and k within 3 sd of 1 with 1e5 histories per cycle.
 """
 import math
 import os
@@ -23,6 +25,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 C5G7 = os.path.join(HERE, "golden", "c5g7")
+SLAB = os.path.join(HERE, "golden", "scone_slab")
 K_REF, K_REF_SD = 1.18655, 0.00008
 
 
@@ -61,3 +64,36 @@ def test_c5g7_keff_matches_benchmark_reference():
     tol = 3.0 * math.sqrt(sd * sd + K_REF_SD * K_REF_SD) + 0.0002
     assert abs(k - K_REF) < tol, (k, sd)
     assert bins_a[:, 0].max() > 0
+
+
+def slab():
+    inp = dictio.read(os.path.join(SLAB, "SCONE_Slab"))
+    return inp, T.compile_problem(inp, base_dir=SLAB)
+
+
+def test_scone_slab_p1_cycle_bit_exact():
+    print("case slab cycle", flush=True)
+    inp, comp = slab()
+    pop = 20000
+    bank = O.fission_source(comp, pop, 5)
+    eng = Engine(comp, pop)
+    o_out, o_bins, o_ks, _ = O.eigen_cycle(comp, bank, 777, 1.0, threads=8)
+    g_out, g_bins, g_ks = eng.eigen_cycle(bank, 777, 1.0)
+    assert_banks_identical(o_out, g_out)
+    assert np.array_equal(o_ks.view(np.uint64), g_ks.view(np.uint64))
+    ok, worst = rel_close(o_bins, g_bins, 1e-12)
+    assert ok, worst
+    eng.close()
+
+
+def test_scone_slab_is_critical_on_the_gpu():
+    print("case slab keff", flush=True)
+    inp, comp = slab()
+    pop, ni, na = 100000, 30, 150
+    eng = Engine(comp, pop)
+    bank = eng.fission_source(pop, 17)
+    R, kh, _, _ = eng.eigen_run(dict(pop=pop, inactive=ni, active=na, seed=17), bank)
+    eng.close()
+    k, sd = R.k_implicit
+    print(f"slab k_implicit {k:.6f} +/- {sd:.6f}", flush=True)
+    assert abs(k - 1.0) < 3 * sd + 1e-4, (k, sd)
