@@ -641,6 +641,7 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         out.clerk_names.append(cname)
         out.clerk_shapes.append([K.width] + shape)
         out.clerk_info.append({"name": cname, "addr": int(K.mem_addr), "size": int(size), "axes": axes,
+                               "multi": "map" in cd and cd["map"]["type"] == "multiMap",
                                "width": K.width, "shape": [K.width] + shape})
 
     # --- assemble
