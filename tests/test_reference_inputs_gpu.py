@@ -97,3 +97,22 @@ def test_scone_slab_is_critical_on_the_gpu():
     k, sd = R.k_implicit
     print(f"slab k_implicit {k:.6f} +/- {sd:.6f}", flush=True)
     assert abs(k - 1.0) < 3 * sd + 1e-4, (k, sd)
+
+
+def test_package_runs_an_input_file_and_writes_scone_output(tmp_path):
+    """scone_amd.package.run_input on the SCONE_Slab input (fewer histories): the result file has
+    SCONE's entries; the fission clerk is normalised to normVal 100 every cycle."""
+    print("case package run", flush=True)
+    from scone_amd import package
+    inp = dictio.read(os.path.join(SLAB, "SCONE_Slab"))
+    inp["pop"], inp["active"], inp["inactive"], inp["seed"] = 20000, 40, 20, 21
+    inp["nuclearData"]["materials"]["fuel"]["xsFile"] = os.path.join(SLAB, "XS", "URRa_2_1_XSS")
+    path = tmp_path / "slab_input"
+    path.write_text(dictio.dumps(inp))
+    out = package.run_input(str(path), str(tmp_path / "slab_out"))
+    text = open(out).read()
+    lines = dict(l.split(" = ", 1) for l in text.strip().split("\n"))
+    assert lines["pop"] == "20000;" and lines["Active_Cycles"] == "40;"
+    k, sd = [float(x) for x in lines["keff_K_EFF"].strip("[ ];").split(",")]
+    assert abs(k - 1.0) < 4 * sd + 1e-3, (k, sd)
+    assert lines["active_fiss_Res"].startswith("[ 1.00000E+02,")
