@@ -11,6 +11,9 @@ generator with a fixed seed, so a problem is reproducible from its name alone.
 * ``pin_lattice_7g``  -- 17x17 square lattice of cylindrical pins (pitch 1.26, r 0.54) with the
                          C5G7 UO2 assembly pin map, reflective box, 7 groups, fuel / structural /
                          moderator materials, per-pin x per-group flux and fission-rate clerks.
+* ``core_3d``         -- pin -> 17x17 assembly -> 15x15 core lattice with a 20 cm reflector ring,
+                         400 cm tall vacuum box, 70 groups, 200x200x50 mesh x 70 group flux tally
+                         (1.4e8 bins).
 
 Fixed-source problems (fixedSourcePhysicsPackage dictionaries, ``source`` and ``tally`` keys):
 
@@ -406,4 +409,60 @@ def lattice_request(i: int, transport: str = "transportOperatorDT", scale: float
     return out, mg, pop, cycles
 
 
-CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g}
+def core_3d(seed: int = 5, nG: int = 70, pop: int = 10_000_000, inactive: int = 30, active: int = 100,
+            transport: str = "transportOperatorDT", n_assemblies: int = 15, mesh=(200, 200, 50),
+            height: float = 400.0, reflector: float = 20.0, tallies: bool = True):
+    """3D nested-universe core: pin (r 0.54) -> 17x17 assembly lattice (pitch 1.26, C5G7 UO2 pin
+    map) -> n x n core lattice (pitch 21.42) whose padding is a reflector ring `reflector` cm thick,
+    in a vacuum box `height` cm tall (2D lattices, finite z).  nG groups of synthetic P0 data
+    from `seed`; per-group mesh flux clerks multiMap(x, y, z) over the box."""
+    mats = pin_lattice_materials(seed, nG)
+    rng = np.random.default_rng(seed + 1000)
+    st_r = np.sort(rng.uniform(0.3, 2.0, nG))
+    c_r = rng.uniform(0.95, 0.995, nG)
+    mats["reflector"] = _mg_dict(st_r, _lower_tri_scatter(rng, nG, st_r, c_r, 0.4, 0.8), (1.0 - c_r) * st_r)
+    lat_map = []
+    for row in UO2_MAP:
+        for tok in row.split():
+            lat_map.append({1: 1, 2: 2, 6: 2}[int(tok)])
+    half = n_assemblies * 21.42 / 2.0 + reflector
+    hz = height / 2.0
+    active_t = {}
+    if tallies:
+        mx, my, mz = mesh
+        for g in range(1, nG + 1):
+            active_t[f"mesh_g{g}"] = {
+                "type": "collisionClerk",
+                "filter": {"type": "energyFilter", "Gtop": g, "Glow": g},
+                "map": {"type": "multiMap", "maps": ["xax", "yax", "zax"],
+                        "xax": {"type": "spaceMap", "axis": "x", "grid": "lin", "min": -half, "max": half, "N": mx},
+                        "yax": {"type": "spaceMap", "axis": "y", "grid": "lin", "min": -half, "max": half, "N": my},
+                        "zax": {"type": "spaceMap", "axis": "z", "grid": "lin", "min": -hz, "max": hz, "N": mz}},
+                "response": ["flux"], "flux": {"type": "fluxResponse"}}
+    inp = {
+        "type": "eigenPhysicsPackage", "pop": pop, "active": active, "inactive": inactive,
+        "XSdata": "mg", "dataType": "mg", "seed": seed,
+        "collisionOperator": {"neutronMG": {"type": "neutronMGstd"}},
+        "transportOperator": {"type": transport},
+        "inactiveTally": {}, "activeTally": active_t,
+        "geometry": {
+            "type": "geometryStd", "boundary": [0, 0, 0, 0, 0, 0], "graph": {"type": "shrunk"},
+            "surfaces": {"box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [half, half, hz]}},
+            "cells": {},
+            "universes": {
+                "root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<100>"},
+                "pin1": {"id": 1, "type": "pinUniverse", "radii": [0.54, 0.0], "fills": ["fuel", "moderator"]},
+                "pin2": {"id": 2, "type": "pinUniverse", "radii": [0.54, 0.0], "fills": ["structural", "moderator"]},
+                "asm": {"id": 10, "type": "latUniverse", "origin": [0.0, 0.0, 0.0], "pitch": [1.26, 1.26, 0.0],
+                        "shape": [17, 17, 0], "padMat": "moderator", "map": lat_map},
+                "core": {"id": 100, "type": "latUniverse", "origin": [0.0, 0.0, 0.0], "pitch": [21.42, 21.42, 0.0],
+                         "shape": [n_assemblies, n_assemblies, 0], "padMat": "reflector",
+                         "map": [10] * (n_assemblies * n_assemblies)},
+            }},
+        "nuclearData": {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
+                        "materials": {n: {"temp": 300, "composition": {}, "xsFile": n} for n in mats}},
+    }
+    return inp, mats
+
+
+CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g, "core_3d": core_3d}

@@ -84,3 +84,19 @@ def test_c5g7_hybrid_cycle_runs_on_the_oracle():
     k_est = O.keff_implicit(ks)
     assert 0.9 < k_est < 1.5
     assert len(out) > 0.7 * pop and bins.sum() > 0
+
+
+def test_core_3d_config_nests_four_levels():
+    """The 3D core configuration: pin -> assembly -> core lattice -> root, reflector ring as the
+    core lattice's padding, vacuum box 400 cm tall."""
+    from scone_amd import configs
+    inp, mg = configs.core_3d(pop=100, mesh=(10, 10, 5))
+    comp = T.compile_problem(inp, mg)
+    assert comp.nG == 70 and comp.problem.n_bins == 70 * 10 * 10 * 5
+    assert comp.bounds == pytest.approx([-180.65, -180.65, -200.0, 180.65, 180.65, 200.0])
+    names = comp.mat_names
+    assert O.place(comp, (0.0, 0.0, 0.0)).matIdx == names.index("structural") + 1   # core centre pin: guide tube
+    assert O.place(comp, (0.6, 0.0, 0.0)).matIdx == names.index("moderator") + 1
+    assert O.place(comp, (0.0 - 1.26, 0.0, 150.0)).matIdx == names.index("fuel") + 1
+    out = O.place(comp, (170.0, 0.0, 0.0))
+    assert out.matIdx == names.index("reflector") + 1 and out.nesting == 2

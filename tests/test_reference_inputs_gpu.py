@@ -116,3 +116,20 @@ def test_package_runs_an_input_file_and_writes_scone_output(tmp_path):
     k, sd = [float(x) for x in lines["keff_K_EFF"].strip("[ ];").split(",")]
     assert abs(k - 1.0) < 4 * sd + 1e-3, (k, sd)
     assert lines["active_fiss_Res"].startswith("[ 1.00000E+02,")
+
+
+def test_core_3d_cycle_bit_exact():
+    """70-group 3D core (four nesting levels, vacuum, 2e5-bin mesh): one cycle bit-exact."""
+    print("case core3d cycle", flush=True)
+    from scone_amd import configs
+    inp, mg = configs.core_3d(pop=20000, mesh=(40, 40, 5))
+    comp = T.compile_problem(inp, mg)
+    bank = O.fission_source(comp, 20000, 9)
+    eng = Engine(comp, 20000)
+    o_out, o_bins, o_ks, _ = O.eigen_cycle(comp, bank, 31337, 0.81, threads=8)
+    g_out, g_bins, g_ks = eng.eigen_cycle(bank, 31337, 0.81)
+    assert_banks_identical(o_out, g_out)
+    assert np.array_equal(o_ks.view(np.uint64), g_ks.view(np.uint64))
+    ok, worst = rel_close(o_bins, g_bins, 1e-12)
+    assert ok, worst
+    eng.close()
