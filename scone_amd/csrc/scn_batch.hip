@@ -342,7 +342,7 @@ __device__ __forceinline__ void flush_counts(SlotState* SS, Cnt& cn, int lane)
 
 // add the wave's LDS copy of the low bins to BIN and clear it (all lanes of the wave call this
 // at the same point; LDS operations of one wave complete in order)
-__device__ __forceinline__ void flush_lds_bins(const Bins& B, int lane)
+__device__ __forceinline__ void flush_lds_bins(Bins& B, int lane)
 {
     __builtin_amdgcn_wave_barrier();
     for (int64_t i = lane; i < B.lds_n; i += BWAVE) {
@@ -387,6 +387,8 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     B.lds = wl;
     B.glob = nullptr;
     B.lds_n = 0;
+    B.c_addr = -1;
+    B.c_acc = 0.0;
     Cnt cn{0, 0, 0};
     Pt<NL> p;
     DistCache<NL> cache;   // surface tracking: the current flight's distance cache
@@ -418,6 +420,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
         if (pend_s >= 0 && live == 0) {
             if (s >= 0) {
                 flush_counts(SS, cn, lane);
+                bins_flush(B);
                 flush_lds_bins(B, lane);
             }
             s = pend_s;
@@ -542,6 +545,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     }
     if (s >= 0) {
         flush_counts(SS, cn, lane);
+        bins_flush(B);
         flush_lds_bins(B, lane);
     }
 }

@@ -616,23 +616,41 @@ __device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, co
     return (mat >= 0 && mat <= nm) ? P.ints[m.data + mat] : P.ints[m.data + nm + 1];
 }
 
-// score memory: bins [0, lds_n) go to a private LDS copy (flushed to BIN by the kernel), the
-// rest straight to the global BIN array
+// score memory: bins [0, lds_n) go to a private LDS copy (flushed to BIN by the kernel); the
+// rest go to the global BIN array through a one-entry per-lane cache: consecutive scores of a
+// lane to the same bin are summed in a register and added with one atomic when the lane scores
+// another bin (bins_flush at the end).  Only the summation order of BIN changes.
 struct Bins {
     double* lds;
     double* glob;
     int64_t lds_n;
+    int64_t c_addr;    // bin held in the lane's cache, -1 none
+    double c_acc;
 };
 
-__device__ __forceinline__ void bin_add(const Bins& B, int64_t addr, double v)
+__device__ __forceinline__ void bins_flush(Bins& B)
 {
-    if (addr < B.lds_n) atomicAdd(&B.lds[addr], v);
-    else unsafeAtomicAdd(&B.glob[addr], v);
+    if (B.c_addr >= 0) unsafeAtomicAdd(&B.glob[B.c_addr], B.c_acc);
+    B.c_addr = -1;
+    B.c_acc = 0.0;
+}
+
+__device__ __forceinline__ void bin_add(Bins& B, int64_t addr, double v)
+{
+    if (addr < B.lds_n) {
+        atomicAdd(&B.lds[addr], v);
+    } else if (addr == B.c_addr) {
+        B.c_acc = B.c_acc + v;
+    } else {
+        if (B.c_addr >= 0) unsafeAtomicAdd(&B.glob[B.c_addr], B.c_acc);
+        B.c_addr = addr;
+        B.c_acc = v;
+    }
 }
 
 // tallyAdmin%reportInColl: keffImplicitClerk first, then the collision clerks
 template <int NL>
-__device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, const Bins& B, DevStatus* st)
+__device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, Bins& B, DevStatus* st)
 {
     int nsc = 0;
     int mat = p.mat;
@@ -683,7 +701,7 @@ enum { STEP_DEAD = 0, STEP_COLLIDE = 1, STEP_MORE = 2 };
 // sampled collision point and the closest surface; `cache` is the flight's distCache (fresh at the
 // start of every flight)
 template <int NL>
-__device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache<NL>& cache, const Bins& B,
+__device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache<NL>& cache, Bins& B,
                                        DevStatus* st, Cnt& cnt)
 {
     double sigmaTrack, sigmaT, dist, invSigmaTrack;
@@ -731,7 +749,7 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache
 // and its inverse depend on the group only, so evaluating them per tentative flight gives the
 // values the flight computes once.
 template <int NL>
-__device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
+__device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
     p.track = maj;
@@ -778,7 +796,7 @@ __device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL>& p, D
 
 // surfaceTracking: segments until a real collision or a leak; returns 1 if alive
 template <int NL>
-__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
+__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     DistCache<NL> cache;
     cache.lvl = 0;
@@ -795,7 +813,7 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, const B
 
 // deltaTracking: tentative flights until a real collision or a leak; returns 1 if alive
 template <int NL>
-__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, const Bins& B, DevStatus* st, Cnt& cnt)
+__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     for (int seg = 0;; ++seg) {
         if (seg >= MAX_SEGMENTS_PER_FLIGHT) {
@@ -884,7 +902,7 @@ __device__ __forceinline__ int legendre_p1(double c, uint64_t* rng, double* mu)
 
 // collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
 template <int NL>
-__device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, const Bins& B, const Sites& S,
+__device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, Bins& B, const Sites& S,
                               DevStatus* st, Cnt& cnt)
 {
     const int nG = P.nG;

@@ -159,6 +159,8 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     B.lds = use_lds ? lds_bins : nullptr;
     B.glob = bins;
     B.lds_n = use_lds ? P.n_bins : 0;
+    B.c_addr = -1;
+    B.c_acc = 0.0;
     Cnt cn{0, 0, 0};
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -277,6 +279,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             if (__shfl(e, 0)) break;
         }
     }
+    bins_flush(B);
     const unsigned long long seg = wave_sum64((unsigned long long)cn.seg);
     const unsigned long long col = wave_sum64((unsigned long long)cn.col);
     const unsigned long long sc = wave_sum64((unsigned long long)cn.sc);

@@ -124,7 +124,8 @@ def test_package_runs_an_input_file_and_writes_scone_output(tmp_path):
     assert lines["pop"] == "20000;" and lines["Active_Cycles"] == "40;"
     k, sd = [float(x) for x in lines["keff_K_EFF"].strip("[ ];").split(",")]
     assert abs(k - 1.0) < 4 * sd + 1e-3, (k, sd)
-    assert lines["active_fiss_Res"].startswith("[ 1.00000E+02,")
+    # a clerk without a map prints Res of shape [2, 1]: the one response, normalised to normVal
+    assert lines["active_fiss_Res"] == "reshape([ 1.00000E+02,0.00000E+00],2,1);"
 
 
 def test_core_3d_cycle_bit_exact():
