@@ -50,7 +50,9 @@ enum {
 #define SCN_BC_PERIODIC 2
 
 enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2, SCN_TRANSPORT_HT = 3 };
-enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3 };
+enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3, SCN_UNI_CELL = 4 };
+enum { SCN_SURF_APLANE = 1, SCN_SURF_PLANE = 2, SCN_SURF_SPHERE = 3, SCN_SURF_CYLINDER = 4,
+       SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7 };
 enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2 };
 enum { SCN_GRID_LIN = 1, SCN_GRID_UNSTRUCT = 3 };
 /* macroscopic MT codes (SharedModules/endfConstants.f90); 0 selects fluxResponse */
@@ -92,13 +94,43 @@ typedef struct scn_box {
     double halfwidth[3];
 } scn_box;
 
-/* Universes (Geometry/Universes/{root,lat,pin}Universe_class.f90).  uniIdx is the 1-based position in the table.
- *  ROOT: rootUniverse_class.f90 -- local cell 1 = inside border, 2 = outside.
+/* Surfaces of cells and of a non-box root border (Geometry/Surfaces).  tol is surfTol: SURF_TOL,
+ * or what the type's init sets (sphere, cylinder 2*r*SURF_TOL; quadric 2*max|coeff|*SURF_TOL).
+ *  APLANE         x/y/zPlane (aPlane_class.f90): axis, c[0] = a0
+ *  PLANE          plane (plane_class.f90): c[0..2] normal, c[3] offset, both divided by
+ *                 norm2(normal) as init does
+ *  SPHERE         sphere (sphere_class.f90): c[0..2] origin, c[3] radius, c[4] radius**2
+ *  CYLINDER       x/y/zCylinder (cylinder_class.f90): axis, c[0..2] origin, c[3] radius, c[4] radius**2
+ *  QUADRIC        quadric (quadric_class.f90): c[0..9] the ten coefficients
+ *  BOX            box or x/y/zSquareCylinder: axis = index into boxes
+ *  TRUNC_CYLINDER x/y/zTruncCylinder (truncCylinder_class.f90): axis, c[0..2] origin, c[3] radius,
+ *                 c[4] halfwidth                                                               */
+typedef struct scn_surface {
+    int32_t type;
+    int32_t axis;
+    double tol;
+    double c[10];
+} scn_surface;
+
+/* Universes (Geometry/Universes/{root,lat,pin,cell}Universe_class.f90).  uniIdx is the 1-based position in the table.
+ *  ROOT: rootUniverse_class.f90 -- local cell 1 = inside border, 2 = outside.  A box or
+ *        squareCylinder border is boxes[border] (border_surf = -1); any other border surface is
+ *        surfaces[border_surf] (vacuum boundary only, as surface_inter.f90 setBC requires).
  *  LAT : latUniverse_class.f90  -- size/pitch after the 2D fix-up (size[2] = 1, pitch[2] = 2*INF),
  *        corner, a_bar and out_local as init computes them; offset_mode 1 = offset every cell,
  *        0 = none, 2 = per-cell flags at int_data[offset_map + localID - 1].
  *  PIN : pinUniverse_class.f90  -- annuli sorted ascending, the last one of radius INF*1.1;
- *        real_data[annuli + i] = radius_i, real_data[annuli + n_annuli + i] = radius_i**2.    */
+ *        real_data[annuli + i] = radius_i, real_data[annuli + n_annuli + i] = radius_i**2.
+ *  CELL: cellUniverse_class.f90 -- n_cells simpleCells in definition order (local IDs
+ *        1..n_cells), local IDs n_cells+1 / n_cells+2 are the UNDEF / OVERLAP cells.
+ *        int_data[cells + i] is the offset in int_data of local cell i+1's record
+ *        [n, s_1 .. s_n]: s_k = +-(1-based index into surfaces), + for the positive halfspace
+ *        (simpleCell_class.f90).
+ * Every type: origin, and rotMat (universe_inter.f90 setTransform, rotationMatrix of
+ * genericProcedures.f90) when rotated = 1, stored row-major: rot[3*(i-1) + j-1] = rotMat(i,j).
+ * global = 1: the universe is entered from the global position ('global' of setupBase).
+ * check_overlap = 1 (CELL): every cell is tested and a point inside several is in the OVERLAP
+ * cell (findCellOverlap); otherwise the first cell in definition order that holds it. */
 typedef struct scn_universe {
     int32_t type;
     int32_t n_local;
@@ -114,6 +146,13 @@ typedef struct scn_universe {
     double outline_hw[3];
     int32_t n_annuli;        /* PIN */
     int32_t annuli;
+    int32_t n_cells;         /* CELL */
+    int32_t cells;
+    int32_t border_surf;     /* ROOT */
+    int32_t rotated;
+    int32_t global;
+    int32_t check_overlap;   /* CELL */
+    double rot[9];
 } scn_universe;
 
 /* ---- multigroup data (NuclearData/mgNeutronData/baseMgNeutron, Reactions/reactionMG) -----------
@@ -180,6 +219,8 @@ typedef struct scn_problem {
     int64_t norm_bin;        /* tallyAdmin 'norm': 0-based BIN address of the norm clerk, -1 none */
     double norm_value;       /* tallyAdmin 'normVal' */
     double ht_cutoff;        /* transportOperatorHT 'cutoff' (default 0.9): DT when sigmaT/majorant > 1 - cutoff */
+    int32_t n_surfaces;      /* surfaces of cells and of a non-box root border */
+    const scn_surface* surfaces;
 } scn_problem;
 
 /* ---- eigenvalue run ------------------------------------------------------------------------ */

@@ -71,6 +71,13 @@ def lib():
         L.orc_box_transform_bc.argtypes = [C.POINTER(T.Box), d3, d3]
         L.orc_cyl_distance.restype = C.c_double
         L.orc_cyl_distance.argtypes = [C.c_double, d3, d3]
+        SP = C.POINTER(T.Surface)
+        L.orc_surf_evaluate.restype = C.c_double
+        L.orc_surf_evaluate.argtypes = [P, SP, d3]
+        L.orc_surf_distance.restype = C.c_double
+        L.orc_surf_distance.argtypes = [P, SP, d3, d3]
+        L.orc_surf_halfspace.restype = C.c_int
+        L.orc_surf_halfspace.argtypes = [P, SP, d3, d3]
         L.orc_uni_findcell.restype = C.c_int
         L.orc_uni_findcell.argtypes = [P, C.c_int32, d3, d3]
         L.orc_uni_distance.restype = C.c_double
@@ -180,6 +187,26 @@ def box_transform_bc(b, r, u):
 
 def cyl_distance(radius, r, u):
     return lib().orc_cyl_distance(radius, _d3(r), _d3(u))
+
+
+def _surf_problem(surf, problem):
+    if problem is not None:
+        return C.byref(problem)
+    if surf.type == T.SURF_BOX:
+        raise ValueError("a box surface needs the problem that holds its box")
+    return C.byref(T.Problem())
+
+
+def surf_evaluate(surf, r, problem=None):
+    return lib().orc_surf_evaluate(_surf_problem(surf, problem), C.byref(surf), _d3(r))
+
+
+def surf_distance(surf, r, u, problem=None):
+    return lib().orc_surf_distance(_surf_problem(surf, problem), C.byref(surf), _d3(r), _d3(u))
+
+
+def surf_halfspace(surf, r, u, problem=None):
+    return bool(lib().orc_surf_halfspace(_surf_problem(surf, problem), C.byref(surf), _d3(r), _d3(u)))
 
 
 def uni_findcell(comp, uniIdx, r, u):

@@ -209,6 +209,309 @@ double orc_cyl_distance(double radius, const double r[3], const double u[3])
     return o_cyl_distance(radius, radius * radius, r, u);
 }
 
+/* ---- surfaces of cells and of a non-box root border ---------------------------------------
+ * Each type restates its class in Geometry/Surfaces; dot_product and sum are evaluated as a
+ * running sum from zero in index order, x**2 as x*x. */
+
+/* the two coordinates across an axis (cylinder_class.f90 build: plane) */
+static void o_plane_of(int axis, int p[2])
+{
+    p[0] = (axis == 0) ? 1 : 0;
+    p[1] = (axis == 2) ? 1 : 2;
+}
+
+static double o_dot3(const double a[3], const double b[3])
+{
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i) s = s + a[i] * b[i];
+    return s;
+}
+
+/* quadric_class.f90 normal: gradient / norm2(gradient) */
+static void o_quadric_normal(const double* a, const double r[3], double n[3])
+{
+    n[0] = 2.0 * a[0] * r[0] + a[3] * r[1] + a[5] * r[2] + a[6];
+    n[1] = 2.0 * a[1] * r[1] + a[3] * r[0] + a[4] * r[2] + a[7];
+    n[2] = 2.0 * a[2] * r[2] + a[4] * r[1] + a[5] * r[0] + a[8];
+    double nn = scn_norm2(n);
+    for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+}
+
+/* surface%evaluate of every type */
+double orc_surf_evaluate(const scn_problem* P, const scn_surface* S, const double r[3])
+{
+    const double* c = S->c;
+    switch (S->type) {
+    case SCN_SURF_APLANE: /* aPlane_class.f90 evaluate */
+        return r[S->axis] - c[0];
+    case SCN_SURF_PLANE: /* plane_class.f90 evaluate */
+        return o_dot3(r, c) - c[3];
+    case SCN_SURF_SPHERE: { /* sphere_class.f90 evaluate */
+        double d[3];
+        for (int i = 0; i < 3; ++i) d[i] = r[i] - c[i];
+        return o_dot3(d, d) - c[4];
+    }
+    case SCN_SURF_CYLINDER: { /* cylinder_class.f90 evaluate */
+        int p[2];
+        o_plane_of(S->axis, p);
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        return (0.0 + d0 * d0 + d1 * d1) - c[4];
+    }
+    case SCN_SURF_QUADRIC: { /* quadric_class.f90 evaluate */
+        double x = r[0], y = r[1], z = r[2];
+        return x * (c[0] * x + c[3] * y + c[5] * z + c[6]) + y * (c[1] * y + c[4] * z + c[7]) +
+               z * (c[2] * z + c[8]) + c[9];
+    }
+    case SCN_SURF_BOX:
+        return orc_box_evaluate(&P->boxes[S->axis], r);
+    case SCN_SURF_TRUNC_CYLINDER: { /* truncCylinder_class.f90 evaluate */
+        int p[2];
+        o_plane_of(S->axis, p);
+        int a = S->axis;
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        double v = ((0.0 + d0 * d0 + d1 * d1) - c[3] * c[3]) / c[3] * 0.5;
+        return scn_fmax(v, fabs(r[a] - c[a]) - c[4]);
+    }
+    }
+    o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
+    return 0.0;
+}
+
+/* the quadratic branch shared by cylinder and quadric distance (a: coefficient of d^2) */
+static double o_quad_distance(double a, double k, double c, double tol)
+{
+    double delta = k * k - a * c;
+    double d;
+    if (delta < 0.0 || a == 0.0) {
+        d = SCN_INF;
+    } else if (fabs(c) < tol) {
+        if (k >= 0.0) {
+            d = SCN_INF;
+        } else {
+            d = -k + sqrt(delta);
+            d = d / a;
+        }
+    } else if (c < 0.0) {
+        d = -k + sqrt(delta);
+        d = d / a;
+    } else {
+        d = -k - sqrt(delta);
+        d = d / a;
+        if (d <= 0.0) d = SCN_INF;
+    }
+    return scn_fmin(d, SCN_INF);
+}
+
+/* surface%distance of every type */
+double orc_surf_distance(const scn_problem* P, const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    switch (S->type) {
+    case SCN_SURF_APLANE: { /* aPlane_class.f90 distance */
+        double ra = c[0] - r[S->axis];
+        double ua = u[S->axis];
+        double d;
+        if (fabs(ra) < S->tol) d = SCN_INF;
+        else if (ua != 0.0) d = ra / ua;
+        else d = SCN_INF;
+        if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        return d;
+    }
+    case SCN_SURF_PLANE: { /* plane_class.f90 distance */
+        double k = o_dot3(u, c);
+        double cc = o_dot3(r, c) - c[3];
+        double d;
+        if (k == 0.0 || fabs(cc) < S->tol) {
+            d = SCN_INF;
+        } else {
+            d = -cc / k;
+            if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        }
+        return d;
+    }
+    case SCN_SURF_SPHERE: { /* sphere_class.f90 distance (no cap at INF) */
+        double dr[3];
+        for (int i = 0; i < 3; ++i) dr[i] = r[i] - c[i];
+        double cc = o_dot3(dr, dr) - c[4];
+        double k = o_dot3(dr, u);
+        double delta = k * k - cc;
+        double d;
+        if (delta < 0.0) {
+            d = SCN_INF;
+        } else if (fabs(cc) < S->tol) {
+            d = (k >= 0.0) ? SCN_INF : -k + sqrt(delta);
+        } else if (cc < 0.0) {
+            d = -k + sqrt(delta);
+        } else {
+            d = -k - sqrt(delta);
+            if (d <= 0.0) d = SCN_INF;
+        }
+        return d;
+    }
+    case SCN_SURF_CYLINDER: { /* cylinder_class.f90 distance */
+        int p[2];
+        o_plane_of(S->axis, p);
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        double cc = (0.0 + d0 * d0 + d1 * d1) - c[4];
+        double k = 0.0 + d0 * u[p[0]] + d1 * u[p[1]];
+        double ua = u[S->axis];
+        double a = 1.0 - ua * ua;
+        return o_quad_distance(a, k, cc, S->tol);
+    }
+    case SCN_SURF_QUADRIC: { /* quadric_class.f90 distance */
+        double x = r[0], y = r[1], z = r[2], u1 = u[0], u2 = u[1], u3 = u[2];
+        double cc = orc_surf_evaluate(P, S, r);
+        double k = c[0] * u1 * x + c[1] * u2 * y + c[2] * u3 * z +
+                   0.5 * (c[3] * (u1 * y + u2 * x) + c[4] * (u2 * z + u3 * y) + c[5] * (u3 * x + u1 * z) +
+                          c[6] * u1 + c[7] * u2 + c[8] * u3);
+        double a = u1 * (c[0] * u1 + c[3] * u2 + c[5] * u3) + u2 * (c[1] * u2 + c[4] * u3) + c[2] * u3 * u3;
+        return o_quad_distance(a, k, cc, S->tol);
+    }
+    case SCN_SURF_BOX:
+        return orc_box_distance(&P->boxes[S->axis], r, u);
+    case SCN_SURF_TRUNC_CYLINDER: { /* truncCylinder_class.f90 distance */
+        int p[2];
+        o_plane_of(S->axis, p);
+        int ax = S->axis;
+        double rad = c[3], hw = c[4];
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        double c1 = (0.0 + d0 * d0 + d1 * d1) - rad * rad;
+        double k = 0.0 + d0 * u[p[0]] + d1 * u[p[1]];
+        double a = 1.0 - u[ax] * u[ax];
+        double delta = k * k - a * c1;
+        double far, near;
+        if (delta <= 0.0 || a == 0.0) {
+            far = SCN_INF;
+            near = scn_sign(SCN_INF, c1);
+        } else {
+            far = (-k + sqrt(delta)) / a;
+            near = (-k - sqrt(delta)) / a;
+            if (far < near) { double t = far; far = near; near = t; }
+        }
+        double rb = r[ax] - c[ax];
+        double tn, tf;
+        if (u[ax] != 0.0) {
+            tn = (-hw - rb) / u[ax];
+            tf = (hw - rb) / u[ax];
+        } else {
+            tn = scn_sign(SCN_INF, -hw - rb);
+            tf = scn_sign(SCN_INF, hw - rb);
+        }
+        if (tf < tn) { double t = tf; tf = tn; tn = t; }
+        far = scn_fmin(far, tf);
+        near = scn_fmax(near, tn);
+        double cc = scn_fmax(c1 / rad * 0.5, fabs(rb) - hw);
+        double d;
+        if (far <= near * SCN_MISS_TOL) d = SCN_INF;
+        else if (fabs(cc) < S->tol) d = (fabs(far) >= fabs(near)) ? far : near;
+        else d = (near <= 0.0) ? far : near;
+        if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        return d;
+    }
+    }
+    o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
+    return SCN_INF;
+}
+
+/* surface%going: the halfspace a point on the surface moves into */
+static int o_surf_going(const scn_problem* P, const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    switch (S->type) {
+    case SCN_SURF_APLANE: { /* aPlane_class.f90 going */
+        double ua = u[S->axis];
+        if (ua == 0.0) return (r[S->axis] - c[0]) >= 0.0;
+        return ua > 0.0;
+    }
+    case SCN_SURF_PLANE: { /* plane_class.f90 going */
+        double proj = o_dot3(u, c);
+        if (proj == 0.0) return (o_dot3(r, c) - c[3]) >= 0.0;
+        return proj > 0.0;
+    }
+    case SCN_SURF_SPHERE: { /* sphere_class.f90 going */
+        double dr[3];
+        for (int i = 0; i < 3; ++i) dr[i] = r[i] - c[i];
+        return o_dot3(dr, u) >= 0.0;
+    }
+    case SCN_SURF_CYLINDER: { /* cylinder_class.f90 going */
+        int p[2];
+        o_plane_of(S->axis, p);
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        return (0.0 + d0 * u[p[0]] + d1 * u[p[1]]) >= 0.0;
+    }
+    case SCN_SURF_QUADRIC: { /* quadric_class.f90 going */
+        double n[3];
+        o_quadric_normal(c, r, n);
+        return o_dot3(n, u) >= 0.0;
+    }
+    case SCN_SURF_TRUNC_CYLINDER: { /* truncCylinder_class.f90 going */
+        int p[2];
+        o_plane_of(S->axis, p);
+        int a = S->axis;
+        double rad = c[3];
+        double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+        double c1 = ((0.0 + d0 * d0 + d1 * d1) - rad * rad) / rad * 0.5;
+        double rv = r[a] - c[a];
+        double c2 = fabs(rv) - c[4];
+        double proj, cc;
+        if (c1 >= 2.0 * rad * c2) {
+            proj = 0.0 + u[p[0]] * d0 + u[p[1]] * d1;
+            cc = c1;
+        } else {
+            proj = u[a] * rv;
+            cc = c2;
+        }
+        if (proj == 0.0) return cc >= 0.0;
+        return proj > 0.0;
+    }
+    }
+    o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
+    return 0;
+}
+
+/* surface_inter.f90:363-378 halfspace (box types keep their own, box_class.f90) */
+int orc_surf_halfspace(const scn_problem* P, const scn_surface* S, const double r[3], const double u[3])
+{
+    if (S->type == SCN_SURF_BOX) return orc_box_halfspace(&P->boxes[S->axis], r, u);
+    double c = orc_surf_evaluate(P, S, r);
+    int hs = c > 0.0;
+    if (fabs(c) < S->tol) hs = o_surf_going(P, S, r, u);
+    return hs;
+}
+
+/* ---- cells: simpleCell_class.f90 (record [n, +-surfIdx ...] in int_data) ------------------ */
+
+/* inside: every surface halfspace agrees with the cell's sense */
+static int o_cell_inside(const scn_problem* P, int rec, const double r[3], const double u[3])
+{
+    const int32_t* c = P->int_data + rec;
+    int isIt = 0;
+    for (int i = 1; i <= c[0]; ++i) {
+        int s = c[i];
+        int sense = s > 0;
+        int hs = orc_surf_halfspace(P, &P->surfaces[abs(s) - 1], r, u);
+        isIt = (hs != 0) == sense;
+        if (!isIt) return 0;
+    }
+    return isIt;
+}
+
+/* distance: the nearest surface of the cell (the first of equal distances) */
+static double o_cell_distance(const scn_problem* P, int rec, const double r[3], const double u[3], int32_t* surfIdx)
+{
+    const int32_t* c = P->int_data + rec;
+    double d = SCN_INF;
+    *surfIdx = 0;
+    for (int i = 1; i <= c[0]; ++i) {
+        double t = orc_surf_distance(P, &P->surfaces[abs(c[i]) - 1], r, u);
+        if (t < d) {
+            d = t;
+            *surfIdx = abs(c[i]);
+        }
+    }
+    return d;
+}
+
 /* ======================================================================================
  * Universes
  * ====================================================================================== */
@@ -249,6 +552,7 @@ int orc_uni_findcell(const scn_problem* P, int32_t uniIdx, const double r[3], co
     const scn_universe* U = o_uni(P, uniIdx);
     switch (U->type) {
     case SCN_UNI_ROOT: /* rootUniverse_class.f90:127-145 */
+        if (U->border_surf >= 0) return orc_surf_halfspace(P, &P->surfaces[U->border_surf], r, u) ? 2 : 1;
         return orc_box_halfspace(&P->boxes[U->border], r, u) ? 2 : 1;
     case SCN_UNI_LAT: { /* latUniverse_class.f90:270-312 */
         int ijk[3];
@@ -271,6 +575,20 @@ int orc_uni_findcell(const scn_problem* P, int32_t uniIdx, const double r[3], co
             if (r_sq < rsq[l - 1] + mul * (2.0 * rad[l - 1] * SCN_SURF_TOL)) return l;
         return l;
     }
+    case SCN_UNI_CELL: { /* cellUniverse_class.f90:203-250 findCell, 262-297 findCellOverlap */
+        const int32_t* cl = P->int_data + U->cells;
+        if (U->check_overlap) {
+            int found = 0, id = 0;
+            for (int i = 1; i <= U->n_cells; ++i)
+                if (o_cell_inside(P, cl[i - 1], r, u)) { ++found; id = i; }
+            if (found == 0) return U->n_cells + 1;
+            if (found > 1) return U->n_cells + 2;
+            return id;
+        }
+        for (int i = 1; i <= U->n_cells; ++i)
+            if (o_cell_inside(P, cl[i - 1], r, u)) return i;
+        return U->n_cells + 1;
+    }
     }
     o_fail(SCN_ERR_ARG, "findCell: unknown universe type %d", U->type);
     return 1;
@@ -283,6 +601,7 @@ double orc_uni_distance(const scn_problem* P, int32_t uniIdx, int32_t localID, c
     switch (U->type) {
     case SCN_UNI_ROOT: /* rootUniverse_class.f90:152-162 */
         *surfIdx = SURF_BORDER;
+        if (U->border_surf >= 0) return orc_surf_distance(P, &P->surfaces[U->border_surf], r, u);
         return orc_box_distance(&P->boxes[U->border], r, u);
     case SCN_UNI_LAT: { /* latUniverse_class.f90:319-373 */
         if (localID == U->out_local) {
@@ -321,6 +640,19 @@ double orc_uni_distance(const scn_problem* P, int32_t uniIdx, int32_t localID, c
         *surfIdx = PIN_MOVING_OUT;
         return d_out;
     }
+    case SCN_UNI_CELL: { /* cellUniverse_class.f90:359-378 */
+        if (localID == U->n_cells + 1) {
+            o_fail(SCN_ERR_MATERIAL, "Particle is in undefined local cell. Local ID: %d", localID);
+            *surfIdx = 0;
+            return SCN_INF;
+        }
+        if (localID == U->n_cells + 2) {
+            o_fail(SCN_ERR_MATERIAL, "Particle is in an overlapping local cell. Local ID: %d", localID);
+            *surfIdx = 0;
+            return SCN_INF;
+        }
+        return o_cell_distance(P, P->int_data[U->cells + localID - 1], r, u, surfIdx);
+    }
     }
     o_fail(SCN_ERR_ARG, "distance: unknown universe type %d", U->type);
     return SCN_INF;
@@ -334,6 +666,12 @@ static void o_uni_cross(const scn_problem* P, o_coord* c, int surfIdx)
         if (surfIdx == PIN_MOVING_IN) c->localID -= 1;
         else if (surfIdx == PIN_MOVING_OUT) c->localID += 1;
         else o_fail(SCN_ERR_ARG, "pin cross: unknown surface memento %d", surfIdx);
+    } else if (U->type == SCN_UNI_CELL) {
+        /* cellUniverse_class.f90:388-436: NUDGE the local position over the surface, then search
+         * (the neighbour lists only speed the search up: cells do not overlap) */
+        for (int i = 0; i < 3; ++i) c->r[i] = c->r[i] + c->dir[i] * SCN_NUDGE;
+        c->localID = orc_uni_findcell(P, c->uniIdx, c->r, c->dir);
+        c->cellIdx = 0;
     } else {
         c->localID = orc_uni_findcell(P, c->uniIdx, c->r, c->dir);
         c->cellIdx = 0;
@@ -355,7 +693,19 @@ static void o_uni_offset(const scn_problem* P, const o_coord* c, double off[3])
     }
 }
 
-/* universe_inter.f90:400-424 enter (no rotation) */
+/* matmul(rotMat, v) (rotMat row-major): each component a running sum from zero */
+static void o_matvec(const double R[9], const double v[3], double out[3])
+{
+    double t[3];
+    for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < 3; ++j) s = s + R[3 * i + j] * v[j];
+        t[i] = s;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = t[i];
+}
+
+/* universe_inter.f90:400-424 enter: rotate position and direction, then translate */
 static void o_uni_enter(const scn_problem* P, int uniIdx, o_coord* nw, const double r[3], const double u[3])
 {
     const scn_universe* U = o_uni(P, uniIdx);
@@ -364,6 +714,10 @@ static void o_uni_enter(const scn_problem* P, int uniIdx, o_coord* nw, const dou
         nw->dir[i] = u[i];
     }
     nw->uniIdx = uniIdx;
+    if (U->rotated) {
+        o_matvec(U->rot, nw->r, nw->r);
+        o_matvec(U->rot, nw->dir, nw->dir);
+    }
     for (int i = 0; i < 3; ++i) nw->r[i] = nw->r[i] - U->origin[i];
     nw->localID = orc_uni_findcell(P, uniIdx, nw->r, nw->dir);
     nw->cellIdx = 0;
@@ -387,6 +741,18 @@ static void o_move_local(o_coords* c, double d, int n)
         for (int i = 0; i < 3; ++i) c->lvl[l].r[i] = c->lvl[l].r[i] + d * c->lvl[l].dir[i];
 }
 
+/* coord_class.f90 rotate / assignDirection: new global direction, lower levels through their
+ * universes' rotation matrices */
+static void o_assign_direction(const scn_problem* P, o_coords* c, const double u[3])
+{
+    for (int k = 0; k < 3; ++k) c->lvl[0].dir[k] = u[k];
+    for (int l = 1; l < c->nesting; ++l) {
+        const scn_universe* U = o_uni(P, c->lvl[l].uniIdx);
+        if (U->rotated) o_matvec(U->rot, c->lvl[l - 1].dir, c->lvl[l].dir);
+        else for (int k = 0; k < 3; ++k) c->lvl[l].dir[k] = c->lvl[l - 1].dir[k];
+    }
+}
+
 /* geometryStd_class.f90:565-619 diveToMat (level is 1-based) */
 static void o_dive(const scn_problem* P, o_coords* c, int start)
 {
@@ -404,7 +770,10 @@ static void o_dive(const scn_problem* P, o_coords* c, int start)
         if (i == SCN_MAX_NEST) break;
         double off[3], r[3];
         o_uni_offset(P, L, off);
-        for (int k = 0; k < 3; ++k) r[k] = L->r[k] - off[k];
+        if (o_uni(P, -fill)->global) /* the nested universe reverts to the global frame */
+            for (int k = 0; k < 3; ++k) r[k] = c->lvl[0].r[k];
+        else
+            for (int k = 0; k < 3; ++k) r[k] = L->r[k] - off[k];
         c->nesting += 1;
         o_uni_enter(P, -fill, &c->lvl[i], r, L->dir);
         c->lvl[i].uniRootID = id;
@@ -471,7 +840,7 @@ static int o_move(const scn_problem* P, o_coords* c, double* maxDist, o_cache* c
         o_move_global(c, dist);
         *maxDist = dist;
         cache->lvl = 0;
-        orc_box_explicit_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
+        if (P->border_box >= 0) orc_box_explicit_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
         o_place_coord(P, c);
         return EV_BOUNDARY;
     } else {
@@ -491,7 +860,7 @@ static void o_teleport(const scn_problem* P, o_coords* c, double dist)
     o_move_global(c, dist);
     o_place_coord(P, c);
     if (c->matIdx == SCN_OUTSIDE_MAT) {
-        orc_box_transform_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
+        if (P->border_box >= 0) orc_box_transform_bc(&P->boxes[P->border_box], c->lvl[0].r, c->lvl[0].dir);
         o_place_coord(P, c);
     }
 }
@@ -933,8 +1302,7 @@ static void o_collide(o_hctx* X, o_part* p)
         p->w = p->w * w_mul;
         double nd[3];
         scn_rotate(p->coords.lvl[0].dir, mu, phi, nd);
-        for (int l = 0; l < p->coords.nesting; ++l)
-            for (int k = 0; k < 3; ++k) p->coords.lvl[l].dir[k] = nd[k];
+        o_assign_direction(P, &p->coords, nd);
     } else if (C == 3 || C == 4) {
         p->isDead = 1;
     } else {

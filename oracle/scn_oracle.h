@@ -62,6 +62,9 @@ double orc_dsum(const double* x, int64_t n);
 double orc_box_distance(const scn_box* b, const double r[3], const double u[3]);
 double orc_box_evaluate(const scn_box* b, const double r[3]);
 int orc_box_halfspace(const scn_box* b, const double r[3], const double u[3]);
+double orc_surf_evaluate(const scn_problem* P, const scn_surface* S, const double r[3]);
+double orc_surf_distance(const scn_problem* P, const scn_surface* S, const double r[3], const double u[3]);
+int orc_surf_halfspace(const scn_problem* P, const scn_surface* S, const double r[3], const double u[3]);
 void orc_box_explicit_bc(const scn_box* b, double r[3], double u[3]);
 void orc_box_transform_bc(const scn_box* b, double r[3], double u[3]);
 double orc_cyl_distance(double radius, const double r[3], const double u[3]);

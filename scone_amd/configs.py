@@ -465,4 +465,98 @@ def core_3d(seed: int = 5, nG: int = 70, pop: int = 10_000_000, inactive: int = 
     return inp, mats
 
 
-CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g, "core_3d": core_3d}
+def absorbing_sphere(radius: float = 1.7, sig: float = 0.9, pop: int = 4000, cycles: int = 20,
+                     transport: str = "transportOperatorST"):
+    """Fixed source in a purely absorbing sphere (the root border, vacuum) with a centred isotropic
+    point source, one total-flux bin: flux per source neutron (1 - exp(-sig R)) / sig."""
+    mg = {"m1": {"numberOfGroups": 1, "capture": [sig], "scatteringMultiplicity": [1.0], "P0": [0.0]}}
+    geom = {"type": "geometryStd", "boundary": [0, 0, 0, 0, 0, 0], "graph": {"type": "shrunk"},
+            "surfaces": {"ball": {"id": 1, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": radius}},
+            "cells": {},
+            "universes": {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "m1"}}}
+    src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "G": 1}
+    return _fixed_inp(geom, mg, src, {"total": _flux_clerk()}, pop, cycles, 3, transport), mg
+
+
+def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int = 10,
+                 transport: str = "transportOperatorST", n: int = 4, fixed: bool = False, outer: bool = False):
+    """Pin lattice built from CSG cells: an n x n lattice (pitch 1.26, reflective box with finite z)
+    of two cellUniverse pins made of every surface type the engine reads -- zCylinder and quadric
+    annuli with a zTruncCylinder insert (pin A); a square fuel block of x/yPlanes cut by a
+    diagonal plane, with a sphere inclusion, in a universe rotated 30 degrees about z (pin B) --
+    the 7-group materials of ``pin_lattice_7g``, and a mesh flux tally.  ``fixed``: a fixed-source
+    solve with a material source in the fuel instead of an eigenvalue run.  ``outer``: the lattice
+    sits in a 1 x 1 lattice of its own (4 nesting levels) entered through a 'global' universe."""
+    mg = pin_lattice_materials(seed, 7)
+    surfs = {
+        "box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [0.63 * n, 0.63 * n, 2.0]},
+        "cyl": {"id": 2, "type": "zCylinder", "origin": [0.0, 0.0, 0.0], "radius": 0.41},
+        "quad": {"id": 3, "type": "quadric", "coeffs": [1.0, 1.0, 0.0, 0, 0, 0, 0, 0, 0, -0.2304]},   # r 0.48
+        "rod": {"id": 4, "type": "zTruncCylinder", "origin": [0.0, 0.0, 0.5], "radius": 0.15, "halfwidth": 1.0},
+        "xl": {"id": 5, "type": "xPlane", "x0": -0.35}, "xh": {"id": 6, "type": "xPlane", "x0": 0.35},
+        "yl": {"id": 7, "type": "yPlane", "y0": -0.35}, "yh": {"id": 8, "type": "yPlane", "y0": 0.35},
+        "diag": {"id": 9, "type": "plane", "coeffs": [1.0, 1.0, 0.0, 0.4]},
+        "ball": {"id": 10, "type": "sphere", "origin": [0.1, -0.1, 0.3], "radius": 0.12},
+    }
+    cells = {
+        "a_rod": {"id": 1, "type": "simpleCell", "surfaces": [-4], "filltype": "mat", "material": "structural"},
+        "a_fuel": {"id": 2, "type": "simpleCell", "surfaces": [4, -2], "filltype": "mat", "material": "fuel"},
+        "a_gap": {"id": 3, "type": "simpleCell", "surfaces": [2, -3], "filltype": "mat", "material": "structural"},
+        "a_mod": {"id": 4, "type": "simpleCell", "surfaces": [3], "filltype": "mat", "material": "moderator"},
+        "b_ball": {"id": 5, "type": "simpleCell", "surfaces": [-10], "filltype": "mat", "material": "structural"},
+        "b_fuel": {"id": 6, "type": "simpleCell", "surfaces": [10, 5, -6, 7, -8, -9], "filltype": "mat",
+                   "material": "fuel"},
+        "b_cut": {"id": 7, "type": "simpleCell", "surfaces": [10, 5, -6, 7, -8, 9], "filltype": "mat",
+                  "material": "moderator"},
+        "b_w": {"id": 8, "type": "simpleCell", "surfaces": [10, -5], "filltype": "mat", "material": "moderator"},
+        "b_e": {"id": 9, "type": "simpleCell", "surfaces": [10, 6], "filltype": "mat", "material": "moderator"},
+        "b_s": {"id": 10, "type": "simpleCell", "surfaces": [10, 5, -6, -7], "filltype": "mat", "material": "moderator"},
+        "b_n": {"id": 11, "type": "simpleCell", "surfaces": [10, 5, -6, 8], "filltype": "mat", "material": "moderator"},
+    }
+    rng = np.random.default_rng(seed + 77)
+    lat_map = [int(x) for x in rng.choice([21, 22], size=n * n)]
+    unis = {"root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<11>" if outer else "u<10>"}}
+    if outer:
+        unis["outer"] = {"id": 11, "type": "latUniverse", "pitch": [1.26 * n, 1.26 * n, 0.0], "shape": [1, 1, 0],
+                         "padMat": "moderator", "map": [12]}
+        unis["wrap"] = {"id": 12, "type": "cellUniverse", "global": 1, "cells": [12]}
+        cells = dict(cells)
+        cells["wrap_all"] = {"id": 12, "type": "simpleCell", "surfaces": [-13], "filltype": "uni", "universe": 10}
+        surfs = dict(surfs)
+        surfs["big"] = {"id": 13, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 100.0}
+    geom = {"type": "geometryStd", "boundary": [1, 1, 1, 1, 1, 1], "graph": {"type": "shrunk"},
+            "surfaces": surfs, "cells": cells,
+            "universes": {
+                **unis,
+                "pinA": {"id": 21, "type": "cellUniverse", "cells": [1, 2, 3, 4]},
+                "pinB": {"id": 22, "type": "cellUniverse", "rotation": [30.0, 0.0, 0.0],
+                         "cells": [5, 6, 7, 8, 9, 10, 11]},
+                "lat": {"id": 10, "type": "latUniverse", "pitch": [1.26, 1.26, 0.0], "shape": [n, n, 0],
+                        "padMat": "moderator", "map": lat_map}}}
+    h = 0.63 * n
+    mesh = {"type": "collisionClerk",
+            "map": {"type": "multiMap", "maps": ["xax", "yax"],
+                    "xax": {"type": "spaceMap", "axis": "x", "grid": "lin", "min": -h, "max": h, "N": 4 * n},
+                    "yax": {"type": "spaceMap", "axis": "y", "grid": "lin", "min": -h, "max": h, "N": 4 * n}},
+            "response": ["flux", "fiss"], "flux": {"type": "fluxResponse"},
+            "fiss": {"type": "macroResponse", "MT": -6}}
+    mats = {"type": "materialMap", "materials": ["fuel", "structural", "moderator"]}
+    tally = {"mesh": mesh, "mats": {"type": "collisionClerk", "map": mats, "response": ["flux"],
+                                    "flux": {"type": "fluxResponse"}}}
+    nd = {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
+          "materials": {m: {"temp": 300, "composition": {}, "xsFile": m} for m in mg}}
+    if fixed:
+        src = {"type": "materialSource", "data": "mg", "mat": "fuel", "G": 1}
+        out = _fixed_inp(geom, mg, src, tally, pop, active, seed, transport)
+        out["buffer"] = 200
+        return out, mg
+    inp = {"type": "eigenPhysicsPackage", "pop": pop, "active": active, "inactive": inactive,
+           "XSdata": "mg", "dataType": "mg", "seed": seed,
+           "collisionOperator": {"neutronMG": {"type": "neutronMGstd"}},
+           "transportOperator": {"type": transport},
+           "inactiveTally": {}, "activeTally": tally, "geometry": geom, "nuclearData": nd}
+    return inp, mg
+
+
+CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g, "core_3d": core_3d,
+           "cell_lattice": cell_lattice}

@@ -97,8 +97,8 @@ struct SlotState {
 // cycle, [3] tile-table overflow, [4] dummy site counter
 constexpr int CTL_WORDS = 8;
 
-template <int NL>
-__device__ __forceinline__ void bload(const BLive& L, int cap, int i, Pt<NL>& p)
+template <int NL, int GEO>
+__device__ __forceinline__ void bload(const BLive& L, int cap, int i, Pt<NL, GEO>& p)
 {
     p.nest = L.nest[i];
 #pragma unroll
@@ -134,8 +134,8 @@ __device__ __forceinline__ void bload(const BLive& L, int cap, int i, Pt<NL>& p)
     p.track = 0.0;
 }
 
-template <int NL>
-__device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<NL>& p)
+template <int NL, int GEO>
+__device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<NL, GEO>& p)
 {
     L.nest[i] = p.nest;
 #pragma unroll
@@ -163,8 +163,8 @@ __device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<
 // (fixedSourcePhysicsPackage_class.f90:187-213): the source is sampled on the seed strided by
 // 2*pop*batch + j (generate, source_inter.f90:110-116) and the history runs on the seed strided
 // by 2*pop*batch + pop + j.  Returns 0 if sampling failed (error raised).
-template <int NL>
-__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int i, Pt<NL>& p,
+template <int NL, int GEO>
+__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int i, Pt<NL, GEO>& p,
                                     DevStatus* st)
 {
     const int64_t j = (int64_t)i + 1;
@@ -363,7 +363,7 @@ __device__ __forceinline__ void flush_lds_bins(Bins& B, int lane)
 // the tile at once, so lanes do not idle while others finish.
 // When the tile is used up the wave takes the next tile: one of the same slot continues the
 // refill directly, one of another slot starts once the live lanes have finished.
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 __global__ void __launch_bounds__(BBLOCK)
 k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __restrict__ tiles, int* ctl, int first,
              int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave, Sites SB)
@@ -390,7 +390,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     B.c_addr = -1;
     B.c_acc = 0.0;
     Cnt cn{0, 0, 0};
-    Pt<NL> p;
+    Pt<NL, GEO> p;
     DistCache<NL> cache;   // surface tracking: the current flight's distance cache
     cache.lvl = 0;
     int alive = 0, steps = 0, nseg = 0;
@@ -621,29 +621,34 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
 typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int,
                              int, int, int, DevStatus*, int, Sites);
 
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles,
                int* ctl, int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave,
                Sites SB)
 {
-    hipLaunchKernelGGL((k_btransport<NL, MODE>), g, b, sh, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
+    hipLaunchKernelGGL((k_btransport<NL, MODE, GEO>), g, b, sh, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
                        max_steps, st, lds_per_wave, SB);
 }
 
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 const void* bt_kernel()
 {
-    return (const void*)k_btransport<NL, MODE>;
+    return (const void*)k_btransport<NL, MODE, GEO>;
 }
 
+// kernel variant key: register depth NL, plus GEO_KEY for the general-geometry kernels
+constexpr int GEO_KEY = 16;
+
 template <int MODE>
-void pick_bt(int nl, BTransportFn* fn, const void** kern)
+void pick_bt(int key, BTransportFn* fn, const void** kern)
 {
-    switch (nl) {
-    case 1: *fn = launch_bt<1, MODE>; *kern = bt_kernel<1, MODE>(); break;
-    case 3: *fn = launch_bt<3, MODE>; *kern = bt_kernel<3, MODE>(); break;
-    case 4: *fn = launch_bt<4, MODE>; *kern = bt_kernel<4, MODE>(); break;
-    default: *fn = launch_bt<8, MODE>; *kern = bt_kernel<8, MODE>(); break;
+    switch (key) {
+    case 1: *fn = launch_bt<1, MODE, 0>; *kern = bt_kernel<1, MODE, 0>(); break;
+    case 3: *fn = launch_bt<3, MODE, 0>; *kern = bt_kernel<3, MODE, 0>(); break;
+    case 4: *fn = launch_bt<4, MODE, 0>; *kern = bt_kernel<4, MODE, 0>(); break;
+    case 3 + GEO_KEY: *fn = launch_bt<3, MODE, 1>; *kern = bt_kernel<3, MODE, 1>(); break;
+    case 8 + GEO_KEY: *fn = launch_bt<8, MODE, 1>; *kern = bt_kernel<8, MODE, 1>(); break;
+    default: *fn = launch_bt<8, MODE, 0>; *kern = bt_kernel<8, MODE, 0>(); break;
     }
 }
 
@@ -654,6 +659,7 @@ struct SlotHost {
     int64_t n_bins = 0;
     int pop = 0, n_batches = 0, record = 0;
     int fissile = 0, buffer = 0;
+    int geo = 0;        // needs the general-geometry kernels
 };
 
 struct Batch {
@@ -812,6 +818,10 @@ int validate(Batch* B, const scn_fixed_request* R, int* depth)
         B->err = "buffer size outside 0..MAX_BUFFER";
         return SCN_ERR_ARG;
     }
+    if (const char* m = check_cell_tables(pr)) {
+        B->err = m;
+        return SCN_ERR_ARG;
+    }
     int d = problem_depth(pr);
     if (d < 1 || d > SCN_MAX_NEST) {
         B->err = "geometry nesting deeper than SCN_MAX_NEST or malformed graph";
@@ -889,12 +899,17 @@ int check_bstatus(Batch* B)
     return 0;
 }
 
+// kernel variant key for the resident slots: the deepest geometry, general kernels if any slot
+// needs them
 int max_depth_nl(Batch* B)
 {
-    int d = 1;
+    int d = 1, geo = 0;
     for (const SlotHost& h : B->hs)
-        if (h.used) d = std::max(d, h.depth);
-    return compiled_nl(d);
+        if (h.used) {
+            d = std::max(d, h.depth);
+            geo |= h.geo;
+        }
+    return compiled_nl(d, geo) + (geo ? GEO_KEY : 0);
 }
 
 size_t lds_bytes(const Batch* B) { return sizeof(double) * (size_t)B->S_lds_per_wave * (BBLOCK / BWAVE); }
@@ -1164,6 +1179,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     Image I;
     size_t o_unis = I.put(pr->universes, pr->n_universes);
     size_t o_boxes = I.put(pr->boxes, pr->n_boxes);
+    size_t o_surfs = I.put(pr->surfaces, (size_t)std::max(0, pr->n_surfaces));
     size_t o_gf = I.put(pr->graph_fill, pr->graph_size);
     size_t o_gi = I.put(pr->graph_id, pr->graph_size);
     size_t o_real = I.put(pr->real_data, pr->n_real);
@@ -1221,6 +1237,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.graph_size = pr->graph_size;
     P.unis = (const scn_universe*)(base + o_unis);
     P.boxes = (const scn_box*)(base + o_boxes);
+    P.surfs = (const scn_surface*)(base + o_surfs);
+    P.any_rot = any_rotated(pr);
     P.gfill = (const int*)(base + o_gf);
     P.gid = (const int*)(base + o_gi);
     P.real = (const double*)(base + o_real);
@@ -1287,6 +1305,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     h.off = off;
     h.bytes = Ly.total;
     h.depth = depth;
+    h.geo = needs_full_geometry(pr);
     h.n_bins = pr->n_bins;
     h.pop = R->pop;
     h.n_batches = R->n_batches;

@@ -1,6 +1,6 @@
 // scn_device.h -- device-side physics of the multigroup tracking loop (gfx950).
 //
-// One particle lives in registers as a Pt<NL> (NL = compile-time nesting depth of the geometry,
+// One particle lives in registers as a Pt<NL, GEO> (NL = compile-time nesting depth of the geometry,
 // so every level array is statically indexed and stays in VGPRs).  The arithmetic of every
 // routine reproduces the reference's operation order -- the same order the CPU oracle uses --
 // so a history computed here is bit-identical to the oracle's:
@@ -45,6 +45,8 @@ struct DevProblem {
     int64_t n_bins;
     double ht_cutoff;
     const double* P1;    // [mat][G_in][G_out] 3 P1/P0, or null (P0 scattering)
+    const scn_surface* surfs;   // surfaces of cells and of a non-box root border
+    int any_rot;         // some universe is rotated: lower-level directions differ from the global one
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -73,7 +75,7 @@ struct Cnt {
     int64_t seg, col, sc;
 };
 
-template <int NL>
+template <int NL, int GEO = 0>
 struct Pt {
     double r[NL][3];   // position at each nesting level (coordList%lvl(l)%r)
     double u[3];       // direction (identical at every level: no rotated universes)
@@ -265,6 +267,303 @@ __device__ __forceinline__ double cyl_distance(double radius, double r_sq, const
     return scn_fmin(d, SCN_INF);
 }
 
+
+// ---- surfaces of cells and of a non-box root border (Geometry/Surfaces) --------------------
+// dot_product / sum as a running sum from zero in index order; x**2 as x*x.
+__device__ __forceinline__ double dot3(const double a[3], const double b[3])
+{
+    return 0.0 + a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// coordinates across an axis (cylinder_class.f90 build: plane)
+__device__ __forceinline__ void plane_of(int axis, int& p0, int& p1)
+{
+    p0 = (axis == 0) ? 1 : 0;
+    p1 = (axis == 2) ? 1 : 2;
+}
+
+// shared quadratic root choice of cylinder_class.f90 / quadric_class.f90 distance
+__device__ __forceinline__ double quad_root(double a, double k, double c, double tol)
+{
+    const double delta = k * k - a * c;
+    double d;
+    if (delta < 0.0 || a == 0.0) {
+        d = SCN_INF;
+    } else if (fabs(c) < tol) {
+        if (k >= 0.0) {
+            d = SCN_INF;
+        } else {
+            d = -k + sqrt(delta);
+            d = d / a;
+        }
+    } else if (c < 0.0) {
+        d = -k + sqrt(delta);
+        d = d / a;
+    } else {
+        d = -k - sqrt(delta);
+        d = d / a;
+        if (d <= 0.0) d = SCN_INF;
+    }
+    return scn_fmin(d, SCN_INF);
+}
+
+__device__ __forceinline__ double surf_evaluate(const DevProblem& P, const scn_surface& S, const double r[3])
+{
+    const double* c = S.c;
+    switch (S.type) {
+    case SCN_SURF_APLANE: return get3(r, S.axis) - c[0];
+    case SCN_SURF_PLANE: {
+        const double n[3] = {c[0], c[1], c[2]};
+        return dot3(r, n) - c[3];
+    }
+    case SCN_SURF_SPHERE: {
+        const double d[3] = {r[0] - c[0], r[1] - c[1], r[2] - c[2]};
+        return dot3(d, d) - c[4];
+    }
+    case SCN_SURF_CYLINDER: {
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        return (0.0 + d0 * d0 + d1 * d1) - c[4];
+    }
+    case SCN_SURF_QUADRIC: {
+        const double x = r[0], y = r[1], z = r[2];
+        return x * (c[0] * x + c[3] * y + c[5] * z + c[6]) + y * (c[1] * y + c[4] * z + c[7]) +
+               z * (c[2] * z + c[8]) + c[9];
+    }
+    case SCN_SURF_BOX: return box_evaluate(P.boxes[S.axis], r);
+    default: {   // SCN_SURF_TRUNC_CYLINDER
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        const double v = ((0.0 + d0 * d0 + d1 * d1) - c[3] * c[3]) / c[3] * 0.5;
+        return scn_fmax(v, fabs(get3(r, S.axis) - c[S.axis]) - c[4]);
+    }
+    }
+}
+
+__device__ __forceinline__ double surf_distance(const DevProblem& P, const scn_surface& S, const double r[3],
+                                                const double u[3])
+{
+    const double* c = S.c;
+    switch (S.type) {
+    case SCN_SURF_APLANE: {
+        const double ra = c[0] - get3(r, S.axis);
+        const double ua = get3(u, S.axis);
+        double d = SCN_INF;
+        if (!(fabs(ra) < S.tol) && ua != 0.0) d = ra / ua;
+        if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        return d;
+    }
+    case SCN_SURF_PLANE: {
+        const double n[3] = {c[0], c[1], c[2]};
+        const double k = dot3(u, n);
+        const double cc = dot3(r, n) - c[3];
+        if (k == 0.0 || fabs(cc) < S.tol) return SCN_INF;
+        double d = -cc / k;
+        if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        return d;
+    }
+    case SCN_SURF_SPHERE: {
+        const double dr[3] = {r[0] - c[0], r[1] - c[1], r[2] - c[2]};
+        const double cc = dot3(dr, dr) - c[4];
+        const double k = dot3(dr, u);
+        const double delta = k * k - cc;
+        double d;
+        if (delta < 0.0) {
+            d = SCN_INF;
+        } else if (fabs(cc) < S.tol) {
+            d = (k >= 0.0) ? SCN_INF : -k + sqrt(delta);
+        } else if (cc < 0.0) {
+            d = -k + sqrt(delta);
+        } else {
+            d = -k - sqrt(delta);
+            if (d <= 0.0) d = SCN_INF;
+        }
+        return d;
+    }
+    case SCN_SURF_CYLINDER: {
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        const double cc = (0.0 + d0 * d0 + d1 * d1) - c[4];
+        const double k = 0.0 + d0 * get3(u, p0) + d1 * get3(u, p1);
+        const double ua = get3(u, S.axis);
+        return quad_root(1.0 - ua * ua, k, cc, S.tol);
+    }
+    case SCN_SURF_QUADRIC: {
+        const double x = r[0], y = r[1], z = r[2], u1 = u[0], u2 = u[1], u3 = u[2];
+        const double cc = x * (c[0] * x + c[3] * y + c[5] * z + c[6]) + y * (c[1] * y + c[4] * z + c[7]) +
+                          z * (c[2] * z + c[8]) + c[9];
+        const double k = c[0] * u1 * x + c[1] * u2 * y + c[2] * u3 * z +
+                         0.5 * (c[3] * (u1 * y + u2 * x) + c[4] * (u2 * z + u3 * y) + c[5] * (u3 * x + u1 * z) +
+                                c[6] * u1 + c[7] * u2 + c[8] * u3);
+        const double a = u1 * (c[0] * u1 + c[3] * u2 + c[5] * u3) + u2 * (c[1] * u2 + c[4] * u3) + c[2] * u3 * u3;
+        return quad_root(a, k, cc, S.tol);
+    }
+    case SCN_SURF_BOX: return box_distance(P.boxes[S.axis], r, u);
+    default: {   // SCN_SURF_TRUNC_CYLINDER: cylinder slab intersected with the axial slab
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double rad = c[3], hw = c[4];
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        const double c1 = (0.0 + d0 * d0 + d1 * d1) - rad * rad;
+        const double k = 0.0 + d0 * get3(u, p0) + d1 * get3(u, p1);
+        const double uax = get3(u, S.axis);
+        const double a = 1.0 - uax * uax;
+        const double delta = k * k - a * c1;
+        double far, near;
+        if (delta <= 0.0 || a == 0.0) {
+            far = SCN_INF;
+            near = scn_sign(SCN_INF, c1);
+        } else {
+            far = (-k + sqrt(delta)) / a;
+            near = (-k - sqrt(delta)) / a;
+            if (far < near) {
+                const double t = far;
+                far = near;
+                near = t;
+            }
+        }
+        const double rb = get3(r, S.axis) - c[S.axis];
+        double tn, tf;
+        if (uax != 0.0) {
+            tn = (-hw - rb) / uax;
+            tf = (hw - rb) / uax;
+        } else {
+            tn = scn_sign(SCN_INF, -hw - rb);
+            tf = scn_sign(SCN_INF, hw - rb);
+        }
+        if (tf < tn) {
+            const double t = tf;
+            tf = tn;
+            tn = t;
+        }
+        far = scn_fmin(far, tf);
+        near = scn_fmax(near, tn);
+        const double cc = scn_fmax(c1 / rad * 0.5, fabs(rb) - hw);
+        double d;
+        if (far <= near * SCN_MISS_TOL) d = SCN_INF;
+        else if (fabs(cc) < S.tol) d = (fabs(far) >= fabs(near)) ? far : near;
+        else d = (near <= 0.0) ? far : near;
+        if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
+        return d;
+    }
+    }
+}
+
+// surface_inter.f90 halfspace with each type's going (box types: box_halfspace)
+__device__ __forceinline__ int surf_halfspace(const DevProblem& P, const scn_surface& S, const double r[3],
+                                              const double u[3])
+{
+    if (S.type == SCN_SURF_BOX) return box_halfspace(P.boxes[S.axis], r, u);
+    const double cv = surf_evaluate(P, S, r);
+    if (!(fabs(cv) < S.tol)) return cv > 0.0;
+    const double* c = S.c;
+    switch (S.type) {
+    case SCN_SURF_APLANE: {
+        const double ua = get3(u, S.axis);
+        return (ua == 0.0) ? (get3(r, S.axis) - c[0]) >= 0.0 : ua > 0.0;
+    }
+    case SCN_SURF_PLANE: {
+        const double n[3] = {c[0], c[1], c[2]};
+        const double proj = dot3(u, n);
+        return (proj == 0.0) ? (dot3(r, n) - c[3]) >= 0.0 : proj > 0.0;
+    }
+    case SCN_SURF_SPHERE: {
+        const double dr[3] = {r[0] - c[0], r[1] - c[1], r[2] - c[2]};
+        return dot3(dr, u) >= 0.0;
+    }
+    case SCN_SURF_CYLINDER: {
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        return (0.0 + d0 * get3(u, p0) + d1 * get3(u, p1)) >= 0.0;
+    }
+    case SCN_SURF_QUADRIC: {   // quadric_class.f90 going: normalised gradient . u
+        double n[3];
+        n[0] = 2.0 * c[0] * r[0] + c[3] * r[1] + c[5] * r[2] + c[6];
+        n[1] = 2.0 * c[1] * r[1] + c[3] * r[0] + c[4] * r[2] + c[7];
+        n[2] = 2.0 * c[2] * r[2] + c[4] * r[1] + c[5] * r[0] + c[8];
+        const double nn = scn_norm2(n);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+        return dot3(n, u) >= 0.0;
+    }
+    default: {   // SCN_SURF_TRUNC_CYLINDER
+        int p0, p1;
+        plane_of(S.axis, p0, p1);
+        const double rad = c[3];
+        const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+        const double c1 = ((0.0 + d0 * d0 + d1 * d1) - rad * rad) / rad * 0.5;
+        const double rv = get3(r, S.axis) - c[S.axis];
+        const double c2 = fabs(rv) - c[4];
+        double proj, cc;
+        if (c1 >= 2.0 * rad * c2) {
+            proj = 0.0 + get3(u, p0) * d0 + get3(u, p1) * d1;
+            cc = c1;
+        } else {
+            proj = get3(u, S.axis) * rv;
+            cc = c2;
+        }
+        return (proj == 0.0) ? cc >= 0.0 : proj > 0.0;
+    }
+    }
+}
+
+// simpleCell_class.f90 inside / distance over the record [n, +-surfIdx ...] at ints[rec]
+__device__ __forceinline__ int cell_inside(const DevProblem& P, int rec, const double r[3], const double u[3])
+{
+    const int* c = P.ints + rec;
+    const int n = c[0];
+    if (n <= 0) return 0;
+    for (int i = 1; i <= n; ++i) {
+        const int s = c[i];
+        const int hs = surf_halfspace(P, P.surfs[abs(s) - 1], r, u);
+        if (hs != (s > 0)) return 0;
+    }
+    return 1;
+}
+
+__device__ __forceinline__ double cell_distance(const DevProblem& P, int rec, const double r[3], const double u[3],
+                                                int* surfIdx)
+{
+    const int* c = P.ints + rec;
+    double d = SCN_INF;
+    int si = 0;
+    for (int i = 1; i <= c[0]; ++i) {
+        const double t = surf_distance(P, P.surfs[abs(c[i]) - 1], r, u);
+        if (t < d) {
+            d = t;
+            si = abs(c[i]);
+        }
+    }
+    *surfIdx = si;
+    return d;
+}
+
+// a non-box root border
+__device__ __forceinline__ int border_halfspace(const DevProblem& P, const scn_surface& S, const double r[3],
+                                             const double u[3])
+{
+    return surf_halfspace(P, S, r, u);
+}
+__device__ __forceinline__ double border_distance(const DevProblem& P, const scn_surface& S, const double r[3],
+                                               const double u[3])
+{
+    return surf_distance(P, S, r, u);
+}
+
+// matmul(rotMat, v) with rotMat row-major (universe_inter.f90 enter, coord_class.f90 rotate)
+__device__ __forceinline__ void rot_apply(const double* R, double v[3])
+{
+    double t[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t[i] = 0.0 + R[3 * i] * v[0] + R[3 * i + 1] * v[1] + R[3 * i + 2] * v[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = t[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // universes
 // ---------------------------------------------------------------------------------------------
@@ -279,6 +578,7 @@ __device__ __forceinline__ void get_ijk(int localID, const int32_t sizeN[3], int
     ijk[2] = base + 1;
 }
 
+template <int GEO>
 __device__ __forceinline__ int uni_findcell(const DevProblem& P, int uniIdx, const double r[3], const double u[3])
 {
     const scn_universe& U = P.unis[uniIdx - 1];
@@ -302,11 +602,30 @@ __device__ __forceinline__ int uni_findcell(const DevProblem& P, int uniIdx, con
             if (r_sq < rsq[l - 1] + mul * (2.0 * rad[l - 1] * SCN_SURF_TOL)) return l;
         return l;
     }
+    if (GEO && U.type == SCN_UNI_CELL) {
+        // cellUniverse_class.f90:203-250 findCell in definition order; 262-297 findCellOverlap
+        const int* cl = P.ints + U.cells;
+        if (U.check_overlap) {
+            int found = 0, id = 0;
+            for (int i = 1; i <= U.n_cells; ++i) {
+                if (cell_inside(P, cl[i - 1], r, u)) {
+                    ++found;
+                    id = i;
+                }
+            }
+            return (found == 0) ? U.n_cells + 1 : (found > 1) ? U.n_cells + 2 : id;
+        }
+        for (int i = 1; i <= U.n_cells; ++i)
+            if (cell_inside(P, cl[i - 1], r, u)) return i;
+        return U.n_cells + 1;
+    }
+    if (GEO && U.border_surf >= 0) return border_halfspace(P, P.surfs[U.border_surf], r, u) ? 2 : 1;
     return box_halfspace(P.boxes[U.border], r, u) ? 2 : 1;
 }
 
+template <int GEO>
 __device__ __forceinline__ double uni_distance(const DevProblem& P, int uniIdx, int localID, const double r[3],
-                                      const double u[3], int* surfIdx)
+                                      const double u[3], int* surfIdx, DevStatus* st)
 {
     const scn_universe& U = P.unis[uniIdx - 1];
     if (U.type == SCN_UNI_LAT) {
@@ -359,7 +678,16 @@ __device__ __forceinline__ double uni_distance(const DevProblem& P, int uniIdx, 
         *surfIdx = PIN_OUT;
         return d_out;
     }
+    if (GEO && U.type == SCN_UNI_CELL) {   // cellUniverse_class.f90:359-378
+        if (localID > U.n_cells) {   // UNDEF or OVERLAP local cell
+            raise_error(st, SCN_ERR_MATERIAL, localID);
+            *surfIdx = 0;
+            return SCN_INF;
+        }
+        return cell_distance(P, P.ints[U.cells + localID - 1], r, u, surfIdx);
+    }
     *surfIdx = SURF_BORDER;
+    if (GEO && U.border_surf >= 0) return border_distance(P, P.surfs[U.border_surf], r, u);
     return box_distance(P.boxes[U.border], r, u);
 }
 
@@ -380,12 +708,22 @@ __device__ __forceinline__ void uni_offset(const DevProblem& P, int uniIdx, int 
 // geometryStd
 // ---------------------------------------------------------------------------------------------
 
-// diveToMat from level `start` (0-based); levels are statically unrolled.
-template <int NL>
-__device__ __forceinline__ void dive(const DevProblem& P, Pt<NL>& p, int start, DevStatus* st)
+// direction of level l from the one above it (coord_class.f90 rotate/assignDirection: a rotated
+// universe applies its rotation matrix, every other level shares the direction above)
+__device__ __forceinline__ void level_dir(const DevProblem& P, int uniIdx, double u[3])
 {
+    const scn_universe& U = P.unis[uniIdx - 1];
+    if (U.rotated) rot_apply(U.rot, u);
+}
+
+// diveToMat from level `start` (0-based); levels are statically unrolled.
+template <int NL, int GEO>
+__device__ __forceinline__ void dive(const DevProblem& P, Pt<NL, GEO>& p, int start, DevStatus* st)
+{
+    double u[3] = {p.u[0], p.u[1], p.u[2]};   // direction at level l
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
+        if (GEO && l > 0 && l <= start && P.any_rot) level_dir(P, p.uni[l], u);
         if (l < start) continue;
         int pos = p.root[l] + p.loc[l] - 1;
         int fill = P.gfill[pos - 1];
@@ -403,17 +741,26 @@ __device__ __forceinline__ void dive(const DevProblem& P, Pt<NL>& p, int start, 
             p.nest = l + 2;
             p.uni[l + 1] = child;
             p.root[l + 1] = id;
+            // universe%enter (universe_inter.f90:400-424): rotate, then translate; a 'global'
+            // universe is entered from the global position
+            double rr[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) p.r[l + 1][k] = (p.r[l][k] - off[k]) - Uc.origin[k];
-            p.loc[l + 1] = uni_findcell(P, child, p.r[l + 1], p.u);
+            for (int k = 0; k < 3; ++k) rr[k] = (GEO && Uc.global) ? p.r[0][k] : p.r[l][k] - off[k];
+            if (GEO && Uc.rotated) {
+                rot_apply(Uc.rot, rr);
+                rot_apply(Uc.rot, u);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) p.r[l + 1][k] = rr[k] - Uc.origin[k];
+            p.loc[l + 1] = uni_findcell<GEO>(P, child, p.r[l + 1], u);
         }
     }
     raise_error(st, SCN_ERR_ARG, 1);
     p.mat = SCN_UNDEF_MAT;
 }
 
-template <int NL>
-__device__ __forceinline__ void place_coord(const DevProblem& P, Pt<NL>& p, DevStatus* st)
+template <int NL, int GEO>
+__device__ __forceinline__ void place_coord(const DevProblem& P, Pt<NL, GEO>& p, DevStatus* st)
 {
     const scn_universe& R = P.unis[P.root_uni - 1];
     p.nest = 1;
@@ -423,13 +770,13 @@ __device__ __forceinline__ void place_coord(const DevProblem& P, Pt<NL>& p, DevS
     for (int k = 0; k < 3; ++k) p.r[0][k] = p.r[0][k] - R.origin[k];
     p.uni[0] = P.root_uni;
     p.root[0] = 1;
-    p.loc[0] = uni_findcell(P, P.root_uni, p.r[0], p.u);
+    p.loc[0] = uni_findcell<GEO>(P, P.root_uni, p.r[0], p.u);
     dive(P, p, 0, st);
 }
 
 // rebuild uni/root of every level from the stored local IDs (geomGraph walk)
-template <int NL>
-__device__ __forceinline__ void relink(const DevProblem& P, Pt<NL>& p)
+template <int NL, int GEO>
+__device__ __forceinline__ void relink(const DevProblem& P, Pt<NL, GEO>& p)
 {
     p.uni[0] = P.root_uni;
     p.root[0] = 1;
@@ -443,19 +790,23 @@ __device__ __forceinline__ void relink(const DevProblem& P, Pt<NL>& p)
     }
 }
 
-template <int NL>
-__device__ __forceinline__ void move_local(Pt<NL>& p, double d, int n)
+template <int NL, int GEO>
+__device__ __forceinline__ void move_local(const DevProblem& P, Pt<NL, GEO>& p, double d, int n)
 {
     p.nest = n;
+    double u[3] = {p.u[0], p.u[1], p.u[2]};
 #pragma unroll
-    for (int l = 0; l < NL; ++l)
-        if (l < n)
+    for (int l = 0; l < NL; ++l) {
+        if (l < n) {
+            if (GEO && l > 0 && P.any_rot) level_dir(P, p.uni[l], u);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) p.r[l][k] = p.r[l][k] + d * p.u[k];
+            for (int k = 0; k < 3; ++k) p.r[l][k] = p.r[l][k] + d * u[k];
+        }
+    }
 }
 
-template <int NL>
-__device__ __forceinline__ void move_global(Pt<NL>& p, double d)
+template <int NL, int GEO>
+__device__ __forceinline__ void move_global(Pt<NL, GEO>& p, double d)
 {
     p.nest = 1;
     p.mat = SCN_UNDEF_MAT;
@@ -472,25 +823,27 @@ struct DistCache {
 };
 
 // geometryStd%move_withCache (cache on) / move_noCache (cache off); returns the event
-template <int NL>
-__device__ __forceinline__ int move(const DevProblem& P, Pt<NL>& p, double& maxDist, DistCache<NL>& cache, DevStatus* st)
+template <int NL, int GEO>
+__device__ __forceinline__ int move(const DevProblem& P, Pt<NL, GEO>& p, double& maxDist, DistCache<NL>& cache, DevStatus* st)
 {
     double dist = SCN_INF;
     int surfIdx = 0, level = 0;
+    double ul[3] = {p.u[0], p.u[1], p.u[2]};   // direction at level l
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         if (l < p.nest) {
+            if (GEO && l > 0 && P.any_rot) level_dir(P, p.uni[l], ul);
             double td;
             int ts;
             if (P.st_cache) {
                 if (cache.lvl < l + 1) {
-                    cache.dist[l] = uni_distance(P, p.uni[l], p.loc[l], p.r[l], p.u, &cache.surf[l]);
+                    cache.dist[l] = uni_distance<GEO>(P, p.uni[l], p.loc[l], p.r[l], ul, &cache.surf[l], st);
                     cache.lvl += 1;
                 }
                 td = cache.dist[l];
                 ts = cache.surf[l];
             } else {
-                td = uni_distance(P, p.uni[l], p.loc[l], p.r[l], p.u, &ts);
+                td = uni_distance<GEO>(P, p.uni[l], p.loc[l], p.r[l], ul, &ts, st);
             }
             if ((dist - td) >= dist * SCN_FP_REL_TOL) {
                 dist = td;
@@ -500,18 +853,18 @@ __device__ __forceinline__ int move(const DevProblem& P, Pt<NL>& p, double& maxD
         }
     }
     if (maxDist < dist && maxDist < SCN_INF) {
-        move_local(p, maxDist, p.nest);
+        move_local(P, p, maxDist, p.nest);
         cache.lvl = 0;
         return EV_COLL;
     } else if (surfIdx == SURF_BORDER && level == 1) {
         move_global(p, dist);
         maxDist = dist;
         cache.lvl = 0;
-        box_explicit_bc(P.boxes[P.border_box], p.r[0], p.u);
+        if (P.border_box >= 0) box_explicit_bc(P.boxes[P.border_box], p.r[0], p.u);
         place_coord(P, p, st);
         return EV_BOUNDARY;
     }
-    move_local(p, dist, level);
+    move_local(P, p, dist, level);
     maxDist = dist;
 #pragma unroll
     for (int l = 0; l < NL; ++l)
@@ -523,18 +876,37 @@ __device__ __forceinline__ int move(const DevProblem& P, Pt<NL>& p, double& maxD
     {
         const int lv = level - 1;
         double rl[3] = {0.0, 0.0, 0.0};
+        double uc[3] = {p.u[0], p.u[1], p.u[2]};   // direction at the crossed level
         int ul = 0, ll = 0;
 #pragma unroll
         for (int l = 0; l < NL; ++l) {
             const uint64_t m = 0ull - (uint64_t)(l == lv);
             const uint32_t m32 = (uint32_t)m;
+            if (GEO && l > 0 && l <= lv && P.any_rot) level_dir(P, p.uni[l], uc);
 #pragma unroll
             for (int k = 0; k < 3; ++k) rl[k] = scn_bitsd((scn_dbits(rl[k]) & ~m) | (scn_dbits(p.r[l][k]) & m));
             ul = (int)(((uint32_t)ul & ~m32) | ((uint32_t)p.uni[l] & m32));
             ll = (int)(((uint32_t)ll & ~m32) | ((uint32_t)p.loc[l] & m32));
         }
         const scn_universe& U = P.unis[ul - 1];
-        int nl = (U.type == SCN_UNI_PIN) ? ll + ((surfIdx == PIN_IN) ? -1 : 1) : uni_findcell(P, ul, rl, p.u);
+        int nl;
+        if (U.type == SCN_UNI_PIN) {
+            nl = ll + ((surfIdx == PIN_IN) ? -1 : 1);
+        } else if (GEO && U.type == SCN_UNI_CELL) {
+            // cellUniverse_class.f90:388-436: NUDGE the local position across, then search
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rl[k] = rl[k] + uc[k] * SCN_NUDGE;
+            nl = uni_findcell<GEO>(P, ul, rl, uc);
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                const uint64_t m = 0ull - (uint64_t)(l == lv);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    p.r[l][k] = scn_bitsd((scn_dbits(p.r[l][k]) & ~m) | (scn_dbits(rl[k]) & m));
+            }
+        } else {
+            nl = uni_findcell<GEO>(P, ul, rl, uc);
+        }
 #pragma unroll
         for (int l = 0; l < NL; ++l) p.loc[l] = (l == lv) ? nl : p.loc[l];
     }
@@ -542,13 +914,13 @@ __device__ __forceinline__ int move(const DevProblem& P, Pt<NL>& p, double& maxD
     return EV_CROSS;
 }
 
-template <int NL>
-__device__ __forceinline__ void teleport(const DevProblem& P, Pt<NL>& p, double dist, DevStatus* st)
+template <int NL, int GEO>
+__device__ __forceinline__ void teleport(const DevProblem& P, Pt<NL, GEO>& p, double dist, DevStatus* st)
 {
     move_global(p, dist);
     place_coord(P, p, st);
     if (p.mat == SCN_OUTSIDE_MAT) {
-        box_transform_bc(P.boxes[P.border_box], p.r[0], p.u);
+        if (P.border_box >= 0) box_transform_bc(P.boxes[P.border_box], p.r[0], p.u);
         place_coord(P, p, st);
     }
 }
@@ -649,8 +1021,8 @@ __device__ __forceinline__ void bin_add(Bins& B, int64_t addr, double v)
 }
 
 // tallyAdmin%reportInColl: keffImplicitClerk first, then the collision clerks
-template <int NL>
-__device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL>& p, int isVirtual, Bins& B, DevStatus* st)
+template <int NL, int GEO>
+__device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& p, int isVirtual, Bins& B, DevStatus* st)
 {
     int nsc = 0;
     int mat = p.mat;
@@ -700,8 +1072,8 @@ enum { STEP_DEAD = 0, STEP_COLLIDE = 1, STEP_MORE = 2 };
 // one segment of surfaceTracking (transportOperatorST_class.f90:59-160): to the nearest of the
 // sampled collision point and the closest surface; `cache` is the flight's distCache (fresh at the
 // start of every flight)
-template <int NL>
-__device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache<NL>& cache, Bins& B,
+template <int NL, int GEO>
+__device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, DistCache<NL>& cache, Bins& B,
                                        DevStatus* st, Cnt& cnt)
 {
     double sigmaTrack, sigmaT, dist, invSigmaTrack;
@@ -748,8 +1120,8 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL>& p, DistCache
 // one tentative flight of deltaTracking (transportOperatorDT_class.f90:47-130).  The majorant
 // and its inverse depend on the group only, so evaluating them per tentative flight gives the
 // values the flight computes once.
-template <int NL>
-__device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
+template <int NL, int GEO>
+__device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL, GEO>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
     p.track = maj;
@@ -785,8 +1157,8 @@ __device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL>& p, Bins& B, 
 
 // transportOperatorHT tracking_selection (transportOperatorHT_class.f90:41-64): delta tracking for
 // this flight when sigmaT * (1 / majorant) > 1 - cutoff (sigmaT = 0 in void)
-template <int NL>
-__device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL>& p, DevStatus* st)
+template <int NL, int GEO>
+__device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL, GEO>& p, DevStatus* st)
 {
     double majorant_inv = 1.0 / scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
     double sigmaT = (p.mat == SCN_VOID_MAT) ? 0.0 : track_mat_xs(P, p.mat, p.G, st);
@@ -795,8 +1167,8 @@ __device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL>& p, D
 }
 
 // surfaceTracking: segments until a real collision or a leak; returns 1 if alive
-template <int NL>
-__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
+template <int NL, int GEO>
+__device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL, GEO>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     DistCache<NL> cache;
     cache.lvl = 0;
@@ -812,8 +1184,8 @@ __device__ __forceinline__ int flight_st(const DevProblem& P, Pt<NL>& p, Bins& B
 }
 
 // deltaTracking: tentative flights until a real collision or a leak; returns 1 if alive
-template <int NL>
-__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL>& p, Bins& B, DevStatus* st, Cnt& cnt)
+template <int NL, int GEO>
+__device__ __forceinline__ int flight_dt(const DevProblem& P, Pt<NL, GEO>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
     for (int seg = 0;; ++seg) {
         if (seg >= MAX_SEGMENTS_PER_FLIGHT) {
@@ -901,8 +1273,8 @@ __device__ __forceinline__ int legendre_p1(double c, uint64_t* rng, double* mu)
 }
 
 // collisionProcessor%collide with neutronMGstd; returns 1 if the particle survives
-template <int NL>
-__device__ __forceinline__ int collide(const DevProblem& P, Pt<NL>& p, double k_eff, Bins& B, const Sites& S,
+template <int NL, int GEO>
+__device__ __forceinline__ int collide(const DevProblem& P, Pt<NL, GEO>& p, double k_eff, Bins& B, const Sites& S,
                               DevStatus* st, Cnt& cnt)
 {
     const int nG = P.nG;

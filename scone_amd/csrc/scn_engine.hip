@@ -62,8 +62,8 @@ struct Counters {
     unsigned long long pad;
 };
 
-template <int NL>
-__device__ inline void load_live(const Live& L, int cap, int i, Pt<NL>& p)
+template <int NL, int GEO>
+__device__ inline void load_live(const Live& L, int cap, int i, Pt<NL, GEO>& p)
 {
     p.nest = L.nest[i];
 #pragma unroll
@@ -99,8 +99,8 @@ __device__ inline void load_live(const Live& L, int cap, int i, Pt<NL>& p)
     p.track = 0.0;
 }
 
-template <int NL>
-__device__ inline void store_live(const Live& L, int cap, int i, const Pt<NL>& p)
+template <int NL, int GEO>
+__device__ inline void store_live(const Live& L, int cap, int i, const Pt<NL, GEO>& p)
 {
     L.nest[i] = p.nest;
 #pragma unroll
@@ -141,7 +141,7 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v)
 // step limit, has run max_steps (flight, collision) pairs and is parked in the output queue --
 // takes the next particle of the chunk at once.  Lanes therefore stay busy until the input is
 // used up, whatever the spread of history lengths.
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 __global__ void __launch_bounds__(BLOCK)
 k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first, int n_first, Live in,
             const int* __restrict__ n_in_ptr, Live out, int* __restrict__ n_out, int cap, int max_steps,
@@ -165,7 +165,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     int next = 0, end = 0, exhausted = 0;
-    Pt<NL> p;
+    Pt<NL, GEO> p;
     DistCache<NL> cache;
     cache.lvl = 0;
     int alive = 0, steps = 0, nseg = 0;
@@ -450,7 +450,7 @@ __global__ void k_norm_dup(const scn_particle_state* in, const int* dflag, const
 }
 
 // ---- initial fission source (fissionSource_class.f90:149-271) ---------------------------------
-template <int NL>
+template <int NL, int GEO>
 __global__ void k_fission_source(DevProblem P, int n, uint64_t master, double b0, double b1, double b2, double t0,
                                  double t1, double t2, int attempts, scn_particle_state* bank, DevStatus* st)
 {
@@ -466,7 +466,7 @@ __global__ void k_fission_source(DevProblem P, int n, uint64_t master, double b0
             raise_error(st, SCN_ERR_SAMPLING, 3);
             break;
         }
-        Pt<NL> p;
+        Pt<NL, GEO> p;
         double r3[3];
         for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&rng);
         for (int k = 0; k < 3; ++k) {
@@ -543,6 +543,7 @@ struct Engine {
     EigenState eig;
     hipStream_t stream = nullptr;
     int nl = 1;             // compiled nesting depth
+    int geo = 0;            // general-geometry kernels (cells, rotations, non-box border)
     int transport = SCN_TRANSPORT_ST;
     DevProblem P{};
     int64_t n_bins = 0;
@@ -742,30 +743,39 @@ typedef void (*TransportFn)(dim3, dim3, size_t, hipStream_t, const DevProblem&, 
                             Live, const int*, Live, int*, int, int, uint64_t, double, Sites, double*, int, double*,
                             int, Counters*, DevStatus*, int*, int);
 
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 void launch_transport(dim3 g, dim3 b, size_t sh, hipStream_t s, const DevProblem& P, const scn_particle_state* bank,
                       int first, int n_first, Live in, const int* n_in_ptr, Live out, int* n_out, int cap,
                       int max_steps, uint64_t master, double k_eff, Sites S, double* bins, int use_lds,
                       double* hist_k, int hist_cap, Counters* cnt, DevStatus* st, int* work, int chunk)
 {
-    hipLaunchKernelGGL((k_transport<NL, MODE>), g, b, sh, s, P, bank, first, n_first, in, n_in_ptr, out, n_out, cap,
+    hipLaunchKernelGGL((k_transport<NL, MODE, GEO>), g, b, sh, s, P, bank, first, n_first, in, n_in_ptr, out, n_out, cap,
                        max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st, work, chunk);
 }
 
-template <int NL, int MODE>
+template <int NL, int MODE, int GEO>
 const void* transport_kernel()
 {
-    return (const void*)k_transport<NL, MODE>;
+    return (const void*)k_transport<NL, MODE, GEO>;
 }
 
+// kernel variant: register depth NL and, for cell / rotated geometries, the general kernels
 template <int MODE>
-TransportFn pick_nl(int nl, const void** kern)
+TransportFn pick_nl(int nl, int geo, const void** kern)
 {
+    if (geo) {
+        if (nl <= 3) {
+            *kern = transport_kernel<3, MODE, 1>();
+            return launch_transport<3, MODE, 1>;
+        }
+        *kern = transport_kernel<8, MODE, 1>();
+        return launch_transport<8, MODE, 1>;
+    }
     switch (nl) {
-    case 1: *kern = transport_kernel<1, MODE>(); return launch_transport<1, MODE>;
-    case 3: *kern = transport_kernel<3, MODE>(); return launch_transport<3, MODE>;
-    case 4: *kern = transport_kernel<4, MODE>(); return launch_transport<4, MODE>;
-    default: *kern = transport_kernel<8, MODE>(); return launch_transport<8, MODE>;
+    case 1: *kern = transport_kernel<1, MODE, 0>(); return launch_transport<1, MODE, 0>;
+    case 3: *kern = transport_kernel<3, MODE, 0>(); return launch_transport<3, MODE, 0>;
+    case 4: *kern = transport_kernel<4, MODE, 0>(); return launch_transport<4, MODE, 0>;
+    default: *kern = transport_kernel<8, MODE, 0>(); return launch_transport<8, MODE, 0>;
     }
 }
 
@@ -780,9 +790,9 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     HIPCHK(hipMemsetAsync(E->counters_i, 0, sizeof(int) * 4, E->stream));
     HIPCHK(hipMemsetAsync(E->sites.count, 0, sizeof(int), E->stream));
     const void* kern = nullptr;
-    TransportFn fn = (E->transport == SCN_TRANSPORT_DT)   ? pick_nl<SCN_TRANSPORT_DT>(E->nl, &kern)
-                     : (E->transport == SCN_TRANSPORT_HT) ? pick_nl<SCN_TRANSPORT_HT>(E->nl, &kern)
-                                                          : pick_nl<SCN_TRANSPORT_ST>(E->nl, &kern);
+    TransportFn fn = (E->transport == SCN_TRANSPORT_DT)   ? pick_nl<SCN_TRANSPORT_DT>(E->nl, E->geo, &kern)
+                     : (E->transport == SCN_TRANSPORT_HT) ? pick_nl<SCN_TRANSPORT_HT>(E->nl, E->geo, &kern)
+                                                          : pick_nl<SCN_TRANSPORT_ST>(E->nl, E->geo, &kern);
     double* bins = score_bins ? E->bins : nullptr;
     // a null BIN pointer means "do not score": point at a scratch copy instead
     double* bins_arg = bins ? bins : E->bins;
@@ -1007,6 +1017,11 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     E->n_bins = pr->n_bins;
     E->norm_bin = pr->norm_bin;
     E->norm_value = pr->norm_value;
+    if (const char* m = check_cell_tables(pr)) {
+        g_create_err = m;
+        delete E;
+        return SCN_ERR_ARG;
+    }
     {
         int d = problem_depth(pr);
         if (d < 1 || d > SCN_MAX_NEST) {
@@ -1014,7 +1029,8 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
             delete E;
             return SCN_ERR_UNSUPPORTED;
         }
-        E->nl = compiled_nl(d);
+        E->geo = needs_full_geometry(pr);
+        E->nl = compiled_nl(d, E->geo);
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) E->n_cu = prop.multiProcessorCount;
@@ -1029,7 +1045,9 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     // clerks admitted per group (energyFilter evaluated once, host side)
     std::vector<int> goff, gidx;
     group_clerks(pr, goff, gidx);
+    scn_surface* d_surfs = nullptr;
     if (!rc) rc |= upload(E, &d_unis, pr->universes, pr->n_universes);
+    if (!rc) rc |= upload(E, &d_surfs, pr->surfaces, (size_t)std::max(1, pr->n_surfaces));
     if (!rc) rc |= upload(E, &d_boxes, pr->boxes, pr->n_boxes);
     if (!rc) rc |= upload(E, &d_gf, pr->graph_fill, pr->graph_size);
     if (!rc) rc |= upload(E, &d_gi, pr->graph_id, pr->graph_size);
@@ -1071,6 +1089,8 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.gclerk_idx = d_gidx;
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
+    P.surfs = d_surfs;
+    P.any_rot = any_rotated(pr);
     if (pr->mg.P1) {
         double* d_P1 = nullptr;
         if (!rc) rc |= upload(E, &d_P1, pr->mg.P1, (size_t)nM * nG * nG);
@@ -1236,23 +1256,21 @@ int scn_fission_source(void* engine, int32_t n, int64_t rng_state, const double 
     if (!E || n < 0 || n > E->bank_cap || !bank) return SCN_ERR_ARG;
     (void)g_in;
     const unsigned g = (unsigned)((n + 255) / 256);
-    switch (E->nl) {
-    case 1:
-        hipLaunchKernelGGL(k_fission_source<1>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
-                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
-        break;
-    case 3:
-        hipLaunchKernelGGL(k_fission_source<3>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
-                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
-        break;
-    case 4:
-        hipLaunchKernelGGL(k_fission_source<4>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
-                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
-        break;
-    default:
-        hipLaunchKernelGGL(k_fission_source<8>, dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state,
-                           bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status);
+#define SCN_FS(NLV, GEOV)                                                                                    \
+    hipLaunchKernelGGL((k_fission_source<NLV, GEOV>), dim3(g), dim3(256), 0, E->stream, E->P, n, (uint64_t)rng_state, \
+                       bottom[0], bottom[1], bottom[2], top[0], top[1], top[2], attempts, E->bankA, E->status)
+    if (E->geo) {
+        if (E->nl <= 3) SCN_FS(3, 1);
+        else SCN_FS(8, 1);
+    } else {
+        switch (E->nl) {
+        case 1: SCN_FS(1, 0); break;
+        case 3: SCN_FS(3, 0); break;
+        case 4: SCN_FS(4, 0); break;
+        default: SCN_FS(8, 0);
+        }
     }
+#undef SCN_FS
     HIPCHK(hipGetLastError());
     int rc = check_status(E);
     if (rc) return rc;

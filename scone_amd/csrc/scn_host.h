@@ -36,9 +36,68 @@ inline int problem_depth(const scn_problem* pr)
     return dep(pr->root_uni, 1);
 }
 
-// compiled register-array depth for a geometry depth
-inline int compiled_nl(int depth)
+// some universe applies a rotation matrix on enter
+inline int any_rotated(const scn_problem* pr)
 {
+    for (int i = 0; i < pr->n_universes; ++i)
+        if (pr->universes[i].rotated) return 1;
+    return 0;
+}
+
+// index checks of the surface and cell tables, so that kernels read only inside them;
+// returns an error message or nullptr
+inline const char* check_cell_tables(const scn_problem* pr)
+{
+    for (int i = 0; i < pr->n_surfaces; ++i) {
+        const scn_surface& S = pr->surfaces[i];
+        if (S.type < SCN_SURF_APLANE || S.type > SCN_SURF_TRUNC_CYLINDER) return "unknown surface type";
+        if (S.type == SCN_SURF_BOX) {
+            if (S.axis < 0 || S.axis >= pr->n_boxes) return "box surface refers to a missing box";
+        } else if (S.type != SCN_SURF_PLANE && S.type != SCN_SURF_QUADRIC && S.type != SCN_SURF_SPHERE) {
+            if (S.axis < 0 || S.axis > 2) return "surface axis outside 0..2";
+        }
+    }
+    if (pr->n_surfaces > 0 && !pr->surfaces) return "surface table missing";
+    for (int u = 0; u < pr->n_universes; ++u) {
+        const scn_universe& U = pr->universes[u];
+        if (U.type < SCN_UNI_ROOT || U.type > SCN_UNI_CELL) return "unknown universe type";
+        if (U.type == SCN_UNI_ROOT) {
+            if (U.border_surf >= pr->n_surfaces) return "root border surface outside the surface table";
+            if (U.border_surf < 0 && (U.border < 0 || U.border >= pr->n_boxes)) return "root border box missing";
+        }
+        if (U.type != SCN_UNI_CELL) continue;
+        if (U.n_cells < 0 || U.n_local != U.n_cells + 2) return "cell universe local cell count";
+        if (U.cells < 0 || (int64_t)U.cells + U.n_cells > pr->n_int) return "cell universe list outside int_data";
+        for (int c = 0; c < U.n_cells; ++c) {
+            const int rec = pr->int_data[U.cells + c];
+            if (rec < 0 || rec >= pr->n_int) return "cell record outside int_data";
+            const int n = pr->int_data[rec];
+            if (n < 0 || (int64_t)rec + 1 + n > pr->n_int) return "cell record outside int_data";
+            for (int k = 1; k <= n; ++k) {
+                const int s = pr->int_data[rec + k];
+                if (s == 0 || s > pr->n_surfaces || -s > pr->n_surfaces) return "cell surface outside the surface table";
+            }
+        }
+    }
+    return nullptr;
+}
+
+// the geometry needs the general kernels: cell universes, rotated or 'global' universes, or a
+// root border that is not a box
+inline int needs_full_geometry(const scn_problem* pr)
+{
+    for (int i = 0; i < pr->n_universes; ++i) {
+        const scn_universe& U = pr->universes[i];
+        if (U.type == SCN_UNI_CELL || U.rotated || U.global) return 1;
+        if (U.type == SCN_UNI_ROOT && U.border_surf >= 0) return 1;
+    }
+    return 0;
+}
+
+// compiled register-array depth for a geometry depth
+inline int compiled_nl(int depth, int geo = 0)
+{
+    if (geo) return depth <= 3 ? 3 : 8;   // general-geometry kernels exist for 3 and 8 levels
     if (depth <= 1) return 1;
     if (depth <= 3) return 3;
     if (depth <= 4) return 4;

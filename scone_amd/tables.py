@@ -33,7 +33,9 @@ INF = 2.0 ** 63
 SURF_TOL = 1.0e-12
 FP_REL_TOL = 1.0e-7
 TRANSPORT_ST, TRANSPORT_DT, TRANSPORT_HT = 1, 2, 3
-UNI_ROOT, UNI_LAT, UNI_PIN = 1, 2, 3
+UNI_ROOT, UNI_LAT, UNI_PIN, UNI_CELL = 1, 2, 3, 4
+SURF_APLANE, SURF_PLANE, SURF_SPHERE, SURF_CYLINDER, SURF_QUADRIC, SURF_BOX, SURF_TRUNC_CYLINDER = 1, 2, 3, 4, 5, 6, 7
+OVERLAP_MAT = 2147483645
 MAP_SPACE, MAP_MATERIAL = 1, 2
 GRID_LIN, GRID_UNSTRUCT = 1, 3
 KAPPA_DEFAULT = float(np.float32(202.27))   # fissionMG_class.f90:64 single-precision literal
@@ -67,7 +69,13 @@ class Universe(C.Structure):
                 ("border", C.c_int32), ("size", C.c_int32 * 3), ("out_local", C.c_int32),
                 ("offset_mode", C.c_int32), ("offset_map", C.c_int32), ("pitch", C.c_double * 3),
                 ("corner", C.c_double * 3), ("a_bar", C.c_double * 3), ("outline_hw", C.c_double * 3),
-                ("n_annuli", C.c_int32), ("annuli", C.c_int32)]
+                ("n_annuli", C.c_int32), ("annuli", C.c_int32), ("n_cells", C.c_int32), ("cells", C.c_int32),
+                ("border_surf", C.c_int32), ("rotated", C.c_int32), ("global_", C.c_int32), ("check_overlap", C.c_int32),
+                ("rot", C.c_double * 9)]
+
+
+class Surface(C.Structure):
+    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("tol", C.c_double), ("c", C.c_double * 10)]
 
 
 class MGData(C.Structure):
@@ -97,7 +105,8 @@ class Problem(C.Structure):
                 ("n_real", C.c_int32), ("real_data", C.POINTER(C.c_double)), ("n_int", C.c_int32),
                 ("int_data", C.POINTER(C.c_int32)), ("mg", MGData), ("n_clerks", C.c_int32),
                 ("clerks", C.POINTER(Clerk)), ("n_bins", C.c_int64), ("norm_bin", C.c_int64),
-                ("norm_value", C.c_double), ("ht_cutoff", C.c_double)]
+                ("norm_value", C.c_double), ("ht_cutoff", C.c_double), ("n_surfaces", C.c_int32),
+                ("surfaces", C.POINTER(Surface))]
 
 
 class EigenSettings(C.Structure):
@@ -241,7 +250,7 @@ class Compiled:
         return arr
 
 
-def _surface_box(sd: dict, bc):
+def _surface_box(sd: dict, bc, raw_bounds=False):
     t = sd["type"]
     b = Box()
     b.bc[:] = [int(x) for x in bc] if bc is not None else [0] * 6
@@ -269,11 +278,147 @@ def _surface_box(sd: dict, bc):
         bounds[3 + axis] = INF
     else:
         raise NotImplementedError(f"border surface type {t} is outside the engine's scope")
-    for i in range(3):  # geometryStd%bounds: infinite axes become (0, 0)
-        if bounds[i] <= -INF and bounds[i + 3] >= INF:
-            bounds[i] = 0.0
-            bounds[i + 3] = 0.0
-    return b, bounds
+    if raw_bounds:
+        return b, bounds
+    return b, _geometry_bounds(bounds)
+
+
+def _bool(v, key):
+    """dictionary%get of a logical: the integer 1 or 0 (dictionary_class.f90 getBool)."""
+    if isinstance(v, bool):
+        return v
+    if isinstance(v, (int, np.integer)) and int(v) in (0, 1):
+        return bool(v)
+    raise ValueError(f"Keyword {key} must be 1 or 0")
+
+
+def fortran_norm2(v):
+    """Fortran NORM2 of a short vector, evaluated by the scaled sum of squares gfortran uses."""
+    result, scale = 0.0, 1.0
+    for x in v:
+        if x != 0.0:
+            ax = abs(x)
+            if scale < ax:
+                val = scale / ax
+                result = 1.0 + result * val * val
+                scale = ax
+            else:
+                val = ax / scale
+                result = result + val * val
+    return scale * math.sqrt(result)
+
+
+def rotation_matrix(phi, theta, psi):
+    """genericProcedures.f90:1098-1140 rotationMatrix (Euler angles in degrees), row-major."""
+    if phi < 0.0 or phi >= 360.0:
+        raise ValueError(f"Angle phi must be in <0;360). Is: {phi}")
+    if theta < 0.0 or theta > 180.0:
+        raise ValueError(f"Angle theta must be in <0;180>. Is: {theta}")
+    if psi < 0.0 or psi >= 360.0:
+        raise ValueError(f"Angle psi must be in <0;360). Is: {psi}")
+    conv = (2.0 * (4.0 * math.atan(1.0))) / 360.0
+    sp, cp = math.sin(phi * conv), math.cos(phi * conv)
+    st, ct = math.sin(theta * conv), math.cos(theta * conv)
+    ss, cs = math.sin(psi * conv), math.cos(psi * conv)
+    return [cs * cp - ct * sp * ss, cs * sp + ct * cp * ss, ss * sp,
+            -ss * cp - ct * sp * cs, -ss * sp + ct * cp * cs, cs * st,
+            st * sp, -st * cp, ct]
+
+
+_AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+def compile_surface(sd: dict, boxes: list | None = None):
+    """One surface dictionary (Geometry/Surfaces: the type's init) -> (Surface, bounding box).
+    Box and squareCylinder surfaces append their Box to `boxes` and refer to it by index."""
+    t = sd["type"]
+    S = Surface()
+    S.tol = SURF_TOL
+    if int(sd["id"]) < 1:
+        raise ValueError("Invalid surface id provided. ID must be > 1")
+    if t in ("xPlane", "yPlane", "zPlane"):                       # aPlane_class.f90 init
+        S.type = SURF_APLANE
+        S.axis = _AXIS[t[0]]
+        S.c[0] = float(sd[t[0] + "0"])
+        bb = [-INF] * 3 + [INF] * 3
+        bb[S.axis] = bb[3 + S.axis] = S.c[0]
+    elif t == "plane":                                            # plane_class.f90 init
+        cf = [float(x) for x in sd["coeffs"]]
+        if len(cf) != 4:
+            raise ValueError(f"4 plane coefficients must be given. There are: {len(cf)}")
+        if all(x == 0.0 for x in cf[:3]):
+            raise ValueError("Invalid plane normal. coefficients 1-3 are all 0.0.")
+        n = fortran_norm2(cf[:3])
+        S.type = SURF_PLANE
+        S.c[:4] = [x / n for x in cf]
+        bb = [-INF] * 3 + [INF] * 3
+    elif t == "sphere":                                           # sphere_class.f90 init
+        o = [float(x) for x in sd["origin"]]
+        r = float(sd["radius"])
+        if len(o) != 3:
+            raise ValueError(f"Origin needs to have size 3. Has: {len(o)}")
+        if r <= 0.0:
+            raise ValueError(f"Radius of sphere must be +ve. Is: {r}")
+        S.type = SURF_SPHERE
+        S.c[:5] = o + [r, r * r]
+        S.tol = 2.0 * r * SURF_TOL
+        bb = [o[i] - r for i in range(3)] + [o[i] + r for i in range(3)]
+    elif t in ("xCylinder", "yCylinder", "zCylinder"):             # cylinder_class.f90 init/build
+        o = [float(x) for x in sd["origin"]]
+        r = float(sd["radius"])
+        if len(o) != 3:
+            raise ValueError(f"Origin needs to have size 3. Has: {len(o)}")
+        if r <= 0.0:
+            raise ValueError(f"Radius of cylinder must be +ve. Is: {r}")
+        S.type = SURF_CYLINDER
+        S.axis = _AXIS[t[0]]
+        S.c[:5] = o + [r, r * r]
+        S.tol = 2.0 * r * SURF_TOL
+        bb = [o[i] - r for i in range(3)] + [o[i] + r for i in range(3)]
+        bb[S.axis], bb[3 + S.axis] = -INF, INF
+    elif t == "quadric":                                          # quadric_class.f90 init
+        cf = [float(x) for x in sd["coeffs"]]
+        if len(cf) != 10:
+            raise ValueError(f"coeffs needs to have size 10. Has: {len(cf)}")
+        S.type = SURF_QUADRIC
+        S.c[:] = cf
+        S.tol = 2.0 * max(abs(x) for x in cf) * SURF_TOL
+        bb = [-INF] * 3 + [INF] * 3
+    elif t in ("xTruncCylinder", "yTruncCylinder", "zTruncCylinder"):   # truncCylinder_class.f90 init
+        o = [float(x) for x in sd["origin"]]
+        r = float(sd["radius"])
+        a = float(sd["halfwidth"])
+        if len(o) != 3:
+            raise ValueError(f"origin must have size 3. Has: {len(o)}")
+        if r <= 0.0:
+            raise ValueError(f"Radius must be +ve. Is: {r}")
+        if a <= 0.0:
+            raise ValueError(f"Halfwidth must be +ve. Is: {a}")
+        S.type = SURF_TRUNC_CYLINDER
+        S.axis = _AXIS[t[0]]
+        S.c[:5] = o + [r, a]
+        bb = [o[i] - r for i in range(3)] + [o[i] + r for i in range(3)]
+        bb[S.axis], bb[3 + S.axis] = o[S.axis] - a, o[S.axis] + a
+    elif t in ("box", "xSquareCylinder", "ySquareCylinder", "zSquareCylinder"):
+        if boxes is None:
+            raise ValueError("a box surface needs a box table")
+        b, bb = _surface_box(sd, None, raw_bounds=True)
+        S.type = SURF_BOX
+        S.axis = len(boxes)
+        boxes.append(b)
+    else:
+        raise NotImplementedError(f"surface type {t} is outside the engine's scope")
+    return S, bb
+
+
+def _geometry_bounds(bb):
+    """geometryStd%bounds: axes that are infinite on both sides become (0, 0)."""
+    bb = list(bb)
+    for i in range(3):
+        if bb[i] <= -INF and bb[i + 3] >= INF:
+            bb[i] = 0.0
+            bb[i + 3] = 0.0
+    return bb
 
 
 def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = ".",
@@ -345,8 +490,42 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     int_data = []
     universes = []
     boxes = []
+    surfaces = []            # Surface entries, in order of first reference
+    surf_index = {}          # surface ID -> 0-based index in surfaces
+    cells_d = gd.get("cells", {})
+    cell_by_id = {}
+    for name, cd in cells_d.items():
+        cid = int(cd["id"])
+        if cid <= 0:
+            raise ValueError(f"Cell ID must be >0. Is {cid}")
+        if cid in cell_by_id:
+            raise ValueError(f"Cells {name} & {cell_by_id[cid][0]} have the same ID: {cid}")
+        cell_by_id[cid] = (name, cd)
     fills = {}
     root_uid = None
+
+    def surf_ref(sid):
+        if sid not in surf_index:
+            if sid not in surf_by_id:
+                raise ValueError(f"Surface ID {sid} is not defined")
+            S, _ = compile_surface(surf_by_id[sid][1], boxes)
+            surf_index[sid] = len(surfaces)
+            surfaces.append(S)
+        return surf_index[sid]
+
+    def cell_fill(cd):
+        # cellShelf_class.f90 init: filltype outside / mat / uni
+        ft = cd["filltype"]
+        if ft == "outside":
+            return OUTSIDE_MAT
+        if ft == "mat":
+            return fill_of(cd["material"])
+        if ft == "uni":
+            u = int(cd["universe"])
+            if u <= 0:
+                raise ValueError(f"Universe ID must be +ve. Is :{u}")
+            return -u
+        raise ValueError(f"Unknown type of cell filling: {ft}")
 
     def fill_of(name):
         # universe_inter.f90 charToFill: 'u<id>' -> -id, material name -> matIdx
@@ -361,21 +540,71 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     for uid in order:
         ud = uni_by_id[uid]
         U = Universe()
+        U.border_surf = -1
+        # universe_inter.f90 setupBase: origin, rotation, global
         org = [float(x) for x in ud.get("origin", [0.0, 0.0, 0.0])]
-        if any(float(x) != 0.0 for x in ud.get("rotation", [0.0, 0.0, 0.0])):
-            raise NotImplementedError("rotated universes are outside the engine's scope")
+        if len(org) != 3:
+            raise ValueError(f"Origin must have size 3. Has: {len(org)}")
+        rot = [float(x) for x in ud.get("rotation", [0.0, 0.0, 0.0])]
+        if len(rot) != 3:
+            raise ValueError(f"3 rotation angles must be given. Has only: {len(rot)}")
+        if any(x != 0.0 for x in rot):
+            U.rotated = 1
+            U.rot[:] = rotation_matrix(*rot)
+        U.global_ = int(_bool(ud.get("global", 0), "global"))
         U.origin[:] = org
         t = ud["type"]
         if t == "rootUniverse":
+            if U.rotated:
+                raise NotImplementedError("a rotated root universe is outside the engine's scope")
             root_uid = uid
             U.type = UNI_ROOT
             sid = int(ud["border"])
-            b, bounds = _surface_box(surf_by_id[sid][1], bc)
+            if sid not in surf_by_id:
+                raise ValueError(f"Surface ID {sid} is not defined")
+            bsd = surf_by_id[sid][1]
+            if bsd["type"] in ("box", "xSquareCylinder", "ySquareCylinder", "zSquareCylinder"):
+                b, bounds = _surface_box(bsd, bc)
+                U.border = len(boxes)
+                boxes.append(b)
+            else:
+                # surface_inter.f90 setBC: other surfaces take vacuum boundaries only
+                if int(bc[0]) != 0:
+                    raise ValueError(f"{bsd['type']} supports only VACUUM BCs. Was given: {int(bc[0])}")
+                U.border = -1
+                U.border_surf = surf_ref(sid)
+                bounds = _geometry_bounds(compile_surface(bsd, [])[1])
             out.bounds = bounds
-            U.border = len(boxes)
-            boxes.append(b)
             U.n_local = 2
             fills[uid] = [fill_of(ud["fill"]), OUTSIDE_MAT]
+        elif t == "cellUniverse":
+            # cellUniverse_class.f90 init: local IDs in the order of 'cells', then UNDEF, OVERLAP
+            U.type = UNI_CELL
+            ids = [int(x) for x in ud["cells"]]
+            U.n_cells = len(ids)
+            U.n_local = len(ids) + 2
+            U.check_overlap = int(_bool(ud.get("checkOverlap", 0), "checkOverlap"))
+            U.cells = len(int_data)
+            int_data.extend([0] * len(ids))
+            f = []
+            for i, cid in enumerate(ids):
+                if cid not in cell_by_id:
+                    raise ValueError(f"Cell ID {cid} is not defined")
+                cname, cd = cell_by_id[cid]
+                if cd["type"] != "simpleCell":
+                    raise NotImplementedError(f"cell type {cd['type']} is outside the engine's scope")
+                sids = [int(x) for x in cd["surfaces"]]
+                if 0 in sids:
+                    raise ValueError(f"Surface ID 0 in cell {cid}")
+                if len({abs(x) for x in sids}) != len(sids):
+                    raise ValueError(f"There are repeated surfaces in definition of cell: {cid}")
+                int_data[U.cells + i] = len(int_data)
+                int_data.append(len(sids))
+                for sid in sids:
+                    k = surf_ref(abs(sid)) + 1
+                    int_data.append(k if sid > 0 else -k)
+                f.append(cell_fill(cd))
+            fills[uid] = f + [UNDEF_MAT, OVERLAP_MAT]
         elif t == "latUniverse":
             U.type = UNI_LAT
             pitch = [float(x) for x in ud["pitch"]]
@@ -666,6 +895,11 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     P.universes = C.cast(uni_arr, C.POINTER(Universe))
     P.n_boxes = len(boxes)
     P.border_box = universes[root_idx - 1].border
+    if surfaces:
+        surf_arr = (Surface * len(surfaces))(*surfaces)
+        out.hold(surf_arr)
+        P.n_surfaces = len(surfaces)
+        P.surfaces = C.cast(surf_arr, C.POINTER(Surface))
     P.boxes = C.cast(box_arr, C.POINTER(Box))
     P.graph_size = len(arr_fill)
     P.graph_fill = _ptr(gf, C.c_int32)

@@ -38,12 +38,13 @@ def dummy_mg(nG=1):
             "P0": [0.5 / nG] * (nG * nG)}
 
 
-def geometry_problem(universes, surfaces, boundary=(0, 0, 0, 0, 0, 0), materials=("m1",), graph="shrunk"):
+def geometry_problem(universes, surfaces, boundary=(0, 0, 0, 0, 0, 0), materials=("m1",), graph="shrunk",
+                     cells=None):
     """Compile a geometry-only SCONE input (1-group dummy materials) for geometry unit tests."""
     inp = {
         "transportOperator": {"type": "transportOperatorST"},
         "geometry": {"type": "geometryStd", "boundary": list(boundary), "graph": {"type": graph},
-                     "surfaces": surfaces, "cells": {}, "universes": universes},
+                     "surfaces": surfaces, "cells": cells or {}, "universes": universes},
         "nuclearData": {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
                         "materials": {m: {"xsFile": m} for m in materials}},
         "activeTally": {},

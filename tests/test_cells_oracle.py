@@ -1,0 +1,449 @@
+"""Surfaces, simpleCell and cellUniverse of the oracle, pinned to the reference's unit tests (CPU).
+
+Known answers transcribed from Geometry/Surfaces/Tests/{aPlane,plane,sphere,cylinder,quadric,
+truncCylinder}_test.f90, Geometry/Cells/Tests/simpleCell_test.f90 and
+Geometry/Universes/Tests/cellUniverse_test.f90 (tolerances as in those tests), plus closed forms:
+rotationMatrix of genericProcedures.f90 against an independent Euler-angle construction, and the
+uncollided escape from a purely absorbing sphere.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import binding as O
+from helpers import geometry_problem
+from scone_amd import configs, tables as T
+
+INF = 2.0 ** 63
+SURF_TOL = 1e-12
+SQRT2 = math.sqrt(2.0)
+
+
+def unit(v):
+    v = np.asarray(v, dtype=float)
+    return list(v / np.linalg.norm(v))
+
+
+def surf(**sd):
+    sd.setdefault("id", 75)
+    S, bb = T.compile_surface(sd, [])
+    return S, bb
+
+
+AXES = [0, 1, 2]
+PLANE = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+
+
+# ---- aPlane_test.f90 ------------------------------------------------------------------------
+@pytest.mark.parametrize("ax", AXES)
+def test_aplane(ax):
+    name = "xyz"[ax]
+    S, bb = surf(type=name + "Plane", **{name + "0": 4.3})
+    ref = [-INF] * 3 + [INF] * 3
+    ref[ax] = ref[3 + ax] = 4.3
+    assert bb == ref
+    e = [0.0, 0.0, 0.0]
+    e[ax] = 1.0
+    r = [0.0, 0.0, 0.0]
+    r[ax] = 4.3
+    mv = lambda r, eps, u: [r[i] + eps * u[i] for i in range(3)]
+    assert O.surf_halfspace(S, r, e)
+    assert O.surf_halfspace(S, mv(r, -0.5 * SURF_TOL, e), e)
+    assert not O.surf_halfspace(S, mv(r, -1.0001 * SURF_TOL, e), e)
+    assert O.surf_halfspace(S, mv(r, 2.0, e), e)
+    assert not O.surf_halfspace(S, mv(r, -2.0, e), e)
+    u2 = list(np.roll(e, 1))
+    assert O.surf_halfspace(S, mv(r, 0.5 * SURF_TOL, e), u2)
+    assert not O.surf_halfspace(S, mv(r, -0.5 * SURF_TOL, e), u2)
+    # distance
+    r = [2.0, 2.0, 2.0]
+    r[ax] = 0.0
+    u = unit([1, 1, 1])
+    assert O.surf_distance(S, r, u) == pytest.approx(4.3 * math.sqrt(3.0), rel=1e-7)
+    r[ax] = 4.3
+    assert O.surf_distance(S, r, u) == INF
+    r[ax] = 4.3 - 0.5 * SURF_TOL
+    assert O.surf_distance(S, r, u) == INF
+    r[ax] = 5.0
+    assert O.surf_distance(S, r, u) == INF
+    assert O.surf_distance(S, r, [-x for x in u]) == pytest.approx(0.7 * math.sqrt(3.0), rel=1e-7)
+    r = [2.0, 2.0, 2.0]
+    r[ax] = 0.0
+    assert O.surf_distance(S, r, u2) == INF
+
+
+# ---- plane_test.f90 -------------------------------------------------------------------------
+def test_plane():
+    S, bb = surf(id=7, type="plane", coeffs=[1.0, 1.0, 1.0, 3.0])
+    assert bb == [-INF] * 3 + [INF] * 3
+    s3 = math.sqrt(3.0)
+    r = [1.0, 1.0, 1.0]
+    u = unit([1, 1, 1])
+    mv = lambda eps, d: [r[i] + eps * d[i] for i in range(3)]
+    assert O.surf_halfspace(S, r, u)
+    assert O.surf_halfspace(S, mv(-0.5 * SURF_TOL * s3, u), u)
+    assert not O.surf_halfspace(S, mv(-1.00001 * SURF_TOL * s3, u), u)
+    assert not O.surf_halfspace(S, mv(-1.0, u), u)
+    assert O.surf_halfspace(S, mv(1.0, u), u)
+    u2 = unit([-1, 0, 1])
+    assert O.surf_halfspace(S, mv(0.5 * SURF_TOL, u), u2)
+    assert not O.surf_halfspace(S, mv(-0.5 * SURF_TOL, u), u2)
+    r = [-1.0, 0.0, 0.0]
+    u = [1.0, 0.0, 0.0]
+    assert O.surf_distance(S, r, u) == pytest.approx(4.0, rel=1e-7)
+    assert O.surf_distance(S, [3.0, 0.0, 0.0], u) == INF
+    assert O.surf_distance(S, [3.0 - SURF_TOL, 0.0, 0.0], u) == INF
+    d_ref = s3 * 1.1 * SURF_TOL
+    assert O.surf_distance(S, [3.0 - d_ref, 0.0, 0.0], u) == pytest.approx(d_ref, rel=1e-2)
+    assert O.surf_distance(S, [4.0, 0.0, 0.0], u) == INF
+    assert O.surf_distance(S, [4.0, 0.0, 0.0], [-1.0, 0.0, 0.0]) == pytest.approx(1.0, rel=1e-7)
+    assert O.surf_distance(S, [-1.0, 0.0, 0.0], unit([-1, 0, 1])) == INF
+
+
+def test_plane_coefficients_normalised_by_norm2():
+    S, _ = surf(type="plane", coeffs=[3.0, 4.0, 12.0, 26.0])
+    assert list(S.c[:4]) == [3.0 / 13.0, 4.0 / 13.0, 12.0 / 13.0, 2.0]
+    with pytest.raises(ValueError, match="all 0.0"):
+        surf(type="plane", coeffs=[0.0, 0.0, 0.0, 1.0])
+
+
+# ---- sphere_test.f90 ------------------------------------------------------------------------
+def test_sphere():
+    S, bb = surf(id=7, type="sphere", origin=[1.0, 2.0, 1.0], radius=2.0)
+    assert bb == [-1.0, 0.0, -1.0, 3.0, 4.0, 3.0]
+    assert S.tol == 2.0 * 2.0 * SURF_TOL
+    r = [-1.0, 2.0, 1.0]
+    u = [1.0, 0.0, 0.0]
+    mv = lambda eps: [r[i] + eps * u[i] for i in range(3)]
+    assert not O.surf_halfspace(S, r, u)
+    assert not O.surf_halfspace(S, mv(-0.5 * SURF_TOL), u)
+    assert O.surf_halfspace(S, mv(-1.00001 * SURF_TOL), u)
+    assert O.surf_halfspace(S, mv(-1.0), u)
+    assert not O.surf_halfspace(S, mv(1.0), u)
+    assert O.surf_halfspace(S, r, [0.0, 1.0, 0.0])
+    r = [-2.0, 2.0, 1.0]
+    assert O.surf_distance(S, r, [1.0, 0.0, 0.0]) == pytest.approx(1.0, rel=1e-7)
+    ref = 1.5
+    u0 = (ref ** 2 + 3.0 ** 2 - 2.0 ** 2) / (2.0 * ref * 3.0)
+    u = unit([u0, math.sqrt(1.0 - u0 ** 2), 0.0])
+    assert O.surf_distance(S, r, u) == pytest.approx(ref, rel=1e-7)
+    r = [-1.0, 2.0, 1.0]
+    assert O.surf_distance(S, r, [0.0, 1.0, 0.0]) == INF
+    assert O.surf_distance(S, r, [1.0, 0.0, 0.0]) == 4.0
+    r2 = [-1.0 - 0.5 * SURF_TOL, 2.0, 1.0]
+    assert O.surf_distance(S, r2, [1.0, 0.0, 0.0]) == 4.0 + 0.5 * SURF_TOL
+    r = [-0.5, 2.0, 1.0]
+    assert O.surf_distance(S, r, [1.0, 0.0, 0.0]) == 3.5
+    assert O.surf_distance(S, r, [-1.0, 0.0, 0.0]) == 0.5
+
+
+# ---- cylinder_test.f90 ----------------------------------------------------------------------
+@pytest.mark.parametrize("a", AXES)
+def test_cylinder(a):
+    S, bb = surf(type="xyz"[a] + "Cylinder", origin=[2.0, 2.0, 2.0], radius=2.0)
+    ref = [0.0, 0.0, 0.0, 4.0, 4.0, 4.0]
+    ref[a], ref[3 + a] = -INF, INF
+    assert bb == ref
+    p1, p2 = PLANE[a]
+
+    def vec(va, v1, v2):
+        v = [0.0] * 3
+        v[a], v[p1], v[p2] = va, v1, v2
+        return v
+
+    r = [2.0] * 3
+    r[p1] = 0.0
+    u = unit(vec(1.0, 1.0, 0.0))
+    mv = lambda eps: [r[i] + eps * u[i] for i in range(3)]
+    assert not O.surf_halfspace(S, r, u)
+    assert not O.surf_halfspace(S, mv(-SURF_TOL), u)
+    assert O.surf_halfspace(S, mv(-SQRT2 * 1.00001 * SURF_TOL), u)
+    assert O.surf_halfspace(S, mv(-2.0), u)
+    assert not O.surf_halfspace(S, mv(2.0), u)
+    assert O.surf_halfspace(S, r, unit(vec(1.0, 0.0, 1.0)))
+    # distance
+    r = [2.0] * 3
+    r[p1] = -1.0
+    assert O.surf_distance(S, r, unit(vec(1.0, 1.0, 0.0))) == pytest.approx(SQRT2, rel=1e-7)
+    u = unit(vec(1.0, math.sqrt(3.0) / 2.0, 0.5))
+    assert O.surf_distance(S, r, u) == pytest.approx(1.275200556 * SQRT2, rel=1e-7)
+    assert O.surf_distance(S, r, unit(vec(1.0e20, 1.0, 0.0))) == INF
+    r = [2.0] * 3
+    r[p1] = 0.0
+    u = unit(vec(1.0, 1.0, 0.0))
+    assert O.surf_distance(S, r, u) == pytest.approx(4.0 * SQRT2, rel=1e-7)
+    assert O.surf_distance(S, r, [-x for x in u]) == INF
+    assert O.surf_distance(S, r, [-x for x in unit(vec(1.0, 0.0, 1.0))]) == INF
+    r[p1] = 0.0 - 0.5 * SURF_TOL
+    assert O.surf_distance(S, r, u) == pytest.approx((4.0 + 0.5 * SURF_TOL) * SQRT2, rel=1e-7)
+    r = [2.0] * 3
+    r[p1] = 1.0
+    assert O.surf_distance(S, r, u) == pytest.approx(3.0 * SQRT2, rel=1e-7)
+    assert O.surf_distance(S, r, [-x for x in u]) == pytest.approx(SQRT2, rel=1e-7)
+
+
+# ---- quadric_test.f90 -----------------------------------------------------------------------
+def test_quadric():
+    cyl, bb = surf(id=7, type="quadric", coeffs=[1.0, 1.0, 0, 0, 0, 0, 0, 0, 0, -4])
+    elp, _ = surf(id=3, type="quadric", coeffs=[0.25, 0.0625, 0.0, 0, 0, 0, -0.5, 0, 0, -0.75])
+    assert bb == [-INF] * 3 + [INF] * 3
+    u = [-1.0, 0.0, 0.0]
+    assert not O.surf_halfspace(cyl, [-0.5, 1.0, 10.0], u)
+    r = [2.0, 0.0, 10.0]
+    mv = lambda r, eps: [r[i] + eps * u[i] for i in range(3)]
+    assert not O.surf_halfspace(cyl, r, u)
+    assert not O.surf_halfspace(cyl, mv(r, -0.5 * SURF_TOL), u)
+    assert O.surf_halfspace(cyl, mv(r, -8.0001 * SURF_TOL), u)
+    assert O.surf_halfspace(cyl, mv(r, -1.0), u)
+    assert not O.surf_halfspace(cyl, mv(r, 1.0), u)
+    assert O.surf_halfspace(cyl, r, [0.0, 1.0, 0.0])
+    assert not O.surf_halfspace(elp, [0.0, 2.0, -8.0], u)
+    assert O.surf_halfspace(elp, [-1.0, 2.0, -8.0], u)
+    r = [3.0, 0.0, -8.0]
+    assert not O.surf_halfspace(elp, r, u)
+    assert not O.surf_halfspace(elp, mv(r, -0.5 * SURF_TOL), u)
+    assert O.surf_halfspace(elp, mv(r, -1.501 * SURF_TOL), u)
+    assert O.surf_halfspace(elp, r, [0.0, 1.0, 0.0])
+    TOL = 1e-7
+    assert O.surf_distance(cyl, [-4.0, 0.0, 1.0], [1.0, 0.0, 0.0]) == pytest.approx(2.0, rel=TOL)
+    u = [0.744208408, 0.248069469, 0.620173673]
+    assert O.surf_distance(cyl, [-3.0, 0.5, 30.0], u) == pytest.approx(1.634986685, rel=TOL)
+    assert O.surf_distance(cyl, [0.0, 2.0, 1.0], [1.0, 0.0, 0.0]) == INF
+    u = [0.707106781187, 0.707106781187, 0.0]
+    assert O.surf_distance(cyl, [-0.5, -1.0, 1.0], u) == pytest.approx(3.0291621403, rel=TOL)
+    r = [-2.0 - 0.5 * SURF_TOL, 0.0, 1.0]
+    assert O.surf_distance(cyl, r, [1.0, 0.0, 0.0]) == pytest.approx(4.0 + 0.5 * SURF_TOL, rel=TOL)
+    assert O.surf_distance(elp, [0.0, -6.0, 12.0], [0.0, 1.0, 0.0]) == pytest.approx(2.53589838486, rel=TOL)
+    assert O.surf_distance(elp, [-1.0, -4.0, 12.0], u) == pytest.approx(1.13137084990, rel=TOL)
+    assert O.surf_distance(elp, [0.0, -2.0, 12.0], u) == pytest.approx(4.16282187604, rel=TOL)
+
+
+# ---- truncCylinder_test.f90 -----------------------------------------------------------------
+@pytest.mark.parametrize("ax", AXES)
+def test_trunc_cylinder(ax):
+    p1, p2 = PLANE[ax]
+    origin = [2.0, 2.0, 2.0]
+    origin[p1], origin[p2] = 1.0, 2.0
+    S, bb = surf(type="xyz"[ax] + "TruncCylinder", origin=origin, halfwidth=1.5, radius=2.0)
+    ref = [0.0] * 6
+    ref[ax], ref[p1], ref[p2] = 0.5, -1.0, 0.0
+    ref[3 + ax], ref[3 + p1], ref[3 + p2] = 3.5, 3.0, 4.0
+    assert bb == pytest.approx(ref)
+
+    def pe(va, v1, v2):
+        v = [0.0] * 3
+        v[ax], v[p1], v[p2] = va, v1, v2
+        return v
+
+    u = pe(0.0, 1.0, 0.0)
+    for pt, expect in [((5.0, 3.0, 4.0), True), ((1.3, 1.0, 5.0), True), ((1.3, -2.0, 3.3), True),
+                       ((3.3, 1.1, 2.2), False), ((2.1, -0.7, 2.1), False)]:
+        assert O.surf_halfspace(S, pe(*pt), u) == expect, pt
+    r = pe(3.3, -1.0, 2.0)
+    u = unit(pe(1.0, 1.0, 0.0))
+    assert not O.surf_halfspace(S, r, u)
+    assert O.surf_halfspace(S, r, [-x for x in u])
+    eps = 0.5 * SURF_TOL
+    assert not O.surf_halfspace(S, pe(3.3, -1.0 - eps, 2.0), u)
+    assert O.surf_halfspace(S, pe(3.3, -1.0 - 1.001 * SURF_TOL, 2.0), u)
+    assert O.surf_halfspace(S, pe(3.3, -1.0 - eps, 2.0), pe(1.0, 0.0, 0.0))
+    assert not O.surf_halfspace(S, pe(3.3, -1.0 + eps, 2.0), pe(1.0, 0.0, 0.0))
+    r = pe(3.5, 1.3, 1.8)
+    u = unit(pe(-1.0, 0.0, 1.0))
+    assert not O.surf_halfspace(S, r, u)
+    assert O.surf_halfspace(S, r, [-x for x in u])
+    u = unit(pe(-1.0, 1.0, 0.0))
+    assert not O.surf_halfspace(S, pe(3.5 + eps, 1.3, 1.8), u)
+    assert O.surf_halfspace(S, pe(3.5 + 1.001 * SURF_TOL, 1.3, 1.8), u)
+    u = unit(pe(0.0, -1.0, 1.0))
+    assert O.surf_halfspace(S, pe(3.5 + eps, 1.3, 1.8), u)
+    assert not O.surf_halfspace(S, pe(3.5 - eps, 1.3, 1.8), u)
+    # distance
+    TOL = 1e-7
+    r = pe(5.0, 1.0, 2.0)
+    assert O.surf_distance(S, r, pe(-1.0, 0.0, 0.0)) == pytest.approx(1.5, rel=TOL)
+    assert O.surf_distance(S, r, unit(pe(-1.0, -1.0, 0.0))) == pytest.approx(1.5 * SQRT2, rel=TOL)
+    assert O.surf_distance(S, r, unit(pe(-1.0, -2.0, -2.0))) == INF
+    assert O.surf_distance(S, r, unit(pe(0.0, -2.0, -2.0))) == INF
+    assert O.surf_distance(S, r, unit(pe(1.0, -1.0, -1.0))) == INF
+    r = pe(1.5, -2.0, 2.0)
+    assert O.surf_distance(S, r, pe(0.0, 1.0, 0.0)) == pytest.approx(1.0, rel=TOL)
+    assert O.surf_distance(S, r, unit(pe(1.0, 1.0, 0.0))) == pytest.approx(SQRT2, rel=TOL)
+    v = pe(0.0, 1.5, math.sqrt(7.0) * 0.5)
+    assert O.surf_distance(S, r, unit(v)) == pytest.approx(float(np.linalg.norm(v)), rel=TOL)
+    assert O.surf_distance(S, r, unit(pe(1.0, 1.0, 2.0))) == INF
+    assert O.surf_distance(S, r, pe(1.0, 0.0, 0.0)) == INF
+    assert O.surf_distance(S, r, pe(1.0, -1.0, 0.0)) == INF
+    assert O.surf_distance(S, pe(3.5 + eps, 1.3, 1.8), pe(-1.0, 0.0, 0.0)) == pytest.approx(3.0, rel=TOL)
+    assert O.surf_distance(S, pe(3.5 - eps, 1.3, 1.8), pe(1.0, 0.0, 0.0)) == INF
+
+
+# ---- rotationMatrix (genericProcedures.f90:1098-1140) ------------------------------------------
+@pytest.mark.parametrize("angles", [(90.0, 90.0, 90.0), (30.0, 45.0, 60.0), (0.0, 180.0, 359.0), (12.5, 0.0, 0.0),
+                                    (200.0, 30.0, 200.0)])
+def test_rotation_matrix_is_zxz_euler_as_written(angles):
+    """The matrix maps global to local coordinates: the transpose of the z-x-z intrinsic rotation,
+    except that the reference writes element (1,3) as sin(psi) sin(phi) where the rotation has
+    sin(psi) sin(theta); the restatement keeps the reference's element, so the matrix is a
+    rotation exactly when sin(phi) sin(psi) = sin(theta) sin(psi)."""
+    phi, theta, psi = (math.radians(a) for a in angles)
+    R = np.array(T.rotation_matrix(*angles)).reshape(3, 3)
+    ref = Rotation.from_euler("ZXZ", angles, degrees=True).as_matrix().T
+    mask = np.ones((3, 3), dtype=bool)
+    mask[0, 2] = False
+    assert np.allclose(R[mask], ref[mask], atol=1e-15)
+    assert R[0, 2] == pytest.approx(math.sin(psi) * math.sin(phi), abs=1e-15)
+    if abs(math.sin(phi) - math.sin(theta)) < 1e-15 or abs(math.sin(psi)) < 1e-15:
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-15)
+    else:
+        assert not np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+    with pytest.raises(ValueError):
+        T.rotation_matrix(360.0, 0.0, 0.0)
+
+
+# ---- simpleCell_test.f90 (one cell in a cellUniverse) -------------------------------------------
+def _single_cell():
+    surfs = {"b": {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [10.0, 10.0, 10.0]},
+             "surf1": {"id": 13, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 2.0},
+             "surf2": {"id": 4, "type": "xPlane", "x0": 0.0},
+             "surf3": {"id": 99, "type": "yPlane", "y0": 0.0}}
+    cells = {"c": {"id": 2, "type": "simpleCell", "surfaces": [-13, 4, 99], "filltype": "outside"}}
+    unis = {"root": {"id": 1, "type": "rootUniverse", "border": 500, "fill": "u<7>"},
+            "cu": {"id": 7, "type": "cellUniverse", "cells": [2]}}
+    return geometry_problem(unis, surfs, cells=cells)
+
+
+def test_simple_cell_inside_and_distance():
+    comp = _single_cell()
+    u = [1.0, 0.0, 0.0]
+    assert O.uni_findcell(comp, 2, [1.0, 0.3, 0.1], u) == 1
+    assert O.uni_findcell(comp, 2, [0.3, 1.3, 0.4], u) == 1
+    assert O.uni_findcell(comp, 2, [-0.1, 1.2, 0.1], u) == 2      # UNDEF local cell
+    assert O.uni_findcell(comp, 2, [1.8, 0.8, 0.8], u) == 2
+    # surfaces are tabled in order of first reference: 13 -> 1, 4 -> 2, 99 -> 3
+    r = [0.3, 0.4, 0.0]
+    d, s = O.uni_distance(comp, 2, 1, r, [-1.0, 0.0, 0.0])
+    assert d == pytest.approx(0.3, rel=1e-7) and s == 2
+    d, s = O.uni_distance(comp, 2, 1, r, [0.0, -1.0, 0.0])
+    assert d == pytest.approx(0.4, rel=1e-7) and s == 3
+    d, s = O.uni_distance(comp, 2, 1, r, [1.0, 0.0, 0.0])
+    assert d == pytest.approx(1.659591794, rel=1e-7) and s == 1
+
+
+# ---- cellUniverse_test.f90 --------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def cell_uni():
+    surfs = {"b": {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [50.0, 50.0, 50.0]},
+             "surf1": {"id": 1, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 2.0},
+             "surf2": {"id": 2, "type": "sphere", "origin": [4.0, 0.0, 0.0], "radius": 1.0}}
+    cells = {"cell1": {"id": 1, "type": "simpleCell", "surfaces": [-1], "filltype": "uni", "universe": 3},
+             "cell2": {"id": 2, "type": "simpleCell", "surfaces": [1, 2], "filltype": "uni", "universe": 4}}
+    unis = {"root": {"id": 100, "type": "rootUniverse", "border": 500, "fill": "u<1>"},
+            "u1": {"id": 1, "type": "cellUniverse", "origin": [2.0, 0.0, 0.0], "rotation": [90.0, 90.0, 90.0],
+                   "cells": [1, 2]},
+            "u3": {"id": 3, "type": "pinUniverse", "radii": [0.0], "fills": ["m1"]},
+            "u4": {"id": 4, "type": "pinUniverse", "radii": [0.0], "fills": ["m2"]}}
+    return geometry_problem(unis, surfs, cells=cells, materials=("m1", "m2"))
+
+
+def test_cell_universe_fill(cell_uni):
+    fill, _ = cell_uni.graph
+    # root [u<1>, outside], cellUniverse [u<3>, u<4>, UNDEF, OVERLAP] (fills as uniIdx)
+    assert list(fill[2:6]) == [-3, -4, T.UNDEF_MAT, T.OVERLAP_MAT]
+
+
+def test_cell_universe_enter_rotated(cell_uni):
+    TOL = 1e-7
+    o = O.place(cell_uni, [0.0, 0.0, 3.0], [0.0, 0.0, 1.0])
+    assert o.nesting == 3 and o.localID[1] == 1 and o.matIdx == 1
+    assert list(o.r[1]) == pytest.approx([1.0, 0.0, 0.0], abs=TOL)
+    o = O.place(cell_uni, [2.0, 0.0, 1.0], [0.0, 1.0, 0.0])
+    assert o.localID[1] == 2 and o.matIdx == 2
+    assert list(o.r[1]) == pytest.approx([-1.0, 0.0, 2.0], abs=TOL)
+    o = O.place(cell_uni, [0.0, 0.0, 6.5], [1.0, 0.0, 0.0])
+    assert o.nesting == 2 and o.localID[1] == 3 and o.matIdx == T.UNDEF_MAT
+    assert list(o.r[1]) == pytest.approx([4.5, 0.0, 0.0], abs=TOL)
+
+
+def test_cell_universe_distance_and_undefined(cell_uni):
+    TOL = 1e-7
+    d, s = O.uni_distance(cell_uni, 2, 1, [-1.0, 0.0, 0.0], [1.0, 0.0, 0.0])
+    assert d == pytest.approx(3.0, rel=TOL) and s == 1
+    d, s = O.uni_distance(cell_uni, 2, 2, [7.0, 0.0, 0.0], [-1.0, 0.0, 0.0])
+    assert d == pytest.approx(2.0, rel=TOL) and s == 2
+    d, s = O.uni_distance(cell_uni, 2, 2, [7.0, 0.0, 0.0], [1.0, 0.0, 0.0])
+    assert d == INF and s == 0
+    # the UNDEF (n+1) and OVERLAP (n+2) local cells have no distance: fatalError in the reference
+    for local, msg in ((3, "undefined local cell"), (4, "overlapping local cell")):
+        O.place(cell_uni, [0.0, 0.0, 3.0])          # clears the error state
+        O.uni_distance(cell_uni, 2, local, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0])
+        with pytest.raises(RuntimeError, match=msg):
+            O.check(O.lib().orc_error_code())
+
+
+def test_cell_universe_cross(cell_uni):
+    # crossing surface 1 from cell 1 at (0, 2, 0) moving +y: NUDGE, then the search finds cell 2;
+    # the ST move from inside cell 1 reaches the same crossing through geometryStd%move
+    o = O.place(cell_uni, [0.0, 0.0, 2.0], [0.0, 1.0, 0.0])      # local (0, 0, 0) in cell 1
+    assert o.localID[1] == 1 and list(o.r[1]) == pytest.approx([0.0, 0.0, 0.0], abs=1e-15)
+    o, u, d, ev = O.move(cell_uni, [0.0, 0.0, 2.0], [0.0, 1.0, 0.0], 10.0)
+    assert ev == 3 and o.localID[1] == 2 and o.matIdx == 2
+    assert d == pytest.approx(2.0, rel=1e-12)
+    # the local position carries the nudge of 1e-8 along the local direction (-y after rotation)
+    assert o.r[1][1] == pytest.approx(-2.0 - 1e-8, abs=1e-14)
+
+
+def test_cell_universe_overlap():
+    surfs = {"b": {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [50.0, 50.0, 50.0]},
+             "surf1": {"id": 1, "type": "sphere", "origin": [21.0, 0.0, 0.0], "radius": 4.5},
+             "surf2": {"id": 2, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 24.1}}
+    cells = {"cell1": {"id": 1, "type": "simpleCell", "surfaces": [-1], "filltype": "mat", "material": "m1"},
+             "cell2": {"id": 2, "type": "simpleCell", "surfaces": [1, -2], "filltype": "mat", "material": "m1"},
+             "cell3": {"id": 3, "type": "simpleCell", "surfaces": [2], "filltype": "mat", "material": "m1"}}
+    unis = {"root": {"id": 100, "type": "rootUniverse", "border": 500, "fill": "u<2>"},
+            "u2": {"id": 2, "type": "cellUniverse", "checkOverlap": 1, "cells": [1, 2, 3]}}
+    comp = geometry_problem(unis, surfs, cells=cells)
+    u = [1.0, 0.0, 0.0]
+    assert O.uni_findcell(comp, 2, [22.0, 0.0, 0.0], u) == 1
+    assert O.uni_findcell(comp, 2, [0.0, 25.0, 0.0], u) == 3
+    assert O.uni_findcell(comp, 2, [24.2, 0.0, 0.0], u) == 5      # in cells 1 and 3: OVERLAP
+    # without checkOverlap the first cell in definition order wins
+    unis["u2"]["checkOverlap"] = 0
+    comp = geometry_problem(unis, surfs, cells=cells)
+    assert O.uni_findcell(comp, 2, [24.2, 0.0, 0.0], u) == 1
+
+
+def test_cell_definitions_rejected():
+    surfs = {"b": {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [5.0, 5.0, 5.0]},
+             "s": {"id": 1, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 1.0}}
+    unis = {"root": {"id": 100, "type": "rootUniverse", "border": 500, "fill": "u<2>"},
+            "u2": {"id": 2, "type": "cellUniverse", "cells": [1]}}
+    bad = {"c": {"id": 1, "type": "simpleCell", "surfaces": [1, -1], "filltype": "mat", "material": "m1"}}
+    with pytest.raises(ValueError, match="repeated surfaces"):
+        geometry_problem(unis, surfs, cells=bad)
+    bad = {"c": {"id": 1, "type": "simpleCell", "surfaces": [-1], "filltype": "pie"}}
+    with pytest.raises(ValueError, match="Unknown type of cell filling"):
+        geometry_problem(unis, surfs, cells=bad)
+    bad = {"c": {"id": 1, "type": "unionCell", "surfaces": [-1], "filltype": "outside"}}
+    with pytest.raises(NotImplementedError):
+        geometry_problem(unis, surfs, cells=bad)
+
+
+# ---- a sphere as the root border (vacuum), closed form ------------------------------------------
+def test_absorbing_sphere_escape_closed_form():
+    """Pure absorber sphere of radius R, centred isotropic point source: every history either
+    collides once (probability 1 - exp(-Sigma R)) or leaks, so the collision-estimator flux per
+    source neutron is (1 - exp(-Sigma R)) / Sigma."""
+    R, sig = 1.7, 0.9
+    pop, cycles = 4000, 20
+    for top in ("transportOperatorST", "transportOperatorDT"):
+        inp, mg = configs.absorbing_sphere(R, sig, pop, cycles, top)
+        comp = T.compile_problem(inp, mg)
+        src = T.compile_source(inp["source"], comp)
+        res, done, h = O.fixed_run(comp, src, pop, cycles, 3, hist=True, threads=4)
+        expect = pop * (1.0 - math.exp(-sig * R)) / sig
+        z = (res[0, 0] - expect) / res[0, 1]
+        assert abs(z) < 3, (top, res[0], expect)
+        assert set(np.unique(h["fate"])) <= {5001, 5002}
