@@ -184,6 +184,11 @@ typedef struct scn_map1d {
                                 int_data[data + n_mat + 1] = bin for any other index */
 } scn_map1d;
 
+/* Clerks (Tallies/TallyClerks): kind SCN_CLERK_COLLISION = collisionClerk (collisionClerk_class.f90,
+ * scores w/Sigma at every collision), SCN_CLERK_TRACK = trackClerk (trackClerk_class.f90, scores
+ * w*L for every surface-tracking path of length L, with map and response evaluated at the
+ * pre-path state; delta-tracking flights report no path). */
+enum { SCN_CLERK_COLLISION = 0, SCN_CLERK_TRACK = 1 };
 typedef struct scn_clerk {
     int32_t gtop, glow;      /* energyFilter window; gtop = 0 means no filter */
     int32_t n_maps;          /* 0 (single bin) .. 3 (multiMap) */
@@ -192,6 +197,7 @@ typedef struct scn_clerk {
     int32_t multi[3];
     int32_t width;           /* number of responses */
     int32_t resp[8];         /* SCN_MT_* */
+    int32_t kind;            /* SCN_CLERK_* */
     int64_t mem_addr;        /* 0-based offset of this clerk's first bin in the score memory */
     int64_t size;            /* width * product(nbins) */
 } scn_clerk;

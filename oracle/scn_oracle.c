@@ -992,10 +992,10 @@ enum { K_IMP_PROD = 0, K_SCATTER_PROD = 1, K_IMP_ABS = 2, K_ANA_LEAK = 3 };
 enum { FATE_NO = 5000, FATE_ABS = 5001, FATE_LEAK = 5002 };
 
 /* spaceMap / materialMap / multiMap (Tallies/TallyMaps) on the particle's global state */
-static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
+static int o_map1d_at(const scn_problem* P, const scn_map1d* m, const double rg[3], int mat)
 {
     if (m->type == SCN_MAP_SPACE) {
-        double v = p->coords.lvl[0].r[m->axis];
+        double v = rg[m->axis];
         int idx;
         if (m->grid_type == SCN_GRID_LIN) { /* grid_class.f90:154-175 */
             const double* g = P->real_data + m->data;
@@ -1015,10 +1015,15 @@ static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
         if (idx < 1 || idx >= m->nbins + 1) idx = 0;
         return idx;
     } else {
-        int mat = p->coords.matIdx;
         int nm = P->mg.n_mat;
         return (mat >= 0 && mat <= nm) ? P->int_data[m->data + mat] : P->int_data[m->data + nm + 1];
     }
+}
+
+/* tally map of the particle's state: global position and material */
+static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
+{
+    return o_map1d_at(P, m, p->coords.lvl[0].r, p->coords.matIdx);
 }
 
 /* tallyAdmin%reportInColl (tallyAdmin_class.f90:466-486): attachment (keffImplicitClerk,
@@ -1038,6 +1043,7 @@ static void o_report_in_coll(o_hctx* X, const o_part* p, int isVirtual)
     if (!X->bins) return;
     for (int ci = 0; ci < P->n_clerks; ++ci) {
         const scn_clerk* C = &P->clerks[ci];
+        if (C->kind != SCN_CLERK_COLLISION) continue;
         if (!C->handle_virtual && isVirtual) continue;
         if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
         int bin = 1;
@@ -1068,6 +1074,34 @@ static void o_report_hist(o_hctx* X, const o_part* p)
 }
 
 /* transportOperatorST_class.f90:48-166 surfaceTracking */
+/* tallyAdmin_class.f90:545-568 reportPath to trackClerks (trackClerk_class.f90 reportPath): the
+ * path of length L scores response(pre-path material, group) * w * L in the bin of the pre-path
+ * state (position and material before the move) */
+static void o_report_path(o_hctx* X, const o_part* p, double L, const double r0[3], int mat0)
+{
+    const scn_problem* P = X->P;
+    if (!X->bins) return;
+    for (int ci = 0; ci < P->n_clerks; ++ci) {
+        const scn_clerk* C = &P->clerks[ci];
+        if (C->kind != SCN_CLERK_TRACK) continue;
+        if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
+        int bin = 1;
+        for (int k = 0; k < C->n_maps; ++k) {
+            int b = o_map1d_at(P, &C->maps[k], r0, mat0);
+            if (b == 0) { bin = 0; break; }
+            bin = bin + (b - 1) * C->multi[k];
+        }
+        if (bin == 0) continue;
+        int64_t addr = C->mem_addr + (int64_t)C->width * (bin - 1);
+        for (int i = 0; i < C->width; ++i) {
+            double resp;
+            if (C->resp[i] == SCN_MT_FLUX) resp = 1.0;
+            else resp = (mat0 == SCN_VOID_MAT) ? 0.0 : o_macro_get(o_xs(P, mat0, p->G), C->resp[i]);
+            X->bins[addr + i] = X->bins[addr + i] + resp * p->w * L;
+        }
+    }
+}
+
 static void o_surface_tracking(o_hctx* X, o_part* p)
 {
     const scn_problem* P = X->P;
@@ -1090,10 +1124,13 @@ static void o_surface_tracking(o_hctx* X, o_part* p)
             sigmaT = o_track_mat_xs(P, mat, p->G);
             if (dist != dist) { o_fail(SCN_ERR_NAN, "Distance is NaN"); p->isDead = 1; return; }
         }
+        double r0[3] = {p->coords.lvl[0].r[0], p->coords.lvl[0].r[1], p->coords.lvl[0].r[2]};  /* savePrePath */
+        int mat0 = p->coords.matIdx;
         int event = o_move(P, &p->coords, &dist, &cache, P->st_cache);
         X->n_events++;
         p->time = p->time + dist / 1.0;
         if (event == EV_COLL) p->fate = FATE_NO;
+        o_report_path(X, p, dist, r0, mat0);
         mat = p->coords.matIdx;
         if (mat == SCN_OUTSIDE_MAT) {
             p->isDead = 1;

@@ -340,12 +340,20 @@ def _mesh_clerk(dim, hw, n_bins):
 
 
 def slab_fixed(seed: int = 4, nG: int = 4, regions: int = 3, pop: int = 2000, cycles: int = 10,
-               transport: str = "transportOperatorST", n_bins: int = 40):
+               transport: str = "transportOperatorST", n_bins: int = 40, track: bool = False):
+    """Fixed source in a 1D slab of `regions` materials.  ``track``: the mesh and a per-material
+    absorption tally are also scored by trackClerks (path-length estimators)."""
     rng = np.random.default_rng(seed)
     mats = {f"m{i + 1}": fixed_material(rng, nG) for i in range(regions)}
     hw = [float(rng.uniform(5.0, 20.0)), 10.0, 10.0]
     geom = _region_geometry(1, hw, list(mats), [0, 0, 1, 1, 1, 1])
     tally = {"total": _flux_clerk(), "mesh": _mesh_clerk(1, hw, n_bins)}
+    if track:
+        tally["track_mesh"] = dict(_mesh_clerk(1, hw, n_bins), type="trackClerk")
+        tally["track_mats"] = {"type": "trackClerk", "map": {"type": "materialMap", "materials": list(mats)},
+                               "filter": {"type": "energyFilter", "Gtop": 1, "Glow": max(1, nG - 1)},
+                               "response": ["flux", "abs"], "flux": {"type": "fluxResponse"},
+                               "abs": {"type": "macroResponse", "MT": -2}}
     src = {"type": "materialSource", "data": "mg", "mat": "m2" if regions > 1 else "m1", "G": 1}
     return _fixed_inp(geom, mats, src, tally, pop, cycles, seed, transport), mats
 
@@ -468,25 +476,29 @@ def core_3d(seed: int = 5, nG: int = 70, pop: int = 10_000_000, inactive: int = 
 def absorbing_sphere(radius: float = 1.7, sig: float = 0.9, pop: int = 4000, cycles: int = 20,
                      transport: str = "transportOperatorST"):
     """Fixed source in a purely absorbing sphere (the root border, vacuum) with a centred isotropic
-    point source, one total-flux bin: flux per source neutron (1 - exp(-sig R)) / sig."""
+    point source; a collision-estimator and a track-length (trackClerk) total-flux bin, both
+    (1 - exp(-sig R)) / sig per source neutron."""
     mg = {"m1": {"numberOfGroups": 1, "capture": [sig], "scatteringMultiplicity": [1.0], "P0": [0.0]}}
     geom = {"type": "geometryStd", "boundary": [0, 0, 0, 0, 0, 0], "graph": {"type": "shrunk"},
             "surfaces": {"ball": {"id": 1, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": radius}},
             "cells": {},
             "universes": {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "m1"}}}
     src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "G": 1}
-    return _fixed_inp(geom, mg, src, {"total": _flux_clerk()}, pop, cycles, 3, transport), mg
+    tally = {"total": _flux_clerk(), "track": dict(_flux_clerk(), type="trackClerk")}
+    return _fixed_inp(geom, mg, src, tally, pop, cycles, 3, transport), mg
 
 
 def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int = 10,
-                 transport: str = "transportOperatorST", n: int = 4, fixed: bool = False, outer: bool = False):
+                 transport: str = "transportOperatorST", n: int = 4, fixed: bool = False, outer: bool = False,
+                 track: bool = True):
     """Pin lattice built from CSG cells: an n x n lattice (pitch 1.26, reflective box with finite z)
     of two cellUniverse pins made of every surface type the engine reads -- zCylinder and quadric
     annuli with a zTruncCylinder insert (pin A); a square fuel block of x/yPlanes cut by a
     diagonal plane, with a sphere inclusion, in a universe rotated 30 degrees about z (pin B) --
     the 7-group materials of ``pin_lattice_7g``, and a mesh flux tally.  ``fixed``: a fixed-source
     solve with a material source in the fuel instead of an eigenvalue run.  ``outer``: the lattice
-    sits in a 1 x 1 lattice of its own (4 nesting levels) entered through a 'global' universe."""
+    sits in a 1 x 1 lattice of its own (4 nesting levels) entered through a 'global' universe.
+    ``track``: a trackClerk scores the per-material flux and fission rate as well."""
     mg = pin_lattice_materials(seed, 7)
     surfs = {
         "box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [0.63 * n, 0.63 * n, 2.0]},
@@ -543,6 +555,9 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
     mats = {"type": "materialMap", "materials": ["fuel", "structural", "moderator"]}
     tally = {"mesh": mesh, "mats": {"type": "collisionClerk", "map": mats, "response": ["flux"],
                                     "flux": {"type": "fluxResponse"}}}
+    if track:
+        tally["track_mats"] = {"type": "trackClerk", "map": mats, "response": ["flux", "fiss"],
+                               "flux": {"type": "fluxResponse"}, "fiss": {"type": "macroResponse", "MT": -6}}
     nd = {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
           "materials": {m: {"temp": 300, "composition": {}, "xsFile": m} for m in mg}}
     if fixed:

@@ -1174,8 +1174,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     const scn_source* sr = R->source;
     const int nG = pr->mg.n_groups, nM = pr->mg.n_mat;
     // table image
-    std::vector<int> goff, gidx;
+    std::vector<int> goff, gidx, tgoff, tgidx;
     group_clerks(pr, goff, gidx);
+    group_clerks(pr, tgoff, tgidx, SCN_CLERK_TRACK);
     Image I;
     size_t o_unis = I.put(pr->universes, pr->n_universes);
     size_t o_boxes = I.put(pr->boxes, pr->n_boxes);
@@ -1193,6 +1194,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     size_t o_clerks = I.put(pr->clerks, (size_t)pr->n_clerks);
     size_t o_goff = I.put(goff.data(), goff.size());
     size_t o_gidx = I.put(gidx.data(), gidx.size());
+    size_t o_tgoff = I.put(tgoff.data(), tgoff.size());
+    size_t o_tgidx = I.put(tgidx.data(), tgidx.size());
     // pointSource%init normalisation (pointSource_class.f90:120-128,150-152)
     std::vector<double> probG;
     double dir[3] = {sr->dir[0], sr->dir[1], sr->dir[2]};
@@ -1239,6 +1242,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.boxes = (const scn_box*)(base + o_boxes);
     P.surfs = (const scn_surface*)(base + o_surfs);
     P.any_rot = any_rotated(pr);
+    P.n_track = (int)tgidx.size();
+    P.gtrack_off = (const int*)(base + o_tgoff);
+    P.gtrack_idx = (const int*)(base + o_tgidx);
     P.gfill = (const int*)(base + o_gf);
     P.gid = (const int*)(base + o_gi);
     P.real = (const double*)(base + o_real);

@@ -47,6 +47,9 @@ struct DevProblem {
     const double* P1;    // [mat][G_in][G_out] 3 P1/P0, or null (P0 scattering)
     const scn_surface* surfs;   // surfaces of cells and of a non-box root border
     int any_rot;         // some universe is rotated: lower-level directions differ from the global one
+    int n_track;         // track-length clerks present (scored by the general kernels only)
+    const int* gtrack_off;   // [nG+1] track clerks whose energy filter admits group g
+    const int* gtrack_idx;
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -1035,7 +1038,7 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         p.k[K_IMP_ABS] = p.k[K_IMP_ABS] + absXS * flux;
     }
     if (P.n_clerks == 0) return 0;
-    int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];
+    int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];   // collision clerks only
     for (int ci = c0; ci < c1; ++ci) {
         const scn_clerk& C = P.clerks[P.gclerk_idx[ci]];
         if (!C.handle_virtual && isVirtual) continue;
@@ -1056,6 +1059,39 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
             if (C.resp[i] == SCN_MT_FLUX) resp = 1.0;
             else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), C.resp[i]);
             bin_add(B, addr + i, resp * flux);
+        }
+        nsc += C.width;
+    }
+    return nsc;
+}
+
+// tallyAdmin%reportPath to the trackClerks (trackClerk_class.f90 reportPath): a surface-tracking
+// path of length L scores response(pre-path material, G) * w * L in the bin of the pre-path
+// state (global position r0 and material mat0 before the move)
+template <int NL, int GEO>
+__device__ __forceinline__ int report_path(const DevProblem& P, const Pt<NL, GEO>& p, double L, const double r0[3],
+                                           int mat0, Bins& B)
+{
+    int nsc = 0;
+    const int c0 = P.gtrack_off[p.G - 1], c1 = P.gtrack_off[p.G];
+    for (int ci = c0; ci < c1; ++ci) {
+        const scn_clerk& C = P.clerks[P.gtrack_idx[ci]];
+        int bin = 1;
+        for (int k = 0; k < C.n_maps; ++k) {
+            const int b = map1d(P, C.maps[k], r0, mat0);
+            if (b == 0) {
+                bin = 0;
+                break;
+            }
+            bin = bin + (b - 1) * C.multi[k];
+        }
+        if (bin == 0) continue;
+        const int64_t addr = C.mem_addr + (int64_t)C.width * (bin - 1);
+        for (int i = 0; i < C.width; ++i) {
+            double resp;
+            if (C.resp[i] == SCN_MT_FLUX) resp = 1.0;
+            else resp = (mat0 == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat0, p.G), C.resp[i]);
+            bin_add(B, addr + i, resp * p.w * L);
         }
         nsc += C.width;
     }
@@ -1095,10 +1131,18 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, Dist
             return STEP_DEAD;
         }
     }
+    double r0[3];   // savePrePath: the pre-path state of track clerks
+    int mat0 = 0;
+    if (GEO && P.n_track) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r0[k] = p.r[0][k];
+        mat0 = p.mat;
+    }
     int event = move(P, p, dist, cache, st);
     cnt.seg++;
     p.t = p.t + dist / 1.0;
     if (event == EV_COLL) p.fate = FATE_NO;
+    if (GEO && P.n_track) cnt.sc += report_path(P, p, dist, r0, mat0, B);
     mat = p.mat;
     if (mat == SCN_OUTSIDE_MAT) {
         p.dead = 1;

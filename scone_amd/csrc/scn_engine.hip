@@ -1043,8 +1043,10 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     double *d_real = nullptr, *d_xs = nullptr, *d_P0 = nullptr, *d_prod = nullptr, *d_chi = nullptr, *d_maj = nullptr;
     scn_clerk* d_clerks = nullptr;
     // clerks admitted per group (energyFilter evaluated once, host side)
-    std::vector<int> goff, gidx;
+    std::vector<int> goff, gidx, tgoff, tgidx;
     group_clerks(pr, goff, gidx);
+    group_clerks(pr, tgoff, tgidx, SCN_CLERK_TRACK);
+    int *d_tgoff = nullptr, *d_tgidx = nullptr;
     scn_surface* d_surfs = nullptr;
     if (!rc) rc |= upload(E, &d_unis, pr->universes, pr->n_universes);
     if (!rc) rc |= upload(E, &d_surfs, pr->surfaces, (size_t)std::max(1, pr->n_surfaces));
@@ -1062,6 +1064,8 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= upload(E, &d_clerks, pr->clerks, std::max(1, pr->n_clerks));
     if (!rc) rc |= upload(E, &d_goff, goff.data(), goff.size());
     if (!rc) rc |= upload(E, &d_gidx, gidx.data(), std::max<size_t>(1, gidx.size()));
+    if (!rc) rc |= upload(E, &d_tgoff, tgoff.data(), tgoff.size());
+    if (!rc) rc |= upload(E, &d_tgidx, tgidx.data(), std::max<size_t>(1, tgidx.size()));
     DevProblem& P = E->P;
     P.transport = pr->transport;
     P.st_cache = pr->st_cache;
@@ -1091,6 +1095,9 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.ht_cutoff = pr->ht_cutoff;
     P.surfs = d_surfs;
     P.any_rot = any_rotated(pr);
+    P.n_track = (int)tgidx.size();
+    P.gtrack_off = d_tgoff;
+    P.gtrack_idx = d_tgidx;
     if (pr->mg.P1) {
         double* d_P1 = nullptr;
         if (!rc) rc |= upload(E, &d_P1, pr->mg.P1, (size_t)nM * nG * nG);

@@ -58,6 +58,9 @@ inline const char* check_cell_tables(const scn_problem* pr)
         }
     }
     if (pr->n_surfaces > 0 && !pr->surfaces) return "surface table missing";
+    for (int c = 0; c < pr->n_clerks; ++c)
+        if (pr->clerks[c].kind != SCN_CLERK_COLLISION && pr->clerks[c].kind != SCN_CLERK_TRACK)
+            return "unknown clerk kind";
     for (int u = 0; u < pr->n_universes; ++u) {
         const scn_universe& U = pr->universes[u];
         if (U.type < SCN_UNI_ROOT || U.type > SCN_UNI_CELL) return "unknown universe type";
@@ -82,10 +85,12 @@ inline const char* check_cell_tables(const scn_problem* pr)
     return nullptr;
 }
 
-// the geometry needs the general kernels: cell universes, rotated or 'global' universes, or a
-// root border that is not a box
+// the problem needs the general kernels: cell universes, rotated or 'global' universes, a root
+// border that is not a box, or track-length clerks
 inline int needs_full_geometry(const scn_problem* pr)
 {
+    for (int c = 0; c < pr->n_clerks; ++c)
+        if (pr->clerks[c].kind == SCN_CLERK_TRACK) return 1;   // path scoring lives in the general kernels
     for (int i = 0; i < pr->n_universes; ++i) {
         const scn_universe& U = pr->universes[i];
         if (U.type == SCN_UNI_CELL || U.rotated || U.global) return 1;
@@ -104,9 +109,10 @@ inline int compiled_nl(int depth, int geo = 0)
     return 8;
 }
 
-// clerks whose energyFilter (energyFilter_class.f90:109-129) admits group g, evaluated once:
+// clerks of one kind whose energyFilter (energyFilter_class.f90:109-129) admits group g, evaluated once:
 // goff[g-1] .. goff[g] index into gidx
-inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vector<int>& gidx)
+inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vector<int>& gidx,
+                         int kind = SCN_CLERK_COLLISION)
 {
     const int nG = pr->mg.n_groups;
     goff.assign(nG + 1, 0);
@@ -114,6 +120,7 @@ inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vec
     for (int g = 1; g <= nG; ++g) {
         for (int c = 0; c < pr->n_clerks; ++c) {
             const scn_clerk& K = pr->clerks[c];
+            if (K.kind != kind) continue;
             if (K.gtop > 0 && !(K.gtop <= g && g <= K.glow)) continue;
             gidx.push_back(c);
         }

@@ -36,6 +36,7 @@ TRANSPORT_ST, TRANSPORT_DT, TRANSPORT_HT = 1, 2, 3
 UNI_ROOT, UNI_LAT, UNI_PIN, UNI_CELL = 1, 2, 3, 4
 SURF_APLANE, SURF_PLANE, SURF_SPHERE, SURF_CYLINDER, SURF_QUADRIC, SURF_BOX, SURF_TRUNC_CYLINDER = 1, 2, 3, 4, 5, 6, 7
 OVERLAP_MAT = 2147483645
+CLERK_COLLISION, CLERK_TRACK = 0, 1
 MAP_SPACE, MAP_MATERIAL = 1, 2
 GRID_LIN, GRID_UNSTRUCT = 1, 3
 KAPPA_DEFAULT = float(np.float32(202.27))   # fissionMG_class.f90:64 single-precision literal
@@ -94,7 +95,7 @@ class Map1D(C.Structure):
 class Clerk(C.Structure):
     _fields_ = [("gtop", C.c_int32), ("glow", C.c_int32), ("n_maps", C.c_int32), ("handle_virtual", C.c_int32),
                 ("maps", Map1D * 3), ("multi", C.c_int32 * 3), ("width", C.c_int32), ("resp", C.c_int32 * 8),
-                ("mem_addr", C.c_int64), ("size", C.c_int64)]
+                ("kind", C.c_int32), ("mem_addr", C.c_int64), ("size", C.c_int64)]
 
 
 class Problem(C.Structure):
@@ -779,12 +780,15 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         if not isinstance(cd, dict):
             continue
         ctype = cd.get("type")
-        if ctype != "collisionClerk":
+        if ctype not in ("collisionClerk", "trackClerk"):
             raise NotImplementedError(f"tally clerk {ctype} is outside the engine's scope")
         K = Clerk()
         K.handle_virtual = 1
-        hv = cd.get("handleVirtual", 1)
-        K.handle_virtual = 0 if hv in (0, "false", "no", "n", "F") else 1
+        if ctype == "trackClerk":   # trackClerk_class.f90: path reports only
+            K.kind = CLERK_TRACK
+        else:
+            hv = cd.get("handleVirtual", 1)
+            K.handle_virtual = 0 if hv in (0, "false", "no", "n", "F") else 1
         if "filter" in cd:
             fd = cd["filter"]
             if fd.get("type") != "energyFilter" or "Gtop" not in fd:
@@ -869,7 +873,7 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         clerks.append(K)
         out.clerk_names.append(cname)
         out.clerk_shapes.append([K.width] + shape)
-        out.clerk_info.append({"name": cname, "addr": int(K.mem_addr), "size": int(size), "axes": axes,
+        out.clerk_info.append({"name": cname, "kind": ctype, "addr": int(K.mem_addr), "size": int(size), "axes": axes,
                                "multi": "map" in cd and cd["map"]["type"] == "multiMap",
                                "width": K.width, "shape": [K.width] + shape})
 

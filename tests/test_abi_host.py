@@ -42,6 +42,9 @@ int main(void) {
          offsetof(scn_problem, mg));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(scn_source), sizeof(scn_fixed_request), sizeof(scn_fixed_result),
          sizeof(scn_batch_settings), offsetof(scn_source, probG), offsetof(scn_fixed_request, stop_rel));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(scn_surface), offsetof(scn_surface, c),
+         offsetof(scn_universe, n_cells), offsetof(scn_universe, check_overlap), offsetof(scn_universe, rot),
+         offsetof(scn_problem, surfaces), offsetof(scn_clerk, kind), offsetof(scn_clerk, mem_addr));
   return 0; }
 '''
     tmp = str(tmp_path / "layout_probe")
@@ -59,6 +62,10 @@ int main(void) {
     fixed = [int(x) for x in out[2].split()]
     assert fixed == [C.sizeof(T.Source), C.sizeof(T.FixedRequest), C.sizeof(T.FixedResult),
                      C.sizeof(T.BatchSettings), T.Source.probG.offset, T.FixedRequest.stop_rel.offset]
+    csg = [int(x) for x in out[3].split()]
+    assert csg == [C.sizeof(T.Surface), T.Surface.c.offset, T.Universe.n_cells.offset,
+                   T.Universe.check_overlap.offset, T.Universe.rot.offset, T.Problem.surfaces.offset,
+                   T.Clerk.kind.offset, T.Clerk.mem_addr.offset]
 
 
 def test_engine_fails_loudly_without_gpu():

@@ -48,6 +48,10 @@ CASES = [
      1500, 4),
     ("fissile-DT", lambda: configs.inf_medium_fixed_fissile(pop=1500, cycles=4, transport="transportOperatorDT"),
      1500, 4),
+    ("slab-track-ST", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorST", track=True),
+     1500, 4),
+    ("slab-track-DT", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorDT", track=True),
+     1500, 4),
 ]
 
 

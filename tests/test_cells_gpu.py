@@ -1,9 +1,9 @@
 """Parity of the HIP engines with the oracle on CSG-cell geometries (needs a GPU).
 
 Cell universes of every surface type, a rotated universe, a 'global' universe at five nesting
-levels, and a sphere as the root border run through the general-geometry kernels; the contract is
-the one of the lattice/pin geometries: per-history results bit-exact, collision-clerk scores within
-1e-12 relative (summation order)."""
+levels, a sphere as the root border and track-length clerks run through the general kernels; the
+contract is the one of the lattice/pin geometries: per-history results bit-exact, clerk scores
+within 1e-12 relative (summation order)."""
 import numpy as np
 import pytest
 

@@ -447,3 +447,58 @@ def test_absorbing_sphere_escape_closed_form():
         z = (res[0, 0] - expect) / res[0, 1]
         assert abs(z) < 3, (top, res[0], expect)
         assert set(np.unique(h["fate"])) <= {5001, 5002}
+
+
+# ---- trackClerk (trackClerk_class.f90 reportPath) -------------------------------------------------
+def test_track_clerk_path_lengths_replayed():
+    """In the absorbing sphere each history's only path is min(d, R) with d = -ln(u)/Sigma and u
+    the first number of the history's stream (source sampling uses its own stream), so the
+    track-length tally of a batch is the sum of those paths, replayed here with Python floats."""
+    R, sig, pop, cycles, seed = 1.7, 0.9, 500, 2, 3
+    inp, mg = configs.absorbing_sphere(R, sig, pop, cycles, "transportOperatorST")
+    comp = T.compile_problem(inp, mg)
+    assert [c["kind"] for c in comp.clerk_info] == ["collisionClerk", "trackClerk"]
+    src = T.compile_source(inp["source"], comp)
+    res, done, _ = O.fixed_run(comp, src, pop, cycles, seed, threads=2)
+    per_batch = []
+    for b in range(cycles):
+        tot = 0.0
+        for j in range(1, pop + 1):
+            u, _ = O.rng_get(O.rng_skip(seed, 152917 * (2 * pop * b + pop + j)))
+            tot += min(-math.log(u) / sig, R)
+        per_batch.append(tot)
+    assert res[1, 0] == pytest.approx(sum(per_batch) / cycles, rel=1e-12)
+    # and both estimators estimate (1 - exp(-sig R)) / sig per source neutron
+    expect = pop * (1.0 - math.exp(-sig * R)) / sig
+    for k in (0, 1):
+        assert abs(res[k, 0] - expect) < 4 * res[k, 1] + 1e-9 * expect
+
+
+def test_track_clerk_scores_nothing_under_delta_tracking():
+    inp, mg = configs.absorbing_sphere(1.7, 0.9, 300, 2, "transportOperatorDT")
+    comp = T.compile_problem(inp, mg)
+    src = T.compile_source(inp["source"], comp)
+    res, _, _ = O.fixed_run(comp, src, 300, 2, 3, threads=2)
+    assert res[0, 0] > 0 and res[1, 0] == 0.0
+
+
+def test_track_and_collision_estimators_agree_in_a_slab():
+    """Region-integrated fluxes: a path never crosses a region boundary (surface tracking stops
+    there), so the track-length estimator of a material's flux is unbiased like the collision
+    estimator (a finer mesh is not: trackClerk puts a whole path in its starting bin)."""
+    pop, cycles = 4000, 10
+    inp, mg = configs.slab_fixed(pop=pop, cycles=cycles, transport="transportOperatorST", track=True)
+    inp["tally"]["track_total"] = {"type": "trackClerk", "response": ["flux"], "flux": {"type": "fluxResponse"}}
+    flt = inp["tally"]["track_mats"]["filter"]
+    inp["tally"]["coll_mats"] = dict(inp["tally"]["track_mats"], type="collisionClerk", filter=flt)
+    comp = T.compile_problem(inp, mg)
+    src = T.compile_source(inp["source"], comp)
+    res, _, _ = O.fixed_run(comp, src, pop, cycles, inp["seed"], threads=4)
+    info = {c["name"]: c for c in comp.clerk_info}
+    g = lambda n: res[info[n]["addr"]: info[n]["addr"] + info[n]["size"]]
+    for a, b in ((g("total"), g("track_total")), (g("coll_mats"), g("track_mats"))):
+        z = (a[:, 0] - b[:, 0]) / np.hypot(a[:, 1], b[:, 1])
+        assert np.all(np.abs(z) < 4), z
+    # the mesh sums to the total flux for both estimators
+    assert g("mesh")[:, 0].sum() == pytest.approx(g("total")[0, 0], rel=1e-9)
+    assert g("track_mesh")[:, 0].sum() == pytest.approx(g("track_total")[0, 0], rel=1e-9)
