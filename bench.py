@@ -210,7 +210,7 @@ def cpu_batch_baseline(problems, seconds):
 # ---------------------------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------------------------
-def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7):
+def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7, window=10):
     from scone_amd import configs, tables as T
     from scone_amd.engine import Batch
     from scone_amd.serve import Request, Server, latency_summary
@@ -221,7 +221,7 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
     def request(i, arrival):
         _, _, pop, cycles = configs.lattice_request(i, transport=transport)
         return Request(comp, src, pop, cycles, 1000 + i, stop_bin=0, stop_rel=stop_rel, arrival=arrival,
-                       buffer=inp["buffer"])
+                       buffer=inp["buffer"], window=min(window, cycles))
 
     slots, budget = 256, 16 << 30
     # capacity: a saturated pool of requests, all admitted at once
@@ -237,7 +237,7 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
     B.close()
     log(f"serving capacity {cap_hist:.3e} histories/s, {cap_req:.1f} requests/s (mean {mean_req:.0f} histories)")
     out = {"capacity_hist_per_s": cap_hist, "capacity_req_per_s": cap_req, "mean_request_histories": mean_req,
-           "loads": {}}
+           "window": window, "loads": {}}
     rng = np.random.default_rng(seed)
     k = 0
     for load in loads:
@@ -370,7 +370,7 @@ def batch_line(args):
     if res_p:
         identical = all(results_agree(res_b[i], res_p[i], args.warmup + args.steps) for i in res_b)
     cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_seconds)
-    serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds)
+    serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds, window=args.window)
     eig = None
     if args.eigen:
         log("eigen leg")
@@ -421,6 +421,7 @@ def main():
     ap.add_argument("--no-tune", dest="tune", action="store_false", default=True)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="offered-load duration per load level")
+    ap.add_argument("--window", type=int, default=10, help="serving: batches of a request transported per cycle")
     ap.add_argument("--no-eigen", dest="eigen", action="store_false", default=True)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -50,6 +50,9 @@ enum {
 #define SCN_BC_PERIODIC 2
 
 enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2, SCN_TRANSPORT_HT = 3 };
+/* neutronMGstd (neutronMGstd_class.f90); neutronMGimp (neutronMGimp_class.f90), whose fission
+ * sites keep the colliding particle's collision count instead of restarting at 0 */
+enum { SCN_COLL_STD = 0, SCN_COLL_IMP = 1 };
 enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3, SCN_UNI_CELL = 4 };
 enum { SCN_SURF_APLANE = 1, SCN_SURF_PLANE = 2, SCN_SURF_SPHERE = 3, SCN_SURF_CYLINDER = 4,
        SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7 };
@@ -227,6 +230,20 @@ typedef struct scn_problem {
     double ht_cutoff;        /* transportOperatorHT 'cutoff' (default 0.9): DT when sigmaT/majorant > 1 - cutoff */
     int32_t n_surfaces;      /* surfaces of cells and of a non-box root border */
     const scn_surface* surfaces;
+    int32_t collision;       /* SCN_COLL_STD / SCN_COLL_IMP (collisionProcessorFactory_func.f90) */
+    int32_t max_split;       /* neutronMGimp 'maxSplit' (default 1000) */
+    /* neutronMGimp weight windows (weightWindowsField_class.f90; fixed-source runs): ww_n bins
+     * located by ww_map (n_maps/maps/multi as a clerk's), lower and upper weights per bin, and
+     * constSurvival.  After a collision a particle heavier than the bin's upper weight is split
+     * into ceiling(w/upper) (maxSplit in all), the copies going to the history's buffer; one
+     * lighter than the lower weight survives Russian roulette with weight lower*constSurvival.
+     * ww_n = 0: no weight windows. */
+    int32_t ww_n;
+    int32_t pad1;
+    const scn_clerk* ww_map;
+    const double* ww_lower;
+    const double* ww_upper;
+    double ww_survival;
 } scn_problem;
 
 /* ---- eigenvalue run ------------------------------------------------------------------------ */
@@ -295,6 +312,11 @@ typedef struct scn_fixed_request {
     double stop_rel;         /* target relative standard deviation for stop_bin (after >= 2 batches) */
     int32_t record;          /* 1: keep each history's final RNG state and collision count */
     int32_t buffer;          /* 'buffer': depth of a history's LIFO buffer of fission secondaries (0: 50) */
+    int32_t window;          /* batches of this solve transported together in one engine cycle (0: 1,
+                                at most 16; more than 1 needs an engine with n_launch 1).  They are
+                                closed in order, so results and the early-stop point are those of
+                                one batch per cycle; batches past an early stop are discarded. */
+    int32_t pad0;
 } scn_fixed_request;
 
 enum { SCN_SLOT_FREE = 0, SCN_SLOT_RUNNING = 1, SCN_SLOT_DONE = 2 };

@@ -72,6 +72,8 @@ module scone_gpu_mod
     real(c_double)     :: stop_rel
     integer(c_int32_t) :: record
     integer(c_int32_t) :: buffer
+    integer(c_int32_t) :: window
+    integer(c_int32_t) :: pad0
   end type scn_fixed_request
 
   type, public, bind(C) :: scn_fixed_result

@@ -976,6 +976,7 @@ typedef struct o_part {
     int type, isMG;
     uint64_t rng;
     int isDead, fate, broodID, collisionN;
+    int splitCount;        /* neutronMGimp splits of this particle (0 when it leaves the buffer) */
 } o_part;
 
 typedef struct o_hctx {
@@ -1247,6 +1248,59 @@ static int o_legendre_p1(double c, uint64_t* rng, double* mu)
 }
 
 /* collisionProcessor_inter.f90:114-194 collide with neutronMGstd_class.f90:85-282 */
+/* neutronMGimp_class.f90 cutoffs with weight windows (weightWindowsField_class.f90 atP): the window
+ * of the particle's state bin; above the upper weight the particle is split (split), below the
+ * lower weight it plays Russian roulette (russianRoulette) for weight lower*constSurvival */
+static void o_weight_window(o_hctx* X, o_part* p, int mat)
+{
+    const scn_problem* P = X->P;
+    const scn_clerk* M = P->ww_map;
+    int bin = 1;
+    for (int k = 0; k < M->n_maps; ++k) {
+        int b = o_map1d_at(P, &M->maps[k], p->coords.lvl[0].r, p->coords.matIdx);
+        if (b == 0) { bin = 0; break; }
+        bin = bin + (b - 1) * M->multi[k];
+    }
+    double minW = 0.0, maxW = 0.0, avW = 0.0;
+    if (bin > 0) {
+        minW = P->ww_lower[bin - 1];
+        maxW = P->ww_upper[bin - 1];
+        avW = minW * P->ww_survival;
+    }
+    if (p->w > maxW && maxW != 0.0 && p->splitCount < P->max_split) {
+        int mult = (int)ceil(p->w / maxW);
+        if (mult + p->splitCount > P->max_split) mult = P->max_split - p->splitCount + 1;
+        scn_particle_state s;   /* pTemp = p */
+        memset(&s, 0, sizeof s);
+        for (int k = 0; k < 3; ++k) { s.r[k] = p->coords.lvl[0].r[k]; s.dir[k] = p->coords.lvl[0].dir[k]; }
+        s.wgt = p->w / mult;
+        s.G = p->G;
+        s.E = p->E;
+        s.isMG = 1;
+        s.type = p->type;
+        s.time = p->time;
+        s.lambda = p->lambda;
+        s.matIdx = mat;
+        s.uniqueID = p->coords.uniqueID;
+        s.collisionN = p->collisionN;
+        s.broodID = p->broodID;
+        for (int i = 1; i <= mult - 1; ++i) {
+            if (X->stack_cap <= 0) { o_fail(SCN_ERR_UNSUPPORTED, "weight-window splitting needs a buffer"); return; }
+            if (X->sites->n >= X->stack_cap) {
+                o_fail(SCN_ERR_OVERFLOW, "Run out of space for particles. Max size:%lld", (long long)X->stack_cap);
+                p->isDead = 1;
+                return;
+            }
+            o_site_push(X->sites, &s);
+        }
+        p->splitCount = p->splitCount + mult;
+        p->w = p->w / mult;
+    } else if (p->w < minW) {
+        if (scn_rng_get(&p->rng) < (1.0 - p->w / avW)) p->isDead = 1;
+        else p->w = avW;
+    }
+}
+
 static void o_collide(o_hctx* X, o_part* p)
 {
     const scn_problem* P = X->P;
@@ -1294,7 +1348,8 @@ static void o_collide(o_hctx* X, o_part* p)
                 for (int k = 0; k < 3; ++k) s.r[k] = p->coords.lvl[0].r[k];
                 s.G = G_out;
                 s.wgt = wgt;
-                s.collisionN = 0;
+                /* neutronMGstd restarts the site's count; neutronMGimp keeps the parent's */
+                s.collisionN = (P->collision == SCN_COLL_IMP) ? p->collisionN : 0;
                 s.E = p->E;
                 s.isMG = 1;
                 s.type = p->type;
@@ -1346,6 +1401,10 @@ static void o_collide(o_hctx* X, o_part* p)
         o_fail(SCN_ERR_UNSUPPORTED, "Unsupported MT number: %d", -3);
         p->isDead = 1;
         return;
+    }
+    if (P->collision == SCN_COLL_IMP && P->ww_n > 0 && !p->isDead) {
+        o_weight_window(X, p, mat);
+        if (g_err) return;
     }
     p->collisionN += 1;
     /* reportOutColl -> keffImplicitClerk (scattering weight gain) */
@@ -1426,6 +1485,7 @@ static void o_history_fixed(o_hctx* X, const scn_particle_state* s, int n, uint6
         p.fate = FATE_NO;
         p.collisionN = cur.collisionN;
         p.broodID = cur.broodID;
+        p.splitCount = 0;
         p.isDead = 0;
         p.k_eff = 1.0;
         o_place_coord(X->P, &p.coords);

@@ -573,5 +573,47 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
     return inp, mg
 
 
+def ww_medium(ww_file: str | None, pop: int = 2000, cycles: int = 10, transport: str = "transportOperatorST",
+              max_split: int = 1000, buffer: int = 200):
+    """Weight windows under neutronMGimp: a 2-group infinite medium (reflective cube, half-width
+    4 cm) whose scattering multiplicities grow weights (1.35 in group 1, 1.2 downscatter) and
+    shrink them (0.55 in group 2), an isotropic point source in group 1, total and per-group flux
+    bins.  ``ww_file``: the weightWindowsField file (None: neutronMGstd, no windows).  The flux
+    per source neutron is (T - M.S)^-1 q with M the multiplicities (``ww_medium_flux``)."""
+    mg = {"medium": {"numberOfGroups": 2, "capture": [0.5, 0.9],
+                     # one line per incident group: P0(g_out, g_in) for g_in = 1, then g_in = 2
+                     "P0": [0.4, 0.3, 0.05, 0.6], "scatteringMultiplicity": [1.35, 1.2, 1.0, 0.55]}}
+    tally = {"total": _flux_clerk(), "flux_g1": _flux_clerk(1), "flux_g2": _flux_clerk(2)}
+    geom = {"type": "geometryStd", "boundary": [1, 1, 1, 1, 1, 1], "graph": {"type": "shrunk"},
+            "surfaces": {"cube": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [4.0, 4.0, 4.0]}},
+            "cells": {},
+            "universes": {"root": {"id": 1, "type": "rootUniverse", "border": 1, "fill": "medium"}}}
+    src = {"type": "pointSource", "r": [0.0, 0.0, 0.0], "particle": "neutron", "G": 1}
+    inp = _fixed_inp(geom, mg, src, tally, pop, cycles, 17, transport)
+    inp["buffer"] = buffer
+    if ww_file is not None:
+        inp["collisionOperator"] = {"neutronMG": {"type": "neutronMGimp", "weightWindows": 1,
+                                                  "maxSplit": max_split}}
+        inp["varianceReduction"] = {"type": "weightWindowsField", "file": ww_file}
+    return inp, mg
+
+
+def ww_medium_flux():
+    """Closed-form group fluxes per source neutron of ``ww_medium``: (T - M.S) phi = e_1."""
+    P0 = np.array([[0.4, 0.05], [0.3, 0.6]])          # [g_out, g_in]
+    mult = np.array([[1.35, 1.0], [1.2, 0.55]])
+    tot = np.array([0.5 + 0.7, 0.9 + 0.65])
+    return np.linalg.solve(np.diag(tot) - mult * P0, np.array([1.0, 0.0]))
+
+
+def ww_file_text(lower, upper, survival=2.0, bins=(-4.0, 0.0, 4.0)):
+    """A weightWindowsField file (the reference's format): windows over an x spaceMap."""
+    b = " ".join(repr(float(x)) for x in bins)
+    return (f"map {{ type spaceMap; axis x; grid unstruct; bins ({b}); }}\n"
+            f"constSurvival {float(survival)!r};\n"
+            f"wLower ({' '.join(repr(float(x)) for x in lower)});\n"
+            f"wUpper ({' '.join(repr(float(x)) for x in upper)});\n")
+
+
 CONFIGS = {"inf_medium_2g": inf_medium_2g, "pin_lattice_7g": pin_lattice_7g, "core_3d": core_3d,
            "cell_lattice": cell_lattice}

@@ -41,6 +41,8 @@ constexpr int MATERIAL_SOURCE_ATTEMPTS = 200;   // materialSource_class.f90:168
 constexpr int MAX_PROBG = 4096;
 constexpr int DEFAULT_BUFFER = 50;   // fixedSourcePhysicsPackage_class.f90:372 'buffer' default
 constexpr int MAX_BUFFER = 1024;
+constexpr int MAX_WINDOW = 16;        // batches of one solve in flight in one cycle (power of two)
+constexpr int WINDOW_SHIFT = 4;
 // a history still alive after this many (flight, collision) pairs in one launch is reported as lost
 // (an error, not a hang)
 constexpr int MAX_PAIRS_PER_LAUNCH = 1 << 22;
@@ -60,6 +62,7 @@ struct BLive {
     int* G;
     int* brood;
     int* coll;
+    int* split;    // neutronMGimp splitCount of a parked particle
 };
 
 // read-only description of a resident solve
@@ -80,6 +83,7 @@ struct SlotDev {
     int src_type, src_iso, src_G, src_mat, n_probG;
     int pop, n_batches;
     int buffer;         // 'buffer' of the package: depth of a history's LIFO buffer of secondaries
+    int window;         // batches transported per cycle; BIN holds one copy of the score memory each
 };
 
 // mutable state of a slot, written by the kernels
@@ -87,9 +91,11 @@ struct SlotState {
     int running;        // 1 while batches remain
     int batch;          // batches closed (scoreMemory batchN)
     int count[2];       // particles written into queue side k by the last launch
-    int n_in;           // particles the current launch reads
+    int n_in;           // particles the current launch reads (per batch)
+    int n_run;          // batches transported this cycle (batch .. batch + n_run - 1)
+    int n_close;        // of those, batches closed (fewer after an early stop)
     int stopped;        // 1 if the early-stop target ended the solve
-    double norm;        // normalisation factor of the batch being closed
+    double norm[MAX_WINDOW];   // normalisation factor of each batch being closed
     unsigned long long segments, collisions, scores, histories;
 };
 
@@ -126,6 +132,7 @@ __device__ __forceinline__ void bload(const BLive& L, int cap, int i, Pt<NL, GEO
     p.G = L.G[i];
     p.brood = L.brood[i];
     p.coll = L.coll[i];
+    p.split = L.split[i];
     p.nsite = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) p.k[k] = 0.0;
@@ -157,6 +164,7 @@ __device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<
     L.G[i] = p.G;
     L.brood[i] = p.brood;
     L.coll[i] = p.coll;
+    L.split[i] = p.split;
 }
 
 // Source particle j = i + 1 of batch `batch`, then the start of its history
@@ -254,6 +262,7 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
     p.G = G;
     p.t = time;
     p.coll = 0;
+    p.split = 0;
     p.brood = (int)j;
     p.nsite = 0;
 #pragma unroll
@@ -269,26 +278,39 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
     return 1;
 }
 
-// tile table of one launch.  One block of 1024 threads; slots in LDS for the search.
+// tile table of one launch.  One block of 1024 threads; slots in LDS for the search.  A tile is
+// (key, first index) with key = slot * MAX_WINDOW + w: the first launch of a cycle covers batches
+// batch .. batch + n_run - 1 of every running slot (w = 0 .. n_run - 1, pop histories each) and
+// no tile straddles two batches; later launches continue parked particles (w = 0).
 __global__ void __launch_bounds__(1024)
 k_tiles(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first, int in_side, int out_side,
         int tile, int2* __restrict__ tiles, int tiles_cap, int* ctl)
 {
     __shared__ int s_start[MAX_SLOTS + 1];
+    __shared__ int s_tpb[MAX_SLOTS];   // tiles per batch
     __shared__ int s_scan[1024];
     int carry = 0;
     for (int base = 0; base < n_slots; base += 1024) {
         const int s = base + (int)threadIdx.x;
-        int n = 0;
+        int nt = 0;
         if (s < n_slots) {
             SlotState& S = ss[s];
-            const int pop = slots[s].pop;
-            if (first) n = S.running ? pop : 0;
-            else n = S.count[in_side] < pop ? S.count[in_side] : pop;   // a queue never holds more than pop
+            const SlotDev& D = slots[s];
+            const int pop = D.pop;
+            int n = 0, nb = 0;
+            if (first) {
+                nb = S.running ? min(D.window, D.n_batches - S.batch) : 0;
+                n = nb > 0 ? pop : 0;
+                S.n_run = nb;
+            } else {
+                n = S.count[in_side] < pop ? S.count[in_side] : pop;   // a queue never holds more than pop
+                nb = n > 0 ? 1 : 0;
+            }
             S.n_in = n;
             S.count[out_side] = 0;
+            s_tpb[s] = (n + tile - 1) / tile;
+            nt = s_tpb[s] * nb;
         }
-        const int nt = (n + tile - 1) / tile;
         s_scan[threadIdx.x] = nt;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -309,7 +331,10 @@ k_tiles(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first
             if (s_start[mid] <= t) lo = mid;
             else hi = mid;
         }
-        tiles[t] = make_int2(lo, (t - s_start[lo]) * tile);
+        const int lt = t - s_start[lo];
+        const int tpb = s_tpb[lo];   // >= 1: slot lo owns tile t
+        const int w = lt / tpb;
+        tiles[t] = make_int2(lo * MAX_WINDOW + w, (lt - w * tpb) * tile);
     }
     if (threadIdx.x == 0) {
         ctl[0] = total;
@@ -376,7 +401,8 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     for (int i = lane; i < lds_per_wave; i += BWAVE) wl[i] = 0.0;
     const int n_tiles = ctl[0];
     const int lane_gid = blockIdx.x * BBLOCK + threadIdx.x;   // owner of one LIFO buffer in SB
-    int s = -1;                      // current slot (wave-uniform)
+    int key = -1;                    // current slot * MAX_WINDOW + batch of its window (wave-uniform)
+    int s = -1;                      // current slot
     int next = 0, end = 0;           // particles [next, end) of the current tile not yet started
     int pend_s = -1, pend_b = 0;     // a tile of another slot, started once the lanes are idle
     int exhausted = 0;               // the tile table is used up
@@ -408,7 +434,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
             const int2 tl = tiles[t];
             const int ts = __builtin_amdgcn_readfirstlane(tl.x);
             const int tb = __builtin_amdgcn_readfirstlane(tl.y);
-            if (ts == s) {
+            if (ts == key) {
                 next = tb;
                 end = min(tb + tile, n_in);
             } else {
@@ -418,19 +444,21 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
         }
         const unsigned long long live = __ballot(alive);
         if (pend_s >= 0 && live == 0) {
-            if (s >= 0) {
+            if (key >= 0) {
                 flush_counts(SS, cn, lane);
                 bins_flush(B);
                 flush_lds_bins(B, lane);
             }
-            s = pend_s;
+            key = pend_s;
             pend_s = -1;
+            s = key >> WINDOW_SHIFT;
+            const int w = key & (MAX_WINDOW - 1);
             D = slots + s;
             SS = ss + s;
             n_in = SS->n_in;
-            batch = SS->batch;
+            batch = SS->batch + w;
             cap = D->pop;
-            B.glob = D->bin;
+            B.glob = D->bin + (size_t)w * (size_t)D->P.n_bins;
             B.lds_n = min((int64_t)lds_per_wave, D->P.n_bins);
             SB.depth = D->buffer;
             next = pend_b;
@@ -511,6 +539,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
                 p.dead = 0;
                 p.fate = FATE_NO;
                 p.track = 0.0;
+                p.split = 0;   // particle_fromParticleState resets splitCount
                 place_coord(P, p, st);
                 cache.lvl = 0;
                 nseg = 0;
@@ -543,14 +572,19 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
             if (__shfl(e, 0)) break;
         }
     }
-    if (s >= 0) {
+    if (key >= 0) {
         flush_counts(SS, cn, lane);
         bins_flush(B);
         flush_lds_bins(B, lane);
     }
 }
 
-// tallyAdmin_class.f90:768-784: normFactor = normValue / BIN(normBin), or 1
+// End of a cycle for each running slot (one thread per slot), over the n_run batches its window
+// transported, in batch order: tallyAdmin_class.f90:768-784 normFactor = normValue / BIN(normBin)
+// (or 1) of each batch, then the early-stop test on the stop bin's relative sd after each batch
+// (scoreMemory%getResult, scoreMemory_class.f90:537-573) evaluated on the CSUM / CSUM2 values that
+// closing the batches one by one gives, so the solve stops after the same batch as with one
+// batch per cycle; batches after the stop are not closed.
 __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, DevStatus* st)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -558,36 +592,78 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
     SlotState& S = ss[s];
     if (!S.running) return;
     const SlotDev& D = slots[s];
-    double f = 1.0;
-    if (D.norm_bin >= 0) {
-        double ns = D.bin[D.norm_bin];
-        if (ns == 0.0) {
-            raise_error(st, SCN_ERR_NORM, s);
-            f = 0.0;
-        } else {
-            f = D.norm_value / ns;
+    const int64_t nb = D.P.n_bins;
+    const int n_run = S.n_run;
+    int n_close = n_run;
+    double cs = 0.0, cs2 = 0.0;
+    if (D.stop_bin >= 0) {
+        cs = D.csum[D.stop_bin];
+        cs2 = D.csum2[D.stop_bin];
+    }
+    for (int w = 0; w < n_run; ++w) {
+        const double* bin = D.bin + (size_t)w * (size_t)nb;
+        double f = 1.0;
+        if (D.norm_bin >= 0) {
+            double v = bin[D.norm_bin];
+            if (v == 0.0) {
+                raise_error(st, SCN_ERR_NORM, s);
+                f = 0.0;
+            } else {
+                f = D.norm_value / v;
+            }
+        }
+        S.norm[w] = f;
+        const int N = S.batch + w + 1;
+        if (D.stop_bin >= 0 && N < D.n_batches && N >= 2) {
+            const double r = bin[D.stop_bin] * f;
+            cs = cs + r;
+            cs2 = cs2 + r * r;
+            double mean = cs / N;
+            double inv_N = 1.0 / N;
+            double inv_Nm1 = 1.0 / (N - 1);
+            double var = cs2 * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
+            double sd = sqrt(var);
+            if (mean != 0.0 && sd / fabs(mean) <= D.stop_rel) {
+                n_close = w + 1;
+                S.stopped = 1;
+                break;
+            }
+        } else if (D.stop_bin >= 0) {
+            const double r = bin[D.stop_bin] * f;
+            cs = cs + r;
+            cs2 = cs2 + r * r;
         }
     }
-    S.norm = f;
+    S.n_close = n_close;
 }
 
-// scoreMemory_class.f90:309-342 closeCycle of every running slot (grid.y = slot)
+// scoreMemory_class.f90:309-342 closeCycle of each closed batch of every running slot, in batch
+// order (grid.y = slot); the BIN copies of every transported batch are cleared
 __global__ void k_bclose(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss)
 {
     const int s = blockIdx.y;
-    if (!ss[s].running) return;
+    const SlotState& S = ss[s];
+    if (!S.running) return;
     const SlotDev& D = slots[s];
-    const double f = ss[s].norm;
     const int64_t n = D.P.n_bins;
+    const int n_run = S.n_run, n_close = S.n_close;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double r = D.bin[i] * f;
-        D.bin[i] = 0.0;
-        D.csum[i] = D.csum[i] + r;
-        D.csum2[i] = D.csum2[i] + r * r;
+        double c = D.csum[i], c2 = D.csum2[i];
+        for (int w = 0; w < n_run; ++w) {
+            double* b = D.bin + (size_t)w * (size_t)n + i;
+            if (w < n_close) {
+                double r = *b * S.norm[w];
+                c = c + r;
+                c2 = c2 + r * r;
+            }
+            *b = 0.0;
+        }
+        D.csum[i] = c;
+        D.csum2[i] = c2;
     }
 }
 
-// end of the batch: batchN + 1, completion, early stop on the relative sd of stop_bin
+// end of the cycle: batchN + n_close, completion (all batches, or the early stop found above)
 __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int* ctl)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -595,23 +671,10 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
     SlotState& S = ss[s];
     if (!S.running) return;
     const SlotDev& D = slots[s];
-    const int N = S.batch + 1;
+    const int N = S.batch + S.n_close;
     S.batch = N;
-    S.histories += (unsigned long long)D.pop;
-    int done = N >= D.n_batches;
-    if (!done && D.stop_bin >= 0 && N >= 2) {
-        // scoreMemory%getResult (scoreMemory_class.f90:537-573)
-        double mean = D.csum[D.stop_bin] / N;
-        double inv_N = 1.0 / N;
-        double inv_Nm1 = 1.0 / (N - 1);
-        double var = D.csum2[D.stop_bin] * inv_N * inv_Nm1 - mean * mean * inv_Nm1;
-        double sd = sqrt(var);
-        if (mean != 0.0 && sd / fabs(mean) <= D.stop_rel) {
-            done = 1;
-            S.stopped = 1;
-        }
-    }
-    if (done) S.running = 0;
+    S.histories += (unsigned long long)D.pop * (unsigned long long)S.n_close;
+    if (N >= D.n_batches || S.stopped) S.running = 0;
     else atomicAdd(&ctl[2], 1);
 }
 
@@ -660,6 +723,7 @@ struct SlotHost {
     int pop = 0, n_batches = 0, record = 0;
     int fissile = 0, buffer = 0;
     int geo = 0;        // needs the general-geometry kernels
+    int ww = 0;         // weight windows: split copies use the history's buffer
 };
 
 struct Batch {
@@ -756,7 +820,7 @@ size_t live_bytes(int depth, int cap)
     b += 3 * align256(sizeof(double) * c);
     b += align256(sizeof(uint64_t) * c);
     b += align256(sizeof(int) * depth * c);
-    b += 6 * align256(sizeof(int) * c);
+    b += 7 * align256(sizeof(int) * c);
     return b;
 }
 
@@ -782,6 +846,7 @@ void carve_live(char* base, int depth, int cap, BLive& L)
     L.G = (int*)take(sizeof(int) * c);
     L.brood = (int*)take(sizeof(int) * c);
     L.coll = (int*)take(sizeof(int) * c);
+    L.split = (int*)take(sizeof(int) * c);
 }
 
 // host image of a slot's tables: appends arrays, returns their offsets
@@ -816,6 +881,14 @@ int validate(Batch* B, const scn_fixed_request* R, int* depth)
     }
     if (R->buffer < 0 || R->buffer > MAX_BUFFER) {
         B->err = "buffer size outside 0..MAX_BUFFER";
+        return SCN_ERR_ARG;
+    }
+    if (R->window < 0 || R->window > MAX_WINDOW) {
+        B->err = "window outside 0..16";
+        return SCN_ERR_ARG;
+    }
+    if (R->window > 1 && B->S.n_launch > 1) {
+        B->err = "a window of several batches needs an engine with n_launch 1";
         return SCN_ERR_ARG;
     }
     if (const char* m = check_cell_tables(pr)) {
@@ -963,12 +1036,13 @@ void drop_graph(Batch* B)
     B->graph = nullptr;
 }
 
-// per-lane LIFO buffers for the persistent grid, as deep as the deepest resident fissile slot needs
+// per-lane LIFO buffers for the persistent grid, as deep as the deepest resident slot with secondaries
+// (fissile, or weight windows that split) needs
 int ensure_stack(Batch* B, int grid)
 {
     int depth = 0;
     for (const SlotHost& h : B->hs)
-        if (h.used && h.fissile) depth = std::max(depth, h.buffer);
+        if (h.used && (h.fissile || h.ww)) depth = std::max(depth, h.buffer);
     if (depth == 0) return 0;
     const int lanes = grid * BBLOCK;
     if (B->stack_mem && B->stack.stride >= lanes && B->stack.depth >= depth) return 0;
@@ -1088,7 +1162,7 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     }
     Batch* B = new Batch();
     B->S = *settings;
-    if (B->S.tile == 0) B->S.tile = 1024;
+    if (B->S.tile == 0) B->S.tile = 256;
     if (B->S.k0 == 0) B->S.k0 = 8;
     if (B->S.n_launch == 0) B->S.n_launch = 1;
     B->n_slots = settings->max_slots;
@@ -1181,6 +1255,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     size_t o_unis = I.put(pr->universes, pr->n_universes);
     size_t o_boxes = I.put(pr->boxes, pr->n_boxes);
     size_t o_surfs = I.put(pr->surfaces, (size_t)std::max(0, pr->n_surfaces));
+    size_t o_wwmap = I.put(pr->ww_map, pr->ww_n > 0 ? (size_t)1 : (size_t)0);
+    size_t o_wwlo = I.put(pr->ww_lower, (size_t)std::max(0, pr->ww_n));
+    size_t o_wwup = I.put(pr->ww_upper, (size_t)std::max(0, pr->ww_n));
     size_t o_gf = I.put(pr->graph_fill, pr->graph_size);
     size_t o_gi = I.put(pr->graph_id, pr->graph_size);
     size_t o_real = I.put(pr->real_data, pr->n_real);
@@ -1217,7 +1294,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     Ly.queue_bytes = live_bytes(depth, R->pop);
     Ly.bins = Ly.queue + 2 * Ly.queue_bytes;
     const size_t nb = (size_t)std::max<int64_t>(1, pr->n_bins);
-    size_t bins_bytes = 3 * align256(sizeof(double) * nb);
+    const int window = R->window > 0 ? R->window : 1;
+    // BIN (one copy per batch of the window), CSUM, CSUM2
+    size_t bins_bytes = align256(sizeof(double) * nb * window) + 2 * align256(sizeof(double) * nb);
     Ly.hist = Ly.bins + bins_bytes;
     size_t nh = R->record ? (size_t)R->n_batches * (size_t)R->pop : 0;
     size_t hist_bytes = R->record ? align256(sizeof(uint64_t) * nh) + align256(sizeof(int) * nh) : 0;
@@ -1241,6 +1320,13 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.unis = (const scn_universe*)(base + o_unis);
     P.boxes = (const scn_box*)(base + o_boxes);
     P.surfs = (const scn_surface*)(base + o_surfs);
+    P.collision = pr->collision;
+    P.max_split = pr->max_split;
+    P.ww_n = pr->ww_n;
+    P.ww_map = (const scn_clerk*)(base + o_wwmap);
+    P.ww_lower = (const double*)(base + o_wwlo);
+    P.ww_upper = (const double*)(base + o_wwup);
+    P.ww_survival = pr->ww_survival;
     P.any_rot = any_rotated(pr);
     P.n_track = (int)tgidx.size();
     P.gtrack_off = (const int*)(base + o_tgoff);
@@ -1265,8 +1351,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     carve_live(base + Ly.queue, depth, R->pop, D.q[0]);
     carve_live(base + Ly.queue + Ly.queue_bytes, depth, R->pop, D.q[1]);
     D.bin = (double*)(base + Ly.bins);
-    D.csum = (double*)(base + Ly.bins + align256(sizeof(double) * nb));
-    D.csum2 = (double*)(base + Ly.bins + 2 * align256(sizeof(double) * nb));
+    D.csum = (double*)(base + Ly.bins + align256(sizeof(double) * nb * window));
+    D.csum2 = (double*)(base + Ly.bins + align256(sizeof(double) * nb * window) + align256(sizeof(double) * nb));
     D.hist_rng = R->record ? (uint64_t*)(base + Ly.hist) : nullptr;
     D.hist_coll = R->record ? (int*)(base + Ly.hist + align256(sizeof(uint64_t) * nh)) : nullptr;
     D.probG = probG.empty() ? nullptr : (const double*)(base + o_probG);
@@ -1291,6 +1377,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     D.pop = R->pop;
     D.n_batches = R->n_batches;
     D.buffer = R->buffer > 0 ? R->buffer : DEFAULT_BUFFER;
+    D.window = window;
     SlotState st{};
     st.running = 1;
     auto undo = [&](int code) {
@@ -1312,6 +1399,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     h.bytes = Ly.total;
     h.depth = depth;
     h.geo = needs_full_geometry(pr);
+    h.ww = pr->ww_n > 0;
     h.n_bins = pr->n_bins;
     h.pop = R->pop;
     h.n_batches = R->n_batches;

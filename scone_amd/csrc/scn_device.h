@@ -48,6 +48,13 @@ struct DevProblem {
     const scn_surface* surfs;   // surfaces of cells and of a non-box root border
     int any_rot;         // some universe is rotated: lower-level directions differ from the global one
     int n_track;         // track-length clerks present (scored by the general kernels only)
+    int collision;       // SCN_COLL_STD / SCN_COLL_IMP
+    int max_split;       // neutronMGimp maxSplit
+    int ww_n;            // weight-window bins (0: none; general kernels only)
+    const scn_clerk* ww_map;
+    const double* ww_lower;
+    const double* ww_upper;
+    double ww_survival;
     const int* gtrack_off;   // [nG+1] track clerks whose energy filter admits group g
     const int* gtrack_idx;
 };
@@ -90,6 +97,7 @@ struct Pt {
     uint64_t rng;
     int brood, nsite, coll;
     int sp, lane;      // fixed source: depth of the lane's LIFO buffer of secondaries, lane index
+    int split;         // neutronMGimp splitCount (0 for a particle taken from a bank or buffer)
     double k[4];       // keffImplicitClerk scores of this history
     double track;      // trackingCache(1)%xs
     int fate;
@@ -1254,6 +1262,7 @@ struct Sites {
     int* G;
     int* mat;
     int* uid;
+    int* coll;        // collisionN of the site (neutronMGimp keeps the parent's; 0 otherwise)
     uint64_t* key;    // broodID << 24 | emission index
     uint32_t* order;  // identity, sorted along with the keys
     int* count;
@@ -1404,6 +1413,7 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL, GEO>& p, doub
                     S.G[slot] = G_out;
                     S.mat[slot] = mat;
                     S.uid[slot] = p.uid;
+                    S.coll[slot] = (P.collision == SCN_COLL_IMP) ? p.coll : 0;
                     S.key[slot] = ((uint64_t)p.brood << 24) | (uint64_t)p.nsite;
                     S.order[slot] = (uint32_t)slot;
                 } else {
@@ -1450,6 +1460,52 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL, GEO>& p, doub
         raise_error(st, SCN_ERR_UNSUPPORTED, -3);
         p.dead = 1;
         return 0;
+    }
+    if (GEO && P.ww_n > 0 && !p.dead) {
+        // neutronMGimp%cutoffs with weight windows (neutronMGimp_class.f90 split / russianRoulette,
+        // weightWindowsField_class.f90 atP): the window of the particle's state bin
+        const scn_clerk& M = *P.ww_map;
+        int bin = 1;
+        for (int k = 0; k < M.n_maps; ++k) {
+            const int b = map1d(P, M.maps[k], p.r[0], p.mat);
+            if (b == 0) {
+                bin = 0;
+                break;
+            }
+            bin = bin + (b - 1) * M.multi[k];
+        }
+        double minW = 0.0, maxW = 0.0, avW = 0.0;
+        if (bin > 0) {
+            minW = P.ww_lower[bin - 1];
+            maxW = P.ww_upper[bin - 1];
+            avW = minW * P.ww_survival;
+        }
+        if (p.w > maxW && maxW != 0.0 && p.split < P.max_split) {
+            int mult = (int)ceil(p.w / maxW);
+            if (mult + p.split > P.max_split) mult = P.max_split - p.split + 1;
+            const double wc = p.w / mult;
+            if (S.stride == 0 || p.sp + (mult - 1) > S.depth) {   // thisCycle%detain beyond the buffer
+                raise_error(st, S.stride == 0 ? SCN_ERR_UNSUPPORTED : SCN_ERR_OVERFLOW, -4);
+                p.dead = 1;
+                return 0;
+            }
+            for (int i = 0; i < mult - 1; ++i) {
+                const size_t e = (size_t)(p.sp++) * S.stride + p.lane;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    S.r[(size_t)k * S.cap + e] = p.r[0][k];
+                    S.u[(size_t)k * S.cap + e] = p.u[k];
+                }
+                S.w[e] = wc;
+                S.t[e] = p.t;
+                S.G[e] = p.G;
+            }
+            p.split = p.split + mult;
+            p.w = p.w / mult;
+        } else if (p.w < minW) {
+            if (scn_rng_get(&p.rng) < (1.0 - p.w / avW)) p.dead = 1;
+            else p.w = avW;
+        }
     }
     p.coll += 1;
     if (C == 2) {

@@ -94,6 +94,7 @@ __device__ inline void load_live(const Live& L, int cap, int i, Pt<NL, GEO>& p)
     p.brood = L.brood[i];
     p.nsite = L.nsite[i];
     p.coll = L.coll[i];
+    p.split = 0;
     p.dead = 0;
     p.fate = FATE_NO;
     p.track = 0.0;
@@ -215,6 +216,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
                     p.k[0] = p.k[1] = p.k[2] = p.k[3] = 0.0;
                     p.dead = 0;
                     p.fate = FATE_NO;
+                    p.split = 0;
                     p.track = 0.0;
                     place_coord(P, p, st);
                 } else {
@@ -321,7 +323,7 @@ __global__ void k_gather_sites(Sites S, const uint32_t* __restrict__ order, int 
     q.matIdx = S.mat[s];
     q.cellIdx = 0;
     q.uniqueID = S.uid[s];
-    q.collisionN = 0;
+    q.collisionN = S.coll[s];
     q.broodID = (int)(keys_sorted[j] >> 24);
     q.pad1 = 0;
     out[j] = q;
@@ -1022,6 +1024,11 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
         delete E;
         return SCN_ERR_ARG;
     }
+    if (pr->ww_n > 0) {   // the eigenvalue engine has no per-history buffer for split particles
+        g_create_err = "weight windows are supported in fixed-source (scn_batch_*) runs only";
+        delete E;
+        return SCN_ERR_UNSUPPORTED;
+    }
     {
         int d = problem_depth(pr);
         if (d < 1 || d > SCN_MAX_NEST) {
@@ -1094,6 +1101,9 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
     P.surfs = d_surfs;
+    P.collision = pr->collision;
+    P.max_split = pr->max_split;
+    P.ww_n = 0;   // weight windows are a fixed-source feature (rejected for eigenvalue runs)
     P.any_rot = any_rotated(pr);
     P.n_track = (int)tgidx.size();
     P.gtrack_off = d_tgoff;
@@ -1119,6 +1129,7 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= dalloc(E, &S.G, S.cap);
     if (!rc) rc |= dalloc(E, &S.mat, S.cap);
     if (!rc) rc |= dalloc(E, &S.uid, S.cap);
+    if (!rc) rc |= dalloc(E, &S.coll, S.cap);
     if (!rc) rc |= dalloc(E, &S.key, S.cap);
     if (!rc) rc |= dalloc(E, &S.order, S.cap);
     if (!rc) rc |= dalloc(E, &S.count, 1);

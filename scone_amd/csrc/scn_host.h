@@ -61,6 +61,17 @@ inline const char* check_cell_tables(const scn_problem* pr)
     for (int c = 0; c < pr->n_clerks; ++c)
         if (pr->clerks[c].kind != SCN_CLERK_COLLISION && pr->clerks[c].kind != SCN_CLERK_TRACK)
             return "unknown clerk kind";
+    if (pr->collision != SCN_COLL_STD && pr->collision != SCN_COLL_IMP) return "unknown collision processor";
+    if (pr->ww_n < 0 || (pr->ww_n > 0 && (!pr->ww_map || !pr->ww_lower || !pr->ww_upper)))
+        return "weight-window tables missing";
+    if (pr->ww_n > 0) {
+        if (pr->ww_map->n_maps < 0 || pr->ww_map->n_maps > 3) return "weight-window map";
+        int64_t n = 1;
+        for (int k = 0; k < pr->ww_map->n_maps; ++k) n *= pr->ww_map->maps[k].nbins;
+        if (n != pr->ww_n) return "weight-window bins do not match the map";
+        if (pr->collision != SCN_COLL_IMP) return "weight windows need neutronMGimp";
+        if (pr->max_split < 1) return "maxSplit must be at least 1";
+    }
     for (int u = 0; u < pr->n_universes; ++u) {
         const scn_universe& U = pr->universes[u];
         if (U.type < SCN_UNI_ROOT || U.type > SCN_UNI_CELL) return "unknown universe type";
@@ -91,6 +102,7 @@ inline int needs_full_geometry(const scn_problem* pr)
 {
     for (int c = 0; c < pr->n_clerks; ++c)
         if (pr->clerks[c].kind == SCN_CLERK_TRACK) return 1;   // path scoring lives in the general kernels
+    if (pr->ww_n > 0) return 1;                                // so do weight windows
     for (int i = 0; i < pr->n_universes; ++i) {
         const scn_universe& U = pr->universes[i];
         if (U.type == SCN_UNI_CELL || U.rotated || U.global) return 1;

@@ -267,8 +267,10 @@ class Batch:
                 raise Busy(msg)
             raise EngineError(f"batch engine error {rc}: {msg}")
 
-    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0) -> int:
-        """Admit a solve; raises Busy if it does not fit now.  Returns its slot."""
+    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0,
+            window=1) -> int:
+        """Admit a solve; raises Busy if it does not fit now.  Returns its slot.  `window` batches of
+        the solve are transported together in each cycle (closed in order: same results)."""
         R = T.FixedRequest()
         R.problem = C.pointer(comp.problem)
         R.source = C.pointer(src.source)
@@ -279,6 +281,7 @@ class Batch:
         R.stop_rel = stop_rel
         R.record = 1 if record else 0
         R.buffer = buffer
+        R.window = window
         slot = C.c_int32(-1)
         self._check(self.L.scn_batch_add(self.h, C.byref(R), C.byref(slot)))
         self._held[slot.value] = (comp, src, pop, record)

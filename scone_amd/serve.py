@@ -2,10 +2,12 @@
 
 Requests are fixed-source solves.  A request is admitted as soon as a slot and enough of the HBM
 budget are free (scn_batch_add refuses it otherwise and it waits, first come first served).
-Every engine cycle advances all resident solves by one batch together, so the work of requests
-that arrive while others run overlaps on the device.  A solve leaves when its batches are done or
-when the relative standard deviation of its stop bin reaches the target; its slot and memory
-return to the pool at once and the next waiting request takes them.
+Every engine cycle advances all resident solves by `window` batches each (one launch transports
+them together and closes them in order), so the work of requests that arrive while others run
+overlaps on the device, and a request's latency is a few cycles rather than one cycle per batch.
+A solve leaves when its batches are done or when the relative standard deviation of its stop bin
+reaches the target; its slot and memory return to the pool at once and the next waiting request
+takes them.
 
 Each solve's histories depend only on its own seed and tables, so its result is the same alone
 and in any mix (tallies to summation order).
@@ -33,6 +35,7 @@ class Request:
     stop_rel: float = 0.0
     record: bool = False
     buffer: int = 0         # depth of a history's buffer of secondaries (0: the package default, 50)
+    window: int = 1         # batches transported together per engine cycle (closed in order)
     arrival: float = 0.0    # seconds (wall mode) or cycle index (cycle mode)
     # filled by the server
     admitted: float = -1.0
@@ -60,7 +63,8 @@ class Server:
             rq = waiting[0]
             try:
                 rq.slot = self.B.add(rq.comp, rq.src, rq.pop, rq.n_batches, rq.seed, stop_bin=rq.stop_bin,
-                                     stop_rel=rq.stop_rel, record=rq.record, buffer=rq.buffer)
+                                     stop_rel=rq.stop_rel, record=rq.record, buffer=rq.buffer,
+                                     window=rq.window)
             except Busy:
                 self.busy_rejections += 1
                 return

@@ -33,6 +33,7 @@ INF = 2.0 ** 63
 SURF_TOL = 1.0e-12
 FP_REL_TOL = 1.0e-7
 TRANSPORT_ST, TRANSPORT_DT, TRANSPORT_HT = 1, 2, 3
+COLL_STD, COLL_IMP = 0, 1
 UNI_ROOT, UNI_LAT, UNI_PIN, UNI_CELL = 1, 2, 3, 4
 SURF_APLANE, SURF_PLANE, SURF_SPHERE, SURF_CYLINDER, SURF_QUADRIC, SURF_BOX, SURF_TRUNC_CYLINDER = 1, 2, 3, 4, 5, 6, 7
 OVERLAP_MAT = 2147483645
@@ -107,7 +108,10 @@ class Problem(C.Structure):
                 ("int_data", C.POINTER(C.c_int32)), ("mg", MGData), ("n_clerks", C.c_int32),
                 ("clerks", C.POINTER(Clerk)), ("n_bins", C.c_int64), ("norm_bin", C.c_int64),
                 ("norm_value", C.c_double), ("ht_cutoff", C.c_double), ("n_surfaces", C.c_int32),
-                ("surfaces", C.POINTER(Surface))]
+                ("surfaces", C.POINTER(Surface)), ("collision", C.c_int32), ("max_split", C.c_int32),
+                ("ww_n", C.c_int32), ("pad1", C.c_int32), ("ww_map", C.POINTER(Clerk)),
+                ("ww_lower", C.POINTER(C.c_double)), ("ww_upper", C.POINTER(C.c_double)),
+                ("ww_survival", C.c_double)]
 
 
 class EigenSettings(C.Structure):
@@ -139,7 +143,8 @@ class Source(C.Structure):
 class FixedRequest(C.Structure):
     _fields_ = [("problem", C.POINTER(Problem)), ("source", C.POINTER(Source)), ("pop", C.c_int32),
                 ("n_batches", C.c_int32), ("seed", C.c_int64), ("stop_bin", C.c_int64),
-                ("stop_rel", C.c_double), ("record", C.c_int32), ("buffer", C.c_int32)]
+                ("stop_rel", C.c_double), ("record", C.c_int32), ("buffer", C.c_int32), ("window", C.c_int32),
+                ("pad0", C.c_int32)]
 
 
 class FixedResult(C.Structure):
@@ -446,6 +451,19 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     else:
         raise NotImplementedError(f"{ttype} is outside the engine's scope")
     P.st_cache = int(top.get("cache", 1))
+
+    # --- collision processor (collisionProcessorFactory_func.f90:36-72, MG neutrons)
+    cop = (inp.get("collisionOperator", {}) or {}).get("neutronMG", {"type": "neutronMGstd"})
+    ctype = cop.get("type", "neutronMGstd")
+    P.max_split = 1000
+    if ctype == "neutronMGstd":
+        P.collision = COLL_STD
+    elif ctype == "neutronMGimp":
+        # neutronMGimp_class.f90 init: maxSplit 1000, weightWindows false
+        P.collision = COLL_IMP
+        P.max_split = int(cop.get("maxSplit", 1000))
+    else:
+        raise NotImplementedError(f"collision processor {ctype} is outside the engine's scope")
 
     # --- materials (materialMenu order = order of the materials dictionary)
     nd = inp["nuclearData"]
@@ -771,6 +789,86 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     # --- tallies
     if inp.get("type") == "fixedSourcePhysicsPackage" and tally_key == "activeTally":
         tally_key = "tally"
+    def compile_map(md, K):
+        """A tally map (tallyMapFactory_func.f90: spaceMap, materialMap, multiMap of them) into
+        K.maps / K.multi / K.n_maps; returns the bin shape and the axes for printing."""
+        if md["type"] == "multiMap":
+            subs = [md[n] for n in md["maps"]]
+        else:
+            subs = [md]
+        if len(subs) > 3:
+            raise NotImplementedError("multiMap with more than 3 maps")
+        K.n_maps = len(subs)
+        mul = 1
+        shape, axes = [], []
+        for k, sd in enumerate(subs):
+            M = Map1D()
+            if sd["type"] == "spaceMap":
+                M.type = MAP_SPACE
+                M.axis = {"x": 0, "y": 1, "z": 2}[sd["axis"]]
+                if sd["grid"] == "lin":
+                    mn, mx, N = float(sd["min"]), float(sd["max"]), int(sd["N"])
+                    M.grid_type = GRID_LIN
+                    M.nbins = N
+                    M.data = len(real_data)
+                    real_data.extend([mn, (mx - mn) / N])
+                    edges = [mn] + [mn + (i - 1) * ((mx - mn) / N) for i in range(2, N + 2)]
+                else:
+                    edges = [float(x) for x in sd["bins"]]
+                    M.grid_type = GRID_UNSTRUCT
+                    M.nbins = len(edges) - 1
+                    M.data = len(real_data)
+                    real_data.extend(edges)
+                axes.append(("space", sd["axis"], edges))
+            elif sd["type"] == "materialMap":
+                M.type = MAP_MATERIAL
+                mlist = list(sd["materials"])
+                undef = sd.get("undefBin", "false") in ("yes", "y", "true", "TRUE", "T")
+                nb = len(mlist) + (1 if undef else 0)
+                default = len(mlist) + 1 if undef else 0
+                table = [default] * (len(mats) + 2)
+                for bi, mn in enumerate(mlist):
+                    table[mat_idx[mn]] = bi + 1
+                M.nbins = nb
+                M.data = len(int_data)
+                int_data.extend(table)
+                axes.append(("material", mlist, undef))
+            else:
+                raise NotImplementedError(f"tally map {sd['type']} is outside the engine's scope")
+            K.maps[k] = M
+            K.multi[k] = mul
+            mul *= M.nbins
+            shape.append(M.nbins)
+        return shape, axes
+
+    # --- weight windows (neutronMGimp 'weightWindows', weightWindowsField_class.f90 init)
+    ww = None
+    if P.collision == COLL_IMP and _bool(cop.get("weightWindows", 0), "weightWindows"):
+        if inp.get("type") == "eigenPhysicsPackage":
+            raise NotImplementedError("weight windows in eigenvalue runs are outside the engine's scope")
+        vr = inp.get("varianceReduction")
+        if not vr or vr.get("type") != "weightWindowsField":
+            raise ValueError("weightWindows needs 'varianceReduction { type weightWindowsField; file ...; }'")
+        path = vr["file"]
+        if not os.path.isabs(path):
+            path = os.path.join(base_dir, path)
+        wd = dictio.read(path)
+        K = Clerk()
+        shape, _ = compile_map(wd["map"], K)
+        n = int(np.prod(shape))
+        lower = np.ascontiguousarray([float(x) for x in wd["wLower"]], dtype=np.float64)
+        upper = np.ascontiguousarray([float(x) for x in wd["wUpper"]], dtype=np.float64)
+        surv = float(wd["constSurvival"])
+        if len(lower) != n or len(upper) != n:
+            raise ValueError("Size of weight window net does not match the map provided")
+        if surv <= 1.0:
+            raise ValueError("Survival constant must be bigger than one")
+        if np.any(lower * surv > upper):
+            raise ValueError("Russian Roulette survival weight must be smaller than the weight window upper weight")
+        if np.any(lower < 0.0) or np.any(upper < 0.0):
+            raise ValueError("Some weights in the weight windows provided are negative")
+        ww = (K, n, lower, upper, surv)
+
     tal = inp.get(tally_key, {}) or {}
     clerks = []
     addr = 0
@@ -796,56 +894,7 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
             K.gtop, K.glow = int(fd["Gtop"]), int(fd["Glow"])
             if K.gtop > K.glow:
                 raise ValueError("Gtop larger than Glow")
-        shape = []
-        axes = []
-        if "map" in cd:
-            md = cd["map"]
-            if md["type"] == "multiMap":
-                subs = [md[n] for n in md["maps"]]
-            else:
-                subs = [md]
-            if len(subs) > 3:
-                raise NotImplementedError("multiMap with more than 3 maps")
-            K.n_maps = len(subs)
-            mul = 1
-            for k, sd in enumerate(subs):
-                M = Map1D()
-                if sd["type"] == "spaceMap":
-                    M.type = MAP_SPACE
-                    M.axis = {"x": 0, "y": 1, "z": 2}[sd["axis"]]
-                    if sd["grid"] == "lin":
-                        mn, mx, N = float(sd["min"]), float(sd["max"]), int(sd["N"])
-                        M.grid_type = GRID_LIN
-                        M.nbins = N
-                        M.data = len(real_data)
-                        real_data.extend([mn, (mx - mn) / N])
-                        edges = [mn] + [mn + (i - 1) * ((mx - mn) / N) for i in range(2, N + 2)]
-                    else:
-                        edges = [float(x) for x in sd["bins"]]
-                        M.grid_type = GRID_UNSTRUCT
-                        M.nbins = len(edges) - 1
-                        M.data = len(real_data)
-                        real_data.extend(edges)
-                    axes.append(("space", sd["axis"], edges))
-                elif sd["type"] == "materialMap":
-                    M.type = MAP_MATERIAL
-                    mlist = list(sd["materials"])
-                    undef = sd.get("undefBin", "false") in ("yes", "y", "true", "TRUE", "T")
-                    nb = len(mlist) + (1 if undef else 0)
-                    default = len(mlist) + 1 if undef else 0
-                    table = [default] * (len(mats) + 2)
-                    for bi, mn in enumerate(mlist):
-                        table[mat_idx[mn]] = bi + 1
-                    M.nbins = nb
-                    M.data = len(int_data)
-                    int_data.extend(table)
-                    axes.append(("material", mlist, undef))
-                else:
-                    raise NotImplementedError(f"tally map {sd['type']} is outside the engine's scope")
-                K.maps[k] = M
-                K.multi[k] = mul
-                mul *= M.nbins
-                shape.append(M.nbins)
+        shape, axes = compile_map(cd["map"], K) if "map" in cd else ([], [])
         resp = list(cd["response"])
         if len(resp) > 8:
             raise NotImplementedError("more than 8 responses")
@@ -899,6 +948,15 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     P.universes = C.cast(uni_arr, C.POINTER(Universe))
     P.n_boxes = len(boxes)
     P.border_box = universes[root_idx - 1].border
+    if ww is not None:
+        K, n, lower, upper, surv = ww
+        ww_arr = (Clerk * 1)(K)
+        out.hold(ww_arr)
+        P.ww_n = n
+        P.ww_map = C.cast(ww_arr, C.POINTER(Clerk))
+        P.ww_lower = _ptr(out.hold(lower), C.c_double)
+        P.ww_upper = _ptr(out.hold(upper), C.c_double)
+        P.ww_survival = surv
     if surfaces:
         surf_arr = (Surface * len(surfaces))(*surfaces)
         out.hold(surf_arr)

@@ -235,3 +235,62 @@ def test_batch_problems_compile_and_run_small():
         res, done, _ = O.fixed_run(comp, src, pop, 2, inp["seed"], threads=4)
         assert done == 2 and res[0, 0] > 0
     assert kinds == {"pointSource", "materialSource"}
+
+
+# ---- neutronMGimp weight windows (neutronMGimp_class.f90 cutoffs/split/russianRoulette) ----------
+def _ww(tmp_path, lower, upper, survival=2.0, **kw):
+    path = tmp_path / "ww"
+    path.write_text(configs.ww_file_text(lower, upper, survival))
+    inp, mg = configs.ww_medium(str(path), **kw)
+    return compiled(inp, mg)
+
+
+def test_weight_windows_that_never_act_change_nothing(tmp_path):
+    """Windows wider than any weight: no split, no roulette (whose draw is the only random number
+    the windows use), so every history and score equals the plain neutronMGstd run."""
+    pop, cycles = 500, 3
+    comp0, src0 = compiled(*configs.ww_medium(None, pop=pop, cycles=cycles))
+    comp1, src1 = _ww(tmp_path, [1e-300, 1e-300], [1e300, 1e300], pop=pop, cycles=cycles)
+    assert comp1.problem.ww_n == 2 and comp1.problem.collision == T.COLL_IMP
+    a, _, ha = O.fixed_run(comp0, src0, pop, cycles, 17, hist=True)
+    b, _, hb = O.fixed_run(comp1, src1, pop, cycles, 17, hist=True)
+    assert np.array_equal(a, b)
+    assert np.array_equal(ha["rng_end"], hb["rng_end"]) and np.array_equal(ha["n_coll"], hb["n_coll"])
+
+
+@pytest.mark.parametrize("top", ["transportOperatorST", "transportOperatorDT"])
+def test_weight_windows_split_and_roulette_are_unbiased(tmp_path, top):
+    """Active windows (split above 1.3, roulette below 0.6 to 1.2 on x < 0; split above 2.0,
+    roulette below 0.8 to 1.6 on x > 0): the flux per source neutron stays the closed form
+    (T - M.S)^-1 q, and the histories really split (more collisions) and roulette (some die)."""
+    pop, cycles = 4000, 20
+    comp0, src0 = compiled(*configs.ww_medium(None, pop=pop, cycles=cycles, transport=top))
+    comp1, src1 = _ww(tmp_path, [0.6, 0.8], [1.3, 2.0], pop=pop, cycles=cycles, transport=top)
+    phi = configs.ww_medium_flux()
+    expect = np.concatenate([[phi.sum()], phi]) * pop
+    a, _, ha = O.fixed_run(comp0, src0, pop, cycles, 17, hist=True, threads=4)
+    b, _, hb = O.fixed_run(comp1, src1, pop, cycles, 17, hist=True, threads=4)
+    for res in (a, b):
+        z = (res[:, 0] - expect) / res[:, 1]
+        assert np.all(np.abs(z) < 3.5), z
+    assert hb["n_coll"].sum() > 1.1 * ha["n_coll"].sum()      # split copies are transported too
+    assert not np.array_equal(ha["rng_end"], hb["rng_end"])
+
+
+def test_weight_window_split_limit_and_buffer(tmp_path):
+    # maxSplit caps the copies; more copies than the buffer holds is the package's error
+    pop = 300
+    comp, src = _ww(tmp_path, [0.01, 0.01], [0.05, 0.05], pop=pop, cycles=1, max_split=3)
+    res, done, _ = O.fixed_run(comp, src, pop, 1, 17, hist=True, buffer=200)
+    assert done == 1 and res[0, 0] > 0
+    comp, src = _ww(tmp_path, [0.01, 0.01], [0.05, 0.05], pop=pop, cycles=1, max_split=1000)
+    with pytest.raises(RuntimeError, match="Run out of space"):
+        O.fixed_run(comp, src, pop, 1, 17, buffer=5)
+    inp, mg = configs.ww_medium(str(tmp_path / "ww"))
+    inp["type"] = "eigenPhysicsPackage"
+    with pytest.raises(NotImplementedError, match="eigenvalue"):
+        T.compile_problem(inp, mg)
+    (tmp_path / "bad").write_text(configs.ww_file_text([1.0, 1.0], [1.5, 1.5], survival=2.0))
+    inp, mg = configs.ww_medium(str(tmp_path / "bad"))
+    with pytest.raises(ValueError, match="Russian Roulette survival weight"):
+        T.compile_problem(inp, mg)

@@ -65,3 +65,23 @@ def test_eigen_run_bit_exact(top):
     assert list(R1.k_implicit) == list(R2.k_implicit)
     ok, worst = rel_close(ba1[:, 0], ba2[:, 0], 1e-12)
     assert ok, worst
+
+
+def test_neutron_mg_imp_cycle_bit_exact():
+    """neutronMGimp: the sites keep the colliding particle's collision count, on both sides."""
+    print("case neutronMGimp cycle", flush=True)
+    from scone_amd import configs, tables as T
+    pop = 3000
+    inp, mg = configs.pin_lattice_7g(pop=pop, transport="transportOperatorDT")
+    inp["collisionOperator"] = {"neutronMG": {"type": "neutronMGimp"}}
+    comp = T.compile_problem(inp, mg)
+    bank = O.fission_source(comp, pop, 4242)
+    eng = Engine(comp, pop)
+    o_out, o_bins, o_ks, _ = O.eigen_cycle(comp, bank, 4243, 1.01)
+    g_out, g_bins, g_ks = eng.eigen_cycle(bank, 4243, 1.01)
+    assert_banks_identical(o_out, g_out)
+    assert o_out["collisionN"].max() > 0
+    o2, _, _, _ = O.eigen_cycle(comp, o_out, 4244, 1.01)
+    g2, _, _ = eng.eigen_cycle(g_out, 4244, 1.01)
+    assert_banks_identical(o2, g2)
+    eng.close()

@@ -167,3 +167,36 @@ def test_sort_by_brood_with_duplicates():
     out = O.norm_size_repr(b, len(broods), 5)       # excess 0: sort only
     assert list(out["broodID"]) == [1, 1, 2, 2, 2, 3, 3]
     assert list(out["G"]) == [2, 5, 1, 3, 7, 4, 6]
+
+
+def test_neutron_mg_imp_sites_keep_the_collision_count():
+    """neutronMGimp without weight windows differs from neutronMGstd only in the fission sites'
+    collisionN (neutronMGimp_class.f90 implicit keeps the colliding particle's count): same RNG
+    use, same bank otherwise, same scores."""
+    import binding as O
+    from scone_amd import configs, tables as T
+    pop = 800
+    inp, mg = configs.pin_lattice_7g(pop=pop, transport="transportOperatorDT")
+    std = T.compile_problem(inp, mg)
+    inp["collisionOperator"] = {"neutronMG": {"type": "neutronMGimp", "maxSplit": 50}}
+    imp = T.compile_problem(inp, mg)
+    assert imp.problem.collision == T.COLL_IMP and imp.problem.max_split == 50
+    bank = O.fission_source(std, pop, 77)
+    a, ab, aks, _ = O.eigen_cycle(std, bank, 99, 1.0)
+    b, bb, bks, _ = O.eigen_cycle(imp, bank, 99, 1.0)
+    assert len(a) == len(b)
+    for f in a.dtype.names:
+        if f != "collisionN":
+            assert np.array_equal(a[f], b[f]), f
+    assert np.all(a["collisionN"] == 0)
+    assert b["collisionN"].min() >= 0 and b["collisionN"].max() > 3
+    assert np.array_equal(ab, bb) and np.array_equal(aks, bks)
+    # sites of one history come in emission order (stable broodID sort), so their counts do not
+    # decrease; the history's first collision gives 0
+    for br in np.unique(b["broodID"]):
+        cn = b["collisionN"][b["broodID"] == br]
+        assert np.all(np.diff(cn) >= 0), br
+    assert (b["collisionN"] == 0).any()
+    inp["collisionOperator"]["neutronMG"]["weightWindows"] = 1
+    with pytest.raises(NotImplementedError):
+        T.compile_problem(inp, mg)

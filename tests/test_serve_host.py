@@ -15,11 +15,11 @@ class FakeBatch:
         self.used = 0
         self.log = []
 
-    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0):
+    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0, window=1):
         if self.used + pop > self.budget or None not in self.slots:
             raise Busy("full")
         s = self.slots.index(None)
-        self.slots[s] = [pop, n_batches, 0, seed]
+        self.slots[s] = [pop, n_batches, 0, seed, max(1, window)]
         self.used += pop
         self.log.append(("add", seed))
         return s
@@ -27,7 +27,7 @@ class FakeBatch:
     def cycle(self, n=1):
         for v in self.slots:
             if v is not None and v[2] < v[1]:
-                v[2] += 1
+                v[2] = min(v[1], v[2] + v[4])
         return sum(1 for v in self.slots if v is not None and v[2] < v[1])
 
     def states(self):
@@ -59,3 +59,13 @@ def test_fifo_admission_and_latency():
     assert [r.result.n_batches for r in reqs] == [3, 2, 1, 1, 2]
     s = latency_summary(reqs)
     assert s["n"] == 5 and s["p50_s"] <= s["p99_s"] <= s["max_s"]
+
+
+def test_window_shortens_latency_in_cycles():
+    B = FakeBatch(slots=4, budget=1000)
+    reqs = [Request(None, None, 10, 10, seed=0, arrival=0, window=1),
+            Request(None, None, 10, 10, seed=1, arrival=0, window=5),
+            Request(None, None, 10, 10, seed=2, arrival=0, window=4)]
+    Server(B).run_cycles(reqs)
+    assert [r.done for r in reqs] == [10, 2, 3]
+    assert [r.result.n_batches for r in reqs] == [10, 10, 10]
