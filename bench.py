@@ -4,8 +4,9 @@
 Default workload (the batched fixed-source benchmark): 512 independent fixed-source problems
 (1D-3D boxes with 1-8 material regions, 2-70 groups, 1e3-1e6 histories in 10 batches, point or
 volume sources, a 1e3-1e5-bin mesh flux tally each; synthetic seeded data), FP64, delta tracking.
-A step is one cycle: one batch of every resident problem.  W warmup cycles, then K timed cycles
-(W + K = batches per problem).
+A step is one batch of every resident problem: W warmup steps, then K timed steps (W + K = batches
+per problem; the default K = 10 times exactly the 10-batch job of every problem).  An engine cycle transports `window` steps (every problem's next `window` batches in
+one launch, closed in order); the autotuner picks the window among the divisors of W and K.
   batched  all 512 problems resident in one batch engine, cycles replayed from a hipGraph, launch
            shape from the autotuner
   plain    the same engine holding one problem at a time (each problem's batches run alone)
@@ -21,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -95,29 +97,38 @@ def batch_problems(n, transport, scale=1.0):
     return out
 
 
-def budget_for(problems, n_batches, slack=1.25):
-    """Device bytes the problems need resident at once (queues, score memory, tables)."""
+def budget_for(problems, n_batches, slack=1.25, window=1):
+    """Device bytes the problems need resident at once (queues, score memory with a BIN copy per
+    batch of the window, tables)."""
     tot = 0
     for (comp, _, pop, _, _) in problems:
         depth = 3
-        live = pop * (8 * (3 * depth + 7) + 4 * (depth + 6))
-        tot += 2 * live + 24 * int(comp.problem.n_bins) + (1 << 20)
+        live = pop * (8 * (3 * depth + 7) + 4 * (depth + 7))
+        tot += 2 * live + 8 * (window + 2) * int(comp.problem.n_bins) + (1 << 20)
     return int(tot * slack) + (256 << 20)
 
 
+def _engine_settings(settings):
+    return {k: v for k, v in settings.items() if k != "window"}
+
+
 def run_batched(problems, transport, settings, warmup, steps, budget):
+    """A step is one batch of every problem; an engine cycle carries `window` steps."""
     from scone_amd.engine import Batch
-    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=1, **settings)
+    w = settings.get("window", 1)
+    assert warmup % w == 0 and steps % w == 0
+    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=1,
+              **_engine_settings(settings))
     t = time.perf_counter()
-    slots = [B.add(c, s, pop, warmup + steps, seed) for (c, s, pop, _, seed) in problems]
+    slots = [B.add(c, s, pop, warmup + steps, seed, window=w) for (c, s, pop, _, seed) in problems]
     t_admit = time.perf_counter() - t
     t_start = time.perf_counter()
     if warmup:
-        B.cycle(warmup)
+        B.cycle(warmup // w)
     s0 = stats_dict(B.stats())
     t0 = time.perf_counter()
     cyc_t = []
-    for _ in range(steps):
+    for _ in range(steps // w):
         a = time.perf_counter()
         B.cycle(1)
         cyc_t.append(time.perf_counter() - a)
@@ -142,9 +153,11 @@ def run_batched(problems, transport, settings, warmup, steps, budget):
 def run_batched_kernel_timing(problems, transport, settings, budget, cycles=2):
     """Same workload without the graph, HIP events around every transport launch."""
     from scone_amd.engine import Batch
-    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=0, **settings)
+    w = settings.get("window", 1)
+    B = Batch(transport=transport, max_slots=len(problems), hbm_budget=budget, use_graph=0,
+              **_engine_settings(settings))
     for (c, s, pop, _, seed) in problems:
-        B.add(c, s, pop, cycles + 1, seed)
+        B.add(c, s, pop, (cycles + 1) * w, seed, window=w)
     B.cycle(1)
     s0 = stats_dict(B.stats())
     B.cycle(cycles)
@@ -162,13 +175,14 @@ def run_batched_kernel_timing(problems, transport, settings, budget, cycles=2):
 def run_plain(problems, transport, settings, warmup, steps):
     from scone_amd.engine import Batch
     B = Batch(transport=transport, max_slots=1, hbm_budget=budget_for(problems[:1], 1) + max(
-        budget_for([p], 1) for p in problems), use_graph=1, **settings)
+        budget_for([p], 1, window=settings.get("window", 1)) for p in problems), use_graph=1,
+        **_engine_settings(settings))
     busy = 0.0
     hist = 0
     lat = []
     results = {}
     for i, (c, s, pop, _, seed) in enumerate(problems):
-        slot = B.add(c, s, pop, warmup + steps, seed)
+        slot = B.add(c, s, pop, warmup + steps, seed, window=settings.get("window", 1))
         t = time.perf_counter()
         B.run_all()
         busy += time.perf_counter() - t
@@ -265,6 +279,33 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
 # ---------------------------------------------------------------------------------------------
 # eigenvalue (plain engine, 17x17 lattice)
 # ---------------------------------------------------------------------------------------------
+def run_core3d(transport, pop=10_000_000, inactive=2, active=3, seed=5):
+    """3D nested core (pin -> 17x17 assembly -> 15x15 core with a 20 cm reflector, 400 cm tall,
+    70 groups, 200x200x50x70 = 1.4e8-bin mesh flux tally), 1e7 histories per cycle: a bounded
+    sample of its 30 + 100 cycles (inactive cycles untimed, active cycles timed with tallies)."""
+    from scone_amd import configs, tables as T
+    from scone_amd.engine import Engine
+    inp, mg = configs.core_3d(pop=pop, transport=transport)
+    comp = T.compile_problem(inp, mg)
+    eng = Engine(comp, pop)
+    bank = eng.fission_source(pop, seed)
+    eng.eigen_start(dict(pop=pop, inactive=inactive, active=active, seed=seed), bank)
+    eng.eigen_step(inactive)
+    s0 = eng.stats()
+    t0 = time.perf_counter()
+    done = eng.eigen_step(active)
+    wall = time.perf_counter() - t0
+    s1 = eng.stats()
+    R, _, _, _, _ = eng.eigen_finish()
+    eng.close()
+    assert done == active
+    d = diff(s0, s1)
+    return {"hist_per_s": d["n_histories"] / wall, "coll_per_s": d["n_collisions"] / wall,
+            "ms_per_cycle": wall / active * 1e3, "bins": int(comp.problem.n_bins),
+            "kernel_ms_per_launch": d["transport_kernel_ms"] / max(1, d["transport_launches"]),
+            "k_implicit": list(R.k_implicit), "cycles_timed": active, "pop": pop}
+
+
 def run_eigen(transport, pop, warmup, steps, seed=2):
     from scone_amd import configs, tables as T
     from scone_amd.engine import Engine
@@ -347,19 +388,21 @@ def batch_line(args):
     transport = T.TRANSPORT_DT if args.transport == "transportOperatorDT" else T.TRANSPORT_ST
     log(f"batch: compiling {args.problems} problems")
     problems = batch_problems(args.problems, args.transport, args.scale)
-    budget = budget_for(problems, args.warmup + args.steps)
+    g = math.gcd(args.warmup, args.steps) if args.warmup else args.steps
+    windows = tuple(w for w in (1, 2, 5, 10) if g % w == 0)
+    budget = budget_for(problems, args.warmup + args.steps, window=max(windows))
     tot_hist = sum(p[2] for p in problems)
-    log(f"batch: {tot_hist} histories per cycle, device budget {budget / 2**30:.2f} GiB")
+    log(f"batch: {tot_hist} histories per batch, device budget {budget / 2**30:.2f} GiB, windows {windows}")
     if args.tune:
         t = time.perf_counter()
         best, table = autotune.tune([(c, s, pop, seed) for (c, s, pop, _, seed) in problems], transport, budget,
-                                    log=log)
+                                    windows=windows, log=log)
         tune_s = time.perf_counter() - t
     else:
-        best, table, tune_s = {"tile": 1024, "k0": 8, "n_launch": 1}, [], 0.0
+        best, table, tune_s = {"tile": 256, "k0": 8, "n_launch": 1, "window": max(windows)}, [], 0.0
     log(f"batch: tuned {best} in {tune_s:.1f} s")
     bat, res_b = run_batched(problems, transport, best, args.warmup, args.steps, budget)
-    log(f"batch: batched {bat['hist_per_s']:.3e} histories/s, {bat['ms_per_step']:.2f} ms/cycle")
+    log(f"batch: batched {bat['hist_per_s']:.3e} histories/s, {bat['ms_per_step']:.2f} ms/step")
     kt = run_batched_kernel_timing(problems, transport, best, budget)
     log(f"batch: transport kernel {kt['kernel_ms_per_launch']:.3f} ms/launch, {kt['achieved_gbs']:.1f} GB/s")
     pla, res_p = (None, {})
@@ -377,13 +420,18 @@ def batch_line(args):
         e = run_eigen(args.transport, args.pop, 3, 10)
         eig = {k: e[k] for k in ("hist_per_s", "coll_per_s", "ms_per_step", "achieved_gbs", "kernel_ms_per_launch",
                                  "k_implicit")}
+    core = None
+    if args.c5:
+        log("core 3D leg")
+        core = run_core3d(args.transport)
+        log(f"core 3D: {core['hist_per_s']:.3e} histories/s, {core['ms_per_cycle']:.1f} ms/cycle")
     line = {
         "metric": METRIC, "value": bat["hist_per_s"], "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded problems and multigroup data)",
         "config": {"workload": "batched fixed source: 512 problems, 1D-3D boxes, 1-8 regions, 2-70 groups, "
                                "1e3-1e6 histories in 10 batches, point/volume sources, 1e3-1e5-bin mesh flux tallies",
-                   "problems": args.problems, "histories_per_cycle": tot_hist, "transport": args.transport[-2:],
+                   "problems": args.problems, "histories_per_step": tot_hist, "transport": args.transport[-2:],
                    "engine": "batched (persistent launch, hipGraph cycles, autotuned)", "launch_shape": best},
         "collisions_per_s": bat["coll_per_s"],
         "latency": {"batched_p50_s": bat["latency_s"], "batched_p99_s": bat["latency_s"],
@@ -395,7 +443,7 @@ def batch_line(args):
         "identical_results_sample": identical,
         "roofline": {"bound": "hbm", "achieved": kt["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kt["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"batch512_{args.transport[-2:]}"),
+                     "traffic": measured_traffic(f"batch512_{args.transport[-2:]}_w{best.get('window', 1)}"),
                      "kernel": "k_btransport", "kernel_ms_per_launch": kt["kernel_ms_per_launch"],
                      "alg_bytes_per_launch": kt["alg_bytes_per_launch"], "launches": kt["launches"],
                      "kernel_share_of_cycle": kt["kernel_share"]},
@@ -404,6 +452,7 @@ def batch_line(args):
         "autotune": {"seconds": tune_s, "candidates": len(table)},
         "serving": serve,
         "eigen_17x17_1e6": eig,
+        "eigen_core3d_70g_1e7": core,
     }
     return line
 
@@ -411,8 +460,8 @@ def batch_line(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="batch", choices=["batch", "eigen"])
     ap.add_argument("--transport", default="transportOperatorDT", choices=["transportOperatorDT", "transportOperatorST"])
     ap.add_argument("--problems", type=int, default=512)
@@ -423,6 +472,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="offered-load duration per load level")
     ap.add_argument("--window", type=int, default=10, help="serving: batches of a request transported per cycle")
     ap.add_argument("--no-eigen", dest="eigen", action="store_false", default=True)
+    ap.add_argument("--no-c5", dest="c5", action="store_false", default=True)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

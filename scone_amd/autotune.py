@@ -3,10 +3,13 @@
 The cycle of the batch engine has three free parameters (scn_batch_settings): the work tile
 (histories a wave takes from the queue at once), K0 (flight/collision pairs per particle in the
 first launch; K doubles per launch) and the number of launches per cycle (the last runs every
-survivor to completion).  Small tiles balance uneven problems better, large ones cut atomics on
-the work counter; small K0 compacts the queues early (fewer idle lanes) at the cost of more
-launches.  The tuner times one cycle of a representative sample for each candidate and keeps the
-fastest.  Results do not depend on the choice: every candidate runs the same histories.
+survivor to completion); with one launch per cycle the requests' window (batches of every solve
+transported per cycle) is a fourth.  Small tiles balance uneven problems better, large ones cut
+atomics on the work counter; small K0 compacts the queues early (fewer idle lanes) at the cost of
+more launches; a wider window spreads a cycle's tail (the wait for its longest histories) over
+more batches.  The tuner times one cycle of a representative sample for each candidate and keeps
+the fastest per batch.  Results do not depend on the choice: every candidate runs the same
+histories and closes the same batches.
 """
 from __future__ import annotations
 
@@ -17,39 +20,41 @@ from .engine import Batch
 
 TILES = (64, 128, 256, 512, 1024)
 K0S = (4, 16)
-LAUNCHES = (1, 2, 3)
+LAUNCHES = (1, 2)
 
 
-def candidates(tiles=TILES, k0s=K0S, launches=LAUNCHES):
-    """(tile, k0, n_launch); with one launch per cycle K0 plays no part."""
+def candidates(tiles=TILES, k0s=K0S, launches=LAUNCHES, windows=(1,)):
+    """(tile, k0, n_launch, window); with one launch per cycle K0 plays no part, and windows of
+    more than one batch need one launch per cycle."""
     out = []
     for tile, nl in itertools.product(tiles, launches):
         if nl == 1:
-            out.append((tile, 8, 1))
+            out.extend((tile, 8, 1, w) for w in windows)
         else:
-            out.extend((tile, k0, nl) for k0 in k0s)
+            out.extend((tile, k0, nl, 1) for k0 in k0s)
     return out
 
 
 def tune(problems, transport, hbm_budget, max_slots=None, tiles=TILES, k0s=K0S, launches=LAUNCHES, cycles=1,
-         log=None):
-    """problems: list of (comp, src, pop, seed).  Returns (best settings dict, table of timings)."""
+         windows=(1,), log=None):
+    """problems: list of (comp, src, pop, seed).  Returns (best settings dict with the window,
+    table of timings per batch)."""
     max_slots = max_slots or max(1, len(problems))
     table = []
     best = None
-    for tile, k0, nl in candidates(tiles, k0s, launches):
+    for tile, k0, nl, w in candidates(tiles, k0s, launches, windows):
         B = Batch(transport=transport, max_slots=max_slots, hbm_budget=hbm_budget, tile=tile, k0=k0, n_launch=nl,
                   use_graph=1)
         for (comp, src, pop, seed) in problems:
-            B.add(comp, src, pop, cycles + 1, seed)
+            B.add(comp, src, pop, (cycles + 1) * w, seed, window=w)
         B.cycle(1)            # first cycle records the graph
         t = time.perf_counter()
         B.cycle(cycles)
-        dt = (time.perf_counter() - t) / cycles
+        dt = (time.perf_counter() - t) / (cycles * w)
         B.close()
-        table.append({"tile": tile, "k0": k0, "n_launch": nl, "ms_per_cycle": dt * 1e3})
+        table.append({"tile": tile, "k0": k0, "n_launch": nl, "window": w, "ms_per_batch": dt * 1e3})
         if log:
-            log(f"[autotune] tile {tile:4d} k0 {k0:3d} launches {nl}: {dt * 1e3:8.3f} ms/cycle")
+            log(f"[autotune] tile {tile:4d} k0 {k0:3d} launches {nl} window {w:2d}: {dt * 1e3:8.3f} ms/batch")
         if best is None or dt < best[0]:
-            best = (dt, {"tile": tile, "k0": k0, "n_launch": nl})
+            best = (dt, {"tile": tile, "k0": k0, "n_launch": nl, "window": w})
     return best[1], table

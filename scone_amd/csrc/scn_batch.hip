@@ -3,17 +3,18 @@
 //
 // Every admitted solve owns a slot: its tables, two live-particle queues (SoA), its score memory
 // (BIN, CSUM, CSUM2) and optional per-history records, all in one block of a device arena that
-// is sized by the HBM budget.  A cycle advances every running slot by one batch of
+// is sized by the HBM budget.  A cycle advances every running slot by `window` batches of
 // fixedSourcePhysicsPackage%cycles (PhysicsPackages/fixedSourcePhysicsPackage_class.f90:187-275):
-//   k_tiles          per slot ceil(n / tile) work tiles (slot, first index) -> tile table
-//   k_btransport     persistent kernel: each wave takes tiles from an atomic counter; each lane runs
-//                    one particle in registers and takes the next particle of the tile as soon as
-//                    its own dies, so lanes stay busy; with a step limit K, survivors are parked in
-//                    the slot's other queue (wave ballot, one atomic per wave).  The first launch
-//                    samples the batch's source particles in place.
-//   (k_tiles, k_btransport) x (n_launch - 1), K doubling, the last launch runs to completion
+//   k_items          one work item per batch of every running slot's window (slot, batch, count)
+//   k_btransport     persistent kernel: waves start spread over the items in proportion to their
+//                    particles and pull chunks from an item's atomic counter until it is used up;
+//                    each lane runs one particle in registers and takes the next particle of the
+//                    chunk as soon as its own dies, so lanes stay busy; with a step limit K,
+//                    survivors are parked in the slot's other queue (wave ballot, one atomic per
+//                    wave).  The first launch samples the batch's source particles in place.
+//   (k_items, k_btransport) x (n_launch - 1), K doubling, the last launch runs to completion
 //   k_bnorm/k_bclose tallyAdmin%reportCycleEnd normalisation and scoreMemory%closeCycle
-//   k_bend           batchN + 1, early stop on a bin's relative sd, slot completion
+//   k_bend           batchN + closed batches, early stop on a bin's relative sd, slot completion
 // Nothing in a cycle depends on host values, so the sequence can be replayed from a hipGraph.
 #include <cstring>
 #include <cstdio>
@@ -99,8 +100,8 @@ struct SlotState {
     unsigned long long segments, collisions, scores, histories;
 };
 
-// control words: [0] tiles in the current launch, [1] work counter, [2] running slots after the
-// cycle, [3] tile-table overflow, [4] dummy site counter
+// control words: [0] work items of the current launch, [2] running slots after the cycle,
+// [5] items whose particles are all handed out
 constexpr int CTL_WORDS = 8;
 
 template <int NL, int GEO>
@@ -278,26 +279,32 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
     return 1;
 }
 
-// tile table of one launch.  One block of 1024 threads; slots in LDS for the search.  A tile is
-// (key, first index) with key = slot * MAX_WINDOW + w: the first launch of a cycle covers batches
-// batch .. batch + n_run - 1 of every running slot (w = 0 .. n_run - 1, pop histories each) and
-// no tile straddles two batches; later launches continue parked particles (w = 0).
+// work item of one launch: the histories of one batch of one slot's window (first launch of a
+// cycle) or one slot's parked particles (later launches), key = slot * MAX_WINDOW + w
+struct Item {
+    int key;
+    int n;                 // particles
+    long long prefix;      // particles of the items before this one
+};
+
+// item table of one launch.  One block of 1024 threads scans the slots: the first launch of a
+// cycle covers batches batch .. batch + n_run - 1 of every running slot (pop histories each), later
+// launches the particles parked in each slot's queue.  Each item's pull counter starts at zero.
 __global__ void __launch_bounds__(1024)
-k_tiles(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first, int in_side, int out_side,
-        int tile, int2* __restrict__ tiles, int tiles_cap, int* ctl)
+k_items(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first, int in_side, int out_side,
+        Item* __restrict__ items, int* __restrict__ item_cnt, int* ctl)
 {
-    __shared__ int s_start[MAX_SLOTS + 1];
-    __shared__ int s_tpb[MAX_SLOTS];   // tiles per batch
     __shared__ int s_scan[1024];
+    __shared__ long long s_pscan[1024];
     int carry = 0;
+    long long pcarry = 0;
     for (int base = 0; base < n_slots; base += 1024) {
         const int s = base + (int)threadIdx.x;
-        int nt = 0;
+        int n = 0, nb = 0;
         if (s < n_slots) {
             SlotState& S = ss[s];
             const SlotDev& D = slots[s];
             const int pop = D.pop;
-            int n = 0, nb = 0;
             if (first) {
                 nb = S.running ? min(D.window, D.n_batches - S.batch) : 0;
                 n = nb > 0 ? pop : 0;
@@ -308,39 +315,34 @@ k_tiles(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first
             }
             S.n_in = n;
             S.count[out_side] = 0;
-            s_tpb[s] = (n + tile - 1) / tile;
-            nt = s_tpb[s] * nb;
         }
-        s_scan[threadIdx.x] = nt;
+        s_scan[threadIdx.x] = nb;
+        s_pscan[threadIdx.x] = (long long)nb * n;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
             int v = (int)threadIdx.x >= off ? s_scan[threadIdx.x - off] : 0;
+            long long pv = (int)threadIdx.x >= off ? s_pscan[threadIdx.x - off] : 0;
             __syncthreads();
             s_scan[threadIdx.x] += v;
+            s_pscan[threadIdx.x] += pv;
             __syncthreads();
         }
-        if (s < n_slots) s_start[s] = carry + s_scan[threadIdx.x] - nt;
+        if (s < n_slots && nb > 0) {
+            const int i0 = carry + s_scan[threadIdx.x] - nb;                  // this slot's first item
+            const long long p0 = pcarry + s_pscan[threadIdx.x] - (long long)nb * n;
+            for (int w = 0; w < nb; ++w) {
+                items[i0 + w] = Item{s * MAX_WINDOW + w, n, p0 + (long long)w * n};
+                item_cnt[i0 + w] = 0;
+            }
+        }
         carry += s_scan[1023];
+        pcarry += s_pscan[1023];
         __syncthreads();
     }
-    const int total = carry < tiles_cap ? carry : tiles_cap;
-    for (int t = threadIdx.x; t < total; t += 1024) {
-        int lo = 0, hi = n_slots;   // s_start[lo] <= t < s_start[hi] (s_start[n_slots] = total)
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (s_start[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        const int lt = t - s_start[lo];
-        const int tpb = s_tpb[lo];   // >= 1: slot lo owns tile t
-        const int w = lt / tpb;
-        tiles[t] = make_int2(lo * MAX_WINDOW + w, (lt - w * tpb) * tile);
-    }
     if (threadIdx.x == 0) {
-        ctl[0] = total;
-        ctl[1] = 0;
+        ctl[0] = carry;      // items (at most n_slots * MAX_WINDOW: the table's size)
+        ctl[5] = 0;          // items whose histories are all handed out
         if (first) ctl[2] = 0;
-        if (carry > tiles_cap) ctl[3] = 1;
     }
 }
 
@@ -390,8 +392,8 @@ __device__ __forceinline__ void flush_lds_bins(Bins& B, int lane)
 // refill directly, one of another slot starts once the live lanes have finished.
 template <int NL, int MODE, int GEO>
 __global__ void __launch_bounds__(BBLOCK)
-k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __restrict__ tiles, int* ctl, int first,
-             int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave, Sites SB)
+k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __restrict__ items, int* item_cnt, int* ctl,
+             int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st, int lds_per_wave, Sites SB)
 {
     extern __shared__ double lds_all[];
     const int lane = threadIdx.x & (BWAVE - 1);
@@ -399,13 +401,32 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     // this wave's private copy of the first lds_per_wave bins of its current slot
     double* wl = lds_all + (size_t)(threadIdx.x / BWAVE) * lds_per_wave;
     for (int i = lane; i < lds_per_wave; i += BWAVE) wl[i] = 0.0;
-    const int n_tiles = ctl[0];
+    const int n_items = ctl[0];
     const int lane_gid = blockIdx.x * BBLOCK + threadIdx.x;   // owner of one LIFO buffer in SB
     int key = -1;                    // current slot * MAX_WINDOW + batch of its window (wave-uniform)
     int s = -1;                      // current slot
-    int next = 0, end = 0;           // particles [next, end) of the current tile not yet started
-    int pend_s = -1, pend_b = 0;     // a tile of another slot, started once the lanes are idle
-    int exhausted = 0;               // the tile table is used up
+    int next = 0, end = 0;           // particles [next, end) of the current chunk not yet started
+    int pend_s = -1, pend_b = 0, pend_e = 0;   // a chunk of another item, started once the lanes are idle
+    int exhausted = n_items == 0;    // every item has been found used up
+    // Waves start spread over the items in proportion to their particles (the item holding
+    // particle g * total / n_waves) and pull chunks from an item's counter until it is used up,
+    // then move to the next item; so a wave switches items (and drains) rarely, and items are
+    // shared by as many waves as their size calls for.  A wave leaves after finding every item
+    // used up once (or all of them closed), so the walk is bounded.
+    int it = 0, visited = 0;
+    if (n_items > 0) {
+        const int n_waves = gridDim.x * (BBLOCK / BWAVE);
+        const int g = blockIdx.x * (BBLOCK / BWAVE) + (int)(threadIdx.x / BWAVE);
+        const long long total = items[n_items - 1].prefix + items[n_items - 1].n;
+        const long long target = total * (long long)g / (long long)n_waves;
+        int lo = 0, hi = n_items;   // items[lo].prefix <= target < items[hi].prefix
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (items[mid].prefix <= target) lo = mid;
+            else hi = mid;
+        }
+        it = __builtin_amdgcn_readfirstlane(lo);
+    }
     const SlotDev* D = slots;
     SlotState* SS = ss;
     int n_in = 0, batch = 0, cap = 0;
@@ -424,22 +445,34 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
     int fresh = 1;                           // the next event starts a flight
     for (int iter = 0;; ++iter) {
         while (next >= end && !exhausted && pend_s < 0) {
-            int t = 0;
-            if (lane == 0) t = atomicAdd(&ctl[1], 1);
-            t = __shfl(t, 0);
-            if (t >= n_tiles) {
-                exhausted = 1;
-                break;
-            }
-            const int2 tl = tiles[t];
-            const int ts = __builtin_amdgcn_readfirstlane(tl.x);
-            const int tb = __builtin_amdgcn_readfirstlane(tl.y);
-            if (ts == key) {
-                next = tb;
-                end = min(tb + tile, n_in);
+            int b = 0;
+            if (lane == 0) b = atomicAdd(&item_cnt[it], chunk);
+            b = __shfl(b, 0);
+            const int ik = __builtin_amdgcn_readfirstlane(items[it].key);
+            const int in_ = __builtin_amdgcn_readfirstlane(items[it].n);
+            if (b < in_) {
+                if (ik == key) {
+                    next = b;
+                    end = min(b + chunk, in_);
+                } else {
+                    pend_s = ik;
+                    pend_b = b;
+                    pend_e = min(b + chunk, in_);
+                }
             } else {
-                pend_s = ts;
-                pend_b = tb;
+                // the first empty pull of an item closes it
+                if (b < in_ + chunk && lane == 0) atomicAdd(&ctl[5], 1);
+                if (++visited >= n_items) {
+                    exhausted = 1;
+                    break;
+                }
+                it = (it + 1 == n_items) ? 0 : it + 1;
+                int closed = 0;
+                if (lane == 0) closed = __atomic_load_n(&ctl[5], __ATOMIC_RELAXED);
+                if (__shfl(closed, 0) >= n_items) {
+                    exhausted = 1;
+                    break;
+                }
             }
         }
         const unsigned long long live = __ballot(alive);
@@ -462,7 +495,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const int2* __res
             B.lds_n = min((int64_t)lds_per_wave, D->P.n_bins);
             SB.depth = D->buffer;
             next = pend_b;
-            end = min(pend_b + tile, n_in);
+            end = pend_e;
         }
         if (next < end) {
             const unsigned long long idle = ~live;
@@ -681,16 +714,16 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
-typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const int2*, int*, int, int,
+typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const Item*, int*, int*, int, int,
                              int, int, int, DevStatus*, int, Sites);
 
 template <int NL, int MODE, int GEO>
-void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const int2* tiles,
-               int* ctl, int first, int in_side, int out_side, int tile, int max_steps, DevStatus* st, int lds_per_wave,
-               Sites SB)
+void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const Item* items,
+               int* item_cnt, int* ctl, int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st,
+               int lds_per_wave, Sites SB)
 {
-    hipLaunchKernelGGL((k_btransport<NL, MODE, GEO>), g, b, sh, s, slots, ss, tiles, ctl, first, in_side, out_side, tile,
-                       max_steps, st, lds_per_wave, SB);
+    hipLaunchKernelGGL((k_btransport<NL, MODE, GEO>), g, b, sh, s, slots, ss, items, item_cnt, ctl, first, in_side,
+                       out_side, chunk, max_steps, st, lds_per_wave, SB);
 }
 
 template <int NL, int MODE, int GEO>
@@ -736,8 +769,8 @@ struct Batch {
     int n_slots = 0;
     SlotDev* d_slots = nullptr;
     SlotState* d_state = nullptr;
-    int2* d_tiles = nullptr;
-    int tiles_cap = 0;
+    Item* d_items = nullptr;         // work items of a launch (n_slots * MAX_WINDOW at most)
+    int* d_item_cnt = nullptr;
     int* d_ctl = nullptr;
     DevStatus* d_status = nullptr;
     std::vector<SlotDev> h_slots;
@@ -1010,13 +1043,13 @@ int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
         const int first = L == 0;
         const int in_side = first ? 0 : side;
         const int out_side = first ? 0 : 1 - side;
-        hipLaunchKernelGGL(k_tiles, dim3(1), dim3(1024), 0, B->stream, B->d_slots, B->d_state, ns, first, in_side,
-                           out_side, tile, B->d_tiles, B->tiles_cap, B->d_ctl);
+        hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, B->stream, B->d_slots, B->d_state, ns, first, in_side,
+                           out_side, B->d_items, B->d_item_cnt, B->d_ctl);
         const int last = L == B->S.n_launch - 1;
         const int steps = last ? (1 << 30) : (B->S.k0 << L);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
-        fn(dim3(grid), dim3(BBLOCK), lds_bytes(B), B->stream, B->d_slots, B->d_state, B->d_tiles, B->d_ctl, first,
-           in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave, B->stack);
+        fn(dim3(grid), dim3(BBLOCK), lds_bytes(B), B->stream, B->d_slots, B->d_state, B->d_items, B->d_item_cnt,
+           B->d_ctl, first, in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave, B->stack);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L + 1], B->stream));
         side = out_side;
     }
@@ -1104,9 +1137,8 @@ int run_cycle(Batch* B, int* n_running_out)
         if (rc) return rc;
     }
     BHIPCHK(hipEventRecord(B->c1, B->stream));
-    int n_running = 0, ovf = 0;
+    int n_running = 0;
     BHIPCHK(hipMemcpyAsync(&n_running, B->d_ctl + 2, sizeof(int), hipMemcpyDeviceToHost, B->stream));
-    BHIPCHK(hipMemcpyAsync(&ovf, B->d_ctl + 3, sizeof(int), hipMemcpyDeviceToHost, B->stream));
     BHIPCHK(hipStreamSynchronize(B->stream));
     float ms = 0.f;
     BHIPCHK(hipEventElapsedTime(&ms, B->c0, B->c1));
@@ -1122,10 +1154,6 @@ int run_cycle(Batch* B, int* n_running_out)
     B->stats.transport_launches += B->S.n_launch;
     int rc = check_bstatus(B);
     if (rc) return rc;
-    if (ovf) {
-        B->err = "tile table overflow";
-        return SCN_ERR_OVERFLOW;
-    }
     *n_running_out = n_running;
     return 0;
 }
@@ -1183,11 +1211,10 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     if (hipMalloc((void**)&B->arena, B->arena_size) != hipSuccess)
         return fail("hipMalloc of the HBM budget failed", SCN_ERR_HIP);
     B->freel[0] = B->arena_size & ~(size_t)255;
-    // tiles: a history of pop takes two queue entries of more than 100 B each, plus one partial
-    // tile per slot
-    size_t max_hist = B->arena_size / 128;
-    B->tiles_cap = (int)std::min<size_t>(max_hist / BWAVE + (size_t)B->n_slots + 16, (size_t)1 << 30);
-    if (hipMalloc((void**)&B->d_tiles, sizeof(int2) * (size_t)B->tiles_cap) != hipSuccess ||
+    // work items: at most one per batch of every slot's window
+    const size_t n_items = (size_t)B->n_slots * MAX_WINDOW;
+    if (hipMalloc((void**)&B->d_items, sizeof(Item) * n_items) != hipSuccess ||
+        hipMalloc((void**)&B->d_item_cnt, sizeof(int) * n_items) != hipSuccess ||
         hipMalloc((void**)&B->d_slots, sizeof(SlotDev) * B->n_slots) != hipSuccess ||
         hipMalloc((void**)&B->d_state, sizeof(SlotState) * B->n_slots) != hipSuccess ||
         hipMalloc((void**)&B->d_ctl, sizeof(int) * CTL_WORDS) != hipSuccess ||
@@ -1213,7 +1240,8 @@ void scn_batch_destroy(void* batch)
     drop_graph(B);
     if (B->arena) (void)hipFree(B->arena);
     if (B->stack_mem) (void)hipFree(B->stack_mem);
-    if (B->d_tiles) (void)hipFree(B->d_tiles);
+    if (B->d_items) (void)hipFree(B->d_items);
+    if (B->d_item_cnt) (void)hipFree(B->d_item_cnt);
     if (B->d_slots) (void)hipFree(B->d_slots);
     if (B->d_state) (void)hipFree(B->d_state);
     if (B->d_ctl) (void)hipFree(B->d_ctl);

@@ -41,11 +41,11 @@ def run(args):
         from scone_amd.engine import Batch
         transport = T.TRANSPORT_DT if args.transport == "transportOperatorDT" else T.TRANSPORT_ST
         problems = bench.batch_problems(512, args.transport, 1.0)
-        budget = bench.budget_for(problems, 2)
+        budget = bench.budget_for(problems, 2, window=args.window)
         B = Batch(transport=transport, max_slots=512, hbm_budget=budget, use_graph=0, tile=args.tile, k0=args.k0,
                   n_launch=args.n_launch)
         for (c, s, pop, _, seed) in problems:
-            B.add(c, s, pop, args.cycles, seed)
+            B.add(c, s, pop, args.cycles * args.window, seed, window=args.window)
         B.run_all()
         s = B.stats()
         st = {k: getattr(s, k) for k, _ in s._fields_}
@@ -127,6 +127,7 @@ def main():
     r.add_argument("--tile", type=int, default=0)
     r.add_argument("--k0", type=int, default=0)
     r.add_argument("--n-launch", type=int, default=0)
+    r.add_argument("--window", type=int, default=1, help="batch512: batches of every problem per cycle")
     s = sub.add_parser("summarize")
     s.add_argument("fetch_dir")
     s.add_argument("write_dir")
