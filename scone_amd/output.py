@@ -1,9 +1,12 @@
-"""Result files in SCONE's output format (UserInterface/fileOutput).
+"""Result files in SCONE's output formats (UserInterface/fileOutput).
 
-`OutputFile` restates outputFile_class.f90 with the asciiMATLAB printer (asciiMATLAB_class.f90):
-blocks prefix entry names (`block_name = ...;`), results are (value, sd) pairs whose array gains
-a leading dimension 2, rank > 1 arrays are written `reshape([ ... ],d1,d2,...)` in Fortran
-column-major order, reals use the ES12.5 edit descriptor (left-adjusted), integers I12.
+`OutputFile` restates outputFile_class.f90: block and array bookkeeping, (value, sd) results whose
+array gains a leading dimension 2, reals through the ES12.5 edit and integers through I12, both
+left-adjusted, and the name-reuse warning.  It drives one of the printers of asciiOutputFactory
+(asciiOutputFactory_func.f90): `asciiMATLAB` (asciiMATLAB_class.f90: block prefixes
+`block_name = ...;`, rank > 1 arrays as `reshape([ ... ],d1,d2,...)` in Fortran column-major
+order), `asciiJSON` (asciiJSON_class.f90: nested objects, arrays nested from the fastest
+dimension inward) and `dummyPrinter` (writes nothing).
 
 `print_tally` restates tallyAdmin%print (Tallies/tallyAdmin_class.f90:436-451) with
 collisionClerk%print (collisionClerk_class.f90:262-301) and the map prints of multiMap,
@@ -32,118 +35,294 @@ def es12_5(x: float) -> str:
     return f"{mant}{'+' if e >= 0 else '-'}{abs(e):03d}"
 
 
-class OutputFile:
-    """outputFile + asciiMATLAB printer."""
+class _Tape:
+    """Character buffer with the delayedStream operations the printers use
+    (delayedStream_class.f90:80-111): append, cut the last n characters, peek the n-th last."""
 
-    def __init__(self, fmt: str = "asciiMATLAB"):
-        if fmt != "asciiMATLAB":
-            raise NotImplementedError(f"output format {fmt}: only asciiMATLAB is written")
-        self.fmt = fmt
-        self.buf = []
+    def __init__(self):
+        self.parts = []
+
+    def append(self, s):
+        self.parts.append(s)
+
+    def cut(self, n):
+        while n > 0 and self.parts:
+            last = self.parts.pop()
+            if len(last) > n:
+                self.parts.append(last[:-n])
+                return
+            n -= len(last)
+
+    def peek(self, n):
+        for part in reversed(self.parts):
+            if len(part) >= n:
+                return part[-n]
+            n -= len(part)
+        raise ValueError("peek beyond the start of the buffer")
+
+    def text(self):
+        return "".join(self.parts)
+
+
+class MatlabPrinter:
+    """asciiMATLAB printer (asciiMATLAB_class.f90)."""
+    ext = "m"
+    _BLOCK, _ENTRY, _ARRAY = 0, 1, 2
+
+    def __init__(self):
+        self.tape = _Tape()
+        self.state = self._BLOCK
         self.prefix = []
-        self.shape = None
-        self.array = None            # None, "undef", "res", "val"
-        self.top = 0
-        self.limit = 0
+        self.shape = []
 
-    # -- asciiMATLAB printer
-    def _entry(self, name):
-        self.buf.append("".join(p + "_" for p in self.prefix) + name + " = ")
-
-    def _start_array(self, shape):
-        self.shape = list(shape)
-        self.buf.append("[ " if len(shape) == 1 else "reshape([ ")
-
-    def _end_array(self):
-        last = self.buf.pop()
-        self.buf.append(last[:-1])               # cut the trailing ','
-        self.buf.append("]")
-        if len(self.shape) > 1:
-            self.buf.append("".join("," + str(d) for d in self.shape) + ")")
-
-    def _num(self, s, in_array):
-        self.buf.append(s + "," if in_array else s)
-
-    def _char(self, s, in_array):
-        self.buf.append("{'" + s + "'}" + ("," if in_array else ""))
-
-    # -- outputFile interface
     def start_block(self, name):
-        if self.array is not None:
-            raise ValueError(f"cannot start block {name} while writing an array")
         self.prefix.append(name)
 
     def end_block(self):
-        if not self.prefix:
-            raise ValueError("trying to exit from the root block")
         self.prefix.pop()
 
-    def start_array(self, name, shape):
-        if self.array is not None or len(shape) == 0 or int(np.prod(shape)) == 0:
-            raise ValueError(f"cannot start array {name}")
+    def start_entry(self, name):
+        self.state = self._ENTRY
+        self.tape.append("".join(p + "_" for p in self.prefix) + name + " = ")
+
+    def end_entry(self):
+        self.state = self._BLOCK
+        self.tape.append(";\n")
+
+    def start_array(self, shape):
+        self.state = self._ARRAY
         self.shape = list(shape)
-        self.array = "undef"
-        self._entry(name)
+        self.tape.append("[ " if len(shape) == 1 else "reshape([ ")
 
-    def add_result(self, val, std):
-        if self.array == "undef":
-            self.array = "res"
-            self.top = 0
-            self.limit = 2 * int(np.prod(self.shape))
-            self._start_array([2] + self.shape)
-        if self.array != "res" or self.top + 2 > self.limit:
-            raise ValueError("result does not fit the array")
-        self._num(es12_5(val), True)
-        self._num(es12_5(std), True)
-        self.top += 2
+    def end_array(self):
+        self.state = self._ENTRY
+        self.tape.cut(1)                          # the separator after the last element
+        self.tape.append("]")
+        if len(self.shape) > 1:
+            self.tape.append("".join("," + str(d) for d in self.shape) + ")")
 
-    def add_value(self, v):
-        if self.array == "undef":
-            self.array = "val"
-            self.top = 0
-            self.limit = int(np.prod(self.shape))
-            self._start_array(self.shape)
-        if self.array != "val" or self.top + 1 > self.limit:
-            raise ValueError("value does not fit the array")
-        if isinstance(v, str):
-            self._char(v, True)
-        elif isinstance(v, (int, np.integer)):
-            self._num(str(int(v)), True)
-        else:
-            self._num(es12_5(float(v)), True)
-        self.top += 1
+    def print_num(self, s):
+        if self.state == self._ARRAY:
+            self.tape.append(s + ",")
+        elif self.state == self._ENTRY:
+            self.tape.append(s)
+
+    def print_char(self, s):
+        if self.state == self._ARRAY:
+            self.tape.append("{'" + s + "'},")
+        elif self.state == self._ENTRY:
+            self.tape.append("{'" + s + "'}")
+
+    def final_text(self):
+        return self.tape.text()
+
+
+class JsonPrinter:
+    """asciiJSON printer (asciiJSON_class.f90): two-space indentation per block level; every
+    value is followed by ',', and closing a block or the file cuts the last ',' + newline."""
+    ext = "json"
+    IND = 2
+
+    def __init__(self):
+        self.tape = _Tape()
+        self.tape.append("{\n")
+        self.level = 1
+        self.shape = []
+        self.in_array = False
+        self.count = 0
+
+    def _cut_separator(self):
+        if len(self.tape.parts) and self.tape.peek(2) == ",":
+            self.tape.cut(2)
+            self.tape.append("\n")
+
+    def start_block(self, name):
+        self.tape.append(" " * (self.level * self.IND) + '"' + name + '":{\n')
+        self.level += 1
+
+    def end_block(self):
+        self._cut_separator()
+        self.level -= 1
+        self.tape.append(" " * (self.level * self.IND) + "},\n")
+
+    def start_entry(self, name):
+        self.tape.append(" " * (self.level * self.IND) + '"' + name + '":')
+
+    def end_entry(self):
+        self.tape.append("\n")
+
+    def start_array(self, shape):
+        self.shape = list(shape)
+        self.in_array = True
+        self.count = 0
+
+    def end_array(self):
+        self.in_array = False
+
+    def _element(self, s):
+        if self.in_array:
+            mul = 1
+            for d in self.shape:                  # open a bracket per dimension starting here
+                mul *= d
+                if self.count % mul == 0:
+                    self.tape.append("[")
+        self.tape.append(s)
+        if self.in_array:
+            self.count += 1
+            mul = 1
+            for d in self.shape:                  # close a bracket per dimension ending here
+                mul *= d
+                if self.count % mul == 0:
+                    self.tape.append("]")
+        self.tape.append(",")
+
+    def print_num(self, s):
+        self._element(s)
+
+    def print_char(self, s):
+        self._element('"' + s + '"')
+
+    def final_text(self):
+        t = self.tape.text()
+        if len(t) >= 2 and t[-2] == ",":
+            t = t[:-2] + "\n"
+        return t + "}\n"
+
+
+class DummyPrinter:
+    """dummyPrinter (dummyPrinter_class.f90): accepts every call and writes nothing."""
+    ext = ""
+
+    def __getattr__(self, name):
+        return lambda *a, **k: None
+
+    def final_text(self):
+        return ""
+
+
+PRINTERS = {"asciiMATLAB": MatlabPrinter, "asciiJSON": JsonPrinter, "dummyPrinter": DummyPrinter}
+
+
+class OutputFile:
+    """outputFile (outputFile_class.f90): checks the block/array protocol, converts numbers and
+    drives the printer named by `fmt`."""
+
+    _NOT, _UNDEF, _RES, _REAL, _INT, _CHAR = range(6)
+
+    def __init__(self, fmt: str = "asciiMATLAB"):
+        if fmt not in PRINTERS:
+            raise ValueError(f"output format {fmt} is not recognised; valid: {', '.join(PRINTERS)}")
+        self.fmt = fmt
+        self.printer = PRINTERS[fmt]()
+        self.array = self._NOT
+        self.shape = []
+        self.top = 0
+        self.limit = 0
+        self.block = [""]
+        self.used = [set()]
+        self.warnings = []
+
+    # -- helpers
+    def _reuse(self, name):
+        if name in self.used[-1]:
+            msg = (f"WARNING: Name {name} is used multiple times in block {self.block[-1]} "
+                   f"|| Output may not parse correctly!")
+            self.warnings.append(msg)
+            print(msg)
+        self.used[-1].add(name)
+
+    @staticmethod
+    def _num(v):
+        if isinstance(v, (bool, np.bool_)):
+            raise TypeError("logical values are not printed")
+        if isinstance(v, (int, np.integer)):
+            return str(int(v))                    # I12, adjustl
+        return es12_5(float(v))
+
+    # -- outputFile interface
+    def start_block(self, name):
+        self._reuse(name)
+        if self.array != self._NOT:
+            raise ValueError(f"cannot start block {name} while writing an array")
+        self.used.append(set())
+        self.block.append(name)
+        self.printer.start_block(name)
+
+    def end_block(self):
+        if self.array != self._NOT:
+            raise ValueError("cannot exit a block while writing an array")
+        if len(self.block) == 1:
+            raise ValueError("trying to exit from the root block")
+        self.block.pop()
+        self.used.pop()
+        self.printer.end_block()
+
+    def start_array(self, name, shape):
+        self._reuse(name)
+        if self.array != self._NOT or len(shape) == 0 or int(np.prod(shape)) == 0:
+            raise ValueError(f"cannot start array {name}")
+        self.shape = [int(d) for d in shape]
+        self.array = self._UNDEF
+        self.printer.start_entry(name)
 
     def end_array(self):
         if self.top != self.limit:
             raise ValueError(f"array closed after {self.top} of {self.limit} entries")
-        self._end_array()
-        self.buf.append(";\n")
-        self.array = None
+        self.array, self.top, self.limit = self._NOT, 0, 0
+        self.printer.end_array()
+        self.printer.end_entry()
+
+    def add_result(self, val, std):
+        if self.array == self._UNDEF:
+            self.array, self.top = self._RES, 0
+            self.limit = 2 * int(np.prod(self.shape))
+            self.printer.start_array([2] + self.shape)
+        if self.array != self._RES or self.top + 2 > self.limit:
+            raise ValueError("result does not fit the array")
+        self.printer.print_num(es12_5(float(val)))
+        self.printer.print_num(es12_5(float(std)))
+        self.top += 2
+
+    def add_value(self, v):
+        kind = self._CHAR if isinstance(v, str) else \
+            self._INT if isinstance(v, (int, np.integer)) else self._REAL
+        if self.array == self._UNDEF:
+            self.array, self.top = kind, 0
+            self.limit = int(np.prod(self.shape))
+            self.printer.start_array(self.shape)
+        if self.array != kind or self.top + 1 > self.limit:
+            raise ValueError("value does not fit the array (mixed content or overflow)")
+        if kind == self._CHAR:
+            self.printer.print_char(v)
+        else:
+            self.printer.print_num(self._num(v))
+        self.top += 1
 
     def print_value(self, v, name):
-        self._entry(name)
+        self._reuse(name)
+        self.printer.start_entry(name)
         if isinstance(v, str):
-            self._char(v, False)
-        elif isinstance(v, (int, np.integer)):
-            self._num(str(int(v)), False)
+            self.printer.print_char(v)
         else:
-            self._num(es12_5(float(v)), False)
-        self.buf.append(";\n")
+            self.printer.print_num(self._num(v))
+        self.printer.end_entry()
 
     def print_result(self, val, std, name):
-        self._entry(name)
-        self._start_array([2])
-        self._num(es12_5(val), True)
-        self._num(es12_5(std), True)
-        self._end_array()
-        self.buf.append(";\n")
+        self._reuse(name)
+        self.printer.start_entry(name)
+        self.printer.start_array([2])
+        self.printer.print_num(es12_5(float(val)))
+        self.printer.print_num(es12_5(float(std)))
+        self.printer.end_array()
+        self.printer.end_entry()
 
     def text(self) -> str:
-        return "".join(self.buf)
+        """The file's contents as closing it would leave them (the printer's endFile)."""
+        return self.printer.final_text()
 
     def write(self, path):
-        if not path.endswith(".m"):
-            path = path + ".m"
+        """Write to `path` + '.' + the printer's extension, as outputFile%init names the file."""
+        path = f"{path}.{self.printer.ext}"
         with open(path, "w") as f:
             f.write(self.text())
         return path

@@ -82,3 +82,90 @@ def test_eigen_results_entries():
     assert order[:6] == ["seed", "pop", "Inactive_Cycles", "Active_Cycles", "Total_CPU_Time", "Total_Transport_Time"]
     assert order.index("keff_IMP_PROD") < order.index("keff_IMP_ABS") < order.index("keff_SCATTER_PROD") \
         < order.index("keff_ANA_LEAK") < order.index("keff_K_EFF") < order.index("active_batchSize")
+
+
+def test_json_printer_layout_by_hand():
+    """asciiJSON: entries end with ',', closing a block or the file cuts the last separator,
+    arrays open a bracket per dimension that starts at the element (asciiJSON_class.f90)."""
+    out = OUT.OutputFile("asciiJSON")
+    out.print_value(7, "seed")
+    out.start_block("b")
+    out.start_array("a", [3])
+    for v in (1, 2, 3):
+        out.add_value(v)
+    out.end_array()
+    out.print_result(1.0, 0.5, "r")
+    out.end_block()
+    assert out.text() == ('{\n  "seed":7,\n  "b":{\n    "a":[1,2,3],\n'
+                          '    "r":[1.00000E+00,5.00000E-01]\n  }\n}\n')
+
+
+def test_json_clerk_results_nest_fastest_dimension_innermost():
+    import json
+    inp, mg = configs.pin_lattice_7g(pop=100)
+    comp = T.compile_problem(inp, mg)
+    nb = int(comp.problem.n_bins)
+    res = np.stack([np.arange(nb) + 0.5, np.arange(nb) * 1e-3], axis=1)
+    out = OUT.OutputFile("asciiJSON")
+    OUT.print_tally(out, comp, res)
+    d = json.loads(out.text())
+    assert d["batchSize"] == 1
+    assert d["pins_g1"]["multiMapOrder"] == ["X", "Y"]
+    xb = np.array(d["pins_g1"]["XBounds"])
+    assert xb.shape == (17, 2) and xb[0, 0] == -10.71 and xb[-1, 1] == 10.71
+    v = np.array(d["pins_g3"]["Res"])
+    assert v.shape == (17, 17, 2, 2)           # Fortran shape [2, 2, 17, 17], outermost last
+    a = comp.clerk_info[2]["addr"]
+    for ix, iy, ir in [(0, 0, 0), (3, 5, 1), (16, 16, 1), (9, 2, 0)]:
+        b = a + 2 * (ix + 17 * iy) + ir
+        assert v[iy, ix, ir, 0] == float(OUT.es12_5(res[b, 0]))
+        assert v[iy, ix, ir, 1] == float(OUT.es12_5(res[b, 1]))
+
+
+def test_json_eigen_results_parse():
+    import json
+    inp, mg = configs.inf_medium_2g(pop=1000, inactive=3, active=7)
+    inp["outputFormat"] = "asciiJSON"
+    comp = T.compile_problem(inp, mg)
+
+    class R:
+        k_implicit = (1.0123, 0.0045)
+        imp = [(2.0, 0.1), (0.3, 0.01), (1.5, 0.05), (0.2, 0.02)]
+
+    out = OUT.eigen_results(comp, inp, R, np.ones((2, 2)), cpu_time=3.25, transport_time=2.5)
+    d = json.loads(out.text())
+    assert list(d)[:6] == ["seed", "pop", "Inactive_Cycles", "Active_Cycles", "Total_CPU_Time",
+                           "Total_Transport_Time"]
+    assert d["pop"] == 1000 and d["Total_Transport_Time"] == 2.5
+    assert d["inactive"] == {"batchSize": 1}
+    assert d["keff"]["K_EFF"] == [1.0123, 0.0045] and d["keff"]["IMP_ABS"] == [1.5, 0.05]
+    assert d["active"]["flux_g2"]["Res"] == [[1.0, 1.0]]
+
+
+def test_output_file_name_and_printer_choice(tmp_path):
+    for fmt, ext in (("asciiMATLAB", "m"), ("asciiJSON", "json")):
+        out = OUT.OutputFile(fmt)
+        out.print_value(1, "x")
+        assert out.write(str(tmp_path / "res")) == str(tmp_path / f"res.{ext}")
+    out = OUT.OutputFile("dummyPrinter")
+    out.start_block("b")
+    out.print_result(1.0, 0.1, "r")
+    out.end_block()
+    assert out.text() == ""
+    with pytest.raises(ValueError, match="not recognised"):
+        OUT.OutputFile("asciiXML")
+
+
+def test_output_protocol_errors_and_name_reuse():
+    out = OUT.OutputFile()
+    out.print_value(1, "x")
+    out.print_value(2, "x")
+    assert out.warnings and "x is used multiple times" in out.warnings[0]
+    with pytest.raises(ValueError):
+        out.end_block()
+    out.start_array("a", [2])
+    out.add_value(1.5)
+    with pytest.raises(ValueError, match="mixed"):
+        out.add_value("c")
+    with pytest.raises(ValueError, match="closed after 1 of 2"):
+        out.end_array()
