@@ -178,7 +178,7 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
 {
     const int64_t j = (int64_t)i + 1;
     const int64_t b2 = 2 * (int64_t)D.pop * batch;
-    uint64_t srng = scn_rng_stride(D.seed, b2 + j);
+    uint64_t srng = rng_stride(D.seed, b2 + j);
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         p.loc[l] = 0;
@@ -252,7 +252,7 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
             break;
         }
     }
-    p.rng = scn_rng_stride(D.seed, b2 + D.pop + j);
+    p.rng = rng_stride(D.seed, b2 + D.pop + j);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         p.r[0][k] = r[k];

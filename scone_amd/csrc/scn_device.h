@@ -23,6 +23,44 @@
 
 namespace scn {
 
+// ---- RNG jumps --------------------------------------------------------------------------------
+// A_i, C_i with s -> A_i*s + C_i (mod 2^63) the LCG advanced by 2^i steps: A_0 = g, C_0 = 1,
+// A_{i+1} = A_i^2, C_{i+1} = C_i*(A_i + 1).  Skipping k steps applies the maps of k's set bits
+// (they commute), one 64-bit multiply per set bit, where Brown's algorithm (scn_rng_skip,
+// RNG_class.f90:251-304) composes the maps with up to five per bit.  Same states: both are the
+// exact k-fold composition of the generator mod 2^63.
+struct RngJumps {
+    uint64_t a[63];
+    uint64_t c[63];
+};
+
+constexpr RngJumps make_rng_jumps()
+{
+    RngJumps t{};
+    uint64_t a = SCN_LCG_G, c = 1ULL;
+    for (int i = 0; i < 63; ++i) {
+        t.a[i] = a;
+        t.c[i] = c;
+        c = (c * (a + 1ULL)) & SCN_LCG_MASK;
+        a = (a * a) & SCN_LCG_MASK;
+    }
+    return t;
+}
+
+static __constant__ RngJumps RNG_JUMPS = make_rng_jumps();
+
+// scn_rng_skip on the device: k < 0 wraps through the period 2^63 the same way
+__device__ __forceinline__ uint64_t rng_skip(uint64_t s, int64_t k_in)
+{
+    uint64_t k = (k_in >= 0) ? (uint64_t)k_in : (uint64_t)(INT64_MAX - (-k_in) + 1);
+    k &= SCN_LCG_MASK;
+    for (int i = 0; k != 0; ++i, k >>= 1)
+        if (k & 1ULL) s = (RNG_JUMPS.a[i] * s + RNG_JUMPS.c[i]) & SCN_LCG_MASK;
+    return s;
+}
+
+__device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n) { return rng_skip(s, SCN_STRIDE * n); }
+
 // ---- problem tables as one by-value kernel argument ---------------------------------------
 struct DevProblem {
     int transport, st_cache, root_uni, border_box, n_mat, nG, n_clerks, graph_size;

@@ -192,7 +192,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             if (!alive && i < end) {
                 if (first) {
                     const scn_particle_state& s = bank[i];
-                    p.rng = scn_rng_stride(master, (int64_t)i + 1);
+                    p.rng = rng_stride(master, (int64_t)i + 1);
 #pragma unroll
                     for (int l = 0; l < NL; ++l) {
                         p.loc[l] = 0;
@@ -406,7 +406,7 @@ __global__ void k_site_rn(uint64_t master, int n, double* rn, uint64_t* rn_bits)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    uint64_t s = scn_rng_skip(master, (int64_t)j);
+    uint64_t s = rng_skip(master, (int64_t)j);
     double u = scn_rng_get(&s);
     rn[j] = u;
     rn_bits[j] = scn_dbits(u);
@@ -458,7 +458,7 @@ __global__ void k_fission_source(DevProblem P, int n, uint64_t master, double b0
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t rng = scn_rng_stride(master, (int64_t)i + 1);
+    uint64_t rng = rng_stride(master, (int64_t)i + 1);
     const double bottom[3] = {b0, b1, b2}, top[3] = {t0, t1, t2};
     scn_particle_state s;
     memset(&s, 0, sizeof s);
