@@ -447,7 +447,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
         while (next >= end && !exhausted && pend_s < 0) {
             int b = 0;
             if (lane == 0) b = atomicAdd(&item_cnt[it], chunk);
-            b = __shfl(b, 0);
+            b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));   // wave-uniform: the slot record loads as scalars
             const int ik = __builtin_amdgcn_readfirstlane(items[it].key);
             const int in_ = __builtin_amdgcn_readfirstlane(items[it].n);
             if (b < in_) {
@@ -469,7 +469,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
                 it = (it + 1 == n_items) ? 0 : it + 1;
                 int closed = 0;
                 if (lane == 0) closed = __atomic_load_n(&ctl[5], __ATOMIC_RELAXED);
-                if (__shfl(closed, 0) >= n_items) {
+                if (__builtin_amdgcn_readfirstlane(__shfl(closed, 0)) >= n_items) {
                     exhausted = 1;
                     break;
                 }
@@ -1300,6 +1300,9 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     size_t o_goff = I.put(goff.data(), goff.size());
     size_t o_gidx = I.put(gidx.data(), gidx.size());
     size_t o_tgoff = I.put(tgoff.data(), tgoff.size());
+    std::vector<int64_t> g_addr;
+    const FastColl fc = fast_collision_layout(pr, goff, gidx, g_addr);
+    size_t o_gaddr = I.put(g_addr.data(), g_addr.size());
     size_t o_tgidx = I.put(tgidx.data(), tgidx.size());
     // pointSource%init normalisation (pointSource_class.f90:120-128,150-152)
     std::vector<double> probG;
@@ -1373,6 +1376,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.clerks = (const scn_clerk*)(base + o_clerks);
     P.gclerk_off = (const int*)(base + o_goff);
     P.gclerk_idx = (const int*)(base + o_gidx);
+    P.fc = fc;
+    P.fc.g_addr = (const int64_t*)(base + o_gaddr);
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
     P.P1 = pr->mg.P1 ? (const double*)(base + o_P1) : nullptr;

@@ -61,6 +61,26 @@ __device__ __forceinline__ uint64_t rng_skip(uint64_t s, int64_t k_in)
 
 __device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n) { return rng_skip(s, SCN_STRIDE * n); }
 
+// ---- collision clerks of one shared layout -----------------------------------------------------
+// When every collision clerk has the same maps, all of them lin-grid spaceMaps (at most 3), only
+// flux responses, the same width and virtual-collision flag, and each group is admitted by at most
+// one clerk (a mesh flux tally, possibly split into one clerk per group), a score needs no walk of
+// the clerk table: the map parameters live in the problem record (wave-uniform loads) and the
+// admitting clerk's first bin comes from one per-group table.  The bins and the arithmetic are the
+// general path's (map1d, collisionClerk reportInColl).
+struct FastColl {
+    int on;
+    int n_maps;
+    int width;
+    int handle_virtual;
+    int axis[3];
+    int nbins[3];
+    int multi[3];
+    double g0[3];        // grid minimum (real_data[data])
+    double step[3];      // grid step (real_data[data + 1])
+    const int64_t* g_addr;   // [nG] mem_addr of the clerk admitting group g, -1 if none
+};
+
 // ---- problem tables as one by-value kernel argument ---------------------------------------
 struct DevProblem {
     int transport, st_cache, root_uni, border_box, n_mat, nG, n_clerks, graph_size;
@@ -95,6 +115,7 @@ struct DevProblem {
     double ww_survival;
     const int* gtrack_off;   // [nG+1] track clerks whose energy filter admits group g
     const int* gtrack_idx;
+    FastColl fc;             // shared-layout collision clerks (fc.on = 0: walk the clerk table)
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -1084,6 +1105,21 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         p.k[K_IMP_ABS] = p.k[K_IMP_ABS] + absXS * flux;
     }
     if (P.n_clerks == 0) return 0;
+    if (P.fc.on) {
+        const int64_t a0 = P.fc.g_addr[p.G - 1];
+        if (a0 < 0 || (!P.fc.handle_virtual && isVirtual)) return 0;
+        int bin = 1;
+        for (int k = 0; k < P.fc.n_maps; ++k) {
+            const double v = get3(p.r[0], P.fc.axis[k]);
+            int idx = (int)floor((v - P.fc.g0[k]) / P.fc.step[k]) + 1;
+            if (idx < 1 || idx >= P.fc.nbins[k] + 1) return 0;
+            bin = bin + (idx - 1) * P.fc.multi[k];
+        }
+        const double flux = P.fc.handle_virtual ? p.w / p.track : p.w / track_mat_xs(P, mat, p.G, st);
+        const int64_t addr = a0 + (int64_t)P.fc.width * (bin - 1);
+        for (int i = 0; i < P.fc.width; ++i) bin_add(B, addr + i, 1.0 * flux);
+        return P.fc.width;
+    }
     int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];   // collision clerks only
     for (int ci = c0; ci < c1; ++ci) {
         const scn_clerk& C = P.clerks[P.gclerk_idx[ci]];

@@ -1073,7 +1073,13 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= upload(E, &d_gidx, gidx.data(), std::max<size_t>(1, gidx.size()));
     if (!rc) rc |= upload(E, &d_tgoff, tgoff.data(), tgoff.size());
     if (!rc) rc |= upload(E, &d_tgidx, tgidx.data(), std::max<size_t>(1, tgidx.size()));
+    std::vector<int64_t> g_addr;
+    const FastColl fc = fast_collision_layout(pr, goff, gidx, g_addr);
+    int64_t* d_gaddr = nullptr;
+    if (!rc) rc |= upload(E, &d_gaddr, g_addr.data(), g_addr.size());
     DevProblem& P = E->P;
+    P.fc = fc;
+    P.fc.g_addr = d_gaddr;
     P.transport = pr->transport;
     P.st_cache = pr->st_cache;
     P.root_uni = pr->root_uni;

@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "scone_gpu.h"
+#include "scn_device.h"
 
 namespace scn {
 
@@ -138,6 +139,60 @@ inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vec
         }
         goff[g] = (int)gidx.size();
     }
+}
+
+// FastColl of the problem's collision clerks, or fc.on = 0 when they do not share one layout;
+// g_addr receives the per-group first-bin table (pointer set by the caller after upload)
+inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<int>& goff,
+                                      const std::vector<int>& gidx, std::vector<int64_t>& g_addr)
+{
+    FastColl F{};
+    const int nG = pr->mg.n_groups;
+    g_addr.assign(std::max(nG, 1), -1);
+    const scn_clerk* first = nullptr;
+    for (int c = 0; c < pr->n_clerks; ++c) {
+        const scn_clerk& K = pr->clerks[c];
+        if (K.kind != SCN_CLERK_COLLISION) continue;
+        if (K.n_maps < 0 || K.n_maps > 3 || K.width < 1 || K.width > 8) return F;
+        for (int i = 0; i < K.width; ++i)
+            if (K.resp[i] != SCN_MT_FLUX) return F;
+        for (int k = 0; k < K.n_maps; ++k) {
+            const scn_map1d& m = K.maps[k];
+            if (m.type != SCN_MAP_SPACE || m.grid_type != SCN_GRID_LIN) return F;
+            if (m.data < 0 || m.data + 1 >= pr->n_real) return F;
+        }
+        if (!first) {
+            first = &K;
+            continue;
+        }
+        if (K.n_maps != first->n_maps || K.width != first->width || K.handle_virtual != first->handle_virtual)
+            return F;
+        for (int k = 0; k < K.n_maps; ++k) {
+            const scn_map1d &m = K.maps[k], &f = first->maps[k];
+            if (m.axis != f.axis || m.nbins != f.nbins || K.multi[k] != first->multi[k]) return F;
+            if (pr->real_data[m.data] != pr->real_data[f.data] ||
+                pr->real_data[m.data + 1] != pr->real_data[f.data + 1])
+                return F;
+        }
+    }
+    if (!first) return F;
+    for (int g = 1; g <= nG; ++g) {
+        const int n = goff[g] - goff[g - 1];
+        if (n > 1) return F;
+        if (n == 1) g_addr[g - 1] = pr->clerks[gidx[goff[g - 1]]].mem_addr;
+    }
+    F.on = 1;
+    F.n_maps = first->n_maps;
+    F.width = first->width;
+    F.handle_virtual = first->handle_virtual;
+    for (int k = 0; k < first->n_maps; ++k) {
+        F.axis[k] = first->maps[k].axis;
+        F.nbins[k] = first->maps[k].nbins;
+        F.multi[k] = first->multi[k];
+        F.g0[k] = pr->real_data[first->maps[k].data];
+        F.step[k] = pr->real_data[first->maps[k].data + 1];
+    }
+    return F;
 }
 
 // scoreMemory_class.f90:537-573 getResult: mean and standard deviation of the mean over N batches
