@@ -55,7 +55,8 @@ enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2, SCN_TRANSPORT_HT = 3 };
 enum { SCN_COLL_STD = 0, SCN_COLL_IMP = 1 };
 enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3, SCN_UNI_CELL = 4 };
 enum { SCN_SURF_APLANE = 1, SCN_SURF_PLANE = 2, SCN_SURF_SPHERE = 3, SCN_SURF_CYLINDER = 4,
-       SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7 };
+       SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7,
+       SCN_SURF_TRUNC_CONE = 8, SCN_SURF_WEDGE = 9 };
 enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2 };
 enum { SCN_GRID_LIN = 1, SCN_GRID_UNSTRUCT = 3 };
 /* macroscopic MT codes (SharedModules/endfConstants.f90); 0 selects fluxResponse */
@@ -107,12 +108,17 @@ typedef struct scn_box {
  *  QUADRIC        quadric (quadric_class.f90): c[0..9] the ten coefficients
  *  BOX            box or x/y/zSquareCylinder: axis = index into boxes
  *  TRUNC_CYLINDER x/y/zTruncCylinder (truncCylinder_class.f90): axis, c[0..2] origin, c[3] radius,
- *                 c[4] halfwidth                                                               */
+ *                 c[4] halfwidth
+ *  TRUNC_CONE     x/y/zTruncCone (truncCone_class.f90): axis, c[0..2] vertex, c[3] tan(angle)**2,
+ *                 c[4] hMin, c[5] hMax; tol = SURF_TOL * (|hMin| + |hMax|)/2 * tan(angle)
+ *  WEDGE          x/y/zWedge (wedge_class.f90): axis, c[0..2] origin, c[3] halfwidth, c[4] altitude,
+ *                 c[5..6] normal1, c[7..8] normal2, c[9..10] normal3 (in-plane components, as
+ *                 build computes them from opening and rotation); tol = SURF_TOL * halfwidth    */
 typedef struct scn_surface {
     int32_t type;
     int32_t axis;
     double tol;
-    double c[10];
+    double c[12];
 } scn_surface;
 
 /* Universes (Geometry/Universes/{root,lat,pin,cell}Universe_class.f90).  uniIdx is the 1-based position in the table.

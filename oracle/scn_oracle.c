@@ -237,6 +237,211 @@ static void o_quadric_normal(const double* a, const double r[3], double n[3])
     for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
 }
 
+#define O_EPS 2.220446049250313e-16   /* epsilon(ONE) */
+
+/* truncCone_class.f90:159-170 evaluate; c = vertex[3], tan^2, hMin, hMax */
+static double o_cone_evaluate(const scn_surface* S, const double r[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int a = S->axis;
+    double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]], da = r[a] - c[a];
+    double v = (0.0 + d0 * d0 + d1 * d1) - c[3] * (da * da);
+    double cMin = -da + c[4];
+    double cMax = da - c[5];
+    return scn_fmax(v, scn_fmax(cMin, cMax));
+}
+
+/* truncCone_class.f90:183-262 distance */
+static double o_cone_distance(const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int ax = S->axis;
+    double diff[3], orient[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; ++i) diff[i] = r[i] - c[i];
+    orient[ax] = scn_sign(1.0, c[4]);
+    double tsq = c[3];
+    double a = (0.0 + u[p[0]] * u[p[0]] + u[p[1]] * u[p[1]]) - tsq * (u[ax] * u[ax]);
+    double b = (0.0 + diff[p[0]] * u[p[0]] + diff[p[1]] * u[p[1]]) - tsq * diff[ax] * u[ax];
+    double cc = (0.0 + diff[p[0]] * diff[p[0]] + diff[p[1]] * diff[p[1]]) - tsq * (diff[ax] * diff[ax]);
+    double delta = b * b - a * cc;
+    double d1, d2, t[3];
+    if (fabs(a) < O_EPS) {
+        d1 = -(0.5 * cc / b);
+        d2 = scn_sign(SCN_INF, o_dot3(u, orient));
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d1 * u[i];
+        if (!(o_dot3(t, orient) > 0.0)) d1 = d2;
+    } else if (delta < O_EPS) {
+        d1 = -SCN_INF;
+        d2 = SCN_INF;
+    } else {
+        d1 = -(b + sqrt(delta)) / a;
+        d2 = -(b - sqrt(delta)) / a;
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d1 * u[i];
+        if (!(o_dot3(t, orient) > 0.0)) d1 = scn_sign(SCN_INF, o_dot3(u, orient));
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d2 * u[i];
+        if (!(o_dot3(t, orient) > 0.0)) d2 = scn_sign(SCN_INF, o_dot3(u, orient));
+    }
+    double near = scn_fmin(d1, d2), far = scn_fmax(d1, d2);
+    if (fabs(u[ax]) > O_EPS) {
+        d1 = (c[5] - diff[ax]) / u[ax];
+        d2 = (c[4] - diff[ax]) / u[ax];
+    } else if (diff[ax] > c[4] && diff[ax] < c[5]) {
+        d1 = -SCN_INF;
+        d2 = SCN_INF;
+    } else {
+        d1 = -SCN_INF;
+        d2 = -SCN_INF;
+    }
+    far = scn_fmin(far, scn_fmax(d1, d2));
+    near = scn_fmax(near, scn_fmin(d1, d2));
+    double dist;
+    if (far <= near * SCN_MISS_TOL) dist = SCN_INF;
+    else if (fabs(o_cone_evaluate(S, r)) < S->tol) dist = (fabs(far) >= fabs(near)) ? far : near;
+    else dist = (near <= 0.0) ? far : near;
+    if (dist <= 0.0 || dist > SCN_INF) dist = SCN_INF;
+    return dist;
+}
+
+/* truncCone_class.f90:272-316 going through normal (normalised by norm2) */
+static int o_cone_going(const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int ax = S->axis;
+    double n[3] = {0.0, 0.0, 0.0};
+    double da = r[ax] - c[ax];
+    if (fabs(da - c[4]) < S->tol) {
+        n[ax] = -1.0;
+    } else if (fabs(da - c[5]) < S->tol) {
+        n[ax] = 1.0;
+    } else {
+        n[p[0]] = r[p[0]] - c[p[0]];
+        n[p[1]] = r[p[1]] - c[p[1]];
+        n[ax] = -c[3] * da;
+    }
+    double nn = scn_norm2(n);
+    for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+    double proj = o_dot3(n, u);
+    if (proj == 0.0) return o_cone_evaluate(S, r) >= 0.0;
+    return proj > 0.0;
+}
+
+/* wedge_class.f90:195-210 evaluate; c = origin[3], halfwidth, altitude, n1[2], n2[2], n3[2] */
+static double o_wedge_evaluate(const scn_surface* S, const double r[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int ax = S->axis;
+    double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]];
+    double cAxis = fabs(r[ax] - c[ax]) - c[3];
+    double cSide1 = 0.0 + c[5] * d0 + c[6] * d1;
+    double cSide2 = 0.0 + c[7] * d0 + c[8] * d1;
+    double cBase = (0.0 + c[9] * d0 + c[10] * d1) - c[4];
+    return scn_fmax(scn_fmax(scn_fmax(cAxis, cBase), cSide1), cSide2);
+}
+
+/* one side plane of the wedge distance: k = u . n, cc = diff . n (minus the altitude for the base) */
+static void o_wedge_side(double k, double cc, double* near, double* far)
+{
+    if (fabs(k) < O_EPS) {
+        if (cc > 0.0) {
+            *near = SCN_INF;
+            *far = -SCN_INF;
+        }
+    } else {
+        double d = -cc / k;
+        if (k > 0.0) *far = scn_fmin(*far, d);
+        else *near = scn_fmax(*near, d);
+    }
+}
+
+/* wedge_class.f90:222-320 distance */
+static double o_wedge_distance(const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int ax = S->axis;
+    double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]], da = r[ax] - c[ax];
+    double near = -SCN_INF, far = SCN_INF;
+    o_wedge_side(0.0 + u[p[0]] * c[5] + u[p[1]] * c[6], 0.0 + d0 * c[5] + d1 * c[6], &near, &far);
+    o_wedge_side(0.0 + u[p[0]] * c[7] + u[p[1]] * c[8], 0.0 + d0 * c[7] + d1 * c[8], &near, &far);
+    {   /* base: c > altitude is outside; d = -(c - altitude) / k */
+        double k = 0.0 + u[p[0]] * c[9] + u[p[1]] * c[10];
+        double cc = 0.0 + d0 * c[9] + d1 * c[10];
+        if (fabs(k) < O_EPS) {
+            if (cc > c[4]) {
+                near = SCN_INF;
+                far = -SCN_INF;
+            }
+        } else {
+            double d = -(cc - c[4]) / k;
+            if (k > 0.0) far = scn_fmin(far, d);
+            else near = scn_fmax(near, d);
+        }
+    }
+    double tn, tf;
+    if (fabs(u[ax]) > O_EPS) {
+        double e1 = (c[3] - da) / u[ax];
+        double e2 = (-c[3] - da) / u[ax];
+        tn = scn_fmin(e1, e2);
+        tf = scn_fmax(e1, e2);
+    } else if (fabs(da) < c[3]) {
+        tn = -SCN_INF;
+        tf = SCN_INF;
+    } else {
+        tn = -SCN_INF;
+        tf = -SCN_INF;
+    }
+    far = scn_fmin(far, tf);
+    near = scn_fmax(near, tn);
+    double dist;
+    if (far <= near * SCN_MISS_TOL) dist = SCN_INF;
+    else if (fabs(o_wedge_evaluate(S, r)) < S->tol) dist = (fabs(far) >= fabs(near)) ? far : near;
+    else if (o_wedge_evaluate(S, r) < 0.0) dist = far;
+    else dist = near;
+    if (dist <= 0.0 || dist > SCN_INF) dist = SCN_INF;
+    return dist;
+}
+
+/* wedge_class.f90:330-380 going through normal (normalised by norm2) */
+static int o_wedge_going(const scn_surface* S, const double r[3], const double u[3])
+{
+    const double* c = S->c;
+    int p[2];
+    o_plane_of(S->axis, p);
+    int ax = S->axis;
+    double d0 = r[p[0]] - c[p[0]], d1 = r[p[1]] - c[p[1]], da = r[ax] - c[ax];
+    double cSide1 = 0.0 + c[5] * d0 + c[6] * d1;
+    double cSide2 = 0.0 + c[7] * d0 + c[8] * d1;
+    double n[3] = {0.0, 0.0, 0.0};
+    if (fabs(da - c[3]) < S->tol) {
+        n[ax] = 1.0;
+    } else if (fabs(da + c[3]) < S->tol) {
+        n[ax] = -1.0;
+    } else if (fabs(cSide1) < S->tol) {
+        n[p[0]] = c[5];
+        n[p[1]] = c[6];
+    } else if (fabs(cSide2) < S->tol) {
+        n[p[0]] = c[7];
+        n[p[1]] = c[8];
+    } else {
+        n[p[0]] = c[9];
+        n[p[1]] = c[10];
+    }
+    double nn = scn_norm2(n);
+    for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+    double proj = o_dot3(n, u);
+    if (fabs(proj) < O_EPS) return o_wedge_evaluate(S, r) >= 0.0;
+    return proj > 0.0;
+}
+
 /* surface%evaluate of every type */
 double orc_surf_evaluate(const scn_problem* P, const scn_surface* S, const double r[3])
 {
@@ -272,6 +477,10 @@ double orc_surf_evaluate(const scn_problem* P, const scn_surface* S, const doubl
         double v = ((0.0 + d0 * d0 + d1 * d1) - c[3] * c[3]) / c[3] * 0.5;
         return scn_fmax(v, fabs(r[a] - c[a]) - c[4]);
     }
+    case SCN_SURF_TRUNC_CONE:
+        return o_cone_evaluate(S, r);
+    case SCN_SURF_WEDGE:
+        return o_wedge_evaluate(S, r);
     }
     o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
     return 0.0;
@@ -408,6 +617,10 @@ double orc_surf_distance(const scn_problem* P, const scn_surface* S, const doubl
         if (d <= 0.0 || d > SCN_INF) d = SCN_INF;
         return d;
     }
+    case SCN_SURF_TRUNC_CONE:
+        return o_cone_distance(S, r, u);
+    case SCN_SURF_WEDGE:
+        return o_wedge_distance(S, r, u);
     }
     o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
     return SCN_INF;
@@ -464,6 +677,10 @@ static int o_surf_going(const scn_problem* P, const scn_surface* S, const double
         if (proj == 0.0) return cc >= 0.0;
         return proj > 0.0;
     }
+    case SCN_SURF_TRUNC_CONE:
+        return o_cone_going(S, r, u);
+    case SCN_SURF_WEDGE:
+        return o_wedge_going(S, r, u);
     }
     o_fail(SCN_ERR_ARG, "unknown surface type %d", S->type);
     return 0;

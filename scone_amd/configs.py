@@ -490,7 +490,7 @@ def absorbing_sphere(radius: float = 1.7, sig: float = 0.9, pop: int = 4000, cyc
 
 def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int = 10,
                  transport: str = "transportOperatorST", n: int = 4, fixed: bool = False, outer: bool = False,
-                 track: bool = True):
+                 track: bool = True, shapes: bool = False):
     """Pin lattice built from CSG cells: an n x n lattice (pitch 1.26, reflective box with finite z)
     of two cellUniverse pins made of every surface type the engine reads -- zCylinder and quadric
     annuli with a zTruncCylinder insert (pin A); a square fuel block of x/yPlanes cut by a
@@ -498,7 +498,8 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
     the 7-group materials of ``pin_lattice_7g``, and a mesh flux tally.  ``fixed``: a fixed-source
     solve with a material source in the fuel instead of an eigenvalue run.  ``outer``: the lattice
     sits in a 1 x 1 lattice of its own (4 nesting levels) entered through a 'global' universe.
-    ``track``: a trackClerk scores the per-material flux and fission rate as well."""
+    ``track``: a trackClerk scores the per-material flux and fission rate as well.  ``shapes``: a
+    third pin (C) joins the lattice: a fuel zTruncCone beside a structural zWedge in moderator."""
     mg = pin_lattice_materials(seed, 7)
     surfs = {
         "box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [0.63 * n, 0.63 * n, 2.0]},
@@ -525,8 +526,20 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
         "b_s": {"id": 10, "type": "simpleCell", "surfaces": [10, 5, -6, -7], "filltype": "mat", "material": "moderator"},
         "b_n": {"id": 11, "type": "simpleCell", "surfaces": [10, 5, -6, 8], "filltype": "mat", "material": "moderator"},
     }
+    if shapes:
+        surfs = dict(surfs)
+        surfs["cone"] = {"id": 14, "type": "zTruncCone", "vertex": [0.05, -0.03, -1.5], "angle": 10.0,
+                         "hMin": 0.5, "hMax": 3.0}
+        surfs["wedge"] = {"id": 15, "type": "zWedge", "origin": [-0.1, -0.2, 0.0], "halfwidth": 1.5,
+                          "altitude": 0.4, "opening": 25.0, "rotation": 225.0}
+        cells = dict(cells)
+        cells["c_cone"] = {"id": 20, "type": "simpleCell", "surfaces": [-14], "filltype": "mat", "material": "fuel"}
+        cells["c_wedge"] = {"id": 21, "type": "simpleCell", "surfaces": [14, -15], "filltype": "mat",
+                            "material": "structural"}
+        cells["c_mod"] = {"id": 22, "type": "simpleCell", "surfaces": [14, 15], "filltype": "mat",
+                          "material": "moderator"}
     rng = np.random.default_rng(seed + 77)
-    lat_map = [int(x) for x in rng.choice([21, 22], size=n * n)]
+    lat_map = [int(x) for x in rng.choice([21, 22, 23] if shapes else [21, 22], size=n * n)]
     unis = {"root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<11>" if outer else "u<10>"}}
     if outer:
         unis["outer"] = {"id": 11, "type": "latUniverse", "pitch": [1.26 * n, 1.26 * n, 0.0], "shape": [1, 1, 0],
@@ -543,6 +556,7 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
                 "pinA": {"id": 21, "type": "cellUniverse", "cells": [1, 2, 3, 4]},
                 "pinB": {"id": 22, "type": "cellUniverse", "rotation": [30.0, 0.0, 0.0],
                          "cells": [5, 6, 7, 8, 9, 10, 11]},
+                **({"pinC": {"id": 23, "type": "cellUniverse", "cells": [20, 21, 22]}} if shapes else {}),
                 "lat": {"id": 10, "type": "latUniverse", "pitch": [1.26, 1.26, 0.0], "shape": [n, n, 0],
                         "padMat": "moderator", "map": lat_map}}}
     h = 0.63 * n

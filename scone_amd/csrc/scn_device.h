@@ -377,6 +377,211 @@ __device__ __forceinline__ double quad_root(double a, double k, double c, double
     return scn_fmin(d, SCN_INF);
 }
 
+// ---- truncCone (truncCone_class.f90) and wedge (wedge_class.f90); same operation order as the
+// oracle's restatement of each routine
+#define SCN_EPS 2.220446049250313e-16   // epsilon(ONE)
+
+__device__ __forceinline__ double cone_evaluate(const scn_surface& S, const double r[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1], da = get3(r, S.axis) - c[S.axis];
+    const double v = (0.0 + d0 * d0 + d1 * d1) - c[3] * (da * da);
+    return scn_fmax(v, scn_fmax(-da + c[4], da - c[5]));
+}
+
+__device__ __forceinline__ double cone_distance(const scn_surface& S, const double r[3], const double u[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const int ax = S.axis;
+    double diff[3], orient[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) diff[i] = r[i] - c[i];
+    set3(orient, ax, scn_sign(1.0, c[4]));
+    const double tsq = c[3];
+    const double up0 = get3(u, p0), up1 = get3(u, p1), ua = get3(u, ax);
+    const double dp0 = get3(diff, p0), dp1 = get3(diff, p1), da = get3(diff, ax);
+    const double a = (0.0 + up0 * up0 + up1 * up1) - tsq * (ua * ua);
+    const double b = (0.0 + dp0 * up0 + dp1 * up1) - tsq * da * ua;
+    const double cc = (0.0 + dp0 * dp0 + dp1 * dp1) - tsq * (da * da);
+    const double delta = b * b - a * cc;
+    double d1, d2, t[3];
+    if (fabs(a) < SCN_EPS) {
+        d1 = -(0.5 * cc / b);
+        d2 = scn_sign(SCN_INF, dot3(u, orient));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d1 * u[i];
+        if (!(dot3(t, orient) > 0.0)) d1 = d2;
+    } else if (delta < SCN_EPS) {
+        d1 = -SCN_INF;
+        d2 = SCN_INF;
+    } else {
+        d1 = -(b + sqrt(delta)) / a;
+        d2 = -(b - sqrt(delta)) / a;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d1 * u[i];
+        if (!(dot3(t, orient) > 0.0)) d1 = scn_sign(SCN_INF, dot3(u, orient));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) t[i] = diff[i] + d2 * u[i];
+        if (!(dot3(t, orient) > 0.0)) d2 = scn_sign(SCN_INF, dot3(u, orient));
+    }
+    double near = scn_fmin(d1, d2), far = scn_fmax(d1, d2);
+    if (fabs(ua) > SCN_EPS) {
+        d1 = (c[5] - da) / ua;
+        d2 = (c[4] - da) / ua;
+    } else if (da > c[4] && da < c[5]) {
+        d1 = -SCN_INF;
+        d2 = SCN_INF;
+    } else {
+        d1 = -SCN_INF;
+        d2 = -SCN_INF;
+    }
+    far = scn_fmin(far, scn_fmax(d1, d2));
+    near = scn_fmax(near, scn_fmin(d1, d2));
+    double dist;
+    if (far <= near * SCN_MISS_TOL) dist = SCN_INF;
+    else if (fabs(cone_evaluate(S, r)) < S.tol) dist = (fabs(far) >= fabs(near)) ? far : near;
+    else dist = (near <= 0.0) ? far : near;
+    if (dist <= 0.0 || dist > SCN_INF) dist = SCN_INF;
+    return dist;
+}
+
+__device__ __forceinline__ int cone_going(const scn_surface& S, const double r[3], const double u[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const int ax = S.axis;
+    const double da = get3(r, ax) - c[ax];
+    double n[3] = {0.0, 0.0, 0.0};
+    if (fabs(da - c[4]) < S.tol) {
+        set3(n, ax, -1.0);
+    } else if (fabs(da - c[5]) < S.tol) {
+        set3(n, ax, 1.0);
+    } else {
+        set3(n, p0, get3(r, p0) - c[p0]);
+        set3(n, p1, get3(r, p1) - c[p1]);
+        set3(n, ax, -c[3] * da);
+    }
+    const double nn = scn_norm2(n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+    const double proj = dot3(n, u);
+    if (proj == 0.0) return cone_evaluate(S, r) >= 0.0;
+    return proj > 0.0;
+}
+
+__device__ __forceinline__ double wedge_evaluate(const scn_surface& S, const double r[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1];
+    const double cAxis = fabs(get3(r, S.axis) - c[S.axis]) - c[3];
+    const double cSide1 = 0.0 + c[5] * d0 + c[6] * d1;
+    const double cSide2 = 0.0 + c[7] * d0 + c[8] * d1;
+    const double cBase = (0.0 + c[9] * d0 + c[10] * d1) - c[4];
+    return scn_fmax(scn_fmax(scn_fmax(cAxis, cBase), cSide1), cSide2);
+}
+
+__device__ __forceinline__ void wedge_side(double k, double cc, double& near, double& far)
+{
+    if (fabs(k) < SCN_EPS) {
+        if (cc > 0.0) {
+            near = SCN_INF;
+            far = -SCN_INF;
+        }
+    } else {
+        const double d = -cc / k;
+        if (k > 0.0) far = scn_fmin(far, d);
+        else near = scn_fmax(near, d);
+    }
+}
+
+__device__ __forceinline__ double wedge_distance(const scn_surface& S, const double r[3], const double u[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const int ax = S.axis;
+    const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1], da = get3(r, ax) - c[ax];
+    const double up0 = get3(u, p0), up1 = get3(u, p1), ua = get3(u, ax);
+    double near = -SCN_INF, far = SCN_INF;
+    wedge_side(0.0 + up0 * c[5] + up1 * c[6], 0.0 + d0 * c[5] + d1 * c[6], near, far);
+    wedge_side(0.0 + up0 * c[7] + up1 * c[8], 0.0 + d0 * c[7] + d1 * c[8], near, far);
+    {
+        const double k = 0.0 + up0 * c[9] + up1 * c[10];
+        const double cc = 0.0 + d0 * c[9] + d1 * c[10];
+        if (fabs(k) < SCN_EPS) {
+            if (cc > c[4]) {
+                near = SCN_INF;
+                far = -SCN_INF;
+            }
+        } else {
+            const double d = -(cc - c[4]) / k;
+            if (k > 0.0) far = scn_fmin(far, d);
+            else near = scn_fmax(near, d);
+        }
+    }
+    double tn, tf;
+    if (fabs(ua) > SCN_EPS) {
+        const double e1 = (c[3] - da) / ua;
+        const double e2 = (-c[3] - da) / ua;
+        tn = scn_fmin(e1, e2);
+        tf = scn_fmax(e1, e2);
+    } else if (fabs(da) < c[3]) {
+        tn = -SCN_INF;
+        tf = SCN_INF;
+    } else {
+        tn = -SCN_INF;
+        tf = -SCN_INF;
+    }
+    far = scn_fmin(far, tf);
+    near = scn_fmax(near, tn);
+    const double ev = wedge_evaluate(S, r);
+    double dist;
+    if (far <= near * SCN_MISS_TOL) dist = SCN_INF;
+    else if (fabs(ev) < S.tol) dist = (fabs(far) >= fabs(near)) ? far : near;
+    else if (ev < 0.0) dist = far;
+    else dist = near;
+    if (dist <= 0.0 || dist > SCN_INF) dist = SCN_INF;
+    return dist;
+}
+
+__device__ __forceinline__ int wedge_going(const scn_surface& S, const double r[3], const double u[3])
+{
+    const double* c = S.c;
+    int p0, p1;
+    plane_of(S.axis, p0, p1);
+    const int ax = S.axis;
+    const double d0 = get3(r, p0) - c[p0], d1 = get3(r, p1) - c[p1], da = get3(r, ax) - c[ax];
+    const double cSide1 = 0.0 + c[5] * d0 + c[6] * d1;
+    const double cSide2 = 0.0 + c[7] * d0 + c[8] * d1;
+    double n[3] = {0.0, 0.0, 0.0};
+    if (fabs(da - c[3]) < S.tol) {
+        set3(n, ax, 1.0);
+    } else if (fabs(da + c[3]) < S.tol) {
+        set3(n, ax, -1.0);
+    } else {
+        int k = 9;
+        if (fabs(cSide1) < S.tol) k = 5;
+        else if (fabs(cSide2) < S.tol) k = 7;
+        const double a0 = (k == 5) ? c[5] : (k == 7) ? c[7] : c[9];
+        const double a1 = (k == 5) ? c[6] : (k == 7) ? c[8] : c[10];
+        set3(n, p0, a0);
+        set3(n, p1, a1);
+    }
+    const double nn = scn_norm2(n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
+    const double proj = dot3(n, u);
+    if (fabs(proj) < SCN_EPS) return wedge_evaluate(S, r) >= 0.0;
+    return proj > 0.0;
+}
+
 __device__ __forceinline__ double surf_evaluate(const DevProblem& P, const scn_surface& S, const double r[3])
 {
     const double* c = S.c;
@@ -402,6 +607,8 @@ __device__ __forceinline__ double surf_evaluate(const DevProblem& P, const scn_s
                z * (c[2] * z + c[8]) + c[9];
     }
     case SCN_SURF_BOX: return box_evaluate(P.boxes[S.axis], r);
+    case SCN_SURF_TRUNC_CONE: return cone_evaluate(S, r);
+    case SCN_SURF_WEDGE: return wedge_evaluate(S, r);
     default: {   // SCN_SURF_TRUNC_CYLINDER
         int p0, p1;
         plane_of(S.axis, p0, p1);
@@ -472,6 +679,8 @@ __device__ __forceinline__ double surf_distance(const DevProblem& P, const scn_s
         return quad_root(a, k, cc, S.tol);
     }
     case SCN_SURF_BOX: return box_distance(P.boxes[S.axis], r, u);
+    case SCN_SURF_TRUNC_CONE: return cone_distance(S, r, u);
+    case SCN_SURF_WEDGE: return wedge_distance(S, r, u);
     default: {   // SCN_SURF_TRUNC_CYLINDER: cylinder slab intersected with the axial slab
         int p0, p1;
         plane_of(S.axis, p0, p1);
@@ -560,6 +769,8 @@ __device__ __forceinline__ int surf_halfspace(const DevProblem& P, const scn_sur
         for (int i = 0; i < 3; ++i) n[i] = n[i] / nn;
         return dot3(n, u) >= 0.0;
     }
+    case SCN_SURF_TRUNC_CONE: return cone_going(S, r, u);
+    case SCN_SURF_WEDGE: return wedge_going(S, r, u);
     default: {   // SCN_SURF_TRUNC_CYLINDER
         int p0, p1;
         plane_of(S.axis, p0, p1);

@@ -51,7 +51,7 @@ inline const char* check_cell_tables(const scn_problem* pr)
 {
     for (int i = 0; i < pr->n_surfaces; ++i) {
         const scn_surface& S = pr->surfaces[i];
-        if (S.type < SCN_SURF_APLANE || S.type > SCN_SURF_TRUNC_CYLINDER) return "unknown surface type";
+        if (S.type < SCN_SURF_APLANE || S.type > SCN_SURF_WEDGE) return "unknown surface type";
         if (S.type == SCN_SURF_BOX) {
             if (S.axis < 0 || S.axis >= pr->n_boxes) return "box surface refers to a missing box";
         } else if (S.type != SCN_SURF_PLANE && S.type != SCN_SURF_QUADRIC && S.type != SCN_SURF_SPHERE) {

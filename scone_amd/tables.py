@@ -31,11 +31,13 @@ VOID_MAT = 2147483647
 UNDEF_MAT = 2147483646
 INF = 2.0 ** 63
 SURF_TOL = 1.0e-12
+PI = 4.0 * math.atan(1.0)                  # numPrecision.f90 PI
 FP_REL_TOL = 1.0e-7
 TRANSPORT_ST, TRANSPORT_DT, TRANSPORT_HT = 1, 2, 3
 COLL_STD, COLL_IMP = 0, 1
 UNI_ROOT, UNI_LAT, UNI_PIN, UNI_CELL = 1, 2, 3, 4
 SURF_APLANE, SURF_PLANE, SURF_SPHERE, SURF_CYLINDER, SURF_QUADRIC, SURF_BOX, SURF_TRUNC_CYLINDER = 1, 2, 3, 4, 5, 6, 7
+SURF_TRUNC_CONE, SURF_WEDGE = 8, 9
 OVERLAP_MAT = 2147483645
 CLERK_COLLISION, CLERK_TRACK = 0, 1
 MAP_SPACE, MAP_MATERIAL = 1, 2
@@ -77,7 +79,7 @@ class Universe(C.Structure):
 
 
 class Surface(C.Structure):
-    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("tol", C.c_double), ("c", C.c_double * 10)]
+    _fields_ = [("type", C.c_int32), ("axis", C.c_int32), ("tol", C.c_double), ("c", C.c_double * 12)]
 
 
 class MGData(C.Structure):
@@ -387,7 +389,7 @@ def compile_surface(sd: dict, boxes: list | None = None):
         if len(cf) != 10:
             raise ValueError(f"coeffs needs to have size 10. Has: {len(cf)}")
         S.type = SURF_QUADRIC
-        S.c[:] = cf
+        S.c[:10] = cf
         S.tol = 2.0 * max(abs(x) for x in cf) * SURF_TOL
         bb = [-INF] * 3 + [INF] * 3
     elif t in ("xTruncCylinder", "yTruncCylinder", "zTruncCylinder"):   # truncCylinder_class.f90 init
@@ -405,6 +407,68 @@ def compile_surface(sd: dict, boxes: list | None = None):
         S.c[:5] = o + [r, a]
         bb = [o[i] - r for i in range(3)] + [o[i] + r for i in range(3)]
         bb[S.axis], bb[3 + S.axis] = o[S.axis] - a, o[S.axis] + a
+    elif t in ("xTruncCone", "yTruncCone", "zTruncCone"):         # truncCone_class.f90 init/build
+        v = [float(x) for x in sd["vertex"]]
+        angle = float(sd["angle"])
+        h_min, h_max = float(sd["hMin"]), float(sd["hMax"])
+        if len(v) != 3:
+            raise ValueError(f"Vertex needs to have size 3. It has size: {len(v)}")
+        if angle <= 0.0 or angle >= 90.0:
+            raise ValueError("Opening angle of cone must be in the range 0-90 degrees "
+                             f"(extremes excluded). It is: {angle}")
+        if h_min >= h_max:
+            raise ValueError("hMin is greater than or equal to hMax.")
+        if math.copysign(abs(h_min), h_max) != h_min and h_min != 0.0 and h_max != 0.0:
+            raise ValueError("hMin and hMax have different signs.")
+        tangent = math.tan(angle * PI / 180.0)
+        S.type = SURF_TRUNC_CONE
+        S.axis = _AXIS[t[0]]
+        S.c[:6] = v + [tangent * tangent, h_min, h_max]
+        S.tol = SURF_TOL * ((abs(h_min) + abs(h_max)) * 0.5 * tangent)
+        ax = S.axis
+        r_max = max(abs(h_min), abs(h_max)) * math.sqrt(tangent * tangent)
+        bb = [v[i] - r_max for i in range(3)] + [v[i] + r_max for i in range(3)]
+        bb[ax], bb[3 + ax] = v[ax] + h_min, v[ax] + h_max
+    elif t in ("xWedge", "yWedge", "zWedge"):                     # wedge_class.f90 init/build
+        o = [float(x) for x in sd["origin"]]
+        hw = float(sd["halfwidth"])
+        height = float(sd["altitude"])
+        if int(sd["id"]) <= 0:
+            raise ValueError(f"ID must be <=0. Is: {sd['id']}")
+        if len(o) != 3:
+            raise ValueError(f"origin must have size 3. Has: {len(o)}")
+        if hw <= 0.0:
+            raise ValueError("halfwidth cannot be -ve.")
+        if height <= 0.0:
+            raise ValueError("altitude cannot be -ve.")
+        theta = float(sd["opening"])
+        phi = float(sd.get("rotation", 0.0))
+        if theta <= 0.0 or theta >= 90.0:
+            raise ValueError("Opening angle of wedge must be in the range 0-90 degrees "
+                             f"(extremes excluded). It is: {theta}")
+        theta = theta * PI / 180.0
+        if phi < 0.0 or phi > 360.0:
+            raise ValueError(f"Rotation angle of wedge must be in the range 0-360 degrees. It is: {phi}")
+        phi = phi * PI / 180.0
+        n1 = [math.sin(phi - theta), -math.cos(phi - theta)]
+        n2 = [-math.sin(phi + theta), math.cos(phi + theta)]
+        n3 = [math.cos(phi), math.sin(phi)]
+        S.type = SURF_WEDGE
+        S.axis = _AXIS[t[0]]
+        S.c[:11] = o + [hw, height] + n1 + n2 + n3
+        S.tol = SURF_TOL * hw
+        ax = S.axis
+        p1, p2 = [i for i in range(3) if i != ax]
+        tg1 = [-n1[1], n1[0]]
+        tg2 = [n2[1], -n2[0]]
+        c_t = math.cos(theta)
+        q0 = [o[p1], o[p2]]
+        q1 = [q0[i] + height * tg1[i] / c_t for i in range(2)]
+        q2 = [q0[i] + height * tg2[i] / c_t for i in range(2)]
+        bb = [0.0] * 6
+        bb[ax], bb[3 + ax] = o[ax] - hw, o[ax] + hw
+        bb[p1], bb[p2] = min(q0[0], q1[0], q2[0]), min(q0[1], q1[1], q2[1])
+        bb[3 + p1], bb[3 + p2] = max(q0[0], q1[0], q2[0]), max(q0[1], q1[1], q2[1])
     elif t in ("box", "xSquareCylinder", "ySquareCylinder", "zSquareCylinder"):
         if boxes is None:
             raise ValueError("a box surface needs a box table")
@@ -587,6 +651,9 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
                 U.border = len(boxes)
                 boxes.append(b)
             else:
+                if bsd["type"] in ("xWedge", "yWedge", "zWedge") and any(int(x) != 0 for x in bc):
+                    raise NotImplementedError("reflective and periodic wedge boundaries (wedge_class.f90 "
+                                              "explicitBC/transformBC) are outside the engine's scope")
                 # surface_inter.f90 setBC: other surfaces take vacuum boundaries only
                 if int(bc[0]) != 0:
                     raise ValueError(f"{bsd['type']} supports only VACUUM BCs. Was given: {int(bc[0])}")

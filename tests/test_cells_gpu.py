@@ -1,6 +1,6 @@
 """Parity of the HIP engines with the oracle on CSG-cell geometries (needs a GPU).
 
-Cell universes of every surface type, a rotated universe, a 'global' universe at five nesting
+Cell universes of every surface type (truncated cone and wedge included), a rotated universe, a 'global' universe at five nesting
 levels, a sphere as the root border and track-length clerks run through the general kernels; the
 contract is the one of the lattice/pin geometries: per-history results bit-exact, clerk scores
 within 1e-12 relative (summation order)."""
@@ -17,12 +17,13 @@ pytestmark = pytest.mark.gpu
 TOPS = ["transportOperatorST", "transportOperatorDT", "transportOperatorHT"]
 
 
-@pytest.mark.parametrize("outer", [False, True], ids=["3-level", "5-level"])
+@pytest.mark.parametrize("variant", ["3-level", "5-level", "cone-wedge"])
 @pytest.mark.parametrize("top", TOPS, ids=["ST", "DT", "HT"])
-def test_cell_lattice_cycle_bit_exact(top, outer):
-    print(f"case cell_lattice {top} outer={outer}", flush=True)
+def test_cell_lattice_cycle_bit_exact(top, variant):
+    print(f"case cell_lattice {top} {variant}", flush=True)
     pop = 3000
-    inp, mg = configs.cell_lattice(pop=pop, transport=top, outer=outer)
+    inp, mg = configs.cell_lattice(pop=pop, transport=top, outer=variant == "5-level",
+                                   shapes=variant == "cone-wedge")
     comp = T.compile_problem(inp, mg)
     master = 0x5C3A84C9
     bank = O.fission_source(comp, pop, master)
@@ -72,7 +73,7 @@ def test_fixed_source_cells_and_sphere_border_in_one_batch(top):
     all three and each matches its oracle run."""
     print(f"case fixed cells {top}", flush=True)
     probs = []
-    inp, mg = configs.cell_lattice(pop=1200, active=3, transport=top, fixed=True)
+    inp, mg = configs.cell_lattice(pop=1200, active=3, transport=top, fixed=True, shapes=True)
     probs.append((inp, mg, 1200, 3, inp["buffer"]))
     inp, mg = configs.absorbing_sphere(pop=1500, cycles=3, transport=top)
     probs.append((inp, mg, 1500, 3, 50))

@@ -1,7 +1,7 @@
 """Surfaces, simpleCell and cellUniverse of the oracle, pinned to the reference's unit tests (CPU).
 
 Known answers transcribed from Geometry/Surfaces/Tests/{aPlane,plane,sphere,cylinder,quadric,
-truncCylinder}_test.f90, Geometry/Cells/Tests/simpleCell_test.f90 and
+truncCylinder,truncCone,wedge}_test.f90, Geometry/Cells/Tests/simpleCell_test.f90 and
 Geometry/Universes/Tests/cellUniverse_test.f90 (tolerances as in those tests), plus closed forms:
 rotationMatrix of genericProcedures.f90 against an independent Euler-angle construction, and the
 uncollided escape from a purely absorbing sphere.
@@ -278,6 +278,202 @@ def test_trunc_cylinder(ax):
     assert O.surf_distance(S, r, pe(1.0, -1.0, 0.0)) == INF
     assert O.surf_distance(S, pe(3.5 + eps, 1.3, 1.8), pe(-1.0, 0.0, 0.0)) == pytest.approx(3.0, rel=TOL)
     assert O.surf_distance(S, pe(3.5 - eps, 1.3, 1.8), pe(1.0, 0.0, 0.0)) == INF
+
+
+def _v(ax, va, v1, v2):
+    """A vector with component va on the axis and v1, v2 on the plane axes (the tests' a, p1, p2)."""
+    p1, p2 = PLANE[ax]
+    v = [0.0, 0.0, 0.0]
+    v[ax], v[p1], v[p2] = va, v1, v2
+    return v
+
+
+def _mv(r, eps, u):
+    return [r[i] + eps * u[i] for i in range(3)]
+
+
+# ---- truncCone_test.f90 ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ax", AXES)
+def test_trunc_cone(ax):
+    S, bb = surf(id=52, type="xyz"[ax] + "TruncCone", vertex=[1.0, 1.0, 1.0], angle=45.0, hMin=1.0, hMax=10.0)
+    ref = _v(ax, 2.0, -9.0, -9.0) + _v(ax, 11.0, 11.0, 11.0)
+    assert bb == pytest.approx(ref, abs=1e-6)
+    assert S.tol == pytest.approx(SURF_TOL * 5.5, abs=1e-6)
+    tol = S.tol
+    hs = lambda r, u: O.surf_halfspace(S, r, u)
+    # testHalfspace
+    r, u = _v(ax, 5.0, 5.0, 1.0), unit(_v(ax, 1.0, -1.0, 0.0))
+    assert not hs(r, u)
+    assert not hs(_mv(r, -0.01 * tol, u), u)
+    assert hs(_mv(r, -2.0 * tol, u), u)
+    assert hs(_mv(r, -0.5, u), u)
+    assert not hs(_mv(r, 0.5, u), u)
+    assert hs(r, _v(ax, 0.0, 0.0, -1.0))
+    r, u = _v(ax, 11.0, 4.0, 4.0), unit(_v(ax, 1.0, -1.0, -1.0))
+    assert hs(r, u)
+    assert hs(_mv(r, -0.5 * tol, u), u)
+    assert not hs(_mv(r, -2.0 * tol, u), u)
+    assert not hs(_mv(r, -0.5, u), u)
+    assert hs(_mv(r, 0.5, u), u)
+    assert hs(r, _v(ax, 0.0, 0.0, -1.0))
+    # testDistance
+    d = lambda r, u: O.surf_distance(S, r, u)
+    r = _v(ax, 0.0, 4.0, 1.0)
+    assert d(r, _v(ax, 1.0, 0.0, 0.0)) == pytest.approx(4.0, rel=1e-7)
+    assert d(r, unit(_v(ax, 2.0, -3.0, 0.0))) == pytest.approx(math.sqrt(13.0), rel=1e-7)
+    assert d(r, unit(_v(ax, 1.0, 1.0, 0.0))) == INF
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == INF
+    assert d(r, _v(ax, 0.0, 1.0, 0.0)) == INF
+    r = _v(ax, 4.0, 4.0, 1.0)
+    assert d(r, _v(ax, 0.0, -1.0, 0.0)) == pytest.approx(6.0, rel=1e-7)
+    assert d(r, _v(ax, 0.0, 1.0, 0.0)) == INF
+    assert d(r, _v(ax, 0.0, 0.0, 1.0)) == INF
+    r = _v(ax, 4.0, 4.0 + 0.1 * tol, 1.0)
+    assert d(r, _v(ax, 0.0, -1.0, 0.0)) == pytest.approx(6.0 + 0.5 * tol, rel=1e-7)
+    r = _v(ax, 4.0 + 0.1 * tol, 4.0, 1.0)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == INF
+    r = _v(ax, 8.0, 3.0, 1.0)
+    assert d(r, _v(ax, 1.0, 0.0, 0.0)) == pytest.approx(3.0, rel=1e-7)
+    assert d(r, _v(ax, 0.0, -1.0, 0.0)) == pytest.approx(9.0, rel=1e-7)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == pytest.approx(5.0, rel=1e-7)
+    assert d(r, unit(_v(ax, 1.0, -1.0, 0.0))) == pytest.approx(math.sqrt(2.0) * 3.0, rel=1e-7)
+    r = _v(ax, 8.0, 0.5, 1.0)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == pytest.approx(6.0, rel=1e-7)
+    r = _v(ax, 10.0, 13.0, 1.0)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == INF
+    assert d(r, _v(ax, 1.0, 0.0, 0.0)) == INF
+    assert d(r, unit(_v(ax, 1.0, -1.5, 0.0))) == INF
+    assert d(r, unit(_v(ax, 1.0, -1.0, 0.0))) == INF
+    r = _v(ax, 16.0, 4.0, 1.0)
+    assert d(r, unit(_v(ax, -5.0, -2.0, 0.0))) == pytest.approx(math.sqrt(29.0), rel=1e-7)
+
+
+def test_trunc_cone_rejected_definitions():
+    base = dict(id=3, type="zTruncCone", vertex=[0.0, 0.0, 0.0], angle=30.0, hMin=1.0, hMax=2.0)
+    for bad, msg in [(dict(angle=90.0), "Opening angle"), (dict(hMin=2.0), "hMin is greater"),
+                     (dict(hMin=-1.0), "different signs"), (dict(vertex=[0.0, 0.0]), "Vertex")]:
+        with pytest.raises(ValueError, match=msg):
+            surf(**{**base, **bad})
+
+
+def test_trunc_cone_distance_closed_form():
+    """Rays from inside the truncated cone against the exit point found independently: the first
+    sign change of evaluate along the ray, refined by bisection."""
+    S, _ = surf(type="zTruncCone", vertex=[0.3, -0.2, 0.1], angle=35.0, hMin=1.0, hMax=4.0)
+    rng = np.random.default_rng(11)
+    n_hit = 0
+    while n_hit < 120:
+        r = list(rng.uniform(-3.0, 4.5, 3))
+        if O.surf_evaluate(S, r) > -1e-3:
+            continue
+        u = unit(rng.normal(size=3))
+        d = O.surf_distance(S, r, u)
+        f = lambda t: O.surf_evaluate(S, [r[i] + t * u[i] for i in range(3)])
+        ts = np.linspace(1e-9, 12.0, 4001)
+        vals = np.array([f(t) for t in ts])
+        sg = np.sign(vals)
+        ch = np.nonzero(sg[1:] != sg[:-1])[0]
+        if len(ch) == 0:
+            assert d == INF, (r, u, d)
+            continue
+        lo, hi = ts[ch[0]], ts[ch[0] + 1]
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            if np.sign(f(mid)) == sg[ch[0]]:
+                lo = mid
+            else:
+                hi = mid
+        assert d == pytest.approx(lo, abs=1e-6), (r, u)
+        n_hit += 1
+
+
+def test_trunc_cone_ray_missing_the_cone_as_written():
+    """truncCone distance treats a ray that misses both nappes (delta < 0, 'should never happen')
+    as inside the cone laterally, so from outside it returns the entry into the axial slab: a ray
+    above the top cap passing wide of the cone gets the distance to the hMax plane."""
+    S, _ = surf(type="zTruncCone", vertex=[0.0, 0.0, 0.0], angle=35.0, hMin=1.0, hMax=4.0)
+    u = unit([1.0, 0.0, -0.05])
+    r = [-10.0, 8.0, 4.5]
+    t_plane = (4.0 - 4.5) / u[2]
+    assert O.surf_distance(S, r, u) == pytest.approx(t_plane, rel=1e-12)
+
+
+# ---- wedge_test.f90 -------------------------------------------------------------------------------
+def _wedge(ax):
+    o = _v(ax, 5.0, 1.0, 0.0)
+    return surf(id=8, type="xyz"[ax] + "Wedge", origin=o, halfwidth=4.0, altitude=3.0, opening=30.0,
+                rotation=60.0)
+
+
+@pytest.mark.parametrize("ax", AXES)
+def test_wedge(ax):
+    S, bb = _wedge(ax)
+    ref = _v(ax, 1.0, 1.0, 0.0) + _v(ax, 9.0, 4.0, 6.0 / math.sqrt(3.0))
+    assert bb == pytest.approx(ref, abs=1e-6)
+    assert S.tol == pytest.approx(SURF_TOL * 4.0, abs=1e-6)
+    tol = S.tol
+    hs = lambda r, u: O.surf_halfspace(S, r, u)
+    # testHalfspace
+    assert hs(_v(ax, 7.0, 3.0, 6.0), _v(ax, 0.0, 1.0, 0.0))
+    r = _v(ax, 7.0, 3.0, math.sqrt(3.0))
+    assert not hs(r, _v(ax, 0.0, 1.0, 0.0))
+    r = _v(ax, 7.0, 1.0, math.sqrt(3.0))
+    u = unit(_v(ax, 2.0, 1.0, 0.0))
+    assert not hs(r, u)
+    u = unit(_v(ax, 2.0, -1.0, 0.0))
+    assert hs(r, u)
+    assert hs(_mv(r, 0.01 * tol, u), u)
+    assert not hs(_mv(r, 0.01 * tol, u), [-x for x in u])
+    assert hs(_mv(r, 2.0 * tol, u), u)
+    assert hs(_mv(r, 0.5, u), u)
+    assert not hs(_mv(r, -0.5, u), u)
+    assert not hs(_mv(r, -0.01 * tol, u), [-x for x in u])
+    u2 = _v(ax, 0.0, 0.0, -1.0)
+    assert hs(_mv(r, 0.01 * tol, u2), u2)
+    r = _v(ax, 9.0, 2.0, 2.0)
+    u = unit(_v(ax, 1.0, -1.0, -1.0))
+    assert hs(r, u)
+    assert hs(_mv(r, -0.5 * tol, u), u)
+    assert not hs(_mv(r, -2.0 * tol, u), u)
+    assert not hs(_mv(r, -0.5, u), u)
+    assert hs(_mv(r, 0.5, u), u)
+    assert hs(r, _v(ax, 0.0, 0.0, -1.0))
+    # testDistance
+    d = lambda r, u: O.surf_distance(S, r, u)
+    u = _v(ax, 1.0 / SQRT2, 1.0 / SQRT2, 0.0)
+    assert d(_v(ax, 4.0, 0.0, 2.0), u) == pytest.approx(SQRT2, rel=1e-7)
+    r = _v(ax, -1.0, 0.0, 2.0)
+    assert d(r, u) == pytest.approx(2.0 * SQRT2, rel=1e-7)
+    assert d(r, _v(ax, 1.0, 0.0, 0.0)) == INF
+    assert d(r, unit(_v(ax, 1.0, -1.0, -1.0))) == INF
+    assert d(r, unit(_v(ax, 0.0, 3.0, 4.0))) == INF
+    s3 = math.sqrt(3.0)
+    r = _v(ax, 4.0, 0.5 * (7.0 + s3), 0.5 * (5.0 * s3 - 1.0))
+    assert d(r, unit(_v(ax, 0.0, -0.5, -0.5 * s3))) == pytest.approx(2.0, rel=1e-12)
+    r = _v(ax, 4.0, 1.0, 3.0 * 0.5 / s3)
+    assert d(r, _v(ax, 0.0, 1.0, 0.0)) == pytest.approx(1.5, rel=1e-7)
+    assert d(r, _v(ax, 0.0, -1.0, 0.0)) == INF
+    assert d(r, _v(ax, 0.0, 0.0, 1.0)) == INF
+    r = _v(ax, 4.0, 1.0 - 0.1 * tol, 3.0 * 0.5 / s3)
+    assert d(r, _v(ax, 0.0, 1.0, 0.0)) == pytest.approx(1.5 + 0.1 * tol, rel=1e-7)
+    r = _v(ax, 4.0, 1.0 + 0.1 * tol, 3.0 * 0.5 / s3)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == INF
+    r = _v(ax, 7.0, 3.0, s3)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == pytest.approx(6.0, rel=1e-7)
+    assert d(r, _v(ax, 0.0, -1.0, 0.0)) == pytest.approx(2.0, rel=1e-7)
+    assert d(r, unit(_v(ax, -3.0, 0.5, 0.0))) == pytest.approx(math.sqrt(37.0), rel=1e-7)
+    r = _v(ax, 10.0, 6.0, 1.0)
+    assert d(r, _v(ax, -1.0, 0.0, 0.0)) == INF
+    assert d(r, unit(_v(ax, -1.0, -0.1, 0.0))) == INF
+    assert d(r, unit(_v(ax, 0.0, -1.0, 1.0))) == INF
+
+
+def test_wedge_rejected_definitions_and_root_bcs():
+    base = dict(id=3, type="zWedge", origin=[0.0, 0.0, 0.0], halfwidth=1.0, altitude=1.0, opening=30.0)
+    for bad, msg in [(dict(opening=0.0), "Opening angle"), (dict(rotation=400.0), "Rotation angle"),
+                     (dict(halfwidth=-1.0), "halfwidth"), (dict(altitude=0.0), "altitude")]:
+        with pytest.raises(ValueError, match=msg):
+            surf(**{**base, **bad})
 
 
 # ---- rotationMatrix (genericProcedures.f90:1098-1140) ------------------------------------------
