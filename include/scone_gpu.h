@@ -54,6 +54,10 @@ enum { SCN_TRANSPORT_ST = 1, SCN_TRANSPORT_DT = 2, SCN_TRANSPORT_HT = 3 };
  * sites keep the colliding particle's collision count instead of restarting at 0 */
 enum { SCN_COLL_STD = 0, SCN_COLL_IMP = 1 };
 enum { SCN_UNI_ROOT = 1, SCN_UNI_LAT = 2, SCN_UNI_PIN = 3, SCN_UNI_CELL = 4 };
+/* unionCell records and tokens (operators above every surface index) */
+enum { SCN_UCELL = -1 };
+enum { SCN_UCELL_UNION = 1073741824, SCN_UCELL_INTERSECT = 1073741825, SCN_UCELL_OPEN = 1073741826,
+       SCN_UCELL_CLOSE = 1073741827, SCN_UCELL_COMPLEM = 1073741828 };
 enum { SCN_SURF_APLANE = 1, SCN_SURF_PLANE = 2, SCN_SURF_SPHERE = 3, SCN_SURF_CYLINDER = 4,
        SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7,
        SCN_SURF_TRUNC_CONE = 8, SCN_SURF_WEDGE = 9 };
@@ -134,7 +138,11 @@ typedef struct scn_surface {
  *        1..n_cells), local IDs n_cells+1 / n_cells+2 are the UNDEF / OVERLAP cells.
  *        int_data[cells + i] is the offset in int_data of local cell i+1's record
  *        [n, s_1 .. s_n]: s_k = +-(1-based index into surfaces), + for the positive halfspace
- *        (simpleCell_class.f90).
+ *        (simpleCell_class.f90), or a unionCell (unionCell_class.f90) record
+ *        [SCN_UCELL, simple, m, t_1 .. t_m]: the evaluation tokens after deMorgan, a surface token
+ *        being +-(1-based surface index) as above and an operator one of SCN_UCELL_UNION ..
+ *        SCN_UCELL_COMPLEM; simple = 1 when no union is left (inside then tests the surfaces in
+ *        order, as the reference does).
  * Every type: origin, and rotMat (universe_inter.f90 setTransform, rotationMatrix of
  * genericProcedures.f90) when rotated = 1, stored row-major: rot[3*(i-1) + j-1] = rotMat(i,j).
  * global = 1: the universe is entered from the global position ('global' of setupBase).

@@ -698,10 +698,54 @@ int orc_surf_halfspace(const scn_problem* P, const scn_surface* S, const double 
 
 /* ---- cells: simpleCell_class.f90 (record [n, +-surfIdx ...] in int_data) ------------------ */
 
-/* inside: every surface halfspace agrees with the cell's sense */
+/* unionCell_class.f90 insideComplex: walk the tokens; an operator that decides its group (a union
+ * after a true operand, an intersection after a false one) returns at the top level and skips to
+ * the group's closing bracket inside one (shortCircuit) */
+static int o_union_inside_complex(const scn_problem* P, const int32_t* tok, int m, const double r[3],
+                                  const double u[3])
+{
+    int isIt = 1, depth = 0;
+    for (int i = 0; i < m; ++i) {
+        int el = tok[i];
+        if (el < SCN_UCELL_UNION) {
+            int hs = orc_surf_halfspace(P, &P->surfaces[abs(el) - 1], r, u);
+            isIt = (hs != 0) == (el > 0);
+        } else if ((el == SCN_UCELL_UNION && isIt) || (el == SCN_UCELL_INTERSECT && !isIt)) {
+            if (depth == 0) return isIt;
+            depth -= 1;
+            int d = 1;
+            while (d > 0) {
+                ++i;
+                if (tok[i] == SCN_UCELL_OPEN) d++;
+                else if (tok[i] == SCN_UCELL_CLOSE) d--;
+            }
+        } else if (el == SCN_UCELL_OPEN) {
+            depth += 1;
+        } else if (el == SCN_UCELL_CLOSE) {
+            depth -= 1;
+        }
+    }
+    return isIt;
+}
+
+/* inside: every surface halfspace agrees with the cell's sense (simpleCell, and a unionCell
+ * without unions); a unionCell with unions evaluates its expression */
 static int o_cell_inside(const scn_problem* P, int rec, const double r[3], const double u[3])
 {
     const int32_t* c = P->int_data + rec;
+    if (c[0] == SCN_UCELL) {
+        const int32_t* tok = c + 3;
+        if (!c[1]) return o_union_inside_complex(P, tok, c[2], r, u);
+        int isIt = 0;
+        for (int i = 0; i < c[2]; ++i) {
+            int s = tok[i];
+            if (s >= SCN_UCELL_UNION) continue;
+            int hs = orc_surf_halfspace(P, &P->surfaces[abs(s) - 1], r, u);
+            isIt = (hs != 0) == (s > 0);
+            if (!isIt) return 0;
+        }
+        return isIt;
+    }
     int isIt = 0;
     for (int i = 1; i <= c[0]; ++i) {
         int s = c[i];
@@ -719,6 +763,18 @@ static double o_cell_distance(const scn_problem* P, int rec, const double r[3], 
     const int32_t* c = P->int_data + rec;
     double d = SCN_INF;
     *surfIdx = 0;
+    if (c[0] == SCN_UCELL) {   /* unionCell: every surface of the definition, in order */
+        for (int i = 0; i < c[2]; ++i) {
+            int s = c[3 + i];
+            if (s >= SCN_UCELL_UNION) continue;
+            double t = orc_surf_distance(P, &P->surfaces[abs(s) - 1], r, u);
+            if (t < d) {
+                d = t;
+                *surfIdx = abs(s);
+            }
+        }
+        return d;
+    }
     for (int i = 1; i <= c[0]; ++i) {
         double t = orc_surf_distance(P, &P->surfaces[abs(c[i]) - 1], r, u);
         if (t < d) {

@@ -499,7 +499,8 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
     solve with a material source in the fuel instead of an eigenvalue run.  ``outer``: the lattice
     sits in a 1 x 1 lattice of its own (4 nesting levels) entered through a 'global' universe.
     ``track``: a trackClerk scores the per-material flux and fission rate as well.  ``shapes``: a
-    third pin (C) joins the lattice: a fuel zTruncCone beside a structural zWedge in moderator."""
+    third pin (C) joins the lattice, a fuel zTruncCone beside a structural zWedge in moderator, and
+    a fourth (D) of two unionCells: fuel in a cylinder or a sphere, and its complement."""
     mg = pin_lattice_materials(seed, 7)
     surfs = {
         "box": {"id": 1, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [0.63 * n, 0.63 * n, 2.0]},
@@ -538,8 +539,14 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
                             "material": "structural"}
         cells["c_mod"] = {"id": 22, "type": "simpleCell", "surfaces": [14, 15], "filltype": "mat",
                           "material": "moderator"}
+        surfs["pinD_cyl"] = {"id": 16, "type": "zCylinder", "origin": [0.2, 0.0, 0.0], "radius": 0.2}
+        surfs["pinD_ball"] = {"id": 17, "type": "sphere", "origin": [-0.2, 0.1, 0.3], "radius": 0.25}
+        cells["d_in"] = {"id": 23, "type": "unionCell", "surfaces": ["<", "-16", ":", "-17", ">"],
+                         "filltype": "mat", "material": "fuel"}
+        cells["d_out"] = {"id": 24, "type": "unionCell", "surfaces": ["#", "<", "-16", ":", "-17", ">"],
+                          "filltype": "mat", "material": "moderator"}
     rng = np.random.default_rng(seed + 77)
-    lat_map = [int(x) for x in rng.choice([21, 22, 23] if shapes else [21, 22], size=n * n)]
+    lat_map = [int(x) for x in rng.choice([21, 22, 23, 24] if shapes else [21, 22], size=n * n)]
     unis = {"root": {"id": 1000, "type": "rootUniverse", "border": 1, "fill": "u<11>" if outer else "u<10>"}}
     if outer:
         unis["outer"] = {"id": 11, "type": "latUniverse", "pitch": [1.26 * n, 1.26 * n, 0.0], "shape": [1, 1, 0],
@@ -556,7 +563,8 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
                 "pinA": {"id": 21, "type": "cellUniverse", "cells": [1, 2, 3, 4]},
                 "pinB": {"id": 22, "type": "cellUniverse", "rotation": [30.0, 0.0, 0.0],
                          "cells": [5, 6, 7, 8, 9, 10, 11]},
-                **({"pinC": {"id": 23, "type": "cellUniverse", "cells": [20, 21, 22]}} if shapes else {}),
+                **({"pinC": {"id": 23, "type": "cellUniverse", "cells": [20, 21, 22]},
+                    "pinD": {"id": 24, "type": "cellUniverse", "cells": [23, 24]}} if shapes else {}),
                 "lat": {"id": 10, "type": "latUniverse", "pitch": [1.26, 1.26, 0.0], "shape": [n, n, 0],
                         "padMat": "moderator", "map": lat_map}}}
     h = 0.63 * n

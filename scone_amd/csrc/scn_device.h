@@ -793,10 +793,51 @@ __device__ __forceinline__ int surf_halfspace(const DevProblem& P, const scn_sur
 }
 
 // simpleCell_class.f90 inside / distance over the record [n, +-surfIdx ...] at ints[rec]
+// unionCell_class.f90 insideComplex over the record's tokens (shortCircuit skips to the group's
+// closing bracket; the host checked that brackets balance)
+__device__ __noinline__ int union_inside_complex(const DevProblem& P, const int* tok, int m, const double r[3],
+                                                 const double u[3])
+{
+    int isIt = 1, depth = 0;
+    for (int i = 0; i < m; ++i) {
+        const int el = tok[i];
+        if (el < SCN_UCELL_UNION) {
+            const int hs = surf_halfspace(P, P.surfs[abs(el) - 1], r, u);
+            isIt = hs == (el > 0);
+        } else if ((el == SCN_UCELL_UNION && isIt) || (el == SCN_UCELL_INTERSECT && !isIt)) {
+            if (depth == 0) return isIt;
+            depth -= 1;
+            int d = 1;
+            while (d > 0 && i + 1 < m) {
+                ++i;
+                if (tok[i] == SCN_UCELL_OPEN) d++;
+                else if (tok[i] == SCN_UCELL_CLOSE) d--;
+            }
+        } else if (el == SCN_UCELL_OPEN) {
+            depth += 1;
+        } else if (el == SCN_UCELL_CLOSE) {
+            depth -= 1;
+        }
+    }
+    return isIt;
+}
+
 __device__ __forceinline__ int cell_inside(const DevProblem& P, int rec, const double r[3], const double u[3])
 {
     const int* c = P.ints + rec;
     const int n = c[0];
+    if (n == SCN_UCELL) {
+        if (!c[1]) return union_inside_complex(P, c + 3, c[2], r, u);
+        int isIt = 0;
+        for (int i = 0; i < c[2]; ++i) {
+            const int s = c[3 + i];
+            if (s >= SCN_UCELL_UNION) continue;
+            const int hs = surf_halfspace(P, P.surfs[abs(s) - 1], r, u);
+            isIt = hs == (s > 0);
+            if (!isIt) return 0;
+        }
+        return isIt;
+    }
     if (n <= 0) return 0;
     for (int i = 1; i <= n; ++i) {
         const int s = c[i];
@@ -812,6 +853,19 @@ __device__ __forceinline__ double cell_distance(const DevProblem& P, int rec, co
     const int* c = P.ints + rec;
     double d = SCN_INF;
     int si = 0;
+    if (c[0] == SCN_UCELL) {   // unionCell: every surface of the definition, in order
+        for (int i = 0; i < c[2]; ++i) {
+            const int s = c[3 + i];
+            if (s >= SCN_UCELL_UNION) continue;
+            const double t = surf_distance(P, P.surfs[abs(s) - 1], r, u);
+            if (t < d) {
+                d = t;
+                si = abs(s);
+            }
+        }
+        *surfIdx = si;
+        return d;
+    }
     for (int i = 1; i <= c[0]; ++i) {
         const double t = surf_distance(P, P.surfs[abs(c[i]) - 1], r, u);
         if (t < d) {

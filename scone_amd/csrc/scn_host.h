@@ -86,6 +86,26 @@ inline const char* check_cell_tables(const scn_problem* pr)
         for (int c = 0; c < U.n_cells; ++c) {
             const int rec = pr->int_data[U.cells + c];
             if (rec < 0 || rec >= pr->n_int) return "cell record outside int_data";
+            if (pr->int_data[rec] == SCN_UCELL) {
+                // unionCell: [SCN_UCELL, simple, m, tokens]; surfaces in range, operators known,
+                // brackets balanced (the short-circuit skip relies on it)
+                if ((int64_t)rec + 3 > pr->n_int) return "cell record outside int_data";
+                const int m = pr->int_data[rec + 2];
+                if (m < 1 || (int64_t)rec + 3 + m > pr->n_int) return "cell record outside int_data";
+                int depth = 0;
+                for (int k = 0; k < m; ++k) {
+                    const int t = pr->int_data[rec + 3 + k];
+                    if (t >= SCN_UCELL_UNION) {
+                        if (t > SCN_UCELL_COMPLEM) return "unknown unionCell operator";
+                        if (t == SCN_UCELL_OPEN) ++depth;
+                        if (t == SCN_UCELL_CLOSE && --depth < 0) return "unbalanced unionCell brackets";
+                    } else if (t == 0 || t > pr->n_surfaces || -t > pr->n_surfaces) {
+                        return "cell surface outside the surface table";
+                    }
+                }
+                if (depth != 0) return "unbalanced unionCell brackets";
+                continue;
+            }
             const int n = pr->int_data[rec];
             if (n < 0 || (int64_t)rec + 1 + n > pr->n_int) return "cell record outside int_data";
             for (int k = 1; k <= n; ++k) {

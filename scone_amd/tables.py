@@ -38,6 +38,8 @@ COLL_STD, COLL_IMP = 0, 1
 UNI_ROOT, UNI_LAT, UNI_PIN, UNI_CELL = 1, 2, 3, 4
 SURF_APLANE, SURF_PLANE, SURF_SPHERE, SURF_CYLINDER, SURF_QUADRIC, SURF_BOX, SURF_TRUNC_CYLINDER = 1, 2, 3, 4, 5, 6, 7
 SURF_TRUNC_CONE, SURF_WEDGE = 8, 9
+UCELL = -1
+U_UNION, U_INTERSECT, U_OPEN, U_CLOSE, U_COMPLEM = 1 << 30, (1 << 30) + 1, (1 << 30) + 2, (1 << 30) + 3, (1 << 30) + 4
 OVERLAP_MAT = 2147483645
 CLERK_COLLISION, CLERK_TRACK = 0, 1
 MAP_SPACE, MAP_MATERIAL = 1, 2
@@ -334,6 +336,123 @@ def rotation_matrix(phi, theta, psi):
 
 
 _AXIS = {"x": 0, "y": 1, "z": 2}
+
+
+def _char_to_int(x):
+    """genericProcedures charToInt: the integer an entry spells, or None."""
+    if isinstance(x, (int, np.integer)) and not isinstance(x, bool):
+        return int(x)
+    try:
+        return int(str(x).strip())
+    except ValueError:
+        return None
+
+
+def union_cell_tokens(definition, cid: int, surf_index):
+    """unionCell_class.f90 init, deMorgan and checkPrecedence: the definition ('<', '>', ':', '#'
+    and signed surface IDs, adjacent items intersected) -> (simple, tokens), surface tokens being
+    +-(surf_index(id) + 1) and operators U_*."""
+    ev, senses = [], []          # ev: 1-based occurrence numbers of surfaces, or operator codes
+    surface_prev = close_prev = False
+    depth = 0
+    for item in definition:
+        num = _char_to_int(item)
+        if num is None:
+            e = str(item).strip()
+            if e == "<":
+                depth += 1
+                if surface_prev or close_prev:
+                    ev.append(U_INTERSECT)
+                ev.append(U_OPEN)
+                close_prev = False
+            elif e == ">":
+                depth -= 1
+                ev.append(U_CLOSE)
+                close_prev = True
+            elif e == ":":
+                ev.append(U_UNION)
+                close_prev = False
+            elif e == "#":
+                if surface_prev or close_prev:
+                    ev.append(U_INTERSECT)
+                ev.append(U_COMPLEM)
+                close_prev = False
+            else:
+                raise ValueError(f"Unrecognised entry in surfaces: {e}")
+            surface_prev = False
+        else:
+            if surface_prev or close_prev:
+                ev.append(U_INTERSECT)
+            if num == 0:
+                raise ValueError(f"Surface ID 0 in cell {cid}")
+            k = surf_index(abs(num)) + 1
+            senses.append(k if num > 0 else -k)
+            ev.append(len(senses))
+            surface_prev = True
+            close_prev = False
+    if not ev:
+        raise ValueError(f"Cell {cid} has no surfaces")
+    if depth != 0:
+        raise ValueError("Inconsistent use of parentheses in cell definition")
+    if ev[-1] == U_UNION:
+        raise ValueError("Cell definition ends with a union")
+    if ev[-1] == U_COMPLEM:
+        raise ValueError("Cell definition ends with a complement")
+    # deMorgan (unionCell_class.f90): drop each complement and flip what it applies to -- the next
+    # surface, or the bracketed group it opens -- in place; the scan then moves on by one
+    i = 0
+    while i < len(ev) - 1:
+        if ev[i] == U_COMPLEM:
+            del ev[i]
+            end = i
+            if ev[i] == U_OPEN:
+                d, j = 1, i + 1
+                while j < len(ev):
+                    if ev[j] == U_OPEN:
+                        d += 1
+                    elif ev[j] == U_CLOSE:
+                        d -= 1
+                    if d == 0:
+                        break
+                    j += 1
+                end = min(j, len(ev) - 1)
+            for j in range(i, end + 1):
+                if ev[j] < U_UNION:
+                    senses[ev[j] - 1] = -senses[ev[j] - 1]
+                elif ev[j] == U_UNION:
+                    ev[j] = U_INTERSECT
+                elif ev[j] == U_INTERSECT:
+                    ev[j] = U_UNION
+        i += 1
+    # checkPrecedence: unions and intersections may not mix at one bracket level
+    i = 0
+    n = len(ev)
+    while i < n - 1:
+        el1 = ev[i]
+        if el1 in (U_UNION, U_INTERSECT):
+            j = i + 1
+            jumped = False
+            while j < n:
+                el2 = ev[j]
+                if el2 in (U_UNION, U_INTERSECT):
+                    if el2 != el1:
+                        raise ValueError("The order of operations in a cell is ambiguous")
+                    i = j
+                    jumped = True
+                    break
+                if el2 in (U_OPEN, U_CLOSE):
+                    i = j + 1
+                    jumped = True
+                    break
+                j += 1
+            if jumped:
+                continue
+        i += 1
+    simple = U_UNION not in ev
+    if simple and len({abs(x) for x in senses}) != len(senses):
+        raise ValueError(f"There are repeated surfaces in definition of cell: {cid}")
+    tokens = [senses[e - 1] if e < U_UNION else e for e in ev]
+    return int(simple), tokens
 
 
 def compile_surface(sd: dict, boxes: list | None = None):
@@ -677,6 +796,12 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
                 if cid not in cell_by_id:
                     raise ValueError(f"Cell ID {cid} is not defined")
                 cname, cd = cell_by_id[cid]
+                if cd["type"] == "unionCell":
+                    simple, tokens = union_cell_tokens(cd["surfaces"], cid, surf_ref)
+                    int_data[U.cells + i] = len(int_data)
+                    int_data.extend([UCELL, simple, len(tokens)] + tokens)
+                    f.append(cell_fill(cd))
+                    continue
                 if cd["type"] != "simpleCell":
                     raise NotImplementedError(f"cell type {cd['type']} is outside the engine's scope")
                 sids = [int(x) for x in cd["surfaces"]]

@@ -17,13 +17,13 @@ pytestmark = pytest.mark.gpu
 TOPS = ["transportOperatorST", "transportOperatorDT", "transportOperatorHT"]
 
 
-@pytest.mark.parametrize("variant", ["3-level", "5-level", "cone-wedge"])
+@pytest.mark.parametrize("variant", ["3-level", "5-level", "cone-wedge-union"])
 @pytest.mark.parametrize("top", TOPS, ids=["ST", "DT", "HT"])
 def test_cell_lattice_cycle_bit_exact(top, variant):
     print(f"case cell_lattice {top} {variant}", flush=True)
     pop = 3000
     inp, mg = configs.cell_lattice(pop=pop, transport=top, outer=variant == "5-level",
-                                   shapes=variant == "cone-wedge")
+                                   shapes=variant == "cone-wedge-union")
     comp = T.compile_problem(inp, mg)
     master = 0x5C3A84C9
     bank = O.fission_source(comp, pop, master)

@@ -1,7 +1,7 @@
 """Surfaces, simpleCell and cellUniverse of the oracle, pinned to the reference's unit tests (CPU).
 
 Known answers transcribed from Geometry/Surfaces/Tests/{aPlane,plane,sphere,cylinder,quadric,
-truncCylinder,truncCone,wedge}_test.f90, Geometry/Cells/Tests/simpleCell_test.f90 and
+truncCylinder,truncCone,wedge}_test.f90, Geometry/Cells/Tests/{simpleCell,unionCell}_test.f90 and
 Geometry/Universes/Tests/cellUniverse_test.f90 (tolerances as in those tests), plus closed forms:
 rotationMatrix of genericProcedures.f90 against an independent Euler-angle construction, and the
 uncollided escape from a purely absorbing sphere.
@@ -528,6 +528,128 @@ def test_simple_cell_inside_and_distance():
     assert d == pytest.approx(1.659591794, rel=1e-7) and s == 1
 
 
+# ---- unionCell_test.f90 (one cell in a cellUniverse) --------------------------------------------
+UNION_SURFS = {"b": {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [10.0, 10.0, 10.0]},
+               "surf1": {"id": 13, "type": "sphere", "origin": [0.0, 0.0, 0.0], "radius": 5.0},
+               "surf2": {"id": 15, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [4.6, 4.6, 4.6]},
+               "surf3": {"id": 4, "type": "xTruncCylinder", "origin": [0, 0, 0], "halfwidth": 5.1, "radius": 1.0},
+               "surf4": {"id": 5, "type": "yTruncCylinder", "origin": [0, 0, 0], "halfwidth": 5.1, "radius": 1.0},
+               "surf5": {"id": 6, "type": "zTruncCylinder", "origin": [0, 0, 0], "halfwidth": 5.1, "radius": 1.0}}
+
+
+def _union_cell(definition):
+    cells = {"c": {"id": 3, "type": "unionCell", "surfaces": definition, "filltype": "outside"}}
+    unis = {"root": {"id": 1, "type": "rootUniverse", "border": 500, "fill": "u<7>"},
+            "cu": {"id": 7, "type": "cellUniverse", "cells": [3]}}
+    return geometry_problem(unis, UNION_SURFS, cells=cells)
+
+
+EASY = ["-13", "-15"]
+COMPLEX = ["<", "-13", "-15", ">", "#", "<", "-4", ":", "-5", ":", "-6", ">"]
+
+
+def test_union_cell_inside():
+    easy, cplx = _union_cell(EASY), _union_cell(COMPLEX)
+    u = [1.0, 0.0, 0.0]
+    inside = lambda comp, r: O.uni_findcell(comp, 2, r, u) == 1
+    assert inside(easy, [-4.59, 0.0, 0.0])
+    assert inside(easy, [0.3, 1.3, 0.4])
+    assert inside(cplx, [3.0, 2.3, 0.4])
+    assert inside(cplx, [-1.1, 1.05, 0.0])
+    assert not inside(easy, [0.0, 4.9, 0.0])
+    assert not inside(easy, [4.58, 4.58, -4.58])
+    assert not inside(cplx, [0.1, 0.3, -0.2])
+    assert not inside(cplx, [-4.59, 0.0, 0.25])
+    assert not inside(cplx, [4.58, 4.58, -4.58])
+
+
+def test_union_cell_distance():
+    easy, cplx = _union_cell(EASY), _union_cell(COMPLEX)
+    # surfaces are tabled in order of first reference: easy 13 -> 1, 15 -> 2 (the root box is
+    # a box-table entry); complex 13, 15, 4, 5, 6 -> 1..5
+    d, s = O.uni_distance(easy, 2, 1, [0.2, -0.1, 0.0], [-0.980580676, 0.0, 0.196116135])
+    assert d == pytest.approx(4.895058733, rel=1e-6) and s == 2
+    d, s = O.uni_distance(easy, 2, 1, [6.0, 0.0, 0.0], [-1.0, 0.0, 0.0])
+    assert d == pytest.approx(1.0, rel=1e-6) and s == 1
+    d, s = O.uni_distance(easy, 2, 1, [4.5, 4.5, 4.5], [-0.577350269] * 3)
+    assert d == pytest.approx(2.794228634, rel=1e-6) and s == 1
+    d, s = O.uni_distance(cplx, 2, 1, [1.2, 0.0, 0.0], [0.0, -1.0, 0.0])
+    assert d == pytest.approx(1.0, rel=1e-6) and s == 3
+
+
+@pytest.mark.parametrize("definition,msg", [
+    (["1", ":", "2", "3"], "ambiguous"), (["<", "1", ":", "2"], "parentheses"),
+    (["1", ":"], "ends with a union"), (["1", "#"], "ends with a complement"),
+    (["1", "?"], "Unrecognised"), (["1", "-1"], "repeated surfaces"),
+])
+def test_union_cell_rejected_definitions(definition, msg):
+    with pytest.raises(ValueError, match=msg):
+        T.union_cell_tokens(definition, 9, lambda sid: sid - 1)
+
+
+def test_union_cell_de_morgan_tokens():
+    U, I, OP, CL = T.U_UNION, T.U_INTERSECT, T.U_OPEN, T.U_CLOSE
+    # #<1 #<2 3>> = not 1 or (2 and 3)
+    simple, tok = T.union_cell_tokens(["#", "<", "1", "#", "<", "2", "3", ">", ">"], 1, lambda sid: sid - 1)
+    assert simple == 0 and tok == [OP, -1, U, OP, 2, I, 3, CL, CL]
+    simple, tok = T.union_cell_tokens(["-1", "#", "2"], 1, lambda sid: sid - 1)
+    assert simple == 1 and tok == [-1, I, -2]
+
+
+def test_union_cell_matches_the_boolean_formula():
+    """Random expressions over six planes: the cell's inside equals the formula evaluated with each
+    surface's halfspace (complements, brackets, short circuits and deMorgan all exercised)."""
+    rng = np.random.default_rng(5)
+    planes = {f"p{k}": {"id": 20 + k, "type": "plane", "coeffs": list(rng.normal(size=3)) + [rng.normal() * 0.3]}
+              for k in range(6)}
+
+    def expr(depth):
+        n = int(rng.integers(1, 4))
+        op = ":" if rng.random() < 0.5 else ""
+        items, py = [], []
+        for k in range(n):
+            neg = rng.random() < 0.3
+            if depth > 0 and rng.random() < 0.4:
+                sub, subpy = expr(depth - 1)
+                items += (["#"] if neg else []) + ["<"] + sub + [">"]
+                py.append(("not " if neg else "") + "(" + subpy + ")")
+            else:
+                sid = int(rng.integers(0, 6))
+                sign = -1 if rng.random() < 0.5 else 1
+                if neg:
+                    items += ["#", str(sign * (20 + sid))]
+                else:
+                    items.append(str(sign * (20 + sid)))
+                term = f"({'' if sign > 0 else 'not '}h[{sid}])"
+                py.append(("not " if neg else "") + term)
+            if k < n - 1 and op:
+                items.append(op)
+        return items, (" or " if op else " and ").join(py)
+
+    n_checked = 0
+    for trial in range(40):
+        items, py = expr(2)
+        surfs = dict(planes)
+        surfs["b"] = {"id": 500, "type": "box", "origin": [0.0, 0.0, 0.0], "halfwidth": [3.0, 3.0, 3.0]}
+        cells = {"c": {"id": 3, "type": "unionCell", "surfaces": items, "filltype": "outside"}}
+        unis = {"root": {"id": 1, "type": "rootUniverse", "border": 500, "fill": "u<7>"},
+                "cu": {"id": 7, "type": "cellUniverse", "cells": [3]}}
+        try:
+            comp = geometry_problem(unis, surfs, cells=cells)
+        except ValueError as e:          # an intersection-only cell naming a surface twice
+            assert "repeated surfaces" in str(e)
+            continue
+        S = [T.compile_surface({**planes[f"p{k}"]}, [])[0] for k in range(6)]
+        for _ in range(25):
+            r = list(rng.uniform(-2.0, 2.0, 3))
+            u = unit(rng.normal(size=3))
+            h = [bool(O.surf_halfspace(S[k], r, u)) for k in range(6)]
+            want = eval(py, {"h": h})
+            assert (O.uni_findcell(comp, 2, r, u) == 1) == want, (items, py, r)
+            n_checked += 1
+    assert n_checked >= 500
+
+
 # ---- cellUniverse_test.f90 --------------------------------------------------------------------
 @pytest.fixture(scope="module")
 def cell_uni():
@@ -622,7 +744,7 @@ def test_cell_definitions_rejected():
     bad = {"c": {"id": 1, "type": "simpleCell", "surfaces": [-1], "filltype": "pie"}}
     with pytest.raises(ValueError, match="Unknown type of cell filling"):
         geometry_problem(unis, surfs, cells=bad)
-    bad = {"c": {"id": 1, "type": "unionCell", "surfaces": [-1], "filltype": "outside"}}
+    bad = {"c": {"id": 1, "type": "fancyCell", "surfaces": [-1], "filltype": "outside"}}
     with pytest.raises(NotImplementedError):
         geometry_problem(unis, surfs, cells=bad)
 
