@@ -61,7 +61,7 @@ enum { SCN_UCELL_UNION = 1073741824, SCN_UCELL_INTERSECT = 1073741825, SCN_UCELL
 enum { SCN_SURF_APLANE = 1, SCN_SURF_PLANE = 2, SCN_SURF_SPHERE = 3, SCN_SURF_CYLINDER = 4,
        SCN_SURF_QUADRIC = 5, SCN_SURF_BOX = 6, SCN_SURF_TRUNC_CYLINDER = 7,
        SCN_SURF_TRUNC_CONE = 8, SCN_SURF_WEDGE = 9 };
-enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2 };
+enum { SCN_MAP_SPACE = 1, SCN_MAP_MATERIAL = 2, SCN_MAP_ENERGY = 3, SCN_MAP_CELL = 4 };
 enum { SCN_GRID_LIN = 1, SCN_GRID_UNSTRUCT = 3 };
 /* macroscopic MT codes (SharedModules/endfConstants.f90); 0 selects fluxResponse */
 enum {
@@ -134,15 +134,16 @@ typedef struct scn_surface {
  *        0 = none, 2 = per-cell flags at int_data[offset_map + localID - 1].
  *  PIN : pinUniverse_class.f90  -- annuli sorted ascending, the last one of radius INF*1.1;
  *        real_data[annuli + i] = radius_i, real_data[annuli + n_annuli + i] = radius_i**2.
- *  CELL: cellUniverse_class.f90 -- n_cells simpleCells in definition order (local IDs
- *        1..n_cells), local IDs n_cells+1 / n_cells+2 are the UNDEF / OVERLAP cells.
- *        int_data[cells + i] is the offset in int_data of local cell i+1's record
- *        [n, s_1 .. s_n]: s_k = +-(1-based index into surfaces), + for the positive halfspace
- *        (simpleCell_class.f90), or a unionCell (unionCell_class.f90) record
- *        [SCN_UCELL, simple, m, t_1 .. t_m]: the evaluation tokens after deMorgan, a surface token
- *        being +-(1-based surface index) as above and an operator one of SCN_UCELL_UNION ..
- *        SCN_UCELL_COMPLEM; simple = 1 when no union is left (inside then tests the surfaces in
- *        order, as the reference does).
+ *  CELL: cellUniverse_class.f90 -- n_cells cells in definition order (local IDs 1..n_cells),
+ *        local IDs n_cells+1 / n_cells+2 are the UNDEF / OVERLAP cells.
+ *        int_data[cells + i] is the offset in int_data of local cell i+1's record and
+ *        int_data[cells + n_cells + i] its cellShelf index (1-based, geometry cells in definition
+ *        order).  A simpleCell (simpleCell_class.f90) record is [n, s_1 .. s_n]:
+ *        s_k = +-(1-based index into surfaces), + for the positive halfspace.  A unionCell
+ *        (unionCell_class.f90) record is [SCN_UCELL, simple, m, t_1 .. t_m]: the evaluation tokens
+ *        after deMorgan, a surface token being +-(1-based surface index) and an operator one of
+ *        SCN_UCELL_UNION .. SCN_UCELL_COMPLEM; simple = 1 when no union is left (inside then tests
+ *        the surfaces in order, as the reference does).
  * Every type: origin, and rotMat (universe_inter.f90 setTransform, rotationMatrix of
  * genericProcedures.f90) when rotated = 1, stored row-major: rot[3*(i-1) + j-1] = rotMat(i,j).
  * global = 1: the universe is entered from the global position ('global' of setupBase).
@@ -192,7 +193,9 @@ typedef struct scn_mg_data {
 
 /* ---- tallies: collisionClerk (Tallies/TallyClerks/collisionClerk_class.f90) ----------------- */
 typedef struct scn_map1d {
-    int32_t type;            /* SCN_MAP_SPACE / SCN_MAP_MATERIAL */
+    int32_t type;            /* SCN_MAP_SPACE / _MATERIAL / _ENERGY (multigroup: always bin 0,
+                                energyMap_class.f90 map) / _CELL (cellMap: int_data[data + c] = bin
+                                of cellShelf index c in 0..axis, c = 0 outside cell universes) */
     int32_t axis;            /* SPACE: 0,1,2 */
     int32_t nbins;
     int32_t grid_type;       /* SPACE: SCN_GRID_LIN (real_data[data] = min, [data+1] = step)

@@ -1265,9 +1265,25 @@ typedef struct o_hctx {
 enum { K_IMP_PROD = 0, K_SCATTER_PROD = 1, K_IMP_ABS = 2, K_ANA_LEAK = 3 };
 enum { FATE_NO = 5000, FATE_ABS = 5001, FATE_LEAK = 5002 };
 
-/* spaceMap / materialMap / multiMap (Tallies/TallyMaps) on the particle's global state */
-static int o_map1d_at(const scn_problem* P, const scn_map1d* m, const double rg[3], int mat)
+/* coordList%cell: the cellShelf index of the cell at the lowest level (0 unless that level is a
+ * cell universe in one of its defined cells) */
+static int o_cell_of(const scn_problem* P, const o_coords* c)
 {
+    const o_coord* l = &c->lvl[(c->nesting > 1 ? c->nesting : 1) - 1];
+    const scn_universe* U = o_uni(P, l->uniIdx);
+    if (U->type != SCN_UNI_CELL || l->localID < 1 || l->localID > U->n_cells) return 0;
+    return P->int_data[U->cells + U->n_cells + l->localID - 1];
+}
+
+/* spaceMap / materialMap / energyMap / cellMap / multiMap (Tallies/TallyMaps) on the particle's
+ * state: global position, material, lowest-level cell */
+static int o_map1d_at(const scn_problem* P, const scn_map1d* m, const double rg[3], int mat, int cell)
+{
+    if (m->type == SCN_MAP_ENERGY) return 0;   /* energyMap_class.f90 map: multigroup -> 0 */
+    if (m->type == SCN_MAP_CELL) {             /* cellMap_class.f90 map: binMap%getOrDefault */
+        if (cell < 0 || cell > m->axis) cell = 0;
+        return P->int_data[m->data + cell];
+    }
     if (m->type == SCN_MAP_SPACE) {
         double v = rg[m->axis];
         int idx;
@@ -1297,7 +1313,7 @@ static int o_map1d_at(const scn_problem* P, const scn_map1d* m, const double rg[
 /* tally map of the particle's state: global position and material */
 static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
 {
-    return o_map1d_at(P, m, p->coords.lvl[0].r, p->coords.matIdx);
+    return o_map1d_at(P, m, p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(P, &p->coords));
 }
 
 /* tallyAdmin%reportInColl (tallyAdmin_class.f90:466-486): attachment (keffImplicitClerk,
@@ -1351,7 +1367,7 @@ static void o_report_hist(o_hctx* X, const o_part* p)
 /* tallyAdmin_class.f90:545-568 reportPath to trackClerks (trackClerk_class.f90 reportPath): the
  * path of length L scores response(pre-path material, group) * w * L in the bin of the pre-path
  * state (position and material before the move) */
-static void o_report_path(o_hctx* X, const o_part* p, double L, const double r0[3], int mat0)
+static void o_report_path(o_hctx* X, const o_part* p, double L, const double r0[3], int mat0, int cell0)
 {
     const scn_problem* P = X->P;
     if (!X->bins) return;
@@ -1361,7 +1377,7 @@ static void o_report_path(o_hctx* X, const o_part* p, double L, const double r0[
         if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
         int bin = 1;
         for (int k = 0; k < C->n_maps; ++k) {
-            int b = o_map1d_at(P, &C->maps[k], r0, mat0);
+            int b = o_map1d_at(P, &C->maps[k], r0, mat0, cell0);
             if (b == 0) { bin = 0; break; }
             bin = bin + (b - 1) * C->multi[k];
         }
@@ -1400,11 +1416,12 @@ static void o_surface_tracking(o_hctx* X, o_part* p)
         }
         double r0[3] = {p->coords.lvl[0].r[0], p->coords.lvl[0].r[1], p->coords.lvl[0].r[2]};  /* savePrePath */
         int mat0 = p->coords.matIdx;
+        int cell0 = o_cell_of(P, &p->coords);
         int event = o_move(P, &p->coords, &dist, &cache, P->st_cache);
         X->n_events++;
         p->time = p->time + dist / 1.0;
         if (event == EV_COLL) p->fate = FATE_NO;
-        o_report_path(X, p, dist, r0, mat0);
+        o_report_path(X, p, dist, r0, mat0, cell0);
         mat = p->coords.matIdx;
         if (mat == SCN_OUTSIDE_MAT) {
             p->isDead = 1;
@@ -1530,7 +1547,7 @@ static void o_weight_window(o_hctx* X, o_part* p, int mat)
     const scn_clerk* M = P->ww_map;
     int bin = 1;
     for (int k = 0; k < M->n_maps; ++k) {
-        int b = o_map1d_at(P, &M->maps[k], p->coords.lvl[0].r, p->coords.matIdx);
+        int b = o_map1d_at(P, &M->maps[k], p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(P, &p->coords));
         if (b == 0) { bin = 0; break; }
         bin = bin + (b - 1) * M->multi[k];
     }

@@ -580,6 +580,16 @@ def cell_lattice(seed: int = 2, pop: int = 20000, inactive: int = 5, active: int
     if track:
         tally["track_mats"] = {"type": "trackClerk", "map": mats, "response": ["flux", "fiss"],
                                "flux": {"type": "fluxResponse"}, "fiss": {"type": "macroResponse", "MT": -6}}
+    if shapes:
+        # per-cell flux (cellMap over every cell, the rest in 'undefined'), the same on paths, and an
+        # energyMap clerk (multigroup particles map to no bin)
+        cmap = {"type": "cellMap", "cells": [int(cd["id"]) for cd in cells.values()], "undefBin": "true"}
+        tally["cells"] = {"type": "collisionClerk", "map": cmap, "response": ["flux"], "flux": {"type": "fluxResponse"}}
+        if track:
+            tally["track_cells"] = {"type": "trackClerk", "map": cmap, "response": ["flux"],
+                                    "flux": {"type": "fluxResponse"}}
+        tally["energy"] = {"type": "collisionClerk", "response": ["flux"], "flux": {"type": "fluxResponse"},
+                           "map": {"type": "energyMap", "grid": "log", "min": 1e-11, "max": 20.0, "N": 4}}
     nd = {"handles": {"mg": {"type": "baseMgNeutronDatabase", "PN": "P0"}},
           "materials": {m: {"temp": 300, "composition": {}, "xsFile": m} for m in mg}}
     if fixed:

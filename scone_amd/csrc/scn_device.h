@@ -1295,8 +1295,30 @@ __device__ __forceinline__ double macro_get(const double* x, int MT)
     }
 }
 
-__device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, const double* rg, int mat)
+// coordList%cell: cellShelf index of the lowest-level cell (0 unless that level is a cell universe
+// in one of its defined cells; always 0 in the lattice/pin kernels, which have no cell universes)
+template <int NL, int GEO>
+__device__ __forceinline__ int lowest_cell(const DevProblem& P, const Pt<NL, GEO>& p)
 {
+    if (!GEO) return 0;
+    int u = p.uni[0], lc = p.loc[0];
+#pragma unroll
+    for (int l = 1; l < NL; ++l) {
+        if (l == p.nest - 1) {
+            u = p.uni[l];
+            lc = p.loc[l];
+        }
+    }
+    const scn_universe& U = P.unis[u - 1];
+    if (U.type != SCN_UNI_CELL || lc < 1 || lc > U.n_cells) return 0;
+    return P.ints[U.cells + U.n_cells + lc - 1];
+}
+
+// tally maps: spaceMap, materialMap, energyMap (multigroup: bin 0), cellMap
+__device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, const double* rg, int mat, int cell)
+{
+    if (m.type == SCN_MAP_ENERGY) return 0;
+    if (m.type == SCN_MAP_CELL) return P.ints[m.data + ((cell < 0 || cell > m.axis) ? 0 : cell)];
     if (m.type == SCN_MAP_SPACE) {
         double v = get3(rg, m.axis);
         const double* g = P.real + m.data;
@@ -1385,13 +1407,14 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         for (int i = 0; i < P.fc.width; ++i) bin_add(B, addr + i, 1.0 * flux);
         return P.fc.width;
     }
+    const int cell = lowest_cell(P, p);
     int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];   // collision clerks only
     for (int ci = c0; ci < c1; ++ci) {
         const scn_clerk& C = P.clerks[P.gclerk_idx[ci]];
         if (!C.handle_virtual && isVirtual) continue;
         int bin = 1;
         for (int k = 0; k < C.n_maps; ++k) {
-            int b = map1d(P, C.maps[k], p.r[0], mat);
+            int b = map1d(P, C.maps[k], p.r[0], mat, cell);
             if (b == 0) {
                 bin = 0;
                 break;
@@ -1417,7 +1440,7 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
 // state (global position r0 and material mat0 before the move)
 template <int NL, int GEO>
 __device__ __forceinline__ int report_path(const DevProblem& P, const Pt<NL, GEO>& p, double L, const double r0[3],
-                                           int mat0, Bins& B)
+                                           int mat0, int cell0, Bins& B)
 {
     int nsc = 0;
     const int c0 = P.gtrack_off[p.G - 1], c1 = P.gtrack_off[p.G];
@@ -1425,7 +1448,7 @@ __device__ __forceinline__ int report_path(const DevProblem& P, const Pt<NL, GEO
         const scn_clerk& C = P.clerks[P.gtrack_idx[ci]];
         int bin = 1;
         for (int k = 0; k < C.n_maps; ++k) {
-            const int b = map1d(P, C.maps[k], r0, mat0);
+            const int b = map1d(P, C.maps[k], r0, mat0, cell0);
             if (b == 0) {
                 bin = 0;
                 break;
@@ -1479,17 +1502,18 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, Dist
         }
     }
     double r0[3];   // savePrePath: the pre-path state of track clerks
-    int mat0 = 0;
+    int mat0 = 0, cell0 = 0;
     if (GEO && P.n_track) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) r0[k] = p.r[0][k];
         mat0 = p.mat;
+        cell0 = lowest_cell(P, p);
     }
     int event = move(P, p, dist, cache, st);
     cnt.seg++;
     p.t = p.t + dist / 1.0;
     if (event == EV_COLL) p.fate = FATE_NO;
-    if (GEO && P.n_track) cnt.sc += report_path(P, p, dist, r0, mat0, B);
+    if (GEO && P.n_track) cnt.sc += report_path(P, p, dist, r0, mat0, cell0, B);
     mat = p.mat;
     if (mat == SCN_OUTSIDE_MAT) {
         p.dead = 1;
@@ -1806,7 +1830,7 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL, GEO>& p, doub
         const scn_clerk& M = *P.ww_map;
         int bin = 1;
         for (int k = 0; k < M.n_maps; ++k) {
-            const int b = map1d(P, M.maps[k], p.r[0], p.mat);
+            const int b = map1d(P, M.maps[k], p.r[0], p.mat, lowest_cell(P, p));
             if (b == 0) {
                 bin = 0;
                 break;

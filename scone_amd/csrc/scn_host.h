@@ -82,7 +82,8 @@ inline const char* check_cell_tables(const scn_problem* pr)
         }
         if (U.type != SCN_UNI_CELL) continue;
         if (U.n_cells < 0 || U.n_local != U.n_cells + 2) return "cell universe local cell count";
-        if (U.cells < 0 || (int64_t)U.cells + U.n_cells > pr->n_int) return "cell universe list outside int_data";
+        if (U.cells < 0 || (int64_t)U.cells + 2 * (int64_t)U.n_cells > pr->n_int)
+            return "cell universe list outside int_data";
         for (int c = 0; c < U.n_cells; ++c) {
             const int rec = pr->int_data[U.cells + c];
             if (rec < 0 || rec >= pr->n_int) return "cell record outside int_data";

@@ -329,10 +329,11 @@ class OutputFile:
 
 
 def _print_map(out, axes, multi):
-    """multiMap%print / spaceMap%print / materialMap%print."""
+    """multiMap%print / spaceMap%print / materialMap%print / energyMap%print / cellMap%print."""
     names = []
     for a in axes:
-        names.append(a[1].upper() if a[0] == "space" else "Material")
+        names.append({"space": a[1].upper() if a[0] == "space" else "", "material": "Material",
+                      "energy": "Energy", "cell": "Cell"}[a[0]])
     if multi:
         out.start_array("multiMapOrder", [len(axes)])
         for n in names:
@@ -345,6 +346,23 @@ def _print_map(out, axes, multi):
             for i in range(len(edges) - 1):
                 out.add_value(float(edges[i]))
                 out.add_value(float(edges[i + 1]))
+            out.end_array()
+        elif a[0] == "energy":                     # lower bounds, then upper bounds
+            edges = a[1]
+            N = len(edges) - 1
+            out.start_array(n + "Bounds", [N, 2])
+            for i in range(N):
+                out.add_value(float(edges[i]))
+            for i in range(N):
+                out.add_value(float(edges[i + 1]))
+            out.end_array()
+        elif a[0] == "cell":                       # cell IDs as text, then 'undefined'
+            clist, undef = a[1], a[2]
+            out.start_array(n + "Bins", [1, len(clist) + (1 if undef else 0)])
+            for c in clist:
+                out.add_value(str(int(c)))
+            if undef:
+                out.add_value("undefined")
             out.end_array()
         else:
             mlist, undef = a[1], a[2]

@@ -42,7 +42,7 @@ UCELL = -1
 U_UNION, U_INTERSECT, U_OPEN, U_CLOSE, U_COMPLEM = 1 << 30, (1 << 30) + 1, (1 << 30) + 2, (1 << 30) + 3, (1 << 30) + 4
 OVERLAP_MAT = 2147483645
 CLERK_COLLISION, CLERK_TRACK = 0, 1
-MAP_SPACE, MAP_MATERIAL = 1, 2
+MAP_SPACE, MAP_MATERIAL, MAP_ENERGY, MAP_CELL = 1, 2, 3, 4
 GRID_LIN, GRID_UNSTRUCT = 1, 3
 KAPPA_DEFAULT = float(np.float32(202.27))   # fissionMG_class.f90:64 single-precision literal
 CHI_TOL = float(np.float32(0.01)) * FP_REL_TOL  # fissionMG_class.f90:268
@@ -703,6 +703,7 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         if cid in cell_by_id:
             raise ValueError(f"Cells {name} & {cell_by_id[cid][0]} have the same ID: {cid}")
         cell_by_id[cid] = (name, cd)
+    cell_shelf = {cid: k + 1 for k, cid in enumerate(cell_by_id)}   # cellShelf index (definition order)
     fills = {}
     root_uid = None
 
@@ -790,12 +791,13 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
             U.n_local = len(ids) + 2
             U.check_overlap = int(_bool(ud.get("checkOverlap", 0), "checkOverlap"))
             U.cells = len(int_data)
-            int_data.extend([0] * len(ids))
+            int_data.extend([0] * (2 * len(ids)))     # record offsets, then cellShelf indices
             f = []
             for i, cid in enumerate(ids):
                 if cid not in cell_by_id:
                     raise ValueError(f"Cell ID {cid} is not defined")
                 cname, cd = cell_by_id[cid]
+                int_data[U.cells + len(ids) + i] = cell_shelf[cid]
                 if cd["type"] == "unionCell":
                     simple, tokens = union_cell_tokens(cd["surfaces"], cid, surf_ref)
                     int_data[U.cells + i] = len(int_data)
@@ -1025,6 +1027,60 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
                 M.data = len(int_data)
                 int_data.extend(table)
                 axes.append(("material", mlist, undef))
+            elif sd["type"] == "energyMap":
+                # energyMap_class.f90: bins by energy; a multigroup particle maps to bin 0 (no score)
+                grid = sd.get("grid")
+                if grid in ("lin", "log"):
+                    mn, mx, N = float(sd["min"]), float(sd["max"]), int(sd["N"])
+                    if N < 1:
+                        raise ValueError("Number of bins must be +ve")
+                    if abs((mx - mn) / mx) < 1e-7:
+                        raise ValueError("Minimum value must be smaller then maximum above realtive FP tolerance")
+                    if grid == "lin":
+                        step = (mx - mn) / N
+                        edges = [mn] + [mn + (i - 1) * step for i in range(2, N + 2)]
+                    else:
+                        if mn <= 0.0:
+                            raise ValueError("For logarithmic grid minimum must be +ve")
+                        step = math.log(mx / mn) / N
+                        edges = [mn]
+                        for _ in range(N):
+                            edges.append(edges[-1] * math.exp(step))
+                elif grid == "unstruct":
+                    edges = [float(x) for x in sd["bins"]]
+                    if len(edges) < 2 or any(edges[i + 1] < edges[i] for i in range(len(edges) - 1)):
+                        raise ValueError("Provided grid is not sorted")
+                elif grid == "predef":
+                    raise NotImplementedError("predefined energy grids (preDefEnergyGrids) are outside the "
+                                              "engine's scope")
+                else:
+                    raise ValueError("'grid' keyword must be: lin, log, usntruct or predef")
+                M.type = MAP_ENERGY
+                M.nbins = len(edges) - 1
+                axes.append(("energy", edges))
+            elif sd["type"] == "cellMap":
+                # cellMap_class.f90: bin of the cellShelf index of the lowest-level cell
+                clist = [int(x) for x in sd["cells"]]
+                ub = str(sd.get("undefBin", "false"))
+                if ub in ("yes", "y", "true", "TRUE", "T"):
+                    undef = True
+                elif ub in ("no", "n", "false", "FALSE", "F"):
+                    undef = False
+                else:
+                    raise ValueError(f"{ub} is an unrecognised entry!")
+                for c_id in clist:
+                    if c_id not in cell_shelf:
+                        raise ValueError(f"Cell ID {c_id} is not defined")
+                default = len(clist) + 1 if undef else 0
+                table = [default] * (len(cell_shelf) + 1)
+                for bi, c_id in enumerate(clist):
+                    table[cell_shelf[c_id]] = bi + 1
+                M.type = MAP_CELL
+                M.nbins = len(clist) + (1 if undef else 0)
+                M.data = len(int_data)
+                M.axis = len(cell_shelf)
+                int_data.extend(table)
+                axes.append(("cell", clist, undef))
             else:
                 raise NotImplementedError(f"tally map {sd['type']} is outside the engine's scope")
             K.maps[k] = M

@@ -820,3 +820,57 @@ def test_track_and_collision_estimators_agree_in_a_slab():
     # the mesh sums to the total flux for both estimators
     assert g("mesh")[:, 0].sum() == pytest.approx(g("total")[0, 0], rel=1e-9)
     assert g("track_mesh")[:, 0].sum() == pytest.approx(g("track_total")[0, 0], rel=1e-9)
+
+
+# ---- cellMap and energyMap (Tallies/TallyMaps/Maps1D/{cellMap,energyMap}_class.f90) ----------------
+def test_cell_map_sums_to_the_material_map():
+    """Collision and track estimators binned by cell add up, cell by cell, to the same estimators
+    binned by each cell's material; a multigroup energyMap bins nothing."""
+    inp, mg = configs.cell_lattice(pop=3000, transport="transportOperatorST", shapes=True)
+    comp = T.compile_problem(inp, mg)
+    bank = O.fission_source(comp, 3000, 4242)
+    _, bins, _, _ = O.eigen_cycle(comp, bank, 777, 0.95)
+    info = {c["name"]: c for c in comp.clerk_info}
+    cells = inp["geometry"]["cells"]
+    ids = [int(cd["id"]) for cd in cells.values()]
+    mat_of = {int(cd["id"]): cd.get("material") for cd in cells.values()}
+    order = ["fuel", "structural", "moderator"]
+    for cell_clerk, mat_clerk, width, col in (("cells", "mats", 1, 0), ("track_cells", "track_mats", 2, 0)):
+        cb = bins[info[cell_clerk]["addr"]: info[cell_clerk]["addr"] + info[cell_clerk]["size"]]
+        mb = bins[info[mat_clerk]["addr"]: info[mat_clerk]["addr"] + info[mat_clerk]["size"]]
+        assert cb[-1] == 0.0                                     # every point is in a defined cell
+        for m, name in enumerate(order):
+            s = sum(cb[i] for i, cid in enumerate(ids) if mat_of[cid] == name)
+            assert s == pytest.approx(mb[m * width + col], rel=1e-12), (cell_clerk, name)
+        assert (cb[:-1] > 0).sum() >= 10
+    e = info["energy"]
+    assert e["size"] == 4 and not bins[e["addr"]: e["addr"] + 4].any()
+
+
+def test_energy_and_cell_map_prints():
+    from scone_amd import output as OUT
+    inp, mg = configs.cell_lattice(pop=100, shapes=True)
+    comp = T.compile_problem(inp, mg)
+    out = OUT.OutputFile()
+    OUT.print_tally(out, comp, np.zeros((int(comp.problem.n_bins), 2)))
+    text = out.text()
+    line = [l for l in text.splitlines() if l.startswith("energy_EnergyBounds")][0]
+    vals = [float(x) for x in line.split("[ ")[1].split("]")[0].split(",")]
+    edges = [1e-11]
+    for _ in range(4):
+        edges.append(edges[-1] * math.exp(math.log(20.0 / 1e-11) / 4))
+    assert line.endswith("],4,2);")
+    assert vals[:4] == pytest.approx(edges[:4], rel=1e-5) and vals[4:] == pytest.approx(edges[1:], rel=1e-5)
+    line = [l for l in text.splitlines() if l.startswith("cells_CellBins")][0]
+    assert line.startswith("cells_CellBins = reshape([ {'1'},{'2'},") and "{'undefined'}],1," in line
+
+
+def test_map_definitions_rejected():
+    inp, mg = configs.cell_lattice(pop=100, shapes=True)
+    inp["activeTally"]["energy"]["map"] = {"type": "energyMap", "grid": "predef", "name": "wims69"}
+    with pytest.raises(NotImplementedError):
+        T.compile_problem(inp, mg)
+    inp, mg = configs.cell_lattice(pop=100, shapes=True)
+    inp["activeTally"]["cells"]["map"]["cells"] = [1, 999]
+    with pytest.raises(ValueError, match="Cell ID 999"):
+        T.compile_problem(inp, mg)
