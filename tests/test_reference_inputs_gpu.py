@@ -128,6 +128,27 @@ def test_package_runs_an_input_file_and_writes_scone_output(tmp_path):
     assert lines["active_fiss_Res"] == "reshape([ 1.00000E+02,0.00000E+00],2,1);"
 
 
+def test_package_writes_ascii_json(tmp_path):
+    """The same runner with outputFormat asciiJSON: the file parses as JSON and holds the keff
+    block and the clerk results nested as asciiJSON prints them."""
+    print("case package run json", flush=True)
+    import json
+    from scone_amd import package
+    inp = dictio.read(os.path.join(SLAB, "SCONE_Slab"))
+    inp["pop"], inp["active"], inp["inactive"], inp["seed"] = 5000, 10, 5, 23
+    inp["outputFormat"] = "asciiJSON"
+    inp["nuclearData"]["materials"]["fuel"]["xsFile"] = os.path.join(SLAB, "XS", "URRa_2_1_XSS")
+    path = tmp_path / "slab_input"
+    path.write_text(dictio.dumps(inp))
+    out = package.run_input(str(path), str(tmp_path / "slab_out"))
+    assert out.endswith(".json")
+    d = json.load(open(out))
+    assert d["pop"] == 5000 and d["Active_Cycles"] == 10
+    k, sd = d["keff"]["K_EFF"]
+    assert abs(k - 1.0) < 5 * sd + 3e-3, (k, sd)
+    assert d["active"]["fiss"]["Res"] == [[100.0, 0.0]]
+
+
 def test_core_3d_cycle_bit_exact():
     """70-group 3D core (four nesting levels, vacuum, 2e5-bin mesh): one cycle bit-exact."""
     print("case core3d cycle", flush=True)
