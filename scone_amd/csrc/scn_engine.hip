@@ -994,7 +994,8 @@ int scn_device_info(char* name, int32_t name_len, int32_t* n_cu)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return SCN_ERR_HIP;
     if (name && name_len > 0) {
-        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        if (prop.name[0]) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        else snprintf(name, name_len, "%s", prop.gcnArchName);   // the runtime may leave the marketing name empty
     }
     if (n_cu) *n_cu = prop.multiProcessorCount;
     return 0;
