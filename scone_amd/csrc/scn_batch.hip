@@ -1295,7 +1295,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     size_t o_prod = I.put(pr->mg.prod, (size_t)nM * nG * nG);
     size_t o_chi = I.put(pr->mg.chi, (size_t)nM * nG);
     size_t o_fiss = I.put(pr->mg.fissile, (size_t)nM);
-    size_t o_maj = I.put(pr->mg.majorant, (size_t)nG);
+    const std::vector<double> maj_tab = majorant_table(pr);
+    size_t o_maj = I.put(maj_tab.data(), maj_tab.size());
     size_t o_clerks = I.put(pr->clerks, (size_t)pr->n_clerks);
     size_t o_goff = I.put(goff.data(), goff.size());
     size_t o_gidx = I.put(gidx.data(), gidx.size());

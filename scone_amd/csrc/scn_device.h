@@ -95,7 +95,7 @@ struct DevProblem {
     const double* prod;
     const double* chi;
     const int* fissile;
-    const double* maj;
+    const double* maj;   // [nG][2]: max(majorant(G) + 0, collisionXS) and its reciprocal
     double collision_xs;
     const scn_clerk* clerks;
     const int* gclerk_off;   // [nG+1] clerks whose energy filter admits group g
@@ -1538,9 +1538,10 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, Dist
 template <int NL, int GEO>
 __device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL, GEO>& p, Bins& B, DevStatus* st, Cnt& cnt)
 {
-    double maj = scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
+    // (majorant, 1 / majorant) of the group, evaluated on the host with the same IEEE operations
+    const double maj = P.maj[2 * (p.G - 1)];
     p.track = maj;
-    double majorant_inv = 1.0 / maj;
+    const double majorant_inv = P.maj[2 * (p.G - 1) + 1];
     if (fabs(majorant_inv) > SCN_HUGE) {
         raise_error(st, SCN_ERR_MAJORANT, p.G);
         p.dead = 1;
@@ -1575,7 +1576,7 @@ __device__ __forceinline__ int dt_step(const DevProblem& P, Pt<NL, GEO>& p, Bins
 template <int NL, int GEO>
 __device__ __forceinline__ int ht_use_dt(const DevProblem& P, const Pt<NL, GEO>& p, DevStatus* st)
 {
-    double majorant_inv = 1.0 / scn_fmax(P.maj[p.G - 1] + 0.0, P.collision_xs);
+    const double majorant_inv = P.maj[2 * (p.G - 1) + 1];
     double sigmaT = (p.mat == SCN_VOID_MAT) ? 0.0 : track_mat_xs(P, p.mat, p.G, st);
     double ratio = sigmaT * majorant_inv;
     return ratio > (1.0 - P.ht_cutoff);
@@ -1696,13 +1697,10 @@ __device__ __forceinline__ int collide(const DevProblem& P, Pt<NL, GEO>& p, doub
     const int nG = P.nG;
     int mat = p.mat;
     cnt.col++;
-    double denom = track_mat_xs(P, mat, p.G, st);
-    double probAlpha = 0.0 / denom;
-    if (scn_rng_get(&p.rng) < probAlpha) {
-        raise_error(st, SCN_ERR_UNSUPPORTED, 1);
-        p.dead = 1;
-        return 0;
-    }
+    (void)track_mat_xs(P, mat, p.G, st);   // its material check
+    // the alpha-reaction draw (collisionProcessor_inter.f90 collide): probAlpha = 0 / sigmaT is
+    // +-0 or NaN in multigroup data, which no draw in [0, 1) falls below; the draw is still made
+    (void)scn_rng_get(&p.rng);
     const double* x = xs_of(P, mat, p.G);
     double rr = scn_rng_get(&p.rng);
     int C = 1;

@@ -1068,7 +1068,8 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= upload(E, &d_prod, pr->mg.prod, (size_t)nM * nG * nG);
     if (!rc) rc |= upload(E, &d_chi, pr->mg.chi, (size_t)nM * nG);
     if (!rc) rc |= upload(E, &d_fiss, pr->mg.fissile, (size_t)nM);
-    if (!rc) rc |= upload(E, &d_maj, pr->mg.majorant, (size_t)nG);
+    const std::vector<double> maj_tab = majorant_table(pr);
+    if (!rc) rc |= upload(E, &d_maj, maj_tab.data(), maj_tab.size());
     if (!rc) rc |= upload(E, &d_clerks, pr->clerks, std::max(1, pr->n_clerks));
     if (!rc) rc |= upload(E, &d_goff, goff.data(), goff.size());
     if (!rc) rc |= upload(E, &d_gidx, gidx.data(), std::max<size_t>(1, gidx.size()));

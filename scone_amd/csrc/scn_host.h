@@ -216,6 +216,20 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
     return F;
 }
 
+// delta tracking's per-group majorant (baseMgNeutronDatabase getMajorantXS, floored by the
+// collision XS) and its reciprocal, as transportOperatorDT_class.f90 computes them each flight
+inline std::vector<double> majorant_table(const scn_problem* pr)
+{
+    const int nG = pr->mg.n_groups;
+    std::vector<double> t(2 * (size_t)std::max(nG, 1), 0.0);
+    for (int g = 0; g < nG; ++g) {
+        const double m = scn_fmax(pr->mg.majorant[g] + 0.0, pr->mg.collision_xs);
+        t[2 * g] = m;
+        t[2 * g + 1] = 1.0 / m;
+    }
+    return t;
+}
+
 // scoreMemory_class.f90:537-573 getResult: mean and standard deviation of the mean over N batches
 inline void score_result(double csum, double csum2, int N, double out[2])
 {
