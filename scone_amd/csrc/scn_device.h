@@ -62,9 +62,9 @@ __device__ __forceinline__ uint64_t rng_skip(uint64_t s, int64_t k_in)
 __device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n) { return rng_skip(s, SCN_STRIDE * n); }
 
 // ---- collision clerks of one shared layout -----------------------------------------------------
-// When every collision clerk has the same maps, all of them lin-grid spaceMaps (at most 3), only
-// flux responses, the same width and virtual-collision flag, and each group is admitted by at most
-// one clerk (a mesh flux tally, possibly split into one clerk per group), a score needs no walk of
+// When every collision clerk has the same maps, all of them lin-grid spaceMaps (at most 3), the
+// same responses and virtual-collision flag, and each group is admitted by at most one clerk (a
+// mesh tally, possibly split into one clerk per group), a score needs no walk of
 // the clerk table: the map parameters live in the problem record (wave-uniform loads) and the
 // admitting clerk's first bin comes from one per-group table.  The bins and the arithmetic are the
 // general path's (map1d, collisionClerk reportInColl).
@@ -73,6 +73,7 @@ struct FastColl {
     int n_maps;
     int width;
     int handle_virtual;
+    int resp[8];         // SCN_MT_* of each response (the same for every clerk)
     int axis[3];
     int nbins[3];
     int multi[3];
@@ -1404,7 +1405,12 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         }
         const double flux = P.fc.handle_virtual ? p.w / p.track : p.w / track_mat_xs(P, mat, p.G, st);
         const int64_t addr = a0 + (int64_t)P.fc.width * (bin - 1);
-        for (int i = 0; i < P.fc.width; ++i) bin_add(B, addr + i, 1.0 * flux);
+        for (int i = 0; i < P.fc.width; ++i) {
+            double resp;
+            if (P.fc.resp[i] == SCN_MT_FLUX) resp = 1.0;
+            else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), P.fc.resp[i]);
+            bin_add(B, addr + i, resp * flux);
+        }
         return P.fc.width;
     }
     const int cell = lowest_cell(P, p);

@@ -175,8 +175,6 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
         const scn_clerk& K = pr->clerks[c];
         if (K.kind != SCN_CLERK_COLLISION) continue;
         if (K.n_maps < 0 || K.n_maps > 3 || K.width < 1 || K.width > 8) return F;
-        for (int i = 0; i < K.width; ++i)
-            if (K.resp[i] != SCN_MT_FLUX) return F;
         for (int k = 0; k < K.n_maps; ++k) {
             const scn_map1d& m = K.maps[k];
             if (m.type != SCN_MAP_SPACE || m.grid_type != SCN_GRID_LIN) return F;
@@ -188,6 +186,8 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
         }
         if (K.n_maps != first->n_maps || K.width != first->width || K.handle_virtual != first->handle_virtual)
             return F;
+        for (int i = 0; i < K.width; ++i)
+            if (K.resp[i] != first->resp[i]) return F;
         for (int k = 0; k < K.n_maps; ++k) {
             const scn_map1d &m = K.maps[k], &f = first->maps[k];
             if (m.axis != f.axis || m.nbins != f.nbins || K.multi[k] != first->multi[k]) return F;
@@ -206,6 +206,7 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
     F.n_maps = first->n_maps;
     F.width = first->width;
     F.handle_virtual = first->handle_virtual;
+    for (int i = 0; i < first->width; ++i) F.resp[i] = first->resp[i];
     for (int k = 0; k < first->n_maps; ++k) {
         F.axis[k] = first->maps[k].axis;
         F.nbins[k] = first->maps[k].nbins;
