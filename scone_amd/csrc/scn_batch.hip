@@ -1297,6 +1297,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     size_t o_fiss = I.put(pr->mg.fissile, (size_t)nM);
     const std::vector<double> maj_tab = majorant_table(pr);
     size_t o_maj = I.put(maj_tab.data(), maj_tab.size());
+    const std::vector<double> st_tab = st_track_table(pr);
+    size_t o_stxs = I.put(st_tab.data(), st_tab.size());
     size_t o_clerks = I.put(pr->clerks, (size_t)pr->n_clerks);
     size_t o_goff = I.put(goff.data(), goff.size());
     size_t o_gidx = I.put(gidx.data(), gidx.size());
@@ -1373,6 +1375,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.chi = (const double*)(base + o_chi);
     P.fissile = (const int*)(base + o_fiss);
     P.maj = (const double*)(base + o_maj);
+    P.st_xs = (const double*)(base + o_stxs);
     P.collision_xs = pr->mg.collision_xs;
     P.clerks = (const scn_clerk*)(base + o_clerks);
     P.gclerk_off = (const int*)(base + o_goff);

@@ -117,6 +117,7 @@ struct DevProblem {
     const int* gtrack_off;   // [nG+1] track clerks whose energy filter admits group g
     const int* gtrack_idx;
     FastColl fc;             // shared-layout collision clerks (fc.on = 0: walk the clerk table)
+    const double* st_xs;     // [n_mat*nG + 1][2]: surface tracking's sigmaTrack and its reciprocal
 };
 
 enum { EV_COLL = 1, EV_BOUNDARY = 2, EV_CROSS = 3 };
@@ -1490,7 +1491,13 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, Dist
 {
     double sigmaTrack, sigmaT, dist, invSigmaTrack;
     int mat = p.mat;
-    if (mat == SCN_VOID_MAT) sigmaTrack = P.collision_xs;
+    // (sigmaTrack, 1 / sigmaTrack) of the material and group, tabled on the host with the same
+    // IEEE operations (the void pair last); an index outside the table takes the checked path
+    const bool tabled = mat == SCN_VOID_MAT || (mat >= 1 && mat <= P.n_mat);
+    const size_t ti = !tabled ? 0 : (mat == SCN_VOID_MAT) ? (size_t)P.n_mat * P.nG
+                                                          : (size_t)(mat - 1) * P.nG + (p.G - 1);
+    const double* tt = P.st_xs + 2 * ti;
+    if (tabled) sigmaTrack = tt[0];
     else sigmaTrack = scn_fmax(track_mat_xs(P, mat, p.G, st), P.collision_xs);
     p.track = sigmaTrack;
     if (sigmaTrack < SCN_ST_TOL) {
@@ -1498,7 +1505,7 @@ __device__ __forceinline__ int st_step(const DevProblem& P, Pt<NL, GEO>& p, Dist
         invSigmaTrack = SCN_INF;
         sigmaT = 0.0;
     } else {
-        invSigmaTrack = 1.0 / sigmaTrack;
+        invSigmaTrack = tabled ? tt[1] : 1.0 / sigmaTrack;
         dist = -scn_log(scn_rng_get(&p.rng)) * invSigmaTrack;
         sigmaT = track_mat_xs(P, mat, p.G, st);
         if (dist != dist) {

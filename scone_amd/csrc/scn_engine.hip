@@ -1070,6 +1070,9 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= upload(E, &d_fiss, pr->mg.fissile, (size_t)nM);
     const std::vector<double> maj_tab = majorant_table(pr);
     if (!rc) rc |= upload(E, &d_maj, maj_tab.data(), maj_tab.size());
+    const std::vector<double> st_tab = st_track_table(pr);
+    double* d_stxs = nullptr;
+    if (!rc) rc |= upload(E, &d_stxs, st_tab.data(), st_tab.size());
     if (!rc) rc |= upload(E, &d_clerks, pr->clerks, std::max(1, pr->n_clerks));
     if (!rc) rc |= upload(E, &d_goff, goff.data(), goff.size());
     if (!rc) rc |= upload(E, &d_gidx, gidx.data(), std::max<size_t>(1, gidx.size()));
@@ -1102,6 +1105,7 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     P.chi = d_chi;
     P.fissile = d_fiss;
     P.maj = d_maj;
+    P.st_xs = d_stxs;
     P.collision_xs = pr->mg.collision_xs;
     P.clerks = d_clerks;
     P.gclerk_off = d_goff;

@@ -231,6 +231,24 @@ inline std::vector<double> majorant_table(const scn_problem* pr)
     return t;
 }
 
+// surface tracking's sigmaTrack = max(total XS, collisionXS) per material and group and its
+// reciprocal (transportOperatorST_class.f90 surfaceTracking), then the void pair (collisionXS)
+inline std::vector<double> st_track_table(const scn_problem* pr)
+{
+    const int nG = pr->mg.n_groups, nM = pr->mg.n_mat;
+    std::vector<double> t(2 * ((size_t)nM * nG + 1), 0.0);
+    for (int m = 0; m < nM; ++m)
+        for (int g = 0; g < nG; ++g) {
+            const size_t k = (size_t)m * nG + g;
+            const double s = scn_fmax(pr->mg.xs[k * 6 + XS_TOTAL] + 0.0, pr->mg.collision_xs);
+            t[2 * k] = s;
+            t[2 * k + 1] = 1.0 / s;
+        }
+    t[2 * (size_t)nM * nG] = pr->mg.collision_xs;
+    t[2 * (size_t)nM * nG + 1] = 1.0 / pr->mg.collision_xs;
+    return t;
+}
+
 // scoreMemory_class.f90:537-573 getResult: mean and standard deviation of the mean over N batches
 inline void score_result(double csum, double csum2, int N, double out[2])
 {
