@@ -478,7 +478,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (the engine runs on a single "
-              f"GPU; DESIGN.md describes how it would shard)", file=sys.stderr)
+              f"GPU; sharding the bank over GPUs is a design, not built)", file=sys.stderr)
         sys.exit(2)
     line = eigen_line(args) if args.workload == "eigen" else batch_line(args)
     print(json.dumps(line))

@@ -144,7 +144,7 @@ def test_fortran_binding_links_and_calls_the_library(tmp_path):
                    check=True, cwd=work)
     out = subprocess.run([os.path.join(work, "drive")], capture_output=True, text=True, check=True).stdout.split()
     rc, size = int(out[0]), int(out[1])
-    assert rc != 0                       # SCN_ERR_NO_DEVICE here, SCN_ERR_ARG on a GPU box
+    assert rc != 0                       # SCN_ERR_NO_DEVICE without a GPU, SCN_ERR_ARG with one
     assert size == C.sizeof(T.ParticleState)
     assert int(out[2]) != 0              # invalid batch settings (or no device)
     assert [int(x) for x in out[3:7]] == [C.sizeof(T.Source), C.sizeof(T.FixedRequest), C.sizeof(T.FixedResult),
