@@ -75,12 +75,16 @@ def diff(a, b):
 
 
 def measured_traffic(workload_key):
+    """HBM bytes per launch of the workload's transport kernel from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/transport_traffic.json (None if not profiled), and
+    the summary record itself."""
     path = os.path.join(ROOT, "profiles", "transport_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(workload_key)
+            rec = json.load(f).get(workload_key)
     except (OSError, ValueError):
-        return None
+        rec = None
+    return (rec["bytes_per_launch"] if rec else None), rec
 
 
 # ---------------------------------------------------------------------------------------------
@@ -375,7 +379,8 @@ def eigen_line(args):
         "collisions_per_s": r["coll_per_s"], "k_implicit": r["k_implicit"],
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"pin_lattice_17x17_7g_pop{args.pop}_{tag}"),
+                     "traffic": measured_traffic(f"pin_lattice_17x17_7g_pop{args.pop}_{tag}")[0],
+                     "traffic_source": measured_traffic(f"pin_lattice_17x17_7g_pop{args.pop}_{tag}")[1],
                      "kernel": "k_transport", "kernel_ms_per_launch": r["kernel_ms_per_launch"],
                      "alg_bytes_per_launch": r["alg_bytes_per_launch"], "launches": r["launches"]},
         "cpu_baseline": cpu,
@@ -443,7 +448,8 @@ def batch_line(args):
         "identical_results_sample": identical,
         "roofline": {"bound": "hbm", "achieved": kt["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kt["achieved_gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"batch512_{args.transport[-2:]}_w{best.get('window', 1)}"),
+                     "traffic": measured_traffic(f"batch512_{args.transport[-2:]}_w{best.get('window', 1)}")[0],
+                     "traffic_source": measured_traffic(f"batch512_{args.transport[-2:]}_w{best.get('window', 1)}")[1],
                      "kernel": "k_btransport", "kernel_ms_per_launch": kt["kernel_ms_per_launch"],
                      "alg_bytes_per_launch": kt["alg_bytes_per_launch"], "launches": kt["launches"],
                      "kernel_share_of_cycle": kt["kernel_share"]},
