@@ -53,11 +53,16 @@ def run(args):
     else:
         from scone_amd import configs, tables as T
         from scone_amd.engine import Engine
-        inp, mg = configs.pin_lattice_7g(pop=1000000, transport=args.transport)
+        if args.workload == "core3d":    # 70-group 3D core, 1e7 histories, 1.4e8-bin mesh tally
+            pop, seed = 10_000_000, 5
+            inp, mg = configs.core_3d(pop=pop, transport=args.transport)
+        else:                            # 17x17 pin lattice, 7 groups, 1e6 histories
+            pop, seed = 1_000_000, 2
+            inp, mg = configs.pin_lattice_7g(pop=pop, transport=args.transport)
         comp = T.compile_problem(inp, mg)
-        eng = Engine(comp, 1000000)
-        bank = eng.fission_source(1000000, 2)
-        eng.eigen_start(dict(pop=1000000, inactive=0, active=args.cycles, seed=2), bank)
+        eng = Engine(comp, pop)
+        bank = eng.fission_source(pop, seed)
+        eng.eigen_start(dict(pop=pop, inactive=0, active=args.cycles, seed=seed), bank)
         eng.eigen_step(args.cycles)
         st = eng.stats()
         eng.eigen_finish()
@@ -121,7 +126,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
-    r.add_argument("--workload", choices=["batch512", "eigen17"], default="batch512")
+    r.add_argument("--workload", choices=["batch512", "eigen17", "core3d"], default="batch512")
     r.add_argument("--transport", default="transportOperatorDT")
     r.add_argument("--cycles", type=int, default=2)
     r.add_argument("--tile", type=int, default=0)
