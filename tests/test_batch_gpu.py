@@ -53,6 +53,10 @@ CASES = [
      1500, 4),
     ("slab-track-DT", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorDT", track=True),
      1500, 4),
+    ("slab-HT", lambda: configs.slab_fixed(pop=1500, cycles=4, transport="transportOperatorHT", track=True),
+     1500, 4),
+    ("fissile-HT", lambda: configs.inf_medium_fixed_fissile(pop=1500, cycles=4, transport="transportOperatorHT"),
+     1500, 4),
 ]
 
 
