@@ -30,6 +30,9 @@
 
 using namespace scn;
 
+// pairwise levels a canonical sum may take above its chunk partials (2^40 partials)
+#define SCN_DSUM_MAX_LEVELS 40
+
 namespace {
 
 constexpr int BLOCK = 256;
@@ -663,11 +666,9 @@ int alloc_live(Engine* E, Live& L, int cap, int nl)
     return rc;
 }
 
-int check_status(Engine* E)
+// the engine error a device status read back to the host names (0: none); clears the device copy
+int status_error(Engine* E, const DevStatus& h)
 {
-    DevStatus h{};
-    HIPCHK(hipMemcpyAsync(&h, E->status, sizeof h, hipMemcpyDeviceToHost, E->stream));
-    HIPCHK(hipStreamSynchronize(E->stream));
     if (h.code) {
         char buf[256];
         const char* what = "engine error";
@@ -687,6 +688,14 @@ int check_status(Engine* E)
         return h.code;
     }
     return 0;
+}
+
+int check_status(Engine* E)
+{
+    DevStatus h{};
+    HIPCHK(hipMemcpyAsync(&h, E->status, sizeof h, hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return status_error(E, h);
 }
 
 int ensure_sum(Engine* E, int64_t n)
@@ -718,24 +727,38 @@ int dsum_batch(Engine* E, SumJobs J)
         dim3 g((unsigned)((mmax + 255) / 256), J.njobs);
         hipLaunchKernelGGL(k_dsum_chunks, g, dim3(256), 0, E->stream, J, E->sum_y, ys);
     }
-    double* a = E->sum_y;
-    double* b = E->sum_y2;
-    while (mmax > 2048) {
-        HIPCHK(hipMemcpyAsync(E->sum_m, m, sizeof(int64_t) * J.njobs, hipMemcpyHostToDevice, E->stream));
-        int64_t h = (mmax + 1) / 2;
-        dim3 g((unsigned)((h + 255) / 256), J.njobs);
-        hipLaunchKernelGGL(k_dsum_level, g, dim3(256), 0, E->stream, a, b, E->sum_m, ys);
-        // jobs already at m <= 1 keep their value in place: copy forward through the level kernel
+    // the element counts of every pairwise level, worked out on the host and uploaded at once, so
+    // the levels run back to back with one synchronisation at the end
+    int64_t mt[SCN_DSUM_MAX_LEVELS][8];
+    int levels = 0;
+    for (int64_t mm = mmax; mm > 2048; mm = (mm + 1) / 2) {
+        if (levels + 1 >= SCN_DSUM_MAX_LEVELS) {
+            E->err = "sum too long";
+            return SCN_ERR_ARG;
+        }
         for (int q = 0; q < J.njobs; ++q) {
-            if (m[q] == 1) HIPCHK(hipMemcpyAsync(b + q * ys, a + q * ys, sizeof(double), hipMemcpyDeviceToDevice, E->stream));
+            mt[levels][q] = m[q];
             m[q] = (m[q] + 1) / 2;
         }
-        mmax = (mmax + 1) / 2;
-        std::swap(a, b);
-        HIPCHK(hipStreamSynchronize(E->stream));
+        ++levels;
     }
-    HIPCHK(hipMemcpyAsync(E->sum_m, m, sizeof(int64_t) * J.njobs, hipMemcpyHostToDevice, E->stream));
-    hipLaunchKernelGGL(k_dsum_final, dim3(J.njobs), dim3(1024), 0, E->stream, a, E->sum_m, ys, E->sum_out);
+    for (int q = 0; q < J.njobs; ++q) mt[levels][q] = m[q];
+    HIPCHK(hipMemcpyAsync(E->sum_m, mt, sizeof(int64_t) * 8 * (levels + 1), hipMemcpyHostToDevice, E->stream));
+    double* a = E->sum_y;
+    double* b = E->sum_y2;
+    for (int l = 0; l < levels; ++l) {
+        int64_t h = 0;
+        for (int q = 0; q < J.njobs; ++q) h = std::max(h, (mt[l][q] + 1) / 2);
+        dim3 g((unsigned)((h + 255) / 256), J.njobs);
+        hipLaunchKernelGGL(k_dsum_level, g, dim3(256), 0, E->stream, a, b, E->sum_m + 8 * l, ys);
+        // jobs already at m <= 1 keep their value in place: copy forward through the level kernel
+        for (int q = 0; q < J.njobs; ++q)
+            if (mt[l][q] == 1)
+                HIPCHK(hipMemcpyAsync(b + q * ys, a + q * ys, sizeof(double), hipMemcpyDeviceToDevice, E->stream));
+        std::swap(a, b);
+    }
+    hipLaunchKernelGGL(k_dsum_final, dim3(J.njobs), dim3(1024), 0, E->stream, a, E->sum_m + 8 * levels, ys,
+                       E->sum_out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(E->stream));
     return 0;
@@ -806,6 +829,7 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     DevProblem P = E->P;
     if (!score_bins) P.n_clerks = 0;
     int n_live = n_in;
+    int n_sites_host = 0;
     int cur = 0;
     int it = 0;
     int K = E->k0;
@@ -824,14 +848,18 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
            E->status, E->counters_i + 2, E->chunk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(E->ev1, E->stream));
+        // survivors, status and site count come back together, behind one synchronisation
         int host_n = 0;
+        DevStatus hs{};
         HIPCHK(hipMemcpyAsync(&host_n, nout_ptr, sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipMemcpyAsync(&hs, E->status, sizeof hs, hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipMemcpyAsync(&n_sites_host, E->sites.count, sizeof(int), hipMemcpyDeviceToHost, E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
         E->stats.transport_kernel_ms += ms;
         E->stats.transport_launches += 1;
-        int rc = check_status(E);
+        int rc = status_error(E, hs);
         if (rc) return rc;
         n_live = host_n;
         cur = 1 - cur;
@@ -839,9 +867,7 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
         K *= 2;
         ++it;
     }
-    int ns = 0;
-    HIPCHK(hipMemcpyAsync(&ns, E->sites.count, sizeof(int), hipMemcpyDeviceToHost, E->stream));
-    HIPCHK(hipStreamSynchronize(E->stream));
+    const int ns = n_sites_host;   // read with the last launch's survivors
     if (ns > E->site_cap || ns > E->bank_cap) {
         E->err = "Run out of space for fission sites";
         return SCN_ERR_OVERFLOW;
@@ -1159,7 +1185,7 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= dalloc(E, &E->counters_i, 4);
     if (!rc) rc |= dalloc(E, &E->counters, 1);
     if (!rc) rc |= dalloc(E, &E->status, 1);
-    if (!rc) rc |= dalloc(E, &E->sum_m, 8);
+    if (!rc) rc |= dalloc(E, &E->sum_m, 8 * SCN_DSUM_MAX_LEVELS);
     if (!rc) rc |= dalloc(E, &E->sum_out, 8);
     if (!rc) rc |= dalloc(E, &E->rn, E->bank_cap);
     if (!rc) rc |= dalloc(E, &E->rn_bits, E->bank_cap);
