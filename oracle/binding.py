@@ -103,6 +103,15 @@ def lib():
         L.orc_legendre_p1.argtypes = [C.c_double, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         L.orc_keff_implicit.restype = C.c_double
         L.orc_keff_implicit.argtypes = [dp]
+        L.orc_clerk_bin.restype = C.c_int
+        L.orc_clerk_bin.argtypes = [P, C.c_int32, d3, C.c_int32, C.c_int32]
+        L.orc_score_collision.argtypes = [P, d3, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                          C.c_int32, dp, dp]
+        L.orc_scatter_gout.restype = C.c_int
+        L.orc_scatter_gout.argtypes = [P, C.c_int32, C.c_int32, C.c_double, dp]
+        L.orc_chi_gout.restype = C.c_int
+        L.orc_chi_gout.argtypes = [P, C.c_int32, C.c_double]
+        L.orc_keff_analog.argtypes = [C.c_double, C.c_double, C.c_double, dp]
         _lib = L
     return _lib
 
@@ -347,3 +356,40 @@ def keff_implicit(scores):
 def _s64(x: int) -> int:
     x &= (1 << 64) - 1
     return x - (1 << 64) if x >= (1 << 63) else x
+
+
+def clerk_bin(comp, clerk, r, mat=1, cell=0):
+    """1-based bin of clerk `clerk` (0-based index) for a state at global position r in material
+    mat and lowest-level cell `cell` (cellShelf index); 0 = no bin."""
+    return lib().orc_clerk_bin(C.byref(comp.problem), clerk, _d3(r), mat, cell)
+
+
+def score_collision(comp, r, mat, G, w, track_xs, is_virtual=False, cell=0, bins=None, keff=None):
+    """One tallyAdmin%reportInColl into `bins` (score memory, n_bins) and keff[4]; returns both."""
+    if bins is None:
+        bins = np.zeros(max(1, int(comp.problem.n_bins)))
+    if keff is None:
+        keff = np.zeros(4)
+    lib().orc_score_collision(C.byref(comp.problem), _d3(r), mat, cell, G, float(w), float(track_xs),
+                              1 if is_virtual else 0, bins.ctypes.data_as(C.POINTER(C.c_double)),
+                              keff.ctypes.data_as(C.POINTER(C.c_double)))
+    return bins, keff
+
+
+def scatter_gout(comp, mat, G_in, u):
+    """multiScatterMG sampleGout with draw u: (G_out, weight multiplier); G_out = nG+1 on failure."""
+    w = C.c_double(0.0)
+    g = lib().orc_scatter_gout(C.byref(comp.problem), mat, G_in, float(u), C.byref(w))
+    return g, w.value
+
+
+def chi_gout(comp, mat, u):
+    """fissionMG sampleOut group scan of chi with draw u (nG+1 on failure)."""
+    return lib().orc_chi_gout(C.byref(comp.problem), mat, float(u))
+
+
+def keff_analog(start_w, end_w, k_norm, acc):
+    """keffAnalogClerk closeCycle: accumulates end/start*k_norm into acc (sum, sum of squares)."""
+    a = (C.c_double * 2)(*acc)
+    lib().orc_keff_analog(float(start_w), float(end_w), float(k_norm), a)
+    return [a[0], a[1]]

@@ -1310,51 +1310,60 @@ static int o_map1d_at(const scn_problem* P, const scn_map1d* m, const double rg[
     }
 }
 
-/* tally map of the particle's state: global position and material */
-static int o_map1d(const scn_problem* P, const scn_map1d* m, const o_part* p)
+/* bin of a clerk's (or weight-window field's) map for a state: multiMap_class.f90:153-173 -- the
+ * product of the 1D maps' bins with strides multi[k] (first map fastest), 0 when any map misses.
+ * A map-less clerk has the single bin 1. */
+static int o_clerk_bin(const scn_problem* P, const scn_clerk* C, const double rg[3], int mat, int cell)
 {
-    return o_map1d_at(P, m, p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(P, &p->coords));
+    int bin = 1;
+    for (int k = 0; k < C->n_maps; ++k) {
+        int b = o_map1d_at(P, &C->maps[k], rg, mat, cell);
+        if (b == 0) return 0;
+        bin = bin + (b - 1) * C->multi[k];
+    }
+    return bin;
 }
 
-/* tallyAdmin%reportInColl (tallyAdmin_class.f90:466-486): attachment (keffImplicitClerk,
- * keffImplicitClerk_class.f90:180-223) first, then the collision clerks (collisionClerk_class.f90:192-242) */
-static void o_report_in_coll(o_hctx* X, const o_part* p, int isVirtual)
+/* tallyAdmin%reportInColl (tallyAdmin_class.f90:466-486) for a collision at global position rg in
+ * material mat (lowest-level cell `cell`) of a particle of group G and weight w whose tracking
+ * cross section was track_xs: the attachment (keffImplicitClerk, keffImplicitClerk_class.f90:
+ * 180-223) first, then every collision clerk (collisionClerk_class.f90:192-242): energyFilter
+ * (energyFilter_class.f90:109-129), map bin, flux = w / sigma (tracking xs when virtual collisions
+ * are handled, the material total otherwise), address mem + width*(bin-1) + i. */
+static void o_score_coll(const scn_problem* P, double* keff, double* bins, const double rg[3], int mat, int cell,
+                         int G, double w, double track_xs, int isVirtual)
 {
-    const scn_problem* P = X->P;
-    int mat = p->coords.matIdx;
     if (mat != SCN_VOID_MAT) {
-        double flux = p->w / p->track_xs;
-        const double* x = o_xs(P, mat, p->G);
+        double flux = w / track_xs;
+        const double* x = o_xs(P, mat, G);
         double nuFiss = x[XS_NUFISSION];
         double absXS = x[XS_CAPTURE] + x[XS_FISSION];
-        X->keff[K_IMP_PROD] = X->keff[K_IMP_PROD] + nuFiss * flux;
-        X->keff[K_IMP_ABS] = X->keff[K_IMP_ABS] + absXS * flux;
+        keff[K_IMP_PROD] = keff[K_IMP_PROD] + nuFiss * flux;
+        keff[K_IMP_ABS] = keff[K_IMP_ABS] + absXS * flux;
     }
-    if (!X->bins) return;
+    if (!bins) return;
     for (int ci = 0; ci < P->n_clerks; ++ci) {
         const scn_clerk* C = &P->clerks[ci];
         if (C->kind != SCN_CLERK_COLLISION) continue;
         if (!C->handle_virtual && isVirtual) continue;
-        if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
-        int bin = 1;
-        if (C->n_maps > 0) {
-            bin = 1;
-            for (int k = 0; k < C->n_maps; ++k) {
-                int b = o_map1d(P, &C->maps[k], p);
-                if (b == 0) { bin = 0; break; }
-                bin = bin + (b - 1) * C->multi[k];
-            }
-        }
+        if (C->gtop > 0 && !(C->gtop <= G && G <= C->glow)) continue;
+        int bin = o_clerk_bin(P, C, rg, mat, cell);
         if (bin == 0) continue;
-        double flux = C->handle_virtual ? p->w / p->track_xs : p->w / o_track_mat_xs(P, mat, p->G);
+        double flux = C->handle_virtual ? w / track_xs : w / o_track_mat_xs(P, mat, G);
         int64_t addr = C->mem_addr + (int64_t)C->width * (bin - 1);
         for (int i = 0; i < C->width; ++i) {
             double resp;
             if (C->resp[i] == SCN_MT_FLUX) resp = 1.0;
-            else resp = (mat == SCN_VOID_MAT) ? 0.0 : o_macro_get(o_xs(P, mat, p->G), C->resp[i]);
-            X->bins[addr + i] = X->bins[addr + i] + resp * flux;
+            else resp = (mat == SCN_VOID_MAT) ? 0.0 : o_macro_get(o_xs(P, mat, G), C->resp[i]);
+            bins[addr + i] = bins[addr + i] + resp * flux;
         }
     }
+}
+
+static void o_report_in_coll(o_hctx* X, const o_part* p, int isVirtual)
+{
+    o_score_coll(X->P, X->keff, X->bins, p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(X->P, &p->coords), p->G,
+                 p->w, p->track_xs, isVirtual);
 }
 
 /* tallyAdmin%reportHist -> keffImplicitClerk%reportHist (keffImplicitClerk_class.f90:289-299) */
@@ -1375,12 +1384,7 @@ static void o_report_path(o_hctx* X, const o_part* p, double L, const double r0[
         const scn_clerk* C = &P->clerks[ci];
         if (C->kind != SCN_CLERK_TRACK) continue;
         if (C->gtop > 0 && !(C->gtop <= p->G && p->G <= C->glow)) continue;
-        int bin = 1;
-        for (int k = 0; k < C->n_maps; ++k) {
-            int b = o_map1d_at(P, &C->maps[k], r0, mat0, cell0);
-            if (b == 0) { bin = 0; break; }
-            bin = bin + (b - 1) * C->multi[k];
-        }
+        int bin = o_clerk_bin(P, C, r0, mat0, cell0);
         if (bin == 0) continue;
         int64_t addr = C->mem_addr + (int64_t)C->width * (bin - 1);
         for (int i = 0; i < C->width; ++i) {
@@ -1537,6 +1541,39 @@ static int o_legendre_p1(double c, uint64_t* rng, double* mu)
     return 0;
 }
 
+/* fissionMG%sampleOut group (fissionMG_class.f90:183-206): subtract chi(G_out) from the draw u
+ * until it falls below 0; nG + 1 when the scan falls through (unnormalised chi) */
+static int o_chi_gout(const double* chi, int nG, double u)
+{
+    double rem = u;
+    int G_out;
+    for (G_out = 1; G_out <= nG; ++G_out) {
+        rem = rem - chi[G_out - 1];
+        if (rem < 0.0) break;
+    }
+    return G_out;
+}
+
+/* multiScatterMG%sampleGout (multiScatterMG_class.f90:265-290) for incident group G_in with draw u:
+ * u * scatterXS(G_in) minus P0(G_out, G_in) over G_out until below 0; *w_mul = production(G_in,
+ * G_out) = scatteringMultiplicity(G_out, G_in) (multiScatterMG_class.f90:199-222, sampleOut's
+ * weight change).  nG + 1 when the scan falls through. */
+static int o_scatter_gout(const scn_problem* P, int mat, int G_in, double u, double* w_mul)
+{
+    const int nG = P->mg.n_groups;
+    const double* x = o_xs(P, mat, G_in);
+    const double* P0 = P->mg.P0 + ((size_t)(mat - 1) * nG + (G_in - 1)) * nG;
+    const double* prod = P->mg.prod + ((size_t)(mat - 1) * nG + (G_in - 1)) * nG;
+    double rem = u * x[XS_SCATTER];
+    int G_out;
+    for (G_out = 1; G_out <= nG; ++G_out) {
+        rem = rem - P0[G_out - 1];
+        if (rem < 0.0) break;
+    }
+    if (G_out <= nG) *w_mul = prod[G_out - 1];
+    return G_out;
+}
+
 /* collisionProcessor_inter.f90:114-194 collide with neutronMGstd_class.f90:85-282 */
 /* neutronMGimp_class.f90 cutoffs with weight windows (weightWindowsField_class.f90 atP): the window
  * of the particle's state bin; above the upper weight the particle is split (split), below the
@@ -1545,12 +1582,7 @@ static void o_weight_window(o_hctx* X, o_part* p, int mat)
 {
     const scn_problem* P = X->P;
     const scn_clerk* M = P->ww_map;
-    int bin = 1;
-    for (int k = 0; k < M->n_maps; ++k) {
-        int b = o_map1d_at(P, &M->maps[k], p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(P, &p->coords));
-        if (b == 0) { bin = 0; break; }
-        bin = bin + (b - 1) * M->multi[k];
-    }
+    int bin = o_clerk_bin(P, M, p->coords.lvl[0].r, p->coords.matIdx, o_cell_of(P, &p->coords));
     double minW = 0.0, maxW = 0.0, avW = 0.0;
     if (bin > 0) {
         minW = P->ww_lower[bin - 1];
@@ -1625,12 +1657,7 @@ static void o_collide(o_hctx* X, o_part* p)
             for (int i = 0; i < n; ++i) {
                 double mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
                 double phi = SCN_TWO_PI * scn_rng_get(&p->rng);
-                double rem = scn_rng_get(&p->rng);
-                int G_out;
-                for (G_out = 1; G_out <= nG; ++G_out) {
-                    rem = rem - chi[G_out - 1];
-                    if (rem < 0.0) break;
-                }
+                int G_out = o_chi_gout(chi, nG, scn_rng_get(&p->rng));
                 if (G_out > nG) { o_fail(SCN_ERR_SAMPLING, "fissionMG sampleOut: unnormalised chi"); p->isDead = 1; return; }
                 scn_particle_state s;
                 memset(&s, 0, sizeof s);
@@ -1661,14 +1688,8 @@ static void o_collide(o_hctx* X, o_part* p)
         }
     }
     if (C == 2) { /* inelastic: multiScatterMG sampleOut */
-        double rem = scn_rng_get(&p->rng) * x[XS_SCATTER];
-        const double* P0 = P->mg.P0 + ((size_t)(mat - 1) * nG + (p->G - 1)) * nG;
-        const double* prod = P->mg.prod + ((size_t)(mat - 1) * nG + (p->G - 1)) * nG;
-        int G_out;
-        for (G_out = 1; G_out <= nG; ++G_out) {
-            rem = rem - P0[G_out - 1];
-            if (rem < 0.0) break;
-        }
+        double w_mul = 0.0;
+        int G_out = o_scatter_gout(P, mat, p->G, scn_rng_get(&p->rng), &w_mul);
         if (G_out > nG) { o_fail(SCN_ERR_SAMPLING, "multiScatterMG sampleGout: sampling failed"); p->isDead = 1; return; }
         double mu;
         if (P->mg.P1) {
@@ -1679,7 +1700,6 @@ static void o_collide(o_hctx* X, o_part* p)
             mu = 2.0 * scn_rng_get(&p->rng) - 1.0;
         }
         double phi = SCN_TWO_PI * scn_rng_get(&p->rng);
-        double w_mul = prod[G_out - 1];
         p->G = G_out;
         p->w = p->w * w_mul;
         double nd[3];
@@ -2054,13 +2074,7 @@ int orc_fission_source(const scn_problem* P, int32_t n, int64_t rng_state, const
             for (int k = 0; k < 3; ++k) s->r[k] = r[k];
             double mu = 2.0 * scn_rng_get(&rng) - 1.0;
             double phi = SCN_TWO_PI * scn_rng_get(&rng);
-            double rem = scn_rng_get(&rng);
-            const double* chi = P->mg.chi + (size_t)(mat - 1) * nG;
-            int G;
-            for (G = 1; G <= nG; ++G) {
-                rem = rem - chi[G - 1];
-                if (rem < 0.0) break;
-            }
+            int G = o_chi_gout(P->mg.chi + (size_t)(mat - 1) * nG, nG, scn_rng_get(&rng));
             if (G > nG) { o_fail(SCN_ERR_SAMPLING, "fissionMG sampleOut: unnormalised chi"); break; }
             s->G = G;
             s->isMG = 1;
@@ -2329,10 +2343,8 @@ int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                 free(w);
             }
             double k_norm = dk[1 - cur];
-            if (phase == 0) { /* keffAnalogClerk_class.f90:162-181 */
-                double k_eff = endW / startW * k_norm;
-                kA[0] = kA[0] + k_eff;
-                kA[1] = kA[1] + k_eff * k_eff;
+            if (phase == 0) {
+                orc_keff_analog(startW, endW, k_norm, kA);
                 nA++;
             } else { /* keffImplicitClerk_class.f90:306-326 */
                 double k_est = ks[0] / (ks[2] + ks[3] - ks[1]);
@@ -2395,4 +2407,40 @@ int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
     free(csumA);
     free(csumI);
     return g_err;
+}
+
+/* ---- single-operation entry points (tests pin these against the reference's unit tests) ---- */
+
+int orc_clerk_bin(const scn_problem* P, int32_t clerk, const double r[3], int32_t mat, int32_t cell)
+{
+    if (clerk < 0 || clerk >= P->n_clerks) return -1;
+    return o_clerk_bin(P, &P->clerks[clerk], r, mat, cell);
+}
+
+void orc_score_collision(const scn_problem* P, const double r[3], int32_t mat, int32_t cell, int32_t G, double w,
+                         double track_xs, int32_t is_virtual, double* bins, double keff[4])
+{
+    o_score_coll(P, keff, bins, r, mat, cell, G, w, track_xs, is_virtual);
+}
+
+int orc_scatter_gout(const scn_problem* P, int32_t mat, int32_t G_in, double u, double* w_mul)
+{
+    *w_mul = 0.0;
+    return o_scatter_gout(P, mat, G_in, u, w_mul);
+}
+
+int orc_chi_gout(const scn_problem* P, int32_t mat, double u)
+{
+    const int nG = P->mg.n_groups;
+    return o_chi_gout(P->mg.chi + (size_t)(mat - 1) * nG, nG, u);
+}
+
+/* keffAnalogClerk%closeCycle (keffAnalogClerk_class.f90:162-181): the cycle's estimate
+ * k = endWgt / startWgt * k_eff (k_eff of the dungeon, the k the cycle ran with), accumulated into
+ * acc[0] (sum) and acc[1] (sum of squares) as scoreMemory%accumulate does */
+void orc_keff_analog(double start_w, double end_w, double k_norm, double acc[2])
+{
+    double k = end_w / start_w * k_norm;
+    acc[0] = acc[0] + k;
+    acc[1] = acc[1] + k * k;
 }

@@ -95,6 +95,13 @@ int orc_fixed_run(const scn_problem* P, const scn_source* src, int32_t pop, int3
 void orc_score_result(double csum, double csum2, int N, double out[2]);
 void orc_close_cycle(double* bins, double* csum, double* csum2, int64_t n, double norm);
 double orc_keff_implicit(const double s[4]);
+/* single operations of the loop, exposed for known-answer tests */
+int orc_clerk_bin(const scn_problem* P, int32_t clerk, const double r[3], int32_t mat, int32_t cell);
+void orc_score_collision(const scn_problem* P, const double r[3], int32_t mat, int32_t cell, int32_t G, double w,
+                         double track_xs, int32_t is_virtual, double* bins, double keff[4]);
+int orc_scatter_gout(const scn_problem* P, int32_t mat, int32_t G_in, double u, double* w_mul);
+int orc_chi_gout(const scn_problem* P, int32_t mat, double u);
+void orc_keff_analog(double start_w, double end_w, double k_norm, double acc[2]);
 int orc_legendre_p1(double c, uint64_t* rng, double* mu);
 int orc_eigen_run(const scn_problem* P, const scn_eigen_settings* S,
                   const scn_particle_state* initial_bank, scn_eigen_result* res, int32_t n_threads);
