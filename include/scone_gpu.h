@@ -442,6 +442,21 @@ int scn_batch_release(void* batch, int32_t slot);
 /* device time and counters of the batch engine */
 int scn_batch_stats(void* batch, scn_stats* out);
 
+/* Population control that allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (serving loops and harnesses that forbid hipMalloc, code that
+ * must not touch the allocator).  It is scn_norm_size_repr on DEVICE banks with no engine handle:
+ * bank_in[0..n) must already be in sortByBroodID order with broodIDs in [0, SCN_NOALLOC_MAX_SITES
+ * + 1] (checked on the device; SCN_ERR_ARG otherwise), bank_out receives the tot_pop sites
+ * (out_cap >= tot_pop), *n_out their number.  Its scratch (draws, flags, prefix sums, brood
+ * bounds, sort storage) is one static device array of fixed size on each GPU the library runs
+ * on, held until the process ends; n above SCN_NOALLOC_MAX_SITES is rejected with SCN_ERR_ARG.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU race and must be
+ * serialised by the caller.  stream: a hipStream_t, or NULL for the default stream; the call
+ * returns after the stream has finished the pass.  Errors: scn_last_error(NULL).              */
+#define SCN_NOALLOC_MAX_SITES (1 << 20)
+int scn_norm_size_repr_noalloc(const scn_particle_state* bank_in, int32_t n, scn_particle_state* bank_out,
+                               int32_t out_cap, int32_t tot_pop, int64_t rng_state, void* stream, int32_t* n_out);
+
 /* Engine counters and timings (no counterpart in the reference; SCONE only times transport). */
 int scn_get_stats(void* engine, scn_stats* out);
 int scn_reset_stats(void* engine);

@@ -22,6 +22,7 @@
 #include <vector>
 #include <algorithm>
 #include <functional>
+#include <mutex>
 
 #include "scone_gpu.h"
 #include "scn_fpmath.h"
@@ -33,6 +34,20 @@ using namespace scn;
 
 // pairwise levels a canonical sum may take above its chunk partials (2^40 partials)
 #define SCN_DSUM_MAX_LEVELS 40
+
+// ---- allocation-free population control ----------------------------------------------------
+// Scratch of scn_norm_size_repr_noalloc: per-site draws, their sorted bits, flags, prefix sums,
+// brood bounds and rocPRIM's temporary storage, carved from one static device array of fixed size
+// (each GPU holds its own copy for the life of the process).  Host code never passes the array's
+// name to a kernel: it takes the array's device address with hipGetSymbolAddress once per device.
+constexpr int NOALLOC_SORT_TMP = 24 << 20;
+constexpr int NOALLOC_SCAN_TMP = 1 << 20;
+constexpr size_t noalloc_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NOALLOC_BYTES =
+    3 * noalloc_round(sizeof(double) * SCN_NOALLOC_MAX_SITES) + 2 * noalloc_round(sizeof(int) * SCN_NOALLOC_MAX_SITES) +
+    2 * noalloc_round(sizeof(int) * (SCN_NOALLOC_MAX_SITES + 2)) + noalloc_round(sizeof(int) * 64) + NOALLOC_SORT_TMP +
+    NOALLOC_SCAN_TMP;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 namespace {
 
@@ -159,11 +174,27 @@ __global__ void k_norm_flags(const double* rn, int n, const uint64_t* thr_bits, 
     flag[j] = keep_mode ? (rn[j] > thr) : (rn[j] <= thr);
 }
 
-__global__ void k_norm_keep(const scn_particle_state* in, const int* flag, const int* pos, int n, scn_particle_state* out)
+__global__ void k_norm_keep(const scn_particle_state* in, const int* flag, const int* pos, int n, scn_particle_state* out,
+                            int out_cap)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    if (flag[j]) out[pos[j]] = in[j];
+    if (flag[j] && pos[j] < out_cap) out[pos[j]] = in[j];
+}
+
+__global__ void k_fill_int(int* p, int n, int v)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = v;
+}
+
+// broodIDs in [0, max_brood] and non-decreasing (sortByBroodID order); bad[0] = 1 otherwise
+__global__ void k_check_broods(const scn_particle_state* in, int n, int max_brood, int* bad)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int b = in[j].broodID;
+    if (b < 0 || b > max_brood || (j > 0 && in[j - 1].broodID > b)) bad[0] = 1;
 }
 
 __global__ void k_brood_bounds(const scn_particle_state* in, int n, int max_brood, int* start_of, int* end_of)
@@ -177,17 +208,24 @@ __global__ void k_brood_bounds(const scn_particle_state* in, int n, int max_broo
 }
 
 __global__ void k_norm_dup(const scn_particle_state* in, const int* dflag, const int* D, const int* start_of,
-                           const int* end_of, int n, int n_copies, scn_particle_state* out)
+                           const int* end_of, int n, int n_copies, int max_brood, scn_particle_state* out, int out_cap)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int b = in[j].broodID;
+    if (b < 0 || b > max_brood) return;   // excluded by k_check_broods / the host check before launch
     int s = start_of[b];
     int kb = end_of[b] - s;
     int64_t base = (int64_t)(n_copies + 1) * s + D[s];
     int rho = j - s;
-    for (int c = 0; c <= n_copies; ++c) out[base + (int64_t)c * kb + rho] = in[j];
-    if (dflag[j]) out[base + (int64_t)(n_copies + 1) * kb + (D[j] - D[s])] = in[j];
+    for (int c = 0; c <= n_copies; ++c) {
+        const int64_t o = base + (int64_t)c * kb + rho;
+        if (o < out_cap) out[o] = in[j];
+    }
+    if (dflag[j]) {
+        const int64_t o = base + (int64_t)(n_copies + 1) * kb + (D[j] - D[s]);
+        if (o < out_cap) out[o] = in[j];
+    }
 }
 
 // ---- initial fission source (fissionSource_class.f90:149-271) ---------------------------------
@@ -632,80 +670,133 @@ int close_bins(Engine* E, int which)
     return 0;
 }
 
-// particleDungeon%normSize_Repr on E->bankB[0..n) (sorted by brood); result in E->bankA
-int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
+// device buffers of one normSize_Repr pass over at most `n_max` sites
+struct NormBufs {
+    double* rn;
+    uint64_t* rn_bits;
+    uint64_t* rn_sorted;
+    int* flag;
+    int* pos;
+    int* start_of;       // [max_brood + 1]
+    int* end_of;
+    int max_brood;
+    void* sort_tmp;      // rocPRIM temporary storage
+    size_t sort_tmp_bytes;
+    void* scan_tmp;
+    size_t scan_tmp_bytes;
+};
+
+// temporary storage rocPRIM needs for n sites (a size query: nothing is allocated or launched)
+hipError_t norm_tmp_needs(const NormBufs& B, int n, hipStream_t stream, size_t* sort_need, size_t* scan_need)
+{
+    hipError_t e = rocprim::radix_sort_keys(nullptr, *sort_need, B.rn_bits, B.rn_sorted, (size_t)n, 0, 64, stream);
+    if (e != hipSuccess) return e;
+    return rocprim::exclusive_scan(nullptr, *scan_need, B.flag, B.pos, 0, (size_t)n, rocprim::plus<int>(), stream);
+}
+
+#define NCHK(expr)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+            return SCN_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+// particleDungeon%normSize_Repr (particleDungeon_class.f90:431-602) of in[0..n), sorted by
+// broodID with every broodID in [0, B.max_brood], into out (capacity out_cap).  The temporary
+// storage in B must hold what norm_tmp_needs reports for n.  *n_new = population after.
+int norm_size_run(const NormBufs& B, hipStream_t stream, const scn_particle_state* in, int n, int tot_pop,
+                  uint64_t master, scn_particle_state* out, int out_cap, int* n_new, std::string& err)
 {
     if (n == 0) {
-        E->err = "Normalisation failed: no fission sites";
+        err = "Normalisation failed: no fission sites";
         return SCN_ERR_NORM;
+    }
+    if (tot_pop > out_cap) {
+        err = "normSize_Repr: bank capacity too small";
+        return SCN_ERR_OVERFLOW;
     }
     int excess = n - tot_pop;
     int heap = excess < 0 ? ((-excess) % n) : excess;
-    if ((int64_t)tot_pop > E->bank_cap) {
-        E->err = "normSize_Repr: bank capacity too small";
-        return SCN_ERR_OVERFLOW;
-    }
     if (excess == 0) {
-        HIPCHK(hipMemcpyAsync(E->bankA, E->bankB, sizeof(scn_particle_state) * n, hipMemcpyDeviceToDevice, E->stream));
+        NCHK(hipMemcpyAsync(out, in, sizeof(scn_particle_state) * n, hipMemcpyDeviceToDevice, stream));
         *n_new = n;
         return 0;
     }
     const unsigned g = (unsigned)((n + 255) / 256);
     uint64_t* thr = nullptr;
+    size_t sort_need = B.sort_tmp_bytes, scan_need = B.scan_tmp_bytes;
     if (heap != 0) {
-        hipLaunchKernelGGL(k_site_rn, dim3(g), dim3(256), 0, E->stream, master, n, E->rn, E->rn_bits);
-        size_t need = 0;
-        HIPCHK(rocprim::radix_sort_keys(nullptr, need, E->rn_bits, E->rn_sorted, (size_t)n, 0, 64, E->stream));
-        if (need > E->rsort_tmp_bytes) {
-            int rc = dalloc(E, (char**)&E->rsort_tmp, need);
-            if (rc) return rc;
-            E->rsort_tmp_bytes = need;
-        }
-        HIPCHK(rocprim::radix_sort_keys(E->rsort_tmp, need, E->rn_bits, E->rn_sorted, (size_t)n, 0, 64, E->stream));
-        thr = E->rn_sorted + (heap - 1);
+        hipLaunchKernelGGL(k_site_rn, dim3(g), dim3(256), 0, stream, master, n, B.rn, B.rn_bits);
+        NCHK(rocprim::radix_sort_keys(B.sort_tmp, sort_need, B.rn_bits, B.rn_sorted, (size_t)n, 0, 64, stream));
+        thr = B.rn_sorted + (heap - 1);
     }
-    size_t need = 0;
-    HIPCHK(rocprim::exclusive_scan(nullptr, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
-    if (need > E->scan_tmp_bytes) {
-        int rc = dalloc(E, (char**)&E->scan_tmp, need);
-        if (rc) return rc;
-        E->scan_tmp_bytes = need;
-    }
+    int last[2];
     if (excess > 0) {
-        hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, E->stream, E->rn, n, thr, 1, E->flag);
-        HIPCHK(rocprim::exclusive_scan(E->scan_tmp, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
-        hipLaunchKernelGGL(k_norm_keep, dim3(g), dim3(256), 0, E->stream, E->bankB, E->flag, E->pos, n, E->bankA);
-        int last[2];
-        HIPCHK(hipMemcpyAsync(&last[0], E->flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
-        HIPCHK(hipMemcpyAsync(&last[1], E->pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
-        HIPCHK(hipStreamSynchronize(E->stream));
+        hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, stream, B.rn, n, thr, 1, B.flag);
+        NCHK(rocprim::exclusive_scan(B.scan_tmp, scan_need, B.flag, B.pos, 0, (size_t)n, rocprim::plus<int>(), stream));
+        hipLaunchKernelGGL(k_norm_keep, dim3(g), dim3(256), 0, stream, in, B.flag, B.pos, n, out, out_cap);
+        NCHK(hipMemcpyAsync(&last[0], B.flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+        NCHK(hipMemcpyAsync(&last[1], B.pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+        NCHK(hipStreamSynchronize(stream));
         *n_new = last[0] + last[1];
     } else {
         int n_copies = (-excess) / n;
         int n_dup = (-excess) % n;
         if (n_dup != 0) {
-            hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, E->stream, E->rn, n, thr, 0, E->flag);
+            hipLaunchKernelGGL(k_norm_flags, dim3(g), dim3(256), 0, stream, B.rn, n, thr, 0, B.flag);
         } else {
-            HIPCHK(hipMemsetAsync(E->flag, 0, sizeof(int) * n, E->stream));
+            NCHK(hipMemsetAsync(B.flag, 0, sizeof(int) * n, stream));
         }
-        HIPCHK(rocprim::exclusive_scan(E->scan_tmp, need, E->flag, E->pos, 0, (size_t)n, rocprim::plus<int>(), E->stream));
-        hipLaunchKernelGGL(k_brood_bounds, dim3(g), dim3(256), 0, E->stream, E->bankB, n, E->cap + 1, E->start_of,
-                           E->end_of);
-        hipLaunchKernelGGL(k_norm_dup, dim3(g), dim3(256), 0, E->stream, E->bankB, E->flag, E->pos, E->start_of,
-                           E->end_of, n, n_copies, E->bankA);
-        int last[2];
-        HIPCHK(hipMemcpyAsync(&last[0], E->flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
-        HIPCHK(hipMemcpyAsync(&last[1], E->pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, E->stream));
-        HIPCHK(hipStreamSynchronize(E->stream));
+        NCHK(rocprim::exclusive_scan(B.scan_tmp, scan_need, B.flag, B.pos, 0, (size_t)n, rocprim::plus<int>(), stream));
+        hipLaunchKernelGGL(k_brood_bounds, dim3(g), dim3(256), 0, stream, in, n, B.max_brood, B.start_of, B.end_of);
+        hipLaunchKernelGGL(k_norm_dup, dim3(g), dim3(256), 0, stream, in, B.flag, B.pos, B.start_of, B.end_of, n,
+                           n_copies, B.max_brood, out, out_cap);
+        NCHK(hipMemcpyAsync(&last[0], B.flag + (n - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+        NCHK(hipMemcpyAsync(&last[1], B.pos + (n - 1), sizeof(int), hipMemcpyDeviceToHost, stream));
+        NCHK(hipStreamSynchronize(stream));
         *n_new = n * (n_copies + 1) + last[0] + last[1];
     }
-    HIPCHK(hipGetLastError());
+    NCHK(hipGetLastError());
     if (*n_new != tot_pop) {
-        E->err = "Normalisation failed!";
+        err = "Normalisation failed!";
         return SCN_ERR_NORM;
     }
     return 0;
 }
+#undef NCHK
+
+// particleDungeon%normSize_Repr on E->bankB[0..n) (sorted by brood); result in E->bankA
+int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
+{
+    if ((int64_t)tot_pop > E->bank_cap) {
+        E->err = "normSize_Repr: bank capacity too small";
+        return SCN_ERR_OVERFLOW;
+    }
+    NormBufs B{E->rn, E->rn_bits, E->rn_sorted, E->flag, E->pos, E->start_of, E->end_of, E->cap + 1,
+               E->rsort_tmp, E->rsort_tmp_bytes, E->scan_tmp, E->scan_tmp_bytes};
+    if (n > 0) {
+        size_t sort_need = 0, scan_need = 0;
+        HIPCHK(norm_tmp_needs(B, n, E->stream, &sort_need, &scan_need));
+        if (sort_need > E->rsort_tmp_bytes) {
+            int rc = dalloc(E, (char**)&E->rsort_tmp, sort_need);
+            if (rc) return rc;
+            E->rsort_tmp_bytes = sort_need;
+        }
+        if (scan_need > E->scan_tmp_bytes) {
+            int rc = dalloc(E, (char**)&E->scan_tmp, scan_need);
+            if (rc) return rc;
+            E->scan_tmp_bytes = scan_need;
+        }
+        B.sort_tmp = E->rsort_tmp;
+        B.sort_tmp_bytes = E->rsort_tmp_bytes;
+        B.scan_tmp = E->scan_tmp;
+        B.scan_tmp_bytes = E->scan_tmp_bytes;
+    }
+    return norm_size_run(B, E->stream, E->bankB, n, tot_pop, master, E->bankA, E->bank_cap, n_new, E->err);
+}
+
 
 }  // namespace
 
@@ -1190,6 +1281,107 @@ int scn_eigen_run(void* engine, const scn_eigen_settings* S, const scn_particle_
     rc = scn_eigen_step(engine, S->n_inactive + S->n_active, &done);
     if (rc) return rc;
     return scn_eigen_finish(engine, res);
+}
+
+// Allocation-free population control (scratch in the static `scratch` array, see above).
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[64];
+
+static int noalloc_scratch(char** base)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_create_err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return SCN_ERR_NO_DEVICE;
+    }
+    if (dev < 0 || dev >= 64) {
+        g_create_err = "device index above 63";
+        return SCN_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return SCN_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    *base = (char*)g_noalloc_ptr[dev];
+    return 0;
+}
+
+int scn_norm_size_repr_noalloc(const scn_particle_state* bank_in, int32_t n, scn_particle_state* bank_out,
+                               int32_t out_cap, int32_t tot_pop, int64_t rng_state, void* stream, int32_t* n_out)
+{
+    if (!bank_in || !bank_out || !n_out || n < 0 || tot_pop < 0 || out_cap < 0) {
+        g_create_err = "scn_norm_size_repr_noalloc: invalid argument";
+        return SCN_ERR_ARG;
+    }
+    if (n > SCN_NOALLOC_MAX_SITES) {
+        g_create_err = "scn_norm_size_repr_noalloc: " + std::to_string(n) + " sites exceed the scratch maximum of " +
+                       std::to_string(SCN_NOALLOC_MAX_SITES) + " (SCN_NOALLOC_MAX_SITES)";
+        return SCN_ERR_ARG;
+    }
+    char* q = nullptr;
+    int rc = noalloc_scratch(&q);
+    if (rc) return rc;
+    auto take = [&q](size_t bytes) {
+        char* r = q;
+        q += noalloc_round(bytes);
+        return r;
+    };
+    const size_t M = SCN_NOALLOC_MAX_SITES;
+    NormBufs B{};
+    B.rn = (double*)take(sizeof(double) * M);
+    B.rn_bits = (uint64_t*)take(sizeof(uint64_t) * M);
+    B.rn_sorted = (uint64_t*)take(sizeof(uint64_t) * M);
+    B.flag = (int*)take(sizeof(int) * M);
+    B.pos = (int*)take(sizeof(int) * M);
+    B.start_of = (int*)take(sizeof(int) * (M + 2));
+    B.end_of = (int*)take(sizeof(int) * (M + 2));
+    B.max_brood = (int)M + 1;
+    int* bad = (int*)take(sizeof(int) * 64);
+    B.sort_tmp = take(NOALLOC_SORT_TMP);
+    B.sort_tmp_bytes = NOALLOC_SORT_TMP;
+    B.scan_tmp = take(NOALLOC_SCAN_TMP);
+    B.scan_tmp_bytes = NOALLOC_SCAN_TMP;
+    hipStream_t st = (hipStream_t)stream;
+    std::string& err = g_create_err;
+    if (n > 0) {
+        // the brood bounds index start_of/end_of by broodID: check range and order first
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, bad, 1, 0);
+        hipLaunchKernelGGL(k_check_broods, dim3((n + 255) / 256), dim3(256), 0, st, bank_in, n, B.max_brood, bad);
+        int hbad = 0;
+        hipError_t e = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            err = std::string("scn_norm_size_repr_noalloc: ") + hipGetErrorString(e);
+            return SCN_ERR_HIP;
+        }
+        if (hbad) {
+            err = "scn_norm_size_repr_noalloc: broodIDs must be sorted (sortByBroodID) and within [0, " +
+                  std::to_string(B.max_brood) + "]";
+            return SCN_ERR_ARG;
+        }
+        size_t sort_need = 0, scan_need = 0;
+        e = norm_tmp_needs(B, n, st, &sort_need, &scan_need);
+        if (e != hipSuccess) {
+            err = std::string("rocprim size query: ") + hipGetErrorString(e);
+            return SCN_ERR_HIP;
+        }
+        if (sort_need > B.sort_tmp_bytes || scan_need > B.scan_tmp_bytes) {
+            err = "scn_norm_size_repr_noalloc: rocPRIM temporary storage exceeds the scratch reserve";
+            return SCN_ERR_ARG;
+        }
+    }
+    int nn = 0;
+    rc = norm_size_run(B, st, bank_in, n, tot_pop, (uint64_t)rng_state, bank_out, out_cap, &nn, err);
+    if (rc) return rc;
+    *n_out = nn;
+    return 0;
 }
 
 }  // extern "C"

@@ -19,7 +19,8 @@ EXPORTS = ["scn_create", "scn_destroy", "scn_last_error", "scn_device_info", "sc
            "scn_norm_size_repr", "scn_eigen_run", "scn_eigen_start", "scn_eigen_step", "scn_eigen_finish",
            "scn_fission_source", "scn_get_stats", "scn_reset_stats",
            "scn_batch_create", "scn_batch_destroy", "scn_batch_last_error", "scn_batch_add", "scn_batch_cycle",
-           "scn_batch_query", "scn_batch_states", "scn_batch_release", "scn_batch_stats"]
+           "scn_batch_query", "scn_batch_states", "scn_batch_release", "scn_batch_stats",
+           "scn_norm_size_repr_noalloc"]
 
 SCN_ERR_BUSY = 11
 
@@ -77,6 +78,8 @@ def load():
     L.scn_batch_states.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
     L.scn_batch_release.argtypes = [C.c_void_p, C.c_int32]
     L.scn_batch_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.scn_norm_size_repr_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int64,
+                                             C.c_void_p, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
