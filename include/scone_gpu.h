@@ -273,6 +273,10 @@ typedef struct scn_eigen_settings {
     double keff0;            /* keff_0, default 1.0 */
     int32_t events_per_launch; /* 0: autotuned schedule */
     int32_t block_size;        /* 0: autotuned */
+    int32_t cycle_graph;       /* 1: every size and scalar of a cycle stays on the device and the cycles
+                                  are replayed from a captured hipGraph (one graph per phase); the
+                                  same histories, banks and k history as 0.  Needs events_per_launch 0. */
+    int32_t pad0;
 } scn_eigen_settings;
 
 typedef struct scn_eigen_result {

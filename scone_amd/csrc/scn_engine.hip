@@ -86,23 +86,27 @@ __global__ void k_gather_sites(Sites S, const uint32_t* __restrict__ order, int 
 // ---- canonical sums ------------------------------------------------------------------------
 struct SumJob {
     const double* x;
-    int64_t n;
+    int64_t n;        // values (with n_dev: the capacity, n_dev's value is clamped to it)
     int64_t stride;   // in doubles
+    const int* n_dev; // null, or the number of values in device memory
 };
 struct SumJobs {
     SumJob j[8];
     int njobs;
 };
 
-__global__ void k_dsum_chunks(SumJobs J, double* y, int64_t ystride)
+__global__ void k_dsum_chunks(SumJobs J, double* y, int64_t ystride, int64_t* m_tab)
 {
     int job = blockIdx.y;
     const SumJob& q = J.j[job];
-    int64_t m = (q.n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    int64_t n = q.n;
+    if (q.n_dev) n = min((int64_t)max(*q.n_dev, 0), q.n);
+    int64_t m = (n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m_tab && c == 0) m_tab[job] = m;   // device plan: level 0 sizes
     if (c >= m) return;
     double s = 0.0;
-    int64_t e = (c + 1) * SCN_DSUM_CHUNK < q.n ? (c + 1) * SCN_DSUM_CHUNK : q.n;
+    int64_t e = (c + 1) * SCN_DSUM_CHUNK < n ? (c + 1) * SCN_DSUM_CHUNK : n;
     for (int64_t i = c * SCN_DSUM_CHUNK; i < e; ++i) s = s + q.x[i * q.stride];
     y[job * ystride + c] = s;
 }
@@ -114,6 +118,20 @@ __global__ void k_dsum_level(const double* y, double* y2, const int64_t* m_in, i
     int64_t h = (m + 1) / 2;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m <= 1 || i >= h) return;
+    const double* a = y + job * ystride;
+    y2[job * ystride + i] = (2 * i + 1 < m) ? a[2 * i] + a[2 * i + 1] : a[2 * i];
+}
+
+// one pairwise level with sizes in device memory: row `level` of m_tab in, row level + 1 out; a
+// value standing alone (m == 1) is carried forward, so any number of levels gives the same tree
+__global__ void k_dsum_level_dev(const double* y, double* y2, int64_t* m_tab, int level, int64_t ystride)
+{
+    int job = blockIdx.y;
+    int64_t m = m_tab[level * 8 + job];
+    int64_t h = (m + 1) / 2;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) m_tab[(level + 1) * 8 + job] = h;
+    if (i >= h) return;
     const double* a = y + job * ystride;
     y2[job * ystride + i] = (2 * i + 1 < m) ? a[2 * i] + a[2 * i + 1] : a[2 * i];
 }
@@ -228,6 +246,308 @@ __global__ void k_norm_dup(const scn_particle_state* in, const int* dflag, const
     }
 }
 
+// ---- device-resident power iteration ----------------------------------------------------------
+// Every scalar a cycle needs (master RNG state, k, the analog/implicit accumulators, the site
+// count, the normalisation plan) lives in one device record, so a cycle is a fixed sequence of
+// launches with no host round trip and can be replayed from a captured hipGraph.  The arithmetic
+// is the host loop's, operation for operation (scn_eigen_step), so the k history is the same bits.
+struct EigCtl {
+    uint64_t master;
+    double k_new;          // k loaded into the histories of the next cycle
+    double keff_0;
+    double dk[2];
+    double kA[2], kI[2], imp[4][2];
+    int nA, nI, cur, cycle;
+    int pop;
+    int n_sites;           // sites banked by the last transport (clamped to the capacity)
+    int excess, heap, n_copies, n_dup;
+    int rs_k;              // radix select: rank still sought among the draws matching the prefix
+    int pad;
+    uint64_t rs_prefix, rs_mask;
+    unsigned long long sites_total;
+};
+
+__global__ void k_dcycle_begin(int* counters_i, int* site_count)
+{
+    if (threadIdx.x < 4) counters_i[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *site_count = 0;
+}
+
+// sites in (broodID, emission index) order without a sort: history b's sites start at the exclusive
+// prefix sum of the site counts of histories 1..b-1 (sortByBroodID of the detain order)
+__global__ void k_place_sites(Sites S, const int* __restrict__ site_off, int pop, int out_cap,
+                              scn_particle_state* __restrict__ out, EigCtl* ctl, DevStatus* st)
+{
+    const int count = *S.count;
+    const int n = min(count, min(S.cap, out_cap));
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) {
+        ctl->n_sites = n;
+        ctl->sites_total += (unsigned long long)n;
+        if (count > n) raise_error(st, SCN_ERR_OVERFLOW, count);
+    }
+    if (j >= n) return;
+    const uint64_t key = S.key[j];
+    const int b = (int)(key >> 24);
+    if (b < 1 || b > pop) return;
+    const int64_t dst = (int64_t)site_off[b - 1] + (int64_t)(key & 0xFFFFFF);
+    if (dst >= n) return;
+    scn_particle_state q;
+    q.wgt = S.w[j];
+    for (int k = 0; k < 3; ++k) {
+        q.r[k] = S.r[(size_t)k * S.cap + j];
+        q.dir[k] = S.u[(size_t)k * S.cap + j];
+    }
+    q.E = 0.0;
+    q.G = S.G[j];
+    q.isMG = 1;
+    q.type = 1;
+    q.pad0 = 0;
+    q.time = S.t[j];
+    q.lambda = SCN_INF;
+    q.matIdx = S.mat[j];
+    q.cellIdx = 0;
+    q.uniqueID = S.uid[j];
+    q.collisionN = S.coll[j];
+    q.broodID = b;
+    q.pad1 = 0;
+    out[dst] = q;
+}
+
+// keffAnalogClerk (keffAnalogClerk_class.f90:162-181) or keffImplicitClerk (:306-326) close of the
+// cycle, after the package's stride of the master state by pop + 1
+__global__ void k_dcycle_keff(EigCtl* ctl, const double* __restrict__ sums, int phase)
+{
+    if (threadIdx.x != 0) return;
+    ctl->master = rng_stride(ctl->master, (int64_t)ctl->pop + 1);
+    const double startW = sums[0], endW = sums[1];
+    const double k_norm = ctl->dk[1 - ctl->cur];
+    if (phase == 0) {
+        const double k_eff = endW / startW * k_norm;
+        ctl->kA[0] = ctl->kA[0] + k_eff;
+        ctl->kA[1] = ctl->kA[1] + k_eff * k_eff;
+        ctl->nA++;
+    } else {
+        const double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
+        ctl->kI[0] = ctl->kI[0] + k_est;
+        ctl->kI[1] = ctl->kI[1] + k_est * k_est;
+        ctl->nI++;
+        for (int k = 0; k < 4; ++k) {
+            ctl->imp[k][0] = ctl->imp[k][0] + sums[2 + k];
+            ctl->imp[k][1] = ctl->imp[k][1] + sums[2 + k] * sums[2 + k];
+        }
+    }
+}
+
+// normSize_Repr plan from the device site count (particleDungeon_class.f90:431-470)
+__global__ void k_norm_plan(EigCtl* ctl, int* hist, DevStatus* st)
+{
+    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x != 0) return;
+    const int n = ctl->n_sites;
+    if (n == 0) raise_error(st, SCN_ERR_NORM, 0);
+    const int excess = n - ctl->pop;
+    ctl->excess = excess;
+    ctl->heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
+    ctl->n_copies = (n == 0 || excess >= 0) ? 0 : (-excess) / n;
+    ctl->n_dup = (n == 0 || excess >= 0) ? 0 : (-excess) % n;
+    ctl->rs_k = ctl->heap;
+    ctl->rs_prefix = 0;
+    ctl->rs_mask = 0;
+}
+
+__global__ void k_site_rn_dev(const EigCtl* ctl, double* rn, uint64_t* rn_bits)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ctl->n_sites || ctl->heap == 0) return;
+    uint64_t s = rng_skip(ctl->master, (int64_t)j);
+    double u = scn_rng_get(&s);
+    rn[j] = u;
+    rn_bits[j] = scn_dbits(u);
+}
+
+// radix select of the heap-th smallest draw, 8 bits a pass from the top: histogram of the digit
+// among the draws that match the digits chosen so far ...
+__global__ void k_rs_hist(const uint64_t* __restrict__ keys, const EigCtl* ctl, int* hist, int pass)
+{
+    __shared__ int h[256];
+    if (ctl->heap == 0) return;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int n = ctl->n_sites;
+    const uint64_t mask = ctl->rs_mask, prefix = ctl->rs_prefix;
+    const int shift = 56 - 8 * pass;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint64_t k = keys[j];
+        if ((k & mask) == prefix) atomicAdd(&h[(int)((k >> shift) & 255)], 1);
+    }
+    __syncthreads();
+    const int c = h[threadIdx.x];
+    if (c) atomicAdd(&hist[pass * 256 + threadIdx.x], c);
+}
+
+// ... then the digit whose cumulative count reaches the rank (one block of 256 threads)
+__global__ void k_rs_pick(EigCtl* ctl, const int* hist, int pass)
+{
+    __shared__ int h[256];
+    if (ctl->heap == 0) return;
+    h[threadIdx.x] = hist[pass * 256 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int shift = 56 - 8 * pass;
+    int k = ctl->rs_k, cum = 0;
+    for (int d = 0; d < 256; ++d) {
+        if (cum + h[d] >= k) {
+            ctl->rs_k = k - cum;
+            ctl->rs_prefix |= (uint64_t)d << shift;
+            ctl->rs_mask |= (uint64_t)255 << shift;
+            return;
+        }
+        cum += h[d];
+    }
+}
+
+__global__ void k_norm_flags_dev(const double* __restrict__ rn, const EigCtl* ctl, int* flag)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ctl->n_sites) return;
+    const double thr = scn_bitsd(ctl->rs_prefix);
+    const int ex = ctl->excess;
+    flag[j] = ex > 0 ? (rn[j] > thr) : (ex < 0 && ctl->n_dup != 0) ? (rn[j] <= thr) : 0;
+}
+
+// exclusive prefix sum of n ints (n in device memory), 1024 values per block: block scans, a scan
+// of the block totals in one block, then the block offsets added
+constexpr int SCAN_PER_BLOCK = 1024;
+__global__ void __launch_bounds__(256) k_scan_blocks(const int* __restrict__ in, int* __restrict__ out, int* bsum,
+                                                     const int* n_ptr)
+{
+    __shared__ int w[256];
+    const int n = *n_ptr;
+    const int base = blockIdx.x * SCAN_PER_BLOCK;
+    if (base >= n) return;
+    int v[4], t = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int j = base + threadIdx.x * 4 + k;
+        v[k] = j < n ? in[j] : 0;
+        t += v[k];
+    }
+    w[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int x = threadIdx.x >= off ? w[threadIdx.x - off] : 0;
+        __syncthreads();
+        w[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = w[threadIdx.x] - t;
+    for (int k = 0; k < 4; ++k) {
+        const int j = base + threadIdx.x * 4 + k;
+        if (j < n) out[j] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = w[255];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(int* bsum, const int* n_ptr)
+{
+    __shared__ int w[1024];
+    const int nb = (*n_ptr + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int t = 0;
+    for (int b = b0; b < min(b0 + per, nb); ++b) t += bsum[b];
+    w[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int x = threadIdx.x >= off ? w[threadIdx.x - off] : 0;
+        __syncthreads();
+        w[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = w[threadIdx.x] - t;
+    for (int b = b0; b < min(b0 + per, nb); ++b) {
+        const int x = bsum[b];
+        bsum[b] = run;
+        run += x;
+    }
+}
+
+__global__ void k_scan_add(int* out, const int* bsum, const int* n_ptr)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *n_ptr) return;
+    out[j] += bsum[j / SCAN_PER_BLOCK];
+}
+
+__global__ void k_brood_bounds_dev(const scn_particle_state* in, const EigCtl* ctl, int max_brood, int* start_of,
+                                   int* end_of)
+{
+    const int n = ctl->n_sites;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || ctl->excess >= 0) return;
+    const int b = in[j].broodID;
+    if (b < 0 || b > max_brood) return;
+    if (j == 0 || in[j - 1].broodID != b) start_of[b] = j;
+    if (j == n - 1 || in[j + 1].broodID != b) end_of[b] = j + 1;
+}
+
+// keep (excess > 0), copy and duplicate (excess < 0) or pass through (excess = 0)
+__global__ void k_norm_apply(const scn_particle_state* __restrict__ in, const int* flag, const int* pos,
+                             const int* start_of, const int* end_of, const EigCtl* ctl, int max_brood,
+                             scn_particle_state* __restrict__ out, int out_cap)
+{
+    const int n = ctl->n_sites;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int ex = ctl->excess;
+    if (ex == 0) {
+        if (j < out_cap) out[j] = in[j];
+    } else if (ex > 0) {
+        if (flag[j] && pos[j] < out_cap) out[pos[j]] = in[j];
+    } else {
+        const int b = in[j].broodID;
+        if (b < 0 || b > max_brood) return;
+        const int s = start_of[b];
+        const int kb = end_of[b] - s;
+        if (s < 0 || s > j || kb <= 0 || s + kb > n) return;   // only after an error left the bank incomplete
+        const int nc = ctl->n_copies;
+        const int64_t base = (int64_t)(nc + 1) * s + pos[s];
+        const int rho = j - s;
+        for (int c = 0; c <= nc; ++c) {
+            const int64_t o = base + (int64_t)c * kb + rho;
+            if (o < out_cap) out[o] = in[j];
+        }
+        if (flag[j]) {
+            const int64_t o = base + (int64_t)(nc + 1) * kb + (pos[j] - pos[s]);
+            if (o < out_cap) out[o] = in[j];
+        }
+    }
+}
+
+// end of the cycle: population check, the package's stride by 1, k for the next cycle
+// (eigenPhysicsPackage_class.f90:262-310)
+__global__ void k_dcycle_end(EigCtl* ctl, const int* flag, const int* pos, double* k_hist, int k_hist_cap, int phase,
+                             DevStatus* st)
+{
+    if (threadIdx.x != 0) return;
+    const int n = ctl->n_sites;
+    if (n > 0) {
+        const int ex = ctl->excess;
+        const int last = flag[n - 1] + pos[n - 1];
+        const int n_new = ex == 0 ? n : ex > 0 ? last : n * (ctl->n_copies + 1) + last;
+        if (n_new != ctl->pop) raise_error(st, SCN_ERR_NORM, n_new);
+    }
+    ctl->master = rng_stride(ctl->master, 1);
+    ctl->cur = 1 - ctl->cur;
+    const double k_new = phase == 0 ? ctl->kA[0] / ctl->nA : ctl->kI[0] / ctl->nI;
+    ctl->k_new = k_new;
+    ctl->dk[1 - ctl->cur] = k_new;
+    ctl->keff_0 = k_new;
+    if (ctl->cycle < k_hist_cap) k_hist[ctl->cycle] = k_new;
+    ctl->cycle++;
+}
+
 // ---- initial fission source (fissionSource_class.f90:149-271) ---------------------------------
 template <int NL, int GEO>
 __global__ void k_fission_source(DevProblem P, int n, uint64_t master, double b0, double b1, double b2, double t0,
@@ -315,6 +635,7 @@ struct EigenState {
     double transport_s = 0.0;
     std::vector<double> k_history, cycle_time;
     Counters c0{};
+    unsigned long long sites_dev = 0;   // cycle_graph runs: sites counted on the device so far
 };
 
 struct Engine {
@@ -371,6 +692,21 @@ struct Engine {
     int lds_bytes = 0;
     scn_stats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device-resident cycles (cycle_graph): control record, per-history site counts and offsets,
+    // scan and radix-select scratch, one captured graph per phase (0 inactive, 1 active)
+    EigCtl* ctl = nullptr;
+    int* hist_nsite = nullptr;
+    int* site_off = nullptr;
+    int* scan_bsum = nullptr;
+    int* rs_hist = nullptr;
+    int64_t* sum_mtab = nullptr;
+    double* k_hist_dev = nullptr;
+    int k_hist_cap = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> cyc_ev;
+    int dev_cycles = 0;     // 1 while a cycle_graph run is open
+    int dc_occ[2] = {0, 0}; // resident transport blocks per CU in each phase's graph
     // tuning
     int k0 = 0;             // 0: one launch per cycle runs every history to completion
     int tail_frac = 16;
@@ -499,7 +835,7 @@ int dsum_batch(Engine* E, SumJobs J)
     }
     if (mmax > 0) {
         dim3 g((unsigned)((mmax + 255) / 256), J.njobs);
-        hipLaunchKernelGGL(k_dsum_chunks, g, dim3(256), 0, E->stream, J, E->sum_y, ys);
+        hipLaunchKernelGGL(k_dsum_chunks, g, dim3(256), 0, E->stream, J, E->sum_y, ys, (int64_t*)nullptr);
     }
     // the element counts of every pairwise level, worked out on the host and uploaded at once, so
     // the levels run back to back with one synchronisation at the end
@@ -592,7 +928,7 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
         HIPCHK(hipEventRecord(E->ev0, E->stream));
         fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, first, n_in, E->live[cur], nin_ptr, E->live[1 - cur],
            nout_ptr, E->cap, steps, master, k_eff, E->sites, bins_arg, use_lds, E->hist_k, E->cap, E->counters,
-           E->status, E->counters_i + 2, E->chunk);
+           E->status, E->counters_i + 2, E->chunk, CycleIn{nullptr, nullptr, nullptr});
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(E->ev1, E->stream));
         // survivors, status and site count come back together, behind one synchronisation
@@ -797,6 +1133,232 @@ int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
     return norm_size_run(B, E->stream, E->bankB, n, tot_pop, master, E->bankA, E->bank_cap, n_new, E->err);
 }
 
+
+// ---- device-resident power iteration: host side --------------------------------------------------
+// exclusive prefix sum of n ints (n in device memory, at most cap)
+void scan_dev(Engine* E, const int* in, int* out, const int* n_ptr, int cap)
+{
+    const int nb = (cap + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(1, nb)), dim3(256), 0, E->stream, in, out, E->scan_bsum, n_ptr);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, E->stream, E->scan_bsum, n_ptr);
+    hipLaunchKernelGGL(k_scan_add, dim3((cap + 255) / 256), dim3(256), 0, E->stream, out, (const int*)E->scan_bsum, n_ptr);
+}
+
+// canonical sums (the tree of dsum_batch) with counts in device memory; J.j[q].n are capacities.
+// Results in E->sum_out[0..njobs).  No synchronisation.
+void dsum_dev(Engine* E, const SumJobs& J)
+{
+    int64_t maxn = 0;
+    for (int q = 0; q < J.njobs; ++q) maxn = std::max(maxn, J.j[q].n);
+    const int64_t ys = E->sum_cap;
+    int64_t mm = (maxn + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    hipLaunchKernelGGL(k_dsum_chunks, dim3((unsigned)std::max<int64_t>(1, (mm + 255) / 256), J.njobs), dim3(256), 0,
+                       E->stream, J, E->sum_y, ys, E->sum_mtab);
+    double* a = E->sum_y;
+    double* b = E->sum_y2;
+    int l = 0;
+    for (; mm > 2048; mm = (mm + 1) / 2, ++l) {
+        const int64_t h = (mm + 1) / 2;
+        hipLaunchKernelGGL(k_dsum_level_dev, dim3((unsigned)((h + 255) / 256), J.njobs), dim3(256), 0, E->stream, a, b,
+                           E->sum_mtab, l, ys);
+        std::swap(a, b);
+    }
+    hipLaunchKernelGGL(k_dsum_final, dim3(J.njobs), dim3(1024), 0, E->stream, a, E->sum_mtab + 8 * l, ys, E->sum_out);
+}
+
+// one cycle as a fixed launch sequence on E->stream (captured into a graph by dcycle_capture)
+int enqueue_dcycle(Engine* E, int phase, int score_bins)
+{
+    const int pop = E->eig.S.pop;
+    const void* kern = nullptr;
+    TransportFn fn = (E->transport == SCN_TRANSPORT_DT)   ? pick_nl<SCN_TRANSPORT_DT>(E->nl, E->geo, &kern)
+                     : (E->transport == SCN_TRANSPORT_HT) ? pick_nl<SCN_TRANSPORT_HT>(E->nl, E->geo, &kern)
+                                                          : pick_nl<SCN_TRANSPORT_ST>(E->nl, E->geo, &kern);
+    const int use_lds = score_bins ? E->use_lds : 0;
+    const size_t sh = use_lds ? (size_t)E->lds_bytes : 0;
+    const int grid = std::max(1, std::min((pop + BLOCK - 1) / BLOCK, E->n_cu * std::max(1, E->dc_occ[phase])));
+    DevProblem P = E->P;
+    if (!score_bins) P.n_clerks = 0;
+    EigCtl* c = E->ctl;
+    hipLaunchKernelGGL(k_dcycle_begin, dim3(1), dim3(64), 0, E->stream, E->counters_i, E->sites.count);
+    fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, 1, pop, E->live[0], E->counters_i, E->live[1],
+       E->counters_i + 1, E->cap, 1 << 30, 0, 0.0, E->sites, E->bins, use_lds, E->hist_k, E->cap, E->counters,
+       E->status, E->counters_i + 2, E->chunk, CycleIn{&c->master, &c->k_new, E->hist_nsite});
+    // next bank in (broodID, emission) order
+    scan_dev(E, E->hist_nsite, E->site_off, &c->pop, E->cap);
+    const int sg = (E->site_cap + 255) / 256;
+    hipLaunchKernelGGL(k_place_sites, dim3(sg), dim3(256), 0, E->stream, E->sites, (const int*)E->site_off, pop,
+                       E->bank_cap, E->bankB, c, E->status);
+    // START / END population weights and the keffImplicit scores, then the k estimates
+    SumJobs J{};
+    J.njobs = 6;
+    const int64_t rec = (int64_t)(sizeof(scn_particle_state) / sizeof(double));
+    J.j[0] = {&E->bankA[0].wgt, pop, rec, nullptr};
+    J.j[1] = {&E->bankB[0].wgt, E->bank_cap, rec, &c->n_sites};
+    for (int k = 0; k < 4; ++k) J.j[2 + k] = {E->hist_k + (size_t)k * E->cap, pop, 1, nullptr};
+    dsum_dev(E, J);
+    hipLaunchKernelGGL(k_dcycle_keff, dim3(1), dim3(64), 0, E->stream, c, (const double*)E->sum_out, phase);
+    if (score_bins) {
+        int rc = close_bins(E, phase == 1 ? 0 : 1);
+        if (rc) return rc;
+    }
+    // normSize_Repr
+    hipLaunchKernelGGL(k_norm_plan, dim3(1), dim3(256), 0, E->stream, c, E->rs_hist, E->status);
+    hipLaunchKernelGGL(k_site_rn_dev, dim3(sg), dim3(256), 0, E->stream, (const EigCtl*)c, E->rn, E->rn_bits);
+    const int hg = std::max(1, std::min(sg, E->n_cu * 4));
+    for (int pass = 0; pass < 8; ++pass) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(hg), dim3(256), 0, E->stream, (const uint64_t*)E->rn_bits, (const EigCtl*)c,
+                           E->rs_hist, pass);
+        hipLaunchKernelGGL(k_rs_pick, dim3(1), dim3(256), 0, E->stream, c, (const int*)E->rs_hist, pass);
+    }
+    hipLaunchKernelGGL(k_norm_flags_dev, dim3(sg), dim3(256), 0, E->stream, (const double*)E->rn, (const EigCtl*)c,
+                       E->flag);
+    scan_dev(E, E->flag, E->pos, &c->n_sites, E->site_cap);
+    hipLaunchKernelGGL(k_brood_bounds_dev, dim3(sg), dim3(256), 0, E->stream, (const scn_particle_state*)E->bankB,
+                       (const EigCtl*)c, E->cap + 1, E->start_of, E->end_of);
+    hipLaunchKernelGGL(k_norm_apply, dim3(sg), dim3(256), 0, E->stream, (const scn_particle_state*)E->bankB,
+                       (const int*)E->flag, (const int*)E->pos, (const int*)E->start_of, (const int*)E->end_of,
+                       (const EigCtl*)c, E->cap + 1, E->bankA, E->bank_cap);
+    hipLaunchKernelGGL(k_dcycle_end, dim3(1), dim3(64), 0, E->stream, c, (const int*)E->flag, (const int*)E->pos,
+                       E->k_hist_dev, E->k_hist_cap, phase, E->status);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+void dcycle_release_graphs(Engine* E)
+{
+    for (int ph = 0; ph < 2; ++ph) {
+        if (E->gexec[ph]) (void)hipGraphExecDestroy(E->gexec[ph]);
+        if (E->graph[ph]) (void)hipGraphDestroy(E->graph[ph]);
+        E->gexec[ph] = nullptr;
+        E->graph[ph] = nullptr;
+    }
+}
+
+// buffers, initial control record and the two phase graphs of a cycle_graph run
+int dcycle_setup(Engine* E, const scn_eigen_settings& S)
+{
+    int rc = 0;
+    if (!E->ctl) {
+        rc |= dalloc(E, &E->ctl, 1);
+        rc |= dalloc(E, &E->hist_nsite, E->cap);
+        rc |= dalloc(E, &E->site_off, E->cap);
+        rc |= dalloc(E, &E->scan_bsum, (size_t)std::max(E->site_cap, E->cap) / SCAN_PER_BLOCK + 2);
+        rc |= dalloc(E, &E->rs_hist, 8 * 256);
+        rc |= dalloc(E, &E->sum_mtab, (size_t)8 * (SCN_DSUM_MAX_LEVELS + 2));
+        if (rc) return rc;
+    }
+    const int ncyc = S.n_inactive + S.n_active;
+    if (ncyc > E->k_hist_cap) {
+        rc = dalloc(E, &E->k_hist_dev, (size_t)ncyc);
+        if (rc) return rc;
+        E->k_hist_cap = ncyc;
+    }
+    while ((int)E->cyc_ev.size() < ncyc + 1) {
+        hipEvent_t ev;
+        HIPCHK(hipEventCreate(&ev));
+        E->cyc_ev.push_back(ev);
+    }
+    rc = ensure_sum(E, (int64_t)E->bank_cap);
+    if (rc) return rc;
+    const EigenState& X = E->eig;
+    EigCtl h{};
+    h.master = X.master;
+    h.k_new = X.k_new;
+    h.keff_0 = X.keff_0;
+    h.dk[0] = X.dk[0];
+    h.dk[1] = X.dk[1];
+    h.cur = X.cur;
+    h.pop = S.pop;
+    COPY_SYNC(E->ctl, &h, sizeof h, hipMemcpyHostToDevice);
+    dcycle_release_graphs(E);
+    for (int ph = 0; ph < 2; ++ph) {
+        const int score_bins = (ph == 1 || S.inactive_tally) && E->n_bins > 0;
+        // resident blocks of the transport kernel, worked out before the capture starts
+        const void* kern = nullptr;
+        if (E->transport == SCN_TRANSPORT_DT) pick_nl<SCN_TRANSPORT_DT>(E->nl, E->geo, &kern);
+        else if (E->transport == SCN_TRANSPORT_HT) pick_nl<SCN_TRANSPORT_HT>(E->nl, E->geo, &kern);
+        else pick_nl<SCN_TRANSPORT_ST>(E->nl, E->geo, &kern);
+        const size_t sh = (score_bins && E->use_lds) ? (size_t)E->lds_bytes : 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&E->dc_occ[ph], kern, BLOCK, sh));
+        HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_dcycle(E, ph, score_bins);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(E->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            E->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+            return SCN_ERR_HIP;
+        }
+        E->graph[ph] = g;
+        HIPCHK(hipGraphInstantiate(&E->gexec[ph], g, nullptr, nullptr, 0));
+    }
+    return 0;
+}
+
+// run up to n cycles from the graphs; the control record comes back to the host once at the end
+int dcycle_step(Engine* E, int n_cycles, int* done)
+{
+    EigenState& X = E->eig;
+    const scn_eigen_settings& S = X.S;
+    int ran = 0;
+    HIPCHK(hipEventRecord(E->cyc_ev[0], E->stream));
+    while (ran < n_cycles && X.phase < 2) {
+        const int ncyc = X.phase == 0 ? S.n_inactive : S.n_active;
+        if (X.c_in_phase >= ncyc) {
+            X.phase += 1;
+            X.c_in_phase = 0;
+            continue;
+        }
+        HIPCHK(hipGraphLaunch(E->gexec[X.phase], E->stream));
+        HIPCHK(hipEventRecord(E->cyc_ev[ran + 1], E->stream));
+        X.c_in_phase += 1;
+        ++ran;
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    int rc = check_status(E);
+    if (rc) return rc;
+    EigCtl h{};
+    COPY_SYNC(&h, E->ctl, sizeof h, hipMemcpyDeviceToHost);
+    const int first = (int)X.k_history.size();
+    std::vector<double> kh(ran > 0 ? ran : 1);
+    if (ran > 0) COPY_SYNC(kh.data(), E->k_hist_dev + first, sizeof(double) * ran, hipMemcpyDeviceToHost);
+    for (int c = 0; c < ran; ++c) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, E->cyc_ev[c], E->cyc_ev[c + 1]));
+        X.k_history.push_back(kh[c]);
+        X.cycle_time.push_back(ms * 1e-3);
+        X.transport_s += ms * 1e-3;
+        E->stats.cycle_ms += ms;
+    }
+    X.master = h.master;
+    X.k_new = h.k_new;
+    X.keff_0 = h.keff_0;
+    X.dk[0] = h.dk[0];
+    X.dk[1] = h.dk[1];
+    X.cur = h.cur;
+    X.kA[0] = h.kA[0];
+    X.kA[1] = h.kA[1];
+    X.kI[0] = h.kI[0];
+    X.kI[1] = h.kI[1];
+    for (int k = 0; k < 4; ++k) {
+        X.imp[k][0] = h.imp[k][0];
+        X.imp[k][1] = h.imp[k][1];
+    }
+    X.nA = h.nA;
+    X.nI = h.nI;
+    X.n_this = S.pop;
+    X.n_hist += (int64_t)S.pop * ran;
+    E->stats.n_histories += (int64_t)S.pop * ran;
+    E->stats.n_sites += (int64_t)(h.sites_total - X.sites_dev);
+    X.sites_dev = h.sites_total;
+    E->stats.n_cycles += ran;
+    if (done) *done = ran;
+    return 0;
+}
 
 }  // namespace
 
@@ -1017,6 +1579,8 @@ void scn_destroy(void* engine)
     Engine* E = (Engine*)engine;
     if (!E) return;
     if (E->stream) (void)hipStreamSynchronize(E->stream);
+    dcycle_release_graphs(E);
+    for (hipEvent_t ev : E->cyc_ev) (void)hipEventDestroy(ev);
     for (auto& b : E->owned) (void)hipFree(b.p);
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -1162,6 +1726,20 @@ int scn_eigen_start(void* engine, const scn_eigen_settings* S, const scn_particl
     X.n_this = pop;
     X.open = 1;
     COPY_SYNC(&X.c0, E->counters, sizeof(Counters), hipMemcpyDeviceToHost);
+    E->dev_cycles = 0;
+    if (S->cycle_graph) {
+        if (E->k0 != 0) {
+            E->err = "cycle_graph needs one transport launch per cycle (events_per_launch 0)";
+            X.open = 0;
+            return SCN_ERR_ARG;
+        }
+        int rc = dcycle_setup(E, *S);
+        if (rc) {
+            X.open = 0;
+            return rc;
+        }
+        E->dev_cycles = 1;
+    }
     return 0;
 }
 
@@ -1169,6 +1747,7 @@ int scn_eigen_step(void* engine, int32_t n_cycles, int32_t* done)
 {
     Engine* E = (Engine*)engine;
     if (!E || !E->eig.open) return SCN_ERR_ARG;
+    if (E->dev_cycles) return dcycle_step(E, n_cycles, done);
     EigenState& X = E->eig;
     const scn_eigen_settings& S = X.S;
     const int pop = S.pop;

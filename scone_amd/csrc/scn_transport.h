@@ -47,9 +47,16 @@ struct Counters {
     unsigned long long pad;
 };
 
+// master RNG state and k of a cycle read from device memory (cycles replayed from a graph)
+struct CycleIn {
+    const uint64_t* master;   // null: use the by-value master and k_eff
+    const double* k_eff;
+    int* hist_nsite;          // [hist_cap]: fission sites each history emitted (null: not recorded)
+};
+
 typedef void (*TransportFn)(dim3, dim3, size_t, hipStream_t, const DevProblem&, const scn_particle_state*, int, int,
                             Live, const int*, Live, int*, int, int, uint64_t, double, Sites, double*, int, double*,
-                            int, Counters*, DevStatus*, int*, int);
+                            int, Counters*, DevStatus*, int*, int, CycleIn);
 
 // launcher and kernel address of one variant: register depth NL, tracking mode MODE, geometry
 // variant GEO (0: lattice/pin/box, 1: cells, rotations, CSG borders)
@@ -143,10 +150,14 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             const int* __restrict__ n_in_ptr, Live out, int* __restrict__ n_out, int cap, int max_steps,
             uint64_t master, double k_eff, Sites S, double* __restrict__ bins, int use_lds,
             double* __restrict__ hist_k, int hist_cap, Counters* __restrict__ cnt, DevStatus* st, int* work,
-            int chunk)
+            int chunk, CycleIn cin)
 {
     extern __shared__ double lds_bins[];
     const int n_in = first ? n_first : *n_in_ptr;
+    if (cin.master) {
+        master = *cin.master;
+        k_eff = *cin.k_eff;
+    }
     if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) lds_bins[i] = 0.0;
     }
@@ -256,6 +267,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
                 alive = 0;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) hist_k[(size_t)k * hist_cap + (p.brood - 1)] = p.k[k];
+                if (cin.hist_nsite) cin.hist_nsite[p.brood - 1] = p.nsite;
             }
         }
         // park step-limited survivors: wave ballot, one atomic per wave
@@ -298,10 +310,10 @@ template <int NL, int MODE, int GEO>
 void launch_transport(dim3 g, dim3 b, size_t sh, hipStream_t s, const DevProblem& P, const scn_particle_state* bank,
                       int first, int n_first, Live in, const int* n_in_ptr, Live out, int* n_out, int cap,
                       int max_steps, uint64_t master, double k_eff, Sites S, double* bins, int use_lds,
-                      double* hist_k, int hist_cap, Counters* cnt, DevStatus* st, int* work, int chunk)
+                      double* hist_k, int hist_cap, Counters* cnt, DevStatus* st, int* work, int chunk, CycleIn cin)
 {
     hipLaunchKernelGGL((k_transport<NL, MODE, GEO>), g, b, sh, s, P, bank, first, n_first, in, n_in_ptr, out, n_out, cap,
-                       max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st, work, chunk);
+                       max_steps, master, k_eff, S, bins, use_lds, hist_k, hist_cap, cnt, st, work, chunk, cin);
 }
 
 template <int NL, int MODE, int GEO>

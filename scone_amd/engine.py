@@ -174,6 +174,7 @@ class Engine:
         S.keff0 = settings.get("keff0", 1.0)
         S.events_per_launch = settings.get("events_per_launch", 0)
         S.block_size = settings.get("block_size", 0)
+        S.cycle_graph = settings.get("cycle_graph", 0)
         return S
 
     def _result(self, S):
