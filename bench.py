@@ -2,19 +2,25 @@
 """Histories/s and solve latency of the MI355X engine, batched vs plain, plus the serving loop.
 
 Default workload (the batched fixed-source benchmark): 512 independent fixed-source problems
-(1D-3D boxes with 1-8 material regions, 2-70 groups, 1e3-1e6 histories in 10 batches, point or
-volume sources, a 1e3-1e5-bin mesh flux tally each; synthetic seeded data), FP64, delta tracking.
-A step is one batch of every resident problem: W warmup steps, then K timed steps (W + K = batches
-per problem; the default K = 10 times exactly the 10-batch job of every problem).  An engine cycle transports `window` steps (every problem's next `window` batches in
-one launch, closed in order); the autotuner picks the window among the divisors of W and K.
-  batched  all 512 problems resident in one batch engine, cycles replayed from a hipGraph, launch
-           shape from the autotuner
+(1D-3D boxes with 1-8 material regions, 2-70 groups, 1e3-1e6 histories split into 10 batches, point
+or volume sources, a 1e3-1e5-bin mesh flux tally each; synthetic seeded data), FP64, delta tracking.
+A step is one batch of every resident problem.  An engine cycle transports WINDOW = 10 steps (every
+problem's next 10 batches, its whole 10-batch job, in one launch, closed in order).  The W warmup
+steps are rounded up to a multiple of the window and run first on the same problems; then exactly
+K timed steps (K a multiple of the window; otherwise the window falls back to the largest of 5, 2, 1
+dividing K, and the line says so).  Every problem therefore runs W' + K batches.
+  batched  all 512 problems resident in one batch engine, cycles replayed from a hipGraph, tile and
+           launch shape from the autotuner
   plain    the same engine holding one problem at a time (each problem's batches run alone)
 Both run the identical histories (checked on a sample of problems: score results equal to 1e-12).
 Also reported: the serving loop (continuous batching with HBM-budget admission and 1 % early stop)
 on the 17x17 pin-lattice request stream (fixed-source solves of the fissile lattice, fission chains
-followed as secondaries) at 50/75/100 % of measured capacity with p50/p99 latency, and the plain
-eigenvalue engine on the 17x17 lattice (1e6 histories per cycle).
+followed as secondaries) at 50/75/100 % of measured capacity with p50/p99 latency, the plain
+eigenvalue engine on the 17x17 lattice (1e6 histories per cycle) and the 3D core (1e7 per cycle).
+
+The CPU baseline is the C oracle on the host: every one of the 512 problems, `--cpu-batches`
+batches each, problems run concurrently (one per worker thread, largest first) on all the cores
+this process may use.
 
 --workload eigen prints the eigenvalue line instead.  Prints one JSON line on stdout.
 """
@@ -37,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 S_P = 96            # bytes of compulsory particle state per event (loaded and stored: 2 * S_P)
 SCORE_B = 16        # FP64 read-modify-write per tally score
 SITE_B = 64         # fission-site write
+WINDOW = 10         # batches of every problem per engine cycle: the 10-batch job of each problem
 
 
 def log(msg):
@@ -206,23 +213,49 @@ def run_plain(problems, transport, settings, warmup, steps):
 SAMPLE = (0, 1, 2, 3, 100, 257, 511)
 
 
-def cpu_batch_baseline(problems, seconds):
+def host_cpu():
+    """Cores this process may run on (affinity, capped by OMP_NUM_THREADS when set), the machine's
+    logical CPU count and the CPU model."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(allowed, int(omp)) if omp.isdigit() and int(omp) > 0 else allowed
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, {"nproc": os.cpu_count(), "affinity": allowed, "omp_num_threads": omp or None, "model": model}
+
+
+def cpu_batch_baseline(problems, n_batches):
+    """The C oracle on every problem for n_batches batches (the per-batch history counts of the
+    workload), problems run concurrently: one single-threaded solve per worker thread, largest
+    first (ctypes releases the interpreter lock during each call)."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as O
-    threads = min(16, os.cpu_count() or 1)
-    n_hist = 0
-    done = 0
+    cores, host = host_cpu()
+    order = sorted(range(len(problems)), key=lambda i: -problems[i][2])
+
+    def solve(i):
+        c, s, pop, _, seed = problems[i]
+        O.fixed_run(c, s, pop, n_batches, seed, threads=1)
+        return pop * n_batches
+
     t0 = time.perf_counter()
-    for (c, s, pop, cycles, seed) in problems:
-        O.fixed_run(c, s, pop, cycles, seed, threads=threads)
-        n_hist += pop * cycles
-        done += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        n_hist = sum(ex.map(solve, order))
     dt = time.perf_counter() - t0
-    return {"value": n_hist / dt, "unit": "histories/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of the 512 problems, all 10 batches each ({n_hist} histories), "
-                      f"OpenMP oracle, {dt:.1f} s"}
+    return {"value": n_hist / dt, "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": f"all {len(problems)} problems, {n_batches} batches each ({n_hist} histories), one problem "
+                      f"per worker thread, C oracle, {dt:.1f} s", "host": host}
 
 
 # ---------------------------------------------------------------------------------------------
@@ -283,41 +316,46 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
 # ---------------------------------------------------------------------------------------------
 # eigenvalue (plain engine, 17x17 lattice)
 # ---------------------------------------------------------------------------------------------
-def run_core3d(transport, pop=10_000_000, inactive=2, active=3, seed=5):
+def run_core3d(transport, pop=10_000_000, inactive=2, active=3, seed=5, cycle_graph=0):
     """3D nested core (pin -> 17x17 assembly -> 15x15 core with a 20 cm reflector, 400 cm tall,
     70 groups, 200x200x50x70 = 1.4e8-bin mesh flux tally), 1e7 histories per cycle: a bounded
-    sample of its 30 + 100 cycles (inactive cycles untimed, active cycles timed with tallies)."""
+    sample of its 30 + 100 cycles (inactive cycles untimed, active cycles timed with tallies).
+    cycle_graph = 1: the device-resident cycles replayed from hipGraphs (no host round trip)."""
     from scone_amd import configs, tables as T
     from scone_amd.engine import Engine
     inp, mg = configs.core_3d(pop=pop, transport=transport)
     comp = T.compile_problem(inp, mg)
     eng = Engine(comp, pop)
     bank = eng.fission_source(pop, seed)
-    eng.eigen_start(dict(pop=pop, inactive=inactive, active=active, seed=seed), bank)
+    eng.eigen_start(dict(pop=pop, inactive=inactive, active=active, seed=seed, cycle_graph=cycle_graph), bank)
     eng.eigen_step(inactive)
     s0 = eng.stats()
     t0 = time.perf_counter()
     done = eng.eigen_step(active)
     wall = time.perf_counter() - t0
     s1 = eng.stats()
-    R, _, _, _, _ = eng.eigen_finish()
+    R, kh, _, _, _ = eng.eigen_finish()
     eng.close()
     assert done == active
     d = diff(s0, s1)
-    return {"hist_per_s": d["n_histories"] / wall, "coll_per_s": d["n_collisions"] / wall,
-            "ms_per_cycle": wall / active * 1e3, "bins": int(comp.problem.n_bins),
-            "kernel_ms_per_launch": d["transport_kernel_ms"] / max(1, d["transport_launches"]),
-            "k_implicit": list(R.k_implicit), "cycles_timed": active, "pop": pop}
+    out = {"hist_per_s": d["n_histories"] / wall, "coll_per_s": d["n_collisions"] / wall,
+           "ms_per_cycle": wall / active * 1e3, "bins": int(comp.problem.n_bins),
+           "k_implicit": list(R.k_implicit), "k_history": [float(x) for x in kh], "cycles_timed": active, "pop": pop,
+           "engine": "cycle_graph" if cycle_graph else "plain"}
+    if not cycle_graph:
+        out["kernel_ms_per_launch"] = d["transport_kernel_ms"] / max(1, d["transport_launches"])
+        out["kernel_share_of_cycle"] = d["transport_kernel_ms"] / max(1e-9, wall * 1e3)
+    return out
 
 
-def run_eigen(transport, pop, warmup, steps, seed=2):
+def run_eigen(transport, pop, warmup, steps, seed=2, cycle_graph=0):
     from scone_amd import configs, tables as T
     from scone_amd.engine import Engine
     inp, mg = configs.pin_lattice_7g(pop=pop, inactive=warmup, active=steps, transport=transport)
     comp = T.compile_problem(inp, mg)
     eng = Engine(comp, pop)
     bank = eng.fission_source(pop, seed)
-    eng.eigen_start(dict(pop=pop, inactive=warmup, active=steps, seed=seed), bank)
+    eng.eigen_start(dict(pop=pop, inactive=warmup, active=steps, seed=seed, cycle_graph=cycle_graph), bank)
     eng.eigen_step(warmup)
     s0 = eng.stats()
     t0 = time.perf_counter()
@@ -343,7 +381,7 @@ def cpu_eigen_baseline(transport, pop, seconds=10.0, seed=2):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as O
     from scone_amd import configs, tables as T
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_cpu()
     inp, mg = configs.pin_lattice_7g(pop=pop, transport=transport)
     comp = T.compile_problem(inp, mg)
     bank = O.fission_source(comp, pop, seed)
@@ -361,7 +399,7 @@ def cpu_eigen_baseline(transport, pop, seconds=10.0, seed=2):
     dt = time.perf_counter() - t0
     return {"value": n_hist / dt, "unit": "histories/s", "cores": threads, "kind": "port",
             "sample": f"{cycles} cycle(s) of {pop} histories ({transport[-2:]}, 17x17 lattice, 7 groups, "
-                      f"collision clerks on), OpenMP oracle, {dt:.1f} s"}
+                      f"collision clerks on), OpenMP oracle, {dt:.1f} s", "host": host}
 
 
 def eigen_line(args):
@@ -393,56 +431,71 @@ def batch_line(args):
     transport = T.TRANSPORT_DT if args.transport == "transportOperatorDT" else T.TRANSPORT_ST
     log(f"batch: compiling {args.problems} problems")
     problems = batch_problems(args.problems, args.transport, args.scale)
-    g = math.gcd(args.warmup, args.steps) if args.warmup else args.steps
-    windows = tuple(w for w in (1, 2, 5, 10) if g % w == 0)
-    budget = budget_for(problems, args.warmup + args.steps, window=max(windows))
+    window = next(w for w in (WINDOW, 5, 2, 1) if args.steps % w == 0)
+    warmup = -(-args.warmup // window) * window
+    if window != WINDOW:
+        log(f"batch: {args.steps} timed steps are not a multiple of {WINDOW}: window {window}")
+    budget = budget_for(problems, warmup + args.steps, window=window)
     tot_hist = sum(p[2] for p in problems)
-    log(f"batch: {tot_hist} histories per batch, device budget {budget / 2**30:.2f} GiB, windows {windows}")
+    log(f"batch: {tot_hist} histories per batch, device budget {budget / 2**30:.2f} GiB, window {window}, "
+        f"warmup {warmup} batches")
     if args.tune:
         t = time.perf_counter()
         best, table = autotune.tune([(c, s, pop, seed) for (c, s, pop, _, seed) in problems], transport, budget,
-                                    windows=windows, log=log)
+                                    launches=(1,), windows=(window,), log=log)
         tune_s = time.perf_counter() - t
     else:
-        best, table, tune_s = {"tile": 256, "k0": 8, "n_launch": 1, "window": max(windows)}, [], 0.0
+        best, table, tune_s = {"tile": 256, "k0": 8, "n_launch": 1, "window": window}, [], 0.0
     log(f"batch: tuned {best} in {tune_s:.1f} s")
-    bat, res_b = run_batched(problems, transport, best, args.warmup, args.steps, budget)
+    bat, res_b = run_batched(problems, transport, best, warmup, args.steps, budget)
     log(f"batch: batched {bat['hist_per_s']:.3e} histories/s, {bat['ms_per_step']:.2f} ms/step")
     kt = run_batched_kernel_timing(problems, transport, best, budget)
     log(f"batch: transport kernel {kt['kernel_ms_per_launch']:.3f} ms/launch, {kt['achieved_gbs']:.1f} GB/s")
     pla, res_p = (None, {})
     if not args.no_plain:
-        pla, res_p = run_plain(problems, transport, best, args.warmup, args.steps)
+        pla, res_p = run_plain(problems, transport, best, warmup, args.steps)
         log(f"batch: plain {pla['hist_per_s']:.3e} histories/s over {pla['wall_s']:.1f} s")
     identical = None
     if res_p:
-        identical = all(results_agree(res_b[i], res_p[i], args.warmup + args.steps) for i in res_b)
-    cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_seconds)
+        identical = all(results_agree(res_b[i], res_p[i], warmup + args.steps) for i in res_b)
+    cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_batches)
     serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds, window=args.window)
     eig = None
     if args.eigen:
         log("eigen leg")
         e = run_eigen(args.transport, args.pop, 3, 10)
+        eg = run_eigen(args.transport, args.pop, 3, 10, cycle_graph=1)
         eig = {k: e[k] for k in ("hist_per_s", "coll_per_s", "ms_per_step", "achieved_gbs", "kernel_ms_per_launch",
                                  "k_implicit")}
+        eig["cycle_graph"] = {k: eg[k] for k in ("hist_per_s", "ms_per_step", "k_implicit")}
+        eig["cycle_graph"]["same_k_as_plain"] = eg["k_implicit"] == e["k_implicit"]
+        log(f"eigen 17x17: plain {e['hist_per_s']:.3e}, graph {eg['hist_per_s']:.3e} histories/s")
     core = None
     if args.c5:
         log("core 3D leg")
         core = run_core3d(args.transport)
-        log(f"core 3D: {core['hist_per_s']:.3e} histories/s, {core['ms_per_cycle']:.1f} ms/cycle")
+        cg = run_core3d(args.transport, cycle_graph=1)
+        core["cycle_graph"] = {k: cg[k] for k in ("hist_per_s", "ms_per_cycle", "k_implicit")}
+        core["cycle_graph"]["same_k_history_as_plain"] = cg["k_history"] == core["k_history"]
+        log(f"core 3D: plain {core['hist_per_s']:.3e} histories/s, {core['ms_per_cycle']:.1f} ms/cycle; graph "
+            f"{cg['hist_per_s']:.3e}, {cg['ms_per_cycle']:.1f} ms/cycle")
     line = {
         "metric": METRIC, "value": bat["hist_per_s"], "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": bat["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded problems and multigroup data)",
-        "config": {"workload": "batched fixed source: 512 problems, 1D-3D boxes, 1-8 regions, 2-70 groups, "
-                               "1e3-1e6 histories in 10 batches, point/volume sources, 1e3-1e5-bin mesh flux tallies",
+        "config": {"workload": f"batched fixed source: {args.problems} problems, 1D-3D boxes, 1-8 regions, 2-70 "
+                               f"groups, a batch of each is 1/10 of its 1e3-1e6 histories, point/volume sources, "
+                               f"1e3-1e5-bin mesh flux tallies; {warmup} warmup + {args.steps} timed batches per "
+                               f"problem, {window} batches of every problem per launch",
                    "problems": args.problems, "histories_per_step": tot_hist, "transport": args.transport[-2:],
+                   "batches_per_problem": warmup + args.steps, "warmup_batches": warmup, "timed_batches": args.steps,
                    "engine": "batched (persistent launch, hipGraph cycles, autotuned)", "launch_shape": best},
         "collisions_per_s": bat["coll_per_s"],
         "latency": {"batched_p50_s": bat["latency_s"], "batched_p99_s": bat["latency_s"],
                     "plain_p50_s": pla["latency_p50_s"] if pla else None,
                     "plain_p99_s": pla["latency_p99_s"] if pla else None,
-                    "note": "all 512 solves submitted at t=0"},
+                    "note": "all solves submitted at t=0; batched: every solve closes its last batch in the "
+                            "same final cycle, so p50 = p99 = the whole run"},
         "plain": {"hist_per_s": pla["hist_per_s"], "wall_s": pla["wall_s"]} if pla else None,
         "batched_over_plain": (bat["hist_per_s"] / pla["hist_per_s"]) if pla else None,
         "identical_results_sample": identical,
@@ -479,7 +532,8 @@ def main():
     ap.add_argument("--window", type=int, default=10, help="serving: batches of a request transported per cycle")
     ap.add_argument("--no-eigen", dest="eigen", action="store_false", default=True)
     ap.add_argument("--no-c5", dest="c5", action="store_false", default=True)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="eigenvalue CPU baseline: seconds of cycles")
+    ap.add_argument("--cpu-batches", type=int, default=10, help="batched CPU baseline: batches of every problem")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
