@@ -442,10 +442,11 @@ def batch_line(args):
     if args.tune:
         t = time.perf_counter()
         best, table = autotune.tune([(c, s, pop, seed) for (c, s, pop, _, seed) in problems], transport, budget,
-                                    launches=(1,), windows=(window,), log=log)
+                                    tiles=(64, 128, 256, 512), launches=(1,), windows=(window,), refills=(16, 32),
+                                    log=log)
         tune_s = time.perf_counter() - t
     else:
-        best, table, tune_s = {"tile": 256, "k0": 8, "n_launch": 1, "window": window}, [], 0.0
+        best, table, tune_s = {"tile": 128, "k0": 8, "n_launch": 1, "window": window, "refill": 32}, [], 0.0
     log(f"batch: tuned {best} in {tune_s:.1f} s")
     bat, res_b = run_batched(problems, transport, best, warmup, args.steps, budget)
     log(f"batch: batched {bat['hist_per_s']:.3e} histories/s, {bat['ms_per_step']:.2f} ms/step")

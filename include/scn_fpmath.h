@@ -57,6 +57,61 @@ SCN_HD double scn_fmin(double a, double b) { return (b < a) ? b : a; }
 SCN_HD double scn_fmax(double a, double b) { return (b > a) ? b : a; }
 SCN_HD double scn_sign(double a, double b) { return copysign(fabs(a), b); }
 
+/* floor(x / y) exactly as the IEEE division rounds it, mostly without the division.
+ * t = x * y_inv with y_inv = fl(1 / y) is within 3.5e-16 |t| of fl(x / y); when t is further than
+ * 4e-15 |t| from both integers around it, floor(t) = floor(fl(x / y)).  Otherwise (t close to an
+ * integer, or not finite) the division decides.  Near zero the sign alone decides and the test is
+ * relative, so small quotients never fall back. */
+SCN_HD double scn_floor_div(double x, double y, double y_inv)
+{
+    const double t = x * y_inv;
+    const double f = floor(t);
+    const double tol = 4.0e-15 * fabs(t);
+    if (t - f > tol && (f + 1.0) - t > tol) return f;
+    return floor(x / y);
+}
+
+/* a / n for 0 <= a < 2^31 and n >= 1 given n_inv = fl(1 / n), without an integer division:
+ * (a + 1/2) / n = q + (r + 1/2) / n lies at least 1 / (2n) inside (q, q + 1), and the product with
+ * n_inv is off by at most a 2^-52 / n < 1 / (2n), so its truncation is q. */
+SCN_HD int scn_idiv(int a, double n_inv) { return (int)(((double)a + 0.5) * n_inv); }
+
+/* Order-independent score accumulation.  A score v (|v| < 2^63) becomes the 128-bit two's-
+ * complement integer q = v * 2^64 truncated below 2^-64: hi = floor(v), lo = (v - floor(v)) 2^64.
+ * It is added as four 32-bit limbs, each into its own 64-bit word, so a word holds 2^31 additions
+ * without overflow and the four words are the same whatever the order of the additions.  The
+ * value of a bin is read back by carrying the words into 128 bits and rounding once per half
+ * (hi exactly, lo * 2^-64 then the sum): deterministic, within 2 ulp of the exact sum of the
+ * truncated scores, each truncation at most 2^-64 absolute. */
+typedef struct scn_fix { uint64_t lo; int64_t hi; } scn_fix;
+
+SCN_HD scn_fix scn_fix_of(double v)
+{
+    scn_fix q;
+    const double h = floor(v);
+    q.hi = (int64_t)h;
+    q.lo = (uint64_t)((v - h) * 18446744073709551616.0);
+    return q;
+}
+
+/* the four limbs of q: w[0] += bits 0-31, w[1] += bits 32-63, w[2] += bits 64-95,
+ * w[3] += bits 96-127 sign-extended (added as two's complement) */
+SCN_HD void scn_fix_limbs(scn_fix q, uint64_t limb[4])
+{
+    limb[0] = q.lo & 0xFFFFFFFFULL;
+    limb[1] = q.lo >> 32;
+    limb[2] = (uint64_t)q.hi & 0xFFFFFFFFULL;
+    limb[3] = (uint64_t)(q.hi >> 32);
+}
+
+SCN_HD double scn_fix_value(const uint64_t w[4])
+{
+    const uint64_t lo = w[0] + (w[1] << 32);
+    const uint64_t carry = lo < w[0] ? 1ULL : 0ULL;
+    const uint64_t hi = (w[1] >> 32) + w[2] + (w[3] << 32) + carry;
+    return (double)(int64_t)hi + (double)lo * 0x1.0p-64;
+}
+
 /* ---- RNG ---------------------------------------------------------------------------------- */
 #define SCN_LCG_G      2806196910506780709ULL
 #define SCN_LCG_MASK   0x7FFFFFFFFFFFFFFFULL
@@ -94,6 +149,56 @@ SCN_HD uint64_t scn_rng_skip(uint64_t s, int64_t k_in)
 
 /* RNG_class.f90:310-316: skip by strideSize * n */
 SCN_HD uint64_t scn_rng_stride(uint64_t s, int64_t n) { return scn_rng_skip(s, SCN_STRIDE * n); }
+
+#ifdef __cplusplus
+/* stride(s, n) = A_n s + C_n (mod 2^63): the maps of strides b * 256^k (k < 5, b < 256), built at
+ * compile time; a stride by n < 2^40 applies the five maps of n's bytes (they are powers of one
+ * map, so they commute).  The same states as scn_rng_stride, with five multiplies. */
+struct scn_rng_stride_table {
+    uint64_t a[5][256];
+    uint64_t c[5][256];
+};
+
+constexpr scn_rng_stride_table scn_make_rng_stride_table()
+{
+    scn_rng_stride_table t{};
+    uint64_t ja[63] = {}, jc[63] = {};   /* the generator advanced by 2^i steps */
+    uint64_t a = SCN_LCG_G, c = 1ULL;
+    for (int i = 0; i < 63; ++i) {
+        ja[i] = a;
+        jc[i] = c;
+        c = (c * (a + 1ULL)) & SCN_LCG_MASK;
+        a = (a * a) & SCN_LCG_MASK;
+    }
+    for (int k = 0; k < 5; ++k) {
+        /* the map of a stride by 256^k histories: skip 152917 * 256^k steps */
+        uint64_t steps = (uint64_t)SCN_STRIDE << (8 * k), ma = 1ULL, mc = 0ULL;
+        for (int i = 0; i < 63; ++i)
+            if ((steps >> i) & 1ULL) {
+                mc = (ja[i] * mc + jc[i]) & SCN_LCG_MASK;
+                ma = (ja[i] * ma) & SCN_LCG_MASK;
+            }
+        t.a[k][0] = 1ULL;
+        t.c[k][0] = 0ULL;
+        for (int b = 1; b < 256; ++b) {
+            t.c[k][b] = (ma * t.c[k][b - 1] + mc) & SCN_LCG_MASK;
+            t.a[k][b] = (ma * t.a[k][b - 1]) & SCN_LCG_MASK;
+        }
+    }
+    return t;
+}
+
+SCN_HD uint64_t scn_rng_stride_bytes(const scn_rng_stride_table& t, uint64_t s, int64_t n)
+{
+    if (n < 0 || n >= ((int64_t)1 << 40)) return scn_rng_stride(s, n);
+    const uint64_t u = (uint64_t)n;
+    for (int k = 0; k < 5; ++k) {
+        const int b = (int)((u >> (8 * k)) & 255ULL);
+        s = (t.a[k][b] * s + t.c[k][b]) & SCN_LCG_MASK;
+    }
+    return s;
+}
+#endif
 
 /* ---- log --------------------------------------------------------------------------------- */
 SCN_HD double scn_log(double x)

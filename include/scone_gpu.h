@@ -256,7 +256,11 @@ typedef struct scn_problem {
      * lighter than the lower weight survives Russian roulette with weight lower*constSurvival.
      * ww_n = 0: no weight windows. */
     int32_t ww_n;
-    int32_t pad1;
+    int32_t fixed_point_tally; /* 1: collision/track clerk BINs accumulate as 128-bit fixed point (each score
+                                  v * 2^64 truncated, added as four 32-bit limbs into 64-bit words): the
+                                  sums are the same bits whatever the order the scores arrive in, so
+                                  runs and batched/plain solves give identical results.  0: FP64 atomics
+                                  (faster; sums equal to rounding of summation order). */
     const scn_clerk* ww_map;
     const double* ww_lower;
     const double* ww_upper;
@@ -359,6 +363,10 @@ typedef struct scn_batch_settings {
     int32_t k0;              /* (flight, collision) pairs per particle in the first launch; 0 = default */
     int32_t n_launch;        /* launches per cycle, the last runs to completion; 0 = default */
     int32_t use_graph;       /* 1: replay each cycle's launch sequence from a captured hipGraph */
+    int32_t refill;          /* idle lanes a wave gathers before it starts new histories (1..64, 0 = default):
+                                a start (source sampling, RNG stride, placement) runs once for many lanes
+                                instead of once for every lane that frees up; results do not change */
+    int32_t pad0;
 } scn_batch_settings;
 
 /* ---- counters of the engine (cumulative since creation or the last reset) ----------------- */

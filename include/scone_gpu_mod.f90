@@ -170,7 +170,7 @@ module scone_gpu_mod
     integer(c_int32_t) :: collision
     integer(c_int32_t) :: max_split
     integer(c_int32_t) :: ww_n
-    integer(c_int32_t) :: pad1
+    integer(c_int32_t) :: fixed_point_tally
     type(c_ptr)        :: ww_map
     type(c_ptr)        :: ww_lower
     type(c_ptr)        :: ww_upper
@@ -252,6 +252,8 @@ module scone_gpu_mod
     integer(c_int32_t) :: k0
     integer(c_int32_t) :: n_launch
     integer(c_int32_t) :: use_graph
+    integer(c_int32_t) :: refill
+    integer(c_int32_t) :: pad0
   end type scn_batch_settings
 
   type, public, bind(C) :: scn_stats

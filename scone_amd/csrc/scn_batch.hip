@@ -103,6 +103,16 @@ k_items(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first
 // (scoreMemory%getResult, scoreMemory_class.f90:537-573) evaluated on the CSUM / CSUM2 values that
 // closing the batches one by one gives, so the solve stops after the same batch as with one
 // batch per cycle; batches after the stop are not closed.
+// value of bin i of BIN copy w: a double, or the four limb words of a fixed-point score memory
+__device__ __forceinline__ double bin_value(const SlotDev& D, int w, int64_t i)
+{
+    const int64_t nb = D.P.n_bins;
+    if (!D.P.fix) return D.bin[(size_t)w * (size_t)nb + i];
+    const unsigned long long* b = (const unsigned long long*)D.bin + 4 * ((size_t)w * (size_t)nb + i);
+    const uint64_t wv[4] = {b[0], b[1], b[2], b[3]};
+    return scn_fix_value(wv);
+}
+
 __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, DevStatus* st)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,7 +120,6 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
     SlotState& S = ss[s];
     if (!S.running) return;
     const SlotDev& D = slots[s];
-    const int64_t nb = D.P.n_bins;
     const int n_run = S.n_run;
     int n_close = n_run;
     double cs = 0.0, cs2 = 0.0;
@@ -119,10 +128,9 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
         cs2 = D.csum2[D.stop_bin];
     }
     for (int w = 0; w < n_run; ++w) {
-        const double* bin = D.bin + (size_t)w * (size_t)nb;
         double f = 1.0;
         if (D.norm_bin >= 0) {
-            double v = bin[D.norm_bin];
+            double v = bin_value(D, w, D.norm_bin);
             if (v == 0.0) {
                 raise_error(st, SCN_ERR_NORM, s);
                 f = 0.0;
@@ -133,7 +141,7 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
         S.norm[w] = f;
         const int N = S.batch + w + 1;
         if (D.stop_bin >= 0 && N < D.n_batches && N >= 2) {
-            const double r = bin[D.stop_bin] * f;
+            const double r = bin_value(D, w, D.stop_bin) * f;
             cs = cs + r;
             cs2 = cs2 + r * r;
             double mean = cs / N;
@@ -147,7 +155,7 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
                 break;
             }
         } else if (D.stop_bin >= 0) {
-            const double r = bin[D.stop_bin] * f;
+            const double r = bin_value(D, w, D.stop_bin) * f;
             cs = cs + r;
             cs2 = cs2 + r * r;
         }
@@ -168,13 +176,17 @@ __global__ void k_bclose(const SlotDev* __restrict__ slots, const SlotState* __r
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double c = D.csum[i], c2 = D.csum2[i];
         for (int w = 0; w < n_run; ++w) {
-            double* b = D.bin + (size_t)w * (size_t)n + i;
             if (w < n_close) {
-                double r = *b * S.norm[w];
+                double r = bin_value(D, w, i) * S.norm[w];
                 c = c + r;
                 c2 = c2 + r * r;
             }
-            *b = 0.0;
+            if (D.P.fix) {
+                unsigned long long* b = (unsigned long long*)D.bin + 4 * ((size_t)w * (size_t)n + i);
+                b[0] = b[1] = b[2] = b[3] = 0ull;
+            } else {
+                D.bin[(size_t)w * (size_t)n + i] = 0.0;
+            }
         }
         D.csum[i] = c;
         D.csum2[i] = c2;
@@ -516,7 +528,7 @@ int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
         const int steps = last ? (1 << 30) : (B->S.k0 << L);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
         fn(dim3(grid), dim3(BBLOCK), lds_bytes(B), B->stream, B->d_slots, B->d_state, B->d_items, B->d_item_cnt,
-           B->d_ctl, first, in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave, B->stack);
+           B->d_ctl, first, in_side, out_side, tile, steps, B->d_status, B->S_lds_per_wave, B->stack, B->S.refill);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L + 1], B->stream));
         side = out_side;
     }
@@ -651,7 +663,7 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     if (!settings || settings->max_slots < 1 || settings->max_slots > MAX_SLOTS || settings->hbm_budget < (1 << 20) ||
         (settings->transport < SCN_TRANSPORT_ST || settings->transport > SCN_TRANSPORT_HT) ||
         (settings->tile != 0 && (settings->tile % BWAVE != 0 || settings->tile > 4096)) || settings->k0 < 0 ||
-        settings->n_launch < 0 || settings->n_launch > 16) {
+        settings->n_launch < 0 || settings->n_launch > 16 || settings->refill < 0 || settings->refill > BWAVE) {
         g_batch_create_err = "invalid batch settings";
         return SCN_ERR_ARG;
     }
@@ -660,6 +672,7 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     if (B->S.tile == 0) B->S.tile = 256;
     if (B->S.k0 == 0) B->S.k0 = 8;
     if (B->S.n_launch == 0) B->S.n_launch = 1;
+    if (B->S.refill == 0) B->S.refill = DEFAULT_REFILL;
     B->n_slots = settings->max_slots;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) B->n_cu = prop.multiProcessorCount;
@@ -773,6 +786,11 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     std::vector<int64_t> g_addr;
     const FastColl fc = fast_collision_layout(pr, goff, gidx, g_addr);
     size_t o_gaddr = I.put(g_addr.data(), g_addr.size());
+    const std::vector<double> uinv = universe_inverse_table(pr);
+    size_t o_uinv = I.put(uinv.data(), uinv.size());
+    std::vector<int> flat;
+    const int flat_lat = flat_lattice(pr, flat);
+    size_t o_flat = I.put(flat.data(), flat.size());
     size_t o_tgidx = I.put(tgidx.data(), tgidx.size());
     // pointSource%init normalisation (pointSource_class.f90:120-128,150-152)
     std::vector<double> probG;
@@ -797,7 +815,8 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     const size_t nb = (size_t)std::max<int64_t>(1, pr->n_bins);
     const int window = R->window > 0 ? R->window : 1;
     // BIN (one copy per batch of the window), CSUM, CSUM2
-    size_t bins_bytes = align256(sizeof(double) * nb * window) + 2 * align256(sizeof(double) * nb);
+    const size_t words = pr->fixed_point_tally ? 4 : 1;   // per bin of a BIN copy (fixed point: limb words)
+    size_t bins_bytes = align256(sizeof(double) * words * nb * window) + 2 * align256(sizeof(double) * nb);
     Ly.hist = Ly.bins + bins_bytes;
     size_t nh = R->record ? (size_t)R->n_batches * (size_t)R->pop : 0;
     size_t hist_bytes = R->record ? align256(sizeof(uint64_t) * nh) + align256(sizeof(int) * nh) : 0;
@@ -849,14 +868,18 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     P.gclerk_idx = (const int*)(base + o_gidx);
     P.fc = fc;
     P.fc.g_addr = (const int64_t*)(base + o_gaddr);
+    P.uni_inv = (const double*)(base + o_uinv);
+    P.flat_lat = flat_lat;
+    P.flat = (const int*)(base + o_flat);
+    P.fix = pr->fixed_point_tally ? 1 : 0;
     P.n_bins = pr->n_bins;
     P.ht_cutoff = pr->ht_cutoff;
     P.P1 = pr->mg.P1 ? (const double*)(base + o_P1) : nullptr;
     carve_live(base + Ly.queue, depth, R->pop, D.q[0]);
     carve_live(base + Ly.queue + Ly.queue_bytes, depth, R->pop, D.q[1]);
     D.bin = (double*)(base + Ly.bins);
-    D.csum = (double*)(base + Ly.bins + align256(sizeof(double) * nb * window));
-    D.csum2 = (double*)(base + Ly.bins + align256(sizeof(double) * nb * window) + align256(sizeof(double) * nb));
+    D.csum = (double*)(base + Ly.bins + align256(sizeof(double) * words * nb * window));
+    D.csum2 = (double*)(base + Ly.bins + align256(sizeof(double) * words * nb * window) + align256(sizeof(double) * nb));
     D.hist_rng = R->record ? (uint64_t*)(base + Ly.hist) : nullptr;
     D.hist_coll = R->record ? (int*)(base + Ly.hist + align256(sizeof(uint64_t) * nh)) : nullptr;
     D.probG = probG.empty() ? nullptr : (const double*)(base + o_probG);

@@ -25,6 +25,7 @@ constexpr int MAX_PROBG = 4096;
 constexpr int DEFAULT_BUFFER = 50;   // fixedSourcePhysicsPackage_class.f90:372 'buffer' default
 constexpr int MAX_BUFFER = 1024;
 constexpr int MAX_WINDOW = 16;        // batches of one solve in flight in one cycle (power of two)
+constexpr int DEFAULT_REFILL = 32;    // idle lanes a wave gathers before it starts new histories
 constexpr int WINDOW_SHIFT = 4;
 // a history still alive after this many (flight, collision) pairs in one launch is reported as lost
 // (an error, not a hang)
@@ -95,7 +96,7 @@ struct Item {
 };
 
 typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const Item*, int*, int*, int, int,
-                             int, int, int, DevStatus*, int, Sites);
+                             int, int, int, DevStatus*, int, Sites, int);
 
 // launcher and kernel address of one variant: register depth NL, tracking mode MODE, geometry
 // variant GEO (0: lattice/pin/box, 1: cells, rotations, CSG borders)
@@ -304,11 +305,22 @@ __device__ __forceinline__ void flush_counts(SlotState* SS, Cnt& cn, int lane)
 __device__ __forceinline__ void flush_lds_bins(Bins& B, int lane)
 {
     __builtin_amdgcn_wave_barrier();
-    for (int64_t i = lane; i < B.lds_n; i += BWAVE) {
-        const double v = B.lds[i];
-        if (v != 0.0) {
-            unsafeAtomicAdd(&B.glob[i], v);
-            B.lds[i] = 0.0;
+    if (B.fix) {
+        unsigned long long* lw = (unsigned long long*)B.lds;
+        for (int64_t i = lane; i < 4 * B.lds_n; i += BWAVE) {
+            const unsigned long long v = lw[i];
+            if (v) {
+                atomicAdd((unsigned long long*)B.glob + i, v);
+                lw[i] = 0ull;
+            }
+        }
+    } else {
+        for (int64_t i = lane; i < B.lds_n; i += BWAVE) {
+            const double v = B.lds[i];
+            if (v != 0.0) {
+                unsafeAtomicAdd(&B.glob[i], v);
+                B.lds[i] = 0.0;
+            }
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -325,7 +337,8 @@ __device__ __forceinline__ void flush_lds_bins(Bins& B, int lane)
 template <int NL, int MODE, int GEO>
 __global__ void __launch_bounds__(BBLOCK)
 k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __restrict__ items, int* item_cnt, int* ctl,
-             int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st, int lds_per_wave, Sites SB)
+             int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st, int lds_per_wave, Sites SB,
+             int refill)
 {
     extern __shared__ double lds_all[];
     const int lane = threadIdx.x & (BWAVE - 1);
@@ -368,6 +381,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
     B.lds_n = 0;
     B.c_addr = -1;
     B.c_acc = 0.0;
+    B.fix = 0;
     Cnt cn{0, 0, 0};
     Pt<NL, GEO> p;
     DistCache<NL> cache;   // surface tracking: the current flight's distance cache
@@ -423,13 +437,17 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
             n_in = SS->n_in;
             batch = SS->batch + w;
             cap = D->pop;
-            B.glob = D->bin + (size_t)w * (size_t)D->P.n_bins;
-            B.lds_n = min((int64_t)lds_per_wave, D->P.n_bins);
+            // fixed point: four limb words per bin, in BIN and in the wave's LDS copy
+            B.fix = D->P.fix;
+            B.glob = D->bin + (size_t)w * (size_t)D->P.n_bins * (B.fix ? 4 : 1);
+            B.lds_n = min((int64_t)(B.fix ? lds_per_wave / 4 : lds_per_wave), D->P.n_bins);
             SB.depth = D->buffer;
             next = pend_b;
             end = pend_e;
         }
-        if (next < end) {
+        // new histories start once `refill` lanes are idle (or none is alive): a start runs for
+        // many lanes at a time rather than once for every lane that frees up
+        if (next < end && (__popcll(~live) >= refill || live == 0)) {
             const unsigned long long idle = ~live;
             const int i = next + __popcll(idle & lt_mask);
             if (!alive && i < end) {
@@ -547,10 +565,10 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
 template <int NL, int MODE, int GEO>
 void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const Item* items,
                int* item_cnt, int* ctl, int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st,
-               int lds_per_wave, Sites SB)
+               int lds_per_wave, Sites SB, int refill)
 {
     hipLaunchKernelGGL((k_btransport<NL, MODE, GEO>), g, b, sh, s, slots, ss, items, item_cnt, ctl, first, in_side,
-                       out_side, chunk, max_steps, st, lds_per_wave, SB);
+                       out_side, chunk, max_steps, st, lds_per_wave, SB, refill);
 }
 
 template <int NL, int MODE, int GEO>

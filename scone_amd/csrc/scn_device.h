@@ -21,6 +21,12 @@
 #include "scone_gpu.h"
 #include "scn_fpmath.h"
 
+// place_coord takes the flat-lattice branch (place_flat) when the problem has one; a translation
+// unit may compile it out (the result is the same, by the general walk)
+#ifndef SCN_FLAT_PLACEMENT
+#define SCN_FLAT_PLACEMENT 1
+#endif
+
 namespace scn {
 
 // ---- RNG jumps --------------------------------------------------------------------------------
@@ -59,7 +65,21 @@ __device__ __forceinline__ uint64_t rng_skip(uint64_t s, int64_t k_in)
     return s;
 }
 
-__device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n) { return rng_skip(s, SCN_STRIDE * n); }
+static __constant__ scn_rng_stride_table RNG_STRIDES = scn_make_rng_stride_table();
+
+// stride by n histories: the five byte maps of RNG_STRIDES (scn_rng_stride_bytes), the same state
+// as rng_skip(s, 152917 n)
+__device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n)
+{
+    if (n < 0 || n >= ((int64_t)1 << 40)) return rng_skip(s, SCN_STRIDE * n);
+    const uint64_t u = (uint64_t)n;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int b = (int)((u >> (8 * k)) & 255ULL);
+        s = (RNG_STRIDES.a[k][b] * s + RNG_STRIDES.c[k][b]) & SCN_LCG_MASK;
+    }
+    return s;
+}
 
 // ---- collision clerks of one shared layout -----------------------------------------------------
 // When every collision clerk has the same maps, all of them lin-grid spaceMaps (at most 3), the
@@ -79,6 +99,7 @@ struct FastColl {
     int multi[3];
     double g0[3];        // grid minimum (real_data[data])
     double step[3];      // grid step (real_data[data + 1])
+    double step_inv[3];  // 1 / step (only steers scn_floor_div, which checks its answer)
     const int64_t* g_addr;   // [nG] mem_addr of the clerk admitting group g, -1 if none
 };
 
@@ -117,6 +138,11 @@ struct DevProblem {
     const int* gtrack_off;   // [nG+1] track clerks whose energy filter admits group g
     const int* gtrack_idx;
     FastColl fc;             // shared-layout collision clerks (fc.on = 0: walk the clerk table)
+    const double* uni_inv;   // [n_universes][6]: 1/pitch(3) and 1/size(0..1) of lattices (scn_floor_div, scn_idiv)
+    int fix;                 // 1: BIN holds four 64-bit limb words per bin (scn_fix), order-independent sums
+    int flat_lat;            // flat lattice (flat_lattice in scn_host.h): its universe index, 0 otherwise
+    const int* flat;         // [2 + 5 n_local]: root-outside (mat, uid), then per lattice cell (mat, uid,
+                             // nest, pin universe, pin graph root)
     const double* st_xs;     // [n_mat*nG + 1][2]: surface tracking's sigmaTrack and its reciprocal
 };
 
@@ -904,15 +930,36 @@ __device__ __forceinline__ void rot_apply(const double* R, double v[3])
 // ---------------------------------------------------------------------------------------------
 // universes
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void get_ijk(int localID, const int32_t sizeN[3], int ijk[3])
+// latUniverse get_ijk (latUniverse_class.f90:489-506), its integer divisions by scn_idiv
+__device__ __forceinline__ void get_ijk(int localID, const int32_t sizeN[3], const double* inv, int ijk[3])
 {
     int temp = localID - 1;
-    int base = temp / sizeN[0];
+    int base = scn_idiv(temp, inv[3]);
     ijk[0] = temp - sizeN[0] * base + 1;
     temp = base;
-    base = temp / sizeN[1];
+    base = scn_idiv(temp, inv[4]);
     ijk[1] = temp - sizeN[1] * base + 1;
     ijk[2] = base + 1;
+}
+
+// latUniverse findCell (latUniverse_class.f90:280-312)
+__device__ __forceinline__ int lat_findcell(const DevProblem& P, const scn_universe& U, int uniIdx, const double r[3],
+                                            const double u[3])
+{
+    const double* inv = P.uni_inv + 6 * (size_t)(uniIdx - 1);
+    int ijk[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ijk[i] = (int)scn_floor_div(r[i] - U.corner[i], U.pitch[i], inv[i]) + 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double r_bar = r[i] - U.corner[i] - (double)ijk[i] * U.pitch[i] + 0.5 * U.pitch[i];
+        if (fabs(r_bar) > U.a_bar[i] && r_bar * u[i] > 0.0) ijk[i] += (u[i] < 0.0) ? -1 : 1;
+    }
+    int out = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out |= (ijk[i] <= 0 || ijk[i] > U.size[i]);
+    if (out) return U.out_local;
+    return ijk[0] + U.size[0] * (ijk[1] - 1 + U.size[1] * (ijk[2] - 1));
 }
 
 template <int GEO>
@@ -920,15 +967,7 @@ __device__ __forceinline__ int uni_findcell(const DevProblem& P, int uniIdx, con
 {
     const scn_universe& U = P.unis[uniIdx - 1];
     if (U.type == SCN_UNI_LAT) {
-        int ijk[3];
-        for (int i = 0; i < 3; ++i) ijk[i] = (int)floor((r[i] - U.corner[i]) / U.pitch[i]) + 1;
-        for (int i = 0; i < 3; ++i) {
-            double r_bar = r[i] - U.corner[i] - (double)ijk[i] * U.pitch[i] + 0.5 * U.pitch[i];
-            if (fabs(r_bar) > U.a_bar[i] && r_bar * u[i] > 0.0) ijk[i] += (u[i] < 0.0) ? -1 : 1;
-        }
-        for (int i = 0; i < 3; ++i)
-            if (ijk[i] <= 0 || ijk[i] > U.size[i]) return U.out_local;
-        return ijk[0] + U.size[0] * (ijk[1] - 1 + U.size[1] * (ijk[2] - 1));
+        return lat_findcell(P, U, uniIdx, r, u);
     } else if (U.type == SCN_UNI_PIN) {
         double r_sq = r[0] * r[0] + r[1] * r[1];
         double mul = (r[0] * u[0] + r[1] * u[1] >= 0.0) ? -1.0 : 1.0;
@@ -980,7 +1019,7 @@ __device__ __forceinline__ double uni_distance(const DevProblem& P, int uniIdx, 
             return box_distance(outline, r, u);
         }
         int ijk[3];
-        get_ijk(localID, U.size, ijk);
+        get_ijk(localID, U.size, P.uni_inv + 6 * (size_t)(uniIdx - 1), ijk);
         double d = SCN_INF;
         int ax = 0;
         double uax = u[0];
@@ -1036,7 +1075,7 @@ __device__ __forceinline__ void uni_offset(const DevProblem& P, int uniIdx, int 
     int doOffset = (U.offset_mode == 2) ? (P.ints[U.offset_map + localID - 1] == 1) : (U.offset_mode == 1);
     if (doOffset && localID != U.out_local) {
         int ijk[3];
-        get_ijk(localID, U.size, ijk);
+        get_ijk(localID, U.size, P.uni_inv + 6 * (size_t)(uniIdx - 1), ijk);
         for (int i = 0; i < 3; ++i) off[i] = ((double)ijk[i] - 0.5) * U.pitch[i] + U.corner[i];
     }
 }
@@ -1096,9 +1135,60 @@ __device__ __forceinline__ void dive(const DevProblem& P, Pt<NL, GEO>& p, int st
     p.mat = SCN_UNDEF_MAT;
 }
 
+// placeCoord in a flat lattice (flat_lattice in scn_host.h): the root's halfspace test, the
+// lattice's findCell and, for a pin cell, the pin level's position, computed as place_coord and
+// diveToMat compute them (the root has no offset; a one-region pin's findCell is always 1); the
+// material, uniqueID and depth of each lattice cell come from the table built on the host.
+template <int NL, int GEO>
+__device__ __forceinline__ void place_flat(const DevProblem& P, Pt<NL, GEO>& p)
+{
+    const scn_universe& R = P.unis[P.root_uni - 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.r[0][k] = p.r[0][k] - R.origin[k];
+    const int l0 = box_halfspace(P.boxes[R.border], p.r[0], p.u) ? 2 : 1;
+    const int L = P.flat_lat;
+    const scn_universe& U = P.unis[L - 1];
+    double r1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r1[k] = (p.r[0][k] - 0.0) - U.origin[k];
+    // every level is written unconditionally (levels past the particle's depth are never read):
+    // outside the root the lattice cell found is simply not used
+    const int l1 = lat_findcell(P, U, L, r1, p.u);
+    const int* c = P.flat + 2 + 5 * (l1 - 1);
+    const int inside = l0 == 1;
+    p.mat = inside ? c[0] : P.flat[0];
+    p.uid = inside ? c[1] : P.flat[1];
+    p.nest = inside ? c[2] : 1;
+    constexpr int L1 = NL > 1 ? 1 : 0, L2 = NL > 2 ? 2 : 0;
+    p.uni[0] = P.root_uni;
+    p.root[0] = 1;
+    p.loc[0] = l0;
+    double off[3];
+    uni_offset(P, L, l1, off);
+    const int pin = c[3] > 0 ? c[3] : L;
+    const scn_universe& Uc = P.unis[pin - 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        p.r[L1][k] = r1[k];
+        if (NL > 2) p.r[L2][k] = (r1[k] - off[k]) - Uc.origin[k];
+    }
+    p.uni[L1] = L;
+    p.root[L1] = P.gid[0];
+    p.loc[L1] = l1;
+    if (NL > 2) {
+        p.uni[L2] = c[3];
+        p.root[L2] = c[4];
+        p.loc[L2] = 1;
+    }
+}
+
 template <int NL, int GEO>
 __device__ __forceinline__ void place_coord(const DevProblem& P, Pt<NL, GEO>& p, DevStatus* st)
 {
+    if (SCN_FLAT_PLACEMENT && !GEO && NL >= 3 && P.flat_lat) {
+        place_flat(P, p);
+        return;
+    }
     const scn_universe& R = P.unis[P.root_uni - 1];
     p.nest = 1;
     p.mat = SCN_UNDEF_MAT;
@@ -1351,13 +1441,25 @@ __device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, co
 // rest go to the global BIN array through a one-entry per-lane cache: consecutive scores of a
 // lane to the same bin are summed in a register and added with one atomic when the lane scores
 // another bin (bins_flush at the end).  Only the summation order of BIN changes.
+// With fix = 1 every score goes straight to its bin's four limb words (scn_fix_limbs, integer
+// atomics), in LDS or global memory; the cache is not used, and the sums are order-independent.
 struct Bins {
     double* lds;
     double* glob;
     int64_t lds_n;
     int64_t c_addr;    // bin held in the lane's cache, -1 none
     double c_acc;
+    int fix;
 };
+
+__device__ __forceinline__ void fix_add(unsigned long long* w, double v)
+{
+    uint64_t l[4];
+    scn_fix_limbs(scn_fix_of(v), l);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (l[k]) atomicAdd(w + k, (unsigned long long)l[k]);
+}
 
 __device__ __forceinline__ void bins_flush(Bins& B)
 {
@@ -1368,6 +1470,10 @@ __device__ __forceinline__ void bins_flush(Bins& B)
 
 __device__ __forceinline__ void bin_add(Bins& B, int64_t addr, double v)
 {
+    if (B.fix) {
+        fix_add((unsigned long long*)(addr < B.lds_n ? B.lds : B.glob) + 4 * addr, v);
+        return;
+    }
     if (addr < B.lds_n) {
         atomicAdd(&B.lds[addr], v);
     } else if (addr == B.c_addr) {
@@ -1400,7 +1506,7 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         int bin = 1;
         for (int k = 0; k < P.fc.n_maps; ++k) {
             const double v = get3(p.r[0], P.fc.axis[k]);
-            int idx = (int)floor((v - P.fc.g0[k]) / P.fc.step[k]) + 1;
+            int idx = (int)scn_floor_div(v - P.fc.g0[k], P.fc.step[k], P.fc.step_inv[k]) + 1;
             if (idx < 1 || idx >= P.fc.nbins[k] + 1) return 0;
             bin = bin + (idx - 1) * P.fc.multi[k];
         }

@@ -173,6 +173,37 @@ __global__ void k_close_bins(double* bins, double* csum, double* csum2, int64_t 
 
 __global__ void k_copy_scalar(const double* src, double* dst) { *dst = *src; }
 
+// the same with a fixed-point score memory: four limb words per bin (scn_fix_value), cleared
+// after the read
+__global__ void k_close_bins_fix(unsigned long long* words, double* csum, double* csum2, int64_t n,
+                                 const double* norm_score, double norm_value)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double nf = norm_score ? norm_value / *norm_score : 1.0;
+    unsigned long long* w = words + 4 * i;
+    const uint64_t wv[4] = {w[0], w[1], w[2], w[3]};
+    double res = scn_fix_value(wv) * nf;
+    w[0] = w[1] = w[2] = w[3] = 0ull;
+    csum[i] = csum[i] + res;
+    csum2[i] = csum2[i] + res * res;
+}
+
+__global__ void k_fix_scalar(const unsigned long long* w, double* dst)
+{
+    const uint64_t wv[4] = {w[0], w[1], w[2], w[3]};
+    *dst = scn_fix_value(wv);
+}
+
+__global__ void k_fix_values(const unsigned long long* words, int64_t n, double* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long* w = words + 4 * i;
+    const uint64_t wv[4] = {w[0], w[1], w[2], w[3]};
+    out[i] = scn_fix_value(wv);
+}
+
 // ---- normSize_Repr ---------------------------------------------------------------------------
 __global__ void k_site_rn(uint64_t master, int n, double* rn, uint64_t* rn_bits)
 {
@@ -663,7 +694,9 @@ struct Engine {
     scn_particle_state* bankA = nullptr;   // thisCycle
     scn_particle_state* bankB = nullptr;   // nextCycle
     double* hist_k = nullptr;
-    double* bins = nullptr;
+    double* bins = nullptr;                // BIN: n_bins doubles, or 4 limb words per bin (P.fix)
+    double* bins_out = nullptr;            // fixed point: BIN values read out for scn_eigen_cycle
+    int bin_words = 1;
     double* csum[2] = {nullptr, nullptr};
     double* csum2[2] = {nullptr, nullptr};
     double* norm_score = nullptr;
@@ -996,12 +1029,25 @@ int close_bins(Engine* E, int which)
 {
     if (E->n_bins == 0) return 0;
     const double* ns = nullptr;
+    const dim3 g((unsigned)((E->n_bins + 255) / 256));
+    if (E->P.fix) {
+        unsigned long long* w = (unsigned long long*)E->bins;
+        if (E->norm_bin >= 0) {
+            hipLaunchKernelGGL(k_fix_scalar, dim3(1), dim3(1), 0, E->stream, (const unsigned long long*)w + 4 * E->norm_bin,
+                               E->norm_score);
+            ns = E->norm_score;
+        }
+        hipLaunchKernelGGL(k_close_bins_fix, g, dim3(256), 0, E->stream, w, E->csum[which], E->csum2[which], E->n_bins,
+                           ns, E->norm_value);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
     if (E->norm_bin >= 0) {
         hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, E->stream, E->bins + E->norm_bin, E->norm_score);
         ns = E->norm_score;
     }
-    hipLaunchKernelGGL(k_close_bins, dim3((unsigned)((E->n_bins + 255) / 256)), dim3(256), 0, E->stream, E->bins,
-                       E->csum[which], E->csum2[which], E->n_bins, ns, E->norm_value);
+    hipLaunchKernelGGL(k_close_bins, g, dim3(256), 0, E->stream, E->bins, E->csum[which], E->csum2[which], E->n_bins,
+                       ns, E->norm_value);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1470,9 +1516,19 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     const FastColl fc = fast_collision_layout(pr, goff, gidx, g_addr);
     int64_t* d_gaddr = nullptr;
     if (!rc) rc |= upload(E, &d_gaddr, g_addr.data(), g_addr.size());
+    const std::vector<double> uinv = universe_inverse_table(pr);
+    double* d_uinv = nullptr;
+    if (!rc) rc |= upload(E, &d_uinv, uinv.data(), uinv.size());
+    std::vector<int> flat;
+    const int flat_lat = flat_lattice(pr, flat);
+    int* d_flat = nullptr;
+    if (!rc) rc |= upload(E, &d_flat, flat.data(), flat.size());
     DevProblem& P = E->P;
     P.fc = fc;
     P.fc.g_addr = d_gaddr;
+    P.uni_inv = d_uinv;
+    P.flat_lat = flat_lat;
+    P.flat = d_flat;
     P.transport = pr->transport;
     P.st_cache = pr->st_cache;
     P.root_uni = pr->root_uni;
@@ -1538,7 +1594,10 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= dalloc(E, &E->bankA, E->bank_cap);
     if (!rc) rc |= dalloc(E, &E->bankB, E->bank_cap);
     if (!rc) rc |= dalloc(E, &E->hist_k, (size_t)4 * cap);
-    if (!rc) rc |= dalloc(E, &E->bins, std::max<int64_t>(1, E->n_bins));
+    P.fix = pr->fixed_point_tally ? 1 : 0;
+    E->bin_words = P.fix ? 4 : 1;   // doubles, or four limb words per bin
+    if (!rc) rc |= dalloc(E, &E->bins, (size_t)E->bin_words * std::max<int64_t>(1, E->n_bins));
+    if (!rc && P.fix) rc |= dalloc(E, &E->bins_out, std::max<int64_t>(1, E->n_bins));
     for (int w = 0; w < 2 && !rc; ++w) {
         rc |= dalloc(E, &E->csum[w], std::max<int64_t>(1, E->n_bins));
         rc |= dalloc(E, &E->csum2[w], std::max<int64_t>(1, E->n_bins));
@@ -1558,13 +1617,13 @@ int scn_create(const scn_problem* pr, int32_t capacity, void** out)
     if (!rc) rc |= dalloc(E, &E->end_of, (size_t)cap + 2);
     if (!rc) rc |= ensure_sum(E, (int64_t)E->bank_cap);
     if (!rc) {
-        if (hipMemset(E->bins, 0, sizeof(double) * std::max<int64_t>(1, E->n_bins)) != hipSuccess ||
+        if (hipMemset(E->bins, 0, sizeof(double) * E->bin_words * std::max<int64_t>(1, E->n_bins)) != hipSuccess ||
             hipMemset(E->status, 0, sizeof(DevStatus)) != hipSuccess ||
             hipMemset(E->counters, 0, sizeof(Counters)) != hipSuccess)
             rc = SCN_ERR_HIP;
     }
-    E->use_lds = (E->n_bins > 0 && E->n_bins <= LDS_BINS_MAX) ? 1 : 0;
-    E->lds_bytes = E->use_lds ? (int)(E->n_bins * sizeof(double)) : 0;
+    E->use_lds = (E->n_bins > 0 && E->n_bins * E->bin_words <= LDS_BINS_MAX) ? 1 : 0;
+    E->lds_bytes = E->use_lds ? (int)(E->n_bins * E->bin_words * sizeof(double)) : 0;
     if (rc) {
         g_create_err = E->err.empty() ? "engine creation failed" : E->err;
         scn_destroy(E);
@@ -1621,7 +1680,7 @@ int scn_eigen_cycle(void* engine, const scn_particle_state* bank_in, int32_t n_i
         return SCN_ERR_OVERFLOW;
     }
     COPY_SYNC(E->bankA, bank_in, sizeof(scn_particle_state) * n_in, hipMemcpyHostToDevice);
-    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->n_bins, E->stream));
+    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->bin_words * E->n_bins, E->stream));
     int ns = 0;
     int rc = transport_cycle(E, n_in, (uint64_t)rng_state, k_eff, &ns, bin_scores != nullptr);
     if (rc) return rc;
@@ -1637,7 +1696,14 @@ int scn_eigen_cycle(void* engine, const scn_particle_state* bank_in, int32_t n_i
         if (rc) return rc;
         for (int k = 0; k < 4; ++k) keff_scores[k] = sums[2 + k];
     }
-    if (bin_scores && E->n_bins) COPY_SYNC(bin_scores, E->bins, sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+    if (bin_scores && E->n_bins && E->P.fix) {
+        hipLaunchKernelGGL(k_fix_values, dim3((unsigned)((E->n_bins + 255) / 256)), dim3(256), 0, E->stream,
+                           (const unsigned long long*)E->bins, E->n_bins, E->bins_out);
+        HIPCHK(hipGetLastError());
+        COPY_SYNC(bin_scores, E->bins_out, sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+    } else if (bin_scores && E->n_bins) {
+        COPY_SYNC(bin_scores, E->bins, sizeof(double) * E->n_bins, hipMemcpyDeviceToHost);
+    }
     return 0;
 }
 
@@ -1716,7 +1782,7 @@ int scn_eigen_start(void* engine, const scn_eigen_settings* S, const scn_particl
         HIPCHK(hipMemsetAsync(E->csum[w], 0, sizeof(double) * E->n_bins, E->stream));
         HIPCHK(hipMemsetAsync(E->csum2[w], 0, sizeof(double) * E->n_bins, E->stream));
     }
-    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->n_bins, E->stream));
+    if (E->n_bins) HIPCHK(hipMemsetAsync(E->bins, 0, sizeof(double) * E->bin_words * E->n_bins, E->stream));
     // eigenPhysicsPackage%run: stride(getOffset) = 0 on one rank; generateInitialState: stride(totalPop)
     X.master = scn_rng_stride((uint64_t)S->seed, 0);
     X.master = scn_rng_stride(X.master, pop);

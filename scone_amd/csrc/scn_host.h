@@ -213,8 +213,70 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
         F.multi[k] = first->multi[k];
         F.g0[k] = pr->real_data[first->maps[k].data];
         F.step[k] = pr->real_data[first->maps[k].data + 1];
+        F.step_inv[k] = 1.0 / F.step[k];
     }
     return F;
+}
+
+// A flat lattice: a box-bordered root filled by one lattice each of whose cells holds a material or
+// a one-region pinUniverse (slab, voxel and mesh-like geometries), nothing rotated or 'global'.
+// Delta tracking then places a particle with the root's halfspace test and one lattice findCell
+// (place_flat on the device), the cell's material, uniqueID and depth coming from `flat`:
+// [0..1] the root's outside cell (mat, uid), then (mat, uid, nest, pin universe, pin graph root)
+// for lattice local IDs 1..n_local, as diveToMat walks the geomGraph to them (pin 0 for a
+// material cell).  Returns the lattice's universe index, or 0.
+inline int flat_lattice(const scn_problem* pr, std::vector<int>& flat)
+{
+    flat.assign(2, 0);
+    if (pr->root_uni < 1 || pr->root_uni > pr->n_universes || pr->graph_size < 2) return 0;
+    const scn_universe& R = pr->universes[pr->root_uni - 1];
+    if (R.type != SCN_UNI_ROOT || R.border_surf >= 0 || R.rotated || R.global) return 0;
+    const int f_in = pr->graph_fill[0], f_out = pr->graph_fill[1];   // root local cells 1 and 2
+    if (f_out < 0 || f_in >= 0) return 0;
+    flat[0] = f_out;
+    flat[1] = pr->graph_id[1];
+    const int L = -f_in;
+    if (L < 1 || L > pr->n_universes) return 0;
+    const scn_universe& U = pr->universes[L - 1];
+    if (U.type != SCN_UNI_LAT || U.rotated || U.global) return 0;
+    const int R1 = pr->graph_id[0];
+    for (int l = 1; l <= U.n_local; ++l) {
+        const int pos = R1 + l - 1;
+        if (pos < 1 || pos > pr->graph_size) return 0;
+        int f = pr->graph_fill[pos - 1], id = pr->graph_id[pos - 1], nest = 2, pin = 0, pin_root = 0;
+        if (f < 0) {
+            const int C = -f;
+            if (C < 1 || C > pr->n_universes) return 0;
+            const scn_universe& Q = pr->universes[C - 1];
+            if (Q.type != SCN_UNI_PIN || Q.n_annuli != 1 || Q.rotated || Q.global) return 0;
+            if (id < 1 || id > pr->graph_size) return 0;   // the pin's only cell: graph position id
+            pin = C;
+            pin_root = id;
+            f = pr->graph_fill[id - 1];
+            id = pr->graph_id[id - 1];
+            if (f < 0) return 0;
+            nest = 3;
+        }
+        flat.push_back(f);
+        flat.push_back(id);
+        flat.push_back(nest);
+        flat.push_back(pin);
+        flat.push_back(pin_root);
+    }
+    return L;
+}
+
+// reciprocals of lattice pitches and sizes, 6 per universe (scn_floor_div and scn_idiv on the device)
+inline std::vector<double> universe_inverse_table(const scn_problem* pr)
+{
+    std::vector<double> t(6 * (size_t)std::max(1, pr->n_universes), 0.0);
+    for (int u = 0; u < pr->n_universes; ++u) {
+        const scn_universe& U = pr->universes[u];
+        if (U.type != SCN_UNI_LAT) continue;
+        for (int i = 0; i < 3; ++i) t[6 * (size_t)u + i] = 1.0 / U.pitch[i];
+        for (int i = 0; i < 2; ++i) t[6 * (size_t)u + 3 + i] = 1.0 / (double)std::max(1, U.size[i]);
+    }
+    return t;
 }
 
 // delta tracking's per-group majorant (baseMgNeutronDatabase getMajorantXS, floored by the

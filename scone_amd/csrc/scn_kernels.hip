@@ -15,6 +15,9 @@
 #endif
 
 #if SCN_KENGINE == 0
+// the eigenvalue kernels keep the general placement: the flat-lattice branch costs them scratch
+// (an index array leaves registers) for geometries they rarely see; both give the same bits
+#define SCN_FLAT_PLACEMENT 0
 #define SCN_TRANSPORT_KERNELS 1
 #include "scn_transport.h"
 #define SCN_INST(NL) template TransportFn variant_fn<NL, SCN_KMODE, SCN_KGEO>(const void** kern);

@@ -158,9 +158,9 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
         master = *cin.master;
         k_eff = *cin.k_eff;
     }
-    if (use_lds) {
-        for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) lds_bins[i] = 0.0;
-    }
+    // LDS copy of the whole score memory: doubles, or four limb words per bin (fixed point)
+    const int lds_words = use_lds ? (int)P.n_bins * (P.fix ? 4 : 1) : 0;
+    for (int i = threadIdx.x; i < lds_words; i += BLOCK) lds_bins[i] = 0.0;
     __syncthreads();
     Bins B;
     B.lds = use_lds ? lds_bins : nullptr;
@@ -168,6 +168,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     B.lds_n = use_lds ? P.n_bins : 0;
     B.c_addr = -1;
     B.c_acc = 0.0;
+    B.fix = P.fix;
     Cnt cn{0, 0, 0};
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -298,7 +299,13 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
         if (sc) atomicAdd(&cnt->scores, sc);
     }
     __syncthreads();
-    if (use_lds) {
+    if (use_lds && P.fix) {
+        const unsigned long long* lw = (const unsigned long long*)lds_bins;
+        for (int i = threadIdx.x; i < lds_words; i += BLOCK) {
+            const unsigned long long v = lw[i];
+            if (v) atomicAdd((unsigned long long*)bins + i, v);
+        }
+    } else if (use_lds) {
         for (int i = threadIdx.x; i < (int)P.n_bins; i += BLOCK) {
             double v = lds_bins[i];
             if (v != 0.0) unsafeAtomicAdd(&bins[i], v);

@@ -234,7 +234,7 @@ class Batch:
     """Fixed-source engine with resident problem slots (scn_batch_* of include/scone_gpu.h)."""
 
     def __init__(self, transport=T.TRANSPORT_DT, max_slots=512, hbm_budget=8 << 30, tile=0, k0=0, n_launch=0,
-                 use_graph=1):
+                 use_graph=1, refill=0):
         L = load()
         S = T.BatchSettings()
         S.transport = transport
@@ -244,6 +244,7 @@ class Batch:
         S.k0 = k0
         S.n_launch = n_launch
         S.use_graph = use_graph
+        S.refill = refill
         self.settings = S
         h = C.c_void_p()
         rc = L.scn_batch_create(C.byref(S), C.byref(h))

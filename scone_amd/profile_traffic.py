@@ -6,8 +6,8 @@ Run under rocprofv3 twice, once per counter (the two cannot share a pass with th
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir_write> -- python3 -m scone_amd.profile_traffic run ...
     python3 -m scone_amd.profile_traffic summarize <dir_fetch> <dir_write> --key <workload key>
 
-`run` first copies a 1 GiB device buffer with a plain device-to-device copy (the HIP runtime's
-copy kernel), so the counters can be scaled: factor = counted KiB of that dispatch / its true KiB.  Then it runs the
+`run` first copies a 1 GiB device buffer with a plain device-to-device copy (hipMemcpy through
+the HIP runtime the engine links, whose copy kernel is a separate dispatch), so the counters can be scaled: factor = counted KiB of that dispatch / its true KiB.  Then it runs the
 workload without hipGraph replay, so every transport launch is a separately counted dispatch.
 `summarize` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, divides them by
 the copy's factor and writes bytes per launch of the transport kernel into
@@ -26,15 +26,32 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 COPY_MIB = 1024
 
 
-def run(args):
-    import torch
-    n = COPY_MIB * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
+def _calibration_copy():
+    """One device-to-device copy of COPY_MIB MiB through the HIP runtime the engine links (its
+    copy kernel is the dispatch the counters are scaled by)."""
+    import ctypes as C
+    from scone_amd.engine import load
+    load()
+    H = C.CDLL("libamdhip64.so.7")
+    H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    H.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    H.hipFree.argtypes = [C.c_void_p]
+    nbytes = COPY_MIB << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if H.hipMalloc(C.byref(a), nbytes) or H.hipMalloc(C.byref(b), nbytes):
+        raise SystemExit("hipMalloc for the calibration copy failed")
+    H.hipMemset(a, 1, nbytes)
+    if H.hipMemcpy(b, a, nbytes, 3) or H.hipDeviceSynchronize():   # hipMemcpyDeviceToDevice
+        raise SystemExit("calibration copy failed")
+    H.hipFree(a)
+    H.hipFree(b)
     print(f"copy calibration: {COPY_MIB} MiB read + {COPY_MIB} MiB written", flush=True)
+
+
+def run(args):
     sys.path.insert(0, ROOT)
+    _calibration_copy()
     if args.workload == "batch512":
         import bench
         from scone_amd import tables as T
@@ -43,7 +60,7 @@ def run(args):
         problems = bench.batch_problems(512, args.transport, 1.0)
         budget = bench.budget_for(problems, 2, window=args.window)
         B = Batch(transport=transport, max_slots=512, hbm_budget=budget, use_graph=0, tile=args.tile, k0=args.k0,
-                  n_launch=args.n_launch)
+                  n_launch=args.n_launch, refill=args.refill)
         for (c, s, pop, _, seed) in problems:
             B.add(c, s, pop, args.cycles * args.window, seed, window=args.window)
         B.run_all()
@@ -133,6 +150,7 @@ def main():
     r.add_argument("--k0", type=int, default=0)
     r.add_argument("--n-launch", type=int, default=0)
     r.add_argument("--window", type=int, default=1, help="batch512: batches of every problem per cycle")
+    r.add_argument("--refill", type=int, default=0, help="batch512: idle lanes gathered before new histories start")
     s = sub.add_parser("summarize")
     s.add_argument("fetch_dir")
     s.add_argument("write_dir")

@@ -113,7 +113,7 @@ class Problem(C.Structure):
                 ("clerks", C.POINTER(Clerk)), ("n_bins", C.c_int64), ("norm_bin", C.c_int64),
                 ("norm_value", C.c_double), ("ht_cutoff", C.c_double), ("n_surfaces", C.c_int32),
                 ("surfaces", C.POINTER(Surface)), ("collision", C.c_int32), ("max_split", C.c_int32),
-                ("ww_n", C.c_int32), ("pad1", C.c_int32), ("ww_map", C.POINTER(Clerk)),
+                ("ww_n", C.c_int32), ("fixed_point_tally", C.c_int32), ("ww_map", C.POINTER(Clerk)),
                 ("ww_lower", C.POINTER(C.c_double)), ("ww_upper", C.POINTER(C.c_double)),
                 ("ww_survival", C.c_double)]
 
@@ -159,7 +159,8 @@ class FixedResult(C.Structure):
 
 class BatchSettings(C.Structure):
     _fields_ = [("transport", C.c_int32), ("max_slots", C.c_int32), ("hbm_budget", C.c_int64),
-                ("tile", C.c_int32), ("k0", C.c_int32), ("n_launch", C.c_int32), ("use_graph", C.c_int32)]
+                ("tile", C.c_int32), ("k0", C.c_int32), ("n_launch", C.c_int32), ("use_graph", C.c_int32),
+                ("refill", C.c_int32), ("pad0", C.c_int32)]
 
 
 def _ptr(arr, ctype):
@@ -1235,6 +1236,9 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
     P.n_bins = addr
     P.norm_bin = norm_bin
     P.norm_value = float(tal.get("normVal", 1.0))
+    # extension of the engine (no SCONE keyword): `fixedPointTally 1;` in the tally dictionary or
+    # at the top of the input makes BIN sums order-independent (scone_gpu.h fixed_point_tally)
+    P.fixed_point_tally = int(_bool(tal.get("fixedPointTally", inp.get("fixedPointTally", 0)), "fixedPointTally"))
     out.xs = xs
     out.majorant = majorant
     out.mats = mats

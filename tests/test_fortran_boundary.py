@@ -32,7 +32,7 @@ LAYOUT = {
     "scn_problem": ["transport", "st_cache", "n_universes", "root_uni", "universes", "n_boxes", "border_box", "boxes",
                     "graph_size", "graph_fill", "graph_id", "n_real", "real_data", "n_int", "int_data", "mg",
                     "n_clerks", "clerks", "n_bins", "norm_bin", "norm_value", "ht_cutoff", "n_surfaces", "surfaces",
-                    "collision", "max_split", "ww_n", "pad1", "ww_map", "ww_lower", "ww_upper", "ww_survival"],
+                    "collision", "max_split", "ww_n", "fixed_point_tally", "ww_map", "ww_lower", "ww_upper", "ww_survival"],
     "scn_eigen_settings": ["pop", "n_inactive", "n_active", "inactive_tally", "seed", "keff0", "events_per_launch",
                            "block_size", "cycle_graph", "pad0"],
     "scn_eigen_result": ["k_analog", "k_implicit", "imp", "k_history", "cycle_time", "bins_active", "bins_inactive",
@@ -42,7 +42,8 @@ LAYOUT = {
     "scn_fixed_request": ["problem", "source", "pop", "n_batches", "seed", "stop_bin", "stop_rel", "record", "buffer",
                           "window", "pad0"],
     "scn_fixed_result": ["state", "n_batches", "n_bins", "n_histories", "n_collisions", "n_segments"],
-    "scn_batch_settings": ["transport", "max_slots", "hbm_budget", "tile", "k0", "n_launch", "use_graph"],
+    "scn_batch_settings": ["transport", "max_slots", "hbm_budget", "tile", "k0", "n_launch", "use_graph", "refill",
+                           "pad0"],
     "scn_stats": ["transport_kernel_ms", "transport_launches", "n_histories", "n_segments", "n_collisions", "n_sites",
                   "n_scores", "n_cycles", "cycle_ms"],
 }
