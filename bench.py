@@ -454,7 +454,8 @@ def batch_line(args):
     log(f"batch: transport kernel {kt['kernel_ms_per_launch']:.3f} ms/launch, {kt['achieved_gbs']:.1f} GB/s")
     pla, res_p = (None, {})
     if not args.no_plain:
-        pla, res_p = run_plain(problems, transport, best, warmup, args.steps)
+        # one problem at a time has far fewer histories in flight: the tile that suits it is smaller
+        pla, res_p = run_plain(problems, transport, dict(best, tile=128), warmup, args.steps)
         log(f"batch: plain {pla['hist_per_s']:.3e} histories/s over {pla['wall_s']:.1f} s")
     identical = None
     if res_p:

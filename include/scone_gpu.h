@@ -280,7 +280,8 @@ typedef struct scn_eigen_settings {
     int32_t cycle_graph;       /* 1: every size and scalar of a cycle stays on the device and the cycles
                                   are replayed from a captured hipGraph (one graph per phase); the
                                   same histories, banks and k history as 0.  Needs events_per_launch 0. */
-    int32_t pad0;
+    int32_t refill;            /* idle lanes a wave gathers before it starts new histories (1..64; 0: default);
+                                  the histories do not change */
 } scn_eigen_settings;
 
 typedef struct scn_eigen_result {

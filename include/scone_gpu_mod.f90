@@ -187,7 +187,7 @@ module scone_gpu_mod
     integer(c_int32_t) :: events_per_launch
     integer(c_int32_t) :: block_size
     integer(c_int32_t) :: cycle_graph
-    integer(c_int32_t) :: pad0
+    integer(c_int32_t) :: refill
   end type scn_eigen_settings
 
   type, public, bind(C) :: scn_eigen_result

@@ -744,6 +744,7 @@ struct Engine {
     int k0 = 0;             // 0: one launch per cycle runs every history to completion
     int tail_frac = 16;
     int chunk = 256;        // histories a wave takes from the input at once
+    int refill = DEFAULT_EIGEN_REFILL;   // idle lanes a wave gathers before it starts new histories
 };
 
 #define HIPCHK(expr)                                                                     \
@@ -961,7 +962,7 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
         HIPCHK(hipEventRecord(E->ev0, E->stream));
         fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, first, n_in, E->live[cur], nin_ptr, E->live[1 - cur],
            nout_ptr, E->cap, steps, master, k_eff, E->sites, bins_arg, use_lds, E->hist_k, E->cap, E->counters,
-           E->status, E->counters_i + 2, E->chunk, CycleIn{nullptr, nullptr, nullptr});
+           E->status, E->counters_i + 2, E->chunk, CycleIn{nullptr, nullptr, nullptr, E->refill});
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(E->ev1, E->stream));
         // survivors, status and site count come back together, behind one synchronisation
@@ -1229,7 +1230,7 @@ int enqueue_dcycle(Engine* E, int phase, int score_bins)
     hipLaunchKernelGGL(k_dcycle_begin, dim3(1), dim3(64), 0, E->stream, E->counters_i, E->sites.count);
     fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, 1, pop, E->live[0], E->counters_i, E->live[1],
        E->counters_i + 1, E->cap, 1 << 30, 0, 0.0, E->sites, E->bins, use_lds, E->hist_k, E->cap, E->counters,
-       E->status, E->counters_i + 2, E->chunk, CycleIn{&c->master, &c->k_new, E->hist_nsite});
+       E->status, E->counters_i + 2, E->chunk, CycleIn{&c->master, &c->k_new, E->hist_nsite, E->refill});
     // next bank in (broodID, emission) order
     scan_dev(E, E->hist_nsite, E->site_off, &c->pop, E->cap);
     const int sg = (E->site_cap + 255) / 256;
@@ -1777,6 +1778,11 @@ int scn_eigen_start(void* engine, const scn_eigen_settings* S, const scn_particl
     X = EigenState{};
     X.S = *S;
     if (S->events_per_launch > 0) E->k0 = S->events_per_launch;
+    if (S->refill < 0 || S->refill > WAVE) {
+        E->err = "refill must be 0 (default) or 1..64";
+        return SCN_ERR_ARG;
+    }
+    E->refill = S->refill > 0 ? S->refill : DEFAULT_EIGEN_REFILL;
     COPY_SYNC(E->bankA, initial_bank, sizeof(scn_particle_state) * pop, hipMemcpyHostToDevice);
     for (int w = 0; w < 2 && E->n_bins; ++w) {
         HIPCHK(hipMemsetAsync(E->csum[w], 0, sizeof(double) * E->n_bins, E->stream));

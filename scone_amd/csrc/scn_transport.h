@@ -20,6 +20,7 @@ using namespace scn;
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_BINS_MAX = 8192;   // collision-clerk memories up to 64 KiB are privatised in LDS
+constexpr int DEFAULT_EIGEN_REFILL = 1;    // idle lanes a wave gathers before it starts new histories
 
 // live particle queue, structure of arrays; level-major position arrays
 struct Live {
@@ -52,6 +53,7 @@ struct CycleIn {
     const uint64_t* master;   // null: use the by-value master and k_eff
     const double* k_eff;
     int* hist_nsite;          // [hist_cap]: fission sites each history emitted (null: not recorded)
+    int refill;               // idle lanes a wave gathers before it starts new histories (1: at once)
 };
 
 typedef void (*TransportFn)(dim3, dim3, size_t, hipStream_t, const DevProblem&, const scn_particle_state*, int, int,
@@ -193,7 +195,8 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
             }
         }
         const unsigned long long live = __ballot(alive);
-        if (next < end) {
+        // histories start once cin.refill lanes are idle (or none is alive), many lanes at a time
+        if (next < end && (__popcll(~live) >= cin.refill || live == 0)) {
             const unsigned long long idle = ~live;
             const int i = next + __popcll(idle & lt_mask);
             if (!alive && i < end) {

@@ -175,6 +175,7 @@ class Engine:
         S.events_per_launch = settings.get("events_per_launch", 0)
         S.block_size = settings.get("block_size", 0)
         S.cycle_graph = settings.get("cycle_graph", 0)
+        S.refill = settings.get("refill", 0)
         return S
 
     def _result(self, S):
