@@ -261,41 +261,55 @@ def cpu_batch_baseline(problems, n_batches):
 # ---------------------------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------------------------
-def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7, window=10):
+def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7, window=10, adapt=50):
+    """Continuous batching of 17x17-lattice fixed-source requests (1e3-1e6 histories in 10 batches,
+    early stop at 1 % relative sd of the total flux).  Capacity: 256 requests admitted at once, with
+    fixed windows of `window` batches and with adaptive windows (`adapt`); latency at offered loads
+    of the adaptive capacity, served with adaptive windows.  The HBM budget holds the per-lane
+    buffers of fission secondaries (the requests' 'buffer' of 1000 entries) next to the slots."""
     from scone_amd import configs, tables as T
-    from scone_amd.engine import Batch
+    from scone_amd.engine import Batch, secondary_bytes
     from scone_amd.serve import Request, Server, latency_summary
     inp, mg, _, _ = configs.lattice_request(0, transport=transport)
     comp = T.compile_problem(inp, mg)
     src = T.compile_source(inp["source"], comp)
 
-    def request(i, arrival):
+    def request(i, arrival, ad):
         _, _, pop, cycles = configs.lattice_request(i, transport=transport)
         return Request(comp, src, pop, cycles, 1000 + i, stop_bin=0, stop_rel=stop_rel, arrival=arrival,
-                       buffer=inp["buffer"], window=min(window, cycles))
+                       buffer=inp["buffer"], window=min(window, cycles), adapt=ad)
 
-    slots, budget = 256, 16 << 30
+    stack = secondary_bytes(comp.problem.transport, inp["buffer"])
+    slots, budget = 256, (8 << 30) + stack
+    out = {"window": window, "adapt": adapt, "secondary_buffer_bytes": stack, "hbm_budget": budget, "loads": {}}
     # capacity: a saturated pool of requests, all admitted at once
-    B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
-    reqs = [request(100000 + i, 0.0) for i in range(slots)]
-    t = time.perf_counter()
-    Server(B).run_wall(reqs)
-    dt = time.perf_counter() - t
-    hist = sum(r.result.n_histories for r in reqs)
-    cap_hist = hist / dt
-    mean_req = hist / len(reqs)
-    cap_req = cap_hist / mean_req
-    B.close()
-    log(f"serving capacity {cap_hist:.3e} histories/s, {cap_req:.1f} requests/s (mean {mean_req:.0f} histories)")
-    out = {"capacity_hist_per_s": cap_hist, "capacity_req_per_s": cap_req, "mean_request_histories": mean_req,
-           "window": window, "loads": {}}
+    for ad in (0, adapt):
+        B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
+        reqs = [request(100000 + i, 0.0, ad) for i in range(slots)]
+        srv = Server(B)
+        t = time.perf_counter()
+        srv.run_wall(reqs)
+        dt = time.perf_counter() - t
+        hist = sum(r.result.n_histories for r in reqs)
+        cap_hist = hist / dt
+        mean_req = hist / len(reqs)
+        cap_req = cap_hist / mean_req
+        B.close()
+        key = "adaptive" if ad else "fixed_window"
+        out[key] = {"capacity_hist_per_s": cap_hist, "capacity_req_per_s": cap_req, "cycles": srv.cycles,
+                    "histories": hist, "elapsed_s": dt}
+        log(f"serving capacity ({key}) {cap_hist:.3e} histories/s, {cap_req:.1f} requests/s (mean {mean_req:.0f} "
+            f"histories), {srv.cycles} cycles")
+    out.update(capacity_hist_per_s=out["adaptive"]["capacity_hist_per_s"],
+               capacity_req_per_s=out["adaptive"]["capacity_req_per_s"], mean_request_histories=mean_req)
+    cap_req = out["capacity_req_per_s"]
     rng = np.random.default_rng(seed)
     k = 0
     for load in loads:
         lam = load * cap_req
         n = max(1, int(lam * seconds_per_load))
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
-        reqs = [request(k + i, float(a)) for i, a in enumerate(arr)]
+        reqs = [request(k + i, float(a), adapt) for i, a in enumerate(arr)]
         k += n
         B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
         srv = Server(B)

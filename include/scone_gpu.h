@@ -342,7 +342,12 @@ typedef struct scn_fixed_request {
                                 at most 16; more than 1 needs an engine with n_launch 1).  They are
                                 closed in order, so results and the early-stop point are those of
                                 one batch per cycle; batches past an early stop are discarded. */
-    int32_t pad0;
+    int32_t adapt;           /* 0: every cycle transports `window` batches.  1-100 (with a stop_bin): the
+                                first cycle transports 2, each later one adapt% of the batches the stop
+                                bin's trend still asks for, adapt/100 (N (rel / stop_rel)^2 - N) with N
+                                batches closed (at least 1, at most `window`), so that few batches are
+                                transported past the stop; results and the stop batch do not change.
+                                (No counterpart in the reference.) */
 } scn_fixed_request;
 
 enum { SCN_SLOT_FREE = 0, SCN_SLOT_RUNNING = 1, SCN_SLOT_DONE = 2 };
@@ -452,6 +457,18 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* result, double*
 int scn_batch_states(void* batch, int32_t* states, int32_t n);
 /* free a DONE slot (its memory returns to the budget) */
 int scn_batch_release(void* batch, int32_t slot);
+/* batches of a running solve transported in each of the next cycles, 1 .. the window it was admitted
+ * with.  The batches are closed in order as before, so the results and the early-stop batch do not
+ * change; a serving loop narrows the window of a solve that is close to its stop target so that
+ * batches past the stop are not transported.  (No counterpart in the reference.) */
+int scn_batch_set_window(void* batch, int32_t slot, int32_t window);
+/* HBM-budget bytes that the per-lane buffers of secondaries take once a solve with fission or weight
+ * windows is resident (`buffer` entries per lane, 0 = the default 50; one block shared by every such
+ * solve, sized by the deepest buffer); -1 without a device.  (No counterpart in the reference.) */
+int64_t scn_batch_secondary_bytes(int32_t transport, int32_t buffer);
+/* scoreMemory%getResult of one bin (mean, sd) over the batches closed so far, for a running or done
+ * solve; *n_batches receives that count (result 0, 0 before the first batch closes) */
+int scn_batch_bin(void* batch, int32_t slot, int64_t bin, double result[2], int32_t* n_batches);
 /* device time and counters of the batch engine */
 int scn_batch_stats(void* batch, scn_stats* out);
 

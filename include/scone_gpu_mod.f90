@@ -232,7 +232,7 @@ module scone_gpu_mod
     integer(c_int32_t) :: record
     integer(c_int32_t) :: buffer
     integer(c_int32_t) :: window
-    integer(c_int32_t) :: pad0
+    integer(c_int32_t) :: adapt
   end type scn_fixed_request
 
   type, public, bind(C) :: scn_fixed_result
@@ -420,6 +420,28 @@ module scone_gpu_mod
       integer(c_int32_t), value :: slot
     end function scn_batch_release
 
+    integer(c_int) function scn_batch_set_window(batch, slot, window) bind(C, name="scn_batch_set_window")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value        :: batch
+      integer(c_int32_t), value :: slot
+      integer(c_int32_t), value :: window
+    end function scn_batch_set_window
+
+    integer(c_int64_t) function scn_batch_secondary_bytes(transport, buffer) bind(C, name="scn_batch_secondary_bytes")
+      import :: c_int32_t, c_int64_t
+      integer(c_int32_t), value :: transport
+      integer(c_int32_t), value :: buffer
+    end function scn_batch_secondary_bytes
+
+    integer(c_int) function scn_batch_bin(batch, slot, bin, result, n_batches) bind(C, name="scn_batch_bin")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value              :: batch
+      integer(c_int32_t), value       :: slot
+      integer(c_int64_t), value       :: bin
+      real(c_double), intent(out)     :: result(2)
+      integer(c_int32_t), intent(out) :: n_batches
+    end function scn_batch_bin
+
     integer(c_int) function scn_batch_stats(batch, out) bind(C, name="scn_batch_stats")
       import :: c_int, c_ptr, scn_stats
       type(c_ptr), value           :: batch
@@ -455,7 +477,8 @@ module scone_gpu_mod
   public :: scn_create, scn_destroy, scn_last_error, scn_device_info, scn_eigen_cycle, scn_norm_size_repr
   public :: scn_eigen_run, scn_eigen_start, scn_eigen_step, scn_eigen_finish, scn_fission_source
   public :: scn_batch_create, scn_batch_destroy, scn_batch_last_error, scn_batch_add, scn_batch_cycle, &
-            scn_batch_query, scn_batch_states, scn_batch_release, scn_batch_stats
+            scn_batch_query, scn_batch_states, scn_batch_release, scn_batch_stats, scn_batch_set_window, &
+            scn_batch_bin, scn_batch_secondary_bytes
   public :: scn_norm_size_repr_noalloc, scn_get_stats, scn_reset_stats
 
 end module scone_gpu_mod

@@ -57,7 +57,8 @@ k_items(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first
             const SlotDev& D = slots[s];
             const int pop = D.pop;
             if (first) {
-                nb = S.running ? min(D.window, D.n_batches - S.batch) : 0;
+                const int w = S.win > 0 ? min(S.win, D.window) : D.window;
+                nb = S.running ? min(w, D.n_batches - S.batch) : 0;
                 n = nb > 0 ? pop : 0;
                 S.n_run = nb;
             } else {
@@ -161,6 +162,28 @@ __global__ void k_bnorm(const SlotDev* __restrict__ slots, SlotState* ss, int n_
         }
     }
     S.n_close = n_close;
+    // adaptive window: the relative sd of the mean falls like 1/sqrt(N), so the target is projected
+    // at N* = N (rel / stop_rel)^2 batches; the next cycle transports adapt% of the batches still
+    // missing to N* (at least one): the solve approaches the projection from below, in a few
+    // cycles, and few batches are transported past the stop.  Which batches are transported in
+    // which cycle changes nothing in the results.
+    if (D.adapt > 0 && D.stop_bin >= 0 && !S.stopped) {
+        const int N = S.batch + n_close;
+        int w = D.window;
+        if (N < 2) {
+            w = 2 - N;
+        } else if (D.stop_rel > 0.0) {
+            const double mean = cs / N;
+            const double var = cs2 * (1.0 / N) * (1.0 / (N - 1)) - mean * mean * (1.0 / (N - 1));
+            const double sd = sqrt(var);
+            if (mean != 0.0 && sd == sd) {
+                const double q = sd / fabs(mean) / D.stop_rel;
+                const double need = ceil(0.01 * D.adapt * ((double)N * q * q - N));
+                w = need < 1.0 ? 1 : (need > (double)D.window ? D.window : (int)need);
+            }
+        }
+        S.win = w;
+    }
 }
 
 // scoreMemory_class.f90:309-342 closeCycle of each closed batch of every running slot, in batch
@@ -236,6 +259,7 @@ struct SlotHost {
     int fissile = 0, buffer = 0;
     int geo = 0;        // needs the general-geometry kernels
     int ww = 0;         // weight windows: split copies use the history's buffer
+    int window_max = 1; // window the solve was admitted with (BIN copies allocated)
 };
 
 struct Batch {
@@ -264,9 +288,13 @@ struct Batch {
     scn_stats stats{};
     int64_t rel_seg = 0, rel_col = 0, rel_sc = 0, rel_hist = 0;   // counters of released slots
     int S_lds_per_wave = 2048;   // low bins of a slot privatised per wave in LDS (16 KiB per wave)
-    // per-lane LIFO buffers of fixed-source secondaries (allocated when a fissile problem is resident)
+    // per-lane LIFO buffers of fixed-source secondaries: a block of the arena (so inside the HBM
+    // budget), reserved when a solve with secondaries is admitted, for max_lanes lanes of the
+    // deepest such solve's buffer, and returned when none is resident
     Sites stack{};
-    void* stack_mem = nullptr;
+    size_t stack_off = SIZE_MAX, stack_bytes = 0;
+    int stack_depth = 0;
+    int max_lanes = 0;   // lanes of the largest persistent grid of any transport kernel variant
 };
 
 #define BHIPCHK(expr)                                                                    \
@@ -397,6 +425,10 @@ int validate(Batch* B, const scn_fixed_request* R, int* depth)
     }
     if (R->window < 0 || R->window > MAX_WINDOW) {
         B->err = "window outside 0..16";
+        return SCN_ERR_ARG;
+    }
+    if (R->adapt < 0 || R->adapt > 100) {
+        B->err = "adapt outside 0..100";
         return SCN_ERR_ARG;
     }
     if (R->window > 1 && B->S.n_launch > 1) {
@@ -548,26 +580,49 @@ void drop_graph(Batch* B)
     B->graph = nullptr;
 }
 
-// per-lane LIFO buffers for the persistent grid, as deep as the deepest resident slot with secondaries
-// (fissile, or weight windows that split) needs
+constexpr size_t STACK_ENTRY_BYTES = 8 * sizeof(double) + sizeof(int);   // r, u, w, t, G
+
+// arena bytes of per-lane buffers of `depth` entries for `lanes` lanes
+size_t stack_bytes_for(int64_t lanes, int depth)
+{
+    return align256((size_t)lanes * (size_t)depth * STACK_ENTRY_BYTES + 1024);
+}
+
+// the largest persistent grid (in lanes) any transport kernel variant of `transport` takes on a
+// device of n_cu CUs with `lds` bytes of LDS per block
+int64_t max_grid_lanes(int transport, int n_cu, size_t lds)
+{
+    int64_t lanes = 0;
+    for (int key : {1, 3, 4, 8, 3 + GEO_KEY, 8 + GEO_KEY}) {
+        BTransportFn f;
+        const void* kern;
+        if (transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(key, &f, &kern);
+        else if (transport == SCN_TRANSPORT_HT) pick_bt<SCN_TRANSPORT_HT>(key, &f, &kern);
+        else pick_bt<SCN_TRANSPORT_ST>(key, &f, &kern);
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, lds) != hipSuccess) occ = 8;
+        lanes = std::max(lanes, (int64_t)n_cu * std::max(1, occ) * BBLOCK);
+    }
+    return lanes;
+}
+
+// the per-lane LIFO buffers of this cycle's grid inside the reserved arena block
 int ensure_stack(Batch* B, int grid)
 {
-    int depth = 0;
-    for (const SlotHost& h : B->hs)
-        if (h.used && (h.fissile || h.ww)) depth = std::max(depth, h.buffer);
-    if (depth == 0) return 0;
-    const int lanes = grid * BBLOCK;
-    if (B->stack_mem && B->stack.stride >= lanes && B->stack.depth >= depth) return 0;
-    drop_graph(B);
-    if (B->stack_mem) {
-        BHIPCHK(hipStreamSynchronize(B->stream));
-        BHIPCHK(hipFree(B->stack_mem));
-        B->stack_mem = nullptr;
+    if (B->stack_depth == 0 || B->stack_off == SIZE_MAX) {
+        B->stack = Sites{};
+        return 0;
     }
+    const int lanes = grid * BBLOCK;
+    if (lanes > B->max_lanes) {
+        B->err = "persistent grid larger than the lanes the secondary buffers were sized for";
+        return SCN_ERR_OVERFLOW;
+    }
+    const int depth = B->stack_depth;
+    if (B->stack.r && B->stack.stride == lanes && B->stack.depth == depth) return 0;
+    drop_graph(B);
     const size_t n = (size_t)lanes * depth;
-    const size_t bytes = n * (8 * sizeof(double) + sizeof(int)) + 1024;
-    BHIPCHK(hipMalloc(&B->stack_mem, bytes));
-    char* q = (char*)B->stack_mem;
+    char* q = B->arena + B->stack_off;
     Sites S{};
     S.r = (double*)q;
     S.u = S.r + 3 * n;
@@ -578,6 +633,30 @@ int ensure_stack(Batch* B, int grid)
     S.stride = lanes;
     S.depth = depth;
     B->stack = S;
+    return 0;
+}
+
+// reserve (grow) or return the secondary buffers for the resident solves plus `extra_depth`
+// (a solve being admitted, 0 for none); SCN_ERR_BUSY if the budget cannot hold them
+int reserve_stack(Batch* B, int extra_depth)
+{
+    int depth = extra_depth;
+    for (const SlotHost& h : B->hs)
+        if (h.used && (h.fissile || h.ww)) depth = std::max(depth, h.buffer);
+    if (depth == B->stack_depth) return 0;
+    if (depth < B->stack_depth && depth > 0) return 0;   // keep the deeper block while it is in use
+    size_t off = SIZE_MAX, bytes = 0;
+    if (depth > 0) {   // the new block first: the resident solves keep theirs if it does not fit
+        bytes = stack_bytes_for(B->max_lanes, depth);
+        off = arena_alloc(B, bytes);
+        if (off == SIZE_MAX) return SCN_ERR_BUSY;
+    }
+    drop_graph(B);
+    B->stack = Sites{};
+    if (B->stack_off != SIZE_MAX) arena_free(B, B->stack_off, B->stack_bytes);
+    B->stack_off = off;
+    B->stack_bytes = bytes;
+    B->stack_depth = depth;
     return 0;
 }
 
@@ -676,6 +755,9 @@ int scn_batch_create(const scn_batch_settings* settings, void** out)
     B->n_slots = settings->max_slots;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) B->n_cu = prop.multiProcessorCount;
+    // the largest persistent grid any transport variant of this engine can take (sizes the
+    // per-lane buffers of secondaries)
+    B->max_lanes = (int)max_grid_lanes(B->S.transport, B->n_cu, lds_bytes(B));
     auto fail = [&](const std::string& m, int code) {
         g_batch_create_err = m;
         scn_batch_destroy(B);
@@ -719,7 +801,6 @@ void scn_batch_destroy(void* batch)
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     drop_graph(B);
     if (B->arena) (void)hipFree(B->arena);
-    if (B->stack_mem) (void)hipFree(B->stack_mem);
     if (B->d_items) (void)hipFree(B->d_items);
     if (B->d_item_cnt) (void)hipFree(B->d_item_cnt);
     if (B->d_slots) (void)hipFree(B->d_slots);
@@ -826,6 +907,18 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
         B->err = "not enough of the HBM budget free for this request";
         return SCN_ERR_BUSY;
     }
+    // secondaries (fission sites, weight-window splits) need the per-lane buffers, which come out
+    // of the same budget
+    int needs_stack = pr->ww_n > 0;
+    for (int m = 0; m < nM; ++m) needs_stack |= pr->mg.fissile[m] ? 1 : 0;
+    if (needs_stack) {
+        const int rs = reserve_stack(B, R->buffer > 0 ? R->buffer : DEFAULT_BUFFER);
+        if (rs) {
+            arena_free(B, off, Ly.total);
+            B->err = "not enough of the HBM budget free for this request's buffers of secondaries";
+            return rs;
+        }
+    }
     char* base = B->arena + off;
     SlotDev D{};
     DevProblem& P = D.P;
@@ -905,8 +998,10 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     D.n_batches = R->n_batches;
     D.buffer = R->buffer > 0 ? R->buffer : DEFAULT_BUFFER;
     D.window = window;
+    D.adapt = R->stop_bin >= 0 ? R->adapt : 0;
     SlotState st{};
     st.running = 1;
+    st.win = D.adapt > 0 ? std::min(2, window) : 0;
     auto undo = [&](int code) {
         arena_free(B, off, Ly.total);
         return code;
@@ -934,6 +1029,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     h.fissile = 0;
     for (int m = 0; m < nM; ++m) h.fissile |= pr->mg.fissile[m] ? 1 : 0;
     h.buffer = D.buffer;
+    h.window_max = window;
     *slot = s;
     return 0;
 }
@@ -999,6 +1095,46 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bi
     return 0;
 }
 
+int scn_batch_set_window(void* batch, int32_t slot, int32_t window)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || slot < 0 || slot >= B->n_slots || !B->hs[slot].used) return SCN_ERR_ARG;
+    if (window < 1 || window > B->hs[slot].window_max) {
+        B->err = "window outside 1 .. the window the solve was admitted with";
+        return SCN_ERR_ARG;
+    }
+    SlotDev& D = B->h_slots[slot];
+    D.window = window;
+    BHIPCHK(hipMemcpyAsync(&B->d_slots[slot].window, &D.window, sizeof(int), hipMemcpyHostToDevice, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    return 0;
+}
+
+int scn_batch_bin(void* batch, int32_t slot, int64_t bin, double result[2], int32_t* n_batches)
+{
+    Batch* B = (Batch*)batch;
+    if (!B || !result || slot < 0 || slot >= B->n_slots || !B->hs[slot].used) return SCN_ERR_ARG;
+    const SlotHost& h = B->hs[slot];
+    if (bin < 0 || bin >= h.n_bins) {
+        B->err = "bin outside the solve's score memory";
+        return SCN_ERR_ARG;
+    }
+    const SlotDev& D = B->h_slots[slot];
+    SlotState st{};
+    double c[2] = {0.0, 0.0};
+    BHIPCHK(hipMemcpyAsync(&st, B->d_state + slot, sizeof st, hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipMemcpyAsync(&c[0], D.csum + bin, sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipMemcpyAsync(&c[1], D.csum2 + bin, sizeof(double), hipMemcpyDeviceToHost, B->stream));
+    BHIPCHK(hipStreamSynchronize(B->stream));
+    if (n_batches) *n_batches = st.batch;
+    if (st.batch < 1) {
+        result[0] = result[1] = 0.0;
+        return 0;
+    }
+    score_result(c[0], c[1], st.batch, result);
+    return 0;
+}
+
 int scn_batch_states(void* batch, int32_t* states, int32_t n)
 {
     Batch* B = (Batch*)batch;
@@ -1035,7 +1171,19 @@ int scn_batch_release(void* batch, int32_t slot)
     arena_free(B, h.off, h.bytes);
     h = SlotHost{};
     B->h_slots[slot] = SlotDev{};
-    return 0;
+    return reserve_stack(B, 0);   // shrinks nothing in use; frees the buffers when no solve needs them
+}
+
+int64_t scn_batch_secondary_bytes(int32_t transport, int32_t buffer)
+{
+    if (transport < SCN_TRANSPORT_ST || transport > SCN_TRANSPORT_HT || buffer < 0) return -1;
+    int ndev = 0, n_cu = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return -1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return -1;
+    n_cu = prop.multiProcessorCount;
+    const size_t lds = sizeof(double) * (size_t)Batch{}.S_lds_per_wave * (BBLOCK / BWAVE);
+    return (int64_t)stack_bytes_for(max_grid_lanes(transport, n_cu, lds), buffer > 0 ? buffer : DEFAULT_BUFFER);
 }
 
 int scn_batch_stats(void* batch, scn_stats* out)

@@ -68,6 +68,7 @@ struct SlotDev {
     int pop, n_batches;
     int buffer;         // 'buffer' of the package: depth of a history's LIFO buffer of secondaries
     int window;         // batches transported per cycle; BIN holds one copy of the score memory each
+    int adapt;          // > 0: each cycle's window follows the stop bin's trend (projection factor in %)
 };
 
 // mutable state of a slot, written by the kernels
@@ -79,6 +80,7 @@ struct SlotState {
     int n_run;          // batches transported this cycle (batch .. batch + n_run - 1)
     int n_close;        // of those, batches closed (fewer after an early stop)
     int stopped;        // 1 if the early-stop target ended the solve
+    int win;            // batches the next cycle transports (adaptive windows); 0 = the slot's window
     double norm[MAX_WINDOW];   // normalisation factor of each batch being closed
     unsigned long long segments, collisions, scores, histories;
 };

@@ -20,7 +20,8 @@ EXPORTS = ["scn_create", "scn_destroy", "scn_last_error", "scn_device_info", "sc
            "scn_fission_source", "scn_get_stats", "scn_reset_stats",
            "scn_batch_create", "scn_batch_destroy", "scn_batch_last_error", "scn_batch_add", "scn_batch_cycle",
            "scn_batch_query", "scn_batch_states", "scn_batch_release", "scn_batch_stats",
-           "scn_norm_size_repr_noalloc"]
+           "scn_norm_size_repr_noalloc", "scn_batch_set_window", "scn_batch_bin",
+           "scn_batch_secondary_bytes"]
 
 SCN_ERR_BUSY = 11
 
@@ -78,6 +79,10 @@ def load():
     L.scn_batch_states.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]
     L.scn_batch_release.argtypes = [C.c_void_p, C.c_int32]
     L.scn_batch_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.scn_batch_secondary_bytes.argtypes = [C.c_int32, C.c_int32]
+    L.scn_batch_secondary_bytes.restype = C.c_int64
+    L.scn_batch_set_window.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    L.scn_batch_bin.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.scn_norm_size_repr_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int64,
                                              C.c_void_p, C.POINTER(C.c_int32)]
     _lib = L
@@ -231,6 +236,15 @@ class Busy(EngineError):
     """The batch engine has no free slot or budget for a request right now."""
 
 
+def secondary_bytes(transport=T.TRANSPORT_DT, buffer=0) -> int:
+    """HBM-budget bytes of the per-lane buffers of secondaries that a batch engine reserves once a
+    solve with fission or weight windows is resident (`buffer` entries per lane, 0 = 50)."""
+    n = load().scn_batch_secondary_bytes(transport, buffer)
+    if n < 0:
+        raise EngineError("scn_batch_secondary_bytes: no GPU visible to HIP")
+    return int(n)
+
+
 class Batch:
     """Fixed-source engine with resident problem slots (scn_batch_* of include/scone_gpu.h)."""
 
@@ -274,9 +288,11 @@ class Batch:
             raise EngineError(f"batch engine error {rc}: {msg}")
 
     def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0,
-            window=1) -> int:
+            window=1, adapt=0) -> int:
         """Admit a solve; raises Busy if it does not fit now.  Returns its slot.  `window` batches of
-        the solve are transported together in each cycle (closed in order: same results)."""
+        the solve are transported together in each cycle (closed in order: same results); with
+        `adapt` (1-100) and a stop bin, each cycle transports only the batches the stop bin's trend
+        still asks for (at most `window`)."""
         R = T.FixedRequest()
         R.problem = C.pointer(comp.problem)
         R.source = C.pointer(src.source)
@@ -288,6 +304,7 @@ class Batch:
         R.record = 1 if record else 0
         R.buffer = buffer
         R.window = window
+        R.adapt = adapt
         slot = C.c_int32(-1)
         self._check(self.L.scn_batch_add(self.h, C.byref(R), C.byref(slot)))
         self._held[slot.value] = (comp, src, pop, record)
@@ -324,6 +341,17 @@ class Batch:
         out = np.zeros(n, dtype=np.int32)
         self._check(self.L.scn_batch_states(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), n))
         return out
+
+    def set_window(self, slot, window):
+        """Batches of a running solve transported per cycle from now on (1 .. its admitted window)."""
+        self._check(self.L.scn_batch_set_window(self.h, slot, window))
+
+    def bin_result(self, slot, b):
+        """(mean, sd, batches closed) of bin b of a running or done solve."""
+        out = (C.c_double * 2)()
+        n = C.c_int32()
+        self._check(self.L.scn_batch_bin(self.h, slot, b, out, C.byref(n)))
+        return out[0], out[1], n.value
 
     def release(self, slot):
         self._check(self.L.scn_batch_release(self.h, slot))

@@ -12,7 +12,7 @@ import os
 import time
 
 from . import dictio, output, tables as T
-from .engine import Batch, Engine
+from .engine import Batch, Engine, secondary_bytes
 
 
 def _seed(inp):
@@ -44,9 +44,12 @@ def run_fixed(inp: dict, base_dir: str = "."):
     src = T.compile_source(inp["source"], comp)
     pop, cycles, seed = int(inp["pop"]), int(inp["cycles"]), _seed(inp)
     c0 = time.process_time()
+    buffer = int(inp.get("buffer", 50))
+    # slot (queues, score memory, tables) plus the per-lane buffers of secondaries
     B = Batch(transport=comp.problem.transport, max_slots=1,
-              hbm_budget=max(256 << 20, 4 * 400 * pop + 64 * int(comp.problem.n_bins) + (64 << 20)))
-    slot = B.add(comp, src, pop, cycles, seed, buffer=int(inp.get("buffer", 50)))
+              hbm_budget=max(256 << 20, 4 * 400 * pop + 64 * int(comp.problem.n_bins) + (64 << 20))
+              + secondary_bytes(comp.problem.transport, buffer))
+    slot = B.add(comp, src, pop, cycles, seed, buffer=buffer)
     t0 = time.perf_counter()
     B.run_all()
     transport = time.perf_counter() - t0

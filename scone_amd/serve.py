@@ -5,6 +5,8 @@ budget are free (scn_batch_add refuses it otherwise and it waits, first come fir
 Every engine cycle advances all resident solves by `window` batches each (one launch transports
 them together and closes them in order), so the work of requests that arrive while others run
 overlaps on the device, and a request's latency is a few cycles rather than one cycle per batch.
+With `adapt`, a solve with a stop bin transports 2 batches in its first cycle and afterwards only
+the batches its stop bin's trend still asks for, so little work is spent on batches past the stop.
 A solve leaves when its batches are done or when the relative standard deviation of its stop bin
 reaches the target; its slot and memory return to the pool at once and the next waiting request
 takes them.
@@ -36,6 +38,7 @@ class Request:
     record: bool = False
     buffer: int = 0         # depth of a history's buffer of secondaries (0: the package default, 50)
     window: int = 1         # batches transported together per engine cycle (closed in order)
+    adapt: int = 0          # 1-100 with a stop bin: each cycle's window follows the stop bin's trend
     arrival: float = 0.0    # seconds (wall mode) or cycle index (cycle mode)
     # filled by the server
     admitted: float = -1.0
@@ -64,7 +67,7 @@ class Server:
             try:
                 rq.slot = self.B.add(rq.comp, rq.src, rq.pop, rq.n_batches, rq.seed, stop_bin=rq.stop_bin,
                                      stop_rel=rq.stop_rel, record=rq.record, buffer=rq.buffer,
-                                     window=rq.window)
+                                     window=rq.window, adapt=rq.adapt)
             except Busy:
                 self.busy_rejections += 1
                 return

@@ -149,7 +149,7 @@ class FixedRequest(C.Structure):
     _fields_ = [("problem", C.POINTER(Problem)), ("source", C.POINTER(Source)), ("pop", C.c_int32),
                 ("n_batches", C.c_int32), ("seed", C.c_int64), ("stop_bin", C.c_int64),
                 ("stop_rel", C.c_double), ("record", C.c_int32), ("buffer", C.c_int32), ("window", C.c_int32),
-                ("pad0", C.c_int32)]
+                ("adapt", C.c_int32)]
 
 
 class FixedResult(C.Structure):

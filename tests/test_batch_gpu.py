@@ -11,7 +11,7 @@ import pytest
 import binding as O
 from helpers import assert_results_close
 from scone_amd import configs, tables as T
-from scone_amd.engine import Batch, Busy, EngineError
+from scone_amd.engine import secondary_bytes, Batch, Busy, EngineError
 
 pytestmark = pytest.mark.gpu
 
@@ -66,7 +66,8 @@ def test_single_solve_bit_exact(label, make, pop, cycles):
     inp, mg = make()
     comp, src = compiled(inp, mg)
     ref = oracle_ref(comp, src, pop, cycles, inp["seed"])
-    B = Batch(transport=comp.problem.transport, max_slots=4, hbm_budget=256 << 20)
+    B = Batch(transport=comp.problem.transport, max_slots=4,
+              hbm_budget=(256 << 20) + secondary_bytes(comp.problem.transport))
     s = B.add(comp, src, pop, cycles, inp["seed"], record=True)
     B.run_all()
     check_slot(B, s, ref, label)
@@ -176,14 +177,16 @@ def test_fissile_lattice_with_secondaries_and_buffer_overflow():
     comp, src = compiled(inp, mg)
     ref = oracle_ref(comp, src, pop, 3, inp["seed"], buffer=1000)
     assert ref[2]["n_coll"].max() > 50
-    B = Batch(transport=comp.problem.transport, max_slots=2, hbm_budget=256 << 20, n_launch=3, k0=4)
+    B = Batch(transport=comp.problem.transport, max_slots=2,
+              hbm_budget=(256 << 20) + secondary_bytes(comp.problem.transport, 1000), n_launch=3, k0=4)
     s = B.add(comp, src, pop, 3, inp["seed"], record=True, buffer=1000)
     B.run_all()
     check_slot(B, s, ref, "lattice")
     B.close()
     with pytest.raises(RuntimeError, match="Run out of space"):
         O.fixed_run(comp, src, pop, 1, inp["seed"], buffer=1)
-    B = Batch(transport=comp.problem.transport, max_slots=1, hbm_budget=256 << 20)
+    B = Batch(transport=comp.problem.transport, max_slots=1,
+              hbm_budget=(256 << 20) + secondary_bytes(comp.problem.transport, 1))
     B.add(comp, src, pop, 1, inp["seed"], buffer=1)
     with pytest.raises(EngineError, match="Run out of space"):
         B.run_all()
@@ -215,8 +218,9 @@ def test_serving_results_same_alone_and_in_mix():
     for i in range(10):
         _, _, pop, cycles = configs.lattice_request(i, transport="transportOperatorDT", scale=0.005)
         reqs.append(Request(comp, src, pop, cycles, 500 + i, stop_bin=0, stop_rel=0.05, record=True, buffer=1000,
-                            arrival=0 if i < 7 else i - 5, window=1 + 3 * (i % 2)))
-    B = Batch(transport=comp.problem.transport, max_slots=4, hbm_budget=256 << 20)
+                            arrival=0 if i < 7 else i - 5, window=1 + 3 * (i % 2), adapt=50 * (i % 3 == 0)))
+    stack = secondary_bytes(comp.problem.transport, 1000)
+    B = Batch(transport=comp.problem.transport, max_slots=4, hbm_budget=(256 << 20) + stack)
     srv = Server(B)
     srv.run_cycles(reqs)
     assert srv.busy_rejections > 0          # more requests than slots: some had to wait
@@ -226,7 +230,7 @@ def test_serving_results_same_alone_and_in_mix():
         assert rq.result.n_batches == ref[1], i
         assert np.array_equal(rq.hist_rng, ref[2]["rng_end"]), i
         assert_results_close(ref[0], rq.bins, ref[1], 1e-12, f"request {i}")
-        A = Batch(transport=comp.problem.transport, max_slots=1, hbm_budget=64 << 20)
+        A = Batch(transport=comp.problem.transport, max_slots=1, hbm_budget=(64 << 20) + stack)
         a = A.add(comp, src, rq.pop, rq.n_batches, rq.seed, stop_bin=0, stop_rel=0.05, record=True, buffer=1000)
         A.run_all()
         r, bins, hr, hc = A.query(a)
@@ -246,7 +250,8 @@ def test_weight_windows_bit_exact(tmp_path, top, n_launch):
     inp, mg = configs.ww_medium(str(path), pop=pop, cycles=cycles, transport=top, max_split=20)
     comp, src = compiled(inp, mg)
     ref = oracle_ref(comp, src, pop, cycles, inp["seed"], buffer=inp["buffer"])
-    B = Batch(transport=comp.problem.transport, max_slots=2, hbm_budget=128 << 20, n_launch=n_launch, k0=2)
+    B = Batch(transport=comp.problem.transport, max_slots=2,
+              hbm_budget=(128 << 20) + secondary_bytes(comp.problem.transport, inp["buffer"]), n_launch=n_launch, k0=2)
     s = B.add(comp, src, pop, cycles, inp["seed"], record=True, buffer=inp["buffer"])
     B.run_all()
     check_slot(B, s, ref, f"weight windows {top}")

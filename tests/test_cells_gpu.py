@@ -10,7 +10,7 @@ import pytest
 import binding as O
 from helpers import assert_banks_identical, assert_results_close, rel_close
 from scone_amd import configs, tables as T
-from scone_amd.engine import Batch, Engine
+from scone_amd.engine import Batch, Engine, secondary_bytes
 
 pytestmark = pytest.mark.gpu
 
@@ -79,8 +79,9 @@ def test_fixed_source_cells_and_sphere_border_in_one_batch(top):
     probs.append((inp, mg, 1500, 3, 50))
     inp, mg = configs.slab_fixed(pop=1000, cycles=3, transport=top)
     probs.append((inp, mg, 1000, 3, 50))
-    B = Batch(transport=T.TRANSPORT_ST if top.endswith("ST") else T.TRANSPORT_DT, max_slots=4,
-              hbm_budget=512 << 20)
+    tcode = T.TRANSPORT_ST if top.endswith("ST") else T.TRANSPORT_DT
+    B = Batch(transport=tcode, max_slots=4,
+              hbm_budget=(512 << 20) + secondary_bytes(tcode, max(p[4] for p in probs)))
     slots = []
     for inp, mg, pop, cycles, buf in probs:
         comp = T.compile_problem(inp, mg)

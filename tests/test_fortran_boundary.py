@@ -40,7 +40,7 @@ LAYOUT = {
     "scn_source": ["type", "isotropic", "G", "mat", "n_probG", "pad", "r", "dir", "bottom", "top", "t_low", "t_high",
                    "probG"],
     "scn_fixed_request": ["problem", "source", "pop", "n_batches", "seed", "stop_bin", "stop_rel", "record", "buffer",
-                          "window", "pad0"],
+                          "window", "adapt"],
     "scn_fixed_result": ["state", "n_batches", "n_bins", "n_histories", "n_collisions", "n_segments"],
     "scn_batch_settings": ["transport", "max_slots", "hbm_budget", "tile", "k0", "n_launch", "use_graph", "refill",
                            "pad0"],

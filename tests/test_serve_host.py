@@ -15,7 +15,8 @@ class FakeBatch:
         self.used = 0
         self.log = []
 
-    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0, window=1):
+    def add(self, comp, src, pop, n_batches, seed, stop_bin=-1, stop_rel=0.0, record=False, buffer=0, window=1,
+            adapt=0):
         if self.used + pop > self.budget or None not in self.slots:
             raise Busy("full")
         s = self.slots.index(None)
