@@ -261,15 +261,19 @@ def cpu_batch_baseline(problems, n_batches):
 # ---------------------------------------------------------------------------------------------
 # serving
 # ---------------------------------------------------------------------------------------------
-def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7, window=10, adapt=50):
+def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=7, window=10, adapt=50,
+            engines=1, slots=256, saturate=1024):
     """Continuous batching of 17x17-lattice fixed-source requests (1e3-1e6 histories in 10 batches,
-    early stop at 1 % relative sd of the total flux).  Capacity: 256 requests admitted at once, with
-    fixed windows of `window` batches and with adaptive windows (`adapt`); latency at offered loads
-    of the adaptive capacity, served with adaptive windows.  The HBM budget holds the per-lane
-    buffers of fission secondaries (the requests' 'buffer' of 1000 entries) next to the slots."""
+    fission chains followed as secondaries, early stop at 1 % relative sd of the total flux).
+    Capacity: `saturate` requests waiting at t = 0, served by one engine with fixed windows of
+    `window` batches and with adaptive windows (`adapt`), and, for `engines` > 1, by a pool of that
+    many engines (own stream and host thread each, `slots` slots in all) with adaptive windows.
+    Latency at offered loads of the last of these capacities, served the same way.  Each engine's HBM budget holds
+    the per-lane buffers of fission secondaries (the requests' 'buffer' of 1000 entries) next to
+    its slots."""
     from scone_amd import configs, tables as T
     from scone_amd.engine import Batch, secondary_bytes
-    from scone_amd.serve import Request, Server, latency_summary
+    from scone_amd.serve import Pool, Request, Server, latency_summary
     inp, mg, _, _ = configs.lattice_request(0, transport=transport)
     comp = T.compile_problem(inp, mg)
     src = T.compile_source(inp["source"], comp)
@@ -280,28 +284,32 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
                        buffer=inp["buffer"], window=min(window, cycles), adapt=ad)
 
     stack = secondary_bytes(comp.problem.transport, inp["buffer"])
-    slots, budget = 256, (8 << 30) + stack
-    out = {"window": window, "adapt": adapt, "secondary_buffer_bytes": stack, "hbm_budget": budget, "loads": {}}
-    # capacity: a saturated pool of requests, all admitted at once
-    for ad in (0, adapt):
-        B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
-        reqs = [request(100000 + i, 0.0, ad) for i in range(slots)]
-        srv = Server(B)
-        t = time.perf_counter()
-        srv.run_wall(reqs)
-        dt = time.perf_counter() - t
+    budget = (2 << 30) + stack
+
+    def make(n_eng):
+        Bs = [Batch(transport=comp.problem.transport, max_slots=slots // n_eng, hbm_budget=budget, use_graph=1)
+              for _ in range(n_eng)]
+        return Bs, (Server(Bs[0]) if n_eng == 1 else Pool(Bs))
+
+    out = {"window": window, "adapt": adapt, "engines": engines, "slots": slots,
+           "secondary_buffer_bytes_per_engine": stack, "hbm_budget_per_engine": budget, "loads": {}}
+    legs = [("one_engine_fixed_window", 1, 0), ("one_engine_adaptive", 1, adapt)]
+    if engines > 1:
+        legs.append(("pool_adaptive", engines, adapt))
+    for key, n_eng, ad in legs:
+        Bs, srv = make(n_eng)
+        reqs = [request(100000 + i, 0.0, ad) for i in range(saturate)]
+        dt = srv.run_wall(reqs)
         hist = sum(r.result.n_histories for r in reqs)
-        cap_hist = hist / dt
-        mean_req = hist / len(reqs)
-        cap_req = cap_hist / mean_req
-        B.close()
-        key = "adaptive" if ad else "fixed_window"
-        out[key] = {"capacity_hist_per_s": cap_hist, "capacity_req_per_s": cap_req, "cycles": srv.cycles,
-                    "histories": hist, "elapsed_s": dt}
-        log(f"serving capacity ({key}) {cap_hist:.3e} histories/s, {cap_req:.1f} requests/s (mean {mean_req:.0f} "
-            f"histories), {srv.cycles} cycles")
-    out.update(capacity_hist_per_s=out["adaptive"]["capacity_hist_per_s"],
-               capacity_req_per_s=out["adaptive"]["capacity_req_per_s"], mean_request_histories=mean_req)
+        for B in Bs:
+            B.close()
+        out[key] = {"capacity_hist_per_s": hist / dt, "capacity_req_per_s": len(reqs) / dt, "cycles": srv.cycles,
+                    "histories": hist, "elapsed_s": dt, "engines": n_eng}
+        log(f"serving capacity ({key}) {hist / dt:.3e} histories/s, {len(reqs) / dt:.1f} requests/s, "
+            f"{srv.cycles} cycles, {dt:.2f} s")
+    served = out[legs[-1][0]]
+    out.update(capacity_hist_per_s=served["capacity_hist_per_s"], capacity_req_per_s=served["capacity_req_per_s"],
+               mean_request_histories=served["histories"] / saturate)
     cap_req = out["capacity_req_per_s"]
     rng = np.random.default_rng(seed)
     k = 0
@@ -311,8 +319,7 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
         reqs = [request(k + i, float(a), adapt) for i, a in enumerate(arr)]
         k += n
-        B = Batch(transport=comp.problem.transport, max_slots=slots, hbm_budget=budget, use_graph=1)
-        srv = Server(B)
+        Bs, srv = make(engines)
         el = srv.run_wall(reqs, progress=lambda t, p, w, r: log(f"serving load {load}: {t:.0f} s, pending {p}, "
                                                                  f"waiting {w}, running {r}"))
         hist = sum(r.result.n_histories for r in reqs)
@@ -321,7 +328,8 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
                      "busy_rejections": srv.busy_rejections,
                      "early_stopped": int(sum(1 for r in reqs if r.result.n_batches < r.n_batches))})
         out["loads"][str(load)] = summ
-        B.close()
+        for B in Bs:
+            B.close()
         log(f"serving load {load}: p50 {summ['p50_s'] * 1e3:.1f} ms, p99 {summ['p99_s'] * 1e3:.1f} ms, "
             f"{summ['hist_per_s']:.3e} histories/s")
     return out
@@ -475,7 +483,8 @@ def batch_line(args):
     if res_p:
         identical = all(results_agree(res_b[i], res_p[i], warmup + args.steps) for i in res_b)
     cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_batches)
-    serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds, window=args.window)
+    serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds, window=args.window,
+                                                                  engines=args.engines)
     eig = None
     if args.eigen:
         log("eigen leg")
@@ -546,6 +555,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="offered-load duration per load level")
     ap.add_argument("--window", type=int, default=10, help="serving: batches of a request transported per cycle")
+    ap.add_argument("--engines", type=int, default=1, help="serving: engines (streams, host threads) in a pool")
     ap.add_argument("--no-eigen", dest="eigen", action="store_false", default=True)
     ap.add_argument("--no-c5", dest="c5", action="store_false", default=True)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="eigenvalue CPU baseline: seconds of cycles")

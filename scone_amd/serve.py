@@ -13,10 +13,18 @@ takes them.
 
 Each solve's histories depend only on its own seed and tables, so its result is the same alone
 and in any mix (tallies to summation order).
+
+A cycle ends when its last history does, and a fixed-source history includes every secondary of
+its fission chain, transported one after another on the history's own random-number stream: in a
+multiplying medium the longest chain of a cycle (thousands of collisions among millions of
+histories) keeps one lane busy long after the rest of the device has drained.  `Pool` serves one
+queue with several engines, each on its own HIP stream and host thread (engine calls release the
+GIL), so the transport of one engine fills the device while another waits on its longest chain.
 """
 from __future__ import annotations
 
 import collections
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -127,6 +135,76 @@ class Server:
             if progress and now() - last > 20.0:
                 last = now()
                 progress(last, len(pending), len(waiting), len(running))
+        return now()
+
+
+class Pool:
+    """Several batch engines serving one first-come-first-served queue in wall-clock mode.  Each
+    engine has its own HIP stream and its own host thread; a waiting request goes to the first
+    engine with a free slot and enough budget."""
+
+    def __init__(self, batches):
+        self.servers = [Server(b) for b in batches]
+
+    @property
+    def cycles(self):
+        return sum(s.cycles for s in self.servers)
+
+    @property
+    def busy_rejections(self):
+        return sum(s.busy_rejections for s in self.servers)
+
+    def run_wall(self, requests, keep=False, progress=None):
+        """request.arrival is seconds after the start.  Returns the elapsed time."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        lock = threading.Lock()
+        active = [0]          # requests admitted and not yet harvested, over all engines
+        errors = []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+
+        def worker(srv):
+            running = {}
+            try:
+                while True:
+                    with lock:
+                        t = now()
+                        while pending and pending[0].arrival <= t:
+                            waiting.append(pending.popleft())
+                        n0 = len(running)
+                        srv._admit(waiting, running, t)
+                        active[0] += len(running) - n0
+                        finished = not pending and not waiting and active[0] == 0
+                        nxt = pending[0].arrival if pending else None
+                    if finished or errors:
+                        return
+                    if running:
+                        srv.B.cycle(1)
+                        srv.cycles += 1
+                        n0 = len(running)
+                        srv._harvest(running, now, keep)
+                        if len(running) != n0:
+                            with lock:
+                                active[0] -= n0 - len(running)
+                    else:
+                        time.sleep(0.0005 if nxt is None else max(0.0, min(0.002, nxt - now())))
+            except BaseException as e:      # reported by the caller's thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(s,), daemon=True) for s in self.servers]
+        for th in threads:
+            th.start()
+        last = 0.0
+        while any(th.is_alive() for th in threads):
+            for th in threads:
+                th.join(timeout=1.0)
+            if progress and now() - last > 20.0:
+                last = now()
+                with lock:
+                    progress(last, len(pending), len(waiting), active[0])
+        if errors:
+            raise errors[0]
         return now()
 
 

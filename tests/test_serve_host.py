@@ -70,3 +70,39 @@ def test_window_shortens_latency_in_cycles():
     Server(B).run_cycles(reqs)
     assert [r.done for r in reqs] == [10, 2, 3]
     assert [r.result.n_batches for r in reqs] == [10, 10, 10]
+
+
+def test_pool_serves_every_request_once_across_engines():
+    """Several engines, each on its own thread, drain one queue: every request is admitted by
+    exactly one engine, in arrival order, and all of them finish with their full batch count."""
+    from scone_amd.serve import Pool
+    engines = [FakeBatch(slots=3, budget=100) for _ in range(3)]
+    reqs = [Request(None, None, 20 + (i % 4) * 10, 1 + i % 5, seed=i, arrival=0.0005 * i, window=1 + i % 3)
+            for i in range(60)]
+    pool = Pool(engines)
+    el = pool.run_wall(reqs)
+    assert el > 0
+    seeds = sorted(s for B in engines for k, s in B.log if k == "add")
+    assert seeds == list(range(60))
+    assert all(r.done >= r.arrival >= 0 for r in reqs)
+    assert [r.result.n_batches for r in reqs] == [1 + i % 5 for i in range(60)]
+    assert sum(1 for B in engines if any(k == "add" for k, _ in B.log)) > 1
+    assert all(B.used == 0 and all(v is None for v in B.slots) for B in engines)
+    assert pool.cycles >= 1
+
+
+def test_pool_reports_an_engine_error():
+    from scone_amd.serve import Pool
+
+    class Failing(FakeBatch):
+        def cycle(self, n=1):
+            raise RuntimeError("engine failed")
+
+    pool = Pool([Failing(slots=2, budget=100), Failing(slots=2, budget=100)])
+    reqs = [Request(None, None, 10, 2, seed=i, arrival=0.0) for i in range(4)]
+    try:
+        pool.run_wall(reqs)
+    except RuntimeError as e:
+        assert "engine failed" in str(e)
+    else:
+        raise AssertionError("the engine's error was not raised")
