@@ -282,6 +282,10 @@ typedef struct scn_eigen_settings {
                                   same histories, banks and k history as 0.  Needs events_per_launch 0. */
     int32_t refill;            /* idle lanes a wave gathers before it starts new histories (1..64; 0: default);
                                   the histories do not change */
+    double select_window;      /* cycle_graph: half-width, in standard deviations of the order statistic, of
+                                  the window of draws gathered to find normSize_Repr's threshold (0: 8).
+                                  The threshold is exact for any value: a draw outside the window only
+                                  moves the search to a pass over all draws. */
 } scn_eigen_settings;
 
 typedef struct scn_eigen_result {

@@ -188,6 +188,7 @@ module scone_gpu_mod
     integer(c_int32_t) :: block_size
     integer(c_int32_t) :: cycle_graph
     integer(c_int32_t) :: refill
+    real(c_double)     :: select_window
   end type scn_eigen_settings
 
   type, public, bind(C) :: scn_eigen_result

@@ -292,159 +292,36 @@ struct EigCtl {
     int pop;
     int n_sites;           // sites banked by the last transport (clamped to the capacity)
     int excess, heap, n_copies, n_dup;
-    int rs_k;              // radix select: rank still sought among the draws matching the prefix
-    int pad;
-    uint64_t rs_prefix, rs_mask;
+    int n_below;           // draws below the selection window
+    int n_cand;            // draws inside it (gathered)
+    double thr;            // the heap-th smallest draw (normSize_Repr's threshold)
+    double sel_sigmas;     // half-width of the selection window in standard deviations
     unsigned long long sites_total;
 };
 
+// zero the transport's queue counters and the site count (before the first cycle; every cycle
+// then clears them for the next one in k_dcycle_end)
 __global__ void k_dcycle_begin(int* counters_i, int* site_count)
 {
     if (threadIdx.x < 4) counters_i[threadIdx.x] = 0;
     if (threadIdx.x == 0) *site_count = 0;
 }
 
-// sites in (broodID, emission index) order without a sort: history b's sites start at the exclusive
-// prefix sum of the site counts of histories 1..b-1 (sortByBroodID of the detain order)
-__global__ void k_place_sites(Sites S, const int* __restrict__ site_off, int pop, int out_cap,
-                              scn_particle_state* __restrict__ out, EigCtl* ctl, DevStatus* st)
-{
-    const int count = *S.count;
-    const int n = min(count, min(S.cap, out_cap));
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) {
-        ctl->n_sites = n;
-        ctl->sites_total += (unsigned long long)n;
-        if (count > n) raise_error(st, SCN_ERR_OVERFLOW, count);
-    }
-    if (j >= n) return;
-    const uint64_t key = S.key[j];
-    const int b = (int)(key >> 24);
-    if (b < 1 || b > pop) return;
-    const int64_t dst = (int64_t)site_off[b - 1] + (int64_t)(key & 0xFFFFFF);
-    if (dst >= n) return;
-    scn_particle_state q;
-    q.wgt = S.w[j];
-    for (int k = 0; k < 3; ++k) {
-        q.r[k] = S.r[(size_t)k * S.cap + j];
-        q.dir[k] = S.u[(size_t)k * S.cap + j];
-    }
-    q.E = 0.0;
-    q.G = S.G[j];
-    q.isMG = 1;
-    q.type = 1;
-    q.pad0 = 0;
-    q.time = S.t[j];
-    q.lambda = SCN_INF;
-    q.matIdx = S.mat[j];
-    q.cellIdx = 0;
-    q.uniqueID = S.uid[j];
-    q.collisionN = S.coll[j];
-    q.broodID = b;
-    q.pad1 = 0;
-    out[dst] = q;
-}
-
-// keffAnalogClerk (keffAnalogClerk_class.f90:162-181) or keffImplicitClerk (:306-326) close of the
-// cycle, after the package's stride of the master state by pop + 1
-__global__ void k_dcycle_keff(EigCtl* ctl, const double* __restrict__ sums, int phase)
-{
-    if (threadIdx.x != 0) return;
-    ctl->master = rng_stride(ctl->master, (int64_t)ctl->pop + 1);
-    const double startW = sums[0], endW = sums[1];
-    const double k_norm = ctl->dk[1 - ctl->cur];
-    if (phase == 0) {
-        const double k_eff = endW / startW * k_norm;
-        ctl->kA[0] = ctl->kA[0] + k_eff;
-        ctl->kA[1] = ctl->kA[1] + k_eff * k_eff;
-        ctl->nA++;
-    } else {
-        const double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
-        ctl->kI[0] = ctl->kI[0] + k_est;
-        ctl->kI[1] = ctl->kI[1] + k_est * k_est;
-        ctl->nI++;
-        for (int k = 0; k < 4; ++k) {
-            ctl->imp[k][0] = ctl->imp[k][0] + sums[2 + k];
-            ctl->imp[k][1] = ctl->imp[k][1] + sums[2 + k] * sums[2 + k];
-        }
-    }
-}
-
-// normSize_Repr plan from the device site count (particleDungeon_class.f90:431-470)
-__global__ void k_norm_plan(EigCtl* ctl, int* hist, DevStatus* st)
-{
-    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x != 0) return;
-    const int n = ctl->n_sites;
-    if (n == 0) raise_error(st, SCN_ERR_NORM, 0);
-    const int excess = n - ctl->pop;
-    ctl->excess = excess;
-    ctl->heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
-    ctl->n_copies = (n == 0 || excess >= 0) ? 0 : (-excess) / n;
-    ctl->n_dup = (n == 0 || excess >= 0) ? 0 : (-excess) % n;
-    ctl->rs_k = ctl->heap;
-    ctl->rs_prefix = 0;
-    ctl->rs_mask = 0;
-}
-
-__global__ void k_site_rn_dev(const EigCtl* ctl, double* rn, uint64_t* rn_bits)
+__global__ void k_bank_weights(const scn_particle_state* __restrict__ bank, int n, double* __restrict__ w)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ctl->n_sites || ctl->heap == 0) return;
-    uint64_t s = rng_skip(ctl->master, (int64_t)j);
-    double u = scn_rng_get(&s);
-    rn[j] = u;
-    rn_bits[j] = scn_dbits(u);
+    if (j < n) w[j] = bank[j].wgt;
 }
 
-// radix select of the heap-th smallest draw, 8 bits a pass from the top: histogram of the digit
-// among the draws that match the digits chosen so far ...
-__global__ void k_rs_hist(const uint64_t* __restrict__ keys, const EigCtl* ctl, int* hist, int pass)
+// window [lo, hi) that holds the k-th smallest of n uniform draws with overwhelming probability:
+// its expected value k/n plus or minus s standard deviations of the order statistic and 2s
+// spacings (s = 8 by default); the selection is exact either way (a miss falls back to all draws)
+__device__ __forceinline__ void sel_window(int k, int n, double s, double* lo, double* hi)
 {
-    __shared__ int h[256];
-    if (ctl->heap == 0) return;
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int n = ctl->n_sites;
-    const uint64_t mask = ctl->rs_mask, prefix = ctl->rs_prefix;
-    const int shift = 56 - 8 * pass;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint64_t k = keys[j];
-        if ((k & mask) == prefix) atomicAdd(&h[(int)((k >> shift) & 255)], 1);
-    }
-    __syncthreads();
-    const int c = h[threadIdx.x];
-    if (c) atomicAdd(&hist[pass * 256 + threadIdx.x], c);
-}
-
-// ... then the digit whose cumulative count reaches the rank (one block of 256 threads)
-__global__ void k_rs_pick(EigCtl* ctl, const int* hist, int pass)
-{
-    __shared__ int h[256];
-    if (ctl->heap == 0) return;
-    h[threadIdx.x] = hist[pass * 256 + threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int shift = 56 - 8 * pass;
-    int k = ctl->rs_k, cum = 0;
-    for (int d = 0; d < 256; ++d) {
-        if (cum + h[d] >= k) {
-            ctl->rs_k = k - cum;
-            ctl->rs_prefix |= (uint64_t)d << shift;
-            ctl->rs_mask |= (uint64_t)255 << shift;
-            return;
-        }
-        cum += h[d];
-    }
-}
-
-__global__ void k_norm_flags_dev(const double* __restrict__ rn, const EigCtl* ctl, int* flag)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ctl->n_sites) return;
-    const double thr = scn_bitsd(ctl->rs_prefix);
-    const int ex = ctl->excess;
-    flag[j] = ex > 0 ? (rn[j] > thr) : (ex < 0 && ctl->n_dup != 0) ? (rn[j] <= thr) : 0;
+    const double p = (double)k / (double)n;
+    const double h = s * (sqrt(p * (1.0 - p) / (double)n) + 2.0 / (double)n);
+    *lo = p - h;
+    *hi = p + h;
 }
 
 // exclusive prefix sum of n ints (n in device memory), 1024 values per block: block scans, a scan
@@ -504,71 +381,375 @@ __global__ void __launch_bounds__(1024) k_scan_top(int* bsum, const int* n_ptr)
     }
 }
 
-__global__ void k_scan_add(int* out, const int* bsum, const int* n_ptr)
+// history b's sites start at the exclusive prefix sum of the site counts of histories 1..b-1
+// (sortByBroodID of the detain order, without a sort); block sums of that scan are added here.
+// Each site's position in the next bank: its weight goes to a contiguous array for the END sum
+// and, when normSize_Repr must draw (heap > 0), the draw of that position
+// (particleDungeon_class.f90:471-487: one number per site from the master stream skipped to it,
+// after the package's stride of pop + 1).  Draws below the selection window are counted, draws
+// inside it gathered.
+__global__ void __launch_bounds__(256) k_site_index(Sites S, const int* __restrict__ site_off,
+                                                    const int* __restrict__ bsum, int pop, int out_cap, EigCtl* ctl,
+                                                    double* __restrict__ w_out, double* __restrict__ rn,
+                                                    double* __restrict__ cand, int cand_cap, DevStatus* st)
 {
+    __shared__ int below;
+    __shared__ uint64_t master;   // the master state after the package's stride of pop + 1
+    __shared__ double lo, hi;
+    const int count = *S.count;
+    const int n = min(count, min(S.cap, out_cap));
+    const int excess = n - pop;
+    const int heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= *n_ptr) return;
-    out[j] += bsum[j / SCAN_PER_BLOCK];
+    if (j == 0) {
+        ctl->n_sites = n;
+        ctl->sites_total += (unsigned long long)n;
+        if (count > n) raise_error(st, SCN_ERR_OVERFLOW, count);
+    }
+    if (threadIdx.x == 0) {
+        below = 0;
+        master = rng_stride(ctl->master, (int64_t)pop + 1);
+        if (heap) sel_window(heap, n, ctl->sel_sigmas, &lo, &hi);
+    }
+    __syncthreads();
+    if (j < n) {
+        const uint64_t key = S.key[j];
+        const int b = (int)(key >> 24);
+        if (b >= 1 && b <= pop) {
+            const int64_t dst = (int64_t)site_off[b - 1] + bsum[(b - 1) / SCAN_PER_BLOCK] + (int64_t)(key & 0xFFFFFF);
+            if (dst < n) {
+                w_out[dst] = S.w[j];
+                if (heap) {
+                    uint64_t s = rng_skip(master, dst);
+                    const double u = scn_rng_get(&s);
+                    rn[dst] = u;
+                    if (u < lo) {
+                        atomicAdd(&below, 1);
+                    } else if (u < hi) {
+                        const int i = atomicAdd(&ctl->n_cand, 1);
+                        if (i < cand_cap) cand[i] = u;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && below) atomicAdd(&ctl->n_below, below);
 }
 
-__global__ void k_brood_bounds_dev(const scn_particle_state* in, const EigCtl* ctl, int max_brood, int* start_of,
-                                   int* end_of)
+// canonical sum, first stage with coalesced loads: block b of a job sums chunks 256b .. 256b + 255
+// (SCN_DSUM_CHUNK consecutive values each, in order) and pairs them 8 levels deep, which is
+// element b of level 8 of the tree (pairs never straddle two blocks: a block's first element is
+// even at every level below 9, and the odd element a level carries alone is its last, which lies
+// in the last block).  m_tab[job] receives level 8's size.
+__global__ void __launch_bounds__(256) k_dsum_tree(SumJobs J, double* y, int64_t ystride, int64_t* m_tab)
 {
-    const int n = ctl->n_sites;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n || ctl->excess >= 0) return;
-    const int b = in[j].broodID;
-    if (b < 0 || b > max_brood) return;
-    if (j == 0 || in[j - 1].broodID != b) start_of[b] = j;
-    if (j == n - 1 || in[j + 1].broodID != b) end_of[b] = j + 1;
+    constexpr int HALF = SCN_DSUM_CHUNK / 2;
+    __shared__ double x[256 * (HALF + 1)];
+    __shared__ double a[256];
+    const int job = blockIdx.y;
+    const SumJob& q = J.j[job];
+    int64_t n = q.n;
+    if (q.n_dev) n = min((int64_t)max(*q.n_dev, 0), q.n);
+    const int64_t m = (n + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    if (blockIdx.x == 0 && threadIdx.x == 0) m_tab[job] = (m + 255) / 256;
+    const int64_t c0 = (int64_t)blockIdx.x * 256;
+    if (c0 >= m) return;   // uniform over the block
+    const int64_t v0 = c0 * SCN_DSUM_CHUNK;
+    const int64_t c = c0 + threadIdx.x;
+    const int64_t e = c < m ? min((c + 1) * SCN_DSUM_CHUNK, n) - c * SCN_DSUM_CHUNK : 0;   // values of chunk c
+    double s = 0.0;
+    for (int half = 0; half < 2; ++half) {   // the chunk's first 16 values, then its last 16
+        for (int k = 0; k < HALF; ++k) {
+            const int l = k * 256 + threadIdx.x;           // 16 consecutive values of one chunk per 16 threads
+            const int ch = l / HALF, off = l % HALF;
+            const int64_t i = v0 + (int64_t)ch * SCN_DSUM_CHUNK + half * HALF + off;
+            x[ch * (HALF + 1) + off] = i < n ? q.x[i * q.stride] : 0.0;
+        }
+        __syncthreads();
+        const double* xc = x + threadIdx.x * (HALF + 1);
+        for (int i = 0; i < HALF; ++i)
+            if (half * HALF + i < e) s = s + xc[i];
+        __syncthreads();
+    }
+    a[threadIdx.x] = s;
+    __syncthreads();
+    int64_t ml = m;   // size of the level being paired
+    for (int lev = 0; lev < 8; ++lev) {
+        const int64_t base = c0 >> lev;               // this block's first element at level lev
+        const int64_t h = 128 >> lev;                 // outputs of this block at level lev + 1
+        double v = 0.0;
+        const int i = threadIdx.x;
+        if (i < h) {
+            const int64_t g = base + 2 * i;           // global index at level lev
+            if (g < ml) v = (g + 1 < ml) ? a[2 * i] + a[2 * i + 1] : a[2 * i];
+        }
+        __syncthreads();
+        if (i < h) a[i] = v;
+        __syncthreads();
+        ml = (ml + 1) / 2;
+    }
+    if (threadIdx.x == 0) y[job * ystride + blockIdx.x] = a[0];
 }
 
-// keep (excess > 0), copy and duplicate (excess < 0) or pass through (excess = 0)
-__global__ void k_norm_apply(const scn_particle_state* __restrict__ in, const int* flag, const int* pos,
-                             const int* start_of, const int* end_of, const EigCtl* ctl, int max_brood,
-                             scn_particle_state* __restrict__ out, int out_cap)
+// close of a cycle's scalars in one block: the canonical sums' last levels for every job
+// (START weight, END weight, the 4 keffImplicit scores), then keffAnalogClerk
+// (keffAnalogClerk_class.f90:162-181) or keffImplicitClerk (:306-326) after the package's stride
+// of the master state by pop + 1, and the normSize_Repr plan from the site count
+// (particleDungeon_class.f90:431-470)
+__global__ void __launch_bounds__(1024) k_dcycle_close(const double* y, const int64_t* m_in, int64_t ystride,
+                                                       int njobs, EigCtl* ctl, int phase, DevStatus* st)
 {
-    const int n = ctl->n_sites;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const int ex = ctl->excess;
-    if (ex == 0) {
-        if (j < out_cap) out[j] = in[j];
-    } else if (ex > 0) {
-        if (flag[j] && pos[j] < out_cap) out[pos[j]] = in[j];
+    __shared__ double a[2048];
+    __shared__ double b[2048];
+    __shared__ double sums[8];
+    for (int job = 0; job < njobs; ++job) {
+        int64_t m = m_in[job];
+        for (int i = threadIdx.x; i < m; i += blockDim.x) a[i] = y[job * ystride + i];
+        __syncthreads();
+        double* src = a;
+        double* dst = b;
+        while (m > 1) {
+            const int64_t h = (m + 1) / 2;
+            for (int i = threadIdx.x; i < h; i += blockDim.x)
+                dst[i] = (2 * i + 1 < m) ? src[2 * i] + src[2 * i + 1] : src[2 * i];
+            __syncthreads();
+            double* t = src;
+            src = dst;
+            dst = t;
+            m = h;
+        }
+        if (threadIdx.x == 0) sums[job] = (m_in[job] > 0) ? src[0] : 0.0;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    ctl->master = rng_stride(ctl->master, (int64_t)ctl->pop + 1);
+    const double startW = sums[0], endW = sums[1];
+    const double k_norm = ctl->dk[1 - ctl->cur];
+    if (phase == 0) {
+        const double k_eff = endW / startW * k_norm;
+        ctl->kA[0] = ctl->kA[0] + k_eff;
+        ctl->kA[1] = ctl->kA[1] + k_eff * k_eff;
+        ctl->nA++;
     } else {
-        const int b = in[j].broodID;
-        if (b < 0 || b > max_brood) return;
-        const int s = start_of[b];
-        const int kb = end_of[b] - s;
-        if (s < 0 || s > j || kb <= 0 || s + kb > n) return;   // only after an error left the bank incomplete
+        const double k_est = sums[2] / (sums[4] + sums[5] - sums[3]);
+        ctl->kI[0] = ctl->kI[0] + k_est;
+        ctl->kI[1] = ctl->kI[1] + k_est * k_est;
+        ctl->nI++;
+        for (int k = 0; k < 4; ++k) {
+            ctl->imp[k][0] = ctl->imp[k][0] + sums[2 + k];
+            ctl->imp[k][1] = ctl->imp[k][1] + sums[2 + k] * sums[2 + k];
+        }
+    }
+    const int n = ctl->n_sites;
+    if (n == 0) raise_error(st, SCN_ERR_NORM, 0);
+    const int excess = n - ctl->pop;
+    ctl->excess = excess;
+    ctl->heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
+    ctl->n_copies = (n == 0 || excess >= 0) ? 0 : (-excess) / n;
+    ctl->n_dup = (n == 0 || excess >= 0) ? 0 : (-excess) % n;
+}
+
+// digit (0..255) of a 256-bin histogram h where the cumulative count reaches k, and the rank
+// left within it; the first 256 threads of the block scan, all threads get the result
+__device__ void pick_digit(const int* h, int k, int* digit, int* rank)
+{
+    __shared__ int c[256];
+    __shared__ int res[2];
+    const int t = threadIdx.x;
+    const int v = t < 256 ? h[t] : 0;
+    if (t < 256) c[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int x = (t < 256 && t >= off) ? c[t - off] : 0;
+        __syncthreads();
+        if (t < 256) c[t] += x;
+        __syncthreads();
+    }
+    if (t < 256) {
+        const int incl = c[t], excl = incl - v;
+        if (excl < k && k <= incl) {
+            res[0] = t;
+            res[1] = k - excl;
+        }
+    }
+    __syncthreads();
+    *digit = res[0];
+    *rank = res[1];
+    __syncthreads();
+}
+
+// the heap-th smallest draw, exactly, in one block of 1024 threads.  When the window held it (the
+// draws below it fewer than heap, with the window's draws at least heap) and every draw inside was
+// gathered, it is the (heap - below)-th smallest of the gathered draws: by counting ranks when they
+// fit in LDS, otherwise by a radix select on their bit patterns (non-negative doubles order as
+// their bits); otherwise the radix select runs over all draws
+constexpr int PICK_LDS = 2048;
+__global__ void __launch_bounds__(1024) k_sel_pick(const double* __restrict__ rn, EigCtl* ctl,
+                                                   const double* __restrict__ cand, int cand_cap)
+{
+    __shared__ double v[PICK_LDS];
+    __shared__ int h[256];
+    if (ctl->heap == 0) return;
+    const int heap = ctl->heap, nb = ctl->n_below, nc = ctl->n_cand;
+    const bool in_window = nb < heap && heap <= nb + nc && nc <= cand_cap;
+    if (in_window && nc <= PICK_LDS) {
+        const int k = heap - nb;
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) v[i] = cand[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            const double x = v[i];
+            int less = 0, leq = 0;
+            for (int q = 0; q < nc; ++q) {
+                less += v[q] < x;
+                leq += v[q] <= x;
+            }
+            if (less < k && k <= leq) ctl->thr = x;   // equal draws write the same value
+        }
+        return;
+    }
+    const double* src = in_window ? cand : rn;
+    const int n = in_window ? nc : ctl->n_sites;
+    int k = in_window ? heap - nb : heap;
+    uint64_t prefix = 0, mask = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        if (threadIdx.x < 256) h[threadIdx.x] = 0;
+        __syncthreads();
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint64_t x = scn_dbits(src[j]);
+            if ((x & mask) == prefix) atomicAdd(&h[(int)((x >> shift) & 255)], 1);
+        }
+        __syncthreads();
+        int d, r;
+        pick_digit(h, k, &d, &r);
+        prefix |= (uint64_t)d << shift;
+        mask |= (uint64_t)255 << shift;
+        k = r;
+    }
+    if (threadIdx.x == 0) ctl->thr = scn_bitsd(prefix);
+}
+
+// normSize_Repr's selection flags (keep the draws above the threshold when there are too many
+// sites, duplicate those at or below it when there are too few; particleDungeon_class.f90:488-
+// 560) and their exclusive prefix sum within blocks of SCAN_PER_BLOCK, block totals to bsum
+__global__ void __launch_bounds__(256) k_norm_flag_scan(const double* __restrict__ rn, const EigCtl* ctl,
+                                                        int* __restrict__ flag, int* __restrict__ pos, int* bsum)
+{
+    __shared__ int w[256];
+    const int n = ctl->n_sites;
+    const int base = blockIdx.x * SCAN_PER_BLOCK;
+    if (base >= n) return;
+    const double thr = ctl->thr;
+    const int ex = ctl->excess;
+    const bool dup = ex < 0 && ctl->n_dup != 0;
+    int v[4], t = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int j = base + threadIdx.x * 4 + k;
+        v[k] = 0;
+        if (j < n) {
+            v[k] = ex > 0 ? (rn[j] > thr) : dup ? (rn[j] <= thr) : 0;
+            flag[j] = v[k];
+        }
+        t += v[k];
+    }
+    w[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int x = threadIdx.x >= off ? w[threadIdx.x - off] : 0;
+        __syncthreads();
+        w[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int run = w[threadIdx.x] - t;
+    for (int k = 0; k < 4; ++k) {
+        const int j = base + threadIdx.x * 4 + k;
+        if (j < n) pos[j] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = w[255];
+}
+
+// the next cycle's bank, straight from the site buffer: keep (excess > 0), copies and duplicates
+// (excess < 0, each history's sites as a group: (n_copies + 1) copies of the group, then its
+// duplicated sites) or all (excess = 0), with the weights to a contiguous array for the next
+// cycle's START sum
+__global__ void __launch_bounds__(256) k_norm_out(Sites S, const int* __restrict__ site_off,
+                                                  const int* __restrict__ bsum_site, const int* __restrict__ hist_nsite,
+                                                  const int* __restrict__ flag, const int* __restrict__ pos,
+                                                  const int* __restrict__ bsum_pos, const EigCtl* ctl, int pop,
+                                                  scn_particle_state* __restrict__ out, double* __restrict__ w_out,
+                                                  int out_cap)
+{
+    const int n = ctl->n_sites;
+    const int count = min(*S.count, S.cap);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const uint64_t key = S.key[j];
+    const int b = (int)(key >> 24);
+    if (b < 1 || b > pop) return;
+    const int64_t s = (int64_t)site_off[b - 1] + bsum_site[(b - 1) / SCAN_PER_BLOCK];
+    const int64_t e = (int64_t)(key & 0xFFFFFF);
+    const int64_t dst = s + e;
+    if (dst >= n) return;
+    scn_particle_state q;
+    q.wgt = S.w[j];
+    for (int k = 0; k < 3; ++k) {
+        q.r[k] = S.r[(size_t)k * S.cap + j];
+        q.dir[k] = S.u[(size_t)k * S.cap + j];
+    }
+    q.E = 0.0;
+    q.G = S.G[j];
+    q.isMG = 1;
+    q.type = 1;
+    q.pad0 = 0;
+    q.time = S.t[j];
+    q.lambda = SCN_INF;
+    q.matIdx = S.mat[j];
+    q.cellIdx = 0;
+    q.uniqueID = S.uid[j];
+    q.collisionN = S.coll[j];
+    q.broodID = b;
+    q.pad1 = 0;
+    const int ex = ctl->excess;
+    auto put = [&](int64_t o) {
+        if (o >= 0 && o < out_cap) {
+            out[o] = q;
+            w_out[o] = q.wgt;
+        }
+    };
+    if (ex == 0) {
+        put(dst);
+    } else if (ex > 0) {
+        if (flag[dst]) put((int64_t)pos[dst] + bsum_pos[dst / SCAN_PER_BLOCK]);
+    } else {
+        const int64_t kb = min((int64_t)hist_nsite[b - 1], (int64_t)n - s);   // the group's sites in the bank
         const int nc = ctl->n_copies;
-        const int64_t base = (int64_t)(nc + 1) * s + pos[s];
-        const int rho = j - s;
-        for (int c = 0; c <= nc; ++c) {
-            const int64_t o = base + (int64_t)c * kb + rho;
-            if (o < out_cap) out[o] = in[j];
-        }
-        if (flag[j]) {
-            const int64_t o = base + (int64_t)(nc + 1) * kb + (pos[j] - pos[s]);
-            if (o < out_cap) out[o] = in[j];
-        }
+        const int64_t ps = (int64_t)pos[s] + bsum_pos[s / SCAN_PER_BLOCK];
+        const int64_t base = (int64_t)(nc + 1) * s + ps;
+        for (int c = 0; c <= nc; ++c) put(base + (int64_t)c * kb + e);
+        if (flag[dst]) put(base + (int64_t)(nc + 1) * kb + ((int64_t)pos[dst] + bsum_pos[dst / SCAN_PER_BLOCK] - ps));
     }
 }
 
 // end of the cycle: population check, the package's stride by 1, k for the next cycle
-// (eigenPhysicsPackage_class.f90:262-310)
-__global__ void k_dcycle_end(EigCtl* ctl, const int* flag, const int* pos, double* k_hist, int k_hist_cap, int phase,
-                             DevStatus* st)
+// (eigenPhysicsPackage_class.f90:262-310); the transport's counters, the site count and the draw
+// histogram are cleared for the next cycle
+__global__ void k_dcycle_end(EigCtl* ctl, const int* flag, const int* pos, const int* bsum_pos, double* k_hist,
+                             int k_hist_cap, int phase, int* counters_i, int* site_count, DevStatus* st)
 {
+    if (threadIdx.x < 4) counters_i[threadIdx.x] = 0;
     if (threadIdx.x != 0) return;
+    *site_count = 0;
     const int n = ctl->n_sites;
     if (n > 0) {
         const int ex = ctl->excess;
-        const int last = flag[n - 1] + pos[n - 1];
+        const int last = flag[n - 1] + pos[n - 1] + bsum_pos[(n - 1) / SCAN_PER_BLOCK];
         const int n_new = ex == 0 ? n : ex > 0 ? last : n * (ctl->n_copies + 1) + last;
         if (n_new != ctl->pop) raise_error(st, SCN_ERR_NORM, n_new);
     }
+    ctl->n_below = 0;
+    ctl->n_cand = 0;
     ctl->master = rng_stride(ctl->master, 1);
     ctl->cur = 1 - ctl->cur;
     const double k_new = phase == 0 ? ctl->kA[0] / ctl->nA : ctl->kI[0] / ctl->nI;
@@ -726,12 +907,17 @@ struct Engine {
     scn_stats stats{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // device-resident cycles (cycle_graph): control record, per-history site counts and offsets,
-    // scan and radix-select scratch, one captured graph per phase (0 inactive, 1 active)
+    // scan block sums, contiguous bank and site weights, the draw histogram and candidates of the
+    // threshold selection, one captured graph per phase (0 inactive, 1 active)
     EigCtl* ctl = nullptr;
     int* hist_nsite = nullptr;
     int* site_off = nullptr;
     int* scan_bsum = nullptr;
-    int* rs_hist = nullptr;
+    int* scan_bsum2 = nullptr;
+    double* wgt_bank = nullptr;
+    double* wgt_sites = nullptr;
+    double* sel_cand = nullptr;
+    int sel_cap = 0;
     int64_t* sum_mtab = nullptr;
     double* k_hist_dev = nullptr;
     int k_hist_cap = 0;
@@ -1182,25 +1368,18 @@ int norm_size_device(Engine* E, int n, int tot_pop, uint64_t master, int* n_new)
 
 
 // ---- device-resident power iteration: host side --------------------------------------------------
-// exclusive prefix sum of n ints (n in device memory, at most cap)
-void scan_dev(Engine* E, const int* in, int* out, const int* n_ptr, int cap)
-{
-    const int nb = (cap + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
-    hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(1, nb)), dim3(256), 0, E->stream, in, out, E->scan_bsum, n_ptr);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, E->stream, E->scan_bsum, n_ptr);
-    hipLaunchKernelGGL(k_scan_add, dim3((cap + 255) / 256), dim3(256), 0, E->stream, out, (const int*)E->scan_bsum, n_ptr);
-}
-
 // canonical sums (the tree of dsum_batch) with counts in device memory; J.j[q].n are capacities.
-// Results in E->sum_out[0..njobs).  No synchronisation.
-void dsum_dev(Engine* E, const SumJobs& J)
+// Level 8 of the tree comes out of k_dsum_tree, the rest is paired here while more than 2048
+// values remain and finished by k_dcycle_close.  Returns the row of sum_mtab with the last sizes.
+int dsum_dev_levels(Engine* E, const SumJobs& J, double** last)
 {
     int64_t maxn = 0;
     for (int q = 0; q < J.njobs; ++q) maxn = std::max(maxn, J.j[q].n);
     const int64_t ys = E->sum_cap;
-    int64_t mm = (maxn + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
-    hipLaunchKernelGGL(k_dsum_chunks, dim3((unsigned)std::max<int64_t>(1, (mm + 255) / 256), J.njobs), dim3(256), 0,
+    const int64_t m0 = (maxn + SCN_DSUM_CHUNK - 1) / SCN_DSUM_CHUNK;
+    hipLaunchKernelGGL(k_dsum_tree, dim3((unsigned)std::max<int64_t>(1, (m0 + 255) / 256), J.njobs), dim3(256), 0,
                        E->stream, J, E->sum_y, ys, E->sum_mtab);
+    int64_t mm = (m0 + 255) / 256;
     double* a = E->sum_y;
     double* b = E->sum_y2;
     int l = 0;
@@ -1210,10 +1389,11 @@ void dsum_dev(Engine* E, const SumJobs& J)
                            E->sum_mtab, l, ys);
         std::swap(a, b);
     }
-    hipLaunchKernelGGL(k_dsum_final, dim3(J.njobs), dim3(1024), 0, E->stream, a, E->sum_mtab + 8 * l, ys, E->sum_out);
+    *last = a;
+    return l;
 }
 
-// one cycle as a fixed launch sequence on E->stream (captured into a graph by dcycle_capture)
+// one cycle as a fixed launch sequence on E->stream (captured into a graph by dcycle_setup)
 int enqueue_dcycle(Engine* E, int phase, int score_bins)
 {
     const int pop = E->eig.S.pop;
@@ -1227,47 +1407,47 @@ int enqueue_dcycle(Engine* E, int phase, int score_bins)
     DevProblem P = E->P;
     if (!score_bins) P.n_clerks = 0;
     EigCtl* c = E->ctl;
-    hipLaunchKernelGGL(k_dcycle_begin, dim3(1), dim3(64), 0, E->stream, E->counters_i, E->sites.count);
+    // the transport's queue counters and the site count were cleared by the previous cycle's end
     fn(dim3(grid), dim3(BLOCK), sh, E->stream, P, E->bankA, 1, pop, E->live[0], E->counters_i, E->live[1],
        E->counters_i + 1, E->cap, 1 << 30, 0, 0.0, E->sites, E->bins, use_lds, E->hist_k, E->cap, E->counters,
        E->status, E->counters_i + 2, E->chunk, CycleIn{&c->master, &c->k_new, E->hist_nsite, E->refill});
-    // next bank in (broodID, emission) order
-    scan_dev(E, E->hist_nsite, E->site_off, &c->pop, E->cap);
+    // offsets of each history's sites in the next bank (broodID, emission order), their weights
+    // and, when the population must be normalised, their draws
+    const int hb = (pop + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(1, hb)), dim3(256), 0, E->stream, (const int*)E->hist_nsite,
+                       E->site_off, E->scan_bsum, (const int*)&c->pop);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, E->stream, E->scan_bsum, (const int*)&c->pop);
     const int sg = (E->site_cap + 255) / 256;
-    hipLaunchKernelGGL(k_place_sites, dim3(sg), dim3(256), 0, E->stream, E->sites, (const int*)E->site_off, pop,
-                       E->bank_cap, E->bankB, c, E->status);
-    // START / END population weights and the keffImplicit scores, then the k estimates
+    hipLaunchKernelGGL(k_site_index, dim3(sg), dim3(256), 0, E->stream, E->sites, (const int*)E->site_off,
+                       (const int*)E->scan_bsum, pop, E->bank_cap, c, E->wgt_sites, E->rn, E->sel_cand, E->sel_cap,
+                       E->status);
+    // START / END population weights and the keffImplicit scores, the k estimates, the plan
     SumJobs J{};
     J.njobs = 6;
-    const int64_t rec = (int64_t)(sizeof(scn_particle_state) / sizeof(double));
-    J.j[0] = {&E->bankA[0].wgt, pop, rec, nullptr};
-    J.j[1] = {&E->bankB[0].wgt, E->bank_cap, rec, &c->n_sites};
+    J.j[0] = {E->wgt_bank, pop, 1, nullptr};
+    J.j[1] = {E->wgt_sites, E->bank_cap, 1, &c->n_sites};
     for (int k = 0; k < 4; ++k) J.j[2 + k] = {E->hist_k + (size_t)k * E->cap, pop, 1, nullptr};
-    dsum_dev(E, J);
-    hipLaunchKernelGGL(k_dcycle_keff, dim3(1), dim3(64), 0, E->stream, c, (const double*)E->sum_out, phase);
+    double* last = nullptr;
+    const int l = dsum_dev_levels(E, J, &last);
+    hipLaunchKernelGGL(k_dcycle_close, dim3(1), dim3(1024), 0, E->stream, (const double*)last,
+                       (const int64_t*)(E->sum_mtab + 8 * l), E->sum_cap, J.njobs, c, phase, E->status);
     if (score_bins) {
         int rc = close_bins(E, phase == 1 ? 0 : 1);
         if (rc) return rc;
     }
-    // normSize_Repr
-    hipLaunchKernelGGL(k_norm_plan, dim3(1), dim3(256), 0, E->stream, c, E->rs_hist, E->status);
-    hipLaunchKernelGGL(k_site_rn_dev, dim3(sg), dim3(256), 0, E->stream, (const EigCtl*)c, E->rn, E->rn_bits);
-    const int hg = std::max(1, std::min(sg, E->n_cu * 4));
-    for (int pass = 0; pass < 8; ++pass) {
-        hipLaunchKernelGGL(k_rs_hist, dim3(hg), dim3(256), 0, E->stream, (const uint64_t*)E->rn_bits, (const EigCtl*)c,
-                           E->rs_hist, pass);
-        hipLaunchKernelGGL(k_rs_pick, dim3(1), dim3(256), 0, E->stream, c, (const int*)E->rs_hist, pass);
-    }
-    hipLaunchKernelGGL(k_norm_flags_dev, dim3(sg), dim3(256), 0, E->stream, (const double*)E->rn, (const EigCtl*)c,
-                       E->flag);
-    scan_dev(E, E->flag, E->pos, &c->n_sites, E->site_cap);
-    hipLaunchKernelGGL(k_brood_bounds_dev, dim3(sg), dim3(256), 0, E->stream, (const scn_particle_state*)E->bankB,
-                       (const EigCtl*)c, E->cap + 1, E->start_of, E->end_of);
-    hipLaunchKernelGGL(k_norm_apply, dim3(sg), dim3(256), 0, E->stream, (const scn_particle_state*)E->bankB,
-                       (const int*)E->flag, (const int*)E->pos, (const int*)E->start_of, (const int*)E->end_of,
-                       (const EigCtl*)c, E->cap + 1, E->bankA, E->bank_cap);
-    hipLaunchKernelGGL(k_dcycle_end, dim3(1), dim3(64), 0, E->stream, c, (const int*)E->flag, (const int*)E->pos,
-                       E->k_hist_dev, E->k_hist_cap, phase, E->status);
+    // normSize_Repr: the heap-th smallest draw, the flags, the next bank
+    hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(1024), 0, E->stream, (const double*)E->rn, c,
+                       (const double*)E->sel_cand, E->sel_cap);
+    const int fb = (E->bank_cap + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+    hipLaunchKernelGGL(k_norm_flag_scan, dim3(std::max(1, fb)), dim3(256), 0, E->stream, (const double*)E->rn,
+                       (const EigCtl*)c, E->flag, E->pos, E->scan_bsum2);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, E->stream, E->scan_bsum2, (const int*)&c->n_sites);
+    hipLaunchKernelGGL(k_norm_out, dim3(sg), dim3(256), 0, E->stream, E->sites, (const int*)E->site_off,
+                       (const int*)E->scan_bsum, (const int*)E->hist_nsite, (const int*)E->flag, (const int*)E->pos,
+                       (const int*)E->scan_bsum2, (const EigCtl*)c, pop, E->bankA, E->wgt_bank, E->bank_cap);
+    hipLaunchKernelGGL(k_dcycle_end, dim3(1), dim3(256), 0, E->stream, c, (const int*)E->flag, (const int*)E->pos,
+                       (const int*)E->scan_bsum2, E->k_hist_dev, E->k_hist_cap, phase, E->counters_i,
+                       E->sites.count, E->status);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1291,7 +1471,13 @@ int dcycle_setup(Engine* E, const scn_eigen_settings& S)
         rc |= dalloc(E, &E->hist_nsite, E->cap);
         rc |= dalloc(E, &E->site_off, E->cap);
         rc |= dalloc(E, &E->scan_bsum, (size_t)std::max(E->site_cap, E->cap) / SCAN_PER_BLOCK + 2);
-        rc |= dalloc(E, &E->rs_hist, 8 * 256);
+        rc |= dalloc(E, &E->scan_bsum2, (size_t)E->bank_cap / SCAN_PER_BLOCK + 2);
+        rc |= dalloc(E, &E->wgt_bank, (size_t)E->bank_cap);
+        rc |= dalloc(E, &E->wgt_sites, (size_t)E->bank_cap);
+        // the selection window holds at most 8 sqrt(n) + 32 draws on average: room for twice that
+        // (more falls back to a pass over all draws, with the same result)
+        E->sel_cap = std::max(1 << 16, (int)(16.0 * std::sqrt((double)E->bank_cap)) + 4096);
+        rc |= dalloc(E, &E->sel_cand, (size_t)E->sel_cap);
         rc |= dalloc(E, &E->sum_mtab, (size_t)8 * (SCN_DSUM_MAX_LEVELS + 2));
         if (rc) return rc;
     }
@@ -1317,7 +1503,15 @@ int dcycle_setup(Engine* E, const scn_eigen_settings& S)
     h.dk[1] = X.dk[1];
     h.cur = X.cur;
     h.pop = S.pop;
+    h.sel_sigmas = S.select_window > 0.0 ? S.select_window : 8.0;
     COPY_SYNC(E->ctl, &h, sizeof h, hipMemcpyHostToDevice);
+    // state the first captured cycle expects: cleared counters (the selection counts start at zero
+    // in the control record), the weights of the starting bank
+    hipLaunchKernelGGL(k_dcycle_begin, dim3(1), dim3(64), 0, E->stream, E->counters_i, E->sites.count);
+    hipLaunchKernelGGL(k_bank_weights, dim3((S.pop + 255) / 256), dim3(256), 0, E->stream,
+                       (const scn_particle_state*)E->bankA, S.pop, E->wgt_bank);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->stream));
     dcycle_release_graphs(E);
     for (int ph = 0; ph < 2; ++ph) {
         const int score_bins = (ph == 1 || S.inactive_tally) && E->n_bins > 0;

@@ -181,6 +181,7 @@ class Engine:
         S.block_size = settings.get("block_size", 0)
         S.cycle_graph = settings.get("cycle_graph", 0)
         S.refill = settings.get("refill", 0)
+        S.select_window = settings.get("select_window", 0.0)
         return S
 
     def _result(self, S):

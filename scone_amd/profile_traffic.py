@@ -79,7 +79,7 @@ def run(args):
         comp = T.compile_problem(inp, mg)
         eng = Engine(comp, pop)
         bank = eng.fission_source(pop, seed)
-        eng.eigen_start(dict(pop=pop, inactive=0, active=args.cycles, seed=seed), bank)
+        eng.eigen_start(dict(pop=pop, inactive=0, active=args.cycles, seed=seed, cycle_graph=args.cycle_graph), bank)
         eng.eigen_step(args.cycles)
         st = eng.stats()
         eng.eigen_finish()
@@ -151,6 +151,7 @@ def main():
     r.add_argument("--n-launch", type=int, default=0)
     r.add_argument("--window", type=int, default=1, help="batch512: batches of every problem per cycle")
     r.add_argument("--refill", type=int, default=0, help="batch512: idle lanes gathered before new histories start")
+    r.add_argument("--cycle-graph", type=int, default=0, help="eigen: replay the device-resident cycles from hipGraphs")
     s = sub.add_parser("summarize")
     s.add_argument("fetch_dir")
     s.add_argument("write_dir")

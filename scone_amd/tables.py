@@ -122,7 +122,7 @@ class EigenSettings(C.Structure):
     _fields_ = [("pop", C.c_int32), ("n_inactive", C.c_int32), ("n_active", C.c_int32),
                 ("inactive_tally", C.c_int32), ("seed", C.c_int64), ("keff0", C.c_double),
                 ("events_per_launch", C.c_int32), ("block_size", C.c_int32), ("cycle_graph", C.c_int32),
-                ("refill", C.c_int32)]
+                ("refill", C.c_int32), ("select_window", C.c_double)]
 
 
 class EigenResult(C.Structure):

@@ -48,6 +48,27 @@ def test_graph_cycles_equal_host_cycles_and_oracle(name, top, keff0, itally):
     eng.close()
 
 
+@pytest.mark.parametrize("window", [1e-9, 1000.0])
+def test_graph_threshold_search_paths(window):
+    """normSize_Repr's threshold is found exactly whichever way the search goes: a window so
+    narrow that the threshold always falls outside it (a pass over all draws), and one so wide
+    that it holds more draws than the rank count takes (a radix select of the gathered draws).
+    keff0 3 makes the first cycles drop most sites, then the population is near pop."""
+    print(f"case threshold window {window}", flush=True)
+    pop = 6000
+    inp, mg, comp = problem("pin_lattice_7g", pop=pop, transport="transportOperatorDT")
+    S = dict(pop=pop, inactive=4, active=3, seed=33, keff0=3.0)
+    bank = O.fission_source(comp, pop, O.rng_skip(33, 0))
+    R0, kh0, ba0, _ = O.eigen_run(comp, S, bank)
+    eng = Engine(comp, pop)
+    R, kh, ba, _ = eng.eigen_run(dict(S, cycle_graph=1, select_window=window), bank)
+    eng.close()
+    assert np.array_equal(kh0.view(np.uint64), kh.view(np.uint64)), (kh0, kh)
+    assert list(R0.k_implicit) == list(R.k_implicit)
+    ok, worst = rel_close(ba0[:, 0], ba[:, 0], 1e-12)
+    assert ok, worst
+
+
 def test_graph_cycles_split_over_steps():
     """start, step(2), step(rest), finish: the graph run is resumable and gives the one-call bits."""
     print("case split steps", flush=True)

@@ -34,7 +34,7 @@ LAYOUT = {
                     "n_clerks", "clerks", "n_bins", "norm_bin", "norm_value", "ht_cutoff", "n_surfaces", "surfaces",
                     "collision", "max_split", "ww_n", "fixed_point_tally", "ww_map", "ww_lower", "ww_upper", "ww_survival"],
     "scn_eigen_settings": ["pop", "n_inactive", "n_active", "inactive_tally", "seed", "keff0", "events_per_launch",
-                           "block_size", "cycle_graph", "refill"],
+                           "block_size", "cycle_graph", "refill", "select_window"],
     "scn_eigen_result": ["k_analog", "k_implicit", "imp", "k_history", "cycle_time", "bins_active", "bins_inactive",
                          "transport_time", "n_collisions", "n_events", "n_histories"],
     "scn_source": ["type", "isotropic", "G", "mat", "n_probG", "pad", "r", "dir", "bottom", "top", "t_low", "t_high",
