@@ -292,10 +292,11 @@ struct EigCtl {
     int pop;
     int n_sites;           // sites banked by the last transport (clamped to the capacity)
     int excess, heap, n_copies, n_dup;
-    int n_below;           // draws below the selection window
-    int n_cand;            // draws inside it (gathered)
+    int n_cand;            // draws inside the selection window (gathered)
+    int pad_sel;
     double thr;            // the heap-th smallest draw (normSize_Repr's threshold)
     double sel_sigmas;     // half-width of the selection window in standard deviations
+    double sel_lo, sel_hi; // the window of the last cycle
     unsigned long long sites_total;
 };
 
@@ -383,58 +384,67 @@ __global__ void __launch_bounds__(1024) k_scan_top(int* bsum, const int* n_ptr)
 
 // history b's sites start at the exclusive prefix sum of the site counts of histories 1..b-1
 // (sortByBroodID of the detain order, without a sort); block sums of that scan are added here.
-// Each site's position in the next bank: its weight goes to a contiguous array for the END sum
-// and, when normSize_Repr must draw (heap > 0), the draw of that position
-// (particleDungeon_class.f90:471-487: one number per site from the master stream skipped to it,
-// after the package's stride of pop + 1).  Draws below the selection window are counted, draws
-// inside it gathered.
+// Each site's weight goes to its position in the next bank (a contiguous array for the END sum).
 __global__ void __launch_bounds__(256) k_site_index(Sites S, const int* __restrict__ site_off,
                                                     const int* __restrict__ bsum, int pop, int out_cap, EigCtl* ctl,
-                                                    double* __restrict__ w_out, double* __restrict__ rn,
-                                                    double* __restrict__ cand, int cand_cap, DevStatus* st)
+                                                    double* __restrict__ w_out, DevStatus* st)
 {
-    __shared__ int below;
-    __shared__ uint64_t master;   // the master state after the package's stride of pop + 1
-    __shared__ double lo, hi;
     const int count = *S.count;
     const int n = min(count, min(S.cap, out_cap));
-    const int excess = n - pop;
-    const int heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) {
         ctl->n_sites = n;
         ctl->sites_total += (unsigned long long)n;
         if (count > n) raise_error(st, SCN_ERR_OVERFLOW, count);
     }
-    if (threadIdx.x == 0) {
-        below = 0;
-        master = rng_stride(ctl->master, (int64_t)pop + 1);
-        if (heap) sel_window(heap, n, ctl->sel_sigmas, &lo, &hi);
-    }
+    if (j >= n) return;
+    const uint64_t key = S.key[j];
+    const int b = (int)(key >> 24);
+    if (b < 1 || b > pop) return;
+    const int64_t dst = (int64_t)site_off[b - 1] + bsum[(b - 1) / SCAN_PER_BLOCK] + (int64_t)(key & 0xFFFFFF);
+    if (dst < n) w_out[dst] = S.w[j];
+}
+
+// when normSize_Repr must draw (heap > 0): the draw of every position of the next bank
+// (particleDungeon_class.f90:471-487: one number per site from the master stream skipped to it;
+// k_dcycle_close has applied the package's stride of pop + 1), in bank order; draws below the selection window are
+// counted per block (into below[block], summed by k_sel_pick), draws inside it gathered (one atomic
+// per wave that has any)
+__global__ void __launch_bounds__(256) k_site_draws(EigCtl* ctl, double* __restrict__ rn, double* __restrict__ cand,
+                                                    int cand_cap, int* __restrict__ below_part)
+{
+    __shared__ int below;
+    const int n = ctl->n_sites;
+    const int heap = ctl->heap;
+    if (heap == 0 || (int64_t)blockIdx.x * blockDim.x >= n) return;   // uniform over the block
+    const double lo = ctl->sel_lo, hi = ctl->sel_hi;
+    const uint64_t master = ctl->master;
+    if (threadIdx.x == 0) below = 0;
     __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int is_below = 0, is_cand = 0;
+    double u = 0.0;
     if (j < n) {
-        const uint64_t key = S.key[j];
-        const int b = (int)(key >> 24);
-        if (b >= 1 && b <= pop) {
-            const int64_t dst = (int64_t)site_off[b - 1] + bsum[(b - 1) / SCAN_PER_BLOCK] + (int64_t)(key & 0xFFFFFF);
-            if (dst < n) {
-                w_out[dst] = S.w[j];
-                if (heap) {
-                    uint64_t s = rng_skip(master, dst);
-                    const double u = scn_rng_get(&s);
-                    rn[dst] = u;
-                    if (u < lo) {
-                        atomicAdd(&below, 1);
-                    } else if (u < hi) {
-                        const int i = atomicAdd(&ctl->n_cand, 1);
-                        if (i < cand_cap) cand[i] = u;
-                    }
-                }
-            }
+        uint64_t s = rng_skip(master, j);
+        u = scn_rng_get(&s);
+        rn[j] = u;
+        is_below = u < lo;
+        is_cand = !is_below && u < hi;
+    }
+    const unsigned long long mb = __ballot(is_below), mc = __ballot(is_cand);
+    const int lane = threadIdx.x & 63;
+    if (mb && lane == 0) atomicAdd(&below, __popcll(mb));
+    if (mc) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->n_cand, __popcll(mc));
+        base = __shfl(base, 0);
+        if (is_cand) {
+            const int i = base + __popcll(mc & ((1ull << lane) - 1ull));
+            if (i < cand_cap) cand[i] = u;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && below) atomicAdd(&ctl->n_below, below);
+    if (threadIdx.x == 0) below_part[blockIdx.x] = below;
 }
 
 // canonical sum, first stage with coalesced loads: block b of a job sums chunks 256b .. 256b + 255
@@ -548,6 +558,7 @@ __global__ void __launch_bounds__(1024) k_dcycle_close(const double* y, const in
     ctl->heap = n == 0 ? 0 : (excess < 0 ? ((-excess) % n) : excess);
     ctl->n_copies = (n == 0 || excess >= 0) ? 0 : (-excess) / n;
     ctl->n_dup = (n == 0 || excess >= 0) ? 0 : (-excess) % n;
+    if (ctl->heap) sel_window(ctl->heap, n, ctl->sel_sigmas, &ctl->sel_lo, &ctl->sel_hi);   // k_site_index's window
 }
 
 // digit (0..255) of a 256-bin histogram h where the cumulative count reaches k, and the rank
@@ -581,30 +592,111 @@ __device__ void pick_digit(const int* h, int k, int* digit, int* rank)
 
 // the heap-th smallest draw, exactly, in one block of 1024 threads.  When the window held it (the
 // draws below it fewer than heap, with the window's draws at least heap) and every draw inside was
-// gathered, it is the (heap - below)-th smallest of the gathered draws: by counting ranks when they
-// fit in LDS, otherwise by a radix select on their bit patterns (non-negative doubles order as
-// their bits); otherwise the radix select runs over all draws
-constexpr int PICK_LDS = 2048;
+// gathered, it is the (heap - below)-th smallest of the gathered draws.  Up to PICK_LDS of them:
+// a histogram of PICK_BUCKETS equal sub-intervals of the window (the bucket index is monotone in
+// the draw) gives the bucket and the rank within it, then ranks are counted among that bucket's
+// draws.  More: a radix select on their bit patterns (non-negative doubles order as their bits).
+// A miss of the window: the radix select over all draws.
+constexpr int PICK_LDS = 4096;
+constexpr int PICK_BUCKETS = 4096;
+constexpr int PICK_MEMBERS = 512;
 __global__ void __launch_bounds__(1024) k_sel_pick(const double* __restrict__ rn, EigCtl* ctl,
-                                                   const double* __restrict__ cand, int cand_cap)
+                                                   const double* __restrict__ cand, int cand_cap,
+                                                   const int* __restrict__ below_part)
 {
     __shared__ double v[PICK_LDS];
+    __shared__ int hb[PICK_BUCKETS];
+    __shared__ int part[1024];
+    __shared__ double mem[PICK_MEMBERS];
     __shared__ int h[256];
+    __shared__ int s_bucket, s_rank, s_m, s_below;
     if (ctl->heap == 0) return;
-    const int heap = ctl->heap, nb = ctl->n_below, nc = ctl->n_cand;
+    const int t = threadIdx.x;
+    // draws below the window: the per-block counts of k_site_draws
+    const int nblk = (ctl->n_sites + 255) / 256;
+    int bl = 0;
+    for (int i = t; i < nblk; i += blockDim.x) bl += below_part[i];
+    if (t == 0) s_below = 0;
+    __syncthreads();
+    if (bl) atomicAdd(&s_below, bl);
+    __syncthreads();
+    const int heap = ctl->heap, nb = s_below, nc = ctl->n_cand;
     const bool in_window = nb < heap && heap <= nb + nc && nc <= cand_cap;
     if (in_window && nc <= PICK_LDS) {
         const int k = heap - nb;
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) v[i] = cand[i];
+        const double lo = ctl->sel_lo, scale = (double)PICK_BUCKETS / (ctl->sel_hi - ctl->sel_lo);
+        auto bucket = [&](double u) {
+            const double x = (u - lo) * scale;   // >= 0, monotone in u
+            return x >= (double)(PICK_BUCKETS - 1) ? PICK_BUCKETS - 1 : (int)x;
+        };
+        for (int i = t; i < PICK_BUCKETS; i += blockDim.x) hb[i] = 0;
+        if (t == 0) s_m = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            const double x = v[i];
-            int less = 0, leq = 0;
-            for (int q = 0; q < nc; ++q) {
-                less += v[q] < x;
-                leq += v[q] <= x;
+        for (int i = t; i < nc; i += blockDim.x) {
+            const double u = cand[i];
+            v[i] = u;
+            atomicAdd(&hb[bucket(u)], 1);
+        }
+        __syncthreads();
+        constexpr int PER = PICK_BUCKETS / 1024;
+        int c[PER], tot = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            c[q] = hb[t * PER + q];
+            tot += c[q];
+        }
+        part[t] = tot;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int x = t >= off ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += x;
+            __syncthreads();
+        }
+        const int incl = part[t], excl = incl - tot;
+        if (excl < k && k <= incl) {
+            int run = excl;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (run + c[q] >= k) {
+                    s_bucket = t * PER + q;
+                    s_rank = k - run;
+                    break;
+                }
+                run += c[q];
             }
-            if (less < k && k <= leq) ctl->thr = x;   // equal draws write the same value
+        }
+        __syncthreads();
+        const int B = s_bucket, r = s_rank;
+        for (int i = t; i < nc; i += blockDim.x)
+            if (bucket(v[i]) == B) {
+                const int m = atomicAdd(&s_m, 1);
+                if (m < PICK_MEMBERS) mem[m] = v[i];
+            }
+        __syncthreads();
+        const int m = s_m;
+        if (m <= PICK_MEMBERS) {
+            for (int i = t; i < m; i += blockDim.x) {
+                const double x = mem[i];
+                int less = 0, leq = 0;
+                for (int q = 0; q < m; ++q) {
+                    less += mem[q] < x;
+                    leq += mem[q] <= x;
+                }
+                if (less < r && r <= leq) ctl->thr = x;   // equal draws write the same value
+            }
+        } else {   // a crowded bucket: count over the whole candidate set restricted to it
+            for (int i = t; i < nc; i += blockDim.x) {
+                const double x = v[i];
+                if (bucket(x) != B) continue;
+                int less = 0, leq = 0;
+                for (int q = 0; q < nc; ++q) {
+                    if (bucket(v[q]) != B) continue;
+                    less += v[q] < x;
+                    leq += v[q] <= x;
+                }
+                if (less < r && r <= leq) ctl->thr = x;
+            }
         }
         return;
     }
@@ -614,20 +706,20 @@ __global__ void __launch_bounds__(1024) k_sel_pick(const double* __restrict__ rn
     uint64_t prefix = 0, mask = 0;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
-        if (threadIdx.x < 256) h[threadIdx.x] = 0;
+        if (t < 256) h[t] = 0;
         __syncthreads();
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        for (int j = t; j < n; j += blockDim.x) {
             const uint64_t x = scn_dbits(src[j]);
             if ((x & mask) == prefix) atomicAdd(&h[(int)((x >> shift) & 255)], 1);
         }
         __syncthreads();
-        int d, r;
-        pick_digit(h, k, &d, &r);
+        int d, rr;
+        pick_digit(h, k, &d, &rr);
         prefix |= (uint64_t)d << shift;
         mask |= (uint64_t)255 << shift;
-        k = r;
+        k = rr;
     }
-    if (threadIdx.x == 0) ctl->thr = scn_bitsd(prefix);
+    if (t == 0) ctl->thr = scn_bitsd(prefix);
 }
 
 // normSize_Repr's selection flags (keep the draws above the threshold when there are too many
@@ -748,7 +840,6 @@ __global__ void k_dcycle_end(EigCtl* ctl, const int* flag, const int* pos, const
         const int n_new = ex == 0 ? n : ex > 0 ? last : n * (ctl->n_copies + 1) + last;
         if (n_new != ctl->pop) raise_error(st, SCN_ERR_NORM, n_new);
     }
-    ctl->n_below = 0;
     ctl->n_cand = 0;
     ctl->master = rng_stride(ctl->master, 1);
     ctl->cur = 1 - ctl->cur;
@@ -917,6 +1008,7 @@ struct Engine {
     double* wgt_bank = nullptr;
     double* wgt_sites = nullptr;
     double* sel_cand = nullptr;
+    int* sel_below = nullptr;
     int sel_cap = 0;
     int64_t* sum_mtab = nullptr;
     double* k_hist_dev = nullptr;
@@ -1412,15 +1504,13 @@ int enqueue_dcycle(Engine* E, int phase, int score_bins)
        E->counters_i + 1, E->cap, 1 << 30, 0, 0.0, E->sites, E->bins, use_lds, E->hist_k, E->cap, E->counters,
        E->status, E->counters_i + 2, E->chunk, CycleIn{&c->master, &c->k_new, E->hist_nsite, E->refill});
     // offsets of each history's sites in the next bank (broodID, emission order), their weights
-    // and, when the population must be normalised, their draws
     const int hb = (pop + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
     hipLaunchKernelGGL(k_scan_blocks, dim3(std::max(1, hb)), dim3(256), 0, E->stream, (const int*)E->hist_nsite,
                        E->site_off, E->scan_bsum, (const int*)&c->pop);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, E->stream, E->scan_bsum, (const int*)&c->pop);
     const int sg = (E->site_cap + 255) / 256;
     hipLaunchKernelGGL(k_site_index, dim3(sg), dim3(256), 0, E->stream, E->sites, (const int*)E->site_off,
-                       (const int*)E->scan_bsum, pop, E->bank_cap, c, E->wgt_sites, E->rn, E->sel_cand, E->sel_cap,
-                       E->status);
+                       (const int*)E->scan_bsum, pop, E->bank_cap, c, E->wgt_sites, E->status);
     // START / END population weights and the keffImplicit scores, the k estimates, the plan
     SumJobs J{};
     J.njobs = 6;
@@ -1435,9 +1525,12 @@ int enqueue_dcycle(Engine* E, int phase, int score_bins)
         int rc = close_bins(E, phase == 1 ? 0 : 1);
         if (rc) return rc;
     }
-    // normSize_Repr: the heap-th smallest draw, the flags, the next bank
+    // normSize_Repr: the draws, the heap-th smallest of them, the flags, the next bank
+    const int bg = (E->bank_cap + 255) / 256;
+    hipLaunchKernelGGL(k_site_draws, dim3(bg), dim3(256), 0, E->stream, c, E->rn, E->sel_cand, E->sel_cap,
+                       E->sel_below);
     hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(1024), 0, E->stream, (const double*)E->rn, c,
-                       (const double*)E->sel_cand, E->sel_cap);
+                       (const double*)E->sel_cand, E->sel_cap, (const int*)E->sel_below);
     const int fb = (E->bank_cap + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
     hipLaunchKernelGGL(k_norm_flag_scan, dim3(std::max(1, fb)), dim3(256), 0, E->stream, (const double*)E->rn,
                        (const EigCtl*)c, E->flag, E->pos, E->scan_bsum2);
@@ -1478,6 +1571,7 @@ int dcycle_setup(Engine* E, const scn_eigen_settings& S)
         // (more falls back to a pass over all draws, with the same result)
         E->sel_cap = std::max(1 << 16, (int)(16.0 * std::sqrt((double)E->bank_cap)) + 4096);
         rc |= dalloc(E, &E->sel_cand, (size_t)E->sel_cap);
+        rc |= dalloc(E, &E->sel_below, (size_t)E->bank_cap / 256 + 1);
         rc |= dalloc(E, &E->sum_mtab, (size_t)8 * (SCN_DSUM_MAX_LEVELS + 2));
         if (rc) return rc;
     }
