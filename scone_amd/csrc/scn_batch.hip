@@ -238,15 +238,15 @@ __global__ void k_bend(const SlotDev* __restrict__ slots, SlotState* ss, int n_s
 constexpr int GEO_KEY = 16;
 
 template <int MODE>
-void pick_bt(int key, BTransportFn* fn, const void** kern)
+void pick_bt(int key, BTransportFn* fn, const void** kern, BSourceFn* src = nullptr)
 {
     switch (key) {
-    case 1: bt_variant<1, MODE, 0>(fn, kern); break;
-    case 3: bt_variant<3, MODE, 0>(fn, kern); break;
-    case 4: bt_variant<4, MODE, 0>(fn, kern); break;
-    case 3 + GEO_KEY: bt_variant<3, MODE, 1>(fn, kern); break;
-    case 8 + GEO_KEY: bt_variant<8, MODE, 1>(fn, kern); break;
-    default: bt_variant<8, MODE, 0>(fn, kern); break;
+    case 1: bt_variant<1, MODE, 0>(fn, kern, src); break;
+    case 3: bt_variant<3, MODE, 0>(fn, kern, src); break;
+    case 4: bt_variant<4, MODE, 0>(fn, kern, src); break;
+    case 3 + GEO_KEY: bt_variant<3, MODE, 1>(fn, kern, src); break;
+    case 8 + GEO_KEY: bt_variant<8, MODE, 1>(fn, kern, src); break;
+    default: bt_variant<8, MODE, 0>(fn, kern, src); break;
     }
 }
 
@@ -259,6 +259,7 @@ struct SlotHost {
     int fissile = 0, buffer = 0;
     int geo = 0;        // needs the general-geometry kernels
     int ww = 0;         // weight windows: split copies use the history's buffer
+    int mat_src = 0;    // material source: the cycle's first launch runs the sampling pre-pass
     int window_max = 1; // window the solve was admitted with (BIN copies allocated)
 };
 
@@ -281,6 +282,7 @@ struct Batch {
     int n_cu = 256;
     // kernel variant of the recorded cycle
     int nl_cur = 0;
+    int ms_cur = 0;     // the captured cycle runs the material-source pre-pass
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<hipEvent_t> ev;
@@ -348,6 +350,7 @@ struct SlotLayout {
     size_t queue_bytes = 0;
     size_t bins = 0;            // BIN, CSUM, CSUM2
     size_t hist = 0;
+    size_t src = 0;             // material-source attempt counts
     size_t total = 0;
 };
 
@@ -532,12 +535,20 @@ int max_depth_nl(Batch* B)
 size_t lds_bytes(const Batch* B) { return sizeof(double) * (size_t)B->S_lds_per_wave * (BBLOCK / BWAVE); }
 
 // persistent grid of the transport kernel variant: resident blocks per CU x CUs
-int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
+// a resident solve samples a material source (the cycle then runs the sampling pre-pass)
+int any_material_source(const Batch* B)
+{
+    for (const SlotHost& h : B->hs)
+        if (h.used && h.mat_src) return 1;
+    return 0;
+}
+
+int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid, BSourceFn* src)
 {
     const void* kern;
-    if (B->S.transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(nl, fn, &kern);
-    else if (B->S.transport == SCN_TRANSPORT_HT) pick_bt<SCN_TRANSPORT_HT>(nl, fn, &kern);
-    else pick_bt<SCN_TRANSPORT_ST>(nl, fn, &kern);
+    if (B->S.transport == SCN_TRANSPORT_DT) pick_bt<SCN_TRANSPORT_DT>(nl, fn, &kern, src);
+    else if (B->S.transport == SCN_TRANSPORT_HT) pick_bt<SCN_TRANSPORT_HT>(nl, fn, &kern, src);
+    else pick_bt<SCN_TRANSPORT_ST>(nl, fn, &kern, src);
     int occ = 0;
     BHIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BBLOCK, lds_bytes(B)));
     *grid = B->n_cu * std::max(1, occ);
@@ -545,7 +556,7 @@ int transport_variant(Batch* B, int nl, BTransportFn* fn, int* grid)
 }
 
 // enqueue one cycle on B->stream (also the body of the captured graph); timed = per-launch events
-int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
+int enqueue_cycle(Batch* B, BTransportFn fn, BSourceFn src, int mat_src, int grid, int timed)
 {
     const int ns = B->n_slots;
     const int tile = B->S.tile;
@@ -556,6 +567,8 @@ int enqueue_cycle(Batch* B, BTransportFn fn, int grid, int timed)
         const int out_side = first ? 0 : 1 - side;
         hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, B->stream, B->d_slots, B->d_state, ns, first, in_side,
                            out_side, B->d_items, B->d_item_cnt, B->d_ctl);
+        if (first && mat_src)
+            src(dim3(B->n_cu), dim3(256), B->stream, B->d_slots, B->d_state, B->d_items, B->d_ctl, B->d_status);
         const int last = L == B->S.n_launch - 1;
         const int steps = last ? (1 << 30) : (B->S.k0 << L);
         if (timed) BHIPCHK(hipEventRecord(B->ev[2 * L], B->stream));
@@ -663,18 +676,20 @@ int reserve_stack(Batch* B, int extra_depth)
 int run_cycle(Batch* B, int* n_running_out)
 {
     const int nl = max_depth_nl(B);
+    const int msrc = any_material_source(B);
     BTransportFn fn;
+    BSourceFn src = nullptr;
     int grid = 0;
-    int rc0 = transport_variant(B, nl, &fn, &grid);
+    int rc0 = transport_variant(B, nl, &fn, &grid, &src);
     if (rc0) return rc0;
     rc0 = ensure_stack(B, grid);
     if (rc0) return rc0;
     BHIPCHK(hipEventRecord(B->c0, B->stream));
     if (B->S.use_graph) {
-        if (!B->gexec || B->nl_cur != nl) {
+        if (!B->gexec || B->nl_cur != nl || B->ms_cur != msrc) {
             drop_graph(B);
             BHIPCHK(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_cycle(B, fn, grid, 0);
+            int rc = enqueue_cycle(B, fn, src, msrc, grid, 0);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(B->stream, &g);
             if (rc) {
@@ -688,10 +703,11 @@ int run_cycle(Batch* B, int* n_running_out)
             B->graph = g;
             BHIPCHK(hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
             B->nl_cur = nl;
+            B->ms_cur = msrc;
         }
         BHIPCHK(hipGraphLaunch(B->gexec, B->stream));
     } else {
-        int rc = enqueue_cycle(B, fn, grid, 1);
+        int rc = enqueue_cycle(B, fn, src, msrc, grid, 1);
         if (rc) return rc;
     }
     BHIPCHK(hipEventRecord(B->c1, B->stream));
@@ -901,7 +917,10 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     Ly.hist = Ly.bins + bins_bytes;
     size_t nh = R->record ? (size_t)R->n_batches * (size_t)R->pop : 0;
     size_t hist_bytes = R->record ? align256(sizeof(uint64_t) * nh) + align256(sizeof(int) * nh) : 0;
-    Ly.total = Ly.hist + hist_bytes;
+    // material source: rejected attempts of every history of the window, from the pre-pass
+    Ly.src = Ly.hist + hist_bytes;
+    const size_t src_bytes = sr->type == SCN_SRC_POINT ? 0 : align256((size_t)window * (size_t)R->pop);
+    Ly.total = Ly.src + src_bytes;
     size_t off = arena_alloc(B, Ly.total);
     if (off == SIZE_MAX) {
         B->err = "not enough of the HBM budget free for this request";
@@ -976,6 +995,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     D.hist_rng = R->record ? (uint64_t*)(base + Ly.hist) : nullptr;
     D.hist_coll = R->record ? (int*)(base + Ly.hist + align256(sizeof(uint64_t) * nh)) : nullptr;
     D.probG = probG.empty() ? nullptr : (const double*)(base + o_probG);
+    D.src_att = sr->type == SCN_SRC_POINT ? nullptr : (uint8_t*)(base + Ly.src);
     for (int k = 0; k < 3; ++k) {
         D.src_r[k] = sr->r[k];
         D.src_dir[k] = dir[k];
@@ -1022,6 +1042,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     h.depth = depth;
     h.geo = needs_full_geometry(pr);
     h.ww = pr->ww_n > 0;
+    h.mat_src = sr->type != SCN_SRC_POINT;
     h.n_bins = pr->n_bins;
     h.pop = R->pop;
     h.n_batches = R->n_batches;

@@ -69,6 +69,8 @@ struct SlotDev {
     int buffer;         // 'buffer' of the package: depth of a history's LIFO buffer of secondaries
     int window;         // batches transported per cycle; BIN holds one copy of the score memory each
     int adapt;          // > 0: each cycle's window follows the stop bin's trend (projection factor in %)
+    uint8_t* src_att;   // material source: attempts rejected before each history's source point,
+                        // [window][pop], written by k_bsource (MATERIAL_SOURCE_ATTEMPTS: sampling failed)
 };
 
 // mutable state of a slot, written by the kernels
@@ -99,11 +101,14 @@ struct Item {
 
 typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const Item*, int*, int*, int, int,
                              int, int, int, DevStatus*, int, Sites, int);
+typedef void (*BSourceFn)(dim3, dim3, hipStream_t, const SlotDev*, const SlotState*, const Item*, const int*,
+                          DevStatus*);
 
-// launcher and kernel address of one variant: register depth NL, tracking mode MODE, geometry
-// variant GEO (0: lattice/pin/box, 1: cells, rotations, CSG borders)
+// launchers and kernel address of one variant: register depth NL, tracking mode MODE, geometry
+// variant GEO (0: lattice/pin/box, 1: cells, rotations, CSG borders); src launches the
+// material-source pre-pass of the same geometry variant
 template <int NL, int MODE, int GEO>
-void bt_variant(BTransportFn* fn, const void** kern);
+void bt_variant(BTransportFn* fn, const void** kern, BSourceFn* src);
 
 #ifdef SCN_BTRANSPORT_KERNELS
 template <int NL, int GEO>
@@ -170,12 +175,34 @@ __device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<
     L.split[i] = p.split;
 }
 
-// Source particle j = i + 1 of batch `batch`, then the start of its history
-// (fixedSourcePhysicsPackage_class.f90:187-213): the source is sampled on the seed strided by
-// 2*pop*batch + j (generate, source_inter.f90:110-116) and the history runs on the seed strided
-// by 2*pop*batch + pop + j.  Returns 0 if sampling failed (error raised).
+// one attempt of materialSource%sampleParticle (materialSource_class.f90:160-190): a point
+// uniform in the bounding box and a time, 4 draws of the source stream, placed with u = [1,0,0];
+// returns the material there
 template <int NL, int GEO>
-__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int i, Pt<NL, GEO>& p,
+__device__ __forceinline__ int src_attempt(const SlotDev& D, const DevProblem& P, uint64_t& srng, Pt<NL, GEO>& p,
+                                           double r[3], double& time, DevStatus* st)
+{
+    double r3[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&srng);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = (D.top[k] - D.bottom[k]) * r3[k] + D.bottom[k];
+    time = D.t_low + scn_rng_get(&srng) * (D.t_high - D.t_low);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.r[0][k] = r[k];
+    p.u[0] = 1.0;
+    p.u[1] = 0.0;
+    p.u[2] = 0.0;
+    place_coord(P, p, st);
+    return p.mat;
+}
+
+// Source particle j = i + 1 of batch `batch` (batch w of the slot's window), then the start of
+// its history (fixedSourcePhysicsPackage_class.f90:187-213): the source is sampled on the seed
+// strided by 2*pop*batch + j (generate, source_inter.f90:110-116) and the history runs on the seed
+// strided by 2*pop*batch + pop + j.  Returns 0 if sampling failed (error raised).
+template <int NL, int GEO>
+__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int w, int i, Pt<NL, GEO>& p,
                                     DevStatus* st)
 {
     const int64_t j = (int64_t)i + 1;
@@ -220,39 +247,22 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
             G = D.src_G;
         }
     } else {
-        // materialSource%sampleParticle: rejection on whatIsAt(r) (u = [1,0,0])
-        int it = 0;
-        for (;;) {
-            if (++it > MATERIAL_SOURCE_ATTEMPTS) {
-                raise_error(st, SCN_ERR_SAMPLING, 4);
-                return 0;
-            }
-            double r3[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) r3[k] = scn_rng_get(&srng);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) r[k] = (D.top[k] - D.bottom[k]) * r3[k] + D.bottom[k];
-            time = D.t_low + scn_rng_get(&srng) * (D.t_high - D.t_low);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) p.r[0][k] = r[k];
-            p.u[0] = 1.0;
-            p.u[1] = 0.0;
-            p.u[2] = 0.0;
-            place_coord(P, p, st);
-            int mat = p.mat;
-            if (mat == SCN_OUTSIDE_MAT) continue;
-            if (mat < 1 || mat > P.n_mat) {
-                raise_error(st, SCN_ERR_MATERIAL, mat);
-                return 0;
-            }
-            if (mat != D.src_mat) continue;
-            double mu = 2.0 * scn_rng_get(&srng) - 1.0;
-            double phi = SCN_TWO_PI * scn_rng_get(&srng);
-            const double x1[3] = {1.0, 0.0, 0.0};
-            scn_rotate(x1, mu, phi, dir);
-            G = D.src_G;
-            break;
+        // materialSource%sampleParticle: rejection on whatIsAt(r).  k_bsource ran the rejection on
+        // this stream before the launch and kept how many attempts failed; skip their draws and
+        // repeat the accepted attempt, so no wave waits on its lanes' longest rejection run
+        const int k = D.src_att[(size_t)w * D.pop + i];
+        if (k >= MATERIAL_SOURCE_ATTEMPTS) return 0;   // the pre-pass raised the error
+        srng = rng_skip(srng, 4 * (int64_t)k);
+        const int mat = src_attempt(D, P, srng, p, r, time, st);
+        if (mat != D.src_mat) {
+            raise_error(st, SCN_ERR_SAMPLING, 7);
+            return 0;
         }
+        double mu = 2.0 * scn_rng_get(&srng) - 1.0;
+        double phi = SCN_TWO_PI * scn_rng_get(&srng);
+        const double x1[3] = {1.0, 0.0, 0.0};
+        scn_rotate(x1, mu, phi, dir);
+        G = D.src_G;
     }
     p.rng = rng_stride(D.seed, b2 + D.pop + j);
 #pragma unroll
@@ -454,7 +464,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
             const int i = next + __popcll(idle & lt_mask);
             if (!alive && i < end) {
                 if (first) {
-                    alive = born(*D, D->P, batch, i, p, st);
+                    alive = born(*D, D->P, batch, key & (MAX_WINDOW - 1), i, p, st);
                 } else {
                     bload(D->q[in_side], cap, i, p);
                     relink(D->P, p);
@@ -564,6 +574,109 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
     }
 }
 
+// Material-source pre-pass of a cycle's first launch: for every history of every item of a
+// material-source slot, the rejection loop of materialSource%sampleParticle on the history's
+// source stream (as born() would run it), keeping only how many attempts failed.  Waves take
+// chunks of SRC_CHUNK consecutive histories round-robin and work through a chunk one item at a
+// time, so the slot (and its tables) is wave-uniform; each lane takes a contiguous run of the
+// item's histories and makes one attempt per iteration, moving to its next history (its stream
+// one stride further: one LCG step) as soon as one is accepted.
+constexpr int SRC_CHUNK = 2048;
+template <int NL, int GEO>
+__global__ void __launch_bounds__(256)
+k_bsource(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss, const Item* __restrict__ items,
+          const int* __restrict__ ctl, DevStatus* st)
+{
+    const int n_items = ctl[0];
+    if (n_items <= 0) return;
+    const int lane = threadIdx.x & (BWAVE - 1);
+    const long long total = items[n_items - 1].prefix + items[n_items - 1].n;
+    const long long n_chunks = (total + SRC_CHUNK - 1) / SRC_CHUNK;
+    const long long n_waves = (long long)gridDim.x * (blockDim.x / BWAVE);
+    const long long wave = (long long)blockIdx.x * (blockDim.x / BWAVE) + threadIdx.x / BWAVE;
+    const uint64_t A1 = RNG_STRIDES.a[0][1], C1 = RNG_STRIDES.c[0][1];   // one stride of the LCG
+    Pt<NL, GEO> p;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        p.loc[l] = 0;
+        p.uni[l] = 0;
+        p.root[l] = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p.r[l][q] = 0.0;
+    }
+    for (long long c = wave; c < n_chunks; c += n_waves) {
+        long long a = c * SRC_CHUNK;
+        const long long e = min(a + SRC_CHUNK, total);
+        while (a < e) {   // one item of the chunk at a time (wave-uniform)
+            int lo = 0, hi = n_items;   // items[lo].prefix <= a < items[lo].prefix + items[lo].n
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (items[mid].prefix <= a) lo = mid;
+                else hi = mid;
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            const long long prefix = items[lo].prefix;
+            const long long seg_end = min(e, prefix + items[lo].n);
+            const int key = __builtin_amdgcn_readfirstlane(items[lo].key);
+            const int s = key >> WINDOW_SHIFT, w = key & (MAX_WINDOW - 1);
+            const SlotDev& D = slots[s];
+            if (D.src_type != SCN_SRC_POINT) {
+                const long long len = seg_end - a, per = (len + BWAVE - 1) / BWAVE;
+                long long g = a + lane * per;
+                const long long g_stop = min(g + per, seg_end);
+                int i = (int)(g - prefix), k = 0, active = g < g_stop;
+                uint64_t start = 0, srng = 0;
+                if (active) {
+                    const int64_t b2 = 2 * (int64_t)D.pop * (ss[s].batch + w);
+                    start = rng_stride(D.seed, b2 + (int64_t)i + 1);
+                    srng = start;
+                }
+                while (active) {
+                    double r[3], time;
+                    const int mat = src_attempt(D, D.P, srng, p, r, time, st);
+                    int done = 0;
+                    if (mat == SCN_OUTSIDE_MAT) {
+                    } else if (mat < 1 || mat > D.P.n_mat) {
+                        raise_error(st, SCN_ERR_MATERIAL, mat);
+                        k = MATERIAL_SOURCE_ATTEMPTS;
+                        done = 1;
+                    } else if (mat == D.src_mat) {
+                        done = 1;
+                    }
+                    if (!done && ++k >= MATERIAL_SOURCE_ATTEMPTS) {
+                        raise_error(st, SCN_ERR_SAMPLING, 4);
+                        done = 1;
+                    }
+                    if (done) {
+                        D.src_att[(size_t)w * D.pop + i] = (uint8_t)k;
+                        if (++g < g_stop) {   // the next history: its stream is one stride on
+                            ++i;
+                            start = (A1 * start + C1) & SCN_LCG_MASK;
+                            srng = start;
+                            k = 0;
+                        } else {
+                            active = 0;
+                        }
+                    }
+                }
+            }
+            a = seg_end;
+        }
+    }
+}
+
+// the pre-pass over a grid of every resident block of its kernel on g.x CUs
+template <int NL, int GEO>
+void launch_bsource(dim3 g, dim3 b, hipStream_t s, const SlotDev* slots, const SlotState* ss, const Item* items,
+                    const int* ctl, DevStatus* st)
+{
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_bsource<NL, GEO>, (int)b.x, 0) != hipSuccess)
+        occ = 2;
+    hipLaunchKernelGGL((k_bsource<NL, GEO>), dim3(g.x * (unsigned)std::max(1, occ)), b, 0, s, slots, ss, items, ctl,
+                       st);
+}
+
 template <int NL, int MODE, int GEO>
 void launch_bt(dim3 g, dim3 b, size_t sh, hipStream_t s, const SlotDev* slots, SlotState* ss, const Item* items,
                int* item_cnt, int* ctl, int first, int in_side, int out_side, int chunk, int max_steps, DevStatus* st,
@@ -580,10 +693,11 @@ const void* bt_kernel()
 }
 
 template <int NL, int MODE, int GEO>
-void bt_variant(BTransportFn* fn, const void** kern)
+void bt_variant(BTransportFn* fn, const void** kern, BSourceFn* src)
 {
     *fn = launch_bt<NL, MODE, GEO>;
     *kern = bt_kernel<NL, MODE, GEO>();
+    if (src) *src = launch_bsource<NL, GEO>;
 }
 #endif  // SCN_BTRANSPORT_KERNELS
 

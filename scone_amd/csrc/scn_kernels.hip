@@ -25,7 +25,7 @@ namespace scn_eng {
 #else
 #define SCN_BTRANSPORT_KERNELS 1
 #include "scn_btransport.h"
-#define SCN_INST(NL) template void bt_variant<NL, SCN_KMODE, SCN_KGEO>(BTransportFn * fn, const void** kern);
+#define SCN_INST(NL) template void bt_variant<NL, SCN_KMODE, SCN_KGEO>(BTransportFn * fn, const void** kern, BSourceFn* src);
 namespace scn_bat {
 #endif
 
