@@ -175,12 +175,9 @@ __device__ __forceinline__ void bstore(const BLive& L, int cap, int i, const Pt<
     L.split[i] = p.split;
 }
 
-// one attempt of materialSource%sampleParticle (materialSource_class.f90:160-190): a point
-// uniform in the bounding box and a time, 4 draws of the source stream, placed with u = [1,0,0];
-// returns the material there
-template <int NL, int GEO>
-__device__ __forceinline__ int src_attempt(const SlotDev& D, const DevProblem& P, uint64_t& srng, Pt<NL, GEO>& p,
-                                           double r[3], double& time, DevStatus* st)
+// the point and time of one attempt of materialSource%sampleParticle
+// (materialSource_class.f90:160-190): uniform in the bounding box, 4 draws of the source stream
+__device__ __forceinline__ void src_point(const SlotDev& D, uint64_t& srng, double r[3], double& time)
 {
     double r3[3];
 #pragma unroll
@@ -188,6 +185,14 @@ __device__ __forceinline__ int src_attempt(const SlotDev& D, const DevProblem& P
 #pragma unroll
     for (int k = 0; k < 3; ++k) r[k] = (D.top[k] - D.bottom[k]) * r3[k] + D.bottom[k];
     time = D.t_low + scn_rng_get(&srng) * (D.t_high - D.t_low);
+}
+
+// one attempt: the point placed with u = [1,0,0]; returns the material there
+template <int NL, int GEO>
+__device__ __forceinline__ int src_attempt(const SlotDev& D, const DevProblem& P, uint64_t& srng, Pt<NL, GEO>& p,
+                                           double r[3], double& time, DevStatus* st)
+{
+    src_point(D, srng, r, time);
 #pragma unroll
     for (int k = 0; k < 3; ++k) p.r[0][k] = r[k];
     p.u[0] = 1.0;
@@ -249,15 +254,12 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
     } else {
         // materialSource%sampleParticle: rejection on whatIsAt(r).  k_bsource ran the rejection on
         // this stream before the launch and kept how many attempts failed; skip their draws and
-        // repeat the accepted attempt, so no wave waits on its lanes' longest rejection run
+        // take the accepted attempt's point and time (its material was checked there), so no wave
+        // waits on its lanes' longest rejection run
         const int k = D.src_att[(size_t)w * D.pop + i];
         if (k >= MATERIAL_SOURCE_ATTEMPTS) return 0;   // the pre-pass raised the error
         srng = rng_skip(srng, 4 * (int64_t)k);
-        const int mat = src_attempt(D, P, srng, p, r, time, st);
-        if (mat != D.src_mat) {
-            raise_error(st, SCN_ERR_SAMPLING, 7);
-            return 0;
-        }
+        src_point(D, srng, r, time);
         double mu = 2.0 * scn_rng_get(&srng) - 1.0;
         double phi = SCN_TWO_PI * scn_rng_get(&srng);
         const double x1[3] = {1.0, 0.0, 0.0};

@@ -81,26 +81,32 @@ __device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n)
     return s;
 }
 
-// ---- collision clerks of one shared layout -----------------------------------------------------
-// When every collision clerk has the same maps, all of them lin-grid spaceMaps (at most 3), the
-// same responses and virtual-collision flag, and each group is admitted by at most one clerk (a
-// mesh tally, possibly split into one clerk per group), a score needs no walk of
-// the clerk table: the map parameters live in the problem record (wave-uniform loads) and the
-// admitting clerk's first bin comes from one per-group table.  The bins and the arithmetic are the
-// general path's (map1d, collisionClerk reportInColl).
-struct FastColl {
-    int on;
+// ---- collision clerks without a table walk ----------------------------------------------------
+// When there are at most FAST_CLERKS collision clerks, each with at most 3 maps, all of them
+// lin-grid spaceMaps (a total-flux clerk has none), a score needs no walk of the clerk table: each
+// clerk's map parameters, responses and virtual-collision flag live in the problem record
+// (wave-uniform loads), and its first bin for the particle's group comes from a per-(clerk, group)
+// table (-1: the clerk does not admit the group).  The bins and the arithmetic are the general
+// path's (map1d, collisionClerk reportInColl).
+constexpr int FAST_CLERKS = 4;
+struct FastClerk {
     int n_maps;
     int width;
     int handle_virtual;
-    int resp[8];         // SCN_MT_* of each response (the same for every clerk)
+    int pad;
+    int resp[8];         // SCN_MT_* of each response
     int axis[3];
     int nbins[3];
     int multi[3];
     double g0[3];        // grid minimum (real_data[data])
     double step[3];      // grid step (real_data[data + 1])
     double step_inv[3];  // 1 / step (only steers scn_floor_div, which checks its answer)
-    const int64_t* g_addr;   // [nG] mem_addr of the clerk admitting group g, -1 if none
+};
+struct FastColl {
+    int on;
+    int n_clerks;
+    FastClerk c[FAST_CLERKS];
+    const int64_t* g_addr;   // [n_clerks][nG]: mem_addr of clerk c if it admits group g, else -1
 };
 
 // ---- problem tables as one by-value kernel argument ---------------------------------------
@@ -1501,24 +1507,38 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
     }
     if (P.n_clerks == 0) return 0;
     if (P.fc.on) {
-        const int64_t a0 = P.fc.g_addr[p.G - 1];
-        if (a0 < 0 || (!P.fc.handle_virtual && isVirtual)) return 0;
-        int bin = 1;
-        for (int k = 0; k < P.fc.n_maps; ++k) {
-            const double v = get3(p.r[0], P.fc.axis[k]);
-            int idx = (int)scn_floor_div(v - P.fc.g0[k], P.fc.step[k], P.fc.step_inv[k]) + 1;
-            if (idx < 1 || idx >= P.fc.nbins[k] + 1) return 0;
-            bin = bin + (idx - 1) * P.fc.multi[k];
+        const double flux_v = p.w / p.track;   // flux of clerks that score virtual collisions
+        int nsc = 0;
+        // unrolled over the FAST_CLERKS slots: the rolled loop, inside the flights' divergent
+        // branches, lost scores in the delta-tracking kernels (tests/test_batch_gpu.py, slab-DT)
+#pragma unroll
+        for (int c = 0; c < FAST_CLERKS; ++c) {
+            if (c >= P.fc.n_clerks) break;
+            const FastClerk& F = P.fc.c[c];
+            const int64_t a0 = P.fc.g_addr[(size_t)c * P.nG + (p.G - 1)];
+            if (a0 < 0 || (!F.handle_virtual && isVirtual)) continue;
+            int bin = 1, inside = 1;
+            for (int k = 0; k < F.n_maps; ++k) {
+                const double v = get3(p.r[0], F.axis[k]);
+                const int idx = (int)scn_floor_div(v - F.g0[k], F.step[k], F.step_inv[k]) + 1;
+                if (idx < 1 || idx >= F.nbins[k] + 1) {
+                    inside = 0;
+                    break;
+                }
+                bin = bin + (idx - 1) * F.multi[k];
+            }
+            if (!inside) continue;
+            const double flux = F.handle_virtual ? flux_v : p.w / track_mat_xs(P, mat, p.G, st);
+            const int64_t addr = a0 + (int64_t)F.width * (bin - 1);
+            for (int i = 0; i < F.width; ++i) {
+                double resp;
+                if (F.resp[i] == SCN_MT_FLUX) resp = 1.0;
+                else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), F.resp[i]);
+                bin_add(B, addr + i, resp * flux);
+            }
+            nsc += F.width;
         }
-        const double flux = P.fc.handle_virtual ? p.w / p.track : p.w / track_mat_xs(P, mat, p.G, st);
-        const int64_t addr = a0 + (int64_t)P.fc.width * (bin - 1);
-        for (int i = 0; i < P.fc.width; ++i) {
-            double resp;
-            if (P.fc.resp[i] == SCN_MT_FLUX) resp = 1.0;
-            else resp = (mat == SCN_VOID_MAT) ? 0.0 : macro_get(xs_of(P, mat, p.G), P.fc.resp[i]);
-            bin_add(B, addr + i, resp * flux);
-        }
-        return P.fc.width;
+        return nsc;
     }
     const int cell = lowest_cell(P, p);
     int c0 = P.gclerk_off[p.G - 1], c1 = P.gclerk_off[p.G];   // collision clerks only

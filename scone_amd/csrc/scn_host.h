@@ -162,15 +162,16 @@ inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vec
     }
 }
 
-// FastColl of the problem's collision clerks, or fc.on = 0 when they do not share one layout;
-// g_addr receives the per-group first-bin table (pointer set by the caller after upload)
+// FastColl of the problem's collision clerks, or fc.on = 0 when they do not fit it (more than
+// FAST_CLERKS, or maps other than lin spaceMaps); g_addr receives the per-(clerk, group) first-bin
+// table (pointer set by the caller after upload)
 inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<int>& goff,
                                       const std::vector<int>& gidx, std::vector<int64_t>& g_addr)
 {
     FastColl F{};
-    const int nG = pr->mg.n_groups;
-    g_addr.assign(std::max(nG, 1), -1);
-    const scn_clerk* first = nullptr;
+    const int nG = std::max(pr->mg.n_groups, 1);
+    g_addr.assign(nG, -1);
+    std::vector<int> ids;   // the collision clerks, in table order
     for (int c = 0; c < pr->n_clerks; ++c) {
         const scn_clerk& K = pr->clerks[c];
         if (K.kind != SCN_CLERK_COLLISION) continue;
@@ -180,41 +181,37 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
             if (m.type != SCN_MAP_SPACE || m.grid_type != SCN_GRID_LIN) return F;
             if (m.data < 0 || m.data + 1 >= pr->n_real) return F;
         }
-        if (!first) {
-            first = &K;
-            continue;
+        ids.push_back(c);
+    }
+    if (ids.empty() || (int)ids.size() > FAST_CLERKS) return F;
+    const int nc = (int)ids.size();
+    std::vector<int64_t> addr((size_t)nc * nG, -1);
+    for (int g = 1; g <= pr->mg.n_groups; ++g)
+        for (int e = goff[g - 1]; e < goff[g]; ++e) {
+            int q = 0;
+            while (q < nc && ids[q] != gidx[e]) ++q;
+            if (q == nc || addr[(size_t)q * nG + g - 1] >= 0) return F;   // not a collision clerk, or listed twice
+            addr[(size_t)q * nG + g - 1] = pr->clerks[gidx[e]].mem_addr;
         }
-        if (K.n_maps != first->n_maps || K.width != first->width || K.handle_virtual != first->handle_virtual)
-            return F;
-        for (int i = 0; i < K.width; ++i)
-            if (K.resp[i] != first->resp[i]) return F;
+    for (int q = 0; q < nc; ++q) {
+        const scn_clerk& K = pr->clerks[ids[q]];
+        FastClerk& C = F.c[q];
+        C.n_maps = K.n_maps;
+        C.width = K.width;
+        C.handle_virtual = K.handle_virtual;
+        for (int i = 0; i < K.width; ++i) C.resp[i] = K.resp[i];
         for (int k = 0; k < K.n_maps; ++k) {
-            const scn_map1d &m = K.maps[k], &f = first->maps[k];
-            if (m.axis != f.axis || m.nbins != f.nbins || K.multi[k] != first->multi[k]) return F;
-            if (pr->real_data[m.data] != pr->real_data[f.data] ||
-                pr->real_data[m.data + 1] != pr->real_data[f.data + 1])
-                return F;
+            C.axis[k] = K.maps[k].axis;
+            C.nbins[k] = K.maps[k].nbins;
+            C.multi[k] = K.multi[k];
+            C.g0[k] = pr->real_data[K.maps[k].data];
+            C.step[k] = pr->real_data[K.maps[k].data + 1];
+            C.step_inv[k] = 1.0 / C.step[k];
         }
     }
-    if (!first) return F;
-    for (int g = 1; g <= nG; ++g) {
-        const int n = goff[g] - goff[g - 1];
-        if (n > 1) return F;
-        if (n == 1) g_addr[g - 1] = pr->clerks[gidx[goff[g - 1]]].mem_addr;
-    }
+    g_addr = addr;
     F.on = 1;
-    F.n_maps = first->n_maps;
-    F.width = first->width;
-    F.handle_virtual = first->handle_virtual;
-    for (int i = 0; i < first->width; ++i) F.resp[i] = first->resp[i];
-    for (int k = 0; k < first->n_maps; ++k) {
-        F.axis[k] = first->maps[k].axis;
-        F.nbins[k] = first->maps[k].nbins;
-        F.multi[k] = first->multi[k];
-        F.g0[k] = pr->real_data[first->maps[k].data];
-        F.step[k] = pr->real_data[first->maps[k].data + 1];
-        F.step_inv[k] = 1.0 / F.step[k];
-    }
+    F.n_clerks = nc;
     return F;
 }
 
