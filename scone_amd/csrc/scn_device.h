@@ -82,12 +82,14 @@ __device__ __forceinline__ uint64_t rng_stride(uint64_t s, int64_t n)
 }
 
 // ---- collision clerks without a table walk ----------------------------------------------------
-// When there are at most FAST_CLERKS collision clerks, each with at most 3 maps, all of them
-// lin-grid spaceMaps (a total-flux clerk has none), a score needs no walk of the clerk table: each
-// clerk's map parameters, responses and virtual-collision flag live in the problem record
-// (wave-uniform loads), and its first bin for the particle's group comes from a per-(clerk, group)
-// table (-1: the clerk does not admit the group).  The bins and the arithmetic are the general
-// path's (map1d, collisionClerk reportInColl).
+// When the collision clerks fall into at most FAST_CLERKS layouts (maps, responses, virtual-
+// collision flag), with at most 3 maps each, all lin-grid spaceMaps (a total-flux clerk has none),
+// and each group is admitted by at most one clerk of a layout (a mesh tally split into one clerk
+// per group is one layout), a score needs no walk of the clerk table: each layout's map
+// parameters, responses and virtual-collision flag live in the problem record (wave-uniform
+// loads), and the admitting clerk's first bin comes from a per-(layout, group) table (-1: no clerk
+// of the layout admits the group).  The bins and the arithmetic are the general path's (map1d,
+// collisionClerk reportInColl).
 constexpr int FAST_CLERKS = 4;
 struct FastClerk {
     int n_maps;
@@ -106,7 +108,7 @@ struct FastColl {
     int on;
     int n_clerks;
     FastClerk c[FAST_CLERKS];
-    const int64_t* g_addr;   // [n_clerks][nG]: mem_addr of clerk c if it admits group g, else -1
+    const int64_t* g_addr;   // [n_clerks][nG]: mem_addr of the layout's clerk admitting group g, or -1
 };
 
 // ---- problem tables as one by-value kernel argument ---------------------------------------

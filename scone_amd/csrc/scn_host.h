@@ -162,16 +162,35 @@ inline void group_clerks(const scn_problem* pr, std::vector<int>& goff, std::vec
     }
 }
 
-// FastColl of the problem's collision clerks, or fc.on = 0 when they do not fit it (more than
-// FAST_CLERKS, or maps other than lin spaceMaps); g_addr receives the per-(clerk, group) first-bin
-// table (pointer set by the caller after upload)
+// FastColl of the problem's collision clerks, or fc.on = 0 when they do not fit it.  Clerks with
+// the same layout (maps, responses, virtual-collision flag) share one of the FAST_CLERKS slots
+// when no group is admitted by two clerks of the slot (a mesh tally split into one clerk per
+// group takes one slot; a total-flux clerk next to per-group flux clerks takes a second); maps
+// must be lin spaceMaps.  g_addr receives the
+// per-(slot, group) first-bin table (pointer set by the caller after upload).
 inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<int>& goff,
                                       const std::vector<int>& gidx, std::vector<int64_t>& g_addr)
 {
     FastColl F{};
     const int nG = std::max(pr->mg.n_groups, 1);
     g_addr.assign(nG, -1);
-    std::vector<int> ids;   // the collision clerks, in table order
+    auto same_layout = [&](const scn_clerk& K, const scn_clerk& R) {
+        if (K.n_maps != R.n_maps || K.width != R.width || K.handle_virtual != R.handle_virtual) return false;
+        for (int i = 0; i < K.width; ++i)
+            if (K.resp[i] != R.resp[i]) return false;
+        for (int k = 0; k < K.n_maps; ++k) {
+            const scn_map1d &m = K.maps[k], &f = R.maps[k];
+            if (m.axis != f.axis || m.nbins != f.nbins || K.multi[k] != R.multi[k]) return false;
+            if (pr->real_data[m.data] != pr->real_data[f.data] || pr->real_data[m.data + 1] != pr->real_data[f.data + 1])
+                return false;
+        }
+        return true;
+    };
+    std::vector<int> slot_of(pr->n_clerks, -1), rep;   // slot of each collision clerk, a clerk of each slot
+    std::vector<std::vector<char>> taken;               // groups admitted by a clerk of each slot
+    std::vector<std::vector<char>> admits(pr->n_clerks, std::vector<char>(nG, 0));
+    for (int g = 1; g <= pr->mg.n_groups; ++g)
+        for (int e = goff[g - 1]; e < goff[g]; ++e) admits[gidx[e]][g - 1] = 1;
     for (int c = 0; c < pr->n_clerks; ++c) {
         const scn_clerk& K = pr->clerks[c];
         if (K.kind != SCN_CLERK_COLLISION) continue;
@@ -181,20 +200,33 @@ inline FastColl fast_collision_layout(const scn_problem* pr, const std::vector<i
             if (m.type != SCN_MAP_SPACE || m.grid_type != SCN_GRID_LIN) return F;
             if (m.data < 0 || m.data + 1 >= pr->n_real) return F;
         }
-        ids.push_back(c);
+        auto fits = [&](int q) {
+            if (!same_layout(K, pr->clerks[rep[q]])) return false;
+            for (int g = 0; g < nG; ++g)
+                if (admits[c][g] && taken[q][g]) return false;
+            return true;
+        };
+        int q = 0;
+        while (q < (int)rep.size() && !fits(q)) ++q;
+        if (q == (int)rep.size()) {
+            if (q == FAST_CLERKS) return F;
+            rep.push_back(c);
+            taken.emplace_back(nG, 0);
+        }
+        for (int g = 0; g < nG; ++g) taken[q][g] |= admits[c][g];
+        slot_of[c] = q;
     }
-    if (ids.empty() || (int)ids.size() > FAST_CLERKS) return F;
-    const int nc = (int)ids.size();
+    if (rep.empty()) return F;
+    const int nc = (int)rep.size();
     std::vector<int64_t> addr((size_t)nc * nG, -1);
     for (int g = 1; g <= pr->mg.n_groups; ++g)
         for (int e = goff[g - 1]; e < goff[g]; ++e) {
-            int q = 0;
-            while (q < nc && ids[q] != gidx[e]) ++q;
-            if (q == nc || addr[(size_t)q * nG + g - 1] >= 0) return F;   // not a collision clerk, or listed twice
+            const int q = slot_of[gidx[e]];
+            if (q < 0 || addr[(size_t)q * nG + g - 1] >= 0) return F;   // two clerks of one slot admit the group
             addr[(size_t)q * nG + g - 1] = pr->clerks[gidx[e]].mem_addr;
         }
     for (int q = 0; q < nc; ++q) {
-        const scn_clerk& K = pr->clerks[ids[q]];
+        const scn_clerk& K = pr->clerks[rep[q]];
         FastClerk& C = F.c[q];
         C.n_maps = K.n_maps;
         C.width = K.width;
