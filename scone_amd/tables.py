@@ -1000,6 +1000,15 @@ def compile_problem(inp: dict, mg_dicts: dict | None = None, base_dir: str = "."
         for k, sd in enumerate(subs):
             M = Map1D()
             if sd["type"] == "spaceMap":
+                # spaceMap%init (spaceMap_class.f90:91-134): its keyword checks and messages
+                if "grid" not in sd:
+                    raise ValueError("Keyword 'grid' must be present")
+                if "axis" not in sd:
+                    raise ValueError("Keyword 'axis' must be present")
+                if sd["axis"] not in ("x", "y", "z"):
+                    raise ValueError(f"Unrecognised axis: {sd['axis']} must be x, y or z")
+                if sd["grid"] not in ("lin", "unstruct"):
+                    raise ValueError("'grid' keyword must be: lin or unstruct")
                 M.type = MAP_SPACE
                 M.axis = {"x": 0, "y": 1, "z": 2}[sd["axis"]]
                 if sd["grid"] == "lin":
