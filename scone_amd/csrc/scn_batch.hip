@@ -94,6 +94,7 @@ k_items(const SlotDev* __restrict__ slots, SlotState* ss, int n_slots, int first
     if (threadIdx.x == 0) {
         ctl[0] = carry;      // items (at most n_slots * MAX_WINDOW: the table's size)
         ctl[5] = 0;          // items whose histories are all handed out
+        ctl[6] = 0;          // chunks of the material-source pre-pass handed out
         if (first) ctl[2] = 0;
     }
 }

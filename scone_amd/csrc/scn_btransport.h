@@ -88,7 +88,7 @@ struct SlotState {
 };
 
 // control words: [0] work items of the current launch, [2] running slots after the cycle,
-// [5] items whose particles are all handed out
+// [5] items whose particles are all handed out, [6] chunks of the source pre-pass handed out
 constexpr int CTL_WORDS = 8;
 
 // work item of one launch: the histories of one batch of one slot's window (first launch of a
@@ -101,8 +101,7 @@ struct Item {
 
 typedef void (*BTransportFn)(dim3, dim3, size_t, hipStream_t, const SlotDev*, SlotState*, const Item*, int*, int*, int, int,
                              int, int, int, DevStatus*, int, Sites, int);
-typedef void (*BSourceFn)(dim3, dim3, hipStream_t, const SlotDev*, const SlotState*, const Item*, const int*,
-                          DevStatus*);
+typedef void (*BSourceFn)(dim3, dim3, hipStream_t, const SlotDev*, const SlotState*, const Item*, int*, DevStatus*);
 
 // launchers and kernel address of one variant: register depth NL, tracking mode MODE, geometry
 // variant GEO (0: lattice/pin/box, 1: cells, rotations, CSG borders); src launches the
@@ -578,24 +577,23 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
 
 // Material-source pre-pass of a cycle's first launch: for every history of every item of a
 // material-source slot, the rejection loop of materialSource%sampleParticle on the history's
-// source stream (as born() would run it), keeping only how many attempts failed.  Waves take
-// chunks of SRC_CHUNK consecutive histories round-robin and work through a chunk one item at a
-// time, so the slot (and its tables) is wave-uniform; each lane takes a contiguous run of the
+// source stream (as born() would run it), keeping only how many attempts failed.  Waves claim
+// chunks of SRC_CHUNK consecutive histories from a counter (chunks of point-source items take no
+// time, so a fixed deal would leave waves idle) and work through a chunk one item at a time, so
+// the slot (and its tables) is wave-uniform; each lane takes a contiguous run of the
 // item's histories and makes one attempt per iteration, moving to its next history (its stream
 // one stride further: one LCG step) as soon as one is accepted.
 constexpr int SRC_CHUNK = 2048;
 template <int NL, int GEO>
 __global__ void __launch_bounds__(256)
-k_bsource(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss, const Item* __restrict__ items,
-          const int* __restrict__ ctl, DevStatus* st)
+k_bsource(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss, const Item* __restrict__ items, int* ctl,
+          DevStatus* st)
 {
     const int n_items = ctl[0];
     if (n_items <= 0) return;
     const int lane = threadIdx.x & (BWAVE - 1);
     const long long total = items[n_items - 1].prefix + items[n_items - 1].n;
     const long long n_chunks = (total + SRC_CHUNK - 1) / SRC_CHUNK;
-    const long long n_waves = (long long)gridDim.x * (blockDim.x / BWAVE);
-    const long long wave = (long long)blockIdx.x * (blockDim.x / BWAVE) + threadIdx.x / BWAVE;
     const uint64_t A1 = RNG_STRIDES.a[0][1], C1 = RNG_STRIDES.c[0][1];   // one stride of the LCG
     Pt<NL, GEO> p;
 #pragma unroll
@@ -606,7 +604,11 @@ k_bsource(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss, c
 #pragma unroll
         for (int q = 0; q < 3; ++q) p.r[l][q] = 0.0;
     }
-    for (long long c = wave; c < n_chunks; c += n_waves) {
+    for (;;) {
+        int claim = 0;
+        if (lane == 0) claim = atomicAdd(&ctl[6], 1);
+        const long long c = __builtin_amdgcn_readfirstlane(__shfl(claim, 0));
+        if (c >= n_chunks) break;
         long long a = c * SRC_CHUNK;
         const long long e = min(a + SRC_CHUNK, total);
         while (a < e) {   // one item of the chunk at a time (wave-uniform)
@@ -670,7 +672,7 @@ k_bsource(const SlotDev* __restrict__ slots, const SlotState* __restrict__ ss, c
 // the pre-pass over a grid of every resident block of its kernel on g.x CUs
 template <int NL, int GEO>
 void launch_bsource(dim3 g, dim3 b, hipStream_t s, const SlotDev* slots, const SlotState* ss, const Item* items,
-                    const int* ctl, DevStatus* st)
+                    int* ctl, DevStatus* st)
 {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_bsource<NL, GEO>, (int)b.x, 0) != hipSuccess)
