@@ -188,6 +188,29 @@ constexpr scn_rng_stride_table scn_make_rng_stride_table()
     return t;
 }
 
+/* the generator advanced by 4b steps, b < 256: skips the 4-draw attempts of a rejection loop */
+struct scn_rng_skip4_table {
+    uint64_t a[256];
+    uint64_t c[256];
+};
+
+constexpr scn_rng_skip4_table scn_make_rng_skip4_table()
+{
+    scn_rng_skip4_table t{};
+    uint64_t a4 = 1ULL, c4 = 0ULL;
+    for (int i = 0; i < 4; ++i) {
+        c4 = (SCN_LCG_G * c4 + 1ULL) & SCN_LCG_MASK;
+        a4 = (SCN_LCG_G * a4) & SCN_LCG_MASK;
+    }
+    t.a[0] = 1ULL;
+    t.c[0] = 0ULL;
+    for (int b = 1; b < 256; ++b) {
+        t.c[b] = (a4 * t.c[b - 1] + c4) & SCN_LCG_MASK;
+        t.a[b] = (a4 * t.a[b - 1]) & SCN_LCG_MASK;
+    }
+    return t;
+}
+
 SCN_HD uint64_t scn_rng_stride_bytes(const scn_rng_stride_table& t, uint64_t s, int64_t n)
 {
     if (n < 0 || n >= ((int64_t)1 << 40)) return scn_rng_stride(s, n);

@@ -203,15 +203,15 @@ __device__ __forceinline__ int src_attempt(const SlotDev& D, const DevProblem& P
 
 // Source particle j = i + 1 of batch `batch` (batch w of the slot's window), then the start of
 // its history (fixedSourcePhysicsPackage_class.f90:187-213): the source is sampled on the seed
-// strided by 2*pop*batch + j (generate, source_inter.f90:110-116) and the history runs on the seed
-// strided by 2*pop*batch + pop + j.  Returns 0 if sampling failed (error raised).
+// strided by 2*pop*batch + j (generate, source_inter.f90:110-116), `src` here, and the history
+// runs on the seed strided by 2*pop*batch + pop + j, which is src strided by pop: the map
+// (run_a, run_c).  Returns 0 if sampling failed (error raised).
 template <int NL, int GEO>
-__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int batch, int w, int i, Pt<NL, GEO>& p,
-                                    DevStatus* st)
+__device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int w, int i, uint64_t src, uint64_t run_a,
+                                    uint64_t run_c, Pt<NL, GEO>& p, DevStatus* st)
 {
     const int64_t j = (int64_t)i + 1;
-    const int64_t b2 = 2 * (int64_t)D.pop * batch;
-    uint64_t srng = rng_stride(D.seed, b2 + j);
+    uint64_t srng = src;
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         p.loc[l] = 0;
@@ -257,7 +257,7 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
         // waits on its lanes' longest rejection run
         const int k = D.src_att[(size_t)w * D.pop + i];
         if (k >= MATERIAL_SOURCE_ATTEMPTS) return 0;   // the pre-pass raised the error
-        srng = rng_skip(srng, 4 * (int64_t)k);
+        srng = (RNG_SKIP4.a[k] * srng + RNG_SKIP4.c[k]) & SCN_LCG_MASK;   // 4k draws on (k < 256)
         src_point(D, srng, r, time);
         double mu = 2.0 * scn_rng_get(&srng) - 1.0;
         double phi = SCN_TWO_PI * scn_rng_get(&srng);
@@ -265,7 +265,7 @@ __device__ __forceinline__ int born(const SlotDev& D, const DevProblem& P, int b
         scn_rotate(x1, mu, phi, dir);
         G = D.src_G;
     }
-    p.rng = rng_stride(D.seed, b2 + D.pop + j);
+    p.rng = (run_a * src + run_c) & SCN_LCG_MASK;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         p.r[0][k] = r[k];
@@ -400,6 +400,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
     DistCache<NL> cache;   // surface tracking: the current flight's distance cache
     cache.lvl = 0;
     int alive = 0, steps = 0, nseg = 0;
+    uint64_t run_a = 1, run_c = 0;   // stride by pop histories, the slot's (wave-uniform)
     int use_dt = MODE == SCN_TRANSPORT_DT;   // this flight's tracking method (HT: chosen per flight)
     int fresh = 1;                           // the next event starts a flight
     for (int iter = 0;; ++iter) {
@@ -450,6 +451,10 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
             n_in = SS->n_in;
             batch = SS->batch + w;
             cap = D->pop;
+            if (first) {   // (a, c) of the stride by pop: s -> a s + c
+                run_c = rng_stride(0, D->pop);
+                run_a = (rng_stride(1, D->pop) - run_c) & SCN_LCG_MASK;
+            }
             // fixed point: four limb words per bin, in BIN and in the wave's LDS copy
             B.fix = D->P.fix;
             B.glob = D->bin + (size_t)w * (size_t)D->P.n_bins * (B.fix ? 4 : 1);
@@ -462,10 +467,16 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
         // many lanes at a time rather than once for every lane that frees up
         if (next < end && (__popcll(~live) >= refill || live == 0)) {
             const unsigned long long idle = ~live;
-            const int i = next + __popcll(idle & lt_mask);
+            const int rank = __popcll(idle & lt_mask);
+            const int i = next + rank;
+            // the source stream of history `next` (wave-uniform), then each lane's own: its
+            // rank more strides on, one map of the first byte table
+            uint64_t src = 0;
+            if (first) src = rng_stride(D->seed, 2 * (int64_t)D->pop * batch + next + 1);
             if (!alive && i < end) {
                 if (first) {
-                    alive = born(*D, D->P, batch, key & (MAX_WINDOW - 1), i, p, st);
+                    src = (RNG_STRIDES.a[0][rank] * src + RNG_STRIDES.c[0][rank]) & SCN_LCG_MASK;
+                    alive = born(*D, D->P, key & (MAX_WINDOW - 1), i, src, run_a, run_c, p, st);
                 } else {
                     bload(D->q[in_side], cap, i, p);
                     relink(D->P, p);

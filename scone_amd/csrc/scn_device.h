@@ -66,6 +66,7 @@ __device__ __forceinline__ uint64_t rng_skip(uint64_t s, int64_t k_in)
 }
 
 static __constant__ scn_rng_stride_table RNG_STRIDES = scn_make_rng_stride_table();
+static __constant__ scn_rng_skip4_table RNG_SKIP4 = scn_make_rng_skip4_table();
 
 // stride by n histories: the five byte maps of RNG_STRIDES (scn_rng_stride_bytes), the same state
 // as rng_skip(s, 152917 n)
