@@ -464,7 +464,7 @@ def batch_line(args):
     if args.tune:
         t = time.perf_counter()
         best, table = autotune.tune([(c, s, pop, seed) for (c, s, pop, _, seed) in problems], transport, budget,
-                                    tiles=(64, 128, 256, 512), launches=(1,), windows=(window,), refills=(16, 32),
+                                    tiles=(64, 128, 256, 512, 1024), launches=(1,), windows=(window,), refills=(16, 32),
                                     log=log)
         tune_s = time.perf_counter() - t
     else:
