@@ -95,7 +95,8 @@ def test_all_512_batched_problems_equal_plain():
         budget += 2 * pop * 200 + 8 * 12 * int(comp.problem.n_bins) + (1 << 20)
     budget = int(budget * 1.25) + (256 << 20)
     print("case batched run", flush=True)
-    B = Batch(transport=T.TRANSPORT_DT, max_slots=n, hbm_budget=budget, use_graph=1, tile=256, n_launch=1)
+    # the launch shape the batched benchmark's autotuner picks (tile 1024, refill 16)
+    B = Batch(transport=T.TRANSPORT_DT, max_slots=n, hbm_budget=budget, use_graph=1, tile=1024, n_launch=1, refill=16)
     slots = [B.add(c, s, pop, cyc, seed, window=10) for (c, s, pop, cyc, seed) in probs]
     B.run_all()
     batched = [B.query(s) for s in slots]
