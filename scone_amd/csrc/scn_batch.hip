@@ -533,7 +533,9 @@ int max_depth_nl(Batch* B)
     return compiled_nl(d, geo) + (geo ? GEO_KEY : 0);
 }
 
-size_t lds_bytes(const Batch* B) { return sizeof(double) * (size_t)B->S_lds_per_wave * (BBLOCK / BWAVE); }
+// a wave's copy of a slot's low bins, then its lanes' queues of evicted scores
+size_t lds_bytes_for(int lds_per_wave) { return sizeof(double) * ((size_t)lds_per_wave + 2 * BQUEUE * BWAVE) * (BBLOCK / BWAVE); }
+size_t lds_bytes(const Batch* B) { return lds_bytes_for(B->S_lds_per_wave); }
 
 // persistent grid of the transport kernel variant: resident blocks per CU x CUs
 // a resident solve samples a material source (the cycle then runs the sampling pre-pass)
@@ -1204,7 +1206,7 @@ int64_t scn_batch_secondary_bytes(int32_t transport, int32_t buffer)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return -1;
     n_cu = prop.multiProcessorCount;
-    const size_t lds = sizeof(double) * (size_t)Batch{}.S_lds_per_wave * (BBLOCK / BWAVE);
+    const size_t lds = lds_bytes_for(Batch{}.S_lds_per_wave);
     return (int64_t)stack_bytes_for(max_grid_lanes(transport, n_cu, lds), buffer > 0 ? buffer : DEFAULT_BUFFER);
 }
 

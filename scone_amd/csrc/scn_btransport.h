@@ -19,6 +19,7 @@ using namespace scn;
 
 constexpr int BBLOCK = 256;
 constexpr int BWAVE = 64;
+constexpr int BQUEUE = 4;   // evicted global scores a lane queues in LDS before its wave adds them
 constexpr int MAX_SLOTS = 4096;
 constexpr int MATERIAL_SOURCE_ATTEMPTS = 200;   // materialSource_class.f90:168
 constexpr int MAX_PROBG = 4096;
@@ -359,6 +360,8 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
     // this wave's private copy of the first lds_per_wave bins of its current slot
     double* wl = lds_all + (size_t)(threadIdx.x / BWAVE) * lds_per_wave;
     for (int i = lane; i < lds_per_wave; i += BWAVE) wl[i] = 0.0;
+    // after the block's bin copies, each wave's queues of evicted scores: 2 BQUEUE words per lane
+    double* wq = lds_all + (size_t)(BBLOCK / BWAVE) * lds_per_wave + (size_t)(threadIdx.x / BWAVE) * 2 * BQUEUE * BWAVE;
     const int n_items = ctl[0];
     const int lane_gid = blockIdx.x * BBLOCK + threadIdx.x;   // owner of one LIFO buffer in SB
     int key = -1;                    // current slot * MAX_WINDOW + batch of its window (wave-uniform)
@@ -394,6 +397,9 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
     B.lds_n = 0;
     B.c_addr = -1;
     B.c_acc = 0.0;
+    B.q = wq + lane;
+    B.qn = 0;
+    B.qcap = BQUEUE;
     B.fix = 0;
     Cnt cn{0, 0, 0};
     Pt<NL, GEO> p;
@@ -562,6 +568,7 @@ k_btransport(const SlotDev* __restrict__ slots, SlotState* ss, const Item* __res
                 }
             }
         }
+        if (__ballot(B.qn >= B.qcap)) bins_drain(B);   // a full queue: the wave adds all of them
         const unsigned long long pm = __ballot(park);
         if (pm) {
             int obase = 0;

@@ -1450,14 +1450,21 @@ __device__ __forceinline__ int map1d(const DevProblem& P, const scn_map1d& m, co
 // rest go to the global BIN array through a one-entry per-lane cache: consecutive scores of a
 // lane to the same bin are summed in a register and added with one atomic when the lane scores
 // another bin (bins_flush at the end).  Only the summation order of BIN changes.
+// A score evicted from the cache waits in the lane's LDS queue (qcap entries, when the kernel
+// gives one) and the wave adds every lane's queue at once (bins_drain): a global atomic is
+// performed beyond L2, and the in-order count of vector memory operations makes the next load
+// wait for it, so one wait per drain replaces one in every iteration that evicts.
 // With fix = 1 every score goes straight to its bin's four limb words (scn_fix_limbs, integer
 // atomics), in LDS or global memory; the cache is not used, and the sums are order-independent.
+constexpr int BINQ_STRIDE = 64;   // lanes of a wave: entry k of a lane is q[2k * 64] (bin), q[(2k + 1) * 64]
 struct Bins {
     double* lds;
     double* glob;
     int64_t lds_n;
     int64_t c_addr;    // bin held in the lane's cache, -1 none
     double c_acc;
+    double* q;         // the lane's queue of evicted scores in LDS (null: none)
+    int qn, qcap;      // entries queued, capacity (wave-uniform)
     int fix;
 };
 
@@ -1470,11 +1477,31 @@ __device__ __forceinline__ void fix_add(unsigned long long* w, double v)
         if (l[k]) atomicAdd(w + k, (unsigned long long)l[k]);
 }
 
+// add the lane's queued scores to BIN
+__device__ __forceinline__ void bins_drain(Bins& B)
+{
+    for (int k = 0; k < B.qn; ++k)
+        unsafeAtomicAdd(&B.glob[__double_as_longlong(B.q[2 * k * BINQ_STRIDE])], B.q[(2 * k + 1) * BINQ_STRIDE]);
+    B.qn = 0;
+}
+
 __device__ __forceinline__ void bins_flush(Bins& B)
 {
     if (B.c_addr >= 0) unsafeAtomicAdd(&B.glob[B.c_addr], B.c_acc);
     B.c_addr = -1;
     B.c_acc = 0.0;
+    bins_drain(B);
+}
+
+__device__ __forceinline__ void bin_evict(Bins& B, int64_t addr, double v)
+{
+    if (B.qn < B.qcap) {
+        B.q[2 * B.qn * BINQ_STRIDE] = __longlong_as_double(addr);
+        B.q[(2 * B.qn + 1) * BINQ_STRIDE] = v;
+        ++B.qn;
+    } else {
+        unsafeAtomicAdd(&B.glob[addr], v);
+    }
 }
 
 __device__ __forceinline__ void bin_add(Bins& B, int64_t addr, double v)
@@ -1488,7 +1515,7 @@ __device__ __forceinline__ void bin_add(Bins& B, int64_t addr, double v)
     } else if (addr == B.c_addr) {
         B.c_acc = B.c_acc + v;
     } else {
-        if (B.c_addr >= 0) unsafeAtomicAdd(&B.glob[B.c_addr], B.c_acc);
+        if (B.c_addr >= 0) bin_evict(B, B.c_addr, B.c_acc);
         B.c_addr = addr;
         B.c_acc = v;
     }
@@ -1518,8 +1545,9 @@ __device__ __forceinline__ int report_in_coll(const DevProblem& P, Pt<NL, GEO>& 
         for (int c = 0; c < FAST_CLERKS; ++c) {
             if (c >= P.fc.n_clerks) break;
             const FastClerk& F = P.fc.c[c];
+            if (!F.handle_virtual && isVirtual) continue;   // wave-uniform: no address load
             const int64_t a0 = P.fc.g_addr[(size_t)c * P.nG + (p.G - 1)];
-            if (a0 < 0 || (!F.handle_virtual && isVirtual)) continue;
+            if (a0 < 0) continue;
             int bin = 1, inside = 1;
             for (int k = 0; k < F.n_maps; ++k) {
                 const double v = get3(p.r[0], F.axis[k]);

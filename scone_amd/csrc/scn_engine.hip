@@ -1217,7 +1217,7 @@ int transport_cycle(Engine* E, int n_in, uint64_t master, double k_eff, int* n_s
     // a null BIN pointer means "do not score": point at a scratch copy instead
     double* bins_arg = bins ? bins : E->bins;
     int use_lds = score_bins ? E->use_lds : 0;
-    size_t sh = use_lds ? (size_t)E->lds_bytes : 0;
+    size_t sh = use_lds ? (size_t)E->lds_bytes : EQUEUE_BYTES;
     int occ = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, BLOCK, sh));
     const int persistent = E->n_cu * std::max(1, occ);
@@ -1494,7 +1494,7 @@ int enqueue_dcycle(Engine* E, int phase, int score_bins)
                      : (E->transport == SCN_TRANSPORT_HT) ? pick_nl<SCN_TRANSPORT_HT>(E->nl, E->geo, &kern)
                                                           : pick_nl<SCN_TRANSPORT_ST>(E->nl, E->geo, &kern);
     const int use_lds = score_bins ? E->use_lds : 0;
-    const size_t sh = use_lds ? (size_t)E->lds_bytes : 0;
+    const size_t sh = use_lds ? (size_t)E->lds_bytes : EQUEUE_BYTES;
     const int grid = std::max(1, std::min((pop + BLOCK - 1) / BLOCK, E->n_cu * std::max(1, E->dc_occ[phase])));
     DevProblem P = E->P;
     if (!score_bins) P.n_clerks = 0;
@@ -1614,7 +1614,7 @@ int dcycle_setup(Engine* E, const scn_eigen_settings& S)
         if (E->transport == SCN_TRANSPORT_DT) pick_nl<SCN_TRANSPORT_DT>(E->nl, E->geo, &kern);
         else if (E->transport == SCN_TRANSPORT_HT) pick_nl<SCN_TRANSPORT_HT>(E->nl, E->geo, &kern);
         else pick_nl<SCN_TRANSPORT_ST>(E->nl, E->geo, &kern);
-        const size_t sh = (score_bins && E->use_lds) ? (size_t)E->lds_bytes : 0;
+        const size_t sh = (score_bins && E->use_lds) ? (size_t)E->lds_bytes : EQUEUE_BYTES;
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&E->dc_occ[ph], kern, BLOCK, sh));
         HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_dcycle(E, ph, score_bins);

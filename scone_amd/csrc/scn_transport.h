@@ -18,6 +18,10 @@ namespace scn_eng {
 using namespace scn;
 
 constexpr int BLOCK = 256;
+// without an LDS copy of the score memory, the dynamic LDS holds each lane's queue of EQUEUE
+// evicted global scores (Bins::q)
+constexpr int EQUEUE = 4;
+constexpr size_t EQUEUE_BYTES = sizeof(double) * 2 * EQUEUE * BLOCK;
 constexpr int WAVE = 64;
 constexpr int LDS_BINS_MAX = 8192;   // collision-clerk memories up to 64 KiB are privatised in LDS
 constexpr int DEFAULT_EIGEN_REFILL = 1;    // idle lanes a wave gathers before it starts new histories
@@ -170,6 +174,9 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
     B.lds_n = use_lds ? P.n_bins : 0;
     B.c_addr = -1;
     B.c_acc = 0.0;
+    B.q = use_lds ? nullptr : lds_bins + (size_t)(threadIdx.x / WAVE) * 2 * EQUEUE * WAVE + (threadIdx.x & (WAVE - 1));
+    B.qn = 0;
+    B.qcap = use_lds ? 0 : EQUEUE;
     B.fix = P.fix;
     Cnt cn{0, 0, 0};
     const int lane = threadIdx.x & (WAVE - 1);
@@ -274,6 +281,7 @@ k_transport(DevProblem P, const scn_particle_state* __restrict__ bank, int first
                 if (cin.hist_nsite) cin.hist_nsite[p.brood - 1] = p.nsite;
             }
         }
+        if (__ballot(B.qn >= B.qcap && B.qcap > 0)) bins_drain(B);   // a full queue: the wave adds all of them
         // park step-limited survivors: wave ballot, one atomic per wave
         const unsigned long long pm = __ballot(park);
         if (pm) {
