@@ -14,8 +14,10 @@ dividing K, and the line says so).  Every problem therefore runs W' + K batches.
   plain    the same engine holding one problem at a time (each problem's batches run alone)
 Both run the identical histories (checked on a sample of problems: score results equal to 1e-12).
 Also reported: the serving loop (continuous batching with HBM-budget admission and 1 % early stop)
-on the 17x17 pin-lattice request stream (fixed-source solves of the fissile lattice, fission chains
-followed as secondaries) at 50/75/100 % of measured capacity with p50/p99 latency, the plain
+at 50/75/100 % of measured capacity with p50/p99 latency, on two request streams: solves drawn
+from the 512 batched-workload problems (a random problem on a seed of its own per request), and
+the 17x17 pin-lattice stream (fixed-source solves of the fissile lattice, fission chains followed
+as secondaries); the plain
 eigenvalue engine on the 17x17 lattice (1e6 histories per cycle) and the 3D core (1e7 per cycle).
 
 The CPU baseline is the C oracle on the host: every one of the 512 problems, `--cpu-batches`
@@ -335,6 +337,60 @@ def serving(transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, 
     return out
 
 
+def serving_mix(problems, transport, seconds_per_load, loads=(0.5, 0.75, 1.0), stop_rel=0.01, seed=11, window=10,
+                adapt=50, slots=256, saturate=4096, budget=8 << 30):
+    """Continuous batching of requests drawn from the batched workload's problems: each request is
+    one of the 512 problems (boxes, 1-8 regions, 2-70 groups, 1e3-1e6 histories in 10 batches, a
+    mesh flux tally), picked at random, on a seed of its own, stopping early at `stop_rel` relative
+    sd of its total-flux bin (bin 0).  Capacity: `saturate` requests waiting at t = 0, adaptive
+    windows; latency at offered Poisson loads of that capacity, `seconds_per_load` each."""
+    from scone_amd import tables as T
+    from scone_amd.engine import Batch
+    from scone_amd.serve import Request, Server, latency_summary
+    rng = np.random.default_rng(seed)
+    k = [0]
+
+    def request(arrival):
+        comp, src, pop, cycles, _ = problems[int(rng.integers(len(problems)))]
+        k[0] += 1
+        return Request(comp, src, pop, cycles, 500000 + k[0], stop_bin=0, stop_rel=stop_rel, arrival=arrival,
+                       window=min(window, cycles), adapt=adapt)
+
+    def make():
+        B = Batch(transport=T.TRANSPORT_DT if transport == "transportOperatorDT" else T.TRANSPORT_ST,
+                  max_slots=slots, hbm_budget=budget, use_graph=1)
+        return B, Server(B)
+
+    out = {"window": window, "adapt": adapt, "slots": slots, "hbm_budget": budget, "stop_rel": stop_rel, "loads": {}}
+    B, srv = make()
+    reqs = [request(0.0) for _ in range(saturate)]
+    dt = srv.run_wall(reqs)
+    hist = sum(r.result.n_histories for r in reqs)
+    B.close()
+    out.update(capacity_hist_per_s=hist / dt, capacity_req_per_s=saturate / dt, capacity_cycles=srv.cycles,
+               mean_request_histories=hist / saturate)
+    log(f"serving (batched-workload requests) capacity {hist / dt:.3e} histories/s, {saturate / dt:.1f} requests/s, "
+        f"{srv.cycles} cycles, {dt:.2f} s")
+    cap_req = saturate / dt
+    for load in loads:
+        lam = load * cap_req
+        n = max(1, int(lam * seconds_per_load))
+        arr = np.cumsum(rng.exponential(1.0 / lam, n))
+        reqs = [request(float(a)) for a in arr]
+        B, srv = make()
+        el = srv.run_wall(reqs)
+        hist = sum(r.result.n_histories for r in reqs)
+        B.close()
+        summ = latency_summary(reqs)
+        summ.update({"offered_req_per_s": lam, "hist_per_s": hist / el, "elapsed_s": el, "cycles": srv.cycles,
+                     "busy_rejections": srv.busy_rejections,
+                     "early_stopped": int(sum(1 for r in reqs if r.result.n_batches < r.n_batches))})
+        out["loads"][str(load)] = summ
+        log(f"serving (batched-workload requests) load {load}: p50 {summ['p50_s'] * 1e3:.1f} ms, "
+            f"p99 {summ['p99_s'] * 1e3:.1f} ms, {summ['hist_per_s']:.3e} histories/s")
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 # eigenvalue (plain engine, 17x17 lattice)
 # ---------------------------------------------------------------------------------------------
@@ -485,6 +541,8 @@ def batch_line(args):
     cpu = None if args.no_cpu else cpu_batch_baseline(problems, args.cpu_batches)
     serve = None if args.serve_seconds <= 0 else serving(args.transport, args.serve_seconds, window=args.window,
                                                                   engines=args.engines)
+    serve_mix = None if args.serve_seconds <= 0 else serving_mix(problems, args.transport, args.serve_seconds,
+                                                                         window=args.window)
     eig = None
     if args.eigen:
         log("eigen leg")
@@ -535,6 +593,7 @@ def batch_line(args):
         "speedup_vs_cpu": (bat["hist_per_s"] / cpu["value"]) if cpu else None,
         "autotune": {"seconds": tune_s, "candidates": len(table)},
         "serving": serve,
+        "serving_batched_requests": serve_mix,
         "eigen_17x17_1e6": eig,
         "eigen_core3d_70g_1e7": core,
     }

@@ -290,6 +290,10 @@ struct Batch {
     hipEvent_t c0 = nullptr, c1 = nullptr;
     scn_stats stats{};
     int64_t rel_seg = 0, rel_col = 0, rel_sc = 0, rel_hist = 0;   // counters of released slots
+    // pinned host staging of the small readbacks (slot states, score sums): a pageable copy
+    // goes through the runtime's own staging and costs several times more per call
+    char* pin = nullptr;
+    size_t pin_bytes = 0;
     int S_lds_per_wave = 2048;   // low bins of a slot privatised per wave in LDS (16 KiB per wave)
     // per-lane LIFO buffers of fixed-source secondaries: a block of the arena (so inside the HBM
     // budget), reserved when a solve with secondaries is admitted, for max_lanes lanes of the
@@ -737,6 +741,23 @@ int run_cycle(Batch* B, int* n_running_out)
 
 }  // namespace
 
+// the pinned staging buffer, grown to at least `bytes` (null if the allocation fails)
+static char* pinned(Batch* B, size_t bytes)
+{
+    if (bytes <= B->pin_bytes) return B->pin;
+    size_t want = std::max<size_t>(size_t(1) << 16, B->pin_bytes);
+    while (want < bytes) want *= 2;
+    if (B->pin) (void)hipHostFree(B->pin);
+    B->pin = nullptr;
+    B->pin_bytes = 0;
+    if (hipHostMalloc((void**)&B->pin, want, hipHostMallocDefault) != hipSuccess) {
+        B->pin = nullptr;
+        return nullptr;
+    }
+    B->pin_bytes = want;
+    return B->pin;
+}
+
 // =============================================================================================
 // C ABI
 // =============================================================================================
@@ -819,6 +840,7 @@ void scn_batch_destroy(void* batch)
     if (!B) return;
     if (B->stream) (void)hipStreamSynchronize(B->stream);
     drop_graph(B);
+    if (B->pin) (void)hipHostFree(B->pin);
     if (B->arena) (void)hipFree(B->arena);
     if (B->d_items) (void)hipFree(B->d_items);
     if (B->d_item_cnt) (void)hipFree(B->d_item_cnt);
@@ -1090,9 +1112,26 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bi
         if (res) *res = r;
         return 0;
     }
-    SlotState st{};
-    BHIPCHK(hipMemcpyAsync(&st, B->d_state + slot, sizeof st, hipMemcpyDeviceToHost, B->stream));
+    // the slot's counters and, when asked for, its score sums come back behind one synchronisation
+    // (the sums of a slot still running are read and dropped)
+    const SlotDev& D = B->h_slots[slot];
+    const bool want_bins = bins && h.n_bins > 0;
+    const size_t nb = want_bins ? (size_t)h.n_bins : 0;
+    char* stage = pinned(B, sizeof(SlotState) + 2 * sizeof(double) * nb);
+    if (!stage) {
+        B->err = "hipHostMalloc of the readback staging failed";
+        return SCN_ERR_HIP;
+    }
+    double* cs = (double*)(stage + sizeof(SlotState));
+    double* cs2 = cs + nb;
+    BHIPCHK(hipMemcpyAsync(stage, B->d_state + slot, sizeof(SlotState), hipMemcpyDeviceToHost, B->stream));
+    if (want_bins) {
+        BHIPCHK(hipMemcpyAsync(cs, D.csum, sizeof(double) * nb, hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipMemcpyAsync(cs2, D.csum2, sizeof(double) * nb, hipMemcpyDeviceToHost, B->stream));
+    }
     BHIPCHK(hipStreamSynchronize(B->stream));
+    SlotState st;
+    std::memcpy(&st, stage, sizeof st);
     r.state = st.running ? SCN_SLOT_RUNNING : SCN_SLOT_DONE;
     r.n_batches = st.batch;
     r.n_bins = h.n_bins;
@@ -1101,12 +1140,7 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bi
     r.n_segments = (int64_t)st.segments;
     if (res) *res = r;
     if (r.state != SCN_SLOT_DONE) return 0;
-    const SlotDev& D = B->h_slots[slot];
-    if (bins && h.n_bins > 0) {
-        std::vector<double> cs(h.n_bins), cs2(h.n_bins);
-        BHIPCHK(hipMemcpyAsync(cs.data(), D.csum, sizeof(double) * h.n_bins, hipMemcpyDeviceToHost, B->stream));
-        BHIPCHK(hipMemcpyAsync(cs2.data(), D.csum2, sizeof(double) * h.n_bins, hipMemcpyDeviceToHost, B->stream));
-        BHIPCHK(hipStreamSynchronize(B->stream));
+    if (want_bins) {
         const int N = std::max(1, st.batch);
         for (int64_t i = 0; i < h.n_bins; ++i) score_result(cs[i], cs2[i], N, bins + 2 * i);
     }
@@ -1164,8 +1198,12 @@ int scn_batch_states(void* batch, int32_t* states, int32_t n)
     Batch* B = (Batch*)batch;
     if (!B || !states || n < 0 || n > B->n_slots) return SCN_ERR_ARG;
     if (n == 0) return 0;
-    std::vector<SlotState> ss(n);
-    BHIPCHK(hipMemcpyAsync(ss.data(), B->d_state, sizeof(SlotState) * n, hipMemcpyDeviceToHost, B->stream));
+    SlotState* ss = (SlotState*)pinned(B, sizeof(SlotState) * n);
+    if (!ss) {
+        B->err = "hipHostMalloc of the readback staging failed";
+        return SCN_ERR_HIP;
+    }
+    BHIPCHK(hipMemcpyAsync(ss, B->d_state, sizeof(SlotState) * n, hipMemcpyDeviceToHost, B->stream));
     BHIPCHK(hipStreamSynchronize(B->stream));
     for (int i = 0; i < n; ++i)
         states[i] = !B->hs[i].used ? SCN_SLOT_FREE : (ss[i].running ? SCN_SLOT_RUNNING : SCN_SLOT_DONE);
@@ -1178,9 +1216,15 @@ int scn_batch_release(void* batch, int32_t slot)
     if (!B || slot < 0 || slot >= B->n_slots) return SCN_ERR_ARG;
     SlotHost& h = B->hs[slot];
     if (!h.used) return 0;
-    SlotState st{};
-    BHIPCHK(hipMemcpyAsync(&st, B->d_state + slot, sizeof st, hipMemcpyDeviceToHost, B->stream));
+    char* stage = pinned(B, sizeof(SlotState));
+    if (!stage) {
+        B->err = "hipHostMalloc of the readback staging failed";
+        return SCN_ERR_HIP;
+    }
+    BHIPCHK(hipMemcpyAsync(stage, B->d_state + slot, sizeof(SlotState), hipMemcpyDeviceToHost, B->stream));
     BHIPCHK(hipStreamSynchronize(B->stream));
+    SlotState st;
+    std::memcpy(&st, stage, sizeof st);
     if (st.running) {
         B->err = "slot still running";
         return SCN_ERR_ARG;
@@ -1189,9 +1233,9 @@ int scn_batch_release(void* batch, int32_t slot)
     B->rel_col += (int64_t)st.collisions;
     B->rel_sc += (int64_t)st.scores;
     B->rel_hist += (int64_t)st.histories;
-    SlotState z{};
-    BHIPCHK(hipMemcpyAsync(B->d_state + slot, &z, sizeof z, hipMemcpyHostToDevice, B->stream));
-    BHIPCHK(hipStreamSynchronize(B->stream));
+    // cleared in stream order (before any later upload or cycle); the source outlives the copy
+    static const SlotState zero_state{};
+    BHIPCHK(hipMemcpyAsync(B->d_state + slot, &zero_state, sizeof zero_state, hipMemcpyHostToDevice, B->stream));
     arena_free(B, h.off, h.bytes);
     h = SlotHost{};
     B->h_slots[slot] = SlotDev{};

@@ -320,6 +320,16 @@ class Batch:
     def query(self, slot):
         """(FixedResult, bins (n_bins, 2) or None, hist_rng or None, hist_coll or None)."""
         r = T.FixedResult()
+        held = self._held.get(slot)
+        if held is not None and not held[3]:
+            # one call: the bins come back with the state (dropped while the solve runs)
+            nb = int(held[0].problem.n_bins)
+            bins = np.zeros(max(1, 2 * nb))
+            self._check(self.L.scn_batch_query(self.h, slot, C.byref(r), bins.ctypes.data_as(C.POINTER(C.c_double)),
+                                               None, None))
+            if r.state != T.SLOT_DONE:
+                return r, None, None, None
+            return r, bins[: 2 * int(r.n_bins)].reshape(-1, 2), None, None
         self._check(self.L.scn_batch_query(self.h, slot, C.byref(r), None, None, None))
         if r.state != T.SLOT_DONE:
             return r, None, None, None
