@@ -262,6 +262,8 @@ struct SlotHost {
     int ww = 0;         // weight windows: split copies use the history's buffer
     int mat_src = 0;    // material source: the cycle's first launch runs the sampling pre-pass
     int window_max = 1; // window the solve was admitted with (BIN copies allocated)
+    int done_known = 0; // a query saw the solve done: its counters (done_st) no longer change
+    SlotState done_st{};
 };
 
 struct Batch {
@@ -1062,6 +1064,7 @@ int scn_batch_add(void* batch, const scn_fixed_request* R, int32_t* slot)
     B->h_slots[s] = D;
     SlotHost& h = B->hs[s];
     h.used = 1;
+    h.done_known = 0;
     h.off = off;
     h.bytes = Ly.total;
     h.depth = depth;
@@ -1105,7 +1108,7 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bi
 {
     Batch* B = (Batch*)batch;
     if (!B || slot < 0 || slot >= B->n_slots) return SCN_ERR_ARG;
-    const SlotHost& h = B->hs[slot];
+    SlotHost& h = B->hs[slot];
     scn_fixed_result r{};
     if (!h.used) {
         r.state = SCN_SLOT_FREE;
@@ -1132,6 +1135,10 @@ int scn_batch_query(void* batch, int32_t slot, scn_fixed_result* res, double* bi
     BHIPCHK(hipStreamSynchronize(B->stream));
     SlotState st;
     std::memcpy(&st, stage, sizeof st);
+    if (!st.running) {
+        h.done_st = st;
+        h.done_known = 1;
+    }
     r.state = st.running ? SCN_SLOT_RUNNING : SCN_SLOT_DONE;
     r.n_batches = st.batch;
     r.n_bins = h.n_bins;
@@ -1216,15 +1223,19 @@ int scn_batch_release(void* batch, int32_t slot)
     if (!B || slot < 0 || slot >= B->n_slots) return SCN_ERR_ARG;
     SlotHost& h = B->hs[slot];
     if (!h.used) return 0;
-    char* stage = pinned(B, sizeof(SlotState));
-    if (!stage) {
-        B->err = "hipHostMalloc of the readback staging failed";
-        return SCN_ERR_HIP;
-    }
-    BHIPCHK(hipMemcpyAsync(stage, B->d_state + slot, sizeof(SlotState), hipMemcpyDeviceToHost, B->stream));
-    BHIPCHK(hipStreamSynchronize(B->stream));
     SlotState st;
-    std::memcpy(&st, stage, sizeof st);
+    if (h.done_known) {
+        st = h.done_st;   // read by the query that saw it done
+    } else {
+        char* stage = pinned(B, sizeof(SlotState));
+        if (!stage) {
+            B->err = "hipHostMalloc of the readback staging failed";
+            return SCN_ERR_HIP;
+        }
+        BHIPCHK(hipMemcpyAsync(stage, B->d_state + slot, sizeof(SlotState), hipMemcpyDeviceToHost, B->stream));
+        BHIPCHK(hipStreamSynchronize(B->stream));
+        std::memcpy(&st, stage, sizeof st);
+    }
     if (st.running) {
         B->err = "slot still running";
         return SCN_ERR_ARG;
