@@ -24,6 +24,8 @@ GIL), so the transport of one engine fills the device while another waits on its
 from __future__ import annotations
 
 import collections
+import contextlib
+import gc
 import threading
 import time
 from dataclasses import dataclass, field
@@ -32,6 +34,20 @@ import numpy as np
 
 from . import tables as T
 from .engine import Batch, Busy
+
+
+@contextlib.contextmanager
+def _no_cyclic_gc():
+    """Pause the cyclic garbage collector for a serving run.  A full collection walks every
+    resident request object (1.35 s with 3 million requests on one host core) and stalls admission
+    and harvest for that long; the loop makes no reference cycles, so nothing accumulates."""
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
 
 
 @dataclass
@@ -114,6 +130,10 @@ class Server:
 
     def run_wall(self, requests, keep=False, progress=None):
         """Wall-clock mode: request.arrival is seconds after the start.  Returns the elapsed time."""
+        with _no_cyclic_gc():
+            return self._run_wall(requests, keep, progress)
+
+    def _run_wall(self, requests, keep, progress):
         order = sorted(requests, key=lambda r: r.arrival)
         pending = collections.deque(order)
         waiting = collections.deque()
@@ -156,6 +176,10 @@ class Pool:
 
     def run_wall(self, requests, keep=False, progress=None):
         """request.arrival is seconds after the start.  Returns the elapsed time."""
+        with _no_cyclic_gc():
+            return self._run_wall(requests, keep, progress)
+
+    def _run_wall(self, requests, keep, progress):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         lock = threading.Lock()
@@ -214,3 +238,77 @@ def latency_summary(requests):
         return {"n": 0}
     return {"n": int(lat.size), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
+
+
+def batched_workload_requests(transport="transportOperatorDT", n_problems=512, seed=11, stop_rel=0.01, window=10,
+                              adapt=50):
+    """A factory of serving requests drawn from the batched fixed-source workload: each request is
+    one of its `n_problems` problems (configs.batch_problem), picked at random, on a seed of its
+    own, stopping at `stop_rel` relative sd of its total-flux bin (bin 0)."""
+    from . import configs
+    problems = []
+    for p in range(n_problems):
+        inp, mg, pop, cycles = configs.batch_problem(p, transport=transport)
+        comp = T.compile_problem(inp, mg)
+        problems.append((comp, T.compile_source(inp["source"], comp), pop, cycles))
+    rng = np.random.default_rng(seed)
+    count = [0]
+
+    def make(arrival):
+        comp, src, pop, cycles = problems[int(rng.integers(len(problems)))]
+        count[0] += 1
+        return Request(comp, src, pop, cycles, 700000 + count[0], stop_bin=0, stop_rel=stop_rel, arrival=arrival,
+                       window=min(window, cycles), adapt=adapt)
+    return make, rng
+
+
+def main(argv=None):
+    """Serving run on one GPU: capacity with `--saturate` requests waiting at t = 0, then a Poisson
+    stream at `--load` of that capacity for `--seconds`; prints one JSON line (latency percentiles
+    over the run and per minute, histories/s), progress on stderr every 20 s."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--load", type=float, default=0.9)
+    ap.add_argument("--seconds", type=float, default=600.0)
+    ap.add_argument("--saturate", type=int, default=4096)
+    ap.add_argument("--slots", type=int, default=256)
+    ap.add_argument("--budget-gib", type=float, default=8.0)
+    args = ap.parse_args(argv)
+    make, rng = batched_workload_requests()
+
+    def engine():
+        B = Batch(transport=T.TRANSPORT_DT, max_slots=args.slots, hbm_budget=int(args.budget_gib * (1 << 30)),
+                  use_graph=1)
+        return B, Server(B)
+
+    B, srv = engine()
+    reqs = [make(0.0) for _ in range(args.saturate)]
+    dt = srv.run_wall(reqs)
+    B.close()
+    cap = args.saturate / dt
+    cap_hist = sum(r.result.n_histories for r in reqs) / dt
+    lam = args.load * cap
+    arr = np.cumsum(rng.exponential(1.0 / lam, max(1, int(lam * args.seconds))))
+    reqs = [make(float(a)) for a in arr]
+    print(f"capacity {cap:.0f} requests/s; {len(reqs)} requests over {arr[-1]:.0f} s", file=sys.stderr, flush=True)
+    B, srv = engine()
+    el = srv.run_wall(reqs, progress=lambda t, p, w, r: print(f"t {t:.0f} s pending {p} waiting {w} running {r}",
+                                                              file=sys.stderr, flush=True))
+    B.close()
+    lat = np.array([r.latency for r in reqs])
+    out = latency_summary(reqs)
+    minutes = [(arr >= 60 * m) & (arr < 60 * (m + 1)) for m in range(int(arr[-1] // 60))]
+    out.update({"capacity_req_per_s": cap, "capacity_hist_per_s": cap_hist, "load": args.load,
+                "offered_req_per_s": lam, "elapsed_s": el, "cycles": srv.cycles,
+                "hist_per_s": sum(r.result.n_histories for r in reqs) / el, "busy_rejections": srv.busy_rejections,
+                "early_stopped": int(sum(1 for r in reqs if r.result.n_batches < r.n_batches)),
+                "p90_s": float(np.percentile(lat, 90)), "p999_s": float(np.percentile(lat, 99.9)),
+                "p50_p99_per_minute_s": [[float(np.percentile(lat[m], 50)), float(np.percentile(lat[m], 99))]
+                                         for m in minutes if m.any()]})
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -106,3 +106,28 @@ def test_pool_reports_an_engine_error():
         assert "engine failed" in str(e)
     else:
         raise AssertionError("the engine's error was not raised")
+
+
+def test_collector_paused_during_a_run_and_restored():
+    """The serving loop runs with the cyclic collector paused (a full collection over millions of
+    resident requests would stall it) and leaves it as it found it."""
+    import gc
+
+    class Watching(FakeBatch):
+        seen = []
+
+        def cycle(self, n=1):
+            Watching.seen.append(gc.isenabled())
+            return super().cycle(n)
+
+    reqs = [Request(None, None, 10, 2, seed=i, arrival=0.0) for i in range(3)]
+    assert gc.isenabled()
+    Server(Watching(slots=2, budget=100)).run_wall(reqs)
+    assert Watching.seen and not any(Watching.seen)
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        Server(Watching(slots=2, budget=100)).run_wall([Request(None, None, 10, 1, seed=9, arrival=0.0)])
+        assert not gc.isenabled()
+    finally:
+        gc.enable()
