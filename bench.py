@@ -346,15 +346,9 @@ def serving_mix(problems, transport, seconds_per_load, loads=(0.5, 0.75, 1.0), s
     windows; latency at offered Poisson loads of that capacity, `seconds_per_load` each."""
     from scone_amd import tables as T
     from scone_amd.engine import Batch
-    from scone_amd.serve import Request, Server, latency_summary
-    rng = np.random.default_rng(seed)
-    k = [0]
-
-    def request(arrival):
-        comp, src, pop, cycles, _ = problems[int(rng.integers(len(problems)))]
-        k[0] += 1
-        return Request(comp, src, pop, cycles, 500000 + k[0], stop_bin=0, stop_rel=stop_rel, arrival=arrival,
-                       window=min(window, cycles), adapt=adapt)
+    from scone_amd.serve import Server, batched_workload_requests, latency_summary
+    request, rng = batched_workload_requests(transport, seed=seed, stop_rel=stop_rel, window=window, adapt=adapt,
+                                             problems=problems)
 
     def make():
         B = Batch(transport=T.TRANSPORT_DT if transport == "transportOperatorDT" else T.TRANSPORT_ST,

@@ -241,21 +241,24 @@ def latency_summary(requests):
 
 
 def batched_workload_requests(transport="transportOperatorDT", n_problems=512, seed=11, stop_rel=0.01, window=10,
-                              adapt=50):
+                              adapt=50, problems=None):
     """A factory of serving requests drawn from the batched fixed-source workload: each request is
     one of its `n_problems` problems (configs.batch_problem), picked at random, on a seed of its
-    own, stopping at `stop_rel` relative sd of its total-flux bin (bin 0)."""
-    from . import configs
-    problems = []
-    for p in range(n_problems):
-        inp, mg, pop, cycles = configs.batch_problem(p, transport=transport)
-        comp = T.compile_problem(inp, mg)
-        problems.append((comp, T.compile_source(inp["source"], comp), pop, cycles))
+    own, stopping at `stop_rel` relative sd of its total-flux bin (bin 0).  `problems`: the
+    workload already compiled, as (comp, src, pop, cycles, ...) tuples.  Returns (make, rng):
+    make(arrival) is the next request, rng the generator it draws from."""
+    if problems is None:
+        from . import configs
+        problems = []
+        for p in range(n_problems):
+            inp, mg, pop, cycles = configs.batch_problem(p, transport=transport)
+            comp = T.compile_problem(inp, mg)
+            problems.append((comp, T.compile_source(inp["source"], comp), pop, cycles))
     rng = np.random.default_rng(seed)
     count = [0]
 
     def make(arrival):
-        comp, src, pop, cycles = problems[int(rng.integers(len(problems)))]
+        comp, src, pop, cycles = problems[int(rng.integers(len(problems)))][:4]
         count[0] += 1
         return Request(comp, src, pop, cycles, 700000 + count[0], stop_bin=0, stop_rel=stop_rel, arrival=arrival,
                        window=min(window, cycles), adapt=adapt)

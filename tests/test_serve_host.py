@@ -131,3 +131,19 @@ def test_collector_paused_during_a_run_and_restored():
         assert not gc.isenabled()
     finally:
         gc.enable()
+
+
+def test_batched_workload_requests_draw_from_the_problems():
+    """Requests of the timed serving run: a problem of the batched workload each, drawn at random,
+    on distinct seeds, stopping on its total-flux bin, with the window capped by its batches."""
+    from scone_amd.serve import batched_workload_requests
+    make, _ = batched_workload_requests(n_problems=3)
+    reqs = [make(0.1 * i) for i in range(40)]
+    assert len({r.seed for r in reqs}) == 40
+    assert len({id(r.comp) for r in reqs}) == 3
+    assert all(r.stop_bin == 0 and r.stop_rel == 0.01 and 1 <= r.window <= r.n_batches for r in reqs)
+    assert [r.arrival for r in reqs] == [0.1 * i for i in range(40)]
+    pre = [("c", "s", 7, 2, 99)]
+    make2, _ = batched_workload_requests(problems=pre)
+    r = make2(0.0)
+    assert (r.comp, r.src, r.pop, r.n_batches, r.window) == ("c", "s", 7, 2, 2)
